@@ -1,0 +1,228 @@
+/* C ABI of libvoxneus_hip.so: hand-written gfx950 (MI355X) kernels for the VoxNeuS
+ * voxel-SDF rendering and regularisation path.
+ *
+ * Conventions (all entry points):
+ *   - return 0 on success or a negative VXN_ERR_* code; vxn_last_error() returns the
+ *     calling thread's last message;
+ *   - every pointer argument that names device memory must be a HIP device pointer;
+ *     arguments documented "host" are read on the host before launch;
+ *   - all work is enqueued on `stream` (a hipStream_t; 0 = the null stream) and is
+ *     asynchronous: no entry point synchronises, allocates or frees device memory.
+ *     Scratch memory is passed in by the caller as `workspace`, sized by the matching
+ *     *_workspace_bytes() query;
+ *   - gradient outputs documented "accumulated" are added to (callers zero them first);
+ *     all other outputs are overwritten.
+ *
+ * Citations name the reference interface each entry point replaces (paths relative to
+ * the reference checkout's Code/ directory).
+ */
+#ifndef VOXNEUS_HIP_H
+#define VOXNEUS_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  VXN_OK = 0,
+  VXN_ERR_NULL = -1,        /* a required pointer is null */
+  VXN_ERR_SHAPE = -2,       /* a size or shape is out of the supported range */
+  VXN_ERR_VALUE = -3,       /* an argument value is invalid */
+  VXN_ERR_UNSUPPORTED = -4, /* a configuration this library does not implement */
+  VXN_ERR_WORKSPACE = -5,   /* workspace missing or too small */
+  VXN_ERR_HIP = -6          /* a HIP runtime call failed */
+};
+
+enum { VXN_DTYPE_F32 = 0, VXN_DTYPE_BF16 = 1 };
+enum { VXN_MLP_FP32 = 0, VXN_MLP_BF16_MFMA = 1 };
+
+#define VXN_MAX_LEVELS 32
+#define VXN_MLP_IN 51
+#define VXN_MLP_HIDDEN 64
+#define VXN_MLP_PARAMS (64 * 51 + 64 + 3 * 64 + 3) /* W0, b0, W1, b1 flattened in that order */
+
+const char* vxn_version(void);
+const char* vxn_last_error(void);
+
+/* ---- Dense SDF grid queries -------------------------------------------------------
+ * grid: [X,Y,Z] float32, z fastest.  box6 (host): xyz_min[3], xyz_max[3].            */
+
+/* Trilinear SDF (F.grid_sample, border, align_corners) + Eq. 7 interpolated gradient,
+ * its normalisation, and optionally the 8 corner indices [n,8,3] int64 and trilinear
+ * weights [n,8].  Replaces DenseSDFGrid.query_sdf + query_normals
+ * (models/sdf_grid.py:90-160, geometry/gradients.py:103-139). Outputs are nullable.  */
+int vxn_grid_query_fwd(const float* grid, int X, int Y, int Z, const float* box6, const float* xyz, int64_t n,
+                       float* sdf, float* grad, float* normal, int64_t* corners, float* corner_weights,
+                       void* stream);
+
+/* Backward of the above w.r.t. the grid: d_sdf [n] and/or d_grad [n,3] (dL w.r.t. the
+ * un-normalised gradient) are scattered into grad_grid (accumulated).               */
+int vxn_grid_query_bwd(const float* grid, int X, int Y, int Z, const float* box6, const float* xyz, int64_t n,
+                       const float* d_sdf, const float* d_grad, float* grad_grid, void* stream);
+
+/* Eq. 7 on continuous grid coordinates [n,3] (already in [0,res-1]); eps3 (host) is the
+ * voxel size.  Replaces _backend.gradient_interp_forward / _backward
+ * (cuda/gradient_interp.cu:154-212, bound by cuda/adam_upd.cpp:118-125).  The backward
+ * accumulates into grad_grid.                                                        */
+int vxn_gradient_interp_fwd(const float* grid, int X, int Y, int Z, const float* coords, int64_t n,
+                            const float* eps3, float* out, void* stream);
+int vxn_gradient_interp_bwd(const float* grad_out, const float* coords, int64_t n, float* grad_grid, int X, int Y,
+                            int Z, const float* eps3, void* stream);
+
+/* SDF value and interpolated unit normal at every point of an NX*NY*NZ lattice spanning
+ * lattice_box6 (host), written [NX,NY,NZ] and [NX,NY,NZ,3].  The mesh-extraction query
+ * of geometry/mesh.py:80-114 (compute_mesh_normals) over a full lattice.             */
+int vxn_lattice_query(const float* grid, int X, int Y, int Z, const float* box6, int NX, int NY, int NZ,
+                      const float* lattice_box6, float* sdf, float* normal, void* stream);
+
+/* ---- Multiresolution hash encoding ------------------------------------------------
+ * table: [L, 2^log2T, F] float32 (dtype 0) or bfloat16 (dtype 1); F must be 2.
+ * level_res (host, L entries): floor(base * b^l) as computed by the reference;
+ * level_w (host, nullable): the NeuS2 lambda weights (1 = active).
+ * x01: [n,3] normalised coordinates, clamped to [0,1] inside.  Replaces
+ * PyTorchHashGrid.forward (models/radiance_hash.py:71-166).  The backward accumulates
+ * float32 gradients into grad_table [L,T,F] regardless of the table dtype.           */
+int vxn_hash_fwd(const void* table, int dtype, int L, int log2T, int F, const int32_t* level_res,
+                 const float* level_w, const float* x01, int64_t n, float* feats, void* stream);
+int vxn_hash_bwd(int L, int log2T, int F, const int32_t* level_res, const float* level_w, const float* x01, int64_t n,
+                 const float* d_feats, float* grad_table, void* stream);
+
+/* ---- Progressive importance upsampler ---------------------------------------------
+ * Replaces RaySampler.sample_multistep_sdf with up_sample / sample_pdf / cat_z_vals
+ * (rendering/sampler.py:127-204, 243-320, 11-66) and OccupancyGrid.is_occupied
+ * (rendering/occupancy_grid.py:121-147).                                             */
+typedef struct vxn_sampler_args {
+  const float* rays_o; /* [R,3] */
+  const float* rays_d; /* [R,3] */
+  const float* near;   /* [R] */
+  const float* far;    /* [R] */
+  int64_t n_rays;
+  const float* grid;
+  int32_t X, Y, Z;
+  float box[6];
+  const uint8_t* occ_mask; /* [occ_res^3] bool, nullable = no occupancy modulation */
+  int32_t occ_res;
+  float occ_box[6];
+  int32_t n_samples, n_importance, up_sample_steps, perturb;
+  const float* t_lin;  /* device [n_samples]: torch.linspace(0, 1, n_samples) */
+  const float* u_det;  /* device [per_step]: torch.linspace(0.5/p, 1-0.5/p, p) */
+  const float* t_rand; /* device [R, n_samples] stratified jitter (perturb only) */
+  const float* u_rand; /* device [steps, R, per_step] inverse-CDF draws (perturb only) */
+  float* z_vals;       /* [R, S], S = n_samples + steps*per_step */
+  float* dists;        /* [R, S] */
+  float* mid_z;        /* [R, S] */
+  int32_t* inds;       /* nullable [steps, R, per_step]: searchsorted indices */
+} vxn_sampler_args;
+
+int vxn_upsample(const vxn_sampler_args* a, void* stream);
+
+/* ---- Fused per-ray render (forward, backward, training step) ----------------------
+ * One ray per work-group: per-sample SDF + Eq. 7 normal, hash features, SH view
+ * embedding, the 51->64->3 radiance MLP, NeuS Eq. 2 alpha, front-to-back transmittance
+ * and compositing.  Replaces VoxNeuS.render passes 2-3 with forward_points and
+ * volume_render_from_samples (models/voxneus.py:115-167, 226-258;
+ * rendering/neus_alpha.py:12-81; rendering/volume_render.py:12-153).                 */
+typedef struct vxn_render_args {
+  const float* rays_o; /* [R,3] */
+  const float* rays_d; /* [R,3] */
+  const float* mid_z;  /* [R,S] */
+  const float* dists;  /* [R,S] */
+  int64_t n_rays;
+  int32_t n_samples; /* S <= 128 */
+  const float* grid;
+  int32_t X, Y, Z;
+  float box[6];
+  const void* table; /* [L,T,F] */
+  int32_t table_dtype;
+  int32_t n_levels, log2_T, n_features;
+  int32_t level_res[VXN_MAX_LEVELS];
+  float level_w[VXN_MAX_LEVELS];
+  const float* mlp; /* [VXN_MLP_PARAMS] float32: W0 [64,51], b0 [64], W1 [3,64], b1 [3] */
+  int32_t mlp_mode;
+  float s;
+  float sphere_radius;
+  float cos_anneal_ratio;
+  int32_t use_iter_cos;
+  /* forward outputs (nullable) */
+  float* rgb;            /* [R,3] */
+  float* depth;          /* [R] */
+  float* normal;         /* [R,3] */
+  float* opacity;        /* [R] */
+  float* weights;        /* [R,S] */
+  float* alpha;          /* [R,S] */
+  float* transmittance;  /* [R,S] */
+  float* sdf_samples;    /* [R,S] */
+  float* grad_samples;   /* [R,S,3] */
+  float* normal_samples; /* [R,S,3] */
+  int64_t* corner_indices; /* [R,S,8,3] */
+  /* backward inputs (nullable = zero) */
+  const float* d_rgb;     /* [R,3] */
+  const float* d_depth;   /* [R] */
+  const float* d_normal;  /* [R,3] */
+  const float* d_opacity; /* [R] */
+  const float* d_weights; /* [R,S] */
+  /* fused training loss: Huber(rgb, target_rgb, delta), mean over R*3 */
+  const float* target_rgb; /* [R,3] */
+  float huber_delta;
+  /* gradient outputs */
+  float* grad_grid;  /* [X,Y,Z], accumulated */
+  float* grad_table; /* [L,T,F] float32, accumulated */
+  float* grad_mlp;   /* [VXN_MLP_PARAMS], overwritten */
+  float* grad_s;     /* [1] dL/ds, overwritten */
+  double* loss;      /* [1] Huber loss (training step only), overwritten */
+  uint32_t* vr_bitmask; /* nullable [ceil(X*Y*Z/32)]: bit set for all 8 corners of every sample */
+  void* workspace;
+  int64_t workspace_bytes;
+} vxn_render_args;
+
+int vxn_render_workspace_bytes(const vxn_render_args* a, int64_t* bytes);
+int vxn_render_forward(const vxn_render_args* a, void* stream);
+int vxn_render_backward(const vxn_render_args* a, void* stream);
+int vxn_render_train(const vxn_render_args* a, void* stream);
+
+/* ---- Active vertex set V_R and closed-form regularisers ---------------------------*/
+
+/* Mark corner indices [n,3] int64 in a bitmask of ceil(X*Y*Z/32) words (accumulated).
+ * The marking half of extract_active_vertices (regularization/active_vertices.py:12-50). */
+int vxn_vr_mark_corners(const int64_t* corners, int64_t n, int X, int Y, int Z, uint32_t* bitmask, void* stream);
+
+/* Compact the bitmask into ascending linear indices ix*Y*Z+iy*Z+iz (the torch.unique
+ * order).  Bit 31 of an entry is set when its world position is NOT inside the relaxed
+ * sphere |x| < relax_radius (training/trainer.py:260-266).  counts (device, [2]) gets
+ * {all active, active inside}.  vr_list must hold X*Y*Z entries.                      */
+int vxn_vr_workspace_bytes(int X, int Y, int Z, int64_t* bytes);
+int vxn_vr_compact(const uint32_t* bitmask, int X, int Y, int Z, const float* box6, float relax_radius,
+                   int32_t* vr_list, int32_t* counts, void* workspace, int64_t workspace_bytes, void* stream);
+
+/* Fused closed-form eikonal (Eq. 10-11) and curvature (Eq. 12-14) gradients over V_R,
+ * accumulated into grad_grid with weights w_eik / w_curv, plus both losses reduced in
+ * FP64 (means over interior vertices) into losses[2] (device).  |V_R| is counts[1] when
+ * non-zero (relaxed-sphere filter applied) else counts[0] (the filter's keep-all case).
+ * Replaces compute_eikonal_loss_and_gradient and compute_curvature_loss_and_gradient
+ * (regularization/eikonal.py:13-171, regularization/curvature.py:13-165).            */
+int vxn_reg_workspace_bytes(int X, int Y, int Z, int64_t* bytes);
+int vxn_reg_eik_curv(const float* grid, int X, int Y, int Z, const float* box6, const int32_t* vr_list,
+                     const int32_t* counts, float w_eik, float w_curv, float* grad_grid, double* losses,
+                     void* workspace, int64_t workspace_bytes, void* stream);
+
+/* Reference-shaped single regularisers over active [V,3] int64 vertices; eps3 (host) is
+ * the voxel size.  Replace _backend.eikonal_grad / curvature_grad
+ * (cuda/eikonal_grad.cu:71-101, cuda/curvature_grad.cu:73-103); accumulate into
+ * grad_grid.                                                                          */
+int vxn_eikonal_grad(const float* grid, int X, int Y, int Z, const int64_t* active, int64_t V, float* grad_grid,
+                     const float* eps3, float weight, void* stream);
+int vxn_curvature_grad(const float* grid, int X, int Y, int Z, const int64_t* active, int64_t V, float* grad_grid,
+                       const float* eps3, float weight, void* stream);
+
+/* s-gradient amplification on the device: grad_v *= k if grad_v < 0.  amplified
+ * (device, nullable) receives 1 or 0.  Replaces amplify_s_gradient
+ * (training/scheduler.py:14-33) without its host read-back.                           */
+int vxn_s_amplify(float* grad_v, float k, int32_t* amplified, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VOXNEUS_HIP_H */

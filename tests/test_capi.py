@@ -1,0 +1,78 @@
+"""The C-ABI library loads and exports every entry point include/voxneus_hip.h declares;
+argument validation rejects bad calls on the host (no GPU needed: the checks run before
+any launch)."""
+
+import ctypes
+import os
+import re
+
+import pytest
+
+from voxneus_amd import _native as N
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(REPO, "include", "voxneus_hip.h")).read()
+    return sorted(set(re.findall(r"\b(vxn_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_and_binding_agree():
+    assert declared_symbols() == sorted(N.exported_symbols())
+
+
+def test_library_exports_every_declared_symbol():
+    L = N.lib()
+    for name in declared_symbols():
+        assert hasattr(L, name), name
+    assert L.vxn_version().decode().startswith("voxneus_hip")
+
+
+def test_struct_layouts_match_header_field_order():
+    text = open(os.path.join(REPO, "include", "voxneus_hip.h")).read()
+    for struct, cls in (("vxn_sampler_args", N.SamplerArgs), ("vxn_render_args", N.RenderArgs)):
+        body = re.search(r"typedef struct %s \{(.*?)\} %s;" % (struct, struct), text, re.S).group(1)
+        body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+        names = []
+        for decl in body.split(";"):
+            decl = decl.strip()
+            if not decl:
+                continue
+            decl = re.sub(r"\[[^\]]*\]", "", decl)
+            for part in decl.split(","):
+                names.append(re.findall(r"[A-Za-z_][A-Za-z0-9_]*", part)[-1])
+        assert names == [f[0] for f in cls._fields_], struct
+
+
+def test_host_validation_errors():
+    L = N.lib()
+    box = N.box6((-1, -1, -1), (1, 1, 1))
+    rc = L.vxn_grid_query_fwd(None, 8, 8, 8, box, None, 1, None, None, None, None, None, None)
+    assert rc == -1 and b"grid" in L.vxn_last_error()
+    rc = L.vxn_grid_query_fwd(ctypes.c_void_p(16), 1, 8, 8, box, ctypes.c_void_p(16), 1, None, None, None, None, None,
+                              None)
+    assert rc == -2
+    bad = N.box6((1, -1, -1), (-1, 1, 1))
+    rc = L.vxn_grid_query_fwd(ctypes.c_void_p(16), 8, 8, 8, bad, ctypes.c_void_p(16), 1, None, None, None, None, None,
+                              None)
+    assert rc == -3
+    res = (ctypes.c_int32 * 16)(*([16] * 16))
+    rc = L.vxn_hash_fwd(ctypes.c_void_p(16), 0, 16, 15, 4, res, None, ctypes.c_void_p(16), 1, ctypes.c_void_p(16),
+                        None)
+    assert rc == -4 and b"features" in L.vxn_last_error()
+    a = N.RenderArgs()
+    a.grid, a.X, a.Y, a.Z, a.box = 16, 8, 8, 8, box
+    a.n_rays, a.n_samples = 4, 200
+    assert L.vxn_render_forward(ctypes.byref(a), None) == -4
+    a.n_samples, a.n_levels, a.n_features = 128, 16, 2
+    a.mlp_mode = 0
+    assert L.vxn_render_forward(ctypes.byref(a), None) == -1  # null rays
+    nb = ctypes.c_int64(0)
+    assert L.vxn_render_workspace_bytes(ctypes.byref(a), ctypes.byref(nb)) == 0 and nb.value > 0
+    s = N.SamplerArgs()
+    s.grid, s.X, s.Y, s.Z, s.box = 16, 8, 8, 8, box
+    s.n_rays, s.n_samples, s.n_importance, s.up_sample_steps = 4, 64, 128, 4
+    assert L.vxn_upsample(ctypes.byref(s), None) == -4  # 64 + 4*32 > 128
+    assert L.vxn_reg_eik_curv(ctypes.c_void_p(16), 8, 8, 8, box, ctypes.c_void_p(16), ctypes.c_void_p(16), 0.1, 0.1,
+                              ctypes.c_void_p(16), ctypes.c_void_p(16), None, 0, None) == -5

@@ -1,0 +1,296 @@
+"""GPU parity of the gfx950 kernels against the reference's golden vectors and the CPU
+oracle (same seeded inputs).  Integer outputs (corners, hash features built from exact
+products, searchsorted indices, z values, V_R) must be bit-identical; float gradients
+are held to 1e-5 of the largest reference magnitude (FP32 form, atomics reorder sums).
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import ref_cpu as R
+from tests.conftest import load_golden
+from voxneus_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+DEV = torch.device("cuda")
+BOX = ((-1.0, -1.0, -1.0), (1.0, 1.0, 1.0))
+BOX6 = (-1.0, -1.0, -1.0, 1.0, 1.0, 1.0)
+
+
+def rel_err(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-30))
+
+
+@pytest.fixture(scope="module")
+def Fn():
+    from voxneus_amd import functional
+    return functional
+
+
+def test_grid_query_matches_reference(Fn):
+    g = load_golden("unit_grid.npz")
+    sc = scenes.config1_scene(seed=0)
+    grid = torch.from_numpy(sc.grid).to(DEV).requires_grad_(True)
+    xyz = torch.from_numpy(g["pts"]).to(DEV)
+    o = Fn.grid_query_raw(grid.detach(), BOX, xyz, True, True, True, True, True)
+    assert np.array_equal(o["sdf"].cpu().numpy(), g["sdf"][:, 0])
+    assert np.array_equal(o["corners"].cpu().numpy().astype(np.int16), g["corners"])
+    assert np.array_equal(o["weights"].cpu().numpy(), g["weights"])
+    assert rel_err(o["grad"].cpu(), g["grad"]) < 1e-6
+    assert np.abs(o["normal"].cpu().numpy() - g["normal"]).max() < 1e-6
+    sdf, gv = Fn.GridQueryFunction.apply(grid, xyz, BOX)
+    loss = (gv * torch.from_numpy(g["r_g"]).to(DEV)).sum() + (sdf * torch.from_numpy(g["r_s"][:, 0]).to(DEV)).sum()
+    loss.backward()
+    ref = np.zeros(grid.numel(), np.float32)
+    ref[g["grid_grad_idx"]] = g["grid_grad_val"]
+    assert rel_err(grid.grad.cpu().numpy().reshape(-1), ref) < 1e-5
+
+
+def test_gradient_interp_reference_shape(Fn):
+    """Pattern of test_cuda_parity.py:31-70: random 32^3 grid, coords in [0, res-1.001]."""
+    torch.manual_seed(42)
+    res = 32
+    grid = torch.randn(1, 1, res, res, res)
+    coords = torch.rand(256, 3) * (res - 1.001)
+    vs = torch.tensor([2.0 / (res - 1)] * 3)
+    gres = torch.tensor([res] * 3, dtype=torch.float32)
+    ref, _, _, _ = R.interpolate_gradients(grid, coords, vs, gres)
+    gg = grid.to(DEV).requires_grad_(True)
+    out = Fn.GradientInterpFunction.apply(gg, coords.to(DEV), vs)
+    assert (out.detach().cpu() - ref).abs().max().item() < 1e-5
+    gr = grid.clone().requires_grad_(True)
+    rg, _, _, _ = R.interpolate_gradients(gr, coords, vs, gres)
+    (rg ** 2).sum().backward()
+    (out ** 2).sum().backward()
+    assert (gg.grad.cpu() - gr.grad).abs().max().item() < 1e-4
+
+
+def test_hash_matches_reference(Fn):
+    g = load_golden("unit_hash.npz")
+    sc = scenes.config1_scene(seed=0)
+    emb = torch.from_numpy(sc.embeddings).to(DEV).requires_grad_(True)
+    x = torch.from_numpy(g["x"]).to(DEV)
+    res = g["level_res"].tolist()
+    f = Fn.HashEncodeFunction.apply(emb, x, res, None)
+    assert np.array_equal(f.detach().cpu().numpy(), g["feats"])
+    from voxneus_amd.hash_grid import level_weights
+    fl = Fn.HashEncodeFunction.apply(emb.detach(), x, res, level_weights(16, 5.5))
+    assert np.array_equal(fl.cpu().numpy(), g["feats_lambda"])
+    (f * torch.from_numpy(g["r"]).to(DEV)).sum().backward()
+    eg = emb.grad.cpu().numpy().reshape(-1)
+    assert int(np.count_nonzero(eg)) == int(g["emb_grad_nnz"])
+    assert rel_err(eg[g["emb_grad_idx"]], g["emb_grad_val"]) < 1e-5
+    assert abs(eg.astype(np.float64).sum() - float(g["emb_grad_sum"])) < 1e-5 * float(g["emb_grad_abs"])
+
+
+def test_hash_bf16_table_reads_exact_values(Fn):
+    sc = scenes.config1_scene(seed=0)
+    emb = torch.from_numpy(sc.embeddings).to(torch.bfloat16)
+    x = torch.rand(2048, 3)
+    res = R.hash_level_resolutions(16, 16, sc.hash_max_resolution)
+    ref = R.hash_encode(x, emb.float(), res)
+    out = Fn.HashEncodeFunction.apply(emb.to(DEV), x.to(DEV), res, None)
+    assert np.array_equal(out.cpu().numpy(), ref.numpy())
+
+
+def _sampler_inputs(g):
+    sc = scenes.config1_scene(seed=0)
+    occ = np.unpackbits(g["occ_mask"])[: 32 ** 3].astype(bool).reshape(32, 32, 32)
+    return sc, occ
+
+
+def test_sampler_bit_exact_deterministic(Fn):
+    g = load_golden("unit_sampler.npz")
+    sc, occ = _sampler_inputs(g)
+    o, d = torch.from_numpy(g["rays_o"]).to(DEV), torch.from_numpy(g["rays_d"]).to(DEV)
+    near, far = torch.from_numpy(g["near"]).to(DEV), torch.from_numpy(g["far"]).to(DEV)
+    grid = torch.from_numpy(sc.grid).to(DEV)
+    z, dists, mid, inds = Fn.upsample(o, d, near, far, grid, BOX, occ_mask=torch.from_numpy(occ).to(DEV),
+                                      return_inds=True)
+    # searchsorted indices against the oracle's, step by step
+    m = R.RefModel(sc.grid, sc.embeddings, **sc.mlp, occupancy_mask=occ)
+    rec = []
+    m.sample(o.cpu(), d.cpu(), near.cpu(), far.cpu(), record=rec)
+    ref_inds = np.stack([r["inds"].numpy() for r in rec if "inds" in r])
+    got = inds.cpu().numpy()
+    mism = int((got != ref_inds).sum())
+    assert mism == 0, f"{mism} searchsorted indices differ"
+    assert np.array_equal(z.cpu().numpy(), g["z_det"])
+    assert np.array_equal(dists.cpu().numpy(), g["dists_det"])
+    assert np.array_equal(mid.cpu().numpy(), g["mid_det"])
+
+
+def test_sampler_bit_exact_injected_uniforms(Fn):
+    g = load_golden("unit_sampler.npz")
+    sc, occ = _sampler_inputs(g)
+    o, d = torch.from_numpy(g["rays_o"]).to(DEV), torch.from_numpy(g["rays_d"]).to(DEV)
+    near, far = torch.from_numpy(g["near"]).to(DEV), torch.from_numpy(g["far"]).to(DEV)
+    grid = torch.from_numpy(sc.grid).to(DEV)
+    z, dists, mid = Fn.upsample(o, d, near, far, grid, BOX, perturb=True, occ_mask=torch.from_numpy(occ).to(DEV),
+                                t_rand=torch.from_numpy(g["t_rand"]).to(DEV),
+                                u_rand=torch.from_numpy(g["u_rnd"]).to(DEV))
+    assert np.array_equal(z.cpu().numpy(), g["z_rnd"])
+    assert np.array_equal(mid.cpu().numpy(), g["mid_rnd"])
+
+
+def _mlp_flat(sc):
+    m = sc.mlp
+    return torch.cat([torch.from_numpy(m[k]).reshape(-1) for k in ("W0", "b0", "W1", "b1")])
+
+
+def _golden_scene(name):
+    if name == "step_c1.npz":
+        return scenes.config1_scene(seed=0)
+    return scenes.make_scene("multi", 128, 17, seed=2)
+
+
+@pytest.mark.parametrize("name", ["step_c1.npz", "step_c2.npz"])
+def test_render_forward_matches_reference(Fn, name):
+    g = load_golden(name)
+    sc = _golden_scene(name)
+    cfg = Fn.RenderConfig(BOX6, R.hash_level_resolutions(16, 16, sc.hash_max_resolution))
+    grid = torch.from_numpy(sc.grid).to(DEV)
+    emb = torch.from_numpy(sc.embeddings).to(DEV)
+    s = torch.tensor(float(np.float32(np.exp(np.float32(0.3) * np.float32(10.0)))), device=DEV)
+    s = torch.clamp(torch.exp(torch.tensor(0.3) * 10.0), 1e-4, 300.0).to(DEV)
+    outs = Fn.RenderFunction.apply(grid, emb, _mlp_flat(sc).to(DEV), s, torch.from_numpy(g["rays_o"]).to(DEV),
+                                   torch.from_numpy(g["rays_d"]).to(DEV), torch.from_numpy(g["mid"]).to(DEV),
+                                   torch.from_numpy(g["dists"]).to(DEV), cfg, True, None)
+    rgb, depth, normal, opac, w, alpha, T, sdf_s, grad_s, nrm_s, corners = [o.detach().cpu().numpy() if o is not None
+                                                                           else None for o in outs]
+    assert np.array_equal(sdf_s, g["sdf_samples"])
+    assert np.array_equal(corners[..., 0, :].astype(np.int16), g["corner_c0"])
+    assert np.abs(grad_s - g["grad_samples"]).max() <= 1e-6 * max(1.0, np.abs(g["grad_samples"]).max())
+    for k, v in (("alpha", alpha), ("transmittance", T), ("weights", w), ("rgb", rgb), ("depth", depth),
+                 ("opacity", opac)):
+        assert np.abs(v - g[k]).max() < 1e-5, k
+    assert np.abs(normal - g["normal"]).max() < 1e-4
+
+
+def test_render_backward_matches_oracle_autograd(Fn):
+    """Autograd through the fused render vs the oracle's PyTorch autograd on the same z."""
+    g = load_golden("step_c1.npz")
+    sc = scenes.config1_scene(seed=0)
+    m = R.RefModel(sc.grid, sc.embeddings, **sc.mlp, hash_max_resolution=sc.hash_max_resolution)
+    o, d = torch.from_numpy(g["rays_o"]), torch.from_numpy(g["rays_d"])
+    zs = tuple(torch.from_numpy(g[k]) for k in ("z_vals", "dists", "mid"))
+    rng = np.random.default_rng(3)
+    rr = [torch.from_numpy(rng.normal(size=sh).astype(np.float32)) for sh in ((64, 3), (64, 1), (64, 3), (64, 1))]
+    out = m.render(o, d, None, None, zs=zs)
+    L = (out["rgb"] * rr[0]).sum() + (out["depth"] * rr[1]).sum() + (out["normal"] * rr[2]).sum() + (
+        out["opacity"] * rr[3]).sum()
+    L.backward()
+    cfg = Fn.RenderConfig(BOX6, m.level_res)
+    grid = torch.from_numpy(sc.grid).to(DEV).requires_grad_(True)
+    emb = torch.from_numpy(sc.embeddings).to(DEV).requires_grad_(True)
+    mlp = _mlp_flat(sc).to(DEV).requires_grad_(True)
+    v = torch.tensor(0.3, device=DEV, requires_grad=True)
+    s = torch.clamp(torch.exp(v * 10.0), min=1e-4, max=300.0)
+    outs = Fn.RenderFunction.apply(grid, emb, mlp, s, o.to(DEV), d.to(DEV), zs[2].to(DEV), zs[1].to(DEV), cfg, False,
+                                   None)
+    Lg = sum((outs[i] * rr[k].to(DEV)).sum() for k, i in enumerate((0, 1, 2, 3)))
+    Lg.backward()
+    assert rel_err(grid.grad.cpu().numpy(), m.grid.grad.numpy()[0, 0]) < 1e-5
+    assert rel_err(emb.grad.cpu().numpy(), m.embeddings.grad.numpy()) < 1e-5
+    ref_mlp = torch.cat([m.W0.grad.reshape(-1), m.b0.grad, m.W1.grad.reshape(-1), m.b1.grad]).numpy()
+    assert rel_err(mlp.grad.cpu().numpy(), ref_mlp) < 1e-5
+    assert abs(v.grad.item() - m.variance.grad.item()) <= 1e-5 * max(1e-3, abs(m.variance.grad.item()))
+
+
+@pytest.mark.parametrize("name", ["step_c1.npz", "step_c2.npz"])
+def test_fused_train_step_matches_reference(name):
+    from voxneus_amd.training import FusedTrainStep, StepConfig
+    g = load_golden(name)
+    sc = _golden_scene(name)
+    st = FusedTrainStep(torch.from_numpy(sc.grid).to(DEV), torch.from_numpy(sc.embeddings).to(DEV),
+                        _mlp_flat(sc).to(DEV), torch.tensor(0.3, device=DEV),
+                        R.hash_level_resolutions(16, 16, sc.hash_max_resolution), cfg=StepConfig(w_eik=0.1, w_curv=5e-4))
+    out = st.step(torch.from_numpy(g["rays_o"]).to(DEV), torch.from_numpy(g["rays_d"]).to(DEV),
+                  torch.from_numpy(g["target"]).to(DEV))
+    assert np.array_equal(out["z_vals"].cpu().numpy(), g["z_vals"])
+    assert np.array_equal(st.active_linear().cpu().numpy(), g["vr_lin"])
+    assert abs(out["loss"].item() - float(g["loss_rgb"])) <= 1e-6 * abs(float(g["loss_rgb"]))
+    le, lc = out["reg_losses"].tolist()
+    assert abs(le - float(g["loss_eik"])) <= 1e-5 * abs(float(g["loss_eik"]))
+    assert abs(lc - float(g["loss_curv"])) <= 1e-5 * abs(float(g["loss_curv"]))
+    ref = np.zeros(st.grad_grid.numel(), np.float32)
+    ref[g["grid_grad_idx"]] = g["grid_grad_val"]
+    assert rel_err(st.grad_grid.cpu().numpy().reshape(-1), ref) < 1e-5
+    eg = st.grad_table.cpu().numpy().reshape(-1)
+    assert int(np.count_nonzero(eg)) == int(g["emb_grad_nnz"])
+    assert rel_err(eg[g["emb_grad_idx"]], g["emb_grad_val"]) < 1e-5
+    ref_mlp = np.concatenate([g["dW0"].reshape(-1), g["db0"], g["dW1"].reshape(-1), g["db1"]])
+    assert rel_err(st.grad_mlp.cpu().numpy(), ref_mlp) < 1e-5
+    assert bool(out["amplified"].item()) == bool(g["amplified"])
+    assert abs(st.grad_v.item() - float(g["dv_post"])) <= 1e-5 * abs(float(g["dv_post"]))
+
+
+def test_vr_compaction_and_regularisers_match_oracle(Fn):
+    rng = np.random.default_rng(4)
+    res = 48
+    data = rng.normal(size=(res, res, res)).astype(np.float32)
+    corners = rng.integers(0, res, size=(20000, 8, 3))
+    verts, lin = R.extract_active_vertices(torch.from_numpy(corners), torch.tensor([res] * 3))
+    bm = torch.zeros(Fn.vr_bitmask_words((res, res, res)), device=DEV, dtype=torch.int32)
+    Fn.vr_mark_corners(torch.from_numpy(corners).to(DEV), (res, res, res), bm)
+    lst, counts = Fn.vr_compact(bm, (res, res, res), BOX6, relax_radius=1.2)
+    n_all, n_in = counts.tolist()
+    got = (lst[:n_all] & 0x7FFFFFFF).cpu().numpy()
+    assert np.array_equal(got, lin.numpy().astype(np.int32))
+    vs = torch.tensor([2.0 / (res - 1)] * 3)
+    world = torch.tensor(BOX[0]) + verts.float() * vs
+    inside = torch.linalg.norm(world, dim=-1) < 1.2
+    assert n_in == int(inside.sum())
+    keep = verts[inside] if inside.any() else verts
+    le, ge = R.eikonal_loss_and_grad(torch.from_numpy(data), keep, vs)
+    lc, gc = R.curvature_loss_and_grad(torch.from_numpy(data), keep, vs)
+    gg = torch.zeros(res, res, res, device=DEV)
+    losses = Fn.reg_eik_curv(torch.from_numpy(data).to(DEV), BOX6, lst, counts, 0.1, 5e-4, gg)
+    ref = (ge * 0.1 + gc * 5e-4).numpy()
+    assert rel_err(gg.cpu().numpy(), ref) < 1e-5
+    assert abs(losses[0].item() - le.item()) <= 1e-5 * le.item()
+    assert abs(losses[1].item() - lc.item()) <= 1e-5 * lc.item()
+
+
+def test_reference_shaped_regularisers(Fn):
+    """Pattern of test_cuda_parity.py:72-126 (in-place accumulation == separate gradient)."""
+    torch.manual_seed(42)
+    res = 32
+    grid = torch.randn(res, res, res)
+    act = torch.randint(1, res - 1, (100, 3))
+    vs = torch.tensor([2.0 / (res - 1)] * 3)
+    _, ge = R.eikonal_loss_and_grad(grid, act, vs)
+    _, gc = R.curvature_loss_and_grad(grid, act, vs)
+    a = torch.zeros(res, res, res, device=DEV)
+    Fn.eikonal_grad_ref(grid.to(DEV), act.to(DEV), a, vs.tolist(), 1.0)
+    assert (a.cpu() - ge).abs().max().item() < 1e-5
+    b = torch.zeros(res, res, res, device=DEV)
+    Fn.curvature_grad_ref(grid.to(DEV), act.to(DEV), b, vs.tolist(), 1.0)
+    assert (b.cpu() - gc).abs().max().item() <= 1e-5 * max(1.0, gc.abs().max().item())
+
+
+def test_s_amplify_device_side(Fn):
+    g = torch.tensor([-0.25], device=DEV)
+    f = torch.zeros(1, device=DEV, dtype=torch.int32)
+    Fn.s_amplify(g, 5.0, f)
+    assert g.item() == -1.25 and f.item() == 1
+    Fn.s_amplify(g.neg_(), 5.0, f)
+    assert g.item() == 1.25 and f.item() == 0
+
+
+def test_lattice_query_equals_point_query(Fn):
+    sc = scenes.config1_scene(seed=0)
+    grid = torch.from_numpy(sc.grid).to(DEV)
+    n = 17
+    sdf, nrm = Fn.lattice_query(grid, BOX, n, n, n, BOX6)
+    ax = torch.linspace(-1.0, 1.0, n)
+    lsx = (1.0 - -1.0) / (n - 1)
+    pts = torch.stack(torch.meshgrid(ax, ax, ax, indexing="ij"), -1).reshape(-1, 3)
+    pts = (-1.0 + torch.arange(n, dtype=torch.float32) * torch.tensor(lsx, dtype=torch.float32))
+    P = torch.stack(torch.meshgrid(pts, pts, pts, indexing="ij"), -1).reshape(-1, 3).to(DEV)
+    o = Fn.grid_query_raw(grid, BOX, P, True, False, True)
+    assert torch.equal(sdf.reshape(-1), o["sdf"]) and torch.equal(nrm.reshape(-1, 3), o["normal"])

@@ -1,0 +1,143 @@
+"""ctypes binding of libvoxneus_hip.so (the C ABI declared in include/voxneus_hip.h).
+
+The library is loaded from ``voxneus_amd/lib`` (built in-tree by ``voxneus_amd.build``).
+There is no fallback: if the library is missing or a tensor is not on a HIP device the
+call raises, so a GPU run can never silently take another code path.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional, Sequence
+
+import torch
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libvoxneus_hip.so")
+
+VXN_DTYPE_F32 = 0
+VXN_DTYPE_BF16 = 1
+VXN_MLP_FP32 = 0
+VXN_MLP_BF16_MFMA = 1
+MAX_LEVELS = 32
+MLP_PARAMS = 64 * 51 + 64 + 3 * 64 + 3
+
+c_f = ctypes.c_float
+c_i32 = ctypes.c_int32
+c_i64 = ctypes.c_int64
+c_vp = ctypes.c_void_p
+
+
+class SamplerArgs(ctypes.Structure):
+    _fields_ = [
+        ("rays_o", c_vp), ("rays_d", c_vp), ("near", c_vp), ("far", c_vp), ("n_rays", c_i64),
+        ("grid", c_vp), ("X", c_i32), ("Y", c_i32), ("Z", c_i32), ("box", c_f * 6),
+        ("occ_mask", c_vp), ("occ_res", c_i32), ("occ_box", c_f * 6),
+        ("n_samples", c_i32), ("n_importance", c_i32), ("up_sample_steps", c_i32), ("perturb", c_i32),
+        ("t_lin", c_vp), ("u_det", c_vp), ("t_rand", c_vp), ("u_rand", c_vp),
+        ("z_vals", c_vp), ("dists", c_vp), ("mid_z", c_vp), ("inds", c_vp),
+    ]
+
+
+class RenderArgs(ctypes.Structure):
+    _fields_ = [
+        ("rays_o", c_vp), ("rays_d", c_vp), ("mid_z", c_vp), ("dists", c_vp), ("n_rays", c_i64),
+        ("n_samples", c_i32), ("grid", c_vp), ("X", c_i32), ("Y", c_i32), ("Z", c_i32), ("box", c_f * 6),
+        ("table", c_vp), ("table_dtype", c_i32), ("n_levels", c_i32), ("log2_T", c_i32), ("n_features", c_i32),
+        ("level_res", c_i32 * MAX_LEVELS), ("level_w", c_f * MAX_LEVELS),
+        ("mlp", c_vp), ("mlp_mode", c_i32), ("s", c_f), ("sphere_radius", c_f), ("cos_anneal_ratio", c_f),
+        ("use_iter_cos", c_i32),
+        ("rgb", c_vp), ("depth", c_vp), ("normal", c_vp), ("opacity", c_vp), ("weights", c_vp), ("alpha", c_vp),
+        ("transmittance", c_vp), ("sdf_samples", c_vp), ("grad_samples", c_vp), ("normal_samples", c_vp),
+        ("corner_indices", c_vp),
+        ("d_rgb", c_vp), ("d_depth", c_vp), ("d_normal", c_vp), ("d_opacity", c_vp), ("d_weights", c_vp),
+        ("target_rgb", c_vp), ("huber_delta", c_f),
+        ("grad_grid", c_vp), ("grad_table", c_vp), ("grad_mlp", c_vp), ("grad_s", c_vp), ("loss", c_vp),
+        ("vr_bitmask", c_vp), ("workspace", c_vp), ("workspace_bytes", c_i64),
+    ]
+
+
+# name -> (restype, argtypes); every entry point returns int except the two strings
+_SIGS = {
+    "vxn_version": (ctypes.c_char_p, []),
+    "vxn_last_error": (ctypes.c_char_p, []),
+    "vxn_grid_query_fwd": (c_i32, [c_vp, c_i32, c_i32, c_i32, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
+    "vxn_grid_query_bwd": (c_i32, [c_vp, c_i32, c_i32, c_i32, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp]),
+    "vxn_gradient_interp_fwd": (c_i32, [c_vp, c_i32, c_i32, c_i32, c_vp, c_i64, c_vp, c_vp, c_vp]),
+    "vxn_gradient_interp_bwd": (c_i32, [c_vp, c_vp, c_i64, c_vp, c_i32, c_i32, c_i32, c_vp, c_vp]),
+    "vxn_lattice_query": (c_i32, [c_vp, c_i32, c_i32, c_i32, c_vp, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp]),
+    "vxn_hash_fwd": (c_i32, [c_vp, c_i32, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_i64, c_vp, c_vp]),
+    "vxn_hash_bwd": (c_i32, [c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp]),
+    "vxn_upsample": (c_i32, [ctypes.POINTER(SamplerArgs), c_vp]),
+    "vxn_render_workspace_bytes": (c_i32, [ctypes.POINTER(RenderArgs), ctypes.POINTER(c_i64)]),
+    "vxn_render_forward": (c_i32, [ctypes.POINTER(RenderArgs), c_vp]),
+    "vxn_render_backward": (c_i32, [ctypes.POINTER(RenderArgs), c_vp]),
+    "vxn_render_train": (c_i32, [ctypes.POINTER(RenderArgs), c_vp]),
+    "vxn_vr_mark_corners": (c_i32, [c_vp, c_i64, c_i32, c_i32, c_i32, c_vp, c_vp]),
+    "vxn_vr_workspace_bytes": (c_i32, [c_i32, c_i32, c_i32, ctypes.POINTER(c_i64)]),
+    "vxn_vr_compact": (c_i32, [c_vp, c_i32, c_i32, c_i32, c_vp, c_f, c_vp, c_vp, c_vp, c_i64, c_vp]),
+    "vxn_reg_workspace_bytes": (c_i32, [c_i32, c_i32, c_i32, ctypes.POINTER(c_i64)]),
+    "vxn_reg_eik_curv": (c_i32, [c_vp, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_f, c_f, c_vp, c_vp, c_vp, c_i64,
+                                 c_vp]),
+    "vxn_eikonal_grad": (c_i32, [c_vp, c_i32, c_i32, c_i32, c_vp, c_i64, c_vp, c_vp, c_f, c_vp]),
+    "vxn_curvature_grad": (c_i32, [c_vp, c_i32, c_i32, c_i32, c_vp, c_i64, c_vp, c_vp, c_f, c_vp]),
+    "vxn_s_amplify": (c_i32, [c_vp, c_f, c_vp, c_vp]),
+}
+
+_lib = None
+
+
+def exported_symbols() -> Sequence[str]:
+    return list(_SIGS)
+
+
+def lib():
+    """Load (once) and return the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m voxneus_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+class HipError(RuntimeError):
+    pass
+
+
+def check(rc: int, what: str = ""):
+    if rc != 0:
+        msg = lib().vxn_last_error().decode(errors="replace")
+        raise HipError(f"{what or 'voxneus_hip'} failed ({rc}): {msg}")
+
+
+def stream_ptr(device: Optional[torch.device] = None) -> int:
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+def dptr(t: Optional[torch.Tensor], dtype: Optional[torch.dtype] = None, name: str = "tensor") -> Optional[int]:
+    """Device pointer of a contiguous HIP tensor (None -> null)."""
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise ValueError(f"{name} must be on a HIP device (got {t.device})")
+    if dtype is not None and t.dtype != dtype:
+        raise TypeError(f"{name} must be {dtype} (got {t.dtype})")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    return t.data_ptr()
+
+
+def box6(xyz_min, xyz_max):
+    vals = [float(v) for v in list(xyz_min) + list(xyz_max)]
+    return (c_f * 6)(*vals)
+
+
+def f3(vals):
+    return (c_f * 3)(*[float(v) for v in vals])

@@ -1,0 +1,227 @@
+// Dense SDF grid queries: trilinear SDF + Eq. 7 interpolated gradient, their backward,
+// the reference-shaped gradient-interp pair and the full-lattice mesh query.
+#include "vxn_host.h"
+
+namespace vxn {
+
+__global__ void __launch_bounds__(256) grid_query_fwd_kernel(GridDesc g, const float* __restrict__ xyz, int64_t n,
+                                                             float* __restrict__ sdf, float* __restrict__ grad,
+                                                             float* __restrict__ normal,
+                                                             int64_t* __restrict__ corners,
+                                                             float* __restrict__ cweights) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const float px = xyz[3 * i], py = xyz[3 * i + 1], pz = xyz[3 * i + 2];
+    if (sdf) {
+      const GsStencil s = gs_stencil(g, px, py, pz);
+      sdf[i] = gs_eval(g, s);
+    }
+    const Corners c = corner_decomp(g, px, py, pz);
+    if (grad || normal) {
+      float gv[3];
+      eq7_gradient(g, c, gv);
+      if (grad) {
+        grad[3 * i] = gv[0];
+        grad[3 * i + 1] = gv[1];
+        grad[3 * i + 2] = gv[2];
+      }
+      if (normal) {
+        const float r = norm3(gv) + 1e-8f;
+        normal[3 * i] = gv[0] / r;
+        normal[3 * i + 1] = gv[1] / r;
+        normal[3 * i + 2] = gv[2] / r;
+      }
+    }
+    if (corners) {
+      for (int k = 0; k < 8; ++k) {
+        int x, y, z;
+        corner_xyz(c, k, x, y, z);
+        corners[24 * i + 3 * k] = x;
+        corners[24 * i + 3 * k + 1] = y;
+        corners[24 * i + 3 * k + 2] = z;
+      }
+    }
+    if (cweights)
+      for (int k = 0; k < 8; ++k) cweights[8 * i + k] = c.w[k];
+  }
+}
+
+__global__ void __launch_bounds__(256) grid_query_bwd_kernel(GridDesc g, const float* __restrict__ xyz, int64_t n,
+                                                             const float* __restrict__ d_sdf,
+                                                             const float* __restrict__ d_grad,
+                                                             float* __restrict__ grad_grid) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const float px = xyz[3 * i], py = xyz[3 * i + 1], pz = xyz[3 * i + 2];
+    if (d_sdf) {
+      const GsStencil s = gs_stencil(g, px, py, pz);
+      gs_scatter(g, s, d_sdf[i], grad_grid);
+    }
+    if (d_grad) {
+      const Corners c = corner_decomp(g, px, py, pz);
+      const float dg[3] = {d_grad[3 * i], d_grad[3 * i + 1], d_grad[3 * i + 2]};
+      eq7_scatter(g, c, dg, grad_grid);
+    }
+  }
+}
+
+// Continuous grid coordinates in, as the reference's compiled backend takes them.
+VXN_DEV Corners corners_from_coords(const GridDesc& g, float cx, float cy, float cz) {
+  const float p[3] = {cx, cy, cz};
+  const int res[3] = {g.X, g.Y, g.Z};
+  float d[3];
+  Corners c;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float cc = fminf(fmaxf(p[a], 0.0f), g.bound[a]);
+    int i0 = (int)floorf(cc);
+    int i1 = i0 + 1;
+    i0 = min(max(i0, 0), res[a] - 1);
+    i1 = min(max(i1, 0), res[a] - 1);
+    c.c0[a] = i0;
+    c.c1[a] = i1;
+    d[a] = cc - (float)i0;
+  }
+  const float ox = 1.0f - d[0], oy = 1.0f - d[1], oz = 1.0f - d[2];
+  c.w[0] = ox * oy * oz;
+  c.w[1] = ox * oy * d[2];
+  c.w[2] = ox * d[1] * oz;
+  c.w[3] = ox * d[1] * d[2];
+  c.w[4] = d[0] * oy * oz;
+  c.w[5] = d[0] * oy * d[2];
+  c.w[6] = d[0] * d[1] * oz;
+  c.w[7] = d[0] * d[1] * d[2];
+  return c;
+}
+
+__global__ void __launch_bounds__(256) gradient_interp_fwd_kernel(GridDesc g, const float* __restrict__ coords,
+                                                                  int64_t n, float* __restrict__ out) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const Corners c = corners_from_coords(g, coords[3 * i], coords[3 * i + 1], coords[3 * i + 2]);
+    float gv[3];
+    eq7_gradient(g, c, gv);
+    out[3 * i] = gv[0];
+    out[3 * i + 1] = gv[1];
+    out[3 * i + 2] = gv[2];
+  }
+}
+
+__global__ void __launch_bounds__(256) gradient_interp_bwd_kernel(GridDesc g, const float* __restrict__ coords,
+                                                                  int64_t n, const float* __restrict__ gout,
+                                                                  float* __restrict__ grad_grid) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const Corners c = corners_from_coords(g, coords[3 * i], coords[3 * i + 1], coords[3 * i + 2]);
+    const float dg[3] = {gout[3 * i], gout[3 * i + 1], gout[3 * i + 2]};
+    eq7_scatter(g, c, dg, grad_grid);
+  }
+}
+
+// Lattice query: one thread per lattice point; consecutive threads walk z, so the
+// 8-corner and 24-neighbour stencil loads of a wavefront fall on a few cache lines.
+__global__ void __launch_bounds__(256) lattice_query_kernel(GridDesc g, int NX, int NY, int NZ, float lx0, float ly0,
+                                                            float lz0, float lsx, float lsy, float lsz,
+                                                            float* __restrict__ sdf, float* __restrict__ normal) {
+  const int64_t n = (int64_t)NX * NY * NZ;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int iz = (int)(i % NZ);
+    const int64_t r = i / NZ;
+    const int iy = (int)(r % NY);
+    const int ix = (int)(r / NY);
+    const float px = lx0 + (float)ix * lsx, py = ly0 + (float)iy * lsy, pz = lz0 + (float)iz * lsz;
+    const GsStencil s = gs_stencil(g, px, py, pz);
+    sdf[i] = gs_eval(g, s);
+    const Corners c = corner_decomp(g, px, py, pz);
+    float gv[3];
+    eq7_gradient(g, c, gv);
+    const float rr = norm3(gv) + 1e-8f;
+    normal[3 * i] = gv[0] / rr;
+    normal[3 * i + 1] = gv[1] / rr;
+    normal[3 * i + 2] = gv[2] / rr;
+  }
+}
+
+static GridDesc coords_desc(const float* grid, int X, int Y, int Z, const float* eps3) {
+  GridDesc g;
+  g.grid = grid;
+  g.X = X;
+  g.Y = Y;
+  g.Z = Z;
+  const int res[3] = {X, Y, Z};
+  for (int a = 0; a < 3; ++a) {
+    g.mn[a] = 0.0f;
+    g.den[a] = 1.0f;
+    g.rm1[a] = (float)res[a] - 1.0f;
+    g.bound[a] = (float)((double)res[a] - 1.0001);
+    g.eps[a] = eps3[a];
+  }
+  return g;
+}
+
+}  // namespace vxn
+
+using namespace vxn;
+
+extern "C" int vxn_grid_query_fwd(const float* grid, int X, int Y, int Z, const float* box6, const float* xyz,
+                                  int64_t n, float* sdf, float* grad, float* normal, int64_t* corners,
+                                  float* corner_weights, void* stream) {
+  if (int e = check_grid(grid, X, Y, Z, box6)) return e;
+  VXN_REQUIRE(n >= 0, VXN_ERR_SHAPE, "n must be >= 0");
+  if (n == 0) return 0;
+  VXN_REQUIRE(xyz != nullptr, VXN_ERR_NULL, "xyz is null");
+  const GridDesc g = make_grid_desc(grid, X, Y, Z, box6);
+  hipLaunchKernelGGL(grid_query_fwd_kernel, dim3(blocks_for(n, 256, 65536)), dim3(256), 0, (hipStream_t)stream, g,
+                     xyz, n, sdf, grad, normal, corners, corner_weights);
+  VXN_LAUNCH_CHECK();
+  return 0;
+}
+
+extern "C" int vxn_grid_query_bwd(const float* grid, int X, int Y, int Z, const float* box6, const float* xyz,
+                                  int64_t n, const float* d_sdf, const float* d_grad, float* grad_grid,
+                                  void* stream) {
+  if (int e = check_grid(grid, X, Y, Z, box6)) return e;
+  VXN_REQUIRE(n >= 0, VXN_ERR_SHAPE, "n must be >= 0");
+  if (n == 0 || (!d_sdf && !d_grad)) return 0;
+  VXN_REQUIRE(xyz != nullptr && grad_grid != nullptr, VXN_ERR_NULL, "xyz or grad_grid is null");
+  const GridDesc g = make_grid_desc(grid, X, Y, Z, box6);
+  hipLaunchKernelGGL(grid_query_bwd_kernel, dim3(blocks_for(n, 256, 65536)), dim3(256), 0, (hipStream_t)stream, g,
+                     xyz, n, d_sdf, d_grad, grad_grid);
+  VXN_LAUNCH_CHECK();
+  return 0;
+}
+
+extern "C" int vxn_gradient_interp_fwd(const float* grid, int X, int Y, int Z, const float* coords, int64_t n,
+                                       const float* eps3, float* out, void* stream) {
+  VXN_REQUIRE(grid && coords && eps3 && out, VXN_ERR_NULL, "null argument");
+  VXN_REQUIRE(X >= 2 && Y >= 2 && Z >= 2 && n >= 0, VXN_ERR_SHAPE, "bad shape");
+  if (n == 0) return 0;
+  const GridDesc g = coords_desc(grid, X, Y, Z, eps3);
+  hipLaunchKernelGGL(gradient_interp_fwd_kernel, dim3(blocks_for(n, 256, 65536)), dim3(256), 0, (hipStream_t)stream,
+                     g, coords, n, out);
+  VXN_LAUNCH_CHECK();
+  return 0;
+}
+
+extern "C" int vxn_gradient_interp_bwd(const float* grad_out, const float* coords, int64_t n, float* grad_grid, int X,
+                                       int Y, int Z, const float* eps3, void* stream) {
+  VXN_REQUIRE(grad_out && coords && eps3 && grad_grid, VXN_ERR_NULL, "null argument");
+  VXN_REQUIRE(X >= 2 && Y >= 2 && Z >= 2 && n >= 0, VXN_ERR_SHAPE, "bad shape");
+  if (n == 0) return 0;
+  const GridDesc g = coords_desc(nullptr, X, Y, Z, eps3);
+  hipLaunchKernelGGL(gradient_interp_bwd_kernel, dim3(blocks_for(n, 256, 65536)), dim3(256), 0, (hipStream_t)stream,
+                     g, coords, n, grad_out, grad_grid);
+  VXN_LAUNCH_CHECK();
+  return 0;
+}
+
+extern "C" int vxn_lattice_query(const float* grid, int X, int Y, int Z, const float* box6, int NX, int NY, int NZ,
+                                 const float* lattice_box6, float* sdf, float* normal, void* stream) {
+  if (int e = check_grid(grid, X, Y, Z, box6)) return e;
+  VXN_REQUIRE(lattice_box6 && sdf && normal, VXN_ERR_NULL, "null argument");
+  VXN_REQUIRE(NX >= 2 && NY >= 2 && NZ >= 2, VXN_ERR_SHAPE, "lattice must be >= 2 per axis");
+  const GridDesc g = make_grid_desc(grid, X, Y, Z, box6);
+  const float* lb = lattice_box6;
+  const int64_t n = (int64_t)NX * NY * NZ;
+  hipLaunchKernelGGL(lattice_query_kernel, dim3(blocks_for(n, 256, 1 << 20)), dim3(256), 0, (hipStream_t)stream, g,
+                     NX, NY, NZ, lb[0], lb[1], lb[2], (lb[3] - lb[0]) / (float)(NX - 1), (lb[4] - lb[1]) / (float)(NY - 1),
+                     (lb[5] - lb[2]) / (float)(NZ - 1), sdf, normal);
+  VXN_LAUNCH_CHECK();
+  return 0;
+}

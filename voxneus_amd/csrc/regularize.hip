@@ -1,0 +1,400 @@
+// Active vertex set V_R (bitmask -> ascending compaction), fused closed-form eikonal +
+// curvature gradients with FP64 loss reduction, reference-shaped single regularisers,
+// and the device-side s-gradient amplification.
+#include "vxn_host.h"
+
+namespace vxn {
+
+constexpr int kCompactThreads = 256;  // bitmask words per block in the compaction
+
+__global__ void __launch_bounds__(256) vr_mark_kernel(const int64_t* __restrict__ corners, int64_t n, int X, int Y,
+                                                      int Z, uint32_t* __restrict__ bits) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    int64_t x = corners[3 * i], y = corners[3 * i + 1], z = corners[3 * i + 2];
+    x = x < 0 ? 0 : (x > X - 1 ? X - 1 : x);  // active_vertices.py:33-35
+    y = y < 0 ? 0 : (y > Y - 1 ? Y - 1 : y);
+    z = z < 0 ? 0 : (z > Z - 1 ? Z - 1 : z);
+    const int64_t lin = (x * Y + y) * Z + z;
+    atomicOr(bits + (lin >> 5), 1u << (lin & 31));
+  }
+}
+
+// Pass 1: per-block popcount totals.
+__global__ void __launch_bounds__(kCompactThreads) vr_count_kernel(const uint32_t* __restrict__ bits, int64_t nwords,
+                                                                   int32_t* __restrict__ block_tot,
+                                                                   int32_t* __restrict__ counts) {
+  __shared__ int32_t red[kCompactThreads / 64];
+  const int64_t w = blockIdx.x * (int64_t)kCompactThreads + threadIdx.x;
+  int c = w < nwords ? __popc(bits[w]) : 0;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int s = 0;
+    for (int i = 0; i < kCompactThreads / 64; ++i) s += red[i];
+    block_tot[blockIdx.x] = s;
+    if (blockIdx.x == 0) counts[1] = 0;
+  }
+}
+
+// Pass 2: exclusive scan of block totals (single work-group, sequential chunks).
+__global__ void __launch_bounds__(256) vr_scan_kernel(int32_t* __restrict__ block_tot, int nblocks,
+                                                      int32_t* __restrict__ counts) {
+  __shared__ int32_t carry_s;
+  __shared__ int32_t wsum[4];
+  if (threadIdx.x == 0) carry_s = 0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int base = 0; base < nblocks; base += 256) {
+    const int i = base + threadIdx.x;
+    const int v = i < nblocks ? block_tot[i] : 0;
+    int incl = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int u = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += u;
+    }
+    if (lane == 63) wsum[wid] = incl;
+    __syncthreads();
+    int before = carry_s;
+    for (int k = 0; k < wid; ++k) before += wsum[k];
+    if (i < nblocks) block_tot[i] = before + incl - v;
+    __syncthreads();
+    if (threadIdx.x == 255) carry_s = before + incl;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) counts[0] = carry_s;
+}
+
+// Pass 3: write ascending linear indices; bit 31 flags "outside the relaxed sphere".
+__global__ void __launch_bounds__(kCompactThreads) vr_write_kernel(const uint32_t* __restrict__ bits, int64_t nwords,
+                                                                   const int32_t* __restrict__ block_off, int X,
+                                                                   int Y, int Z, float mnx, float mny, float mnz,
+                                                                   float ex, float ey, float ez, float relax,
+                                                                   int32_t* __restrict__ out,
+                                                                   int32_t* __restrict__ counts) {
+  __shared__ int32_t wsum[kCompactThreads / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int64_t w = blockIdx.x * (int64_t)kCompactThreads + threadIdx.x;
+  uint32_t word = w < nwords ? bits[w] : 0u;
+  const int c = __popc(word);
+  int incl = c;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int u = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += u;
+  }
+  if (lane == 63) wsum[wid] = incl;
+  __syncthreads();
+  int pos = block_off[blockIdx.x] + incl - c;
+  for (int k = 0; k < wid; ++k) pos += wsum[k];
+  int inside_n = 0;
+  const int64_t YZ = (int64_t)Y * Z;
+  while (word) {
+    const int b = __ffs(word) - 1;
+    word &= word - 1;
+    const int64_t lin = w * 32 + b;
+    const int ix = (int)(lin / YZ);
+    const int64_t r = lin - (int64_t)ix * YZ;
+    const int iy = (int)(r / Z);
+    const int iz = (int)(r - (int64_t)iy * Z);
+    // world = xyz_min + idx * voxel_size; |world| < relax  (trainer.py:263-264)
+    const float wx = mnx + (float)ix * ex, wy = mny + (float)iy * ey, wz = mnz + (float)iz * ez;
+    const float rr = sqrtf(wx * wx + wy * wy + wz * wz);
+    const bool inside = rr < relax;
+    inside_n += inside ? 1 : 0;
+    out[pos++] = (int32_t)lin | (inside ? 0 : (int32_t)0x80000000);
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) inside_n += __shfl_xor(inside_n, o, 64);
+  if (lane == 0 && inside_n) atomicAdd(counts + 1, inside_n);
+}
+
+constexpr int kRegThreads = 256;
+
+// Fused eikonal + curvature over V_R.  Partial FP64 sums per block: {eik, curv, n_interior}.
+__global__ void __launch_bounds__(kRegThreads) reg_eik_curv_kernel(GridDesc g, const int32_t* __restrict__ list,
+                                                                   const int32_t* __restrict__ counts, float w_eik,
+                                                                   float w_curv, float* __restrict__ gg,
+                                                                   double* __restrict__ partial) {
+  const int n_all = counts[0], n_in = counts[1];
+  const bool filt = n_in > 0;
+  const int nact = filt ? n_in : n_all;
+  const float scale = (float)(2.0 / (double)(nact > 0 ? nact : 1));
+  const float dx = 2.0f * g.eps[0], dy = 2.0f * g.eps[1], dz = 2.0f * g.eps[2];
+  const float ex2 = g.eps[0] * g.eps[0], ey2 = g.eps[1] * g.eps[1], ez2 = g.eps[2] * g.eps[2];
+  const int64_t YZ = (int64_t)g.Y * g.Z;
+  double se = 0.0, sc = 0.0, cnt = 0.0;
+  for (int i = blockIdx.x * kRegThreads + threadIdx.x; i < n_all; i += gridDim.x * kRegThreads) {
+    const int32_t e = list[i];
+    if (filt && e < 0) continue;
+    const int64_t lin = e & 0x7fffffff;
+    const int ix = (int)(lin / YZ);
+    const int64_t r = lin - (int64_t)ix * YZ;
+    const int iy = (int)(r / g.Z);
+    const int iz = (int)(r - (int64_t)iy * g.Z);
+    if (!(ix > 0 && ix < g.X - 1 && iy > 0 && iy < g.Y - 1 && iz > 0 && iz < g.Z - 1)) continue;
+    const float* f = g.grid;
+    const float fc = f[lin];
+    const float fxp = f[lin + YZ], fxm = f[lin - YZ];
+    const float fyp = f[lin + g.Z], fym = f[lin - g.Z];
+    const float fzp = f[lin + 1], fzm = f[lin - 1];
+    cnt += 1.0;
+    // eikonal (eikonal.py:181-218)
+    const float nx = (fxp - fxm) / dx, ny = (fyp - fym) / dy, nz = (fzp - fzm) / dz;
+    const float nn = sqrtf(nx * nx + ny * ny + nz * nz);
+    const double de = (double)nn - 1.0;
+    se += de * de;
+    if (w_eik != 0.0f) {
+      const float sv = scale * (1.0f - 1.0f / (nn + 1e-8f));
+      const float vx = sv * nx, vy = sv * ny, vz = sv * nz;
+      atomicAdd(gg + lin + YZ, (vx / dx) * w_eik);
+      atomicAdd(gg + lin - YZ, (-vx / dx) * w_eik);
+      atomicAdd(gg + lin + g.Z, (vy / dy) * w_eik);
+      atomicAdd(gg + lin - g.Z, (-vy / dy) * w_eik);
+      atomicAdd(gg + lin + 1, (vz / dz) * w_eik);
+      atomicAdd(gg + lin - 1, (-vz / dz) * w_eik);
+    }
+    // curvature (curvature.py:351-384)
+    const float Lx = ((fxp + fxm) - 2.0f * fc) / ex2;
+    const float Ly = ((fyp + fym) - 2.0f * fc) / ey2;
+    const float Lz = ((fzp + fzm) - 2.0f * fc) / ez2;
+    sc += (double)Lx * Lx + (double)Ly * Ly + (double)Lz * Lz;
+    if (w_curv != 0.0f) {
+      const float gc = scale * ((-2.0f * Lx / ex2 - 2.0f * Ly / ey2) - 2.0f * Lz / ez2);
+      const float gx = scale * (Lx / ex2), gy = scale * (Ly / ey2), gz = scale * (Lz / ez2);
+      atomicAdd(gg + lin, gc * w_curv);
+      atomicAdd(gg + lin + YZ, gx * w_curv);
+      atomicAdd(gg + lin - YZ, gx * w_curv);
+      atomicAdd(gg + lin + g.Z, gy * w_curv);
+      atomicAdd(gg + lin - g.Z, gy * w_curv);
+      atomicAdd(gg + lin + 1, gz * w_curv);
+      atomicAdd(gg + lin - 1, gz * w_curv);
+    }
+  }
+  se = wave_sum_d(se);
+  sc = wave_sum_d(sc);
+  cnt = wave_sum_d(cnt);
+  __shared__ double red[3][kRegThreads / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) {
+    red[0][wid] = se;
+    red[1][wid] = sc;
+    red[2][wid] = cnt;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0, b = 0, c = 0;
+    for (int k = 0; k < kRegThreads / 64; ++k) {
+      a += red[0][k];
+      b += red[1][k];
+      c += red[2][k];
+    }
+    partial[3 * blockIdx.x] = a;
+    partial[3 * blockIdx.x + 1] = b;
+    partial[3 * blockIdx.x + 2] = c;
+  }
+}
+
+// Deterministic final reduction of the per-block partials (fixed order).
+__global__ void __launch_bounds__(64) reg_finish_kernel(const double* __restrict__ partial, int nblocks,
+                                                        double* __restrict__ losses) {
+  double a = 0, b = 0, c = 0;
+  for (int i = threadIdx.x; i < nblocks; i += 64) {
+    a += partial[3 * i];
+    b += partial[3 * i + 1];
+    c += partial[3 * i + 2];
+  }
+  a = wave_sum_d(a);
+  b = wave_sum_d(b);
+  c = wave_sum_d(c);
+  if (threadIdx.x == 0) {
+    losses[0] = c > 0 ? a / c : 0.0;
+    losses[1] = c > 0 ? b / c : 0.0;
+  }
+}
+
+constexpr int kRegBlocks = 1024;
+
+// Reference-shaped regularisers over int64 [V,3] vertices (cuda/eikonal_grad.cu:8-69,
+// cuda/curvature_grad.cu:8-71 semantics: boundary vertices skipped, 2/V scale).
+__global__ void __launch_bounds__(256) eikonal_ref_kernel(GridDesc g, const int64_t* __restrict__ act, int64_t V,
+                                                          float weight, float* __restrict__ gg) {
+  const float scale0 = (float)(2.0 / (double)V);
+  const float dx = 2.0f * g.eps[0], dy = 2.0f * g.eps[1], dz = 2.0f * g.eps[2];
+  const int64_t YZ = (int64_t)g.Y * g.Z;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < V; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t ix = act[3 * i], iy = act[3 * i + 1], iz = act[3 * i + 2];
+    if (ix <= 0 || ix >= g.X - 1 || iy <= 0 || iy >= g.Y - 1 || iz <= 0 || iz >= g.Z - 1) continue;
+    const int64_t lin = ix * YZ + iy * g.Z + iz;
+    const float* f = g.grid;
+    const float nx = (f[lin + YZ] - f[lin - YZ]) / dx;
+    const float ny = (f[lin + g.Z] - f[lin - g.Z]) / dy;
+    const float nz = (f[lin + 1] - f[lin - 1]) / dz;
+    const float nn = sqrtf(nx * nx + ny * ny + nz * nz);
+    const float sv = scale0 * (1.0f - 1.0f / (nn + 1e-8f));
+    const float vx = sv * nx, vy = sv * ny, vz = sv * nz;
+    atomicAdd(gg + lin + YZ, (vx / dx) * weight);
+    atomicAdd(gg + lin - YZ, (-vx / dx) * weight);
+    atomicAdd(gg + lin + g.Z, (vy / dy) * weight);
+    atomicAdd(gg + lin - g.Z, (-vy / dy) * weight);
+    atomicAdd(gg + lin + 1, (vz / dz) * weight);
+    atomicAdd(gg + lin - 1, (-vz / dz) * weight);
+  }
+}
+
+__global__ void __launch_bounds__(256) curvature_ref_kernel(GridDesc g, const int64_t* __restrict__ act, int64_t V,
+                                                            float weight, float* __restrict__ gg) {
+  const float scale = (float)(2.0 / (double)V);
+  const float ex2 = g.eps[0] * g.eps[0], ey2 = g.eps[1] * g.eps[1], ez2 = g.eps[2] * g.eps[2];
+  const int64_t YZ = (int64_t)g.Y * g.Z;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < V; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t ix = act[3 * i], iy = act[3 * i + 1], iz = act[3 * i + 2];
+    if (ix <= 0 || ix >= g.X - 1 || iy <= 0 || iy >= g.Y - 1 || iz <= 0 || iz >= g.Z - 1) continue;
+    const int64_t lin = ix * YZ + iy * g.Z + iz;
+    const float* f = g.grid;
+    const float fc = f[lin];
+    const float Lx = ((f[lin + YZ] + f[lin - YZ]) - 2.0f * fc) / ex2;
+    const float Ly = ((f[lin + g.Z] + f[lin - g.Z]) - 2.0f * fc) / ey2;
+    const float Lz = ((f[lin + 1] + f[lin - 1]) - 2.0f * fc) / ez2;
+    const float gc = scale * ((-2.0f * Lx / ex2 - 2.0f * Ly / ey2) - 2.0f * Lz / ez2);
+    const float gx = scale * (Lx / ex2), gy = scale * (Ly / ey2), gz = scale * (Lz / ez2);
+    atomicAdd(gg + lin, gc * weight);
+    atomicAdd(gg + lin + YZ, gx * weight);
+    atomicAdd(gg + lin - YZ, gx * weight);
+    atomicAdd(gg + lin + g.Z, gy * weight);
+    atomicAdd(gg + lin - g.Z, gy * weight);
+    atomicAdd(gg + lin + 1, gz * weight);
+    atomicAdd(gg + lin - 1, gz * weight);
+  }
+}
+
+__global__ void s_amplify_kernel(float* grad_v, float k, int32_t* amplified) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    const float gv = grad_v[0];
+    const bool neg = gv < 0.0f;
+    if (neg) grad_v[0] = gv * k;
+    if (amplified) amplified[0] = neg ? 1 : 0;
+  }
+}
+
+static int64_t vr_nwords(int X, int Y, int Z) { return ((int64_t)X * Y * Z + 31) / 32; }
+static int vr_nblocks(int X, int Y, int Z) { return (int)((vr_nwords(X, Y, Z) + kCompactThreads - 1) / kCompactThreads); }
+
+}  // namespace vxn
+
+using namespace vxn;
+
+extern "C" int vxn_vr_mark_corners(const int64_t* corners, int64_t n, int X, int Y, int Z, uint32_t* bitmask,
+                                   void* stream) {
+  VXN_REQUIRE(bitmask != nullptr, VXN_ERR_NULL, "bitmask is null");
+  VXN_REQUIRE(X >= 1 && Y >= 1 && Z >= 1 && n >= 0, VXN_ERR_SHAPE, "bad shape");
+  if (n == 0) return 0;
+  VXN_REQUIRE(corners != nullptr, VXN_ERR_NULL, "corners is null");
+  hipLaunchKernelGGL(vr_mark_kernel, dim3(blocks_for(n, 256, 65536)), dim3(256), 0, (hipStream_t)stream, corners, n,
+                     X, Y, Z, bitmask);
+  VXN_LAUNCH_CHECK();
+  return 0;
+}
+
+extern "C" int vxn_vr_workspace_bytes(int X, int Y, int Z, int64_t* bytes) {
+  VXN_REQUIRE(bytes != nullptr, VXN_ERR_NULL, "bytes is null");
+  VXN_REQUIRE(X >= 1 && Y >= 1 && Z >= 1, VXN_ERR_SHAPE, "bad shape");
+  *bytes = (int64_t)vr_nblocks(X, Y, Z) * (int64_t)sizeof(int32_t);
+  return 0;
+}
+
+extern "C" int vxn_vr_compact(const uint32_t* bitmask, int X, int Y, int Z, const float* box6, float relax_radius,
+                              int32_t* vr_list, int32_t* counts, void* workspace, int64_t workspace_bytes,
+                              void* stream) {
+  VXN_REQUIRE(bitmask && box6 && vr_list && counts, VXN_ERR_NULL, "null argument");
+  VXN_REQUIRE(X >= 2 && Y >= 2 && Z >= 2, VXN_ERR_SHAPE, "bad shape");
+  VXN_REQUIRE((int64_t)X * Y * Z < ((int64_t)1 << 31), VXN_ERR_SHAPE, "grid too large for int32 V_R entries");
+  const int nb = vr_nblocks(X, Y, Z);
+  VXN_REQUIRE(workspace != nullptr && workspace_bytes >= (int64_t)nb * 4, VXN_ERR_WORKSPACE,
+              "V_R workspace needs %lld bytes", (long long)nb * 4);
+  const int64_t nwords = vr_nwords(X, Y, Z);
+  hipStream_t s = (hipStream_t)stream;
+  int32_t* bt = (int32_t*)workspace;
+  hipLaunchKernelGGL(vr_count_kernel, dim3(nb), dim3(kCompactThreads), 0, s, bitmask, nwords, bt, counts);
+  VXN_LAUNCH_CHECK();
+  hipLaunchKernelGGL(vr_scan_kernel, dim3(1), dim3(256), 0, s, bt, nb, counts);
+  VXN_LAUNCH_CHECK();
+  const int res[3] = {X, Y, Z};
+  float e[3];
+  for (int a = 0; a < 3; ++a) e[a] = (box6[3 + a] - box6[a]) / ((float)res[a] - 1.0f);
+  hipLaunchKernelGGL(vr_write_kernel, dim3(nb), dim3(kCompactThreads), 0, s, bitmask, nwords, bt, X, Y, Z, box6[0],
+                     box6[1], box6[2], e[0], e[1], e[2], relax_radius, vr_list, counts);
+  VXN_LAUNCH_CHECK();
+  return 0;
+}
+
+extern "C" int vxn_reg_workspace_bytes(int X, int Y, int Z, int64_t* bytes) {
+  VXN_REQUIRE(bytes != nullptr, VXN_ERR_NULL, "bytes is null");
+  (void)X;
+  (void)Y;
+  (void)Z;
+  *bytes = (int64_t)kRegBlocks * 3 * (int64_t)sizeof(double);
+  return 0;
+}
+
+extern "C" int vxn_reg_eik_curv(const float* grid, int X, int Y, int Z, const float* box6, const int32_t* vr_list,
+                                const int32_t* counts, float w_eik, float w_curv, float* grad_grid, double* losses,
+                                void* workspace, int64_t workspace_bytes, void* stream) {
+  if (int e = check_grid(grid, X, Y, Z, box6)) return e;
+  VXN_REQUIRE(vr_list && counts && grad_grid && losses, VXN_ERR_NULL, "null argument");
+  VXN_REQUIRE(workspace != nullptr && workspace_bytes >= (int64_t)kRegBlocks * 3 * 8, VXN_ERR_WORKSPACE,
+              "regulariser workspace needs %lld bytes", (long long)kRegBlocks * 24);
+  const GridDesc g = make_grid_desc(grid, X, Y, Z, box6);
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(reg_eik_curv_kernel, dim3(kRegBlocks), dim3(kRegThreads), 0, s, g, vr_list, counts, w_eik, w_curv,
+                     grad_grid, (double*)workspace);
+  VXN_LAUNCH_CHECK();
+  hipLaunchKernelGGL(reg_finish_kernel, dim3(1), dim3(64), 0, s, (const double*)workspace, kRegBlocks, losses);
+  VXN_LAUNCH_CHECK();
+  return 0;
+}
+
+extern "C" int vxn_eikonal_grad(const float* grid, int X, int Y, int Z, const int64_t* active, int64_t V,
+                                float* grad_grid, const float* eps3, float weight, void* stream) {
+  VXN_REQUIRE(grid && grad_grid && eps3, VXN_ERR_NULL, "null argument");
+  VXN_REQUIRE(X >= 2 && Y >= 2 && Z >= 2 && V >= 0, VXN_ERR_SHAPE, "bad shape");
+  if (V == 0) return 0;
+  VXN_REQUIRE(active != nullptr, VXN_ERR_NULL, "active is null");
+  GridDesc g = {};
+  g.grid = grid;
+  g.X = X;
+  g.Y = Y;
+  g.Z = Z;
+  for (int a = 0; a < 3; ++a) g.eps[a] = eps3[a];
+  hipLaunchKernelGGL(eikonal_ref_kernel, dim3(blocks_for(V, 256, 65536)), dim3(256), 0, (hipStream_t)stream, g,
+                     active, V, weight, grad_grid);
+  VXN_LAUNCH_CHECK();
+  return 0;
+}
+
+extern "C" int vxn_curvature_grad(const float* grid, int X, int Y, int Z, const int64_t* active, int64_t V,
+                                  float* grad_grid, const float* eps3, float weight, void* stream) {
+  VXN_REQUIRE(grid && grad_grid && eps3, VXN_ERR_NULL, "null argument");
+  VXN_REQUIRE(X >= 2 && Y >= 2 && Z >= 2 && V >= 0, VXN_ERR_SHAPE, "bad shape");
+  if (V == 0) return 0;
+  VXN_REQUIRE(active != nullptr, VXN_ERR_NULL, "active is null");
+  GridDesc g = {};
+  g.grid = grid;
+  g.X = X;
+  g.Y = Y;
+  g.Z = Z;
+  for (int a = 0; a < 3; ++a) g.eps[a] = eps3[a];
+  hipLaunchKernelGGL(curvature_ref_kernel, dim3(blocks_for(V, 256, 65536)), dim3(256), 0, (hipStream_t)stream, g,
+                     active, V, weight, grad_grid);
+  VXN_LAUNCH_CHECK();
+  return 0;
+}
+
+extern "C" int vxn_s_amplify(float* grad_v, float k, int32_t* amplified, void* stream) {
+  VXN_REQUIRE(grad_v != nullptr, VXN_ERR_NULL, "grad_v is null");
+  hipLaunchKernelGGL(s_amplify_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, grad_v, k, amplified);
+  VXN_LAUNCH_CHECK();
+  return 0;
+}

@@ -1,0 +1,559 @@
+// Fused per-ray render: forward, backward and the fused training step.
+//
+// One ray per 128-thread work-group (two wave64s), thread t = sample t of the ray.  A
+// persistent grid walks the rays; per work-group the MLP parameter gradients are
+// accumulated in registers across all its rays and written once as partials that a
+// fixed-order reduction sums (deterministic).  Per-sample state is recomputed in the
+// backward instead of being stored.
+//
+// Reference: VoxNeuS.forward_points and the render passes 2-3
+// (models/voxneus.py:115-167, 226-258), compute_neus_alpha (rendering/neus_alpha.py:12-81),
+// composite_weights and volume_render_from_samples (rendering/volume_render.py:12-153),
+// and the Huber colour loss (rendering/volume_render.py:43-54).
+#include "vxn_hash.h"
+#include "vxn_host.h"
+
+namespace vxn {
+
+constexpr int kRT = 128;
+constexpr int kNP = VXN_MLP_PARAMS;
+constexpr int kOffB0 = 64 * 51;
+constexpr int kOffW1 = kOffB0 + 64;
+constexpr int kOffB1 = kOffW1 + 3 * 64;
+constexpr int kXS = 37;  // LDS row stride (floats): 32 hash features + 3 normal components
+constexpr int kCS = 17;  // LDS row stride (floats) for a 16-hidden-unit chunk
+constexpr int kStage = kRT * kXS + kRT * 3 + 2 * kRT * kCS;
+
+enum { kFwd = 0, kBwd = 1, kTrain = 2 };
+
+struct RenderDev {
+  const float *o, *d, *mid, *dist;
+  int64_t R;
+  int S;
+  GridDesc g;
+  HashDesc h;
+  const float* mlp;
+  float s, sphere_r, cos_anneal;
+  int iter_cos;
+  float *rgb, *depth, *normal, *opacity, *weights, *alpha, *trans, *sdf_s, *grad_s, *normal_s;
+  int64_t* corners;
+  const float *d_rgb, *d_depth, *d_normal, *d_opacity, *d_weights;
+  const float* target;
+  float huber_delta;
+  float inv_n3;
+  float* grad_grid;
+  float* grad_table;
+  uint32_t* vr;
+  float* part_mlp;  // [gridDim.x][kNP]
+  double* part_sc;  // [gridDim.x][2]: dL/ds, Huber sum
+};
+
+struct Red {
+  double d[2];
+  float f[2][12];
+};
+
+// Exclusive prefix product over the work-group's 128 samples, in double.
+VXN_DEV double block_excl_prod(double v, Red& red, int lane, int wid) {
+  double inc = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const double u = __shfl_up(inc, o, 64);
+    if (lane >= o) inc = u * inc;
+  }
+  if (lane == 63) red.d[wid] = inc;
+  __syncthreads();
+  const double pre = wid == 0 ? 1.0 : red.d[0];
+  const double incl = pre * inc;
+  double ex = __shfl_up(incl, 1, 64);
+  if (lane == 0) ex = pre;
+  __syncthreads();
+  return ex;
+}
+
+// Inclusive prefix sum over the 128 samples and the total, in double.
+VXN_DEV double block_incl_sum(double v, double& total, Red& red, int lane, int wid) {
+  double inc = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const double u = __shfl_up(inc, o, 64);
+    if (lane >= o) inc = u + inc;
+  }
+  if (lane == 63) red.d[wid] = inc;
+  __syncthreads();
+  const double pre = wid == 0 ? 0.0 : red.d[0];
+  total = red.d[0] + red.d[1];
+  __syncthreads();
+  return pre + inc;
+}
+
+// Sums of N floats over the 128 samples (returned to every thread).
+template <int N>
+VXN_DEV void block_sums(float (&v)[N], Red& red, int lane, int wid) {
+#pragma unroll
+  for (int k = 0; k < N; ++k) v[k] = wave_sum(v[k]);
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < N; ++k) red.f[wid][k] = v[k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < N; ++k) v[k] = red.f[0][k] + red.f[1][k];
+  __syncthreads();
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(kRT) render_kernel(RenderDev a) {
+  __shared__ Red red;
+  __shared__ float stage[MODE == kFwd ? 1 : kStage];
+  const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+  const float* W0 = a.mlp;
+  const float* b0 = a.mlp + kOffB0;
+  const float* W1 = a.mlp + kOffW1;
+  const float* b1 = a.mlp + kOffB1;
+  // LDS staging for the parameter gradients: inputs Xs[128][kXS] (hash features and
+  // normal), output grads Os[128][3], and per 16-hidden-unit chunk Hs/Ds[128][kCS].
+  float* Xs = stage;
+  float* Os = stage + kRT * kXS;
+  float* Hs = Os + kRT * 3;
+  float* Ds = Hs + kRT * kCS;
+  // Per-thread ownership of parameter-gradient outputs within each hidden chunk:
+  // hidden j = 16*chunk + (t & 15); input columns {g, g+8, ...} of the 35 staged ones
+  // (g = t >> 4) and SH columns {g, g+8}; W1 (c, j) for t < 48; b1[c] for t < 3.
+  const int jl = t & 15, grp = t >> 4;
+  float accX[4][5], accSH[4][2], accW1[4], accB0[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+#pragma unroll
+    for (int k = 0; k < 5; ++k) accX[c][k] = 0.0f;
+    accSH[c][0] = accSH[c][1] = 0.0f;
+    accW1[c] = 0.0f;
+    accB0[c] = 0.0f;
+  }
+  float accb1 = 0.0f;
+  double ds_acc = 0.0, loss_acc = 0.0;
+
+  for (int64_t ray = blockIdx.x; ray < a.R; ray += gridDim.x) {
+    const bool valid = t < a.S;
+    const int64_t si = ray * a.S + t;
+    const float ox = a.o[3 * ray], oy = a.o[3 * ray + 1], oz = a.o[3 * ray + 2];
+    const float dx = a.d[3 * ray], dy = a.d[3 * ray + 1], dz = a.d[3 * ray + 2];
+    const float mid = valid ? a.mid[si] : 0.0f;
+    const float dist = valid ? a.dist[si] : 0.0f;
+    // ---------------------------------------------------------------- sample forward
+    const float px = ox + dx * mid, py = oy + dy * mid, pz = oz + dz * mid;
+    const GsStencil gs = gs_stencil(a.g, px, py, pz);
+    const float sdf = gs_eval(a.g, gs);
+    const Corners cn = corner_decomp(a.g, px, py, pz);
+    float gv[3];
+    eq7_gradient(a.g, cn, gv);
+    const float gr = norm3(gv);
+    const float nrm[3] = {gv[0] / (gr + 1e-8f), gv[1] / (gr + 1e-8f), gv[2] / (gr + 1e-8f)};
+    const float x0 = fminf(fmaxf((px - a.g.mn[0]) / a.g.den[0], 0.0f), 1.0f);
+    const float x1 = fminf(fmaxf((py - a.g.mn[1]) / a.g.den[1], 0.0f), 1.0f);
+    const float x2 = fminf(fmaxf((pz - a.g.mn[2]) / a.g.den[2], 0.0f), 1.0f);
+    float x[51];
+#pragma unroll
+    for (int l = 0; l < 16; ++l) hash_level_fwd(a.h, l, x0, x1, x2, x[2 * l], x[2 * l + 1]);
+    sh16(dx, dy, dz, x + 32);
+    x[48] = nrm[0];
+    x[49] = nrm[1];
+    x[50] = nrm[2];
+    // MLP forward, streaming the hidden units (one W0 row per iteration)
+    float o3[3] = {b1[0], b1[1], b1[2]};
+#pragma unroll 2
+    for (int j = 0; j < 64; ++j) {
+      const float* wr = W0 + j * 51;
+      float acc = b0[j];
+#pragma unroll
+      for (int i = 0; i < 51; ++i) acc = fmaf(wr[i], x[i], acc);
+      const float hj = fmaxf(acc, 0.0f);
+      o3[0] = fmaf(W1[j], hj, o3[0]);
+      o3[1] = fmaf(W1[64 + j], hj, o3[1]);
+      o3[2] = fmaf(W1[128 + j], hj, o3[2]);
+    }
+    float rgb[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) rgb[c] = sigmoid_ref(o3[c]);
+    // ---------------------------------------------------------------- Eq. 2 alpha
+    const float tcos = (nrm[0] * dx + nrm[1] * dy) + nrm[2] * dz;
+    float cosv = tcos;
+    if (a.iter_cos) {
+      const float ra = fmaxf(-tcos * 0.5f + 0.5f, 0.0f), rb = fmaxf(-tcos, 0.0f);
+      cosv = -(ra * (1.0f - a.cos_anneal) + rb * a.cos_anneal);
+    }
+    const float off = 0.5f * dist * cosv;
+    const float fp = sdf - off, fn = sdf + off;
+    const float pp = sigmoid_ref(a.s * fp), pn = sigmoid_ref(a.s * fn);
+    const float den = fmaxf(pp, 1e-5f);
+    const float q = (pp - pn) / den;
+    const float araw = fminf(fmaxf(q, 0.0f), 1.0f);
+    const float inside = (a.sphere_r > 0.0f) ? ((sqrtf(px * px + py * py + pz * pz) < a.sphere_r) ? 1.0f : 0.0f) : 1.0f;
+    const float alpha = valid ? araw * inside : 0.0f;
+    const float ov = (1.0f - alpha) + 1e-7f;
+    const float oma = fminf(fmaxf(ov, 0.0f), 1.0f);
+    // ---------------------------------------------------------------- transmittance
+    const float T = (float)block_excl_prod((double)oma, red, lane, wid);
+    const bool keep = !(T < 1e-4f);
+    const float w = keep ? alpha * T : 0.0f;
+    float comp[8] = {w * rgb[0], w * rgb[1], w * rgb[2], w * mid, w, w * nrm[0], w * nrm[1], w * nrm[2]};
+    block_sums<8>(comp, red, lane, wid);
+    const float Nr = sqrtf(comp[5] * comp[5] + comp[6] * comp[6] + comp[7] * comp[7]);
+
+    if (MODE == kFwd) {
+      if (t == 0) {
+        if (a.rgb) {
+          a.rgb[3 * ray] = comp[0];
+          a.rgb[3 * ray + 1] = comp[1];
+          a.rgb[3 * ray + 2] = comp[2];
+        }
+        if (a.depth) a.depth[ray] = comp[3];
+        if (a.opacity) a.opacity[ray] = comp[4];
+        if (a.normal) {
+          a.normal[3 * ray] = comp[5] / (Nr + 1e-8f);
+          a.normal[3 * ray + 1] = comp[6] / (Nr + 1e-8f);
+          a.normal[3 * ray + 2] = comp[7] / (Nr + 1e-8f);
+        }
+      }
+      if (valid) {
+        if (a.weights) a.weights[si] = w;
+        if (a.alpha) a.alpha[si] = alpha;
+        if (a.trans) a.trans[si] = T;
+        if (a.sdf_s) a.sdf_s[si] = sdf;
+        if (a.grad_s) {
+          a.grad_s[3 * si] = gv[0];
+          a.grad_s[3 * si + 1] = gv[1];
+          a.grad_s[3 * si + 2] = gv[2];
+        }
+        if (a.normal_s) {
+          a.normal_s[3 * si] = nrm[0];
+          a.normal_s[3 * si + 1] = nrm[1];
+          a.normal_s[3 * si + 2] = nrm[2];
+        }
+        if (a.corners) {
+#pragma unroll
+          for (int k = 0; k < 8; ++k) {
+            int cx, cy, cz;
+            corner_xyz(cn, k, cx, cy, cz);
+            a.corners[24 * si + 3 * k] = cx;
+            a.corners[24 * si + 3 * k + 1] = cy;
+            a.corners[24 * si + 3 * k + 2] = cz;
+          }
+        }
+      }
+    }
+    if (a.vr && valid) {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        int cx, cy, cz;
+        corner_xyz(cn, k, cx, cy, cz);
+        const int64_t lin = lin3(a.g, cx, cy, cz);
+        atomicOr(a.vr + (lin >> 5), 1u << (lin & 31));
+      }
+    }
+    if (MODE == kFwd) continue;
+
+    // ---------------------------------------------------------------- upstream gradients
+    float dr[3] = {0.0f, 0.0f, 0.0f}, dd = 0.0f, dop = 0.0f, dN[3] = {0.0f, 0.0f, 0.0f};
+    if (MODE == kTrain) {
+      double ls = 0.0;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const float df = comp[c] - a.target[3 * ray + c];
+        const float ad = fabsf(df);
+        const bool small = ad < a.huber_delta;
+        ls += small ? (double)(0.5f * (ad * ad) / a.huber_delta) : (double)(ad - 0.5f * a.huber_delta);
+        const float sg = df > 0.0f ? 1.0f : (df < 0.0f ? -1.0f : 0.0f);
+        dr[c] = (small ? (ad / a.huber_delta) : 1.0f) * sg * a.inv_n3;
+      }
+      if (t == 0) loss_acc += ls;
+    } else {
+      if (a.d_rgb) {
+        dr[0] = a.d_rgb[3 * ray];
+        dr[1] = a.d_rgb[3 * ray + 1];
+        dr[2] = a.d_rgb[3 * ray + 2];
+      }
+      if (a.d_depth) dd = a.d_depth[ray];
+      if (a.d_opacity) dop = a.d_opacity[ray];
+      if (a.d_normal) {
+        // normal = N / (|N| + 1e-8)
+        const float gn[3] = {a.d_normal[3 * ray], a.d_normal[3 * ray + 1], a.d_normal[3 * ray + 2]};
+        const float rr = Nr + 1e-8f;
+        const float dotv = comp[5] * gn[0] + comp[6] * gn[1] + comp[7] * gn[2];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) dN[k] = gn[k] / rr - (Nr > 0.0f ? comp[5 + k] * dotv / (Nr * rr * rr) : 0.0f);
+      }
+    }
+    float gw = (dr[0] * rgb[0] + dr[1] * rgb[1]) + dr[2] * rgb[2] + dd * mid + dop +
+               ((dN[0] * nrm[0] + dN[1] * nrm[1]) + dN[2] * nrm[2]);
+    if (MODE == kBwd && a.d_weights && valid) gw += a.d_weights[si];
+    if (!valid || !keep) gw = 0.0f;
+    // suffix sums of g_i * alpha_i * T_i for the d(T)/d(oma) terms, in double
+    double tot;
+    const double qi = (double)gw * (double)alpha * (double)T;
+    const double pre = block_incl_sum(qi, tot, red, lane, wid);
+    const double suffix = tot - pre;
+    float dA = gw * T;
+    if (ov <= 1.0f && ov >= 0.0f) dA -= (float)(suffix / (double)oma);
+    const float dAraw = valid ? dA * inside : 0.0f;
+    const float dq = (q >= 0.0f && q <= 1.0f) ? dAraw : 0.0f;
+    float dpp = dq / den;
+    const float dpn = -dq / den;
+    if (pp >= 1e-5f) dpp -= dq * q / den;
+    const float dzp = dpp * (1.0f - pp) * pp;
+    const float dzn = dpn * (1.0f - pn) * pn;
+    ds_acc += (double)dzp * fp + (double)dzn * fn;
+    const float dfp = dzp * a.s, dfn = dzn * a.s;
+    const float dsdf = dfp + dfn;
+    const float doff = dfn - dfp;
+    float dcos = doff * 0.5f * dist;
+    if (a.iter_cos) {
+      const float ra = -tcos * 0.5f + 0.5f, rb = -tcos;
+      dcos = dcos * (-((ra > 0.0f ? -0.5f : 0.0f) * (1.0f - a.cos_anneal) + (rb > 0.0f ? -1.0f : 0.0f) * a.cos_anneal));
+    }
+    float dn[3] = {dcos * dx + w * dN[0], dcos * dy + w * dN[1], dcos * dz + w * dN[2]};
+
+    // ---------------------------------------------------------------- MLP backward
+    float dout[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) dout[c] = valid ? (w * dr[c]) * (1.0f - rgb[c]) * rgb[c] : 0.0f;
+#pragma unroll
+    for (int i = 0; i < 32; ++i) Xs[t * kXS + i] = valid ? x[i] : 0.0f;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      Xs[t * kXS + 32 + k] = valid ? x[48 + k] : 0.0f;
+      Os[t * 3 + k] = dout[k];
+    }
+    float dxa[35];
+#pragma unroll
+    for (int i = 0; i < 35; ++i) dxa[i] = 0.0f;
+#pragma unroll 1
+    for (int cc = 0; cc < 4; ++cc) {
+      // recompute this chunk's hidden units; dz0 = relu'(h) * (W1^T dout); dx += W0^T dz0
+#pragma unroll 2
+      for (int jj = 0; jj < 16; ++jj) {
+        const int j = 16 * cc + jj;
+        const float* wr = W0 + j * 51;
+        float acc = b0[j];
+#pragma unroll
+        for (int i = 0; i < 51; ++i) acc = fmaf(wr[i], x[i], acc);
+        const float hj = fmaxf(acc, 0.0f);
+        const float dh = (W1[j] * dout[0] + W1[64 + j] * dout[1]) + W1[128 + j] * dout[2];
+        const float dzj = hj > 0.0f ? dh : 0.0f;
+#pragma unroll
+        for (int i = 0; i < 32; ++i) dxa[i] = fmaf(wr[i], dzj, dxa[i]);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) dxa[32 + k] = fmaf(wr[48 + k], dzj, dxa[32 + k]);
+        Hs[t * kCS + jj] = valid ? hj : 0.0f;
+        Ds[t * kCS + jj] = valid ? dzj : 0.0f;
+      }
+      __syncthreads();
+      // accumulate this chunk's parameter gradients over the ray's 128 samples
+      float aX[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f}, dsum = 0.0f, aW1 = 0.0f;
+      const int c1 = t < 48 ? (t >> 4) : 0;
+#pragma unroll 4
+      for (int s2 = 0; s2 < kRT; ++s2) {
+        const float dv = Ds[s2 * kCS + jl];
+        const float* xr = Xs + s2 * kXS + grp;
+#pragma unroll
+        for (int k = 0; k < 5; ++k)
+          if (grp + 8 * k < 35) aX[k] = fmaf(dv, xr[8 * k], aX[k]);
+        dsum += dv;
+        aW1 = fmaf(Hs[s2 * kCS + jl], Os[s2 * 3 + c1], aW1);
+      }
+#pragma unroll
+      for (int k = 0; k < 5; ++k) accX[cc][k] += aX[k];
+      accSH[cc][0] = fmaf(x[32 + grp], dsum, accSH[cc][0]);
+      accSH[cc][1] = fmaf(x[32 + grp + 8], dsum, accSH[cc][1]);
+      accB0[cc] += dsum;
+      accW1[cc] += aW1;
+      __syncthreads();
+    }
+    if (t < 3)
+      for (int s2 = 0; s2 < kRT; ++s2) accb1 += Os[s2 * 3 + t];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) dn[k] += dxa[32 + k];
+    // ---------------------------------------------------------------- scatters
+    if (valid) {
+#pragma unroll
+      for (int l = 0; l < 16; ++l) hash_level_bwd(a.h, l, x0, x1, x2, dxa[2 * l], dxa[2 * l + 1], a.grad_table);
+      const float rr = gr + 1e-8f;
+      const float dotv = (dn[0] * gv[0] + dn[1] * gv[1]) + dn[2] * gv[2];
+      float dg[3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) dg[k] = dn[k] / rr - (gr > 0.0f ? gv[k] * dotv / (gr * rr * rr) : 0.0f);
+      eq7_scatter(a.g, cn, dg, a.grad_grid);
+      if (dsdf != 0.0f) gs_scatter(a.g, gs, dsdf, a.grad_grid);
+    }
+    __syncthreads();
+  }
+
+  if (MODE == kFwd) return;
+  // ---------------------------------------------------------------- write partials
+  float* P = a.part_mlp + (int64_t)blockIdx.x * kNP;
+#pragma unroll
+  for (int cc = 0; cc < 4; ++cc) {
+    const int j = 16 * cc + jl;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      const int col = grp + 8 * k;  // 0..31 hash features, 32..34 normal -> W0 columns 48..50
+      if (col < 35) P[j * 51 + (col < 32 ? col : col + 16)] = accX[cc][k];
+    }
+    P[j * 51 + 32 + grp] = accSH[cc][0];
+    P[j * 51 + 32 + grp + 8] = accSH[cc][1];
+    if (grp == 0) P[kOffB0 + j] = accB0[cc];
+    if (t < 48) P[kOffW1 + (t >> 4) * 64 + j] = accW1[cc];
+  }
+  if (t < 3) P[kOffB1 + t] = accb1;
+  double dsw = wave_sum_d(ds_acc);
+  if (lane == 0) red.d[wid] = dsw;
+  __syncthreads();
+  if (t == 0) {
+    a.part_sc[2 * blockIdx.x] = red.d[0] + red.d[1];
+    a.part_sc[2 * blockIdx.x + 1] = loss_acc;
+  }
+}
+
+// Fixed-order reduction of the per-work-group partials.
+__global__ void __launch_bounds__(256) render_reduce_kernel(const float* __restrict__ part_mlp,
+                                                            const double* __restrict__ part_sc, int nb,
+                                                            float* __restrict__ grad_mlp, float* __restrict__ grad_s,
+                                                            double* __restrict__ loss, float inv_n3) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < kNP) {
+    double acc = 0.0;
+    for (int b = 0; b < nb; ++b) acc += (double)part_mlp[(int64_t)b * kNP + k];
+    grad_mlp[k] = (float)acc;
+  }
+  if (k == kNP) {
+    double ds = 0.0, ls = 0.0;
+    for (int b = 0; b < nb; ++b) {
+      ds += part_sc[2 * b];
+      ls += part_sc[2 * b + 1];
+    }
+    if (grad_s) grad_s[0] = (float)ds;
+    if (loss) loss[0] = ls * (double)inv_n3;
+  }
+}
+
+static int device_cus() {
+  static int cached[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (!cached[dev]) {
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+    cached[dev] = n;
+  }
+  return cached[dev];
+}
+
+static int render_blocks(int64_t R) {
+  const int64_t cap = (int64_t)device_cus() * 4;
+  return (int)(R < cap ? (R > 0 ? R : 1) : cap);
+}
+
+static int64_t render_sc_offset(int64_t R) {
+  const int64_t b = (int64_t)render_blocks(R) * kNP * (int64_t)sizeof(float);
+  return (b + 255) & ~(int64_t)255;
+}
+
+static int64_t render_ws_bytes(int64_t R) {
+  return render_sc_offset(R) + (int64_t)render_blocks(R) * 2 * (int64_t)sizeof(double);
+}
+
+static int build_render_dev(const vxn_render_args& A, RenderDev& a, int mode) {
+  if (int e = check_grid(A.grid, A.X, A.Y, A.Z, A.box)) return e;
+  VXN_REQUIRE(A.n_rays >= 0, VXN_ERR_SHAPE, "n_rays must be >= 0");
+  VXN_REQUIRE(A.n_samples >= 1 && A.n_samples <= kRT, VXN_ERR_UNSUPPORTED, "n_samples must be in [1, %d] (got %d)",
+              kRT, A.n_samples);
+  VXN_REQUIRE(A.n_levels == 16 && A.n_features == 2, VXN_ERR_UNSUPPORTED,
+              "the fused render implements the 16-level x 2-feature hash feeding the 51-input MLP");
+  VXN_REQUIRE(A.mlp_mode == VXN_MLP_FP32, VXN_ERR_UNSUPPORTED, "mlp_mode %d is not available in this build",
+              A.mlp_mode);
+  VXN_REQUIRE(A.rays_o && A.rays_d && A.mid_z && A.dists && A.table && A.mlp, VXN_ERR_NULL, "null input");
+  HashDesc h;
+  if (int e = make_hash_desc(h, A.table, A.table_dtype, A.n_levels, A.log2_T, A.n_features, A.level_res, A.level_w))
+    return e;
+  a.o = A.rays_o;
+  a.d = A.rays_d;
+  a.mid = A.mid_z;
+  a.dist = A.dists;
+  a.R = A.n_rays;
+  a.S = A.n_samples;
+  a.g = make_grid_desc(A.grid, A.X, A.Y, A.Z, A.box);
+  a.h = h;
+  a.mlp = A.mlp;
+  a.s = A.s;
+  a.sphere_r = A.sphere_radius;
+  a.cos_anneal = A.cos_anneal_ratio;
+  a.iter_cos = A.use_iter_cos;
+  a.rgb = A.rgb;
+  a.depth = A.depth;
+  a.normal = A.normal;
+  a.opacity = A.opacity;
+  a.weights = A.weights;
+  a.alpha = A.alpha;
+  a.trans = A.transmittance;
+  a.sdf_s = A.sdf_samples;
+  a.grad_s = A.grad_samples;
+  a.normal_s = A.normal_samples;
+  a.corners = A.corner_indices;
+  a.d_rgb = A.d_rgb;
+  a.d_depth = A.d_depth;
+  a.d_normal = A.d_normal;
+  a.d_opacity = A.d_opacity;
+  a.d_weights = A.d_weights;
+  a.target = A.target_rgb;
+  a.huber_delta = A.huber_delta;
+  a.inv_n3 = A.n_rays > 0 ? (float)(1.0 / (3.0 * (double)A.n_rays)) : 0.0f;
+  a.grad_grid = A.grad_grid;
+  a.grad_table = A.grad_table;
+  a.vr = A.vr_bitmask;
+  a.part_mlp = nullptr;
+  a.part_sc = nullptr;
+  if (mode != kFwd) {
+    VXN_REQUIRE(A.grad_grid && A.grad_table && A.grad_mlp && A.grad_s, VXN_ERR_NULL, "null gradient output");
+    const int64_t need = render_ws_bytes(A.n_rays);
+    VXN_REQUIRE(A.workspace != nullptr && A.workspace_bytes >= need, VXN_ERR_WORKSPACE,
+                "render workspace needs %lld bytes", (long long)need);
+    a.part_mlp = (float*)A.workspace;
+    a.part_sc = (double*)((char*)A.workspace + render_sc_offset(A.n_rays));
+  }
+  if (mode == kTrain) {
+    VXN_REQUIRE(A.target_rgb != nullptr, VXN_ERR_NULL, "target_rgb is null");
+    VXN_REQUIRE(A.huber_delta > 0.0f, VXN_ERR_VALUE, "huber_delta must be > 0");
+  }
+  return 0;
+}
+
+template <int MODE>
+static int launch_render(const vxn_render_args* p, void* stream) {
+  VXN_REQUIRE(p != nullptr, VXN_ERR_NULL, "args is null");
+  RenderDev a;
+  if (int e = build_render_dev(*p, a, MODE)) return e;
+  if (p->n_rays == 0) return 0;
+  const int nb = render_blocks(p->n_rays);
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(render_kernel<MODE>, dim3(nb), dim3(kRT), 0, s, a);
+  VXN_LAUNCH_CHECK();
+  if (MODE != kFwd) {
+    hipLaunchKernelGGL(render_reduce_kernel, dim3((kNP + 1 + 255) / 256), dim3(256), 0, s, a.part_mlp, a.part_sc, nb,
+                       p->grad_mlp, p->grad_s, MODE == kTrain ? p->loss : nullptr, a.inv_n3);
+    VXN_LAUNCH_CHECK();
+  }
+  return 0;
+}
+
+}  // namespace vxn
+
+using namespace vxn;
+
+extern "C" int vxn_render_workspace_bytes(const vxn_render_args* a, int64_t* bytes) {
+  VXN_REQUIRE(a != nullptr && bytes != nullptr, VXN_ERR_NULL, "null argument");
+  *bytes = render_ws_bytes(a->n_rays);
+  return 0;
+}
+
+extern "C" int vxn_render_forward(const vxn_render_args* a, void* stream) { return launch_render<kFwd>(a, stream); }
+extern "C" int vxn_render_backward(const vxn_render_args* a, void* stream) { return launch_render<kBwd>(a, stream); }
+extern "C" int vxn_render_train(const vxn_render_args* a, void* stream) { return launch_render<kTrain>(a, stream); }

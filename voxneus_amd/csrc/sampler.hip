@@ -1,0 +1,303 @@
+// 4-step progressive importance upsampler (NeuS up_sample + sample_pdf + cat_z_vals),
+// one ray per 128-thread work-group, all per-ray state in LDS.
+//
+// Rounding follows the reference's PyTorch CPU evaluation so that the searchsorted
+// indices, and therefore the z values, come out bit-identical:
+//   * sigmoid = SLEEF expf_u10 based (vxn_device.h);
+//   * torch.cumprod / torch.cumsum accumulate float32 inputs in double, sequentially;
+//   * torch.sum over a row of m floats uses an 8-lane vector cascade: 4 vector
+//     accumulators over rows of 4 vectors, leftovers into accumulator 0, accumulators
+//     folded 0+1+2+3, a sequential scalar tail, then the 8 lanes folded in order.
+#include "vxn_host.h"
+
+namespace vxn {
+
+constexpr int kSampThreads = 128;
+constexpr int kMaxS = 128;
+constexpr int kMaxPer = 64;
+
+struct SamplerDev {
+  const float *o, *d, *near, *far;
+  int64_t R;
+  GridDesc g;
+  const uint8_t* occ;
+  int occ_res;
+  float occ_mn[3], occ_den[3];
+  float occ_bound;
+  int n0, per, steps, perturb;
+  const float *t_lin, *u_det, *t_rand, *u_rand;
+  float *z_out, *dists_out, *mid_out;
+  int32_t* inds;
+};
+
+// OccupancyGrid.is_occupied (occupancy_grid.py:121-147) folded into the modulation factor
+// occ.float() * 0.9 + 0.1 (sampler.py:285-289).
+VXN_DEV float occ_factor(const SamplerDev& a, float px, float py, float pz) {
+  const float p[3] = {px, py, pz};
+  bool inb = true;
+  int c[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float nrm = (p[k] - a.occ_mn[k]) / a.occ_den[k];
+    inb = inb && (nrm >= 0.0f) && (nrm <= 1.0f);
+    const float v = fminf(fmaxf(nrm * (float)a.occ_res, 0.0f), a.occ_bound);
+    c[k] = (int)v;
+  }
+  const int64_t r = a.occ_res;
+  const bool occ = inb && a.occ[((int64_t)c[0] * r + c[1]) * r + c[2]] != 0;
+  const float f = occ ? 1.0f : 0.0f;
+  return f * 0.9f + 0.1f;
+}
+
+__global__ void __launch_bounds__(kSampThreads) upsample_kernel(SamplerDev a) {
+  __shared__ float zs[kMaxS], sd[kMaxS], wv[kMaxS], cdf[kMaxS + 1], cosv[kMaxS];
+  __shared__ float nz[kMaxPer], nzs[kMaxPer];
+  __shared__ unsigned char ins[kMaxS];
+  __shared__ float acc8[8];
+  const int t = threadIdx.x;
+  for (int64_t ray = blockIdx.x; ray < a.R; ray += gridDim.x) {
+    const float ox = a.o[3 * ray], oy = a.o[3 * ray + 1], oz = a.o[3 * ray + 2];
+    const float dx = a.d[3 * ray], dy = a.d[3 * ray + 1], dz = a.d[3 * ray + 2];
+    const float nr = a.near[ray], fr = a.far[ray];
+    int n = a.n0;
+    // 1. stratified samples (sampler.py:85-125)
+    float z = 0.0f;
+    if (t < n) {
+      const float tl = a.t_lin[t];
+      z = nr * (1.0f - tl) + fr * tl;
+      zs[t] = z;
+    }
+    __syncthreads();
+    if (a.perturb) {
+      if (t < n) {
+        const float lower = t == 0 ? zs[0] : 0.5f * (zs[t] + zs[t - 1]);
+        const float upper = t == n - 1 ? zs[n - 1] : 0.5f * (zs[t + 1] + zs[t]);
+        z = lower + (upper - lower) * a.t_rand[ray * n + t];
+      }
+      __syncthreads();
+      if (t < n) zs[t] = z;
+      __syncthreads();
+    }
+    // 2. SDF at the stratified points
+    if (t < n) {
+      const float px = ox + dx * z, py = oy + dy * z, pz = oz + dz * z;
+      sd[t] = gs_eval(a.g, gs_stencil(a.g, px, py, pz));
+    }
+    __syncthreads();
+    for (int step = 0; step < a.steps; ++step) {
+      const float inv_s = 64.0f * (float)(1 << step);
+      const int m = n - 1;
+      const bool last = step + 1 == a.steps;
+      if (t < n) {
+        const float zz = zs[t];
+        const float px = ox + dx * zz, py = oy + dy * zz, pz = oz + dz * zz;
+        ins[t] = sqrtf(px * px + py * py + pz * pz) < 1.0f;
+      }
+      if (t < m) cosv[t] = (sd[t + 1] - sd[t]) / ((zs[t + 1] - zs[t]) + 1e-5f);
+      __syncthreads();
+      float alpha = 0.0f;
+      if (t < m) {
+        const float pc = t == 0 ? 0.0f : cosv[t - 1];
+        float c = fminf(pc, cosv[t]);
+        c = fminf(fmaxf(c, -1e3f), 0.0f) * ((ins[t] | ins[t + 1]) ? 1.0f : 0.0f);
+        const float dist = zs[t + 1] - zs[t];
+        const float mid = (sd[t] + sd[t + 1]) * 0.5f;
+        const float pe = mid - c * dist * 0.5f;
+        const float ne = mid + c * dist * 0.5f;
+        const float pcdf = sigmoid_ref(pe * inv_s);
+        const float ncdf = sigmoid_ref(ne * inv_s);
+        alpha = ((pcdf - ncdf) + 1e-5f) / (pcdf + 1e-5f);
+        wv[t] = (1.0f - alpha) + 1e-7f;
+      }
+      __syncthreads();
+      if (t == 0) {  // exclusive cumprod in double, sequential (as torch.cumprod on the CPU)
+        double T = 1.0;
+        for (int j = 0; j < m; ++j) {
+          cdf[j] = (float)T;
+          T *= (double)wv[j];
+        }
+      }
+      __syncthreads();
+      if (t < m) {
+        float w = alpha * cdf[t];
+        if (a.occ) {
+          const float mz = (zs[t] + zs[t + 1]) * 0.5f;
+          w = w * occ_factor(a, ox + dx * mz, oy + dy * mz, oz + dz * mz);
+        }
+        wv[t] = w + 1e-5f;
+      }
+      __syncthreads();
+      // 3. sample_pdf: row sum in the 8-lane cascade order
+      const int nvec = m >> 3, ilp = nvec >> 2;
+      if (t < 8) {
+        float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        for (int i = 0; i < ilp; ++i)
+#pragma unroll
+          for (int k = 0; k < 4; ++k) acc[k] = acc[k] + wv[8 * (4 * i + k) + t];
+        for (int i = 4 * ilp; i < nvec; ++i) acc[0] = acc[0] + wv[8 * i + t];
+        acc8[t] = ((acc[0] + acc[1]) + acc[2]) + acc[3];
+      }
+      __syncthreads();
+      if (t == 0) {
+        float tot = 0.0f;
+        if (nvec == 0) {  // rows shorter than one vector: scalar 4-accumulator cascade
+          float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+          const int q = m >> 2;
+          for (int i = 0; i < q; ++i)
+            for (int k = 0; k < 4; ++k) acc[k] = acc[k] + wv[4 * i + k];
+          for (int i = 4 * q; i < m; ++i) acc[0] = acc[0] + wv[i];
+          tot = ((acc[0] + acc[1]) + acc[2]) + acc[3];
+        } else {
+          for (int k = 8 * nvec; k < m; ++k) tot = tot + wv[k];
+          for (int l = 0; l < 8; ++l) tot = tot + acc8[l];
+        }
+        double c = 0.0;
+        cdf[0] = 0.0f;
+        for (int j = 0; j < m; ++j) {
+          c += (double)(wv[j] / tot);
+          cdf[j + 1] = (float)c;
+        }
+      }
+      __syncthreads();
+      if (t < a.per) {
+        const float u = a.perturb ? a.u_rand[((int64_t)step * a.R + ray) * a.per + t] : a.u_det[t];
+        int lo = 0, hi = n;  // searchsorted(cdf[0..n-1], u, right=True)
+        while (lo < hi) {
+          const int md = (lo + hi) >> 1;
+          if (cdf[md] <= u)
+            lo = md + 1;
+          else
+            hi = md;
+        }
+        const int ind = lo;
+        const int below = max(ind - 1, 0), above = min(ind, n - 1);
+        const float cg0 = cdf[below], cg1 = cdf[above];
+        float den = cg1 - cg0;
+        if (den < 1e-5f) den = 1.0f;
+        const float tt = (u - cg0) / den;
+        const float b0 = zs[below], b1 = zs[above];
+        nz[t] = b0 + tt * (b1 - b0);
+        if (a.inds) a.inds[((int64_t)step * a.R + ray) * a.per + t] = ind;
+      }
+      __syncthreads();
+      // 4. cat_z_vals: stable rank of the new samples, then merge
+      if (t < a.per) {
+        const float v = nz[t];
+        int r = 0;
+        for (int j = 0; j < a.per; ++j) {
+          const float u2 = nz[j];
+          r += (u2 < v) || (u2 == v && j < t);
+        }
+        nzs[r] = v;
+      }
+      __syncthreads();
+      float zo = 0.0f, so = 0.0f, zn = 0.0f, sn = 0.0f;
+      int po = -1, pn = -1;
+      if (t < n) {
+        zo = zs[t];
+        so = sd[t];
+        int lo = 0, hi = a.per;  // count of new < zo
+        while (lo < hi) {
+          const int md = (lo + hi) >> 1;
+          if (nzs[md] < zo)
+            lo = md + 1;
+          else
+            hi = md;
+        }
+        po = t + lo;
+      }
+      if (t < a.per) {
+        zn = nzs[t];
+        int lo = 0, hi = n;  // count of old <= zn
+        while (lo < hi) {
+          const int md = (lo + hi) >> 1;
+          if (zs[md] <= zn)
+            lo = md + 1;
+          else
+            hi = md;
+        }
+        pn = t + lo;
+        if (!last) {
+          const float px = ox + dx * zn, py = oy + dy * zn, pz = oz + dz * zn;
+          sn = gs_eval(a.g, gs_stencil(a.g, px, py, pz));
+        }
+      }
+      __syncthreads();
+      if (po >= 0) {
+        zs[po] = zo;
+        sd[po] = so;
+      }
+      if (pn >= 0) {
+        zs[pn] = zn;
+        sd[pn] = sn;
+      }
+      __syncthreads();
+      n += a.per;
+    }
+    // 5. final intervals and midpoints (sampler.py:197-202)
+    if (t < n) {
+      const float zz = zs[t];
+      const float dist = t < n - 1 ? zs[t + 1] - zz : zs[n - 1] - zs[n - 2];
+      a.z_out[ray * n + t] = zz;
+      a.dists_out[ray * n + t] = dist;
+      a.mid_out[ray * n + t] = zz + dist * 0.5f;
+    }
+    __syncthreads();
+  }
+}
+
+}  // namespace vxn
+
+using namespace vxn;
+
+extern "C" int vxn_upsample(const vxn_sampler_args* p, void* stream) {
+  VXN_REQUIRE(p != nullptr, VXN_ERR_NULL, "args is null");
+  const vxn_sampler_args& A = *p;
+  if (int e = check_grid(A.grid, A.X, A.Y, A.Z, A.box)) return e;
+  VXN_REQUIRE(A.n_rays >= 0, VXN_ERR_SHAPE, "n_rays must be >= 0");
+  VXN_REQUIRE(A.n_samples >= 2, VXN_ERR_SHAPE, "n_samples must be >= 2");
+  VXN_REQUIRE(A.up_sample_steps >= 0 && A.n_importance >= 0, VXN_ERR_VALUE, "negative importance settings");
+  const int steps = (A.n_importance > 0 && A.up_sample_steps > 0) ? A.up_sample_steps : 0;
+  const int per = steps ? (A.n_importance / steps > 1 ? A.n_importance / steps : 1) : 0;
+  const int S = A.n_samples + steps * per;
+  VXN_REQUIRE(S <= kMaxS, VXN_ERR_UNSUPPORTED, "at most %d samples per ray are supported (got %d)", kMaxS, S);
+  VXN_REQUIRE(per <= kMaxPer, VXN_ERR_UNSUPPORTED, "at most %d importance samples per step", kMaxPer);
+  VXN_REQUIRE(steps <= 20, VXN_ERR_UNSUPPORTED, "at most 20 upsampling steps");
+  if (A.n_rays == 0) return 0;
+  VXN_REQUIRE(A.rays_o && A.rays_d && A.near && A.far && A.t_lin && A.z_vals && A.dists && A.mid_z, VXN_ERR_NULL,
+              "null argument");
+  VXN_REQUIRE(!steps || A.perturb || A.u_det, VXN_ERR_NULL, "u_det is required in deterministic mode");
+  VXN_REQUIRE(!A.perturb || (A.t_rand && (!steps || A.u_rand)), VXN_ERR_NULL,
+              "t_rand and u_rand are required when perturb is set");
+  if (A.occ_mask) VXN_REQUIRE(A.occ_res >= 1, VXN_ERR_SHAPE, "occ_res must be >= 1");
+  SamplerDev a;
+  a.o = A.rays_o;
+  a.d = A.rays_d;
+  a.near = A.near;
+  a.far = A.far;
+  a.R = A.n_rays;
+  a.g = make_grid_desc(A.grid, A.X, A.Y, A.Z, A.box);
+  a.occ = A.occ_mask;
+  a.occ_res = A.occ_res;
+  for (int k = 0; k < 3; ++k) {
+    a.occ_mn[k] = A.occ_box[k];
+    a.occ_den[k] = (A.occ_box[3 + k] - A.occ_box[k]) + 1e-8f;
+  }
+  a.occ_bound = (float)((double)A.occ_res - 1.0001);
+  a.n0 = A.n_samples;
+  a.per = per;
+  a.steps = steps;
+  a.perturb = A.perturb;
+  a.t_lin = A.t_lin;
+  a.u_det = A.u_det;
+  a.t_rand = A.t_rand;
+  a.u_rand = A.u_rand;
+  a.z_out = A.z_vals;
+  a.dists_out = A.dists;
+  a.mid_out = A.mid_z;
+  a.inds = A.inds;
+  const int nb = A.n_rays < 8192 ? (int)A.n_rays : 8192;
+  hipLaunchKernelGGL(upsample_kernel, dim3(nb), dim3(kSampThreads), 0, (hipStream_t)stream, a);
+  VXN_LAUNCH_CHECK();
+  return 0;
+}

@@ -1,0 +1,312 @@
+// Device-side building blocks shared by the VoxNeuS gfx950 kernels.
+//
+// All translation units are compiled with -ffp-contract=off so that every float
+// expression rounds exactly like the PyTorch CPU operations of the reference path
+// (one rounding per elementwise op).  Fused multiply-adds are written explicitly
+// (fmaf) only where the reference itself has no rounding order to preserve
+// (accumulations whose order is not defined by the reference, MLP dot products).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define VXN_DEV __device__ __forceinline__
+
+namespace vxn {
+
+// ------------------------------------------------------------------------------------
+// Geometry of one dense SDF grid: [X, Y, Z] float32, z fastest.
+// ------------------------------------------------------------------------------------
+struct GridDesc {
+  const float* grid;  // X*Y*Z
+  int X, Y, Z;
+  float mn[3];     // xyz_min
+  float den[3];    // (xyz_max - xyz_min) + 1e-8f           (sdf_grid.py:106, voxneus.py:144)
+  float rm1[3];    // float(res) - 1.0f                      (coordinates.py:28)
+  float bound[3];  // float32(res - 1.0001)                  (coordinates.py:73)
+  float eps[3];    // (xyz_max - xyz_min) / (res - 1)        (sdf_grid.py:83-88)
+};
+
+VXN_DEV int64_t lin3(const GridDesc& g, int x, int y, int z) {
+  return ((int64_t)x * g.Y + y) * g.Z + z;
+}
+
+// ------------------------------------------------------------------------------------
+// exp and sigmoid exactly as PyTorch's CPU float kernels evaluate them: the vectorised
+// path calls SLEEF's expf_u10 (Shibata, "SLEEF: A Portable Vectorized Library of C
+// Standard Mathematical Functions", xexpf: Cody-Waite reduction by ln2 split in two
+// parts, degree-5 minimax polynomial, ldexp by two power-of-two multiplies), and
+// sigmoid(a) = 1 / (1 + exp(0 - a)).
+// ------------------------------------------------------------------------------------
+VXN_DEV float pow2i(int q) { return __int_as_float((q + 127) << 23); }
+
+VXN_DEV float sleef_expf(float d) {
+  const float kLog2e = 1.442695040888963407359924681001892137426645954152985934135449406931f;
+  const float kL2U = 0.693145751953125f;
+  const float kL2L = 1.428606765330187045e-06f;
+  const float qf = rintf(d * kLog2e);
+  const int q = (int)qf;
+  float s = fmaf(qf, -kL2U, d);
+  s = fmaf(qf, -kL2L, s);
+  float u = 0.000198527617612853646278381f;
+  u = fmaf(u, s, 0.00139304355252534151077271f);
+  u = fmaf(u, s, 0.00833336077630519866943359f);
+  u = fmaf(u, s, 0.0416664853692054748535156f);
+  u = fmaf(u, s, 0.166666671633720397949219f);
+  u = fmaf(u, s, 0.5f);
+  u = 1.0f + fmaf(s * s, u, s);
+  const int qh = q >> 1;
+  u = (u * pow2i(qh)) * pow2i(q - qh);
+  if (d < -104.0f) u = 0.0f;
+  if (d > 100.0f) u = __int_as_float(0x7f800000);
+  return u;
+}
+
+VXN_DEV float sigmoid_ref(float a) {
+  const float e = sleef_expf(0.0f - a);
+  return 1.0f / (1.0f + e);
+}
+
+// ------------------------------------------------------------------------------------
+// Trilinear SDF as F.grid_sample(mode="bilinear", padding_mode="border",
+// align_corners=True) on (z,y,x)-flipped [-1,1] coordinates (sdf_grid.py:90-120),
+// with PyTorch's CPU 3-D kernel operation order.  Corner k = 4*ox + 2*oy + oz.
+// ------------------------------------------------------------------------------------
+struct GsStencil {
+  int x0, y0, z0;   // floor corner
+  float w[8];       // weights in corner order
+  unsigned valid;   // bit k: corner k inside the grid
+};
+
+VXN_DEV float gs_unnorm(float p, float mn, float den, int size) {
+  float n = (p - mn) / den;
+  n = n * 2.0f - 1.0f;
+  float c = ((n + 1.0f) / 2.0f) * (float)(size - 1);
+  return fminf((float)(size - 1), fmaxf(c, 0.0f));
+}
+
+VXN_DEV GsStencil gs_stencil(const GridDesc& g, float px, float py, float pz) {
+  const float cx = gs_unnorm(px, g.mn[0], g.den[0], g.X);
+  const float cy = gs_unnorm(py, g.mn[1], g.den[1], g.Y);
+  const float cz = gs_unnorm(pz, g.mn[2], g.den[2], g.Z);
+  const float fx = floorf(cx), fy = floorf(cy), fz = floorf(cz);
+  const float gx = fx + 1.0f, gy = fy + 1.0f, gz = fz + 1.0f;
+  GsStencil s;
+  s.x0 = (int)fx;
+  s.y0 = (int)fy;
+  s.z0 = (int)fz;
+  // PyTorch multiplies the W (z), H (y), D (x) factors in that order.
+  s.w[0] = (gz - cz) * (gy - cy) * (gx - cx);
+  s.w[1] = (cz - fz) * (gy - cy) * (gx - cx);
+  s.w[2] = (gz - cz) * (cy - fy) * (gx - cx);
+  s.w[3] = (cz - fz) * (cy - fy) * (gx - cx);
+  s.w[4] = (gz - cz) * (gy - cy) * (cx - fx);
+  s.w[5] = (cz - fz) * (gy - cy) * (cx - fx);
+  s.w[6] = (gz - cz) * (cy - fy) * (cx - fx);
+  s.w[7] = (cz - fz) * (cy - fy) * (cx - fx);
+  unsigned v = 0;
+  for (int k = 0; k < 8; ++k) {
+    const int x = s.x0 + ((k >> 2) & 1), y = s.y0 + ((k >> 1) & 1), z = s.z0 + (k & 1);
+    if (x < g.X && y < g.Y && z < g.Z) v |= 1u << k;
+  }
+  s.valid = v;
+  return s;
+}
+
+VXN_DEV float gs_eval(const GridDesc& g, const GsStencil& s) {
+  float out = 0.0f;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    if (s.valid & (1u << k)) {
+      const int x = s.x0 + ((k >> 2) & 1), y = s.y0 + ((k >> 1) & 1), z = s.z0 + (k & 1);
+      out = out + g.grid[lin3(g, x, y, z)] * s.w[k];
+    }
+  }
+  return out;
+}
+
+// ------------------------------------------------------------------------------------
+// Corner decomposition (coordinates.py:10-29, 55-124) and the Eq. 7 interpolated
+// central-difference gradient (gradients.py:59-139).
+// ------------------------------------------------------------------------------------
+struct Corners {
+  int c0[3], c1[3];
+  float w[8];
+};
+
+VXN_DEV Corners corner_decomp(const GridDesc& g, float px, float py, float pz) {
+  const float p[3] = {px, py, pz};
+  const int res[3] = {g.X, g.Y, g.Z};
+  float d[3];
+  Corners c;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float coord = ((p[a] - g.mn[a]) / g.den[a]) * g.rm1[a];
+    const float cc = fminf(fmaxf(coord, 0.0f), g.bound[a]);
+    int i0 = (int)floorf(cc);
+    int i1 = i0 + 1;
+    i0 = min(max(i0, 0), res[a] - 1);
+    i1 = min(max(i1, 0), res[a] - 1);
+    c.c0[a] = i0;
+    c.c1[a] = i1;
+    d[a] = cc - (float)i0;
+  }
+  const float ox = 1.0f - d[0], oy = 1.0f - d[1], oz = 1.0f - d[2];
+  c.w[0] = ox * oy * oz;
+  c.w[1] = ox * oy * d[2];
+  c.w[2] = ox * d[1] * oz;
+  c.w[3] = ox * d[1] * d[2];
+  c.w[4] = d[0] * oy * oz;
+  c.w[5] = d[0] * oy * d[2];
+  c.w[6] = d[0] * d[1] * oz;
+  c.w[7] = d[0] * d[1] * d[2];
+  return c;
+}
+
+VXN_DEV void corner_xyz(const Corners& c, int k, int& x, int& y, int& z) {
+  x = ((k >> 2) & 1) ? c.c1[0] : c.c0[0];
+  y = ((k >> 1) & 1) ? c.c1[1] : c.c0[1];
+  z = (k & 1) ? c.c1[2] : c.c0[2];
+}
+
+// Per-axis finite-difference stencil of one vertex: (i+1, i-1) clamped, divisor
+// max((i_p - i_m) * eps, eps).
+VXN_DEV void fd_axis(int i, int res, float eps, int& ip, int& im, float& dv) {
+  ip = min(i + 1, res - 1);
+  im = max(i - 1, 0);
+  dv = fmaxf((float)(ip - im) * eps, eps);
+}
+
+// Eq. 7: grad = sum_k w_k n_k, summed in PyTorch's CPU order for an 8-long reduction
+// over a strided dimension: ((t0 + t4) + (t1 + t5)) + (t2 + t6)) + (t3 + t7).
+VXN_DEV void eq7_gradient(const GridDesc& g, const Corners& c, float out[3]) {
+  float t[8][3];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    int x, y, z;
+    corner_xyz(c, k, x, y, z);
+    int ip, im;
+    float dv;
+    fd_axis(x, g.X, g.eps[0], ip, im, dv);
+    const float nx = (g.grid[lin3(g, ip, y, z)] - g.grid[lin3(g, im, y, z)]) / dv;
+    fd_axis(y, g.Y, g.eps[1], ip, im, dv);
+    const float ny = (g.grid[lin3(g, x, ip, z)] - g.grid[lin3(g, x, im, z)]) / dv;
+    fd_axis(z, g.Z, g.eps[2], ip, im, dv);
+    const float nz = (g.grid[lin3(g, x, y, ip)] - g.grid[lin3(g, x, y, im)]) / dv;
+    t[k][0] = c.w[k] * nx;
+    t[k][1] = c.w[k] * ny;
+    t[k][2] = c.w[k] * nz;
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+    out[a] = (((t[0][a] + t[4][a]) + (t[1][a] + t[5][a])) + (t[2][a] + t[6][a])) + (t[3][a] + t[7][a]);
+}
+
+// Scatter dL/dgrad (the un-normalised Eq. 7 gradient) into the grid gradient:
+// d n_k,a / d f[i_p] = +1/dv, d f[i_m] = -1/dv, weighted by w_k.
+VXN_DEV void eq7_scatter(const GridDesc& g, const Corners& c, const float dg[3], float* grad_grid) {
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    int x, y, z;
+    corner_xyz(c, k, x, y, z);
+    int ip, im;
+    float dv;
+    fd_axis(x, g.X, g.eps[0], ip, im, dv);
+    float v = (c.w[k] * dg[0]) / dv;
+    atomicAdd(grad_grid + lin3(g, ip, y, z), v);
+    atomicAdd(grad_grid + lin3(g, im, y, z), -v);
+    fd_axis(y, g.Y, g.eps[1], ip, im, dv);
+    v = (c.w[k] * dg[1]) / dv;
+    atomicAdd(grad_grid + lin3(g, x, ip, z), v);
+    atomicAdd(grad_grid + lin3(g, x, im, z), -v);
+    fd_axis(z, g.Z, g.eps[2], ip, im, dv);
+    v = (c.w[k] * dg[2]) / dv;
+    atomicAdd(grad_grid + lin3(g, x, y, ip), v);
+    atomicAdd(grad_grid + lin3(g, x, y, im), -v);
+  }
+}
+
+VXN_DEV void gs_scatter(const GridDesc& g, const GsStencil& s, float dsdf, float* grad_grid) {
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    if (s.valid & (1u << k)) {
+      const int x = s.x0 + ((k >> 2) & 1), y = s.y0 + ((k >> 1) & 1), z = s.z0 + (k & 1);
+      atomicAdd(grad_grid + lin3(g, x, y, z), s.w[k] * dsdf);
+    }
+  }
+}
+
+// normal = grad / (|grad| + 1e-8)  (gradients.py:136-137)
+VXN_DEV float norm3(const float v[3]) { return sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); }
+
+// ------------------------------------------------------------------------------------
+// Multiresolution hash (radiance_hash.py:53-69): (x*1) ^ (y*2654435761) ^ (z*805459861)
+// mod T.  Coordinates are non-negative and T is a power of two, so the int64 remainder
+// equals the low bits of the 32-bit wrapped products.
+// ------------------------------------------------------------------------------------
+VXN_DEV uint32_t hash3(uint32_t x, uint32_t y, uint32_t z, uint32_t mask) {
+  return (x ^ (y * 2654435761u) ^ (z * 805459861u)) & mask;
+}
+
+struct HashCell {
+  uint32_t idx[8];
+  float w[8];
+};
+
+VXN_DEV HashCell hash_cell(float x0, float x1, float x2, float res, uint32_t mask) {
+  const float s0 = x0 * res, s1 = x1 * res, s2 = x2 * res;
+  const float f0 = floorf(s0), f1 = floorf(s1), f2 = floorf(s2);
+  const uint32_t c0 = (uint32_t)(int)f0, c1 = (uint32_t)(int)f1, c2 = (uint32_t)(int)f2;
+  const float dx = s0 - f0, dy = s1 - f1, dz = s2 - f2;
+  const float ox = 1.0f - dx, oy = 1.0f - dy, oz = 1.0f - dz;
+  HashCell h;
+  h.w[0] = ox * oy * oz;
+  h.w[1] = ox * oy * dz;
+  h.w[2] = ox * dy * oz;
+  h.w[3] = ox * dy * dz;
+  h.w[4] = dx * oy * oz;
+  h.w[5] = dx * oy * dz;
+  h.w[6] = dx * dy * oz;
+  h.w[7] = dx * dy * dz;
+#pragma unroll
+  for (int k = 0; k < 8; ++k)
+    h.idx[k] = hash3(c0 + ((k >> 2) & 1), c1 + ((k >> 1) & 1), c2 + (k & 1), mask);
+  return h;
+}
+
+// Real spherical harmonics, degree 0-3 (embedder.py:55-87), same operation order.
+VXN_DEV void sh16(float x, float y, float z, float o[16]) {
+  o[0] = 0.28209479177387814f;
+  o[1] = -0.4886025119029199f * y;
+  o[2] = 0.4886025119029199f * z;
+  o[3] = -0.4886025119029199f * x;
+  o[4] = 1.0925484305920792f * x * y;
+  o[5] = -1.0925484305920792f * y * z;
+  o[6] = 0.31539156525252005f * (2.0f * z * z - x * x - y * y);
+  o[7] = -1.0925484305920792f * x * z;
+  o[8] = 0.5462742152960396f * (x * x - y * y);
+  o[9] = -0.5900435899266435f * y * (3.0f * x * x - y * y);
+  o[10] = 2.890611442640554f * x * y * z;
+  o[11] = -0.4570457994644658f * y * (4.0f * z * z - x * x - y * y);
+  o[12] = 0.3731763325901154f * z * (2.0f * z * z - 3.0f * x * x - 3.0f * y * y);
+  o[13] = -0.4570457994644658f * x * (4.0f * z * z - x * x - y * y);
+  o[14] = 1.445305721320277f * z * (x * x - y * y);
+  o[15] = -0.5900435899266435f * x * (x * x - 3.0f * y * y);
+}
+
+// ------------------------------------------------------------------------------------
+// Wave64 helpers
+// ------------------------------------------------------------------------------------
+VXN_DEV float wave_sum(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+VXN_DEV double wave_sum_d(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+}  // namespace vxn

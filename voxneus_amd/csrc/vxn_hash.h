@@ -1,0 +1,82 @@
+// Per-level hash-grid lookups shared by the standalone and fused kernels
+// (radiance_hash.py:71-166: hashes every level, 8 corners with no top clamp,
+// feature = w0*e0 + w1*e1 + ... + w7*e7 summed left to right, times the lambda weight).
+#pragma once
+
+#include "vxn_device.h"
+
+namespace vxn {
+
+struct HashDesc {
+  const void* table;  // [L, T, 2] float32 or bfloat16
+  int dtype;          // 0 = float32, 1 = bfloat16
+  int L;
+  int64_t T;
+  uint32_t mask;
+  float res[32];
+  float lw[32];
+};
+
+VXN_DEV float bf16_to_f32(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+
+VXN_DEV void load_entry(const HashDesc& h, int l, uint32_t idx, float& e0, float& e1) {
+  const int64_t off = (int64_t)l * h.T + idx;
+  if (h.dtype == 0) {
+    const float2 v = reinterpret_cast<const float2*>(h.table)[off];
+    e0 = v.x;
+    e1 = v.y;
+  } else {
+    const uint32_t v = reinterpret_cast<const uint32_t*>(h.table)[off];
+    e0 = __uint_as_float(v << 16);
+    e1 = __uint_as_float(v & 0xffff0000u);
+  }
+}
+
+VXN_DEV void hash_level_fwd(const HashDesc& h, int l, float x, float y, float z, float& f0, float& f1) {
+  const float lw = h.lw[l];
+  if (lw == 0.0f) {
+    f0 = 0.0f;
+    f1 = 0.0f;
+    return;
+  }
+  const HashCell c = hash_cell(x, y, z, h.res[l], h.mask);
+  float e[8][2];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) load_entry(h, l, c.idx[k], e[k][0], e[k][1]);
+  float a0 = c.w[0] * e[0][0], a1 = c.w[0] * e[0][1];
+#pragma unroll
+  for (int k = 1; k < 8; ++k) {
+    a0 = a0 + c.w[k] * e[k][0];
+    a1 = a1 + c.w[k] * e[k][1];
+  }
+  if (lw < 1.0f) {
+    a0 = a0 * lw;
+    a1 = a1 * lw;
+  }
+  f0 = a0;
+  f1 = a1;
+}
+
+VXN_DEV void hash_level_bwd(const HashDesc& h, int l, float x, float y, float z, float d0, float d1,
+                            float* __restrict__ grad_table) {
+  const float lw = h.lw[l];
+  if (lw == 0.0f || (d0 == 0.0f && d1 == 0.0f)) return;
+  if (lw < 1.0f) {
+    d0 = d0 * lw;
+    d1 = d1 * lw;
+  }
+  const HashCell c = hash_cell(x, y, z, h.res[l], h.mask);
+  float* base = grad_table + (int64_t)l * h.T * 2;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    float* p = base + (int64_t)c.idx[k] * 2;
+    atomicAdd(p, d0 * c.w[k]);
+    atomicAdd(p + 1, d1 * c.w[k]);
+  }
+}
+
+// Validates the hash configuration and fills a HashDesc (defined in hash.hip).
+int make_hash_desc(HashDesc& h, const void* table, int dtype, int L, int log2T, int F, const int32_t* level_res,
+                   const float* level_w);
+
+}  // namespace vxn

@@ -1,0 +1,414 @@
+"""torch.autograd.Functions and thin wrappers over the HIP C ABI.
+
+Every function here runs hand-written gfx950 kernels through ``_native``; torch only
+allocates device memory and supplies the stream.  Inputs must live on a HIP device.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import torch
+
+from . import _native as N
+
+
+def _stream(t: torch.Tensor) -> int:
+    return N.stream_ptr(t.device)
+
+
+def _grid3(grid: torch.Tensor) -> torch.Tensor:
+    g = grid.reshape(grid.shape[-3:]) if grid.ndim == 5 else grid
+    if g.ndim != 3:
+        raise ValueError(f"grid must be [X,Y,Z] or [1,1,X,Y,Z] (got {tuple(grid.shape)})")
+    return g
+
+
+# ----------------------------------------------------------------------------------------
+# Dense SDF grid
+# ----------------------------------------------------------------------------------------
+
+def grid_query_raw(grid3: torch.Tensor, box, xyz: torch.Tensor, want_sdf=True, want_grad=True,
+                   want_normal=False, want_corners=False, want_weights=False):
+    """Run the fused SDF + Eq. 7 query on flattened points; returns a dict of tensors."""
+    grid3 = grid3.contiguous()
+    xyz = xyz.reshape(-1, 3).contiguous()
+    n = xyz.shape[0]
+    dev = xyz.device
+    out = {}
+    if want_sdf:
+        out["sdf"] = torch.empty(n, device=dev, dtype=torch.float32)
+    if want_grad:
+        out["grad"] = torch.empty(n, 3, device=dev, dtype=torch.float32)
+    if want_normal:
+        out["normal"] = torch.empty(n, 3, device=dev, dtype=torch.float32)
+    if want_corners:
+        out["corners"] = torch.empty(n, 8, 3, device=dev, dtype=torch.int64)
+    if want_weights:
+        out["weights"] = torch.empty(n, 8, device=dev, dtype=torch.float32)
+    X, Y, Z = grid3.shape
+    N.check(N.lib().vxn_grid_query_fwd(
+        N.dptr(grid3, torch.float32, "grid"), X, Y, Z, N.box6(*box), N.dptr(xyz, torch.float32, "xyz"), n,
+        N.dptr(out.get("sdf")), N.dptr(out.get("grad")), N.dptr(out.get("normal")), N.dptr(out.get("corners")),
+        N.dptr(out.get("weights")), _stream(xyz)), "grid_query_fwd")
+    return out
+
+
+class GridQueryFunction(torch.autograd.Function):
+    """(grid, xyz) -> (sdf [n], grad [n,3]); backward scatters into the grid."""
+
+    @staticmethod
+    def forward(ctx, grid, xyz, box):
+        g3 = _grid3(grid)
+        o = grid_query_raw(g3, box, xyz, want_sdf=True, want_grad=True)
+        ctx.save_for_backward(g3, xyz)
+        ctx.box = box
+        ctx.grid_shape = grid.shape
+        return o["sdf"], o["grad"]
+
+    @staticmethod
+    def backward(ctx, d_sdf, d_grad):
+        g3, xyz = ctx.saved_tensors
+        gg = None
+        if ctx.needs_input_grad[0]:
+            gg = torch.zeros_like(g3)
+            xf = xyz.reshape(-1, 3).contiguous()
+            X, Y, Z = g3.shape
+            ds = d_sdf.contiguous() if d_sdf is not None else None
+            dg = d_grad.contiguous() if d_grad is not None else None
+            N.check(N.lib().vxn_grid_query_bwd(
+                N.dptr(g3), X, Y, Z, N.box6(*ctx.box), N.dptr(xf), xf.shape[0], N.dptr(ds), N.dptr(dg),
+                N.dptr(gg), _stream(gg)), "grid_query_bwd")
+            gg = gg.reshape(ctx.grid_shape)
+        return gg, None, None
+
+
+class GradientInterpFunction(torch.autograd.Function):
+    """Reference-shaped Eq. 7 on continuous grid coordinates (cuda/__init__.py:29-59)."""
+
+    @staticmethod
+    def forward(ctx, grid, coords, voxel_size):
+        g3 = _grid3(grid).contiguous()
+        c = coords.reshape(-1, 3).contiguous()
+        eps = [float(v) for v in voxel_size.detach().cpu().tolist()]
+        out = torch.empty(c.shape[0], 3, device=c.device, dtype=torch.float32)
+        X, Y, Z = g3.shape
+        N.check(N.lib().vxn_gradient_interp_fwd(N.dptr(g3, torch.float32, "grid"), X, Y, Z, N.dptr(c), c.shape[0],
+                                                N.f3(eps), N.dptr(out), _stream(c)), "gradient_interp_fwd")
+        ctx.save_for_backward(c)
+        ctx.eps, ctx.shape3, ctx.grid_shape = eps, g3.shape, grid.shape
+        return out
+
+    @staticmethod
+    def backward(ctx, gout):
+        (c,) = ctx.saved_tensors
+        gg = None
+        if ctx.needs_input_grad[0]:
+            X, Y, Z = ctx.shape3
+            gg = torch.zeros(ctx.shape3, device=c.device, dtype=torch.float32)
+            go = gout.contiguous()
+            N.check(N.lib().vxn_gradient_interp_bwd(N.dptr(go), N.dptr(c), c.shape[0], N.dptr(gg), X, Y, Z,
+                                                    N.f3(ctx.eps), _stream(gg)), "gradient_interp_bwd")
+            gg = gg.reshape(ctx.grid_shape)
+        return gg, None, None
+
+
+def lattice_query(grid3, box, NX, NY, NZ, lattice_box):
+    g3 = grid3.contiguous()
+    sdf = torch.empty(NX, NY, NZ, device=g3.device, dtype=torch.float32)
+    nrm = torch.empty(NX, NY, NZ, 3, device=g3.device, dtype=torch.float32)
+    X, Y, Z = g3.shape
+    N.check(N.lib().vxn_lattice_query(N.dptr(g3, torch.float32, "grid"), X, Y, Z, N.box6(*box), NX, NY, NZ,
+                                      N.box6(*lattice_box), N.dptr(sdf), N.dptr(nrm), _stream(g3)), "lattice_query")
+    return sdf, nrm
+
+
+# ----------------------------------------------------------------------------------------
+# Hash encoding
+# ----------------------------------------------------------------------------------------
+
+def _levels(level_res: Sequence[int], level_w: Optional[Sequence[float]]):
+    L = len(level_res)
+    res = (ctypes.c_int32 * L)(*[int(r) for r in level_res])
+    w = None if level_w is None else (ctypes.c_float * L)(*[float(v) for v in level_w])
+    return res, w
+
+
+def _table_dtype(table: torch.Tensor) -> int:
+    if table.dtype == torch.float32:
+        return N.VXN_DTYPE_F32
+    if table.dtype == torch.bfloat16:
+        return N.VXN_DTYPE_BF16
+    raise TypeError(f"hash table must be float32 or bfloat16 (got {table.dtype})")
+
+
+class HashEncodeFunction(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, table, x01, level_res, level_w):
+        L, T, F = table.shape
+        log2T = int(round(math.log2(T)))
+        if 1 << log2T != T:
+            raise ValueError("hash table size must be a power of two")
+        tc = table.contiguous()
+        xf = x01.reshape(-1, 3).contiguous()
+        feats = torch.empty(xf.shape[0], L * F, device=xf.device, dtype=torch.float32)
+        res, w = _levels(level_res, level_w)
+        N.check(N.lib().vxn_hash_fwd(N.dptr(tc), _table_dtype(tc), L, log2T, F, res, w, N.dptr(xf, torch.float32, "x"),
+                                     xf.shape[0], N.dptr(feats), _stream(xf)), "hash_fwd")
+        ctx.save_for_backward(xf)
+        ctx.meta = (L, T, F, log2T, list(level_res), None if level_w is None else list(level_w), table.dtype)
+        return feats.reshape(*x01.shape[:-1], L * F)
+
+    @staticmethod
+    def backward(ctx, d_feats):
+        (xf,) = ctx.saved_tensors
+        L, T, F, log2T, lres, lw, tdt = ctx.meta
+        g = None
+        if ctx.needs_input_grad[0]:
+            g = torch.zeros(L, T, F, device=xf.device, dtype=torch.float32)
+            df = d_feats.reshape(-1, L * F).contiguous()
+            res, w = _levels(lres, lw)
+            N.check(N.lib().vxn_hash_bwd(L, log2T, F, res, w, N.dptr(xf), xf.shape[0], N.dptr(df), N.dptr(g),
+                                         _stream(g)), "hash_bwd")
+            g = g.to(tdt)
+        return g, None, None, None
+
+
+# ----------------------------------------------------------------------------------------
+# Progressive upsampler
+# ----------------------------------------------------------------------------------------
+
+_table_cache: Dict[Tuple, torch.Tensor] = {}
+
+
+def _linspace_dev(a: float, b: float, n: int, device) -> torch.Tensor:
+    key = (a, b, n, str(device))
+    t = _table_cache.get(key)
+    if t is None:
+        # built by torch on the CPU exactly as the reference builds it, then copied over
+        t = torch.linspace(a, b, steps=n).to(device)
+        _table_cache[key] = t
+    return t
+
+
+def upsample(rays_o, rays_d, near, far, grid3, box, n_samples=64, n_importance=64, up_sample_steps=4,
+             perturb=False, occ_mask=None, occ_box=None, t_rand=None, u_rand=None, return_inds=False):
+    """Multi-step SDF upsampling on the GPU; returns (z_vals, dists, mid_z[, inds])."""
+    dev = rays_o.device
+    R = rays_o.shape[0]
+    steps = up_sample_steps if (n_importance > 0 and up_sample_steps > 0) else 0
+    per = max(n_importance // steps, 1) if steps else 0
+    S = n_samples + steps * per
+    z = torch.empty(R, S, device=dev, dtype=torch.float32)
+    dists = torch.empty_like(z)
+    mid = torch.empty_like(z)
+    inds = torch.empty(steps, R, per, device=dev, dtype=torch.int32) if (return_inds and steps) else None
+    ro, rd = rays_o.contiguous(), rays_d.contiguous()
+    nr, fr = near.reshape(-1).contiguous(), far.reshape(-1).contiguous()
+    if perturb:
+        if t_rand is None:
+            t_rand = torch.rand(R, n_samples, device=dev)
+        if u_rand is None and steps:
+            u_rand = torch.rand(steps, R, per, device=dev)
+        t_rand = t_rand.contiguous()
+        u_rand = u_rand.contiguous() if u_rand is not None else None
+    a = N.SamplerArgs()
+    a.rays_o, a.rays_d, a.near, a.far = N.dptr(ro, torch.float32), N.dptr(rd, torch.float32), N.dptr(nr), N.dptr(fr)
+    a.n_rays = R
+    g3 = grid3.contiguous()
+    a.grid = N.dptr(g3, torch.float32, "grid")
+    a.X, a.Y, a.Z = g3.shape
+    a.box = N.box6(*box)
+    if occ_mask is not None:
+        om = occ_mask.contiguous()
+        if om.dtype != torch.bool:
+            raise TypeError("occupancy mask must be bool")
+        a.occ_mask = om.data_ptr()
+        a.occ_res = om.shape[0]
+        a.occ_box = N.box6(*(occ_box if occ_box is not None else box))
+    a.n_samples, a.n_importance, a.up_sample_steps, a.perturb = n_samples, n_importance, up_sample_steps, int(perturb)
+    a.t_lin = N.dptr(_linspace_dev(0.0, 1.0, n_samples, dev))
+    if steps:
+        a.u_det = N.dptr(_linspace_dev(0.5 / per, 1.0 - 0.5 / per, per, dev))
+    a.t_rand = N.dptr(t_rand) if perturb else None
+    a.u_rand = N.dptr(u_rand) if (perturb and steps) else None
+    a.z_vals, a.dists, a.mid_z = N.dptr(z), N.dptr(dists), N.dptr(mid)
+    a.inds = N.dptr(inds)
+    N.check(N.lib().vxn_upsample(ctypes.byref(a), _stream(ro)), "upsample")
+    if return_inds:
+        return z, dists, mid, inds
+    return z, dists, mid
+
+
+# ----------------------------------------------------------------------------------------
+# Fused render
+# ----------------------------------------------------------------------------------------
+
+class RenderConfig:
+    """Static settings of one fused render call."""
+
+    def __init__(self, box, level_res, level_w=None, sphere_radius=1.0, cos_anneal_ratio=0.0,
+                 use_iter_cos=False, mlp_mode=N.VXN_MLP_FP32):
+        self.box = tuple(float(v) for v in box)
+        self.level_res = list(level_res)
+        self.level_w = None if level_w is None else list(level_w)
+        self.sphere_radius = float(sphere_radius) if sphere_radius is not None else 0.0
+        self.cos_anneal_ratio = float(cos_anneal_ratio)
+        self.use_iter_cos = bool(use_iter_cos)
+        self.mlp_mode = mlp_mode
+
+
+def fill_render_args(a: "N.RenderArgs", cfg: RenderConfig, grid3, table, mlp_flat, s: float, rays_o, rays_d,
+                     mid, dists):
+    R, S = mid.shape
+    a.rays_o, a.rays_d = N.dptr(rays_o, torch.float32, "rays_o"), N.dptr(rays_d, torch.float32, "rays_d")
+    a.mid_z, a.dists = N.dptr(mid, torch.float32, "mid_z"), N.dptr(dists, torch.float32, "dists")
+    a.n_rays, a.n_samples = R, S
+    a.grid = N.dptr(grid3, torch.float32, "grid")
+    a.X, a.Y, a.Z = grid3.shape
+    a.box = N.box6(cfg.box[:3], cfg.box[3:])
+    L, T, F = table.shape
+    a.table, a.table_dtype = N.dptr(table), _table_dtype(table)
+    a.n_levels, a.log2_T, a.n_features = L, int(round(math.log2(T))), F
+    for i, r in enumerate(cfg.level_res):
+        a.level_res[i] = int(r)
+        a.level_w[i] = 1.0 if cfg.level_w is None else float(cfg.level_w[i])
+    a.mlp = N.dptr(mlp_flat, torch.float32, "mlp")
+    a.mlp_mode = cfg.mlp_mode
+    a.s = float(s)
+    a.sphere_radius = cfg.sphere_radius
+    a.cos_anneal_ratio = cfg.cos_anneal_ratio
+    a.use_iter_cos = int(cfg.use_iter_cos)
+
+
+def render_workspace(a: "N.RenderArgs", device) -> torch.Tensor:
+    nb = ctypes.c_int64(0)
+    N.check(N.lib().vxn_render_workspace_bytes(ctypes.byref(a), ctypes.byref(nb)), "render_workspace_bytes")
+    return torch.empty(max(int(nb.value), 8), device=device, dtype=torch.uint8)
+
+
+class RenderFunction(torch.autograd.Function):
+    """Fused render of passes 2-3.  Differentiable outputs: rgb, depth, normal, opacity,
+    weights; the per-sample diagnostics (alpha, transmittance, sdf/grad/normal samples)
+    are returned without gradient."""
+
+    @staticmethod
+    def forward(ctx, grid, table, mlp_flat, s, rays_o, rays_d, mid, dists, cfg: RenderConfig, want_corners,
+                vr_bitmask):
+        g3 = _grid3(grid).contiguous()
+        tb = table.contiguous()
+        mf = mlp_flat.contiguous()
+        ro, rd, md, ds = rays_o.contiguous(), rays_d.contiguous(), mid.contiguous(), dists.contiguous()
+        R, S = md.shape
+        dev = md.device
+        f = lambda *sh: torch.empty(*sh, device=dev, dtype=torch.float32)
+        out = {"rgb": f(R, 3), "depth": f(R, 1), "normal": f(R, 3), "opacity": f(R, 1), "weights": f(R, S),
+               "alpha": f(R, S), "transmittance": f(R, S), "sdf_samples": f(R, S, 1), "grad_samples": f(R, S, 3),
+               "normal_samples": f(R, S, 3)}
+        corners = torch.empty(R, S, 8, 3, device=dev, dtype=torch.int64) if want_corners else None
+        a = N.RenderArgs()
+        s_val = float(s.detach()) if torch.is_tensor(s) else float(s)
+        fill_render_args(a, cfg, g3, tb, mf, s_val, ro, rd, md, ds)
+        for k in ("rgb", "depth", "normal", "opacity", "weights", "alpha", "transmittance"):
+            setattr(a, k, N.dptr(out[k]))
+        a.sdf_samples, a.grad_samples, a.normal_samples = (N.dptr(out["sdf_samples"]), N.dptr(out["grad_samples"]),
+                                                           N.dptr(out["normal_samples"]))
+        a.corner_indices = N.dptr(corners)
+        a.vr_bitmask = N.dptr(vr_bitmask)
+        N.check(N.lib().vxn_render_forward(ctypes.byref(a), _stream(md)), "render_forward")
+        ctx.save_for_backward(g3, tb, mf, ro, rd, md, ds)
+        ctx.cfg, ctx.s_val, ctx.grid_shape = cfg, s_val, grid.shape
+        ctx.mark_non_differentiable(out["alpha"], out["transmittance"], out["sdf_samples"], out["grad_samples"],
+                                    out["normal_samples"])
+        if corners is not None:
+            ctx.mark_non_differentiable(corners)
+        return (out["rgb"], out["depth"], out["normal"], out["opacity"], out["weights"], out["alpha"],
+                out["transmittance"], out["sdf_samples"], out["grad_samples"], out["normal_samples"], corners)
+
+    @staticmethod
+    def backward(ctx, d_rgb, d_depth, d_normal, d_opacity, d_weights, *unused):
+        g3, tb, mf, ro, rd, md, ds = ctx.saved_tensors
+        dev = md.device
+        a = N.RenderArgs()
+        fill_render_args(a, ctx.cfg, g3, tb, mf, ctx.s_val, ro, rd, md, ds)
+        c = lambda t: None if t is None else t.contiguous()
+        d_rgb, d_depth, d_normal, d_opacity, d_weights = map(c, (d_rgb, d_depth, d_normal, d_opacity, d_weights))
+        a.d_rgb, a.d_depth, a.d_normal, a.d_opacity, a.d_weights = (N.dptr(d_rgb), N.dptr(d_depth), N.dptr(d_normal),
+                                                                     N.dptr(d_opacity), N.dptr(d_weights))
+        gg = torch.zeros_like(g3)
+        gt = torch.zeros(tb.shape, device=dev, dtype=torch.float32)
+        gm = torch.empty(N.MLP_PARAMS, device=dev, dtype=torch.float32)
+        gs = torch.empty(1, device=dev, dtype=torch.float32)
+        a.grad_grid, a.grad_table, a.grad_mlp, a.grad_s = N.dptr(gg), N.dptr(gt), N.dptr(gm), N.dptr(gs)
+        ws = render_workspace(a, dev)
+        a.workspace, a.workspace_bytes = ws.data_ptr(), ws.numel()
+        N.check(N.lib().vxn_render_backward(ctypes.byref(a), _stream(md)), "render_backward")
+        return (gg.reshape(ctx.grid_shape), gt.to(tb.dtype), gm, gs.reshape(()), None, None, None, None, None, None,
+                None)
+
+
+# ----------------------------------------------------------------------------------------
+# V_R and regularisers
+# ----------------------------------------------------------------------------------------
+
+def vr_bitmask_words(grid_shape) -> int:
+    X, Y, Z = grid_shape
+    return (X * Y * Z + 31) // 32
+
+
+def vr_mark_corners(corners: torch.Tensor, grid_shape, bitmask: torch.Tensor):
+    X, Y, Z = grid_shape
+    c = corners.reshape(-1, 3).to(torch.int64).contiguous()
+    N.check(N.lib().vxn_vr_mark_corners(N.dptr(c), c.shape[0], X, Y, Z, N.dptr(bitmask, torch.int32, "bitmask"),
+                                        _stream(c)), "vr_mark_corners")
+
+
+def vr_compact(bitmask: torch.Tensor, grid_shape, box, relax_radius: float = 1.2):
+    """-> (list int32 [X*Y*Z] with bit 31 = outside the relaxed sphere, counts int32 [2])."""
+    X, Y, Z = grid_shape
+    dev = bitmask.device
+    lst = torch.empty(X * Y * Z, device=dev, dtype=torch.int32)
+    counts = torch.zeros(2, device=dev, dtype=torch.int32)
+    nb = ctypes.c_int64(0)
+    N.check(N.lib().vxn_vr_workspace_bytes(X, Y, Z, ctypes.byref(nb)), "vr_workspace_bytes")
+    ws = torch.empty(max(int(nb.value), 8), device=dev, dtype=torch.uint8)
+    N.check(N.lib().vxn_vr_compact(N.dptr(bitmask, torch.int32, "bitmask"), X, Y, Z, N.box6(*box), float(relax_radius),
+                                   N.dptr(lst), N.dptr(counts), ws.data_ptr(), ws.numel(), _stream(bitmask)),
+            "vr_compact")
+    return lst, counts
+
+
+def reg_eik_curv(grid3, box, vr_list, counts, w_eik, w_curv, grad_grid):
+    """Fused eikonal + curvature over V_R; returns FP64 losses [2] (device)."""
+    X, Y, Z = grid3.shape
+    dev = grid3.device
+    losses = torch.empty(2, device=dev, dtype=torch.float64)
+    nb = ctypes.c_int64(0)
+    N.check(N.lib().vxn_reg_workspace_bytes(X, Y, Z, ctypes.byref(nb)), "reg_workspace_bytes")
+    ws = torch.empty(int(nb.value), device=dev, dtype=torch.uint8)
+    N.check(N.lib().vxn_reg_eik_curv(N.dptr(grid3, torch.float32, "grid"), X, Y, Z, N.box6(*box), N.dptr(vr_list),
+                                     N.dptr(counts), float(w_eik), float(w_curv),
+                                     N.dptr(grad_grid, torch.float32, "grad_grid"), N.dptr(losses), ws.data_ptr(),
+                                     ws.numel(), _stream(grid3)), "reg_eik_curv")
+    return losses
+
+
+def eikonal_grad_ref(grid3, active, grad3, eps3, weight):
+    X, Y, Z = grid3.shape
+    a = active.to(torch.int64).contiguous()
+    N.check(N.lib().vxn_eikonal_grad(N.dptr(grid3.contiguous()), X, Y, Z, N.dptr(a), a.shape[0], N.dptr(grad3),
+                                     N.f3(eps3), float(weight), _stream(grid3)), "eikonal_grad")
+
+
+def curvature_grad_ref(grid3, active, grad3, eps3, weight):
+    X, Y, Z = grid3.shape
+    a = active.to(torch.int64).contiguous()
+    N.check(N.lib().vxn_curvature_grad(N.dptr(grid3.contiguous()), X, Y, Z, N.dptr(a), a.shape[0], N.dptr(grad3),
+                                       N.f3(eps3), float(weight), _stream(grid3)), "curvature_grad")
+
+
+def s_amplify(grad_v: torch.Tensor, k: float, amplified: Optional[torch.Tensor] = None):
+    N.check(N.lib().vxn_s_amplify(N.dptr(grad_v, torch.float32, "grad_v"), float(k), N.dptr(amplified),
+                                  _stream(grad_v)), "s_amplify")

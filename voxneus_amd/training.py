@@ -1,0 +1,176 @@
+"""The fused VoxNeuS training step (fwd + bwd + regularisers) on one GPU.
+
+``FusedTrainStep.step`` follows ``VoxNeuSTrainer.train_step`` (training/trainer.py:188-326)
+in order - ray bounds, multi-step sampling, render, Huber colour loss, backward, V_R
+with the relaxed-sphere filter, closed-form eikonal + curvature gradients, and the k=5
+negative s-gradient amplification - without any host synchronisation: losses, counts
+and the amplification flag stay on the device until the caller reads them.
+
+``amplify_s_gradient`` mirrors training/scheduler.py:14-33 (it returns a bool, which
+needs one device read; ``FusedTrainStep`` uses the device-side form instead).
+"""
+
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+from typing import Dict, Optional
+
+import torch
+
+from . import _native as N
+from . import functional as Fn
+
+
+def near_far_from_sphere(rays_o, rays_d, radius=1.0, default_near=0.05, default_far=4.0):
+    """rendering/rays.py:102-128 (elementwise, on the rays' device)."""
+    a = torch.sum(rays_d ** 2, dim=-1, keepdim=True)
+    b = 2.0 * torch.sum(rays_o * rays_d, dim=-1, keepdim=True)
+    mid = 0.5 * (-b) / a
+    near = torch.clamp(mid - radius, min=default_near)
+    far = torch.clamp(mid + radius, min=near + 0.1)
+    return near, far
+
+
+def amplify_s_gradient(variance_param: torch.nn.Parameter, k: float = 5.0) -> bool:
+    """training/scheduler.py:14-33: grad <- k*grad when grad < 0; returns whether it fired."""
+    if variance_param.grad is None:
+        return False
+    flag = torch.zeros(1, device=variance_param.device, dtype=torch.int32)
+    g = variance_param.grad
+    Fn.s_amplify(g.reshape(1), float(k), flag)
+    return bool(flag.item())
+
+
+def s_grad_to_v(grad_s: torch.Tensor, variance: torch.Tensor, s_max: float) -> torch.Tensor:
+    """dL/dv from dL/ds for s = clamp(exp(10 v), 1e-4, s_max) (models/variance.py:38-41)."""
+    e = torch.exp(variance.detach() * 10.0)
+    inside = ((e >= 1e-4) & (e <= s_max)).to(e.dtype)
+    return grad_s.reshape(()) * e * inside * 10.0
+
+
+@dataclass
+class StepConfig:
+    n_samples: int = 64
+    n_importance: int = 64
+    up_sample_steps: int = 4
+    perturb: bool = False
+    sphere_radius: float = 1.0
+    huber_delta: float = 0.1
+    w_eik: float = 0.1
+    w_curv: float = 5e-4
+    s_amp_k: float = 5.0
+    amplify: bool = True
+    relax_radius: float = 1.2
+    mlp_mode: int = N.VXN_MLP_FP32
+    active_levels: Optional[float] = None
+
+
+class FusedTrainStep:
+    """Owns the device workspaces of one scene's training step.
+
+    Parameters are plain device tensors: ``grid`` [X,Y,Z] f32, ``table`` [L,T,2] f32 or
+    bf16, ``mlp`` [VXN_MLP_PARAMS] f32 (W0, b0, W1, b1 flattened), ``variance`` scalar.
+    Gradients land in ``grad_grid`` [X,Y,Z], ``grad_table`` [L,T,2] f32, ``grad_mlp``,
+    ``grad_v`` (after amplification); ``loss`` / ``reg_losses`` are FP64 device scalars.
+    """
+
+    def __init__(self, grid: torch.Tensor, table: torch.Tensor, mlp: torch.Tensor, variance: torch.Tensor,
+                 level_res, box=(-1.0, -1.0, -1.0, 1.0, 1.0, 1.0), s_max: float = 300.0,
+                 occ_mask: Optional[torch.Tensor] = None, cfg: StepConfig = StepConfig()):
+        self.grid, self.table, self.mlp, self.variance = grid, table, mlp, variance
+        self.level_res = list(level_res)
+        self.box = tuple(float(v) for v in box)
+        self.s_max = float(s_max)
+        self.occ_mask = occ_mask
+        self.cfg = cfg
+        dev = grid.device
+        X, Y, Z = grid.shape
+        self.grad_grid = torch.zeros_like(grid)
+        self.grad_table = torch.zeros(table.shape, device=dev, dtype=torch.float32)
+        self.grad_mlp = torch.zeros(N.MLP_PARAMS, device=dev, dtype=torch.float32)
+        self.grad_s = torch.zeros(1, device=dev, dtype=torch.float32)
+        self.grad_v = torch.zeros((), device=dev, dtype=torch.float32)
+        self.loss = torch.zeros(1, device=dev, dtype=torch.float64)
+        self.amplified = torch.zeros(1, device=dev, dtype=torch.int32)
+        self.bitmask = torch.zeros(Fn.vr_bitmask_words((X, Y, Z)), device=dev, dtype=torch.int32)
+        self.vr_list = torch.empty(X * Y * Z, device=dev, dtype=torch.int32)
+        self.vr_counts = torch.zeros(2, device=dev, dtype=torch.int32)
+        nb = ctypes.c_int64(0)
+        N.check(N.lib().vxn_vr_workspace_bytes(X, Y, Z, ctypes.byref(nb)))
+        self.vr_ws = torch.empty(max(int(nb.value), 8), device=dev, dtype=torch.uint8)
+        N.check(N.lib().vxn_reg_workspace_bytes(X, Y, Z, ctypes.byref(nb)))
+        self.reg_ws = torch.empty(int(nb.value), device=dev, dtype=torch.uint8)
+        self.reg_losses = torch.zeros(2, device=dev, dtype=torch.float64)
+        self._render_ws = None
+        self.render_events = None  # optional (start, end) events recorded around the render kernel
+        self.level_w = None
+        if cfg.active_levels is not None:
+            from .hash_grid import level_weights
+            self.level_w = level_weights(len(self.level_res), cfg.active_levels)
+
+    def _render_args(self, rays_o, rays_d, mid, dists, target):
+        cfg = self.cfg
+        rc = Fn.RenderConfig(self.box, self.level_res, self.level_w, cfg.sphere_radius, mlp_mode=cfg.mlp_mode)
+        a = N.RenderArgs()
+        s = torch.clamp(torch.exp(self.variance.detach() * 10.0), min=1e-4, max=self.s_max)
+        self.s_value = s
+        Fn.fill_render_args(a, rc, self.grid, self.table, self.mlp, float(s), rays_o, rays_d, mid, dists)
+        a.target_rgb = N.dptr(target, torch.float32, "target")
+        a.huber_delta = cfg.huber_delta
+        a.grad_grid, a.grad_table = N.dptr(self.grad_grid), N.dptr(self.grad_table)
+        a.grad_mlp, a.grad_s, a.loss = N.dptr(self.grad_mlp), N.dptr(self.grad_s), N.dptr(self.loss)
+        a.vr_bitmask = N.dptr(self.bitmask)
+        need = ctypes.c_int64(0)
+        N.check(N.lib().vxn_render_workspace_bytes(ctypes.byref(a), ctypes.byref(need)))
+        if self._render_ws is None or self._render_ws.numel() < need.value:
+            self._render_ws = torch.empty(int(need.value), device=mid.device, dtype=torch.uint8)
+        a.workspace, a.workspace_bytes = self._render_ws.data_ptr(), self._render_ws.numel()
+        return a
+
+    def sample(self, rays_o, rays_d, t_rand=None, u_rand=None):
+        cfg = self.cfg
+        near, far = near_far_from_sphere(rays_o, rays_d, radius=1.0)
+        return Fn.upsample(rays_o, rays_d, near, far, self.grid, self.box, cfg.n_samples, cfg.n_importance,
+                           cfg.up_sample_steps, cfg.perturb, self.occ_mask, self.box, t_rand, u_rand)
+
+    def step(self, rays_o: torch.Tensor, rays_d: torch.Tensor, target: torch.Tensor, t_rand=None, u_rand=None,
+             zs=None) -> Dict[str, torch.Tensor]:
+        """One fwd+bwd step; gradients are overwritten.  Returns device tensors."""
+        cfg = self.cfg
+        rays_o, rays_d, target = rays_o.contiguous(), rays_d.contiguous(), target.contiguous()
+        if zs is None:
+            z, dists, mid = self.sample(rays_o, rays_d, t_rand, u_rand)
+        else:
+            z, dists, mid = zs
+        self.grad_grid.zero_()
+        self.grad_table.zero_()
+        self.bitmask.zero_()
+        a = self._render_args(rays_o, rays_d, mid.contiguous(), dists.contiguous(), target)
+        ev = self.render_events
+        if ev is not None:
+            ev[0].record()
+        N.check(N.lib().vxn_render_train(ctypes.byref(a), Fn._stream(mid)), "render_train")
+        if ev is not None:
+            ev[1].record()
+        N.check(N.lib().vxn_vr_compact(N.dptr(self.bitmask), *self.grid.shape, N.box6(self.box[:3], self.box[3:]),
+                                       float(cfg.relax_radius), N.dptr(self.vr_list), N.dptr(self.vr_counts),
+                                       self.vr_ws.data_ptr(), self.vr_ws.numel(), Fn._stream(mid)), "vr_compact")
+        N.check(N.lib().vxn_reg_eik_curv(N.dptr(self.grid), *self.grid.shape, N.box6(self.box[:3], self.box[3:]),
+                                         N.dptr(self.vr_list), N.dptr(self.vr_counts), float(cfg.w_eik),
+                                         float(cfg.w_curv), N.dptr(self.grad_grid), N.dptr(self.reg_losses),
+                                         self.reg_ws.data_ptr(), self.reg_ws.numel(), Fn._stream(mid)),
+                "reg_eik_curv")
+        self.grad_v = s_grad_to_v(self.grad_s, self.variance, self.s_max).reshape(1).contiguous()
+        if cfg.amplify:
+            Fn.s_amplify(self.grad_v, cfg.s_amp_k, self.amplified)
+        return {"loss": self.loss, "reg_losses": self.reg_losses, "vr_counts": self.vr_counts,
+                "amplified": self.amplified, "z_vals": z, "dists": dists, "mid_z_vals": mid}
+
+    def active_linear(self) -> torch.Tensor:
+        """Sorted V_R linear indices after the relaxed-sphere filter (one host read)."""
+        n_all, n_in = (int(v) for v in self.vr_counts.tolist())
+        lst = self.vr_list[:n_all]
+        if n_in > 0:
+            lst = lst[lst >= 0]
+        return lst & 0x7FFFFFFF
