@@ -71,6 +71,12 @@ int vxn_gradient_interp_fwd(const float* grid, int X, int Y, int Z, const float*
 int vxn_gradient_interp_bwd(const float* grad_out, const float* coords, int64_t n, float* grad_grid, int X, int Y,
                             int Z, const float* eps3, void* stream);
 
+/* Corner indices [n,8,3] int64 and trilinear weights [n,8] (both nullable) of continuous
+ * grid coordinates [n,3].  Replaces get_voxel_corners_and_weights
+ * (geometry/coordinates.py:55-124).                                                   */
+int vxn_voxel_corners(const float* coords, int64_t n, int X, int Y, int Z, int64_t* corners, float* corner_weights,
+                      void* stream);
+
 /* SDF value and interpolated unit normal at every point of an NX*NY*NZ lattice spanning
  * lattice_box6 (host), written [NX,NY,NZ] and [NX,NY,NZ,3].  The mesh-extraction query
  * of geometry/mesh.py:80-114 (compute_mesh_normals) over a full lattice.             */
@@ -114,6 +120,8 @@ typedef struct vxn_sampler_args {
   float* dists;        /* [R, S] */
   float* mid_z;        /* [R, S] */
   int32_t* inds;       /* nullable [steps, R, per_step]: searchsorted indices */
+  float bound_radius;  /* when near and far are both null: near/far from this bounding sphere
+                          (near_far_from_sphere, rendering/rays.py:102-128), computed in-kernel */
 } vxn_sampler_args;
 
 int vxn_upsample(const vxn_sampler_args* a, void* stream);
@@ -200,10 +208,11 @@ int vxn_vr_compact(const uint32_t* bitmask, int X, int Y, int Z, const float* bo
  * accumulated into grad_grid with weights w_eik / w_curv, plus both losses reduced in
  * FP64 (means over interior vertices) into losses[2] (device).  |V_R| is counts[1] when
  * non-zero (relaxed-sphere filter applied) else counts[0] (the filter's keep-all case).
- * Replaces compute_eikonal_loss_and_gradient and compute_curvature_loss_and_gradient
- * (regularization/eikonal.py:13-171, regularization/curvature.py:13-165).            */
+ * eps3 (host) is the voxel size.  Replaces compute_eikonal_loss_and_gradient and
+ * compute_curvature_loss_and_gradient (regularization/eikonal.py:13-171,
+ * regularization/curvature.py:13-165).                                               */
 int vxn_reg_workspace_bytes(int X, int Y, int Z, int64_t* bytes);
-int vxn_reg_eik_curv(const float* grid, int X, int Y, int Z, const float* box6, const int32_t* vr_list,
+int vxn_reg_eik_curv(const float* grid, int X, int Y, int Z, const float* eps3, const int32_t* vr_list,
                      const int32_t* counts, float w_eik, float w_curv, float* grad_grid, double* losses,
                      void* workspace, int64_t workspace_bytes, void* stream);
 

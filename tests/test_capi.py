@@ -74,5 +74,9 @@ def test_host_validation_errors():
     s.grid, s.X, s.Y, s.Z, s.box = 16, 8, 8, 8, box
     s.n_rays, s.n_samples, s.n_importance, s.up_sample_steps = 4, 64, 128, 4
     assert L.vxn_upsample(ctypes.byref(s), None) == -4  # 64 + 4*32 > 128
-    assert L.vxn_reg_eik_curv(ctypes.c_void_p(16), 8, 8, 8, box, ctypes.c_void_p(16), ctypes.c_void_p(16), 0.1, 0.1,
+    eps = N.f3((0.1, 0.1, 0.1))
+    assert L.vxn_reg_eik_curv(ctypes.c_void_p(16), 8, 8, 8, eps, ctypes.c_void_p(16), ctypes.c_void_p(16), 0.1, 0.1,
                               ctypes.c_void_p(16), ctypes.c_void_p(16), None, 0, None) == -5
+    assert L.vxn_reg_eik_curv(ctypes.c_void_p(16), 8, 8, 8, N.f3((0.1, 0.0, 0.1)), ctypes.c_void_p(16),
+                              ctypes.c_void_p(16), 0.1, 0.1, ctypes.c_void_p(16), ctypes.c_void_p(16), None, 0,
+                              None) == -3

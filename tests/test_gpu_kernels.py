@@ -249,7 +249,7 @@ def test_vr_compaction_and_regularisers_match_oracle(Fn):
     le, ge = R.eikonal_loss_and_grad(torch.from_numpy(data), keep, vs)
     lc, gc = R.curvature_loss_and_grad(torch.from_numpy(data), keep, vs)
     gg = torch.zeros(res, res, res, device=DEV)
-    losses = Fn.reg_eik_curv(torch.from_numpy(data).to(DEV), BOX6, lst, counts, 0.1, 5e-4, gg)
+    losses = Fn.reg_eik_curv(torch.from_numpy(data).to(DEV), vs.tolist(), lst, counts, 0.1, 5e-4, gg)
     ref = (ge * 0.1 + gc * 5e-4).numpy()
     assert rel_err(gg.cpu().numpy(), ref) < 1e-5
     assert abs(losses[0].item() - le.item()) <= 1e-5 * le.item()

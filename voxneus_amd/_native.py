@@ -36,7 +36,7 @@ class SamplerArgs(ctypes.Structure):
         ("occ_mask", c_vp), ("occ_res", c_i32), ("occ_box", c_f * 6),
         ("n_samples", c_i32), ("n_importance", c_i32), ("up_sample_steps", c_i32), ("perturb", c_i32),
         ("t_lin", c_vp), ("u_det", c_vp), ("t_rand", c_vp), ("u_rand", c_vp),
-        ("z_vals", c_vp), ("dists", c_vp), ("mid_z", c_vp), ("inds", c_vp),
+        ("z_vals", c_vp), ("dists", c_vp), ("mid_z", c_vp), ("inds", c_vp), ("bound_radius", c_f),
     ]
 
 
@@ -66,6 +66,7 @@ _SIGS = {
     "vxn_grid_query_bwd": (c_i32, [c_vp, c_i32, c_i32, c_i32, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp]),
     "vxn_gradient_interp_fwd": (c_i32, [c_vp, c_i32, c_i32, c_i32, c_vp, c_i64, c_vp, c_vp, c_vp]),
     "vxn_gradient_interp_bwd": (c_i32, [c_vp, c_vp, c_i64, c_vp, c_i32, c_i32, c_i32, c_vp, c_vp]),
+    "vxn_voxel_corners": (c_i32, [c_vp, c_i64, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp]),
     "vxn_lattice_query": (c_i32, [c_vp, c_i32, c_i32, c_i32, c_vp, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp]),
     "vxn_hash_fwd": (c_i32, [c_vp, c_i32, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_i64, c_vp, c_vp]),
     "vxn_hash_bwd": (c_i32, [c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp]),
@@ -134,9 +135,17 @@ def dptr(t: Optional[torch.Tensor], dtype: Optional[torch.dtype] = None, name: s
     return t.data_ptr()
 
 
-def box6(xyz_min, xyz_max):
-    vals = [float(v) for v in list(xyz_min) + list(xyz_max)]
-    return (c_f * 6)(*vals)
+def box6(*box):
+    """Host float[6] {xyz_min, xyz_max} from (min3, max3), a flat 6-sequence or 6 floats."""
+    if len(box) == 2:
+        vals = list(box[0]) + list(box[1])
+    elif len(box) == 1:
+        vals = list(box[0])
+    else:
+        vals = list(box)
+    if len(vals) != 6:
+        raise ValueError(f"a box needs 6 values (got {len(vals)})")
+    return (c_f * 6)(*[float(v) for v in vals])
 
 
 def f3(vals):

@@ -114,6 +114,25 @@ __global__ void __launch_bounds__(256) gradient_interp_bwd_kernel(GridDesc g, co
   }
 }
 
+__global__ void __launch_bounds__(256) voxel_corners_kernel(GridDesc g, const float* __restrict__ coords, int64_t n,
+                                                            int64_t* __restrict__ corners,
+                                                            float* __restrict__ cweights) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const Corners c = corners_from_coords(g, coords[3 * i], coords[3 * i + 1], coords[3 * i + 2]);
+    if (corners) {
+      for (int k = 0; k < 8; ++k) {
+        int x, y, z;
+        corner_xyz(c, k, x, y, z);
+        corners[24 * i + 3 * k] = x;
+        corners[24 * i + 3 * k + 1] = y;
+        corners[24 * i + 3 * k + 2] = z;
+      }
+    }
+    if (cweights)
+      for (int k = 0; k < 8; ++k) cweights[8 * i + k] = c.w[k];
+  }
+}
+
 // Lattice query: one thread per lattice point; consecutive threads walk z, so the
 // 8-corner and 24-neighbour stencil loads of a wavefront fall on a few cache lines.
 __global__ void __launch_bounds__(256) lattice_query_kernel(GridDesc g, int NX, int NY, int NZ, float lx0, float ly0,
@@ -207,6 +226,19 @@ extern "C" int vxn_gradient_interp_bwd(const float* grad_out, const float* coord
   const GridDesc g = coords_desc(nullptr, X, Y, Z, eps3);
   hipLaunchKernelGGL(gradient_interp_bwd_kernel, dim3(blocks_for(n, 256, 65536)), dim3(256), 0, (hipStream_t)stream,
                      g, coords, n, grad_out, grad_grid);
+  VXN_LAUNCH_CHECK();
+  return 0;
+}
+
+extern "C" int vxn_voxel_corners(const float* coords, int64_t n, int X, int Y, int Z, int64_t* corners,
+                                 float* corner_weights, void* stream) {
+  VXN_REQUIRE(X >= 1 && Y >= 1 && Z >= 1 && n >= 0, VXN_ERR_SHAPE, "bad shape");
+  if (n == 0) return 0;
+  VXN_REQUIRE(coords != nullptr, VXN_ERR_NULL, "coords is null");
+  const float eps3[3] = {1.0f, 1.0f, 1.0f};
+  const GridDesc g = coords_desc(nullptr, X, Y, Z, eps3);
+  hipLaunchKernelGGL(voxel_corners_kernel, dim3(blocks_for(n, 256, 65536)), dim3(256), 0, (hipStream_t)stream, g,
+                     coords, n, corners, corner_weights);
   VXN_LAUNCH_CHECK();
   return 0;
 }

@@ -339,14 +339,21 @@ extern "C" int vxn_reg_workspace_bytes(int X, int Y, int Z, int64_t* bytes) {
   return 0;
 }
 
-extern "C" int vxn_reg_eik_curv(const float* grid, int X, int Y, int Z, const float* box6, const int32_t* vr_list,
+extern "C" int vxn_reg_eik_curv(const float* grid, int X, int Y, int Z, const float* eps3, const int32_t* vr_list,
                                 const int32_t* counts, float w_eik, float w_curv, float* grad_grid, double* losses,
                                 void* workspace, int64_t workspace_bytes, void* stream) {
-  if (int e = check_grid(grid, X, Y, Z, box6)) return e;
-  VXN_REQUIRE(vr_list && counts && grad_grid && losses, VXN_ERR_NULL, "null argument");
+  VXN_REQUIRE(grid && eps3 && vr_list && counts && grad_grid && losses, VXN_ERR_NULL, "null argument");
+  VXN_REQUIRE(X >= 3 && Y >= 3 && Z >= 3, VXN_ERR_SHAPE, "grid resolution must be >= 3 per axis");
+  VXN_REQUIRE((int64_t)X * Y * Z < ((int64_t)1 << 31), VXN_ERR_SHAPE, "grid too large for int32 V_R entries");
+  for (int a = 0; a < 3; ++a) VXN_REQUIRE(eps3[a] > 0.0f, VXN_ERR_VALUE, "voxel size must be > 0");
   VXN_REQUIRE(workspace != nullptr && workspace_bytes >= (int64_t)kRegBlocks * 3 * 8, VXN_ERR_WORKSPACE,
               "regulariser workspace needs %lld bytes", (long long)kRegBlocks * 24);
-  const GridDesc g = make_grid_desc(grid, X, Y, Z, box6);
+  GridDesc g = {};
+  g.grid = grid;
+  g.X = X;
+  g.Y = Y;
+  g.Z = Z;
+  for (int a = 0; a < 3; ++a) g.eps[a] = eps3[a];
   hipStream_t s = (hipStream_t)stream;
   hipLaunchKernelGGL(reg_eik_curv_kernel, dim3(kRegBlocks), dim3(kRegThreads), 0, s, g, vr_list, counts, w_eik, w_curv,
                      grad_grid, (double*)workspace);

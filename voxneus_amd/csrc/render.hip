@@ -1,15 +1,23 @@
 // Fused per-ray render: forward, backward and the fused training step.
 //
-// One ray per 128-thread work-group (two wave64s), thread t = sample t of the ray.  A
-// persistent grid walks the rays; per work-group the MLP parameter gradients are
-// accumulated in registers across all its rays and written once as partials that a
-// fixed-order reduction sums (deterministic).  Per-sample state is recomputed in the
-// backward instead of being stored.
+// render_kernel: one ray per 128-thread work-group (two wave64s), thread t = sample t of
+// the ray.  A persistent grid walks the rays; the MLP parameter gradients are accumulated
+// in registers across all rays of a work-group and written once as partials that a
+// fixed-order reduction sums (deterministic).  In the backward the per-sample gradients
+// w.r.t. the hash features and the SDF geometry are written as records (feature-major,
+// so every store coalesces) instead of being scattered in place.
+//
+// hash_scatter_kernel / grid_scatter_kernel replay those records at full occupancy: a
+// wavefront covers 64 consecutive samples, samples whose gradient is exactly zero are
+// skipped, and runs of consecutive samples in the same cell are summed with a
+// wavefront segmented scan so each run issues one set of atomics.
 //
 // Reference: VoxNeuS.forward_points and the render passes 2-3
 // (models/voxneus.py:115-167, 226-258), compute_neus_alpha (rendering/neus_alpha.py:12-81),
 // composite_weights and volume_render_from_samples (rendering/volume_render.py:12-153),
-// and the Huber colour loss (rendering/volume_render.py:43-54).
+// the Huber colour loss (rendering/volume_render.py:43-54), and the scatters of the
+// reference backward: index_put into the hash table (radiance_hash.py:151-160) and the
+// Eq. 7 / grid_sample gradients (cuda/gradient_interp.cu:86-152).
 #include "vxn_hash.h"
 #include "vxn_host.h"
 
@@ -23,6 +31,7 @@ constexpr int kOffB1 = kOffW1 + 3 * 64;
 constexpr int kXS = 37;  // LDS row stride (floats): 32 hash features + 3 normal components
 constexpr int kCS = 17;  // LDS row stride (floats) for a 16-hidden-unit chunk
 constexpr int kStage = kRT * kXS + kRT * 3 + 2 * kRT * kCS;
+constexpr int kGeo = 4;  // geometry gradient record: dL/dsdf, dL/dgrad[3]
 
 enum { kFwd = 0, kBwd = 1, kTrain = 2 };
 
@@ -46,6 +55,8 @@ struct RenderDev {
   uint32_t* vr;
   float* part_mlp;  // [gridDim.x][kNP]
   double* part_sc;  // [gridDim.x][2]: dL/ds, Huber sum
+  float* gfeat;     // [32][R*S] dL/d(hash features), feature-major
+  float* ggeo;      // [4][R*S]  dL/dsdf, dL/dgrad
 };
 
 struct Red {
@@ -102,14 +113,16 @@ VXN_DEV void block_sums(float (&v)[N], Red& red, int lane, int wid) {
 }
 
 template <int MODE>
-__global__ void __launch_bounds__(kRT) render_kernel(RenderDev a) {
+__global__ void __launch_bounds__(kRT, 2) render_kernel(RenderDev a) {
   __shared__ Red red;
+  __shared__ float preh[64];  // b0 + W0[:, 32:48] . SH(d): the per-ray part of the hidden layer
   __shared__ float stage[MODE == kFwd ? 1 : kStage];
   const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
   const float* W0 = a.mlp;
   const float* b0 = a.mlp + kOffB0;
   const float* W1 = a.mlp + kOffW1;
   const float* b1 = a.mlp + kOffB1;
+  const int64_t NS = a.R * (int64_t)a.S;
   // LDS staging for the parameter gradients: inputs Xs[128][kXS] (hash features and
   // normal), output grads Os[128][3], and per 16-hidden-unit chunk Hs/Ds[128][kCS].
   float* Xs = stage;
@@ -139,6 +152,17 @@ __global__ void __launch_bounds__(kRT) render_kernel(RenderDev a) {
     const float dx = a.d[3 * ray], dy = a.d[3 * ray + 1], dz = a.d[3 * ray + 2];
     const float mid = valid ? a.mid[si] : 0.0f;
     const float dist = valid ? a.dist[si] : 0.0f;
+    // per-ray SH embedding and its contribution to the hidden pre-activations
+    float sh[16];
+    sh16(dx, dy, dz, sh);
+    if (t < 64) {
+      const float* wr = W0 + t * 51 + 32;
+      float acc = b0[t];
+#pragma unroll
+      for (int i = 0; i < 16; ++i) acc = fmaf(wr[i], sh[i], acc);
+      preh[t] = acc;
+    }
+    const float shA = sh[grp & 7], shB = sh[(grp & 7) + 8];
     // ---------------------------------------------------------------- sample forward
     const float px = ox + dx * mid, py = oy + dy * mid, pz = oz + dz * mid;
     const GsStencil gs = gs_stencil(a.g, px, py, pz);
@@ -151,21 +175,23 @@ __global__ void __launch_bounds__(kRT) render_kernel(RenderDev a) {
     const float x0 = fminf(fmaxf((px - a.g.mn[0]) / a.g.den[0], 0.0f), 1.0f);
     const float x1 = fminf(fmaxf((py - a.g.mn[1]) / a.g.den[1], 0.0f), 1.0f);
     const float x2 = fminf(fmaxf((pz - a.g.mn[2]) / a.g.den[2], 0.0f), 1.0f);
-    float x[51];
+    float x[35];  // 32 hash features, then the unit normal (MLP input columns 0-31, 48-50)
 #pragma unroll
     for (int l = 0; l < 16; ++l) hash_level_fwd(a.h, l, x0, x1, x2, x[2 * l], x[2 * l + 1]);
-    sh16(dx, dy, dz, x + 32);
-    x[48] = nrm[0];
-    x[49] = nrm[1];
-    x[50] = nrm[2];
+    x[32] = nrm[0];
+    x[33] = nrm[1];
+    x[34] = nrm[2];
+    __syncthreads();  // preh ready
     // MLP forward, streaming the hidden units (one W0 row per iteration)
     float o3[3] = {b1[0], b1[1], b1[2]};
 #pragma unroll 2
     for (int j = 0; j < 64; ++j) {
       const float* wr = W0 + j * 51;
-      float acc = b0[j];
+      float acc = preh[j];
 #pragma unroll
-      for (int i = 0; i < 51; ++i) acc = fmaf(wr[i], x[i], acc);
+      for (int i = 0; i < 32; ++i) acc = fmaf(wr[i], x[i], acc);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) acc = fmaf(wr[48 + k], x[32 + k], acc);
       const float hj = fmaxf(acc, 0.0f);
       o3[0] = fmaf(W1[j], hj, o3[0]);
       o3[1] = fmaf(W1[64 + j], hj, o3[1]);
@@ -317,12 +343,9 @@ __global__ void __launch_bounds__(kRT) render_kernel(RenderDev a) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) dout[c] = valid ? (w * dr[c]) * (1.0f - rgb[c]) * rgb[c] : 0.0f;
 #pragma unroll
-    for (int i = 0; i < 32; ++i) Xs[t * kXS + i] = valid ? x[i] : 0.0f;
+    for (int i = 0; i < 35; ++i) Xs[t * kXS + i] = valid ? x[i] : 0.0f;
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      Xs[t * kXS + 32 + k] = valid ? x[48 + k] : 0.0f;
-      Os[t * 3 + k] = dout[k];
-    }
+    for (int k = 0; k < 3; ++k) Os[t * 3 + k] = dout[k];
     float dxa[35];
 #pragma unroll
     for (int i = 0; i < 35; ++i) dxa[i] = 0.0f;
@@ -333,9 +356,11 @@ __global__ void __launch_bounds__(kRT) render_kernel(RenderDev a) {
       for (int jj = 0; jj < 16; ++jj) {
         const int j = 16 * cc + jj;
         const float* wr = W0 + j * 51;
-        float acc = b0[j];
+        float acc = preh[j];
 #pragma unroll
-        for (int i = 0; i < 51; ++i) acc = fmaf(wr[i], x[i], acc);
+        for (int i = 0; i < 32; ++i) acc = fmaf(wr[i], x[i], acc);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) acc = fmaf(wr[48 + k], x[32 + k], acc);
         const float hj = fmaxf(acc, 0.0f);
         const float dh = (W1[j] * dout[0] + W1[64 + j] * dout[1]) + W1[128 + j] * dout[2];
         const float dzj = hj > 0.0f ? dh : 0.0f;
@@ -362,8 +387,8 @@ __global__ void __launch_bounds__(kRT) render_kernel(RenderDev a) {
       }
 #pragma unroll
       for (int k = 0; k < 5; ++k) accX[cc][k] += aX[k];
-      accSH[cc][0] = fmaf(x[32 + grp], dsum, accSH[cc][0]);
-      accSH[cc][1] = fmaf(x[32 + grp + 8], dsum, accSH[cc][1]);
+      accSH[cc][0] = fmaf(shA, dsum, accSH[cc][0]);
+      accSH[cc][1] = fmaf(shB, dsum, accSH[cc][1]);
       accB0[cc] += dsum;
       accW1[cc] += aW1;
       __syncthreads();
@@ -372,17 +397,16 @@ __global__ void __launch_bounds__(kRT) render_kernel(RenderDev a) {
       for (int s2 = 0; s2 < kRT; ++s2) accb1 += Os[s2 * 3 + t];
 #pragma unroll
     for (int k = 0; k < 3; ++k) dn[k] += dxa[32 + k];
-    // ---------------------------------------------------------------- scatters
+    // ---------------------------------------------------------------- gradient records
     if (valid) {
 #pragma unroll
-      for (int l = 0; l < 16; ++l) hash_level_bwd(a.h, l, x0, x1, x2, dxa[2 * l], dxa[2 * l + 1], a.grad_table);
+      for (int i = 0; i < 32; ++i) a.gfeat[i * NS + si] = dxa[i];
       const float rr = gr + 1e-8f;
       const float dotv = (dn[0] * gv[0] + dn[1] * gv[1]) + dn[2] * gv[2];
-      float dg[3];
+      a.ggeo[si] = dsdf;
 #pragma unroll
-      for (int k = 0; k < 3; ++k) dg[k] = dn[k] / rr - (gr > 0.0f ? gv[k] * dotv / (gr * rr * rr) : 0.0f);
-      eq7_scatter(a.g, cn, dg, a.grad_grid);
-      if (dsdf != 0.0f) gs_scatter(a.g, gs, dsdf, a.grad_grid);
+      for (int k = 0; k < 3; ++k)
+        a.ggeo[(1 + k) * NS + si] = dn[k] / rr - (gr > 0.0f ? gv[k] * dotv / (gr * rr * rr) : 0.0f);
     }
     __syncthreads();
   }
@@ -413,15 +437,203 @@ __global__ void __launch_bounds__(kRT) render_kernel(RenderDev a) {
   }
 }
 
-// Fixed-order reduction of the per-work-group partials.
-__global__ void __launch_bounds__(256) render_reduce_kernel(const float* __restrict__ part_mlp,
-                                                            const double* __restrict__ part_sc, int nb,
-                                                            float* __restrict__ grad_mlp, float* __restrict__ grad_s,
-                                                            double* __restrict__ loss, float inv_n3) {
+// ------------------------------------------------------------------------------------
+// Wavefront segmented sum: lanes with equal keys in consecutive runs; every lane gets
+// the sum over [run start, itself]; returns true on the run's last lane.
+// ------------------------------------------------------------------------------------
+template <int NV>
+VXN_DEV bool seg_sum(uint64_t key, float (&v)[NV], int lane) {
+  const uint64_t kp = __shfl_up(key, 1, 64);
+  const uint64_t kn = __shfl_down(key, 1, 64);
+  int st = (lane == 0 || kp != key) ? lane : 0;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int u = __shfl_up(st, o, 64);
+    if (lane >= o) st = max(st, u);
+  }
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      const float u = __shfl_up(v[k], o, 64);
+      if (lane - o >= st) v[k] += u;
+    }
+  }
+  return lane == 63 || kn != key;
+}
+
+// Sample position of record i (ray = i / S).
+VXN_DEV void sample_pos(const RenderDev& a, int64_t i, float& px, float& py, float& pz) {
+  const int64_t ray = i / a.S;
+  const float m = a.mid[i];
+  px = a.o[3 * ray] + a.d[3 * ray] * m;
+  py = a.o[3 * ray + 1] + a.d[3 * ray + 1] * m;
+  pz = a.o[3 * ray + 2] + a.d[3 * ray + 2] * m;
+}
+
+// One thread per sample; the hash levels are walked in order so a wavefront handles one
+// level of 64 consecutive samples at a time.  Levels below `dedup_levels` merge runs.
+__global__ void __launch_bounds__(256) hash_scatter_kernel(RenderDev a, int dedup_levels) {
+  const int64_t NS = a.R * (int64_t)a.S;
+  const int lane = threadIdx.x & 63;
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const bool in = i < NS;
+  float x0 = 0.0f, x1 = 0.0f, x2 = 0.0f;
+  if (in) {
+    float px, py, pz;
+    sample_pos(a, i, px, py, pz);
+    x0 = fminf(fmaxf((px - a.g.mn[0]) / a.g.den[0], 0.0f), 1.0f);
+    x1 = fminf(fmaxf((py - a.g.mn[1]) / a.g.den[1], 0.0f), 1.0f);
+    x2 = fminf(fmaxf((pz - a.g.mn[2]) / a.g.den[2], 0.0f), 1.0f);
+  }
+  for (int l = 0; l < a.h.L; ++l) {
+    float d0 = in ? a.gfeat[(2 * l) * NS + i] : 0.0f;
+    float d1 = in ? a.gfeat[(2 * l + 1) * NS + i] : 0.0f;
+    const float lw = a.h.lw[l];
+    if (lw == 0.0f) continue;
+    if (lw < 1.0f) {
+      d0 = d0 * lw;
+      d1 = d1 * lw;
+    }
+    const bool live = d0 != 0.0f || d1 != 0.0f;
+    if (__ballot(live) == 0) continue;
+    const HashCell c = hash_cell(x0, x1, x2, a.h.res[l], a.h.mask);
+    float v[16];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      v[2 * k] = d0 * c.w[k];
+      v[2 * k + 1] = d1 * c.w[k];
+    }
+    bool tail = live;
+    if (l < dedup_levels) {
+      const float res = a.h.res[l];
+      const uint64_t cx = (uint64_t)(int)floorf(x0 * res), cy = (uint64_t)(int)floorf(x1 * res),
+                     cz = (uint64_t)(int)floorf(x2 * res);
+      const uint64_t key = live ? ((cx << 42) | (cy << 21) | cz) : (~0ull - (uint64_t)lane);
+      tail = seg_sum<16>(key, v, lane) && live;
+    }
+    if (tail) {
+      float* base = a.grad_table + (int64_t)l * a.h.T * 2;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        float* p = base + (int64_t)c.idx[k] * 2;
+        atomicAdd(p, v[2 * k]);
+        atomicAdd(p + 1, v[2 * k + 1]);
+      }
+    }
+  }
+}
+
+// Brick slots of the Eq. 7 stencil around a cell with base corner c0: 8 cell corners
+// (ox,oy,oz) then the 24 face neighbours at offset -1 / +2 along one axis.
+VXN_DEV int slot_corner(int ox, int oy, int oz) { return 4 * ox + 2 * oy + oz; }
+VXN_DEV int slot_face(int axis, int side, int u, int v) { return 8 + axis * 8 + side * 4 + 2 * u + v; }
+
+__global__ void __launch_bounds__(256) grid_scatter_kernel(RenderDev a) {
+  const int64_t NS = a.R * (int64_t)a.S;
+  const int lane = threadIdx.x & 63;
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const bool in = i < NS;
+  float dsdf = 0.0f, dg[3] = {0.0f, 0.0f, 0.0f};
+  float px = 0.0f, py = 0.0f, pz = 0.0f;
+  if (in) {
+    dsdf = a.ggeo[i];
+    dg[0] = a.ggeo[NS + i];
+    dg[1] = a.ggeo[2 * NS + i];
+    dg[2] = a.ggeo[3 * NS + i];
+    sample_pos(a, i, px, py, pz);
+  }
+  const bool live = dsdf != 0.0f || dg[0] != 0.0f || dg[1] != 0.0f || dg[2] != 0.0f;
+  if (__ballot(live) == 0) return;
+  const Corners cn = corner_decomp(a.g, px, py, pz);
+  const GsStencil gs = gs_stencil(a.g, px, py, pz);
+  const GridDesc& g = a.g;
+  // interior cells: the whole 4x4x4 brick is inside the grid and the divisors are 2*eps
+  const bool interior = cn.c0[0] >= 1 && cn.c0[0] + 2 <= g.X - 1 && cn.c0[1] >= 1 && cn.c0[1] + 2 <= g.Y - 1 &&
+                        cn.c0[2] >= 1 && cn.c0[2] + 2 <= g.Z - 1;
+  const bool gs_same = gs.x0 == cn.c0[0] && gs.y0 == cn.c0[1] && gs.z0 == cn.c0[2] && gs.valid == 0xffu;
+  if (live && !interior) {
+    eq7_scatter(g, cn, dg, a.grad_grid);
+    if (dsdf != 0.0f) gs_scatter(g, gs, dsdf, a.grad_grid);
+  }
+  const bool brick = live && interior;
+  float v[32];
+#pragma unroll
+  for (int k = 0; k < 32; ++k) v[k] = 0.0f;
+  if (brick) {
+    // interior divisor of every corner: max((i_p - i_m) * eps, eps) = 2 * eps
+    const float dvx = fmaxf(2.0f * g.eps[0], g.eps[0]), dvy = fmaxf(2.0f * g.eps[1], g.eps[1]),
+                dvz = fmaxf(2.0f * g.eps[2], g.eps[2]);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int ox = (k >> 2) & 1, oy = (k >> 1) & 1, oz = k & 1;
+      const float vx = (cn.w[k] * dg[0]) / dvx;
+      const float vy = (cn.w[k] * dg[1]) / dvy;
+      const float vz = (cn.w[k] * dg[2]) / dvz;
+      // x stencil of corner k: +v at ox+1, -v at ox-1
+      v[ox ? slot_face(0, 1, oy, oz) : slot_corner(1, oy, oz)] += vx;
+      v[ox ? slot_corner(0, oy, oz) : slot_face(0, 0, oy, oz)] -= vx;
+      v[oy ? slot_face(1, 1, ox, oz) : slot_corner(ox, 1, oz)] += vy;
+      v[oy ? slot_corner(ox, 0, oz) : slot_face(1, 0, ox, oz)] -= vy;
+      v[oz ? slot_face(2, 1, ox, oy) : slot_corner(ox, oy, 1)] += vz;
+      v[oz ? slot_corner(ox, oy, 0) : slot_face(2, 0, ox, oy)] -= vz;
+    }
+    if (dsdf != 0.0f) {
+      if (gs_same) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] += gs.w[k] * dsdf;
+      } else {
+        gs_scatter(g, gs, dsdf, a.grad_grid);
+      }
+    }
+  }
+  const uint64_t key = brick ? (((uint64_t)cn.c0[0] << 42) | ((uint64_t)cn.c0[1] << 21) | (uint64_t)cn.c0[2])
+                             : (~0ull - (uint64_t)lane);
+  const bool tail = seg_sum<32>(key, v, lane) && brick;
+  if (tail) {
+    const int64_t base = lin3(g, cn.c0[0], cn.c0[1], cn.c0[2]);
+    const int64_t sx = (int64_t)g.Y * g.Z, sy = g.Z;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int ox = (k >> 2) & 1, oy = (k >> 1) & 1, oz = k & 1;
+      atomicAdd(a.grad_grid + base + ox * sx + oy * sy + oz, v[k]);
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+#pragma unroll
+      for (int w = 0; w < 2; ++w) {
+        atomicAdd(a.grad_grid + base - sx + u * sy + w, v[slot_face(0, 0, u, w)]);
+        atomicAdd(a.grad_grid + base + 2 * sx + u * sy + w, v[slot_face(0, 1, u, w)]);
+        atomicAdd(a.grad_grid + base + u * sx - sy + w, v[slot_face(1, 0, u, w)]);
+        atomicAdd(a.grad_grid + base + u * sx + 2 * sy + w, v[slot_face(1, 1, u, w)]);
+        atomicAdd(a.grad_grid + base + u * sx + w * sy - 1, v[slot_face(2, 0, u, w)]);
+        atomicAdd(a.grad_grid + base + u * sx + w * sy + 2, v[slot_face(2, 1, u, w)]);
+      }
+  }
+}
+
+// Fixed-order reduction of the per-work-group partials: work-group b2 sums the blocks
+// {b2, b2 + 64, ...} for 256 consecutive parameters; a second pass sums the 64 rows.
+constexpr int kRedRows = 64;
+
+__global__ void __launch_bounds__(256) render_reduce1_kernel(const float* __restrict__ part_mlp, int nb,
+                                                             float* __restrict__ rows) {
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  const int r = blockIdx.y;
+  if (k >= kNP) return;
+  float acc = 0.0f;
+  for (int b = r; b < nb; b += kRedRows) acc += part_mlp[(int64_t)b * kNP + k];
+  rows[(int64_t)r * kNP + k] = acc;
+}
+
+__global__ void __launch_bounds__(256) render_reduce2_kernel(const float* __restrict__ rows,
+                                                             const double* __restrict__ part_sc, int nb,
+                                                             float* __restrict__ grad_mlp, float* __restrict__ grad_s,
+                                                             double* __restrict__ loss, float inv_n3) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k < kNP) {
     double acc = 0.0;
-    for (int b = 0; b < nb; ++b) acc += (double)part_mlp[(int64_t)b * kNP + k];
+    for (int r = 0; r < kRedRows; ++r) acc += (double)rows[(int64_t)r * kNP + k];
     grad_mlp[k] = (float)acc;
   }
   if (k == kNP) {
@@ -452,13 +664,23 @@ static int render_blocks(int64_t R) {
   return (int)(R < cap ? (R > 0 ? R : 1) : cap);
 }
 
-static int64_t render_sc_offset(int64_t R) {
-  const int64_t b = (int64_t)render_blocks(R) * kNP * (int64_t)sizeof(float);
-  return (b + 255) & ~(int64_t)255;
-}
+static int64_t align256(int64_t b) { return (b + 255) & ~(int64_t)255; }
 
-static int64_t render_ws_bytes(int64_t R) {
-  return render_sc_offset(R) + (int64_t)render_blocks(R) * 2 * (int64_t)sizeof(double);
+// Workspace layout: MLP partials | scalar partials | reduction rows | gfeat | ggeo.
+struct RenderWs {
+  int64_t off_sc, off_rows, off_feat, off_geo, total;
+};
+
+static RenderWs render_ws(int64_t R, int S) {
+  RenderWs w;
+  const int64_t nb = render_blocks(R);
+  const int64_t ns = R * (int64_t)S;
+  w.off_sc = align256(nb * kNP * (int64_t)sizeof(float));
+  w.off_rows = align256(w.off_sc + nb * 2 * (int64_t)sizeof(double));
+  w.off_feat = align256(w.off_rows + (int64_t)kRedRows * kNP * (int64_t)sizeof(float));
+  w.off_geo = align256(w.off_feat + 32 * ns * (int64_t)sizeof(float));
+  w.total = align256(w.off_geo + kGeo * ns * (int64_t)sizeof(float));
+  return w;
 }
 
 static int build_render_dev(const vxn_render_args& A, RenderDev& a, int mode) {
@@ -471,6 +693,7 @@ static int build_render_dev(const vxn_render_args& A, RenderDev& a, int mode) {
   VXN_REQUIRE(A.mlp_mode == VXN_MLP_FP32, VXN_ERR_UNSUPPORTED, "mlp_mode %d is not available in this build",
               A.mlp_mode);
   VXN_REQUIRE(A.rays_o && A.rays_d && A.mid_z && A.dists && A.table && A.mlp, VXN_ERR_NULL, "null input");
+  VXN_REQUIRE(A.n_rays * (int64_t)A.n_samples < ((int64_t)1 << 40), VXN_ERR_SHAPE, "too many samples");
   HashDesc h;
   if (int e = make_hash_desc(h, A.table, A.table_dtype, A.n_levels, A.log2_T, A.n_features, A.level_res, A.level_w))
     return e;
@@ -511,19 +734,31 @@ static int build_render_dev(const vxn_render_args& A, RenderDev& a, int mode) {
   a.vr = A.vr_bitmask;
   a.part_mlp = nullptr;
   a.part_sc = nullptr;
+  a.gfeat = nullptr;
+  a.ggeo = nullptr;
   if (mode != kFwd) {
     VXN_REQUIRE(A.grad_grid && A.grad_table && A.grad_mlp && A.grad_s, VXN_ERR_NULL, "null gradient output");
-    const int64_t need = render_ws_bytes(A.n_rays);
-    VXN_REQUIRE(A.workspace != nullptr && A.workspace_bytes >= need, VXN_ERR_WORKSPACE,
-                "render workspace needs %lld bytes", (long long)need);
-    a.part_mlp = (float*)A.workspace;
-    a.part_sc = (double*)((char*)A.workspace + render_sc_offset(A.n_rays));
+    const RenderWs w = render_ws(A.n_rays, A.n_samples);
+    VXN_REQUIRE(A.workspace != nullptr && A.workspace_bytes >= w.total, VXN_ERR_WORKSPACE,
+                "render workspace needs %lld bytes", (long long)w.total);
+    char* ws = (char*)A.workspace;
+    a.part_mlp = (float*)ws;
+    a.part_sc = (double*)(ws + w.off_sc);
+    a.gfeat = (float*)(ws + w.off_feat);
+    a.ggeo = (float*)(ws + w.off_geo);
   }
   if (mode == kTrain) {
     VXN_REQUIRE(A.target_rgb != nullptr, VXN_ERR_NULL, "target_rgb is null");
     VXN_REQUIRE(A.huber_delta > 0.0f, VXN_ERR_VALUE, "huber_delta must be > 0");
   }
   return 0;
+}
+
+// Hash levels whose cells are wide enough for consecutive samples to share them.
+static int dedup_levels(const HashDesc& h) {
+  int n = 0;
+  while (n < h.L && h.res[n] <= 1024.0f) ++n;
+  return n;
 }
 
 template <int MODE>
@@ -537,7 +772,18 @@ static int launch_render(const vxn_render_args* p, void* stream) {
   hipLaunchKernelGGL(render_kernel<MODE>, dim3(nb), dim3(kRT), 0, s, a);
   VXN_LAUNCH_CHECK();
   if (MODE != kFwd) {
-    hipLaunchKernelGGL(render_reduce_kernel, dim3((kNP + 1 + 255) / 256), dim3(256), 0, s, a.part_mlp, a.part_sc, nb,
+    const int64_t ns = p->n_rays * (int64_t)p->n_samples;
+    const int sb = (int)((ns + 255) / 256);
+    hipLaunchKernelGGL(hash_scatter_kernel, dim3(sb), dim3(256), 0, s, a, dedup_levels(a.h));
+    VXN_LAUNCH_CHECK();
+    hipLaunchKernelGGL(grid_scatter_kernel, dim3(sb), dim3(256), 0, s, a);
+    VXN_LAUNCH_CHECK();
+    const RenderWs w = render_ws(p->n_rays, p->n_samples);
+    float* rows = (float*)((char*)p->workspace + w.off_rows);
+    hipLaunchKernelGGL(render_reduce1_kernel, dim3((kNP + 255) / 256, kRedRows), dim3(256), 0, s, a.part_mlp, nb,
+                       rows);
+    VXN_LAUNCH_CHECK();
+    hipLaunchKernelGGL(render_reduce2_kernel, dim3((kNP + 1 + 255) / 256), dim3(256), 0, s, rows, a.part_sc, nb,
                        p->grad_mlp, p->grad_s, MODE == kTrain ? p->loss : nullptr, a.inv_n3);
     VXN_LAUNCH_CHECK();
   }
@@ -550,7 +796,8 @@ using namespace vxn;
 
 extern "C" int vxn_render_workspace_bytes(const vxn_render_args* a, int64_t* bytes) {
   VXN_REQUIRE(a != nullptr && bytes != nullptr, VXN_ERR_NULL, "null argument");
-  *bytes = render_ws_bytes(a->n_rays);
+  VXN_REQUIRE(a->n_rays >= 0 && a->n_samples >= 0, VXN_ERR_SHAPE, "bad shape");
+  *bytes = render_ws(a->n_rays, a->n_samples).total;
   return 0;
 }
 

@@ -28,6 +28,7 @@ struct SamplerDev {
   const float *t_lin, *u_det, *t_rand, *u_rand;
   float *z_out, *dists_out, *mid_out;
   int32_t* inds;
+  float bound_r;
 };
 
 // OccupancyGrid.is_occupied (occupancy_grid.py:121-147) folded into the modulation factor
@@ -58,7 +59,17 @@ __global__ void __launch_bounds__(kSampThreads) upsample_kernel(SamplerDev a) {
   for (int64_t ray = blockIdx.x; ray < a.R; ray += gridDim.x) {
     const float ox = a.o[3 * ray], oy = a.o[3 * ray + 1], oz = a.o[3 * ray + 2];
     const float dx = a.d[3 * ray], dy = a.d[3 * ray + 1], dz = a.d[3 * ray + 2];
-    const float nr = a.near[ray], fr = a.far[ray];
+    float nr, fr;
+    if (a.near) {
+      nr = a.near[ray];
+      fr = a.far[ray];
+    } else {  // near_far_from_sphere with the reference's float operation order
+      const float qa = (dx * dx + dy * dy) + dz * dz;
+      const float qb = 2.0f * ((ox * dx + oy * dy) + oz * dz);
+      const float mid = (0.5f * (-qb)) / qa;
+      nr = fmaxf(mid - a.bound_r, 0.05f);
+      fr = fmaxf(mid + a.bound_r, nr + 0.1f);
+    }
     int n = a.n0;
     // 1. stratified samples (sampler.py:85-125)
     float z = 0.0f;
@@ -264,8 +275,9 @@ extern "C" int vxn_upsample(const vxn_sampler_args* p, void* stream) {
   VXN_REQUIRE(per <= kMaxPer, VXN_ERR_UNSUPPORTED, "at most %d importance samples per step", kMaxPer);
   VXN_REQUIRE(steps <= 20, VXN_ERR_UNSUPPORTED, "at most 20 upsampling steps");
   if (A.n_rays == 0) return 0;
-  VXN_REQUIRE(A.rays_o && A.rays_d && A.near && A.far && A.t_lin && A.z_vals && A.dists && A.mid_z, VXN_ERR_NULL,
-              "null argument");
+  VXN_REQUIRE(A.rays_o && A.rays_d && A.t_lin && A.z_vals && A.dists && A.mid_z, VXN_ERR_NULL, "null argument");
+  VXN_REQUIRE((A.near == nullptr) == (A.far == nullptr), VXN_ERR_NULL, "near and far must both be given or both null");
+  if (!A.near) VXN_REQUIRE(A.bound_radius > 0.0f, VXN_ERR_VALUE, "bound_radius must be > 0 when near/far are null");
   VXN_REQUIRE(!steps || A.perturb || A.u_det, VXN_ERR_NULL, "u_det is required in deterministic mode");
   VXN_REQUIRE(!A.perturb || (A.t_rand && (!steps || A.u_rand)), VXN_ERR_NULL,
               "t_rand and u_rand are required when perturb is set");
@@ -296,6 +308,7 @@ extern "C" int vxn_upsample(const vxn_sampler_args* p, void* stream) {
   a.dists_out = A.dists;
   a.mid_out = A.mid_z;
   a.inds = A.inds;
+  a.bound_r = A.bound_radius;
   const int nb = A.n_rays < 8192 ? (int)A.n_rays : 8192;
   hipLaunchKernelGGL(upsample_kernel, dim3(nb), dim3(kSampThreads), 0, (hipStream_t)stream, a);
   VXN_LAUNCH_CHECK();

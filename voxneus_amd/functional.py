@@ -115,6 +115,18 @@ class GradientInterpFunction(torch.autograd.Function):
         return gg, None, None
 
 
+def voxel_corners(coords: torch.Tensor, grid_shape):
+    """-> (corners [n,8,3] int64, weights [n,8]) of continuous grid coordinates."""
+    c = coords.reshape(-1, 3).contiguous()
+    n = c.shape[0]
+    corners = torch.empty(n, 8, 3, device=c.device, dtype=torch.int64)
+    weights = torch.empty(n, 8, device=c.device, dtype=torch.float32)
+    X, Y, Z = grid_shape
+    N.check(N.lib().vxn_voxel_corners(N.dptr(c, torch.float32, "coords"), n, X, Y, Z, N.dptr(corners),
+                                      N.dptr(weights), _stream(c)), "voxel_corners")
+    return corners, weights
+
+
 def lattice_query(grid3, box, NX, NY, NZ, lattice_box):
     g3 = grid3.contiguous()
     sdf = torch.empty(NX, NY, NZ, device=g3.device, dtype=torch.float32)
@@ -194,8 +206,10 @@ def _linspace_dev(a: float, b: float, n: int, device) -> torch.Tensor:
 
 
 def upsample(rays_o, rays_d, near, far, grid3, box, n_samples=64, n_importance=64, up_sample_steps=4,
-             perturb=False, occ_mask=None, occ_box=None, t_rand=None, u_rand=None, return_inds=False):
-    """Multi-step SDF upsampling on the GPU; returns (z_vals, dists, mid_z[, inds])."""
+             perturb=False, occ_mask=None, occ_box=None, t_rand=None, u_rand=None, return_inds=False,
+             bound_radius=1.0):
+    """Multi-step SDF upsampling on the GPU; returns (z_vals, dists, mid_z[, inds]).
+    With ``near = far = None`` the kernel derives them from the ``bound_radius`` sphere."""
     dev = rays_o.device
     R = rays_o.shape[0]
     steps = up_sample_steps if (n_importance > 0 and up_sample_steps > 0) else 0
@@ -206,7 +220,8 @@ def upsample(rays_o, rays_d, near, far, grid3, box, n_samples=64, n_importance=6
     mid = torch.empty_like(z)
     inds = torch.empty(steps, R, per, device=dev, dtype=torch.int32) if (return_inds and steps) else None
     ro, rd = rays_o.contiguous(), rays_d.contiguous()
-    nr, fr = near.reshape(-1).contiguous(), far.reshape(-1).contiguous()
+    nr = None if near is None else near.reshape(-1).contiguous()
+    fr = None if far is None else far.reshape(-1).contiguous()
     if perturb:
         if t_rand is None:
             t_rand = torch.rand(R, n_samples, device=dev)
@@ -236,6 +251,7 @@ def upsample(rays_o, rays_d, near, far, grid3, box, n_samples=64, n_importance=6
     a.u_rand = N.dptr(u_rand) if (perturb and steps) else None
     a.z_vals, a.dists, a.mid_z = N.dptr(z), N.dptr(dists), N.dptr(mid)
     a.inds = N.dptr(inds)
+    a.bound_radius = float(bound_radius)
     N.check(N.lib().vxn_upsample(ctypes.byref(a), _stream(ro)), "upsample")
     if return_inds:
         return z, dists, mid, inds
@@ -380,7 +396,14 @@ def vr_compact(bitmask: torch.Tensor, grid_shape, box, relax_radius: float = 1.2
     return lst, counts
 
 
-def reg_eik_curv(grid3, box, vr_list, counts, w_eik, w_curv, grad_grid):
+def voxel_size_of(box6, grid_shape):
+    """(xyz_max - xyz_min) / (res - 1) in float32, as DenseSDFGrid.voxel_size computes it."""
+    mn = torch.tensor(box6[:3], dtype=torch.float32)
+    mx = torch.tensor(box6[3:], dtype=torch.float32)
+    return ((mx - mn) / (torch.tensor(list(grid_shape), dtype=torch.float32) - 1.0)).tolist()
+
+
+def reg_eik_curv(grid3, eps3, vr_list, counts, w_eik, w_curv, grad_grid):
     """Fused eikonal + curvature over V_R; returns FP64 losses [2] (device)."""
     X, Y, Z = grid3.shape
     dev = grid3.device
@@ -388,7 +411,7 @@ def reg_eik_curv(grid3, box, vr_list, counts, w_eik, w_curv, grad_grid):
     nb = ctypes.c_int64(0)
     N.check(N.lib().vxn_reg_workspace_bytes(X, Y, Z, ctypes.byref(nb)), "reg_workspace_bytes")
     ws = torch.empty(int(nb.value), device=dev, dtype=torch.uint8)
-    N.check(N.lib().vxn_reg_eik_curv(N.dptr(grid3, torch.float32, "grid"), X, Y, Z, N.box6(*box), N.dptr(vr_list),
+    N.check(N.lib().vxn_reg_eik_curv(N.dptr(grid3, torch.float32, "grid"), X, Y, Z, N.f3(eps3), N.dptr(vr_list),
                                      N.dptr(counts), float(w_eik), float(w_curv),
                                      N.dptr(grad_grid, torch.float32, "grad_grid"), N.dptr(losses), ws.data_ptr(),
                                      ws.numel(), _stream(grid3)), "reg_eik_curv")

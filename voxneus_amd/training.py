@@ -82,9 +82,13 @@ class FusedTrainStep:
         self.level_res = list(level_res)
         self.box = tuple(float(v) for v in box)
         self.s_max = float(s_max)
-        self.occ_mask = occ_mask
         self.cfg = cfg
         dev = grid.device
+        if occ_mask is None:
+            # the reference always samples with an occupancy grid; a fresh one is all True
+            # (occupancy_grid.py:51-54), i.e. factor 1 inside the box and 0.1 outside it
+            occ_mask = torch.ones((128, 128, 128), device=dev, dtype=torch.bool)
+        self.occ_mask = occ_mask
         X, Y, Z = grid.shape
         self.grad_grid = torch.zeros_like(grid)
         self.grad_table = torch.zeros(table.shape, device=dev, dtype=torch.float32)
@@ -102,6 +106,7 @@ class FusedTrainStep:
         N.check(N.lib().vxn_reg_workspace_bytes(X, Y, Z, ctypes.byref(nb)))
         self.reg_ws = torch.empty(int(nb.value), device=dev, dtype=torch.uint8)
         self.reg_losses = torch.zeros(2, device=dev, dtype=torch.float64)
+        self.eps3 = Fn.voxel_size_of(self.box, (X, Y, Z))
         self._render_ws = None
         self.render_events = None  # optional (start, end) events recorded around the render kernel
         self.level_w = None
@@ -130,9 +135,10 @@ class FusedTrainStep:
 
     def sample(self, rays_o, rays_d, t_rand=None, u_rand=None):
         cfg = self.cfg
-        near, far = near_far_from_sphere(rays_o, rays_d, radius=1.0)
-        return Fn.upsample(rays_o, rays_d, near, far, self.grid, self.box, cfg.n_samples, cfg.n_importance,
-                           cfg.up_sample_steps, cfg.perturb, self.occ_mask, self.box, t_rand, u_rand)
+        # ray bounds from the unit sphere (trainer.py:188) are computed inside the sampler
+        return Fn.upsample(rays_o, rays_d, None, None, self.grid, self.box, cfg.n_samples, cfg.n_importance,
+                           cfg.up_sample_steps, cfg.perturb, self.occ_mask, self.box, t_rand, u_rand,
+                           bound_radius=1.0)
 
     def step(self, rays_o: torch.Tensor, rays_d: torch.Tensor, target: torch.Tensor, t_rand=None, u_rand=None,
              zs=None) -> Dict[str, torch.Tensor]:
@@ -156,7 +162,7 @@ class FusedTrainStep:
         N.check(N.lib().vxn_vr_compact(N.dptr(self.bitmask), *self.grid.shape, N.box6(self.box[:3], self.box[3:]),
                                        float(cfg.relax_radius), N.dptr(self.vr_list), N.dptr(self.vr_counts),
                                        self.vr_ws.data_ptr(), self.vr_ws.numel(), Fn._stream(mid)), "vr_compact")
-        N.check(N.lib().vxn_reg_eik_curv(N.dptr(self.grid), *self.grid.shape, N.box6(self.box[:3], self.box[3:]),
+        N.check(N.lib().vxn_reg_eik_curv(N.dptr(self.grid), *self.grid.shape, N.f3(self.eps3),
                                          N.dptr(self.vr_list), N.dptr(self.vr_counts), float(cfg.w_eik),
                                          float(cfg.w_curv), N.dptr(self.grad_grid), N.dptr(self.reg_losses),
                                          self.reg_ws.data_ptr(), self.reg_ws.numel(), Fn._stream(mid)),

@@ -1,0 +1,97 @@
+"""The assembled VoxNeuS model, drop-in for models/voxneus.py:21-260.
+
+``render`` runs pass 1 (the multi-step upsampler) and passes 2-3 (per-sample geometry,
+hash features, SH, MLP, Eq. 2 alpha, transmittance and compositing) as two gfx950
+kernels; autograd runs the fused backward kernel.  ``forward_points`` keeps the
+reference's per-point interface (grid-query and hash kernels, MLP on the device).
+"""
+
+from __future__ import annotations
+
+from typing import Dict, List, Optional, Tuple, Union
+
+import torch
+import torch.nn as nn
+
+from . import _native as N
+from . import functional as Fn
+from .hash_grid import RadianceHashGrid, level_weights
+from .radiance import RadianceMLP, SingleVarianceNetwork, SphericalHarmonicsEmbedder
+from .rendering import OccupancyGrid, RaySampler, apply_background
+from .sdf_grid import DenseSDFGrid
+
+
+class VoxNeuS(nn.Module):
+    def __init__(self, grid_res: Union[int, List[int]] = 96, xyz_min=(-1.0, -1.0, -1.0), xyz_max=(1.0, 1.0, 1.0),
+                 gradient_mode: str = "interpolated", init_s_val: float = 0.3, s_max: float = 300.0,
+                 hash_n_levels: int = 16, hash_n_features: int = 2, hash_log2_size: int = 19,
+                 geom_feature_dim: int = 0, mlp_hidden_dim: int = 64, view_embed_type: str = "sh",
+                 entangled: bool = False, occupancy_res: int = 128, force_pytorch: bool = False,
+                 hash_base_resolution: int = 16, hash_max_resolution: float = 2048.0,
+                 hash_dtype: torch.dtype = torch.float32, mlp_mode: int = N.VXN_MLP_FP32):
+        super().__init__()
+        if gradient_mode.lower() != "interpolated":
+            raise NotImplementedError("the fused path implements the Eq. 7 interpolated gradient")
+        if view_embed_type.lower() != "sh":
+            raise NotImplementedError("the fused path implements the SH view embedding")
+        self.gradient_mode = gradient_mode
+        self.entangled = entangled
+        self.geom_feature_dim = geom_feature_dim
+        self.mlp_mode = mlp_mode
+        self.sdf_grid = DenseSDFGrid(grid_res, xyz_min, xyz_max, init_mode="sphere", feat_dim=geom_feature_dim)
+        self.variance_net = SingleVarianceNetwork(init_val=init_s_val, s_max=s_max)
+        self.occupancy_grid = OccupancyGrid(occupancy_res, xyz_min, xyz_max)
+        self.radiance_hash = RadianceHashGrid(hash_n_levels, hash_n_features, hash_log2_size, hash_base_resolution,
+                                              hash_max_resolution, dtype=hash_dtype)
+        self.view_embedder = SphericalHarmonicsEmbedder(4)
+        self.radiance_mlp = RadianceMLP(self.radiance_hash.out_dim, 16, 3, geom_feature_dim, mlp_hidden_dim,
+                                        entangled=entangled)
+
+    @property
+    def s(self) -> torch.Tensor:
+        return self.variance_net.s
+
+    @property
+    def std(self) -> torch.Tensor:
+        return self.variance_net.std
+
+    def forward_points(self, xyz, view_dirs, active_levels: Optional[float] = None) -> Dict[str, torch.Tensor]:
+        """voxneus.py:115-167."""
+        sdf = self.sdf_grid.query_sdf(xyz)
+        grad, normal, corners, weights = self.sdf_grid.query_normals(xyz)
+        g = self.sdf_grid
+        norm_xyz = (xyz - g.xyz_min) / (g.xyz_max - g.xyz_min + 1e-8)
+        feats = self.radiance_hash(norm_xyz, active_levels=active_levels)
+        rgb = self.radiance_mlp(feats, self.view_embedder(view_dirs), normal)
+        return {"sdf": sdf, "grad": grad, "normal": normal, "corner_indices": corners, "weights": weights,
+                "geom_feats": None, "rgb": rgb}
+
+    def render_config(self, active_levels=None, sphere_radius=1.0, cos_anneal_ratio=0.0, use_iter_cos=False):
+        enc = self.radiance_hash.encoder
+        return Fn.RenderConfig(self.sdf_grid.box6, enc.level_res, level_weights(enc.n_levels, active_levels),
+                               sphere_radius, cos_anneal_ratio, use_iter_cos, self.mlp_mode)
+
+    def render(self, rays_o, rays_d, sampler: RaySampler, near, far, background_rgb=None, cos_anneal_ratio=0.0,
+               use_iter_cos_annealing=False, active_levels=None, up_sample_steps=4, sphere_radius=1.0,
+               return_corner_indices: bool = True, vr_bitmask: Optional[torch.Tensor] = None):
+        """voxneus.py:169-260.  ``corner_indices`` ([R,S,8,3] int64) is materialised only when
+        ``return_corner_indices``; ``vr_bitmask`` (optional) receives the V_R marks directly."""
+        z_vals, dists, mid = sampler.sample_multistep_sdf(rays_o, rays_d, near, far, grid=self.sdf_grid,
+                                                          n_samples=sampler.n_samples,
+                                                          n_importance=sampler.n_importance,
+                                                          up_sample_steps=up_sample_steps,
+                                                          occupancy_grid=self.occupancy_grid)
+        cfg = self.render_config(active_levels, sphere_radius, cos_anneal_ratio, use_iter_cos_annealing)
+        outs = Fn.RenderFunction.apply(self.sdf_grid.grid, self.radiance_hash.encoder.embeddings,
+                                       self.radiance_mlp.flat(), self.s, rays_o, rays_d, mid, dists, cfg,
+                                       return_corner_indices, vr_bitmask)
+        rgb, depth, normal, opacity, weights, alpha, T, sdf_s, grad_s, nrm_s, corners = outs
+        pts = rays_o.unsqueeze(1) + rays_d.unsqueeze(1) * mid.unsqueeze(-1)
+        return {"rgb": apply_background(rgb, opacity, background_rgb), "depth": depth, "normal": normal,
+                "opacity": opacity, "weights": weights, "alpha": alpha, "transmittance": T, "pts": pts,
+                "sdf_samples": sdf_s, "corner_indices": corners, "grad_samples": grad_s, "normal_samples": nrm_s,
+                "z_vals": z_vals, "dists": dists, "mid_z_vals": mid}
+
+    # ---- parameter views for the fused training step -------------------------------
+    def flat_mlp(self) -> torch.Tensor:
+        return self.radiance_mlp.flat().detach().contiguous()
