@@ -116,6 +116,7 @@ template <int MODE>
 __global__ void __launch_bounds__(kRT, 2) render_kernel(RenderDev a) {
   __shared__ Red red;
   __shared__ float preh[64];  // b0 + W0[:, 32:48] . SH(d): the per-ray part of the hidden layer
+  __shared__ int nlive_s[2];  // samples per wavefront with a non-zero output gradient
   __shared__ float stage[MODE == kFwd ? 1 : kStage];
   const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
   const float* W0 = a.mlp;
@@ -267,13 +268,25 @@ __global__ void __launch_bounds__(kRT, 2) render_kernel(RenderDev a) {
         }
       }
     }
-    if (a.vr && valid) {
+    if (a.vr) {
+      // V_R marks: a sample in the same cell as the previous lane's marks nothing new;
+      // the two z-corners of each (x, y) column share a word unless z0 ends one.
+      const int key = valid ? (int)lin3(a.g, cn.c0[0], cn.c0[1], cn.c0[2]) : -1;
+      const int kp = __shfl_up(key, 1, 64);
+      if (valid && (lane == 0 || kp != key)) {
 #pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        int cx, cy, cz;
-        corner_xyz(cn, k, cx, cy, cz);
-        const int64_t lin = lin3(a.g, cx, cy, cz);
-        atomicOr(a.vr + (lin >> 5), 1u << (lin & 31));
+        for (int k = 0; k < 8; k += 2) {
+          int cx, cy, cz0, cz1;
+          corner_xyz(cn, k, cx, cy, cz0);
+          corner_xyz(cn, k + 1, cx, cy, cz1);
+          const int64_t l0 = lin3(a.g, cx, cy, cz0), l1 = lin3(a.g, cx, cy, cz1);
+          if ((l0 >> 5) == (l1 >> 5)) {
+            atomicOr(a.vr + (l0 >> 5), (1u << (l0 & 31)) | (1u << (l1 & 31)));
+          } else {
+            atomicOr(a.vr + (l0 >> 5), 1u << (l0 & 31));
+            atomicOr(a.vr + (l1 >> 5), 1u << (l1 & 31));
+          }
+        }
       }
     }
     if (MODE == kFwd) continue;
@@ -342,41 +355,56 @@ __global__ void __launch_bounds__(kRT, 2) render_kernel(RenderDev a) {
     float dout[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) dout[c] = valid ? (w * dr[c]) * (1.0f - rgb[c]) * rgb[c] : 0.0f;
+    // Samples with a zero output gradient add nothing to the parameter gradients: stage
+    // only the live ones, compacted to rows [0, nlive).
+    const bool mlive = valid && (dout[0] != 0.0f || dout[1] != 0.0f || dout[2] != 0.0f);
+    const uint64_t bal = __ballot(mlive);
+    if (lane == 0) nlive_s[wid] = __popcll(bal);
+    __syncthreads();
+    const int nlive = nlive_s[0] + nlive_s[1];
+    const int slot = (wid == 0 ? 0 : nlive_s[0]) + __popcll(bal & ((1ull << lane) - 1ull));
+    const bool wave_live = bal != 0;
+    if (mlive) {
 #pragma unroll
-    for (int i = 0; i < 35; ++i) Xs[t * kXS + i] = valid ? x[i] : 0.0f;
+      for (int i = 0; i < 35; ++i) Xs[slot * kXS + i] = x[i];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) Os[t * 3 + k] = dout[k];
+      for (int k = 0; k < 3; ++k) Os[slot * 3 + k] = dout[k];
+    }
     float dxa[35];
 #pragma unroll
     for (int i = 0; i < 35; ++i) dxa[i] = 0.0f;
 #pragma unroll 1
-    for (int cc = 0; cc < 4; ++cc) {
+    for (int cc = 0; cc < (nlive > 0 ? 4 : 0); ++cc) {
       // recompute this chunk's hidden units; dz0 = relu'(h) * (W1^T dout); dx += W0^T dz0
+      if (wave_live) {
 #pragma unroll 2
-      for (int jj = 0; jj < 16; ++jj) {
-        const int j = 16 * cc + jj;
-        const float* wr = W0 + j * 51;
-        float acc = preh[j];
+        for (int jj = 0; jj < 16; ++jj) {
+          const int j = 16 * cc + jj;
+          const float* wr = W0 + j * 51;
+          float acc = preh[j];
 #pragma unroll
-        for (int i = 0; i < 32; ++i) acc = fmaf(wr[i], x[i], acc);
+          for (int i = 0; i < 32; ++i) acc = fmaf(wr[i], x[i], acc);
 #pragma unroll
-        for (int k = 0; k < 3; ++k) acc = fmaf(wr[48 + k], x[32 + k], acc);
-        const float hj = fmaxf(acc, 0.0f);
-        const float dh = (W1[j] * dout[0] + W1[64 + j] * dout[1]) + W1[128 + j] * dout[2];
-        const float dzj = hj > 0.0f ? dh : 0.0f;
+          for (int k = 0; k < 3; ++k) acc = fmaf(wr[48 + k], x[32 + k], acc);
+          const float hj = fmaxf(acc, 0.0f);
+          const float dh = (W1[j] * dout[0] + W1[64 + j] * dout[1]) + W1[128 + j] * dout[2];
+          const float dzj = hj > 0.0f ? dh : 0.0f;
 #pragma unroll
-        for (int i = 0; i < 32; ++i) dxa[i] = fmaf(wr[i], dzj, dxa[i]);
+          for (int i = 0; i < 32; ++i) dxa[i] = fmaf(wr[i], dzj, dxa[i]);
 #pragma unroll
-        for (int k = 0; k < 3; ++k) dxa[32 + k] = fmaf(wr[48 + k], dzj, dxa[32 + k]);
-        Hs[t * kCS + jj] = valid ? hj : 0.0f;
-        Ds[t * kCS + jj] = valid ? dzj : 0.0f;
+          for (int k = 0; k < 3; ++k) dxa[32 + k] = fmaf(wr[48 + k], dzj, dxa[32 + k]);
+          if (mlive) {
+            Hs[slot * kCS + jj] = hj;
+            Ds[slot * kCS + jj] = dzj;
+          }
+        }
       }
       __syncthreads();
-      // accumulate this chunk's parameter gradients over the ray's 128 samples
+      // accumulate this chunk's parameter gradients over the ray's live samples
       float aX[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f}, dsum = 0.0f, aW1 = 0.0f;
       const int c1 = t < 48 ? (t >> 4) : 0;
 #pragma unroll 4
-      for (int s2 = 0; s2 < kRT; ++s2) {
+      for (int s2 = 0; s2 < nlive; ++s2) {
         const float dv = Ds[s2 * kCS + jl];
         const float* xr = Xs + s2 * kXS + grp;
 #pragma unroll
@@ -394,7 +422,7 @@ __global__ void __launch_bounds__(kRT, 2) render_kernel(RenderDev a) {
       __syncthreads();
     }
     if (t < 3)
-      for (int s2 = 0; s2 < kRT; ++s2) accb1 += Os[s2 * 3 + t];
+      for (int s2 = 0; s2 < nlive; ++s2) accb1 += Os[s2 * 3 + t];
 #pragma unroll
     for (int k = 0; k < 3; ++k) dn[k] += dxa[32 + k];
     // ---------------------------------------------------------------- gradient records
@@ -512,13 +540,26 @@ __global__ void __launch_bounds__(256) hash_scatter_kernel(RenderDev a, int dedu
       const uint64_t key = live ? ((cx << 42) | (cy << 21) | cz) : (~0ull - (uint64_t)lane);
       tail = seg_sum<16>(key, v, lane) && live;
     }
-    if (tail) {
-      float* base = a.grad_table + (int64_t)l * a.h.T * 2;
+    if (__ballot(tail) == 0) continue;
+    // Lane quad (4m .. 4m+3) adds, for sample m (then m+16, m+32, m+48), both features of
+    // the corner pair that differs only in x: x and x+1 change only the low bits of the
+    // hash unless x ends in binary 111, so the two entries usually share a 64-byte segment
+    // and the four adjacent floats merge into one memory request.
+    float* base = a.grad_table + (int64_t)l * a.h.T * 2;
+    const int f = lane & 1, xs = (lane >> 1) & 1;
 #pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        float* p = base + (int64_t)c.idx[k] * 2;
-        atomicAdd(p, v[2 * k]);
-        atomicAdd(p + 1, v[2 * k + 1]);
+    for (int pass = 0; pass < 4; ++pass) {
+      const int src = (lane >> 2) + 16 * pass;
+      const bool st = __shfl((int)tail, src, 64) != 0;
+      if (__ballot(st) == 0) continue;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const float v00 = __shfl(v[2 * k], src, 64), v01 = __shfl(v[2 * k + 1], src, 64);
+        const float v10 = __shfl(v[2 * (k + 4)], src, 64), v11 = __shfl(v[2 * (k + 4) + 1], src, 64);
+        const uint32_t i0 = (uint32_t)__shfl((int)c.idx[k], src, 64);
+        const uint32_t i1 = (uint32_t)__shfl((int)c.idx[k + 4], src, 64);
+        const float val = xs ? (f ? v11 : v10) : (f ? v01 : v00);
+        if (st) atomicAdd(base + (int64_t)(xs ? i1 : i0) * 2 + f, val);
       }
     }
   }
@@ -636,14 +677,18 @@ __global__ void __launch_bounds__(256) render_reduce2_kernel(const float* __rest
     for (int r = 0; r < kRedRows; ++r) acc += (double)rows[(int64_t)r * kNP + k];
     grad_mlp[k] = (float)acc;
   }
-  if (k == kNP) {
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x < 64) {  // one wavefront: fixed-order strided sums
     double ds = 0.0, ls = 0.0;
-    for (int b = 0; b < nb; ++b) {
+    for (int b = threadIdx.x; b < nb; b += 64) {
       ds += part_sc[2 * b];
       ls += part_sc[2 * b + 1];
     }
-    if (grad_s) grad_s[0] = (float)ds;
-    if (loss) loss[0] = ls * (double)inv_n3;
+    ds = wave_sum_d(ds);
+    ls = wave_sum_d(ls);
+    if (threadIdx.x == 0) {
+      if (grad_s) grad_s[0] = (float)ds;
+      if (loss) loss[0] = ls * (double)inv_n3;
+    }
   }
 }
 
