@@ -32,12 +32,21 @@ sys.path.insert(0, REPO)
 HBM_PEAK_GBS = 8000.0
 RAYS = 8192
 SAMPLES = 128
-# Algorithmic bytes per sample of the fused render training kernel (no cache credit):
-# grid forward 32 words (8 corners + 24 face neighbours) = 128 B; hash forward
-# 16 levels x 8 corners x 2 features = 256 entries -> 1024 B (FP32) / 512 B (BF16);
-# grid backward 32 words read-modify-write = 256 B; hash backward 256 FP32 RMW = 2048 B;
-# per-ray streaming (o, d, target, z/dist in, outputs) ~= 9 B per sample.
-BYTES_PER_SAMPLE = {"fp32": 128 + 1024 + 256 + 2048 + 9, "bf16": 128 + 512 + 256 + 2048 + 9}
+LATTICE = 320
+# Algorithmic bytes per sample (no cache credit), per kernel of the fused render launch:
+#   render_kernel  grid forward 32 words (8 corners + 24 face neighbours) = 128 B, hash
+#                  forward 16 levels x 8 corners x 2 features = 256 entries = 1024 B FP32
+#                  / 512 B BF16, per-ray streaming (o, d, target, z/dist in, outputs) ~9 B;
+#   hash_scatter   256 FP32 read-modify-writes = 2048 B;
+#   grid_scatter   32 words read-modify-write = 256 B.
+KERNEL_BYTES = {
+    "fp32": {"render_kernel": 128 + 1024 + 9, "hash_scatter": 2048, "grid_scatter": 256},
+    "bf16": {"render_kernel": 128 + 512 + 9, "hash_scatter": 2048, "grid_scatter": 256},
+}
+# event index pairs bracketing each kernel in vxn_render_args.kernel_events
+KERNEL_EVENTS = {"render_kernel": (0, 1), "hash_scatter": (1, 2), "grid_scatter": (2, 3), "reductions": (3, 4)}
+# lattice query: 32-word stencil (128 B) in, SDF + normal (16 B) out per lattice point
+LATTICE_BYTES_PER_POINT = 128 + 16
 
 
 def log(*a):
@@ -66,28 +75,29 @@ def mlp_flat(sc):
 def run_gpu(sc, batches, form, steps, warmup):
     import torch
 
-    from oracle import ref_cpu as R  # only for the hash level table
+    from voxneus_amd import _native as N
     from voxneus_amd.hash_grid import level_resolutions
     from voxneus_amd.training import FusedTrainStep, StepConfig
 
     dev = torch.device("cuda:0")
     table = torch.from_numpy(sc.embeddings).to(dev)
+    mlp_mode = N.VXN_MLP_FP32
     if form == "bf16":
         table = table.to(torch.bfloat16)
+        mlp_mode = N.VXN_MLP_BF16_MFMA
     st = FusedTrainStep(torch.from_numpy(sc.grid).to(dev), table, mlp_flat(sc).to(dev),
                         torch.tensor(0.3, device=dev), level_resolutions(16, 16, sc.hash_max_resolution),
-                        cfg=StepConfig(perturb=True))
+                        cfg=StepConfig(perturb=True, mlp_mode=mlp_mode))
     gb = [tuple(torch.from_numpy(a).to(dev) for a in b) for b in batches]
     ev = []
 
     def one(i, timed):
         o, d, t = gb[i % len(gb)]
         if timed:
-            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            st.render_events = (e0, e1)
-            ev.append((e0, e1))
+            st.kernel_events = N.KernelEvents(5)
+            ev.append(st.kernel_events)
         else:
-            st.render_events = None
+            st.kernel_events = None
         return st.step(o, d, t)
 
     for i in range(warmup):
@@ -98,14 +108,58 @@ def run_gpu(sc, batches, form, steps, warmup):
         out = one(i, True)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    kern_ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+    kern = {k: float(np.mean([e.elapsed_ms(i, j) for e in ev])) for k, (i, j) in KERNEL_EVENTS.items()}
     loss = out["loss"].item()
-    return {"ms_per_step": dt / steps * 1e3, "rays_per_s": RAYS * steps / dt, "render_kernel_ms": kern_ms,
-            "loss": loss}
+    return {"ms_per_step": dt / steps * 1e3, "rays_per_s": RAYS * steps / dt, "kernel_ms": kern, "loss": loss}
 
 
-def rgb_error_vs_fp64(sc, batch, n_rays=256):
-    """Max-abs RGB of the HIP FP32 forward vs the FP64 oracle on the same z values."""
+def run_lattice(reps=10):
+    """SDF + normal at every point of a 320^3 lattice over the box of a 320^3 grid."""
+    import torch
+
+    from voxneus_amd import _native as N
+    from voxneus_amd import functional as Fn
+    from voxneus_amd import scenes
+
+    dev = torch.device("cuda:0")
+    shape = scenes.MultiPrimitive.seeded(3)
+    grid = torch.from_numpy(scenes.sdf_grid_from(shape.sdf, LATTICE)).to(dev)
+    box = ((-1.0, -1.0, -1.0), (1.0, 1.0, 1.0))
+    Fn.lattice_query(grid, box, LATTICE, LATTICE, LATTICE, box)  # warm-up
+    evs = []
+    for _ in range(reps):
+        e = N.KernelEvents(2)
+        s = torch.cuda.current_stream().cuda_stream
+        e.record(0, s)
+        Fn.lattice_query(grid, box, LATTICE, LATTICE, LATTICE, box)
+        e.record(1, s)
+        evs.append(e)
+    ms = float(np.mean([e.elapsed_ms(0, 1) for e in evs]))
+    pts = LATTICE ** 3
+    achieved = LATTICE_BYTES_PER_POINT * pts / (ms * 1e-3) / 1e9
+    return {"kernel": "lattice_query_kernel", "grid": f"{LATTICE}^3 FP32", "points": pts, "kernel_ms": ms,
+            "bytes_per_point": LATTICE_BYTES_PER_POINT, "achieved": achieved, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "compulsory_GBps": (grid.numel() * 4 + pts * 16) / (ms * 1e-3) / 1e9}
+
+
+def profiled_traffic(kernel, form):
+    """HBM bytes per dispatch of ``kernel`` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (already scaled by the copy calibration), or None."""
+    import re
+
+    path = os.path.join(REPO, "profiles", f"fused_step_grid256_rays8192_{form}.md")
+    try:
+        text = open(path).read()
+    except OSError:
+        return None
+    m = re.search(r"^traffic_bytes_per_dispatch %s: (\d+)" % re.escape(kernel), text, re.M)
+    return int(m.group(1)) if m else None
+
+
+def rgb_error_vs_fp64(sc, batch, form="fp32", n_rays=256):
+    """Max-abs RGB of the HIP forward (FP32 or BF16 form) vs the FP64 oracle on the same z
+    values and the same (FP32) parameters."""
     import torch
 
     from oracle import ref_cpu as R
@@ -121,11 +175,16 @@ def rgb_error_vs_fp64(sc, batch, n_rays=256):
                      dtype=torch.float64)
     with torch.no_grad():
         ref = m64.render(o.double(), d.double(), None, None, zs=tuple(z.double() for z in zs))["rgb"]
-    cfg = Fn.RenderConfig((-1, -1, -1, 1, 1, 1), res)
+    from voxneus_amd import _native as N
+
+    table = torch.from_numpy(sc.embeddings).to(dev)
+    mode = N.VXN_MLP_FP32
+    if form == "bf16":
+        table, mode = table.to(torch.bfloat16), N.VXN_MLP_BF16_MFMA
+    cfg = Fn.RenderConfig((-1, -1, -1, 1, 1, 1), res, mlp_mode=mode)
     s = torch.clamp(torch.exp(torch.tensor(0.3) * 10.0), 1e-4, 300.0)
-    out = Fn.RenderFunction.apply(torch.from_numpy(sc.grid).to(dev), torch.from_numpy(sc.embeddings).to(dev),
-                                  mlp_flat(sc).to(dev), s.to(dev), o.to(dev), d.to(dev), zs[2].to(dev),
-                                  zs[1].to(dev), cfg, False, None)
+    out = Fn.RenderFunction.apply(torch.from_numpy(sc.grid).to(dev), table, mlp_flat(sc).to(dev), s.to(dev),
+                                  o.to(dev), d.to(dev), zs[2].to(dev), zs[1].to(dev), cfg, False, None)
     return float((out[0].detach().cpu().double() - ref).abs().max())
 
 
@@ -165,6 +224,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-rays", type=int, default=128)
     ap.add_argument("--no-error-check", action="store_true", help="skip the FP64 RGB error check (profiling runs)")
+    ap.add_argument("--no-lattice", action="store_true", help="skip the 320^3 lattice-query measurement")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 256 MiB device copy (3 dispatches): the known-size kernel that scales "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (rays shard across GPUs by data "
@@ -173,14 +236,32 @@ def main():
         sys.exit(2)
     import torch
 
+    if a.copy_calibration:
+        src = torch.ones(64 << 20, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.mul(src, 1.0, out=dst)
+        torch.cuda.synchronize()
+        log("copied 3 x 256 MiB")
+        return
     sc, batches = make_workload()
     log(f"GPU: {torch.cuda.get_device_name(0)}; form {a.form}; warmup {a.warmup}; steps {a.steps}")
     r = run_gpu(sc, batches, a.form, a.steps, a.warmup)
     log(f"gpu: {r}")
-    nbytes = BYTES_PER_SAMPLE[a.form] * RAYS * SAMPLES
-    achieved = nbytes / (r["render_kernel_ms"] * 1e-3) / 1e9
-    err = None if a.no_error_check else rgb_error_vs_fp64(sc, batches[0])
+    ns = RAYS * SAMPLES
+    kernels = {}
+    for k, ms in r["kernel_ms"].items():
+        bps = KERNEL_BYTES[a.form].get(k)
+        ach = None if bps is None else bps * ns / (ms * 1e-3) / 1e9
+        kernels[k] = {"ms": ms, "bytes_per_sample": bps, "achieved_GBps": ach}
+    dom = max(KERNEL_BYTES[a.form], key=lambda k: r["kernel_ms"][k])
+    nbytes = KERNEL_BYTES[a.form][dom] * ns
+    achieved = kernels[dom]["achieved_GBps"]
+    err = None if a.no_error_check else rgb_error_vs_fp64(sc, batches[0], a.form)
     log(f"rgb max-abs err vs fp64: {err}")
+    lat = None if a.no_lattice else run_lattice()
+    if lat:
+        log(f"lattice: {lat}")
     cpu = None if a.no_cpu_baseline else cpu_baseline(sc, batches[0], a.cpu_rays)
     if cpu:
         log(f"cpu baseline: {cpu}")
@@ -191,13 +272,15 @@ def main():
         "dtype": "f32" if a.form == "fp32" else "bf16", "data": "synthetic",
         "config": {"workload": "fused fwd+bwd train step: 256^3 FP32 SDF grid (seeded multi-primitive), 16x2 hash "
                                "2^19/level, 8192 rays x (64 + 4x16) samples, hemisphere cameras", "form": a.form,
-                   "rays_per_step": RAYS, "samples_per_ray": SAMPLES, "grid": 256, "log2_hash": 19},
+                   "rays_per_step": RAYS, "samples_per_ray": SAMPLES, "grid": 256, "log2_hash": 19,
+                   "inv_s_reading": "init_s_val=0.3 -> s = exp(10*0.3)"},
         "samples_per_s": r["rays_per_s"] * SAMPLES,
         "rgb_max_abs_err_vs_fp64": err,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "render_kernel<2> (fused render train)", "kernel_ms": r["render_kernel_ms"],
-                     "bytes_per_launch": nbytes},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(dom, a.form),
+                     "kernel": dom, "kernel_ms": r["kernel_ms"][dom], "bytes_per_launch": nbytes},
+        "kernels": kernels,
+        "grid_gather": lat,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

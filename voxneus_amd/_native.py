@@ -55,6 +55,7 @@ class RenderArgs(ctypes.Structure):
         ("target_rgb", c_vp), ("huber_delta", c_f),
         ("grad_grid", c_vp), ("grad_table", c_vp), ("grad_mlp", c_vp), ("grad_s", c_vp), ("loss", c_vp),
         ("vr_bitmask", c_vp), ("workspace", c_vp), ("workspace_bytes", c_i64),
+        ("kernel_events", c_vp),
     ]
 
 
@@ -110,6 +111,55 @@ def lib():
 
 class HipError(RuntimeError):
     pass
+
+
+class KernelEvents:
+    """``n`` HIP timing events for ``vxn_render_args.kernel_events``.
+
+    The events come from the HIP runtime torch has loaded (the library and torch share it
+    through the ``libamdhip64.so.7`` soname), so they are recorded on torch's streams.
+    """
+
+    _hip = None
+
+    def __init__(self, n: int = 5):
+        if KernelEvents._hip is None:
+            h = ctypes.CDLL("libamdhip64.so.7")
+            h.hipEventCreate.argtypes = [ctypes.POINTER(c_vp)]
+            h.hipEventElapsedTime.argtypes = [ctypes.POINTER(c_f), c_vp, c_vp]
+            h.hipEventSynchronize.argtypes = [c_vp]
+            h.hipEventDestroy.argtypes = [c_vp]
+            h.hipEventRecord.argtypes = [c_vp, c_vp]
+            KernelEvents._hip = h
+        self.n = n
+        self.array = (c_vp * n)()
+        for i in range(n):
+            ev = c_vp()
+            if KernelEvents._hip.hipEventCreate(ctypes.byref(ev)) != 0:
+                raise HipError("hipEventCreate failed")
+            self.array[i] = ev
+
+    def record(self, i: int, stream: int):
+        """Record event i on a raw hipStream_t (torch's ``stream.cuda_stream``)."""
+        if KernelEvents._hip.hipEventRecord(self.array[i], c_vp(stream)) != 0:
+            raise HipError("hipEventRecord failed")
+
+    def elapsed_ms(self, i: int, j: int) -> float:
+        """Time between events i and j (waits for j)."""
+        h = KernelEvents._hip
+        if h.hipEventSynchronize(self.array[j]) != 0:
+            raise HipError("hipEventSynchronize failed")
+        ms = c_f(0.0)
+        if h.hipEventElapsedTime(ctypes.byref(ms), self.array[i], self.array[j]) != 0:
+            raise HipError("hipEventElapsedTime failed")
+        return float(ms.value)
+
+    def __del__(self):
+        h = KernelEvents._hip
+        if h is not None:
+            for ev in self.array:
+                if ev:
+                    h.hipEventDestroy(ev)
 
 
 def check(rc: int, what: str = ""):

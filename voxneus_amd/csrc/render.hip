@@ -814,15 +814,20 @@ static int launch_render(const vxn_render_args* p, void* stream) {
   if (p->n_rays == 0) return 0;
   const int nb = render_blocks(p->n_rays);
   hipStream_t s = (hipStream_t)stream;
+  hipEvent_t const* ev = (hipEvent_t const*)p->kernel_events;
+  if (ev) VXN_HIP(hipEventRecord(ev[0], s));
   hipLaunchKernelGGL(render_kernel<MODE>, dim3(nb), dim3(kRT), 0, s, a);
   VXN_LAUNCH_CHECK();
+  if (ev) VXN_HIP(hipEventRecord(ev[1], s));
   if (MODE != kFwd) {
     const int64_t ns = p->n_rays * (int64_t)p->n_samples;
     const int sb = (int)((ns + 255) / 256);
     hipLaunchKernelGGL(hash_scatter_kernel, dim3(sb), dim3(256), 0, s, a, dedup_levels(a.h));
     VXN_LAUNCH_CHECK();
+    if (ev) VXN_HIP(hipEventRecord(ev[2], s));
     hipLaunchKernelGGL(grid_scatter_kernel, dim3(sb), dim3(256), 0, s, a);
     VXN_LAUNCH_CHECK();
+    if (ev) VXN_HIP(hipEventRecord(ev[3], s));
     const RenderWs w = render_ws(p->n_rays, p->n_samples);
     float* rows = (float*)((char*)p->workspace + w.off_rows);
     hipLaunchKernelGGL(render_reduce1_kernel, dim3((kNP + 255) / 256, kRedRows), dim3(256), 0, s, a.part_mlp, nb,
@@ -831,6 +836,7 @@ static int launch_render(const vxn_render_args* p, void* stream) {
     hipLaunchKernelGGL(render_reduce2_kernel, dim3((kNP + 1 + 255) / 256), dim3(256), 0, s, rows, a.part_sc, nb,
                        p->grad_mlp, p->grad_s, MODE == kTrain ? p->loss : nullptr, a.inv_n3);
     VXN_LAUNCH_CHECK();
+    if (ev) VXN_HIP(hipEventRecord(ev[4], s));
   }
   return 0;
 }

@@ -108,7 +108,7 @@ class FusedTrainStep:
         self.reg_losses = torch.zeros(2, device=dev, dtype=torch.float64)
         self.eps3 = Fn.voxel_size_of(self.box, (X, Y, Z))
         self._render_ws = None
-        self.render_events = None  # optional (start, end) events recorded around the render kernel
+        self.kernel_events = None  # optional N.KernelEvents(5) recorded between the render kernels
         self.level_w = None
         if cfg.active_levels is not None:
             from .hash_grid import level_weights
@@ -131,6 +131,8 @@ class FusedTrainStep:
         if self._render_ws is None or self._render_ws.numel() < need.value:
             self._render_ws = torch.empty(int(need.value), device=mid.device, dtype=torch.uint8)
         a.workspace, a.workspace_bytes = self._render_ws.data_ptr(), self._render_ws.numel()
+        if self.kernel_events is not None:
+            a.kernel_events = ctypes.cast(self.kernel_events.array, ctypes.c_void_p)
         return a
 
     def sample(self, rays_o, rays_d, t_rand=None, u_rand=None):
@@ -153,12 +155,7 @@ class FusedTrainStep:
         self.grad_table.zero_()
         self.bitmask.zero_()
         a = self._render_args(rays_o, rays_d, mid.contiguous(), dists.contiguous(), target)
-        ev = self.render_events
-        if ev is not None:
-            ev[0].record()
         N.check(N.lib().vxn_render_train(ctypes.byref(a), Fn._stream(mid)), "render_train")
-        if ev is not None:
-            ev[1].record()
         N.check(N.lib().vxn_vr_compact(N.dptr(self.bitmask), *self.grid.shape, N.box6(self.box[:3], self.box[3:]),
                                        float(cfg.relax_radius), N.dptr(self.vr_list), N.dptr(self.vr_counts),
                                        self.vr_ws.data_ptr(), self.vr_ws.numel(), Fn._stream(mid)), "vr_compact")
