@@ -1,6 +1,6 @@
 """Benchmark: fused VoxNeuS fwd+bwd training step on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--form fp32] [--no-cpu-baseline]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--form both|bf16|fp32] [--no-cpu-baseline]
 
 Workload (the precision/throughput configuration): FP32 SDF grid 256^3 of a seeded
 multi-primitive scene, 16-level x 2-feature hash with 2^19 entries per level (per-level
@@ -10,10 +10,16 @@ ray bounds, 4-step upsampler, fused render forward + Huber loss + backward, V_R 
 closed-form eikonal (w=0.1) + curvature (w=5e-4) gradients, k=5 s-amplification
 (training/trainer.py:188-326).  Inputs are resident in HBM before timing starts.
 
-Prints one JSON line.  ``roofline`` covers the dominant kernel (the fused render
-training kernel), timed live with HIP events on the stream it runs on; ``cpu_baseline``
-times the CPU oracle (an op-for-op restatement of the reference's PyTorch path) on a
-bounded sample of the same workload.
+Forms: BF16 (bf16 hash table, bf16 MLP on matrix cores with FP32 accumulation; the
+headline ``value``) and FP32 (every operand FP32), each with its max-abs RGB error against
+the FP64 oracle on the same z values.
+
+Prints one JSON line.  ``roofline`` covers the dominant kernel of the render launch (by
+time), timed live with HIP events recorded between the kernels on the stream they run on;
+``traffic`` is read from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary.
+``grid_gather`` is the SDF + normal query over a 320^3 lattice.  ``cpu_baseline`` times the
+CPU oracle (an op-for-op restatement of the reference's PyTorch path) on a bounded sample
+of the same workload.
 """
 
 from __future__ import annotations
@@ -220,7 +226,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--form", default="fp32", choices=["fp32", "bf16"])
+    ap.add_argument("--form", default="both", choices=["both", "fp32", "bf16"],
+                    help="both (default): the BF16 form is the headline value, the FP32 form is reported beside it")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-rays", type=int, default=128)
     ap.add_argument("--no-error-check", action="store_true", help="skip the FP64 RGB error check (profiling runs)")
@@ -245,20 +252,28 @@ def main():
         log("copied 3 x 256 MiB")
         return
     sc, batches = make_workload()
-    log(f"GPU: {torch.cuda.get_device_name(0)}; form {a.form}; warmup {a.warmup}; steps {a.steps}")
-    r = run_gpu(sc, batches, a.form, a.steps, a.warmup)
-    log(f"gpu: {r}")
+    forms = ["bf16", "fp32"] if a.form == "both" else [a.form]
     ns = RAYS * SAMPLES
-    kernels = {}
-    for k, ms in r["kernel_ms"].items():
-        bps = KERNEL_BYTES[a.form].get(k)
-        ach = None if bps is None else bps * ns / (ms * 1e-3) / 1e9
-        kernels[k] = {"ms": ms, "bytes_per_sample": bps, "achieved_GBps": ach}
-    dom = max(KERNEL_BYTES[a.form], key=lambda k: r["kernel_ms"][k])
-    nbytes = KERNEL_BYTES[a.form][dom] * ns
-    achieved = kernels[dom]["achieved_GBps"]
-    err = None if a.no_error_check else rgb_error_vs_fp64(sc, batches[0], a.form)
-    log(f"rgb max-abs err vs fp64: {err}")
+    res = {}
+    for form in forms:
+        log(f"GPU: {torch.cuda.get_device_name(0)}; form {form}; warmup {a.warmup}; steps {a.steps}")
+        r = run_gpu(sc, batches, form, a.steps, a.warmup)
+        log(f"gpu {form}: {r}")
+        kernels = {}
+        for k, ms in r["kernel_ms"].items():
+            bps = KERNEL_BYTES[form].get(k)
+            ach = None if bps is None else bps * ns / (ms * 1e-3) / 1e9
+            kernels[k] = {"ms": ms, "bytes_per_sample": bps, "achieved_GBps": ach}
+        err = None if a.no_error_check else rgb_error_vs_fp64(sc, batches[0], form)
+        log(f"rgb max-abs err vs fp64 ({form}): {err}")
+        res[form] = {"rays_per_s": r["rays_per_s"], "samples_per_s": r["rays_per_s"] * SAMPLES,
+                     "ms_per_step": r["ms_per_step"], "rgb_max_abs_err_vs_fp64": err, "kernels": kernels}
+    main_form = forms[0]
+    r = res[main_form]
+    kms = {k: v["ms"] for k, v in r["kernels"].items()}
+    dom = max(KERNEL_BYTES[main_form], key=lambda k: kms[k])
+    nbytes = KERNEL_BYTES[main_form][dom] * ns
+    achieved = r["kernels"][dom]["achieved_GBps"]
     lat = None if a.no_lattice else run_lattice()
     if lat:
         log(f"lattice: {lat}")
@@ -269,17 +284,19 @@ def main():
         "metric": "train rays/s fwd+bwd @256^3 (BF16 vs FP32 at stated RGB err); render p50/p99 ms",
         "value": r["rays_per_s"], "unit": "rays/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32" if a.form == "fp32" else "bf16", "data": "synthetic",
+        "dtype": "f32" if main_form == "fp32" else "bf16", "data": "synthetic",
         "config": {"workload": "fused fwd+bwd train step: 256^3 FP32 SDF grid (seeded multi-primitive), 16x2 hash "
-                               "2^19/level, 8192 rays x (64 + 4x16) samples, hemisphere cameras", "form": a.form,
+                               "2^19/level, 8192 rays x (64 + 4x16) samples, hemisphere cameras; BF16 form = bf16 "
+                               "hash table + bf16 MLP on matrix cores with FP32 accumulation", "form": main_form,
                    "rays_per_step": RAYS, "samples_per_ray": SAMPLES, "grid": 256, "log2_hash": 19,
                    "inv_s_reading": "init_s_val=0.3 -> s = exp(10*0.3)"},
-        "samples_per_s": r["rays_per_s"] * SAMPLES,
-        "rgb_max_abs_err_vs_fp64": err,
+        "samples_per_s": r["samples_per_s"],
+        "rgb_max_abs_err_vs_fp64": r["rgb_max_abs_err_vs_fp64"],
+        "forms": res,
+        "bf16_over_fp32": (res["bf16"]["rays_per_s"] / res["fp32"]["rays_per_s"]) if len(res) == 2 else None,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(dom, a.form),
-                     "kernel": dom, "kernel_ms": r["kernel_ms"][dom], "bytes_per_launch": nbytes},
-        "kernels": kernels,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(dom, main_form),
+                     "kernel": dom, "kernel_ms": kms[dom], "bytes_per_launch": nbytes},
         "grid_gather": lat,
         "cpu_baseline": cpu,
     }

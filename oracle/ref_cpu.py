@@ -243,6 +243,51 @@ def radiance_mlp(hash_feats, view_embed, normals, W0, b0, W1, b1):
     return torch.sigmoid(F.linear(torch.relu(F.linear(x, W0, b0)), W1, b1))
 
 
+def _bf16(t):
+    return t.to(torch.bfloat16).to(t.dtype)
+
+
+class _RadianceMlpBF16(torch.autograd.Function):
+    """The BF16 matrix-core form of the radiance MLP (this build's definition, not the
+    reference's): bf16 operands and FP32 accumulation at the points the device kernel
+    rounds.  Forward: the 35 per-sample inputs (hash features, normal) and W0's columns
+    for them are rounded to bf16; the SH part b0 + W0[:, 32:48] . sh stays FP32; relu(h)
+    and W1 are rounded to bf16.  Backward: the output gradient and dL/dh are rounded to
+    bf16; dW0's SH columns use bf16 SH values; db1 sums the unrounded output gradient."""
+
+    @staticmethod
+    def forward(ctx, feats, sh, nrm, W0, b0, W1, b1):
+        x = _bf16(torch.cat([feats, nrm], -1))
+        W0x = _bf16(torch.cat([W0[:, :32], W0[:, 48:51]], 1))
+        hid = (b0 + sh @ W0[:, 32:48].T) + x @ W0x.T
+        hb = _bf16(torch.relu(hid))
+        W1b = _bf16(W1)
+        rgb = torch.sigmoid(b1 + hb @ W1b.T)
+        ctx.save_for_backward(x, W0x, hid > 0, hb, W1b, sh, rgb)
+        return rgb
+
+    @staticmethod
+    def backward(ctx, g):
+        x, W0x, mask, hb, W1b, sh, rgb = ctx.saved_tensors
+        shp = x.shape[:-1]
+        dout = (g * (1.0 - rgb) * rgb).reshape(-1, 3)
+        doutb = _bf16(dout)
+        dh = _bf16((doutb @ W1b) * mask.reshape(-1, 64))
+        dx = dh @ W0x
+        xf, shf = x.reshape(-1, 35), _bf16(sh.reshape(-1, 16))
+        dW0 = torch.zeros(64, 51, dtype=g.dtype)
+        dW0[:, :32] = dh.T @ xf[:, :32]
+        dW0[:, 32:48] = dh.T @ shf
+        dW0[:, 48:51] = dh.T @ xf[:, 32:35]
+        return (dx[:, :32].reshape(*shp, 32), None, dx[:, 32:35].reshape(*shp, 3), dW0, dh.sum(0), doutb.T @ hb.reshape(-1, 64),
+                dout.sum(0))
+
+
+def radiance_mlp_bf16(hash_feats, view_embed, normals, W0, b0, W1, b1):
+    """The BF16 form of ``radiance_mlp`` (see ``_RadianceMlpBF16``)."""
+    return _RadianceMlpBF16.apply(hash_feats, view_embed, normals, W0, b0, W1, b1)
+
+
 def variance_s(v, s_max=300.0):
     """variance.py:38-41: s = clamp(exp(10 v), 1e-4, s_max)."""
     return torch.clamp(torch.exp(v * 10.0), min=1e-4, max=float(s_max))
@@ -547,9 +592,11 @@ class RefModel:
 
     def __init__(self, grid, embeddings, W0, b0, W1, b1, variance=0.3, s_max=300.0,
                  xyz_min=(-1.0, -1.0, -1.0), xyz_max=(1.0, 1.0, 1.0), hash_base_resolution=16,
-                 hash_max_resolution=2048.0, occupancy_mask=None, occupancy_res=128, dtype=torch.float32):
+                 hash_max_resolution=2048.0, occupancy_mask=None, occupancy_res=128, dtype=torch.float32,
+                 mlp_form="fp32"):
         t = lambda a: torch.as_tensor(a).to(dtype).clone()
         self.dtype = dtype
+        self.mlp_form = mlp_form
         self.grid = t(grid).reshape(1, 1, *grid.shape[-3:]).requires_grad_(True)
         self.embeddings = t(embeddings).requires_grad_(True)
         self.W0, self.b0 = t(W0).requires_grad_(True), t(b0).requires_grad_(True)
@@ -580,7 +627,8 @@ class RefModel:
         grad, normal, corners, weights = query_normals(self.grid, self.box, xyz)
         norm_xyz = (xyz - self.box.xyz_min) / (self.box.xyz_max - self.box.xyz_min + 1e-8)
         feats = hash_encode(torch.clamp(norm_xyz, min=0.0, max=1.0), self.embeddings, self.level_res, active_levels)
-        rgb = radiance_mlp(feats, sh_embed(view_dirs), normal, self.W0, self.b0, self.W1, self.b1)
+        mlp = radiance_mlp_bf16 if self.mlp_form == "bf16" else radiance_mlp
+        rgb = mlp(feats, sh_embed(view_dirs), normal, self.W0, self.b0, self.W1, self.b1)
         return {"sdf": sdf, "grad": grad, "normal": normal, "corner_indices": corners, "weights": weights,
                 "rgb": rgb, "hash_feats": feats}
 

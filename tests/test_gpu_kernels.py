@@ -294,3 +294,71 @@ def test_lattice_query_equals_point_query(Fn):
     P = torch.stack(torch.meshgrid(pts, pts, pts, indexing="ij"), -1).reshape(-1, 3).to(DEV)
     o = Fn.grid_query_raw(grid, BOX, P, True, False, True)
     assert torch.equal(sdf.reshape(-1), o["sdf"]) and torch.equal(nrm.reshape(-1, 3), o["normal"])
+
+
+# ----------------------------------------------------------------------------------------
+# BF16 form: bf16 hash table + matrix-core MLP (bf16 operands, FP32 accumulation)
+# ----------------------------------------------------------------------------------------
+
+def _bf16_render_pair(Fn, name, n_rays=None):
+    """The BF16-form render + backward on the GPU and in the oracle's BF16 emulation,
+    same z values and a fixed random upstream gradient."""
+    g = load_golden(name)
+    sc = _golden_scene(name)
+    emb_b = torch.from_numpy(sc.embeddings).to(torch.bfloat16)
+    m = R.RefModel(sc.grid, emb_b.float().numpy(), **sc.mlp, hash_max_resolution=sc.hash_max_resolution,
+                   mlp_form="bf16")
+    sl = slice(0, n_rays)
+    o, d = torch.from_numpy(g["rays_o"][sl]), torch.from_numpy(g["rays_d"][sl])
+    zs = tuple(torch.from_numpy(g[k][sl]) for k in ("z_vals", "dists", "mid"))
+    R_ = o.shape[0]
+    rng = np.random.default_rng(5)
+    rr = [torch.from_numpy(rng.normal(size=sh).astype(np.float32)) for sh in ((R_, 3), (R_, 1), (R_, 3), (R_, 1))]
+    out = m.render(o, d, None, None, zs=zs)
+    L = (out["rgb"] * rr[0]).sum() + (out["depth"] * rr[1]).sum() + (out["normal"] * rr[2]).sum() + (
+        out["opacity"] * rr[3]).sum()
+    L.backward()
+    gt = torch.zeros(emb_b.shape, device=DEV, dtype=torch.float32)
+    cfg = Fn.RenderConfig(BOX6, m.level_res, mlp_mode=Fn.N.VXN_MLP_BF16_MFMA, table_grad_out=gt)
+    grid = torch.from_numpy(sc.grid).to(DEV).requires_grad_(True)
+    mlp = _mlp_flat(sc).to(DEV).requires_grad_(True)
+    v = torch.tensor(0.3, device=DEV, requires_grad=True)
+    s = torch.clamp(torch.exp(v * 10.0), min=1e-4, max=300.0)
+    outs = Fn.RenderFunction.apply(grid, emb_b.to(DEV), mlp, s, o.to(DEV), d.to(DEV), zs[2].to(DEV), zs[1].to(DEV),
+                                   cfg, False, None)
+    Lg = sum((outs[i] * rr[k].to(DEV)).sum() for k, i in enumerate((0, 1, 2, 3)))
+    Lg.backward()
+    ref_mlp = torch.cat([m.W0.grad.reshape(-1), m.b0.grad, m.W1.grad.reshape(-1), m.b1.grad]).numpy()
+    return {"out": out, "outs": [t.detach().cpu() for t in outs[:4]], "m": m, "grid_grad": grid.grad.cpu().numpy(),
+            "table_grad": gt.cpu().numpy(), "mlp_grad": mlp.grad.cpu().numpy(), "ref_mlp": ref_mlp,
+            "v_grad": v.grad.item(), "sc": sc, "o": o, "d": d, "zs": zs}
+
+
+@pytest.mark.parametrize("name", ["step_c1.npz", "step_c2.npz"])
+def test_bf16_form_matches_emulation(Fn, name):
+    """Forward and backward of the BF16 form vs the oracle's emulation of the same
+    rounding points (FP32 sums differ only in order inside the matrix cores)."""
+    r = _bf16_render_pair(Fn, name)
+    out, m = r["out"], r["m"]
+    for k, i in (("rgb", 0), ("depth", 1), ("normal", 2), ("opacity", 3)):
+        err = float((r["outs"][i] - out[k].detach()).abs().max())
+        assert err < 1e-4, (k, err)
+    e_grid = rel_err(r["grid_grad"], m.grid.grad.numpy()[0, 0])
+    e_tab = rel_err(r["table_grad"], m.embeddings.grad.numpy())
+    e_mlp = rel_err(r["mlp_grad"], r["ref_mlp"])
+    e_v = abs(r["v_grad"] - m.variance.grad.item()) / max(1e-3, abs(m.variance.grad.item()))
+    print(f"bf16 vs emulation: grid {e_grid:.2e} table {e_tab:.2e} mlp {e_mlp:.2e} v {e_v:.2e}")
+    assert e_grid < 1e-4 and e_tab < 1e-4 and e_mlp < 1e-4 and e_v < 1e-4
+
+
+def test_bf16_form_rgb_error_vs_fp64(Fn):
+    """Max-abs RGB of the BF16 form against the FP64 oracle with the FP32 parameters."""
+    r = _bf16_render_pair(Fn, "step_c2.npz")
+    sc = r["sc"]
+    m64 = R.RefModel(sc.grid, sc.embeddings, **sc.mlp, hash_max_resolution=sc.hash_max_resolution,
+                     dtype=torch.float64)
+    with torch.no_grad():
+        ref = m64.render(r["o"].double(), r["d"].double(), None, None, zs=tuple(z.double() for z in r["zs"]))["rgb"]
+    err = float((r["outs"][0].double() - ref).abs().max())
+    print(f"bf16 form rgb max-abs err vs fp64: {err:.3e}")
+    assert err <= 5e-3

@@ -20,6 +20,7 @@
 // Eq. 7 / grid_sample gradients (cuda/gradient_interp.cu:86-152).
 #include "vxn_hash.h"
 #include "vxn_host.h"
+#include "vxn_mfma.h"
 
 namespace vxn {
 
@@ -34,6 +35,17 @@ constexpr int kStage = kRT * kXS + kRT * 3 + 2 * kRT * kCS;
 constexpr int kGeo = 4;  // geometry gradient record: dL/dsdf, dL/dgrad[3]
 
 enum { kFwd = 0, kBwd = 1, kTrain = 2 };
+enum { kMlpF32 = 0, kMlpBf16 = 1 };
+
+// BF16 matrix-core MLP: weight fragments staged in LDS once per work-group (u32x4 units,
+// one per lane of a fragment; fragments that only a few lanes use are stored compactly).
+constexpr int kFragFW = 0;                 // layer 1, W0x [h][k]: 6 fragments (mt, ks)
+constexpr int kFragDX = kFragFW + 6 * 64;  // dX, W0x^T [k<32][h]: 4 fragments (mt, q)
+constexpr int kFragDXN = kFragDX + 4 * 64; // dX, W0x^T [32..34][h]: lanes j < 3 only
+constexpr int kFragL2 = kFragDXN + 4 * 6;  // layer 2, W1 [c][h]: lanes j < 3 only
+constexpr int kFragDH = kFragL2 + 4 * 6;   // dH, W1^T [h][c]: lanes g = 0, as u32x2
+constexpr int kFragWords = kFragDH + 2 * 32 / 2;
+constexpr int kTS = 136;  // LDS row stride (bf16) of the 128-sample transposed tiles
 
 struct RenderDev {
   const float *o, *d, *mid, *dist;
@@ -112,12 +124,59 @@ VXN_DEV void block_sums(float (&v)[N], Red& red, int lane, int wid) {
   __syncthreads();
 }
 
-template <int MODE>
+// W0 restricted to the 35 per-sample inputs: k < 32 hash features, 32..34 the normal
+// (W0 columns 48..50), zero beyond.
+VXN_DEV float w0x(const float* W0, int h, int k) {
+  return k < 32 ? W0[h * 51 + k] : (k < 35 ? W0[h * 51 + 16 + k] : 0.0f);
+}
+
+// Builds the BF16 weight fragments of the matrix-core MLP in LDS (all threads).
+VXN_DEV void build_mlp_fragments(const float* W0, const float* W1, u32x4_t* frag, int t) {
+  for (int idx = t; idx < kFragWords; idx += kRT) {
+    float v[8];
+    if (idx < kFragDX) {  // A = W0x: row h = 32mt + j, K = k = 16ks + 8g + e
+      const int f = idx >> 6, l = idx & 63, mt = f / 3, ks = f % 3;
+      for (int e = 0; e < 8; ++e) v[e] = w0x(W0, 32 * mt + (l & 31), 16 * ks + 8 * (l >> 5) + e);
+    } else if (idx < kFragDXN) {  // A = W0x^T: row k = j, K = hidden kperm(mt, q, g, e)
+      const int f = (idx - kFragDX) >> 6, l = (idx - kFragDX) & 63;
+      for (int e = 0; e < 8; ++e) v[e] = w0x(W0, kperm(f >> 1, f & 1, l >> 5, e), l & 31);
+    } else if (idx < kFragL2) {  // rows k = 32 + j (j < 3) of W0x^T
+      const int f = (idx - kFragDXN) / 6, r = (idx - kFragDXN) % 6;
+      for (int e = 0; e < 8; ++e) v[e] = w0x(W0, kperm(f >> 1, f & 1, r / 3, e), 32 + r % 3);
+    } else if (idx < kFragDH) {  // A = W1: row c = j (j < 3), K = hidden kperm(mt, q, g, e)
+      const int f = (idx - kFragL2) / 6, r = (idx - kFragL2) % 6;
+      for (int e = 0; e < 8; ++e) v[e] = W1[(r % 3) * 64 + kperm(f >> 1, f & 1, r / 3, e)];
+    } else {  // A = W1^T: row h = 32mt + j, K = c = e (< 3), lanes g = 0; two per u32x4
+      const int i2 = 2 * (idx - kFragDH);
+      for (int u = 0; u < 2; ++u) {
+        const int h = 32 * ((i2 + u) >> 5) + ((i2 + u) & 31);
+        v[4 * u] = W1[h];
+        v[4 * u + 1] = W1[64 + h];
+        v[4 * u + 2] = W1[128 + h];
+        v[4 * u + 3] = 0.0f;
+      }
+    }
+    frag[idx] = u32x4_t{pk_bf16(v[0], v[1]), pk_bf16(v[2], v[3]), pk_bf16(v[4], v[5]), pk_bf16(v[6], v[7])};
+  }
+}
+
+template <int MODE, int MLPM>
 __global__ void __launch_bounds__(kRT, 2) render_kernel(RenderDev a) {
+  constexpr bool kBF = MLPM == kMlpBf16;
+  constexpr bool kBFB = kBF && MODE != kFwd;
   __shared__ Red red;
-  __shared__ float preh[64];  // b0 + W0[:, 32:48] . SH(d): the per-ray part of the hidden layer
+  __shared__ __align__(16) float preh[64];  // b0 + W0[:, 32:48] . SH(d): the per-ray part of the hidden layer
   __shared__ int nlive_s[2];  // samples per wavefront with a non-zero output gradient
-  __shared__ float stage[MODE == kFwd ? 1 : kStage];
+  __shared__ float stage[(MODE == kFwd || kBF) ? 1 : kStage];
+  // BF16 form: weight fragments, the ray's SH values, and the transposed bf16 tiles of the
+  // parameter-gradient products: inputs xt[35][s], hidden (then dL/dhidden) hdt[64][s],
+  // output gradients ot[3][s]; s = sample of the ray (0..127).
+  __shared__ u32x4_t bfrag[kBF ? kFragWords : 1];
+  __shared__ float shl[16];
+  __shared__ uint64_t live_s[2];
+  __shared__ __align__(16) uint16_t xt[kBFB ? 35 * kTS : 8];
+  __shared__ __align__(16) uint16_t hdt[kBFB ? 64 * kTS : 8];
+  __shared__ __align__(16) uint16_t ot[kBFB ? 3 * kTS : 8];
   const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
   const float* W0 = a.mlp;
   const float* b0 = a.mlp + kOffB0;
@@ -145,6 +204,15 @@ __global__ void __launch_bounds__(kRT, 2) render_kernel(RenderDev a) {
   }
   float accb1 = 0.0f;
   double ds_acc = 0.0, loss_acc = 0.0;
+  // BF16 form: parameter-gradient accumulators; wave wid owns hidden rows 32*wid + crow.
+  //   gw0[kt]: columns k = 32kt + j of [W0x | b0 | W0 SH columns]; gw1: W1^T columns c = j.
+  const int jq = lane & 31, gq = lane >> 5;
+  f32x16_t gw0[2] = {}, gw1 = {};
+  float accb[3] = {0.0f, 0.0f, 0.0f};
+  if constexpr (kBF) {
+    build_mlp_fragments(W0, W1, bfrag, t);
+    __syncthreads();
+  }
 
   for (int64_t ray = blockIdx.x; ray < a.R; ray += gridDim.x) {
     const bool valid = t < a.S;
@@ -164,6 +232,10 @@ __global__ void __launch_bounds__(kRT, 2) render_kernel(RenderDev a) {
       preh[t] = acc;
     }
     const float shA = sh[grp & 7], shB = sh[(grp & 7) + 8];
+    if (kBF && t == 0) {
+#pragma unroll
+      for (int i = 0; i < 16; ++i) shl[i] = sh[i];
+    }
     // ---------------------------------------------------------------- sample forward
     const float px = ox + dx * mid, py = oy + dy * mid, pz = oz + dz * mid;
     const GsStencil gs = gs_stencil(a.g, px, py, pz);
@@ -183,20 +255,94 @@ __global__ void __launch_bounds__(kRT, 2) render_kernel(RenderDev a) {
     x[33] = nrm[1];
     x[34] = nrm[2];
     __syncthreads();  // preh ready
-    // MLP forward, streaming the hidden units (one W0 row per iteration)
     float o3[3] = {b1[0], b1[1], b1[2]};
+    uint64_t hmask = 0;  // BF16 form: relu'(hidden) per C element, bit 16 (2 mt + nt) + v
+    if constexpr (!kBF) {
+      // MLP forward, streaming the hidden units (one W0 row per iteration)
 #pragma unroll 2
-    for (int j = 0; j < 64; ++j) {
-      const float* wr = W0 + j * 51;
-      float acc = preh[j];
+      for (int j = 0; j < 64; ++j) {
+        const float* wr = W0 + j * 51;
+        float acc = preh[j];
 #pragma unroll
-      for (int i = 0; i < 32; ++i) acc = fmaf(wr[i], x[i], acc);
+        for (int i = 0; i < 32; ++i) acc = fmaf(wr[i], x[i], acc);
 #pragma unroll
-      for (int k = 0; k < 3; ++k) acc = fmaf(wr[48 + k], x[32 + k], acc);
-      const float hj = fmaxf(acc, 0.0f);
-      o3[0] = fmaf(W1[j], hj, o3[0]);
-      o3[1] = fmaf(W1[64 + j], hj, o3[1]);
-      o3[2] = fmaf(W1[128 + j], hj, o3[2]);
+        for (int k = 0; k < 3; ++k) acc = fmaf(wr[48 + k], x[32 + k], acc);
+        const float hj = fmaxf(acc, 0.0f);
+        o3[0] = fmaf(W1[j], hj, o3[0]);
+        o3[1] = fmaf(W1[64 + j], hj, o3[1]);
+        o3[2] = fmaf(W1[128 + j], hj, o3[2]);
+      }
+    } else {
+      // Matrix cores, transposed: hidden^T [64 h x 64 samples] = W0x [64 x 48] . X^T,
+      // samples of wave half nt = 0/1 in the N dimension (lane j <-> sample 32nt + j).
+      uint32_t xp[24];
+#pragma unroll
+      for (int r = 0; r < 16; ++r) xp[r] = pk_bf16(x[2 * r], x[2 * r + 1]);
+      xp[16] = pk_bf16(x[32], x[33]);
+      xp[17] = pk_bf16(x[34], 0.0f);
+#pragma unroll
+      for (int r = 18; r < 24; ++r) xp[r] = 0u;
+      if constexpr (kBFB) {
+        // stage the inputs for the parameter gradients
+#pragma unroll
+        for (int k = 0; k < 35; ++k) xt[k * kTS + t] = (uint16_t)(xp[k >> 1] >> (16 * (k & 1)));
+      }
+      // one 32-sample tile nt at a time: hidden^T tile pair -> relu -> layer 2
+      // out^T [3 (of 32) x 32 samples] = W1 . relu(hidden)^T
+      float on[2][3];
+#pragma unroll
+      for (int nt = 0; nt < 2; ++nt) {
+        u32x4_t hp[2][2];  // relu(hidden) as bf16, [mt][q]: the B operand of layer 2
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt) {
+          f32x16_t c;
+#pragma unroll
+          for (int q4 = 0; q4 < 4; ++q4) {
+            const float4 p = *(const float4*)(preh + 32 * mt + 8 * q4 + 4 * gq);
+            c[4 * q4] = p.x;
+            c[4 * q4 + 1] = p.y;
+            c[4 * q4 + 2] = p.z;
+            c[4 * q4 + 3] = p.w;
+          }
+#pragma unroll
+          for (int ks = 0; ks < 3; ++ks) {
+            u32x4_t bx;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) bx[r] = half_swap(xp[8 * ks + r], xp[8 * ks + 4 + r])[nt];
+            c = mfma_bf16(bfrag[kFragFW + (3 * mt + ks) * 64 + lane], bx, c);
+          }
+#pragma unroll
+          for (int q = 0; q < 2; ++q)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const int v = 8 * q + 2 * r;
+              const float h0 = c[v], h1 = c[v + 1];
+              hmask |= (uint64_t)(h0 > 0.0f) << (16 * (2 * mt + nt) + v);
+              hmask |= (uint64_t)(h1 > 0.0f) << (16 * (2 * mt + nt) + v + 1);
+              const uint32_t pk = pk_bf16(fmaxf(h0, 0.0f), fmaxf(h1, 0.0f));
+              hp[mt][q][r] = pk;
+              if constexpr (kBFB) {
+                uint16_t* col = hdt + 64 * wid + 32 * nt + jq;
+                col[(32 * mt + crow(v, gq)) * kTS] = (uint16_t)pk;
+                col[(32 * mt + crow(v + 1, gq)) * kTS] = (uint16_t)(pk >> 16);
+              }
+            }
+        }
+        f32x16_t oc = {};
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+          for (int q = 0; q < 2; ++q) {
+            const u32x4_t A = jq < 3 ? bfrag[kFragL2 + (2 * mt + q) * 6 + 3 * gq + jq] : u32x4_t{0u, 0u, 0u, 0u};
+            oc = mfma_bf16(A, hp[mt][q], oc);
+          }
+        on[nt][0] = oc[0];
+        on[nt][1] = oc[1];
+        on[nt][2] = oc[2];
+      }
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        o3[c] += __uint_as_float(half_swap(__float_as_uint(on[0][c]), __float_as_uint(on[1][c]))[0]);
     }
     float rgb[3];
 #pragma unroll
@@ -355,6 +501,7 @@ __global__ void __launch_bounds__(kRT, 2) render_kernel(RenderDev a) {
     float dout[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) dout[c] = valid ? (w * dr[c]) * (1.0f - rgb[c]) * rgb[c] : 0.0f;
+    if constexpr (!kBF) {
     // Samples with a zero output gradient add nothing to the parameter gradients: stage
     // only the live ones, compacted to rows [0, nlive).
     const bool mlive = valid && (dout[0] != 0.0f || dout[1] != 0.0f || dout[2] != 0.0f);
@@ -425,10 +572,106 @@ __global__ void __launch_bounds__(kRT, 2) render_kernel(RenderDev a) {
       for (int s2 = 0; s2 < nlive; ++s2) accb1 += Os[s2 * 3 + t];
 #pragma unroll
     for (int k = 0; k < 3; ++k) dn[k] += dxa[32 + k];
-    // ---------------------------------------------------------------- gradient records
     if (valid) {
 #pragma unroll
       for (int i = 0; i < 32; ++i) a.gfeat[i * NS + si] = dxa[i];
+    }
+    } else {
+      // Parameter gradients are products over the ray's 128 samples (K); 16-sample chunks
+      // without an output gradient are skipped.  Wave wid produces hidden rows 32 wid + crow.
+      const bool mlive = valid && (dout[0] != 0.0f || dout[1] != 0.0f || dout[2] != 0.0f);
+      const uint64_t bal = __ballot(mlive);
+      if (lane == 0) live_s[wid] = bal;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        accb[c] += dout[c];
+        ot[c * kTS + t] = bf16_bits(dout[c]);
+      }
+      __syncthreads();
+      const uint64_t live[2] = {live_s[0], live_s[1]};
+      const uint16_t* hrow = hdt + (32 * wid + jq) * kTS + 8 * gq;
+      // dW1^T [32 h x 3 (of 32)] += relu(hidden) . dout
+#pragma unroll 2
+      for (int ks = 0; ks < 8; ++ks) {
+        if (((live[ks >> 2] >> (16 * (ks & 3))) & 0xFFFFull) == 0) continue;
+        const u32x4_t A = *(const u32x4_t*)(hrow + 16 * ks);
+        const u32x4_t B = jq < 3 ? *(const u32x4_t*)(ot + jq * kTS + 16 * ks + 8 * gq) : u32x4_t{0u, 0u, 0u, 0u};
+        gw1 = mfma_bf16(A, B, gw1);
+      }
+      __syncthreads();  // the hidden tile is consumed; it now receives dL/dhidden
+      // dL/dhidden^T [64 x 64 samples] = relu' * (W1^T [64 x 3 (of 16)] . dout^T)
+      const u32x2_t s0 = half_swap(pk_bf16(dout[0], dout[1]), 0u);
+      const u32x2_t s1 = half_swap(pk_bf16(dout[2], 0.0f), 0u);
+      float dxn[2][3];
+#pragma unroll
+      for (int nt = 0; nt < 2; ++nt) {
+        u32x4_t dp[2][2];  // dL/dhidden of sample tile nt as bf16, [mt][q]
+        uint16_t* col = hdt + 64 * wid + 32 * nt + jq;
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt) {
+          u32x4_t A = {0u, 0u, 0u, 0u};
+          if (gq == 0) {
+            const u32x2_t w = ((const u32x2_t*)(bfrag + kFragDH))[32 * mt + jq];
+            A = u32x4_t{w[0], w[1], 0u, 0u};
+          }
+          const f32x16_t c = mfma_bf16(A, u32x4_t{s0[nt], s1[nt], 0u, 0u}, f32x16_t{});
+#pragma unroll
+          for (int q = 0; q < 2; ++q)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const int v = 8 * q + 2 * r, b = 16 * (2 * mt + nt) + v;
+              const uint32_t pk =
+                  pk_bf16(((hmask >> b) & 1) ? c[v] : 0.0f, ((hmask >> (b + 1)) & 1) ? c[v + 1] : 0.0f);
+              dp[mt][q][r] = pk;
+              col[(32 * mt + crow(v, gq)) * kTS] = (uint16_t)pk;
+              col[(32 * mt + crow(v + 1, gq)) * kTS] = (uint16_t)(pk >> 16);
+            }
+        }
+        // dL/dX^T [35 (of 64) x 32 samples] = W0x^T . dL/dhidden^T: the hash-feature rows
+        // go straight to the records (row k, 32 consecutive samples per half-wave store),
+        // the normal rows back to the sample's own lane
+        f32x16_t c = {}, cn = {};
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+          for (int q = 0; q < 2; ++q) {
+            c = mfma_bf16(bfrag[kFragDX + (2 * mt + q) * 64 + lane], dp[mt][q], c);
+            const u32x4_t A =
+                jq < 3 ? bfrag[kFragDXN + (2 * mt + q) * 6 + 3 * gq + jq] : u32x4_t{0u, 0u, 0u, 0u};
+            cn = mfma_bf16(A, dp[mt][q], cn);
+          }
+        const int sl = 64 * wid + 32 * nt + jq;
+        if (sl < a.S) {
+          float* rec = a.gfeat + ray * a.S + sl;
+#pragma unroll
+          for (int v = 0; v < 16; ++v) rec[crow(v, gq) * NS] = c[v];
+        }
+        dxn[nt][0] = cn[0];
+        dxn[nt][1] = cn[1];
+        dxn[nt][2] = cn[2];
+      }
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+        dn[k] += __uint_as_float(half_swap(__float_as_uint(dxn[0][k]), __float_as_uint(dxn[1][k]))[0]);
+      __syncthreads();  // dL/dhidden staged
+      // [dW0x | db0 | dW0 SH] [32 h x 64] += dL/dhidden . [X | 1 | SH]; columns 32 + j of the
+      // second tile: j < 3 the normal, 3 the bias, 4..19 the SH inputs, the rest zero
+      uint32_t bc = 0u;
+      if (jq == 3) bc = pk_bf16(1.0f, 1.0f);
+      else if (jq >= 4 && jq < 20) bc = pk_bf16(shl[jq - 4], shl[jq - 4]);
+      const u32x4_t Bc = {bc, bc, bc, bc};
+#pragma unroll 2
+      for (int ks = 0; ks < 8; ++ks) {
+        if (((live[ks >> 2] >> (16 * (ks & 3))) & 0xFFFFull) == 0) continue;
+        const u32x4_t A = *(const u32x4_t*)(hrow + 16 * ks);
+        const u32x4_t B0 = *(const u32x4_t*)(xt + jq * kTS + 16 * ks + 8 * gq);
+        const u32x4_t B1 = jq < 3 ? *(const u32x4_t*)(xt + (32 + jq) * kTS + 16 * ks + 8 * gq) : Bc;
+        gw0[0] = mfma_bf16(A, B0, gw0[0]);
+        gw0[1] = mfma_bf16(A, B1, gw0[1]);
+      }
+    }
+    // ---------------------------------------------------------------- gradient records
+    if (valid) {
       const float rr = gr + 1e-8f;
       const float dotv = (dn[0] * gv[0] + dn[1] * gv[1]) + dn[2] * gv[2];
       a.ggeo[si] = dsdf;
@@ -442,6 +685,24 @@ __global__ void __launch_bounds__(kRT, 2) render_kernel(RenderDev a) {
   if (MODE == kFwd) return;
   // ---------------------------------------------------------------- write partials
   float* P = a.part_mlp + (int64_t)blockIdx.x * kNP;
+  if constexpr (kBF) {
+#pragma unroll
+    for (int kt = 0; kt < 2; ++kt)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const int h = 32 * wid + crow(v, gq), k = 32 * kt + jq;
+        // k: 0..31 hash features, 32..34 normal (W0 columns 48..50), 35 bias, 36..51 SH (columns 32..47)
+        if (k < 32) P[h * 51 + k] = gw0[kt][v];
+        else if (k < 35) P[h * 51 + 16 + k] = gw0[kt][v];
+        else if (k == 35) P[kOffB0 + h] = gw0[kt][v];
+        else if (k < 52) P[h * 51 + k - 4] = gw0[kt][v];
+      }
+#pragma unroll
+    for (int v = 0; v < 16; ++v)
+      if (jq < 3) P[kOffW1 + jq * 64 + 32 * wid + crow(v, gq)] = gw1[v];
+    block_sums<3>(accb, red, lane, wid);
+    if (t < 3) P[kOffB1 + t] = accb[t];
+  } else {
 #pragma unroll
   for (int cc = 0; cc < 4; ++cc) {
     const int j = 16 * cc + jl;
@@ -456,6 +717,7 @@ __global__ void __launch_bounds__(kRT, 2) render_kernel(RenderDev a) {
     if (t < 48) P[kOffW1 + (t >> 4) * 64 + j] = accW1[cc];
   }
   if (t < 3) P[kOffB1 + t] = accb1;
+  }
   double dsw = wave_sum_d(ds_acc);
   if (lane == 0) red.d[wid] = dsw;
   __syncthreads();
@@ -735,7 +997,7 @@ static int build_render_dev(const vxn_render_args& A, RenderDev& a, int mode) {
               kRT, A.n_samples);
   VXN_REQUIRE(A.n_levels == 16 && A.n_features == 2, VXN_ERR_UNSUPPORTED,
               "the fused render implements the 16-level x 2-feature hash feeding the 51-input MLP");
-  VXN_REQUIRE(A.mlp_mode == VXN_MLP_FP32, VXN_ERR_UNSUPPORTED, "mlp_mode %d is not available in this build",
+  VXN_REQUIRE(A.mlp_mode == VXN_MLP_FP32 || A.mlp_mode == VXN_MLP_BF16_MFMA, VXN_ERR_VALUE, "unknown mlp_mode %d",
               A.mlp_mode);
   VXN_REQUIRE(A.rays_o && A.rays_d && A.mid_z && A.dists && A.table && A.mlp, VXN_ERR_NULL, "null input");
   VXN_REQUIRE(A.n_rays * (int64_t)A.n_samples < ((int64_t)1 << 40), VXN_ERR_SHAPE, "too many samples");
@@ -816,7 +1078,10 @@ static int launch_render(const vxn_render_args* p, void* stream) {
   hipStream_t s = (hipStream_t)stream;
   hipEvent_t const* ev = (hipEvent_t const*)p->kernel_events;
   if (ev) VXN_HIP(hipEventRecord(ev[0], s));
-  hipLaunchKernelGGL(render_kernel<MODE>, dim3(nb), dim3(kRT), 0, s, a);
+  if (p->mlp_mode == VXN_MLP_BF16_MFMA)
+    hipLaunchKernelGGL((render_kernel<MODE, kMlpBf16>), dim3(nb), dim3(kRT), 0, s, a);
+  else
+    hipLaunchKernelGGL((render_kernel<MODE, kMlpF32>), dim3(nb), dim3(kRT), 0, s, a);
   VXN_LAUNCH_CHECK();
   if (ev) VXN_HIP(hipEventRecord(ev[1], s));
   if (MODE != kFwd) {

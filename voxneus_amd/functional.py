@@ -266,7 +266,11 @@ class RenderConfig:
     """Static settings of one fused render call."""
 
     def __init__(self, box, level_res, level_w=None, sphere_radius=1.0, cos_anneal_ratio=0.0,
-                 use_iter_cos=False, mlp_mode=N.VXN_MLP_FP32):
+                 use_iter_cos=False, mlp_mode=N.VXN_MLP_FP32, table_grad_out=None):
+        # table_grad_out: optional FP32 [L,T,F] tensor that the backward accumulates the
+        # hash-table gradient into (for bf16 tables with an FP32 master copy); the table
+        # input then receives no gradient through autograd
+        self.table_grad_out = table_grad_out
         self.box = tuple(float(v) for v in box)
         self.level_res = list(level_res)
         self.level_w = None if level_w is None else list(level_w)
@@ -354,15 +358,19 @@ class RenderFunction(torch.autograd.Function):
         a.d_rgb, a.d_depth, a.d_normal, a.d_opacity, a.d_weights = (N.dptr(d_rgb), N.dptr(d_depth), N.dptr(d_normal),
                                                                      N.dptr(d_opacity), N.dptr(d_weights))
         gg = torch.zeros_like(g3)
-        gt = torch.zeros(tb.shape, device=dev, dtype=torch.float32)
+        gt = ctx.cfg.table_grad_out
+        if gt is None:
+            gt = torch.zeros(tb.shape, device=dev, dtype=torch.float32)
+        elif gt.shape != tb.shape or gt.dtype != torch.float32 or not gt.is_contiguous():
+            raise ValueError("table_grad_out must be a contiguous float32 tensor of the table's shape")
         gm = torch.empty(N.MLP_PARAMS, device=dev, dtype=torch.float32)
         gs = torch.empty(1, device=dev, dtype=torch.float32)
         a.grad_grid, a.grad_table, a.grad_mlp, a.grad_s = N.dptr(gg), N.dptr(gt), N.dptr(gm), N.dptr(gs)
         ws = render_workspace(a, dev)
         a.workspace, a.workspace_bytes = ws.data_ptr(), ws.numel()
         N.check(N.lib().vxn_render_backward(ctypes.byref(a), _stream(md)), "render_backward")
-        return (gg.reshape(ctx.grid_shape), gt.to(tb.dtype), gm, gs.reshape(()), None, None, None, None, None, None,
-                None)
+        gtab = None if ctx.cfg.table_grad_out is not None else gt.to(tb.dtype)
+        return (gg.reshape(ctx.grid_shape), gtab, gm, gs.reshape(()), None, None, None, None, None, None, None)
 
 
 # ----------------------------------------------------------------------------------------
