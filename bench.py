@@ -149,6 +149,71 @@ def run_lattice(reps=10):
             "compulsory_GBps": (grid.numel() * 4 + pts * 16) / (ms * 1e-3) / 1e9}
 
 
+SERVE_RENDER_SIZES = ((64, 64), (200, 150), (400, 300), (800, 600))
+
+
+def serving_requests(n, n_scenes, lam, seed=11, render_frac=2.0 / 3.0):
+    """Poisson arrivals (rate ``lam`` per second) of renders of the four image sizes from
+    random hemisphere poses (f = 1.5 W) and SDF/normal queries of 10^4..10^6 uniform points
+    (log-uniform), each on a random resident scene."""
+    import torch
+
+    from voxneus_amd import scenes as S
+    from voxneus_amd.serving import QueryRequest, RenderRequest
+
+    rng = np.random.default_rng(seed)
+    cams = S.hemisphere_cameras(49)
+    t = 0.0
+    reqs = []
+    for i in range(n):
+        t += rng.exponential(1.0 / lam)
+        sc = int(rng.integers(n_scenes))
+        if rng.random() < render_frac:
+            W, H = SERVE_RENDER_SIZES[int(rng.integers(len(SERVE_RENDER_SIZES)))]
+            reqs.append(RenderRequest(sc, cams[int(rng.integers(49))], W, H, 1.5 * W, 1.5 * W, W / 2, H / 2, t))
+        else:
+            npts = int(round(10 ** rng.uniform(4.0, 6.0)))
+            g = torch.Generator(device="cuda:0")
+            g.manual_seed(seed * 100003 + i)
+            pts = torch.rand((npts, 3), generator=g, device="cuda:0") * 2.0 - 1.0
+            reqs.append(QueryRequest(sc, pts, t))
+    return reqs
+
+
+def run_serving(form, n_requests=160, utilisation=0.7):
+    """Config-5 serving loop: 24 resident scenes, Poisson arrivals at a fixed offered load
+    (``utilisation`` of the measured service rate), p50/p99 latency per request kind."""
+    import torch
+
+    from voxneus_amd import _native as N
+    from voxneus_amd.serving import Server, latency_stats, make_resident_scene
+
+    dev = torch.device("cuda:0")
+    grids, logs = (96, 128, 192, 256), (17, 18, 19)
+    tdt = torch.bfloat16 if form == "bf16" else torch.float32
+    scenes = [make_resident_scene(grids[i % 4], logs[i % 3], 1000 + i, dev, table_dtype=tdt) for i in range(24)]
+    mode = N.VXN_MLP_BF16_MFMA if form == "bf16" else N.VXN_MLP_FP32
+    srv = Server(scenes, mlp_mode=mode, ray_capacity=1 << 16, point_capacity=1 << 20, ray_arena=1 << 23,
+                 point_arena=1 << 24)
+    # service rate: the same request mix, all admitted at once
+    cal = serving_requests(24, len(scenes), 1.0, seed=7)
+    srv.serve(cal[:4], realtime=False)  # warm-up
+    t0 = time.perf_counter()
+    srv.serve(cal, realtime=False)
+    busy = time.perf_counter() - t0
+    lam = utilisation * len(cal) / busy
+    reqs = serving_requests(n_requests, len(scenes), lam)
+    t0 = time.perf_counter()
+    res = srv.serve(reqs, realtime=True)
+    wall = time.perf_counter() - t0
+    out = {"scenes": len(scenes), "resident_MB": sum(sc.nbytes for sc in scenes) / 1e6, "requests": n_requests,
+           "offered_req_per_s": lam, "utilisation_target": utilisation, "wall_s": wall,
+           "render": latency_stats(res, "render"), "query": latency_stats(res, "query")}
+    del srv, scenes
+    torch.cuda.empty_cache()
+    return out
+
+
 def profiled_traffic(kernel, form):
     """HBM bytes per dispatch of ``kernel`` from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE summary (already scaled by the copy calibration), or None."""
@@ -232,6 +297,7 @@ def main():
     ap.add_argument("--cpu-rays", type=int, default=128)
     ap.add_argument("--no-error-check", action="store_true", help="skip the FP64 RGB error check (profiling runs)")
     ap.add_argument("--no-lattice", action="store_true", help="skip the 320^3 lattice-query measurement")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop latency measurement")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB device copy (3 dispatches): the known-size kernel that scales "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes")
@@ -277,6 +343,12 @@ def main():
     lat = None if a.no_lattice else run_lattice()
     if lat:
         log(f"lattice: {lat}")
+    serving = None
+    if not a.no_serving:
+        serving = {}
+        for form in forms:
+            serving[form] = run_serving(form)
+            log(f"serving {form}: {serving[form]}")
     cpu = None if a.no_cpu_baseline else cpu_baseline(sc, batches[0], a.cpu_rays)
     if cpu:
         log(f"cpu baseline: {cpu}")
@@ -298,6 +370,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(dom, main_form),
                      "kernel": dom, "kernel_ms": kms[dom], "bytes_per_launch": nbytes},
         "grid_gather": lat,
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

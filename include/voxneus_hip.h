@@ -234,6 +234,80 @@ int vxn_curvature_grad(const float* grid, int X, int Y, int Z, const int64_t* ac
  * (training/scheduler.py:14-33) without its host read-back.                           */
 int vxn_s_amplify(float* grad_v, float k, int32_t* amplified, void* stream);
 
+/* ---- Serving: many resident scenes, packed request batches -------------------------
+ * The reference renders one scene in 2048-ray chunks (scripts/render.py:35-54:
+ * near_far_from_sphere + VoxNeuS.render, no_grad, det sampler) and queries SDF and
+ * normals in 65536-point batches (geometry/mesh.py:80-114); it has no request loop.
+ * Here a batch packs rays (or points) of many requests on many resident scenes: a device
+ * table holds one packed scene record per scene (vxn_serve_pack_scene), a per-ray scene
+ * index selects it, and the batch size is read from device memory so that one captured
+ * HIP graph serves batches of any size up to the capacity.  Each ray's result does not
+ * depend on the other rays of the batch.                                              */
+typedef struct vxn_scene_desc {
+  const float* grid; /* [X,Y,Z] float32 */
+  int32_t X, Y, Z;
+  float box[6];               /* xyz_min, xyz_max */
+  const uint8_t* occ_mask;    /* nullable [r,r,r] bool occupancy (sampler modulation) */
+  int32_t occ_res;
+  float occ_box[6];
+  const void* table;          /* [L,T,2] float32 or bfloat16 */
+  int32_t table_dtype;
+  int32_t n_levels, log2_T, n_features;
+  int32_t level_res[VXN_MAX_LEVELS];
+  float level_w[VXN_MAX_LEVELS];
+  const float* mlp;           /* [VXN_MLP_PARAMS] float32 */
+  float s;                    /* NeuS sharpness */
+  float sphere_radius;        /* compositing inside-sphere radius (1.0 in the reference) */
+} vxn_scene_desc;
+
+/* Size of one packed scene record of the device table. */
+int vxn_serve_scene_bytes(int64_t* bytes);
+/* Validates a scene and writes its packed record into host memory `out` (the caller
+ * copies records to the device table, record i at byte offset i * vxn_serve_scene_bytes). */
+int vxn_serve_pack_scene(const vxn_scene_desc* scene, void* out, int64_t out_bytes);
+
+typedef struct vxn_serve_render_args {
+  const void* scenes;        /* device table of packed scene records */
+  int32_t n_scenes;
+  const int32_t* ray_scene;  /* [capacity] scene of each ray (outside [0, n_scenes): zeros) */
+  const int32_t* n_rays;     /* device scalar: rays in this batch (clamped to capacity) */
+  int64_t capacity;
+  const float* rays_o;       /* [capacity,3] */
+  const float* rays_d;       /* [capacity,3] */
+  int32_t n_samples, n_importance, up_sample_steps; /* 64, 64, 4 in the reference */
+  const float* t_lin;        /* [n_samples] linspace(0, 1) */
+  const float* u_det;        /* [n_importance / steps] the deterministic sample_pdf u */
+  float bound_radius;        /* near/far from this sphere (1.0 in the reference) */
+  int32_t mlp_mode;
+  float cos_anneal_ratio;
+  int32_t use_iter_cos;
+  float* rgb;                /* [capacity,3] */
+  float* depth;              /* nullable [capacity] */
+  float* opacity;            /* nullable [capacity] */
+  float* normal;             /* nullable [capacity,3] */
+  void* workspace;           /* z, dists, mid of the batch */
+  int64_t workspace_bytes;
+} vxn_serve_render_args;
+
+int vxn_serve_render_workspace_bytes(const vxn_serve_render_args* a, int64_t* bytes);
+/* Sampler + fused render forward over a packed batch (two kernels, no allocation, no
+ * synchronisation: capturable in a HIP graph). */
+int vxn_serve_render(const vxn_serve_render_args* a, void* stream);
+/* SDF and unit normal at a packed batch of points: pt_scene [capacity], n_points device
+ * scalar, xyz [capacity,3] -> sdf [capacity], normal [capacity,3] (nullable). */
+int vxn_serve_query(const void* scenes, int32_t n_scenes, const int32_t* pt_scene, const int32_t* n_points,
+                    int64_t capacity, const float* xyz, float* sdf, float* normal, void* stream);
+
+/* Moves rows between request ranges of a device arena and the batch buffers of a step.
+ * segs (device, [max_segs][3] int32): arena row offset, row count, scene; n_segs (device
+ * scalar) segments are used, in order, filling batch rows 0.. up to the capacity.
+ * to_batch = 1: batch0/1[i] = arena0/1[row(i)] (w0/w1 floats per row), batch_scene[i] =
+ * the segment's scene and *batch_n = the row total; to_batch = 0: arena0/1[row(i)] =
+ * batch0/1[i].  Rows outside [0, arena_rows) read as zero (scene -1) and are not written. */
+int vxn_serve_move(const int32_t* segs, const int32_t* n_segs, int32_t max_segs, int64_t capacity,
+                   int64_t arena_rows, int32_t to_batch, float* arena0, float* batch0, int32_t w0, float* arena1,
+                   float* batch1, int32_t w1, int32_t* batch_scene, int32_t* batch_n, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -31,7 +31,8 @@ def test_library_exports_every_declared_symbol():
 
 def test_struct_layouts_match_header_field_order():
     text = open(os.path.join(REPO, "include", "voxneus_hip.h")).read()
-    for struct, cls in (("vxn_sampler_args", N.SamplerArgs), ("vxn_render_args", N.RenderArgs)):
+    for struct, cls in (("vxn_sampler_args", N.SamplerArgs), ("vxn_render_args", N.RenderArgs),
+                        ("vxn_scene_desc", N.SceneDesc), ("vxn_serve_render_args", N.ServeRenderArgs)):
         body = re.search(r"typedef struct %s \{(.*?)\} %s;" % (struct, struct), text, re.S).group(1)
         body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
         names = []
@@ -80,3 +81,32 @@ def test_host_validation_errors():
     assert L.vxn_reg_eik_curv(ctypes.c_void_p(16), 8, 8, 8, N.f3((0.1, 0.0, 0.1)), ctypes.c_void_p(16),
                               ctypes.c_void_p(16), 0.1, 0.1, ctypes.c_void_p(16), ctypes.c_void_p(16), None, 0,
                               None) == -3
+
+
+def test_serving_validation_errors():
+    L = N.lib()
+    nb = ctypes.c_int64(0)
+    assert L.vxn_serve_scene_bytes(ctypes.byref(nb)) == 0 and nb.value > 0
+    d = N.SceneDesc()
+    buf = (ctypes.c_uint8 * nb.value)()
+    assert L.vxn_serve_pack_scene(ctypes.byref(d), buf, nb.value) == -1  # null grid
+    d.grid, d.X, d.Y, d.Z = 16, 8, 8, 8
+    d.box = (ctypes.c_float * 6)(-1, -1, -1, 1, 1, 1)
+    d.table, d.mlp, d.n_levels, d.n_features, d.log2_T = 16, 16, 8, 2, 15
+    assert L.vxn_serve_pack_scene(ctypes.byref(d), buf, nb.value) == -4  # 8 levels: unsupported
+    d.n_levels = 16
+    for l in range(16):
+        d.level_res[l] = 16 + l
+    assert L.vxn_serve_pack_scene(ctypes.byref(d), buf, nb.value - 1) == -5  # record too small
+    assert L.vxn_serve_pack_scene(ctypes.byref(d), buf, nb.value) == 0
+    a = N.ServeRenderArgs()
+    a.capacity, a.n_samples, a.n_importance, a.up_sample_steps = 4, 64, 64, 4
+    assert L.vxn_serve_render_workspace_bytes(ctypes.byref(a), ctypes.byref(nb)) == 0
+    assert nb.value >= 3 * 4 * 128 * 4
+    assert L.vxn_serve_render(ctypes.byref(a), None) == -1  # null pointers
+    a.n_samples = 100
+    assert L.vxn_serve_render(ctypes.byref(a), None) == -4  # 164 samples per ray
+    assert L.vxn_serve_move(None, None, 0, 4, 4, 1, None, None, 3, None, None, 0, None, None, None) == -3
+    assert L.vxn_serve_move(None, None, 8, 4, 4, 1, None, None, 3, None, None, 0, None, None, None) == -1
+    assert L.vxn_serve_query(None, 1, None, None, 4, None, None, None, None) == -1
+

@@ -1,0 +1,90 @@
+"""The serving loop on the GPU: every request's result equals the standalone
+single-scene call on the same rays / points, and is the same alone and in a mix
+(bit-exact: per-ray work never depends on the rest of a batch)."""
+
+import numpy as np
+import pytest
+import torch
+
+from voxneus_amd import _native as N
+from voxneus_amd import functional as Fn
+from voxneus_amd import scenes as S
+from voxneus_amd.serving import QueryRequest, RenderRequest, Server, make_resident_scene, pinhole_rays
+
+pytestmark = pytest.mark.gpu
+
+DEV = torch.device("cuda")
+
+
+@pytest.fixture(scope="module")
+def scenes():
+    spec = [(64, 15, 1, torch.float32), (96, 16, 2, torch.bfloat16), (48, 15, 3, torch.float32)]
+    return [make_resident_scene(r, t, seed, DEV, table_dtype=dt) for r, t, seed, dt in spec]
+
+
+def _requests(n, n_scenes, seed, sizes=((24, 16), (40, 30), (7, 5)), arrival_gap=0.0):
+    rng = np.random.default_rng(seed)
+    cams = S.hemisphere_cameras(49)
+    reqs = []
+    for i in range(n):
+        sc = int(rng.integers(n_scenes))
+        if rng.random() < 0.7:
+            W, H = sizes[int(rng.integers(len(sizes)))]
+            c2w = cams[int(rng.integers(49))]
+            reqs.append(RenderRequest(sc, c2w, W, H, 1.5 * W, 1.5 * W, W / 2, H / 2, arrival=i * arrival_gap))
+        else:
+            g = torch.Generator(device=DEV)
+            g.manual_seed(seed * 1000 + i)
+            pts = torch.rand((int(rng.integers(100, 3000)), 3), generator=g, device=DEV) * 2.2 - 1.1
+            reqs.append(QueryRequest(sc, pts, arrival=i * arrival_gap))
+    return reqs
+
+
+def _standalone(sc, req, mlp_mode):
+    if isinstance(req, QueryRequest):
+        out = Fn.grid_query_raw(sc.grid, sc.box, req.points, want_sdf=True, want_grad=False, want_normal=True)
+        return {"sdf": out["sdf"].reshape(-1, 1), "normal": out["normal"]}
+    o, d = pinhole_rays(req, DEV)
+    z, dists, mid = Fn.upsample(o, d, None, None, sc.grid, sc.box, 64, 64, 4, False, sc.occ_mask, sc.occ_box,
+                                bound_radius=1.0)
+    cfg = Fn.RenderConfig(sc.box, sc.level_res, mlp_mode=mlp_mode)
+    out = Fn.RenderFunction.apply(sc.grid, sc.table, sc.mlp, torch.tensor(sc.s, device=DEV), o, d, mid, dists, cfg,
+                                  False, None)
+    return {"rgb": out[0], "depth": out[1]}
+
+
+@pytest.mark.parametrize("mode", [N.VXN_MLP_FP32, N.VXN_MLP_BF16_MFMA])
+def test_serving_matches_standalone_calls(scenes, mode):
+    reqs = _requests(14, len(scenes), seed=5)
+    srv = Server(scenes, mlp_mode=mode, ray_capacity=1500, point_capacity=2048, ray_arena=20000, point_arena=16000)
+    res = srv.serve(reqs, realtime=False)
+    for r, q in zip(res, reqs):
+        ref = _standalone(scenes[q.scene], q, mode)
+        for k, v in ref.items():
+            assert torch.equal(r.outputs[k], v.reshape(r.outputs[k].shape)), (r.index, r.kind, k)
+
+
+def test_serving_alone_equals_mixed_and_graph_equals_eager(scenes):
+    reqs = _requests(20, len(scenes), seed=7)
+    mixed = Server(scenes, ray_capacity=777, point_capacity=999, ray_arena=9000, point_arena=9000).serve(
+        reqs, realtime=False)
+    eager = Server(scenes, ray_capacity=777, point_capacity=999, ray_arena=9000, point_arena=9000,
+                   use_graphs=False).serve(reqs, realtime=False)
+    for i in (0, 3, 11):
+        alone = Server(scenes, ray_capacity=1 << 12, point_capacity=1 << 12, ray_arena=4096,
+                       point_arena=4096).serve([reqs[i]], realtime=False)[0]
+        for k in alone.outputs:
+            assert torch.equal(alone.outputs[k], mixed[i].outputs[k]), (i, k)
+    for a, b in zip(mixed, eager):
+        for k in a.outputs:
+            assert torch.equal(a.outputs[k], b.outputs[k]), (a.index, k)
+
+
+def test_serving_realtime_admission_and_latency(scenes):
+    reqs = _requests(24, len(scenes), seed=9, arrival_gap=0.002)
+    srv = Server(scenes, ray_capacity=1024, point_capacity=1024, ray_arena=3000, point_arena=3000)
+    res = srv.serve(reqs, realtime=True)
+    for r in res:
+        assert r.done >= r.arrival and r.admitted >= r.arrival - 1e-3
+        assert all(torch.isfinite(v).all() for v in r.outputs.values())
+    assert srv.ray_pool.free_rows == 3000 and srv.pt_pool.free_rows == 3000

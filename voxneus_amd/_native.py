@@ -59,6 +59,28 @@ class RenderArgs(ctypes.Structure):
     ]
 
 
+class SceneDesc(ctypes.Structure):
+    _fields_ = [
+        ("grid", c_vp), ("X", c_i32), ("Y", c_i32), ("Z", c_i32), ("box", c_f * 6),
+        ("occ_mask", c_vp), ("occ_res", c_i32), ("occ_box", c_f * 6),
+        ("table", c_vp), ("table_dtype", c_i32), ("n_levels", c_i32), ("log2_T", c_i32), ("n_features", c_i32),
+        ("level_res", c_i32 * MAX_LEVELS), ("level_w", c_f * MAX_LEVELS),
+        ("mlp", c_vp), ("s", c_f), ("sphere_radius", c_f),
+    ]
+
+
+class ServeRenderArgs(ctypes.Structure):
+    _fields_ = [
+        ("scenes", c_vp), ("n_scenes", c_i32), ("ray_scene", c_vp), ("n_rays", c_vp), ("capacity", c_i64),
+        ("rays_o", c_vp), ("rays_d", c_vp),
+        ("n_samples", c_i32), ("n_importance", c_i32), ("up_sample_steps", c_i32),
+        ("t_lin", c_vp), ("u_det", c_vp), ("bound_radius", c_f), ("mlp_mode", c_i32),
+        ("cos_anneal_ratio", c_f), ("use_iter_cos", c_i32),
+        ("rgb", c_vp), ("depth", c_vp), ("opacity", c_vp), ("normal", c_vp),
+        ("workspace", c_vp), ("workspace_bytes", c_i64),
+    ]
+
+
 # name -> (restype, argtypes); every entry point returns int except the two strings
 _SIGS = {
     "vxn_version": (ctypes.c_char_p, []),
@@ -85,6 +107,13 @@ _SIGS = {
     "vxn_eikonal_grad": (c_i32, [c_vp, c_i32, c_i32, c_i32, c_vp, c_i64, c_vp, c_vp, c_f, c_vp]),
     "vxn_curvature_grad": (c_i32, [c_vp, c_i32, c_i32, c_i32, c_vp, c_i64, c_vp, c_vp, c_f, c_vp]),
     "vxn_s_amplify": (c_i32, [c_vp, c_f, c_vp, c_vp]),
+    "vxn_serve_scene_bytes": (c_i32, [ctypes.POINTER(c_i64)]),
+    "vxn_serve_pack_scene": (c_i32, [ctypes.POINTER(SceneDesc), c_vp, c_i64]),
+    "vxn_serve_render_workspace_bytes": (c_i32, [ctypes.POINTER(ServeRenderArgs), ctypes.POINTER(c_i64)]),
+    "vxn_serve_render": (c_i32, [ctypes.POINTER(ServeRenderArgs), c_vp]),
+    "vxn_serve_query": (c_i32, [c_vp, c_i32, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp]),
+    "vxn_serve_move": (c_i32, [c_vp, c_vp, c_i32, c_i64, c_i64, c_i32, c_vp, c_vp, c_i32, c_vp, c_vp, c_i32, c_vp,
+                               c_vp, c_vp]),
 }
 
 _lib = None

@@ -1,6 +1,7 @@
 // Dense SDF grid queries: trilinear SDF + Eq. 7 interpolated gradient, their backward,
 // the reference-shaped gradient-interp pair and the full-lattice mesh query.
 #include "vxn_host.h"
+#include "vxn_scene.h"
 
 namespace vxn {
 
@@ -174,9 +175,50 @@ static GridDesc coords_desc(const float* grid, int X, int Y, int Z, const float*
   return g;
 }
 
+// SDF + unit normal over a packed batch of points of many scenes (the serving query).
+__global__ void __launch_bounds__(256) serve_query_kernel(const SceneDev* __restrict__ scenes, int n_scenes,
+                                                          const int32_t* __restrict__ pt_scene,
+                                                          const int32_t* __restrict__ n_dev, int64_t cap,
+                                                          const float* __restrict__ xyz, float* __restrict__ sdf,
+                                                          float* __restrict__ normal) {
+  const int64_t n = min((int64_t)max(*n_dev, 0), cap);
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int si = pt_scene[i];
+    if (si < 0 || si >= n_scenes) {
+      sdf[i] = 0.0f;
+      if (normal) normal[3 * i] = normal[3 * i + 1] = normal[3 * i + 2] = 0.0f;
+      continue;
+    }
+    const GridDesc& g = scenes[si].g;
+    const float px = xyz[3 * i], py = xyz[3 * i + 1], pz = xyz[3 * i + 2];
+    sdf[i] = gs_eval(g, gs_stencil(g, px, py, pz));
+    if (normal) {
+      float gv[3];
+      eq7_gradient(g, corner_decomp(g, px, py, pz), gv);
+      const float r = norm3(gv) + 1e-8f;
+      normal[3 * i] = gv[0] / r;
+      normal[3 * i + 1] = gv[1] / r;
+      normal[3 * i + 2] = gv[2] / r;
+    }
+  }
+}
+
 }  // namespace vxn
 
 using namespace vxn;
+
+extern "C" int vxn_serve_query(const void* scenes, int32_t n_scenes, const int32_t* pt_scene, const int32_t* n_points,
+                               int64_t capacity, const float* xyz, float* sdf, float* normal, void* stream) {
+  VXN_REQUIRE(capacity >= 0, VXN_ERR_SHAPE, "capacity must be >= 0");
+  if (capacity == 0) return 0;
+  VXN_REQUIRE(scenes && pt_scene && n_points && xyz && sdf, VXN_ERR_NULL, "null argument");
+  VXN_REQUIRE(n_scenes >= 1, VXN_ERR_VALUE, "n_scenes must be >= 1");
+  const int64_t nb = min((capacity + 255) / 256, (int64_t)8192);
+  hipLaunchKernelGGL(serve_query_kernel, dim3((int)nb), dim3(256), 0, (hipStream_t)stream, (const SceneDev*)scenes,
+                     n_scenes, pt_scene, n_points, capacity, xyz, sdf, normal);
+  VXN_LAUNCH_CHECK();
+  return 0;
+}
 
 extern "C" int vxn_grid_query_fwd(const float* grid, int X, int Y, int Z, const float* box6, const float* xyz,
                                   int64_t n, float* sdf, float* grad, float* normal, int64_t* corners,

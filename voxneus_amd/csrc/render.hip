@@ -18,9 +18,12 @@
 // the Huber colour loss (rendering/volume_render.py:43-54), and the scatters of the
 // reference backward: index_put into the hash table (radiance_hash.py:151-160) and the
 // Eq. 7 / grid_sample gradients (cuda/gradient_interp.cu:86-152).
+#include <string.h>
+
 #include "vxn_hash.h"
 #include "vxn_host.h"
 #include "vxn_mfma.h"
+#include "vxn_scene.h"
 
 namespace vxn {
 
@@ -69,6 +72,12 @@ struct RenderDev {
   double* part_sc;  // [gridDim.x][2]: dL/ds, Huber sum
   float* gfeat;     // [32][R*S] dL/d(hash features), feature-major
   float* ggeo;      // [4][R*S]  dL/dsdf, dL/dgrad
+  // serving batches (forward only): per-ray scene of a packed batch; R is the capacity
+  // and the batch size is read from n_rays_dev
+  const SceneDev* scenes;
+  const int32_t* ray_scene;
+  const int32_t* n_rays_dev;
+  int n_scenes;
 };
 
 struct Red {
@@ -160,7 +169,7 @@ VXN_DEV void build_mlp_fragments(const float* W0, const float* W1, u32x4_t* frag
   }
 }
 
-template <int MODE, int MLPM>
+template <int MODE, int MLPM, int MULTI>
 __global__ void __launch_bounds__(kRT, 2) render_kernel(RenderDev a) {
   constexpr bool kBF = MLPM == kMlpBf16;
   constexpr bool kBFB = kBF && MODE != kFwd;
@@ -209,14 +218,44 @@ __global__ void __launch_bounds__(kRT, 2) render_kernel(RenderDev a) {
   const int jq = lane & 31, gq = lane >> 5;
   f32x16_t gw0[2] = {}, gw1 = {};
   float accb[3] = {0.0f, 0.0f, 0.0f};
-  if constexpr (kBF) {
+  if constexpr (kBF && !MULTI) {
     build_mlp_fragments(W0, W1, bfrag, t);
     __syncthreads();
   }
+  int frag_scene = -1;  // serving batches: scene whose MLP the fragments hold
 
-  for (int64_t ray = blockIdx.x; ray < a.R; ray += gridDim.x) {
+  int64_t r0, r1, rstep;
+  ray_range<MULTI>(a.R, a.n_rays_dev, r0, r1, rstep);
+  for (int64_t ray = r0; ray < r1; ray += rstep) {
     const bool valid = t < a.S;
     const int64_t si = ray * a.S + t;
+    const SceneDev* scn = nullptr;
+    if constexpr (MULTI) {
+      const int sx = a.ray_scene[ray];
+      if (sx < 0 || sx >= a.n_scenes) {  // no such scene: zeros
+        if (t < 3) a.rgb[3 * ray + t] = 0.0f;
+        if (t == 0 && a.depth) a.depth[ray] = 0.0f;
+        if (t == 0 && a.opacity) a.opacity[ray] = 0.0f;
+        if (t < 3 && a.normal) a.normal[3 * ray + t] = 0.0f;
+        continue;
+      }
+      scn = a.scenes + sx;
+      if constexpr (kBF) {
+        if (sx != frag_scene) {
+          __syncthreads();
+          build_mlp_fragments(scn->mlp, scn->mlp + kOffW1, bfrag, t);
+          frag_scene = sx;
+        }
+      }
+    }
+    const GridDesc& G = MULTI ? scn->g : a.g;
+    const HashDesc& Hd = MULTI ? scn->h : a.h;
+    const float sv = MULTI ? scn->s : a.s;
+    const float sr = MULTI ? scn->sphere_r : a.sphere_r;
+    const float* W0 = MULTI ? scn->mlp : a.mlp;
+    const float* b0 = W0 + kOffB0;
+    const float* W1 = W0 + kOffW1;
+    const float* b1 = W0 + kOffB1;
     const float ox = a.o[3 * ray], oy = a.o[3 * ray + 1], oz = a.o[3 * ray + 2];
     const float dx = a.d[3 * ray], dy = a.d[3 * ray + 1], dz = a.d[3 * ray + 2];
     const float mid = valid ? a.mid[si] : 0.0f;
@@ -238,19 +277,19 @@ __global__ void __launch_bounds__(kRT, 2) render_kernel(RenderDev a) {
     }
     // ---------------------------------------------------------------- sample forward
     const float px = ox + dx * mid, py = oy + dy * mid, pz = oz + dz * mid;
-    const GsStencil gs = gs_stencil(a.g, px, py, pz);
-    const float sdf = gs_eval(a.g, gs);
-    const Corners cn = corner_decomp(a.g, px, py, pz);
+    const GsStencil gs = gs_stencil(G, px, py, pz);
+    const float sdf = gs_eval(G, gs);
+    const Corners cn = corner_decomp(G, px, py, pz);
     float gv[3];
-    eq7_gradient(a.g, cn, gv);
+    eq7_gradient(G, cn, gv);
     const float gr = norm3(gv);
     const float nrm[3] = {gv[0] / (gr + 1e-8f), gv[1] / (gr + 1e-8f), gv[2] / (gr + 1e-8f)};
-    const float x0 = fminf(fmaxf((px - a.g.mn[0]) / a.g.den[0], 0.0f), 1.0f);
-    const float x1 = fminf(fmaxf((py - a.g.mn[1]) / a.g.den[1], 0.0f), 1.0f);
-    const float x2 = fminf(fmaxf((pz - a.g.mn[2]) / a.g.den[2], 0.0f), 1.0f);
+    const float x0 = fminf(fmaxf((px - G.mn[0]) / G.den[0], 0.0f), 1.0f);
+    const float x1 = fminf(fmaxf((py - G.mn[1]) / G.den[1], 0.0f), 1.0f);
+    const float x2 = fminf(fmaxf((pz - G.mn[2]) / G.den[2], 0.0f), 1.0f);
     float x[35];  // 32 hash features, then the unit normal (MLP input columns 0-31, 48-50)
 #pragma unroll
-    for (int l = 0; l < 16; ++l) hash_level_fwd(a.h, l, x0, x1, x2, x[2 * l], x[2 * l + 1]);
+    for (int l = 0; l < 16; ++l) hash_level_fwd(Hd, l, x0, x1, x2, x[2 * l], x[2 * l + 1]);
     x[32] = nrm[0];
     x[33] = nrm[1];
     x[34] = nrm[2];
@@ -356,11 +395,11 @@ __global__ void __launch_bounds__(kRT, 2) render_kernel(RenderDev a) {
     }
     const float off = 0.5f * dist * cosv;
     const float fp = sdf - off, fn = sdf + off;
-    const float pp = sigmoid_ref(a.s * fp), pn = sigmoid_ref(a.s * fn);
+    const float pp = sigmoid_ref(sv * fp), pn = sigmoid_ref(sv * fn);
     const float den = fmaxf(pp, 1e-5f);
     const float q = (pp - pn) / den;
     const float araw = fminf(fmaxf(q, 0.0f), 1.0f);
-    const float inside = (a.sphere_r > 0.0f) ? ((sqrtf(px * px + py * py + pz * pz) < a.sphere_r) ? 1.0f : 0.0f) : 1.0f;
+    const float inside = (sr > 0.0f) ? ((sqrtf(px * px + py * py + pz * pz) < sr) ? 1.0f : 0.0f) : 1.0f;
     const float alpha = valid ? araw * inside : 0.0f;
     const float ov = (1.0f - alpha) + 1e-7f;
     const float oma = fminf(fmaxf(ov, 0.0f), 1.0f);
@@ -417,7 +456,7 @@ __global__ void __launch_bounds__(kRT, 2) render_kernel(RenderDev a) {
     if (a.vr) {
       // V_R marks: a sample in the same cell as the previous lane's marks nothing new;
       // the two z-corners of each (x, y) column share a word unless z0 ends one.
-      const int key = valid ? (int)lin3(a.g, cn.c0[0], cn.c0[1], cn.c0[2]) : -1;
+      const int key = valid ? (int)lin3(G, cn.c0[0], cn.c0[1], cn.c0[2]) : -1;
       const int kp = __shfl_up(key, 1, 64);
       if (valid && (lane == 0 || kp != key)) {
 #pragma unroll
@@ -425,7 +464,7 @@ __global__ void __launch_bounds__(kRT, 2) render_kernel(RenderDev a) {
           int cx, cy, cz0, cz1;
           corner_xyz(cn, k, cx, cy, cz0);
           corner_xyz(cn, k + 1, cx, cy, cz1);
-          const int64_t l0 = lin3(a.g, cx, cy, cz0), l1 = lin3(a.g, cx, cy, cz1);
+          const int64_t l0 = lin3(G, cx, cy, cz0), l1 = lin3(G, cx, cy, cz1);
           if ((l0 >> 5) == (l1 >> 5)) {
             atomicOr(a.vr + (l0 >> 5), (1u << (l0 & 31)) | (1u << (l1 & 31)));
           } else {
@@ -487,7 +526,7 @@ __global__ void __launch_bounds__(kRT, 2) render_kernel(RenderDev a) {
     const float dzp = dpp * (1.0f - pp) * pp;
     const float dzn = dpn * (1.0f - pn) * pn;
     ds_acc += (double)dzp * fp + (double)dzn * fn;
-    const float dfp = dzp * a.s, dfn = dzn * a.s;
+    const float dfp = dzp * sv, dfn = dzn * sv;
     const float dsdf = dfp + dfn;
     const float doff = dfn - dfp;
     float dcos = doff * 0.5f * dist;
@@ -1043,6 +1082,10 @@ static int build_render_dev(const vxn_render_args& A, RenderDev& a, int mode) {
   a.part_sc = nullptr;
   a.gfeat = nullptr;
   a.ggeo = nullptr;
+  a.scenes = nullptr;
+  a.ray_scene = nullptr;
+  a.n_rays_dev = nullptr;
+  a.n_scenes = 0;
   if (mode != kFwd) {
     VXN_REQUIRE(A.grad_grid && A.grad_table && A.grad_mlp && A.grad_s, VXN_ERR_NULL, "null gradient output");
     const RenderWs w = render_ws(A.n_rays, A.n_samples);
@@ -1079,9 +1122,9 @@ static int launch_render(const vxn_render_args* p, void* stream) {
   hipEvent_t const* ev = (hipEvent_t const*)p->kernel_events;
   if (ev) VXN_HIP(hipEventRecord(ev[0], s));
   if (p->mlp_mode == VXN_MLP_BF16_MFMA)
-    hipLaunchKernelGGL((render_kernel<MODE, kMlpBf16>), dim3(nb), dim3(kRT), 0, s, a);
+    hipLaunchKernelGGL((render_kernel<MODE, kMlpBf16, 0>), dim3(nb), dim3(kRT), 0, s, a);
   else
-    hipLaunchKernelGGL((render_kernel<MODE, kMlpF32>), dim3(nb), dim3(kRT), 0, s, a);
+    hipLaunchKernelGGL((render_kernel<MODE, kMlpF32, 0>), dim3(nb), dim3(kRT), 0, s, a);
   VXN_LAUNCH_CHECK();
   if (ev) VXN_HIP(hipEventRecord(ev[1], s));
   if (MODE != kFwd) {
@@ -1120,3 +1163,105 @@ extern "C" int vxn_render_workspace_bytes(const vxn_render_args* a, int64_t* byt
 extern "C" int vxn_render_forward(const vxn_render_args* a, void* stream) { return launch_render<kFwd>(a, stream); }
 extern "C" int vxn_render_backward(const vxn_render_args* a, void* stream) { return launch_render<kBwd>(a, stream); }
 extern "C" int vxn_render_train(const vxn_render_args* a, void* stream) { return launch_render<kTrain>(a, stream); }
+
+// ------------------------------------------------------------------------------------
+// Serving batches
+// ------------------------------------------------------------------------------------
+
+extern "C" int vxn_serve_scene_bytes(int64_t* bytes) {
+  VXN_REQUIRE(bytes != nullptr, VXN_ERR_NULL, "bytes is null");
+  *bytes = (int64_t)sizeof(SceneDev);
+  return 0;
+}
+
+extern "C" int vxn_serve_pack_scene(const vxn_scene_desc* d, void* out, int64_t out_bytes) {
+  VXN_REQUIRE(d != nullptr && out != nullptr, VXN_ERR_NULL, "null argument");
+  VXN_REQUIRE(out_bytes >= (int64_t)sizeof(SceneDev), VXN_ERR_WORKSPACE, "a scene record needs %lld bytes",
+              (long long)sizeof(SceneDev));
+  if (int e = check_grid(d->grid, d->X, d->Y, d->Z, d->box)) return e;
+  VXN_REQUIRE(d->n_levels == 16 && d->n_features == 2, VXN_ERR_UNSUPPORTED,
+              "the fused render implements the 16-level x 2-feature hash feeding the 51-input MLP");
+  VXN_REQUIRE(d->mlp != nullptr && d->table != nullptr, VXN_ERR_NULL, "null table or mlp");
+  if (d->occ_mask) VXN_REQUIRE(d->occ_res >= 1, VXN_ERR_SHAPE, "occ_res must be >= 1");
+  SceneDev sc{};
+  sc.g = make_grid_desc(d->grid, d->X, d->Y, d->Z, d->box);
+  if (int e = make_hash_desc(sc.h, d->table, d->table_dtype, d->n_levels, d->log2_T, d->n_features, d->level_res,
+                             d->level_w))
+    return e;
+  sc.oc = make_occ_desc(d->occ_mask, d->occ_mask ? d->occ_res : 1, d->occ_box);
+  sc.mlp = d->mlp;
+  sc.s = d->s;
+  sc.sphere_r = d->sphere_radius;
+  memcpy(out, &sc, sizeof(SceneDev));
+  return 0;
+}
+
+static int serve_shape(const vxn_serve_render_args& A, int& steps, int& per, int& S) {
+  VXN_REQUIRE(A.capacity >= 0, VXN_ERR_SHAPE, "capacity must be >= 0");
+  VXN_REQUIRE(A.n_samples >= 2, VXN_ERR_SHAPE, "n_samples must be >= 2");
+  VXN_REQUIRE(A.up_sample_steps >= 0 && A.n_importance >= 0, VXN_ERR_VALUE, "negative importance settings");
+  steps = (A.n_importance > 0 && A.up_sample_steps > 0) ? A.up_sample_steps : 0;
+  per = steps ? (A.n_importance / steps > 1 ? A.n_importance / steps : 1) : 0;
+  S = A.n_samples + steps * per;
+  VXN_REQUIRE(S <= kRT, VXN_ERR_UNSUPPORTED, "at most %d samples per ray are supported (got %d)", kRT, S);
+  VXN_REQUIRE(per <= 64 && steps <= 20, VXN_ERR_UNSUPPORTED, "at most 64 samples per step and 20 steps");
+  return 0;
+}
+
+extern "C" int vxn_serve_render_workspace_bytes(const vxn_serve_render_args* a, int64_t* bytes) {
+  VXN_REQUIRE(a != nullptr && bytes != nullptr, VXN_ERR_NULL, "null argument");
+  int steps, per, S;
+  if (int e = serve_shape(*a, steps, per, S)) return e;
+  *bytes = 3 * align256(a->capacity * (int64_t)S * (int64_t)sizeof(float));
+  return 0;
+}
+
+extern "C" int vxn_serve_render(const vxn_serve_render_args* p, void* stream) {
+  VXN_REQUIRE(p != nullptr, VXN_ERR_NULL, "args is null");
+  const vxn_serve_render_args& A = *p;
+  int steps, per, S;
+  if (int e = serve_shape(A, steps, per, S)) return e;
+  if (A.capacity == 0) return 0;
+  VXN_REQUIRE(A.scenes && A.ray_scene && A.n_rays && A.rays_o && A.rays_d && A.t_lin && A.rgb, VXN_ERR_NULL,
+              "null argument");
+  VXN_REQUIRE(A.n_scenes >= 1, VXN_ERR_VALUE, "n_scenes must be >= 1");
+  VXN_REQUIRE(!steps || A.u_det, VXN_ERR_NULL, "u_det is required");
+  VXN_REQUIRE(A.bound_radius > 0.0f, VXN_ERR_VALUE, "bound_radius must be > 0");
+  VXN_REQUIRE(A.mlp_mode == VXN_MLP_FP32 || A.mlp_mode == VXN_MLP_BF16_MFMA, VXN_ERR_VALUE, "unknown mlp_mode %d",
+              A.mlp_mode);
+  const int64_t plane = align256(A.capacity * (int64_t)S * (int64_t)sizeof(float));
+  VXN_REQUIRE(A.workspace != nullptr && A.workspace_bytes >= 3 * plane, VXN_ERR_WORKSPACE,
+              "serve workspace needs %lld bytes", (long long)(3 * plane));
+  float* z = (float*)A.workspace;
+  float* dists = (float*)((char*)A.workspace + plane);
+  float* mid = (float*)((char*)A.workspace + 2 * plane);
+  const SceneDev* scenes = (const SceneDev*)A.scenes;
+  hipStream_t s = (hipStream_t)stream;
+  if (int e = launch_serve_upsample(scenes, A.n_scenes, A.ray_scene, A.n_rays, A.capacity, A.rays_o, A.rays_d,
+                                    A.n_samples, per, steps, A.t_lin, A.u_det, A.bound_radius, z, dists, mid, s))
+    return e;
+  RenderDev a{};
+  a.o = A.rays_o;
+  a.d = A.rays_d;
+  a.mid = mid;
+  a.dist = dists;
+  a.R = A.capacity;
+  a.S = S;
+  a.cos_anneal = A.cos_anneal_ratio;
+  a.iter_cos = A.use_iter_cos;
+  a.rgb = A.rgb;
+  a.depth = A.depth;
+  a.opacity = A.opacity;
+  a.normal = A.normal;
+  a.scenes = scenes;
+  a.ray_scene = A.ray_scene;
+  a.n_rays_dev = A.n_rays;
+  a.n_scenes = A.n_scenes;
+  const int nb = render_blocks(A.capacity);
+  if (A.mlp_mode == VXN_MLP_BF16_MFMA)
+    hipLaunchKernelGGL((render_kernel<kFwd, kMlpBf16, 1>), dim3(nb), dim3(kRT), 0, s, a);
+  else
+    hipLaunchKernelGGL((render_kernel<kFwd, kMlpF32, 1>), dim3(nb), dim3(kRT), 0, s, a);
+  VXN_LAUNCH_CHECK();
+  return 0;
+}

@@ -9,6 +9,7 @@
 //     accumulators over rows of 4 vectors, leftovers into accumulator 0, accumulators
 //     folded 0+1+2+3, a sequential scalar tail, then the 8 lanes folded in order.
 #include "vxn_host.h"
+#include "vxn_scene.h"
 
 namespace vxn {
 
@@ -20,43 +21,38 @@ struct SamplerDev {
   const float *o, *d, *near, *far;
   int64_t R;
   GridDesc g;
-  const uint8_t* occ;
-  int occ_res;
-  float occ_mn[3], occ_den[3];
-  float occ_bound;
+  OccDesc oc;
   int n0, per, steps, perturb;
   const float *t_lin, *u_det, *t_rand, *u_rand;
   float *z_out, *dists_out, *mid_out;
   int32_t* inds;
   float bound_r;
+  // serving batches: per-ray scene of a packed batch; R is then the capacity and the
+  // batch size is read from n_rays_dev
+  const SceneDev* scenes;
+  const int32_t* ray_scene;
+  const int32_t* n_rays_dev;
+  int n_scenes;
 };
 
-// OccupancyGrid.is_occupied (occupancy_grid.py:121-147) folded into the modulation factor
-// occ.float() * 0.9 + 0.1 (sampler.py:285-289).
-VXN_DEV float occ_factor(const SamplerDev& a, float px, float py, float pz) {
-  const float p[3] = {px, py, pz};
-  bool inb = true;
-  int c[3];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    const float nrm = (p[k] - a.occ_mn[k]) / a.occ_den[k];
-    inb = inb && (nrm >= 0.0f) && (nrm <= 1.0f);
-    const float v = fminf(fmaxf(nrm * (float)a.occ_res, 0.0f), a.occ_bound);
-    c[k] = (int)v;
-  }
-  const int64_t r = a.occ_res;
-  const bool occ = inb && a.occ[((int64_t)c[0] * r + c[1]) * r + c[2]] != 0;
-  const float f = occ ? 1.0f : 0.0f;
-  return f * 0.9f + 0.1f;
-}
-
+template <int MULTI>
 __global__ void __launch_bounds__(kSampThreads) upsample_kernel(SamplerDev a) {
   __shared__ float zs[kMaxS], sd[kMaxS], wv[kMaxS], cdf[kMaxS + 1], cosv[kMaxS];
   __shared__ float nz[kMaxPer], nzs[kMaxPer];
   __shared__ unsigned char ins[kMaxS];
   __shared__ float acc8[8];
   const int t = threadIdx.x;
-  for (int64_t ray = blockIdx.x; ray < a.R; ray += gridDim.x) {
+  int64_t r0, r1, rstep;
+  ray_range<MULTI>(a.R, a.n_rays_dev, r0, r1, rstep);
+  for (int64_t ray = r0; ray < r1; ray += rstep) {
+    const SceneDev* sc = nullptr;
+    if constexpr (MULTI) {
+      const int si = a.ray_scene[ray];
+      if (si < 0 || si >= a.n_scenes) continue;  // the render writes zeros for such rays
+      sc = a.scenes + si;
+    }
+    const GridDesc& G = MULTI ? sc->g : a.g;
+    const OccDesc& OC = MULTI ? sc->oc : a.oc;
     const float ox = a.o[3 * ray], oy = a.o[3 * ray + 1], oz = a.o[3 * ray + 2];
     const float dx = a.d[3 * ray], dy = a.d[3 * ray + 1], dz = a.d[3 * ray + 2];
     float nr, fr;
@@ -92,7 +88,7 @@ __global__ void __launch_bounds__(kSampThreads) upsample_kernel(SamplerDev a) {
     // 2. SDF at the stratified points
     if (t < n) {
       const float px = ox + dx * z, py = oy + dy * z, pz = oz + dz * z;
-      sd[t] = gs_eval(a.g, gs_stencil(a.g, px, py, pz));
+      sd[t] = gs_eval(G, gs_stencil(G, px, py, pz));
     }
     __syncthreads();
     for (int step = 0; step < a.steps; ++step) {
@@ -131,9 +127,9 @@ __global__ void __launch_bounds__(kSampThreads) upsample_kernel(SamplerDev a) {
       __syncthreads();
       if (t < m) {
         float w = alpha * cdf[t];
-        if (a.occ) {
+        if (OC.occ) {
           const float mz = (zs[t] + zs[t + 1]) * 0.5f;
-          w = w * occ_factor(a, ox + dx * mz, oy + dy * mz, oz + dz * mz);
+          w = w * occ_factor(OC, ox + dx * mz, oy + dy * mz, oz + dz * mz);
         }
         wv[t] = w + 1e-5f;
       }
@@ -230,7 +226,7 @@ __global__ void __launch_bounds__(kSampThreads) upsample_kernel(SamplerDev a) {
         pn = t + lo;
         if (!last) {
           const float px = ox + dx * zn, py = oy + dy * zn, pz = oz + dz * zn;
-          sn = gs_eval(a.g, gs_stencil(a.g, px, py, pz));
+          sn = gs_eval(G, gs_stencil(G, px, py, pz));
         }
       }
       __syncthreads();
@@ -255,6 +251,37 @@ __global__ void __launch_bounds__(kSampThreads) upsample_kernel(SamplerDev a) {
     }
     __syncthreads();
   }
+}
+
+int launch_serve_upsample(const SceneDev* scenes, int n_scenes, const int32_t* ray_scene, const int32_t* n_dev,
+                          int64_t cap, const float* o, const float* d, int n0, int per, int steps,
+                          const float* t_lin, const float* u_det, float bound_r, float* z, float* dists, float* mid,
+                          hipStream_t stream) {
+  SamplerDev a{};
+  a.o = o;
+  a.d = d;
+  a.near = nullptr;
+  a.far = nullptr;
+  a.R = cap;
+  a.n0 = n0;
+  a.per = per;
+  a.steps = steps;
+  a.perturb = 0;
+  a.t_lin = t_lin;
+  a.u_det = u_det;
+  a.z_out = z;
+  a.dists_out = dists;
+  a.mid_out = mid;
+  a.inds = nullptr;
+  a.bound_r = bound_r;
+  a.scenes = scenes;
+  a.ray_scene = ray_scene;
+  a.n_rays_dev = n_dev;
+  a.n_scenes = n_scenes;
+  const int nb = cap < 8192 ? (int)cap : 8192;
+  hipLaunchKernelGGL(upsample_kernel<1>, dim3(nb), dim3(kSampThreads), 0, stream, a);
+  VXN_LAUNCH_CHECK();
+  return 0;
 }
 
 }  // namespace vxn
@@ -289,13 +316,7 @@ extern "C" int vxn_upsample(const vxn_sampler_args* p, void* stream) {
   a.far = A.far;
   a.R = A.n_rays;
   a.g = make_grid_desc(A.grid, A.X, A.Y, A.Z, A.box);
-  a.occ = A.occ_mask;
-  a.occ_res = A.occ_res;
-  for (int k = 0; k < 3; ++k) {
-    a.occ_mn[k] = A.occ_box[k];
-    a.occ_den[k] = (A.occ_box[3 + k] - A.occ_box[k]) + 1e-8f;
-  }
-  a.occ_bound = (float)((double)A.occ_res - 1.0001);
+  a.oc = make_occ_desc(A.occ_mask, A.occ_res, A.occ_box);
   a.n0 = A.n_samples;
   a.per = per;
   a.steps = steps;
@@ -309,8 +330,12 @@ extern "C" int vxn_upsample(const vxn_sampler_args* p, void* stream) {
   a.mid_out = A.mid_z;
   a.inds = A.inds;
   a.bound_r = A.bound_radius;
+  a.scenes = nullptr;
+  a.ray_scene = nullptr;
+  a.n_rays_dev = nullptr;
+  a.n_scenes = 0;
   const int nb = A.n_rays < 8192 ? (int)A.n_rays : 8192;
-  hipLaunchKernelGGL(upsample_kernel, dim3(nb), dim3(kSampThreads), 0, (hipStream_t)stream, a);
+  hipLaunchKernelGGL(upsample_kernel<0>, dim3(nb), dim3(kSampThreads), 0, (hipStream_t)stream, a);
   VXN_LAUNCH_CHECK();
   return 0;
 }
