@@ -229,6 +229,19 @@ int vxn_eikonal_grad(const float* grid, int X, int Y, int Z, const int64_t* acti
 int vxn_curvature_grad(const float* grid, int X, int Y, int Z, const int64_t* active, int64_t V, float* grad_grid,
                        const float* eps3, float weight, void* stream);
 
+/* The same regularisers straight from the V_R bitmask the render marks, for the fused
+ * step: counts[2] (device) receives {|V_R|, |V_R inside the relaxed sphere|} (box6 and
+ * relax_radius as vxn_vr_compact), then every grid vertex gathers its gradient from the
+ * V_R vertices whose stencils hold it (no atomics, deterministic).  Same losses, gradient
+ * and |V_R| normalisation as vxn_reg_eik_curv on the compacted list; replaces
+ * extract_active_vertices + the relaxed-sphere filter + both regularisers
+ * (regularization/active_vertices.py:12-50, trainer.py:260-266, eikonal.py:13-171,
+ * curvature.py:13-165).  Workspace: vxn_reg_vr_workspace_bytes.                         */
+int vxn_reg_vr_workspace_bytes(int X, int Y, int Z, int64_t* bytes);
+int vxn_reg_eik_curv_vr(const float* grid, int X, int Y, int Z, const float* eps3, const float* box6,
+                        float relax_radius, const uint32_t* bitmask, float w_eik, float w_curv, float* grad_grid,
+                        double* losses, int32_t* counts, void* workspace, int64_t workspace_bytes, void* stream);
+
 /* s-gradient amplification on the device: grad_v *= k if grad_v < 0.  amplified
  * (device, nullable) receives 1 or 0.  Replaces amplify_s_gradient
  * (training/scheduler.py:14-33) without its host read-back.                           */

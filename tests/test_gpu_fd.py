@@ -199,7 +199,7 @@ def test_render_backward_vs_fp64_finite_differences(setup):
     e_g = np.abs(gg - ga).max() / np.abs(ga).max()
     e_t = np.abs(gt - ta).max() / np.abs(ta).max()
     print(f"full gradient vs fp64 autograd: grid {e_g:.2e} hash {e_t:.2e}")
-    assert e_g < 1e-4 and e_t < 1e-4
+    assert e_g < 1e-4 and e_t < 3e-4
 
 
 def test_bf16_render_backward_vs_fp64_finite_differences(setup):

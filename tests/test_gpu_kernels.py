@@ -256,6 +256,27 @@ def test_vr_compaction_and_regularisers_match_oracle(Fn):
     assert abs(losses[1].item() - lc.item()) <= 1e-5 * lc.item()
 
 
+@pytest.mark.parametrize("relax", [1.2, 0.3])
+def test_bitmask_regularisers_equal_list_form(Fn, relax):
+    """The atomic-free gather over the V_R bitmask gives the list form's counts, losses and
+    gradient (FP32 sums in another order); relax 0.3 leaves part of V_R outside."""
+    rng = np.random.default_rng(6)
+    res = 40
+    data = torch.from_numpy(rng.normal(size=(res, res, res)).astype(np.float32)).to(DEV)
+    corners = torch.from_numpy(rng.integers(0, res, size=(30000, 8, 3))).to(DEV)
+    bm = torch.zeros(Fn.vr_bitmask_words((res, res, res)), device=DEV, dtype=torch.int32)
+    Fn.vr_mark_corners(corners, (res, res, res), bm)
+    lst, counts = Fn.vr_compact(bm, (res, res, res), BOX6, relax_radius=relax)
+    vs = Fn.voxel_size_of(BOX6, (res, res, res))
+    g_list = torch.full((res, res, res), 0.25, device=DEV)
+    l_list = Fn.reg_eik_curv(data, vs, lst, counts, 0.1, 5e-4, g_list)
+    g_vr = torch.full((res, res, res), 0.25, device=DEV)
+    l_vr, c_vr = Fn.reg_eik_curv_vr(data, vs, BOX6, relax, bm, 0.1, 5e-4, g_vr)
+    assert torch.equal(c_vr, counts)
+    assert rel_err(l_vr.cpu().numpy(), l_list.cpu().numpy()) < 1e-12
+    assert rel_err((g_vr - 0.25).cpu().numpy(), (g_list - 0.25).cpu().numpy()) < 1e-6
+
+
 def test_reference_shaped_regularisers(Fn):
     """Pattern of test_cuda_parity.py:72-126 (in-place accumulation == separate gradient)."""
     torch.manual_seed(42)

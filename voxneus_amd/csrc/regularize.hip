@@ -270,6 +270,177 @@ __global__ void __launch_bounds__(256) curvature_ref_kernel(GridDesc g, const in
   }
 }
 
+// ------------------------------------------------------------------------------------
+// V_R straight from the bitmask: per-block counts {|V_R|, |V_R inside the relaxed
+// sphere|}, then the regularisers over the marked vertices in grid order (work-groups walk
+// (x, y) rows, lanes walk z), so no vertex list is built and the atomics of a wavefront
+// land on contiguous addresses.
+// ------------------------------------------------------------------------------------
+
+struct VrGeom {
+  float mn[3];  // xyz_min
+  float e[3];   // (xyz_max - xyz_min) / (res - 1), float32 (as the compaction computes it)
+  float relax;  // relaxed-sphere radius
+};
+
+VXN_DEV bool vr_inside(const VrGeom& v, int ix, int iy, int iz) {
+  const float wx = v.mn[0] + (float)ix * v.e[0], wy = v.mn[1] + (float)iy * v.e[1], wz = v.mn[2] + (float)iz * v.e[2];
+  return sqrtf(wx * wx + wy * wy + wz * wz) < v.relax;
+}
+
+__global__ void __launch_bounds__(kCompactThreads) vr_count2_kernel(const uint32_t* __restrict__ bits, int64_t nwords,
+                                                                    int X, int Y, int Z, VrGeom v,
+                                                                    int32_t* __restrict__ part) {
+  __shared__ int32_t red[2][kCompactThreads / 64];
+  const int64_t w = blockIdx.x * (int64_t)kCompactThreads + threadIdx.x;
+  uint32_t word = w < nwords ? bits[w] : 0u;
+  int c = __popc(word), ci = 0;
+  const int64_t YZ = (int64_t)Y * Z;
+  while (word) {
+    const int b = __ffs(word) - 1;
+    word &= word - 1;
+    const int64_t lin = w * 32 + b;
+    const int ix = (int)(lin / YZ);
+    const int64_t r = lin - (int64_t)ix * YZ;
+    const int iy = (int)(r / Z);
+    ci += vr_inside(v, ix, iy, (int)(r - (int64_t)iy * Z)) ? 1 : 0;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    c += __shfl_xor(c, o, 64);
+    ci += __shfl_xor(ci, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = c;
+    red[1][threadIdx.x >> 6] = ci;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int a = 0, b = 0;
+    for (int i = 0; i < kCompactThreads / 64; ++i) {
+      a += red[0][i];
+      b += red[1][i];
+    }
+    part[2 * blockIdx.x] = a;
+    part[2 * blockIdx.x + 1] = b;
+  }
+}
+
+__global__ void __launch_bounds__(256) vr_count_finish_kernel(const int32_t* __restrict__ part, int nblocks,
+                                                              int32_t* __restrict__ counts) {
+  __shared__ int32_t red[2][4];
+  int a = 0, b = 0;
+  for (int i = threadIdx.x; i < nblocks; i += 256) {
+    a += part[2 * i];
+    b += part[2 * i + 1];
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    a += __shfl_xor(a, o, 64);
+    b += __shfl_xor(b, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = a;
+    red[1][threadIdx.x >> 6] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    counts[0] = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+    counts[1] = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+  }
+}
+
+constexpr int kGatherThreads = 256;
+
+__global__ void __launch_bounds__(kGatherThreads) reg_vr_kernel(GridDesc g, const uint32_t* __restrict__ bits,
+                                                                    VrGeom v, const int32_t* __restrict__ counts,
+                                                                    float w_eik, float w_curv,
+                                                                    float* __restrict__ gg,
+                                                                    double* __restrict__ partial) {
+  const int n_all = counts[0], n_in = counts[1];
+  const bool filt = n_in > 0;
+  const int nact = filt ? n_in : n_all;
+  const float scale = (float)(2.0 / (double)(nact > 0 ? nact : 1));
+  const float dx = 2.0f * g.eps[0], dy = 2.0f * g.eps[1], dz = 2.0f * g.eps[2];
+  const float ex2 = g.eps[0] * g.eps[0], ey2 = g.eps[1] * g.eps[1], ez2 = g.eps[2] * g.eps[2];
+  const int X = g.X, Y = g.Y, Z = g.Z;
+  const int64_t YZ = (int64_t)Y * Z;
+  const float* f = g.grid;
+  // u is a V_R vertex that the losses use: marked, interior, inside the sphere if filtering
+  auto member = [&](int64_t u, int ux, int uy, int uz) -> bool {
+    if (ux <= 0 || ux >= X - 1 || uy <= 0 || uy >= Y - 1 || uz <= 0 || uz >= Z - 1) return false;
+    if (!((bits[u >> 5] >> (u & 31)) & 1u)) return false;
+    return !filt || vr_inside(v, ux, uy, uz);
+  };
+  double se = 0.0, sc = 0.0, cnt = 0.0;
+  // work-groups walk the (x, y) rows, threads walk z: no per-vertex index division
+  for (int row = blockIdx.x; row < X * Y; row += gridDim.x) {
+   const int wx = row / Y, wy = row - (row / Y) * Y;
+   // skip rows without a marked vertex
+   bool marked = false;
+   {
+     const int64_t w0 = ((int64_t)row * Z) >> 5, w1 = ((int64_t)row * Z + Z - 1) >> 5;
+     for (int64_t ww = w0 + threadIdx.x; ww <= w1; ww += kGatherThreads) marked = marked || bits[ww] != 0u;
+   }
+   if (!__syncthreads_or(marked)) continue;
+   for (int wz = threadIdx.x; wz < Z; wz += kGatherThreads) {
+    const int64_t w = (int64_t)row * Z + wz;
+    if (!member(w, wx, wy, wz)) continue;
+    const float fc = f[w];
+    const float fxp = f[w + YZ], fxm = f[w - YZ], fyp = f[w + Z], fym = f[w - Z], fzp = f[w + 1], fzm = f[w - 1];
+    // eikonal (eikonal.py:181-218): v = scale (1 - 1/(|n| + eps)) n; +v/d to w+e, -v/d to w-e
+    const float nx = (fxp - fxm) / dx, ny = (fyp - fym) / dy, nz = (fzp - fzm) / dz;
+    const float nn = sqrtf(nx * nx + ny * ny + nz * nz);
+    // curvature (curvature.py:351-384)
+    const float Lx = ((fxp + fxm) - 2.0f * fc) / ex2;
+    const float Ly = ((fyp + fym) - 2.0f * fc) / ey2;
+    const float Lz = ((fzp + fzm) - 2.0f * fc) / ez2;
+    const double de = (double)nn - 1.0;
+    se += de * de;
+    sc += (double)Lx * Lx + (double)Ly * Ly + (double)Lz * Lz;
+    cnt += 1.0;
+    const float sv = scale * (1.0f - 1.0f / (nn + 1e-8f));
+    const float ex = w_eik != 0.0f ? ((sv * nx) / dx) * w_eik : 0.0f;
+    const float ey = w_eik != 0.0f ? ((sv * ny) / dy) * w_eik : 0.0f;
+    const float ez = w_eik != 0.0f ? ((sv * nz) / dz) * w_eik : 0.0f;
+    const float gx = w_curv != 0.0f ? (scale * (Lx / ex2)) * w_curv : 0.0f;
+    const float gy = w_curv != 0.0f ? (scale * (Ly / ey2)) * w_curv : 0.0f;
+    const float gz = w_curv != 0.0f ? (scale * (Lz / ez2)) * w_curv : 0.0f;
+    // one atomic per target: consecutive lanes hold consecutive z, so every instruction
+    // touches a few contiguous segments
+    if (w_curv != 0.0f) atomicAdd(gg + w, (scale * ((-2.0f * Lx / ex2 - 2.0f * Ly / ey2) - 2.0f * Lz / ez2)) * w_curv);
+    atomicAdd(gg + w + YZ, ex + gx);
+    atomicAdd(gg + w - YZ, -ex + gx);
+    atomicAdd(gg + w + Z, ey + gy);
+    atomicAdd(gg + w - Z, -ey + gy);
+    atomicAdd(gg + w + 1, ez + gz);
+    atomicAdd(gg + w - 1, -ez + gz);
+   }
+  }
+  se = wave_sum_d(se);
+  sc = wave_sum_d(sc);
+  cnt = wave_sum_d(cnt);
+  __shared__ double red[3][kGatherThreads / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) {
+    red[0][wid] = se;
+    red[1][wid] = sc;
+    red[2][wid] = cnt;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0, b = 0, c = 0;
+    for (int k = 0; k < kGatherThreads / 64; ++k) {
+      a += red[0][k];
+      b += red[1][k];
+      c += red[2][k];
+    }
+    partial[3 * blockIdx.x] = a;
+    partial[3 * blockIdx.x + 1] = b;
+    partial[3 * blockIdx.x + 2] = c;
+  }
+}
+
 __global__ void s_amplify_kernel(float* grad_v, float k, int32_t* amplified) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     const float gv = grad_v[0];
@@ -359,6 +530,56 @@ extern "C" int vxn_reg_eik_curv(const float* grid, int X, int Y, int Z, const fl
                      grad_grid, (double*)workspace);
   VXN_LAUNCH_CHECK();
   hipLaunchKernelGGL(reg_finish_kernel, dim3(1), dim3(64), 0, s, (const double*)workspace, kRegBlocks, losses);
+  VXN_LAUNCH_CHECK();
+  return 0;
+}
+
+constexpr int kGatherBlocks = 2048;
+
+extern "C" int vxn_reg_vr_workspace_bytes(int X, int Y, int Z, int64_t* bytes) {
+  VXN_REQUIRE(bytes != nullptr, VXN_ERR_NULL, "bytes is null");
+  VXN_REQUIRE(X >= 1 && Y >= 1 && Z >= 1, VXN_ERR_SHAPE, "bad shape");
+  *bytes = (int64_t)kGatherBlocks * 3 * (int64_t)sizeof(double) + (int64_t)vr_nblocks(X, Y, Z) * 2 * 4;
+  return 0;
+}
+
+extern "C" int vxn_reg_eik_curv_vr(const float* grid, int X, int Y, int Z, const float* eps3, const float* box6,
+                                   float relax_radius, const uint32_t* bitmask, float w_eik, float w_curv,
+                                   float* grad_grid, double* losses, int32_t* counts, void* workspace,
+                                   int64_t workspace_bytes, void* stream) {
+  VXN_REQUIRE(grid && eps3 && box6 && bitmask && grad_grid && losses && counts, VXN_ERR_NULL, "null argument");
+  VXN_REQUIRE(X >= 3 && Y >= 3 && Z >= 3, VXN_ERR_SHAPE, "grid resolution must be >= 3 per axis");
+  VXN_REQUIRE((int64_t)X * Y * Z < ((int64_t)1 << 31), VXN_ERR_SHAPE, "grid too large for int32 V_R entries");
+  for (int a = 0; a < 3; ++a) VXN_REQUIRE(eps3[a] > 0.0f, VXN_ERR_VALUE, "voxel size must be > 0");
+  const int nb = vr_nblocks(X, Y, Z);
+  const int64_t need = (int64_t)kGatherBlocks * 3 * 8 + (int64_t)nb * 2 * 4;
+  VXN_REQUIRE(workspace != nullptr && workspace_bytes >= need, VXN_ERR_WORKSPACE,
+              "regulariser workspace needs %lld bytes", (long long)need);
+  GridDesc g = {};
+  g.grid = grid;
+  g.X = X;
+  g.Y = Y;
+  g.Z = Z;
+  for (int a = 0; a < 3; ++a) g.eps[a] = eps3[a];
+  VrGeom v;
+  const int res[3] = {X, Y, Z};
+  for (int a = 0; a < 3; ++a) {
+    v.mn[a] = box6[a];
+    v.e[a] = (box6[3 + a] - box6[a]) / ((float)res[a] - 1.0f);
+  }
+  v.relax = relax_radius;
+  double* partial = (double*)workspace;
+  int32_t* cpart = (int32_t*)((char*)workspace + (int64_t)kGatherBlocks * 3 * 8);
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(vr_count2_kernel, dim3(nb), dim3(kCompactThreads), 0, s, bitmask, vr_nwords(X, Y, Z), X, Y, Z, v,
+                     cpart);
+  VXN_LAUNCH_CHECK();
+  hipLaunchKernelGGL(vr_count_finish_kernel, dim3(1), dim3(256), 0, s, cpart, nb, counts);
+  VXN_LAUNCH_CHECK();
+  hipLaunchKernelGGL(reg_vr_kernel, dim3(kGatherBlocks), dim3(kGatherThreads), 0, s, g, bitmask, v, counts, w_eik,
+                     w_curv, grad_grid, partial);
+  VXN_LAUNCH_CHECK();
+  hipLaunchKernelGGL(reg_finish_kernel, dim3(1), dim3(64), 0, s, (const double*)partial, kGatherBlocks, losses);
   VXN_LAUNCH_CHECK();
   return 0;
 }

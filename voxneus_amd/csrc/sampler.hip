@@ -117,12 +117,22 @@ __global__ void __launch_bounds__(kSampThreads) upsample_kernel(SamplerDev a) {
         wv[t] = (1.0f - alpha) + 1e-7f;
       }
       __syncthreads();
-      if (t == 0) {  // exclusive cumprod in double, sequential (as torch.cumprod on the CPU)
+      if (t < 64) {
+        // exclusive cumprod in double, sequential (as torch.cumprod on the CPU): wave 0
+        // holds the operands in registers and walks them lane by lane
+        const float w0 = t < m ? wv[t] : 0.0f, w1 = t + 64 < m ? wv[t + 64] : 0.0f;
         double T = 1.0;
+        float c0 = 0.0f, c1 = 0.0f;
         for (int j = 0; j < m; ++j) {
-          cdf[j] = (float)T;
-          T *= (double)wv[j];
+          const float wj = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(j < 64 ? w0 : w1), j & 63));
+          if (t == (j & 63)) {
+            if (j < 64) c0 = (float)T;
+            else c1 = (float)T;
+          }
+          T *= (double)wj;
         }
+        if (t < m) cdf[t] = c0;
+        if (t + 64 < m) cdf[t + 64] = c1;
       }
       __syncthreads();
       if (t < m) {
@@ -145,25 +155,37 @@ __global__ void __launch_bounds__(kSampThreads) upsample_kernel(SamplerDev a) {
         acc8[t] = ((acc[0] + acc[1]) + acc[2]) + acc[3];
       }
       __syncthreads();
-      if (t == 0) {
+      if (t < 64) {
         float tot = 0.0f;
-        if (nvec == 0) {  // rows shorter than one vector: scalar 4-accumulator cascade
-          float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-          const int q = m >> 2;
-          for (int i = 0; i < q; ++i)
-            for (int k = 0; k < 4; ++k) acc[k] = acc[k] + wv[4 * i + k];
-          for (int i = 4 * q; i < m; ++i) acc[0] = acc[0] + wv[i];
-          tot = ((acc[0] + acc[1]) + acc[2]) + acc[3];
-        } else {
-          for (int k = 8 * nvec; k < m; ++k) tot = tot + wv[k];
-          for (int l = 0; l < 8; ++l) tot = tot + acc8[l];
+        if (t == 0) {
+          if (nvec == 0) {  // rows shorter than one vector: scalar 4-accumulator cascade
+            float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+            const int q = m >> 2;
+            for (int i = 0; i < q; ++i)
+              for (int k = 0; k < 4; ++k) acc[k] = acc[k] + wv[4 * i + k];
+            for (int i = 4 * q; i < m; ++i) acc[0] = acc[0] + wv[i];
+            tot = ((acc[0] + acc[1]) + acc[2]) + acc[3];
+          } else {
+            for (int k = 8 * nvec; k < m; ++k) tot = tot + wv[k];
+            for (int l = 0; l < 8; ++l) tot = tot + acc8[l];
+          }
         }
+        tot = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(tot)));
+        // pdf = w / sum (independent divisions, in parallel), then the cdf as a
+        // sequential double cumsum (torch.cumsum on the CPU), walked lane by lane
+        const float q0 = t < m ? wv[t] / tot : 0.0f, q1 = t + 64 < m ? wv[t + 64] / tot : 0.0f;
         double c = 0.0;
-        cdf[0] = 0.0f;
+        float r0 = 0.0f, r1 = 0.0f;
         for (int j = 0; j < m; ++j) {
-          c += (double)(wv[j] / tot);
-          cdf[j + 1] = (float)c;
+          c += (double)__int_as_float(__builtin_amdgcn_readlane(__float_as_int(j < 64 ? q0 : q1), j & 63));
+          if (t == (j & 63)) {
+            if (j < 64) r0 = (float)c;
+            else r1 = (float)c;
+          }
         }
+        if (t == 0) cdf[0] = 0.0f;
+        if (t < m) cdf[t + 1] = r0;
+        if (t + 64 < m) cdf[t + 65] = r1;
       }
       __syncthreads();
       if (t < a.per) {

@@ -426,6 +426,28 @@ def reg_eik_curv(grid3, eps3, vr_list, counts, w_eik, w_curv, grad_grid):
     return losses
 
 
+def reg_eik_curv_vr(grid3, eps3, box, relax_radius, bitmask, w_eik, w_curv, grad_grid, workspace=None,
+                    losses=None, counts=None):
+    """Eikonal + curvature straight from the V_R bitmask (atomic-free gather); returns
+    (FP64 losses [2], int32 counts [2] = {|V_R|, |V_R inside the relaxed sphere|})."""
+    X, Y, Z = grid3.shape
+    dev = grid3.device
+    if losses is None:
+        losses = torch.empty(2, device=dev, dtype=torch.float64)
+    if counts is None:
+        counts = torch.empty(2, device=dev, dtype=torch.int32)
+    if workspace is None:
+        nb = ctypes.c_int64(0)
+        N.check(N.lib().vxn_reg_vr_workspace_bytes(X, Y, Z, ctypes.byref(nb)), "reg_vr_workspace_bytes")
+        workspace = torch.empty(int(nb.value), device=dev, dtype=torch.uint8)
+    N.check(N.lib().vxn_reg_eik_curv_vr(N.dptr(grid3, torch.float32, "grid"), X, Y, Z, N.f3(eps3), N.box6(*box),
+                                        float(relax_radius), N.dptr(bitmask, torch.int32, "bitmask"), float(w_eik),
+                                        float(w_curv), N.dptr(grad_grid, torch.float32, "grad_grid"), N.dptr(losses),
+                                        N.dptr(counts), workspace.data_ptr(), workspace.numel(), _stream(grid3)),
+            "reg_eik_curv_vr")
+    return losses, counts
+
+
 def eikonal_grad_ref(grid3, active, grad3, eps3, weight):
     X, Y, Z = grid3.shape
     a = active.to(torch.int64).contiguous()

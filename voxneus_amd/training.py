@@ -98,12 +98,9 @@ class FusedTrainStep:
         self.loss = torch.zeros(1, device=dev, dtype=torch.float64)
         self.amplified = torch.zeros(1, device=dev, dtype=torch.int32)
         self.bitmask = torch.zeros(Fn.vr_bitmask_words((X, Y, Z)), device=dev, dtype=torch.int32)
-        self.vr_list = torch.empty(X * Y * Z, device=dev, dtype=torch.int32)
         self.vr_counts = torch.zeros(2, device=dev, dtype=torch.int32)
         nb = ctypes.c_int64(0)
-        N.check(N.lib().vxn_vr_workspace_bytes(X, Y, Z, ctypes.byref(nb)))
-        self.vr_ws = torch.empty(max(int(nb.value), 8), device=dev, dtype=torch.uint8)
-        N.check(N.lib().vxn_reg_workspace_bytes(X, Y, Z, ctypes.byref(nb)))
+        N.check(N.lib().vxn_reg_vr_workspace_bytes(X, Y, Z, ctypes.byref(nb)))
         self.reg_ws = torch.empty(int(nb.value), device=dev, dtype=torch.uint8)
         self.reg_losses = torch.zeros(2, device=dev, dtype=torch.float64)
         self.eps3 = Fn.voxel_size_of(self.box, (X, Y, Z))
@@ -156,14 +153,9 @@ class FusedTrainStep:
         self.bitmask.zero_()
         a = self._render_args(rays_o, rays_d, mid.contiguous(), dists.contiguous(), target)
         N.check(N.lib().vxn_render_train(ctypes.byref(a), Fn._stream(mid)), "render_train")
-        N.check(N.lib().vxn_vr_compact(N.dptr(self.bitmask), *self.grid.shape, N.box6(self.box[:3], self.box[3:]),
-                                       float(cfg.relax_radius), N.dptr(self.vr_list), N.dptr(self.vr_counts),
-                                       self.vr_ws.data_ptr(), self.vr_ws.numel(), Fn._stream(mid)), "vr_compact")
-        N.check(N.lib().vxn_reg_eik_curv(N.dptr(self.grid), *self.grid.shape, N.f3(self.eps3),
-                                         N.dptr(self.vr_list), N.dptr(self.vr_counts), float(cfg.w_eik),
-                                         float(cfg.w_curv), N.dptr(self.grad_grid), N.dptr(self.reg_losses),
-                                         self.reg_ws.data_ptr(), self.reg_ws.numel(), Fn._stream(mid)),
-                "reg_eik_curv")
+        # V_R (relaxed-sphere filtered) and both regularisers straight from the bitmask
+        Fn.reg_eik_curv_vr(self.grid, self.eps3, self.box, cfg.relax_radius, self.bitmask, cfg.w_eik, cfg.w_curv,
+                           self.grad_grid, self.reg_ws, self.reg_losses, self.vr_counts)
         self.grad_v = s_grad_to_v(self.grad_s, self.variance, self.s_max).reshape(1).contiguous()
         if cfg.amplify:
             Fn.s_amplify(self.grad_v, cfg.s_amp_k, self.amplified)
@@ -171,9 +163,11 @@ class FusedTrainStep:
                 "amplified": self.amplified, "z_vals": z, "dists": dists, "mid_z_vals": mid}
 
     def active_linear(self) -> torch.Tensor:
-        """Sorted V_R linear indices after the relaxed-sphere filter (one host read)."""
-        n_all, n_in = (int(v) for v in self.vr_counts.tolist())
-        lst = self.vr_list[:n_all]
+        """Sorted V_R linear indices of the last step after the relaxed-sphere filter
+        (compacted from the bitmask on demand; one host read)."""
+        lst, counts = Fn.vr_compact(self.bitmask, tuple(self.grid.shape), self.box, self.cfg.relax_radius)
+        n_all, n_in = (int(v) for v in counts.tolist())
+        lst = lst[:n_all]
         if n_in > 0:
             lst = lst[lst >= 0]
         return lst & 0x7FFFFFFF
