@@ -40,17 +40,19 @@ RAYS = 8192
 SAMPLES = 128
 LATTICE = 320
 # Algorithmic bytes per sample (no cache credit), per kernel of the fused render launch:
-#   render_kernel  grid forward 32 words (8 corners + 24 face neighbours) = 128 B, hash
-#                  forward 16 levels x 8 corners x 2 features = 256 entries = 1024 B FP32
-#                  / 512 B BF16, per-ray streaming (o, d, target, z/dist in, outputs) ~9 B;
+#   hash_features  16 levels x 8 corners x 2 features = 256 table entries = 1024 B FP32 /
+#                  512 B BF16 gathered, 16 feature pairs written (128 B / 64 B);
+#   render_kernel  grid forward 32 words (8 corners + 24 face neighbours) = 128 B, the 16
+#                  feature pairs read back (128 B / 64 B), per-ray streaming ~9 B;
 #   hash_scatter   256 FP32 read-modify-writes = 2048 B;
 #   grid_scatter   32 words read-modify-write = 256 B.
 KERNEL_BYTES = {
-    "fp32": {"render_kernel": 128 + 1024 + 9, "hash_scatter": 2048, "grid_scatter": 256},
-    "bf16": {"render_kernel": 128 + 512 + 9, "hash_scatter": 2048, "grid_scatter": 256},
+    "fp32": {"hash_features": 1024 + 128, "render_kernel": 128 + 128 + 9, "hash_scatter": 2048, "grid_scatter": 256},
+    "bf16": {"hash_features": 512 + 64, "render_kernel": 128 + 64 + 9, "hash_scatter": 2048, "grid_scatter": 256},
 }
 # event index pairs bracketing each kernel in vxn_render_args.kernel_events
-KERNEL_EVENTS = {"render_kernel": (0, 1), "hash_scatter": (1, 2), "grid_scatter": (2, 3), "reductions": (3, 4)}
+KERNEL_EVENTS = {"hash_features": (0, 1), "render_kernel": (1, 2), "hash_scatter": (2, 3), "grid_scatter": (3, 4),
+                 "reductions": (4, 5)}
 # lattice query: 32-word stencil (128 B) in, SDF + normal (16 B) out per lattice point
 LATTICE_BYTES_PER_POINT = 128 + 16
 
@@ -100,7 +102,7 @@ def run_gpu(sc, batches, form, steps, warmup):
     def one(i, timed):
         o, d, t = gb[i % len(gb)]
         if timed:
-            st.kernel_events = N.KernelEvents(5)
+            st.kernel_events = N.KernelEvents(6)
             ev.append(st.kernel_events)
         else:
             st.kernel_events = None

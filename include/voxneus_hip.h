@@ -183,9 +183,9 @@ typedef struct vxn_render_args {
   uint32_t* vr_bitmask; /* nullable [ceil(X*Y*Z/32)]: bit set for all 8 corners of every sample */
   void* workspace;
   int64_t workspace_bytes;
-  /* nullable profiling hook: hipEvent_t[5] recorded on the stream before the render
-     kernel and after the render kernel, the hash scatter, the grid scatter and the
-     reductions (forward: only the first two) */
+  /* nullable profiling hook: hipEvent_t[6] recorded on the stream before the hash-feature
+     gather and after it, the render kernel, the hash scatter, the grid scatter and the
+     reductions (forward: only the first three) */
   void* const* kernel_events;
 } vxn_render_args;
 

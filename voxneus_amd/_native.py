@@ -154,7 +154,7 @@ class KernelEvents:
 
     _hip = None
 
-    def __init__(self, n: int = 5):
+    def __init__(self, n: int = 6):
         if KernelEvents._hip is None:
             h = ctypes.CDLL("libamdhip64.so.7")
             h.hipEventCreate.argtypes = [ctypes.POINTER(c_vp)]

@@ -72,6 +72,9 @@ struct RenderDev {
   double* part_sc;  // [gridDim.x][2]: dL/ds, Huber sum
   float* gfeat;     // [32][R*S] dL/d(hash features), feature-major
   float* ggeo;      // [4][R*S]  dL/dsdf, dL/dgrad
+  // BF16 backward / training: hash features gathered beforehand by hash_features_kernel,
+  // [16][R*S] bf16 pairs packed in uint32
+  void* hfeat;
   // serving batches (forward only): per-ray scene of a packed batch; R is the capacity
   // and the batch size is read from n_rays_dev
   const SceneDev* scenes;
@@ -173,6 +176,8 @@ template <int MODE, int MLPM, int MULTI>
 __global__ void __launch_bounds__(kRT, 2) render_kernel(RenderDev a) {
   constexpr bool kBF = MLPM == kMlpBf16;
   constexpr bool kBFB = kBF && MODE != kFwd;
+  // BF16 backward / training: the hash features come from hash_features_kernel (a.hfeat)
+  constexpr bool kPre = kBF && MODE != kFwd && !MULTI;
   __shared__ Red red;
   __shared__ __align__(16) float preh[64];  // b0 + W0[:, 32:48] . SH(d): the per-ray part of the hidden layer
   __shared__ int nlive_s[2];  // samples per wavefront with a non-zero output gradient
@@ -288,8 +293,10 @@ __global__ void __launch_bounds__(kRT, 2) render_kernel(RenderDev a) {
     const float x1 = fminf(fmaxf((py - G.mn[1]) / G.den[1], 0.0f), 1.0f);
     const float x2 = fminf(fmaxf((pz - G.mn[2]) / G.den[2], 0.0f), 1.0f);
     float x[35];  // 32 hash features, then the unit normal (MLP input columns 0-31, 48-50)
+    if constexpr (!kPre) {
 #pragma unroll
-    for (int l = 0; l < 16; ++l) hash_level_fwd(Hd, l, x0, x1, x2, x[2 * l], x[2 * l + 1]);
+      for (int l = 0; l < 16; ++l) hash_level_fwd(Hd, l, x0, x1, x2, x[2 * l], x[2 * l + 1]);
+    }
     x[32] = nrm[0];
     x[33] = nrm[1];
     x[34] = nrm[2];
@@ -315,8 +322,13 @@ __global__ void __launch_bounds__(kRT, 2) render_kernel(RenderDev a) {
       // Matrix cores, transposed: hidden^T [64 h x 64 samples] = W0x [64 x 48] . X^T,
       // samples of wave half nt = 0/1 in the N dimension (lane j <-> sample 32nt + j).
       uint32_t xp[24];
+      if constexpr (!kPre) {
 #pragma unroll
-      for (int r = 0; r < 16; ++r) xp[r] = pk_bf16(x[2 * r], x[2 * r + 1]);
+        for (int r = 0; r < 16; ++r) xp[r] = pk_bf16(x[2 * r], x[2 * r + 1]);
+      } else {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) xp[r] = valid ? ((const uint32_t*)a.hfeat)[r * NS + si] : 0u;
+      }
       xp[16] = pk_bf16(x[32], x[33]);
       xp[17] = pk_bf16(x[34], 0.0f);
 #pragma unroll
@@ -800,6 +812,29 @@ VXN_DEV void sample_pos(const RenderDev& a, int64_t i, float& px, float& py, flo
   pz = a.o[3 * ray + 2] + a.d[3 * ray + 2] * m;
 }
 
+// Hash features of every sample, level-major and XCD-aware.  Work-groups are dispatched to
+// the 8 XCDs round-robin (work-group b runs on XCD b % 8), so work-group b gathers level
+// (b % 8) + 8 * pass for one 256-sample chunk, pass 0 for all chunks first: each XCD reads
+// one level's table at a time (2^19 entries: 2 MB bf16) instead of every XCD streaming
+// all 16 tables through its L2.  The values are those of hash_level_fwd at the render
+// kernel's sample position (same operations), rounded to bf16 as the BF16 MLP's inputs.
+__global__ void __launch_bounds__(256) hash_features_kernel(RenderDev a, int64_t chunks) {
+  const int64_t NS = a.R * (int64_t)a.S;
+  const int64_t r = blockIdx.x >> 3;
+  const int l = (int)(blockIdx.x & 7) + 8 * (int)(r / chunks);
+  if (l >= a.h.L) return;
+  const int64_t i = (r % chunks) * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= NS) return;
+  float px, py, pz;
+  sample_pos(a, i, px, py, pz);
+  const float x0 = fminf(fmaxf((px - a.g.mn[0]) / a.g.den[0], 0.0f), 1.0f);
+  const float x1 = fminf(fmaxf((py - a.g.mn[1]) / a.g.den[1], 0.0f), 1.0f);
+  const float x2 = fminf(fmaxf((pz - a.g.mn[2]) / a.g.den[2], 0.0f), 1.0f);
+  float f0, f1;
+  hash_level_fwd(a.h, l, x0, x1, x2, f0, f1);
+  ((uint32_t*)a.hfeat)[l * NS + i] = pk_bf16(f0, f1);
+}
+
 // One thread per sample; the hash levels are walked in order so a wavefront handles one
 // level of 64 consecutive samples at a time.  Levels below `dedup_levels` merge runs.
 __global__ void __launch_bounds__(256) hash_scatter_kernel(RenderDev a, int dedup_levels) {
@@ -932,25 +967,50 @@ __global__ void __launch_bounds__(256) grid_scatter_kernel(RenderDev a) {
   const uint64_t key = brick ? (((uint64_t)cn.c0[0] << 42) | ((uint64_t)cn.c0[1] << 21) | (uint64_t)cn.c0[2])
                              : (~0ull - (uint64_t)lane);
   const bool tail = seg_sum<32>(key, v, lane) && brick;
-  if (tail) {
-    const int64_t base = lin3(g, cn.c0[0], cn.c0[1], cn.c0[2]);
-    const int64_t sx = (int64_t)g.Y * g.Z, sy = g.Z;
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      const int ox = (k >> 2) & 1, oy = (k >> 1) & 1, oz = k & 1;
-      atomicAdd(a.grad_grid + base + ox * sx + oy * sy + oz, v[k]);
+  const uint64_t tb = __ballot(tail);
+  if (tb == 0) return;
+  // Flush the runs' bricks column by column: the 32 slots form 12 (x, y) columns of
+  // contiguous z (4 of length 4 through the cell, 8 of length 2 on the x and y faces).
+  // Passes of 16 runs; lane quad q takes the q-th run of the pass and lane 4q + e its
+  // e-th z, so each atomic instruction writes whole columns and merges into few requests.
+  const int64_t sx = (int64_t)g.Y * g.Z, sy = g.Z;
+  const int64_t base_own = brick ? lin3(g, cn.c0[0], cn.c0[1], cn.c0[2]) : 0;
+  const int e = lane & 3, q = lane >> 2;
+  const int ntail = __popcll(tb);
+  for (int pass = 0; 16 * pass < ntail; ++pass) {
+    const int want = 16 * pass + q;
+    const bool act = want < ntail;
+    // lane of the want-th run tail: the want-th set bit of tb
+    int lo = 0, hi = 63;
+    while (lo < hi) {
+      const int md = (lo + hi) >> 1;
+      const uint64_t msk = md == 63 ? ~0ull : ((1ull << (md + 1)) - 1ull);
+      if (__popcll(tb & msk) >= want + 1) hi = md;
+      else lo = md + 1;
     }
+    const int src = act ? lo : lane;
+    const int64_t base = (int64_t)(((uint64_t)(uint32_t)__shfl((int)(base_own >> 32), src, 64) << 32) |
+                                   (uint64_t)(uint32_t)__shfl((int)base_own, src, 64));
+    float* gp = a.grad_grid + base;
+    // through-cell columns (ox, oy): z = -1, 0, 1, 2
 #pragma unroll
-    for (int u = 0; u < 2; ++u)
+    for (int c = 0; c < 4; ++c) {
+      const int ox = c >> 1, oy = c & 1;
+      const float s0 = __shfl(v[slot_face(2, 0, ox, oy)], src, 64), s1 = __shfl(v[slot_corner(ox, oy, 0)], src, 64);
+      const float s2 = __shfl(v[slot_corner(ox, oy, 1)], src, 64), s3 = __shfl(v[slot_face(2, 1, ox, oy)], src, 64);
+      const float val = e == 0 ? s0 : (e == 1 ? s1 : (e == 2 ? s2 : s3));
+      if (act) atomicAdd(gp + ox * sx + oy * sy + (e - 1), val);
+    }
+    // face columns: x = -1 / 2 at y = u, y = -1 / 2 at x = u; z = 0, 1
 #pragma unroll
-      for (int w = 0; w < 2; ++w) {
-        atomicAdd(a.grad_grid + base - sx + u * sy + w, v[slot_face(0, 0, u, w)]);
-        atomicAdd(a.grad_grid + base + 2 * sx + u * sy + w, v[slot_face(0, 1, u, w)]);
-        atomicAdd(a.grad_grid + base + u * sx - sy + w, v[slot_face(1, 0, u, w)]);
-        atomicAdd(a.grad_grid + base + u * sx + 2 * sy + w, v[slot_face(1, 1, u, w)]);
-        atomicAdd(a.grad_grid + base + u * sx + w * sy - 1, v[slot_face(2, 0, u, w)]);
-        atomicAdd(a.grad_grid + base + u * sx + w * sy + 2, v[slot_face(2, 1, u, w)]);
-      }
+    for (int c = 0; c < 4; ++c) {
+      const int side = c >> 1, u = c & 1;
+      const float ax0 = __shfl(v[slot_face(0, side, u, 0)], src, 64), ax1 = __shfl(v[slot_face(0, side, u, 1)], src, 64);
+      const float ay0 = __shfl(v[slot_face(1, side, u, 0)], src, 64), ay1 = __shfl(v[slot_face(1, side, u, 1)], src, 64);
+      const int64_t off = side ? 2 : -1;
+      if (act && e < 2) atomicAdd(gp + off * sx + u * sy + e, e == 0 ? ax0 : ax1);
+      if (act && e >= 2) atomicAdd(gp + u * sx + off * sy + (e - 2), e == 2 ? ay0 : ay1);
+    }
   }
 }
 
@@ -1012,9 +1072,9 @@ static int render_blocks(int64_t R) {
 
 static int64_t align256(int64_t b) { return (b + 255) & ~(int64_t)255; }
 
-// Workspace layout: MLP partials | scalar partials | reduction rows | gfeat | ggeo.
+// Workspace layout: MLP partials | scalar partials | reduction rows | gfeat | ggeo | hfeat.
 struct RenderWs {
-  int64_t off_sc, off_rows, off_feat, off_geo, total;
+  int64_t off_sc, off_rows, off_feat, off_geo, off_hf, total;
 };
 
 static RenderWs render_ws(int64_t R, int S) {
@@ -1025,7 +1085,8 @@ static RenderWs render_ws(int64_t R, int S) {
   w.off_rows = align256(w.off_sc + nb * 2 * (int64_t)sizeof(double));
   w.off_feat = align256(w.off_rows + (int64_t)kRedRows * kNP * (int64_t)sizeof(float));
   w.off_geo = align256(w.off_feat + 32 * ns * (int64_t)sizeof(float));
-  w.total = align256(w.off_geo + kGeo * ns * (int64_t)sizeof(float));
+  w.off_hf = align256(w.off_geo + kGeo * ns * (int64_t)sizeof(float));
+  w.total = align256(w.off_hf + 16 * ns * (int64_t)sizeof(uint32_t));
   return w;
 }
 
@@ -1082,6 +1143,7 @@ static int build_render_dev(const vxn_render_args& A, RenderDev& a, int mode) {
   a.part_sc = nullptr;
   a.gfeat = nullptr;
   a.ggeo = nullptr;
+  a.hfeat = nullptr;
   a.scenes = nullptr;
   a.ray_scene = nullptr;
   a.n_rays_dev = nullptr;
@@ -1096,6 +1158,7 @@ static int build_render_dev(const vxn_render_args& A, RenderDev& a, int mode) {
     a.part_sc = (double*)(ws + w.off_sc);
     a.gfeat = (float*)(ws + w.off_feat);
     a.ggeo = (float*)(ws + w.off_geo);
+    a.hfeat = ws + w.off_hf;
   }
   if (mode == kTrain) {
     VXN_REQUIRE(A.target_rgb != nullptr, VXN_ERR_NULL, "target_rgb is null");
@@ -1121,21 +1184,29 @@ static int launch_render(const vxn_render_args* p, void* stream) {
   hipStream_t s = (hipStream_t)stream;
   hipEvent_t const* ev = (hipEvent_t const*)p->kernel_events;
   if (ev) VXN_HIP(hipEventRecord(ev[0], s));
+  if (MODE != kFwd && p->mlp_mode == VXN_MLP_BF16_MFMA) {
+    const int64_t ns = p->n_rays * (int64_t)p->n_samples;
+    const int64_t chunks = (ns + 255) / 256;
+    const unsigned nb_hf = (unsigned)(8 * ((a.h.L + 7) / 8) * chunks);
+    hipLaunchKernelGGL(hash_features_kernel, dim3(nb_hf), dim3(256), 0, s, a, chunks);
+    VXN_LAUNCH_CHECK();
+  }
+  if (ev) VXN_HIP(hipEventRecord(ev[1], s));
   if (p->mlp_mode == VXN_MLP_BF16_MFMA)
     hipLaunchKernelGGL((render_kernel<MODE, kMlpBf16, 0>), dim3(nb), dim3(kRT), 0, s, a);
   else
     hipLaunchKernelGGL((render_kernel<MODE, kMlpF32, 0>), dim3(nb), dim3(kRT), 0, s, a);
   VXN_LAUNCH_CHECK();
-  if (ev) VXN_HIP(hipEventRecord(ev[1], s));
+  if (ev) VXN_HIP(hipEventRecord(ev[2], s));
   if (MODE != kFwd) {
     const int64_t ns = p->n_rays * (int64_t)p->n_samples;
     const int sb = (int)((ns + 255) / 256);
     hipLaunchKernelGGL(hash_scatter_kernel, dim3(sb), dim3(256), 0, s, a, dedup_levels(a.h));
     VXN_LAUNCH_CHECK();
-    if (ev) VXN_HIP(hipEventRecord(ev[2], s));
+    if (ev) VXN_HIP(hipEventRecord(ev[3], s));
     hipLaunchKernelGGL(grid_scatter_kernel, dim3(sb), dim3(256), 0, s, a);
     VXN_LAUNCH_CHECK();
-    if (ev) VXN_HIP(hipEventRecord(ev[3], s));
+    if (ev) VXN_HIP(hipEventRecord(ev[4], s));
     const RenderWs w = render_ws(p->n_rays, p->n_samples);
     float* rows = (float*)((char*)p->workspace + w.off_rows);
     hipLaunchKernelGGL(render_reduce1_kernel, dim3((kNP + 255) / 256, kRedRows), dim3(256), 0, s, a.part_mlp, nb,
@@ -1144,7 +1215,7 @@ static int launch_render(const vxn_render_args* p, void* stream) {
     hipLaunchKernelGGL(render_reduce2_kernel, dim3((kNP + 1 + 255) / 256), dim3(256), 0, s, rows, a.part_sc, nb,
                        p->grad_mlp, p->grad_s, MODE == kTrain ? p->loss : nullptr, a.inv_n3);
     VXN_LAUNCH_CHECK();
-    if (ev) VXN_HIP(hipEventRecord(ev[4], s));
+    if (ev) VXN_HIP(hipEventRecord(ev[5], s));
   }
   return 0;
 }

@@ -105,7 +105,7 @@ class FusedTrainStep:
         self.reg_losses = torch.zeros(2, device=dev, dtype=torch.float64)
         self.eps3 = Fn.voxel_size_of(self.box, (X, Y, Z))
         self._render_ws = None
-        self.kernel_events = None  # optional N.KernelEvents(5) recorded between the render kernels
+        self.kernel_events = None  # optional N.KernelEvents(6) recorded between the render kernels
         self.level_w = None
         if cfg.active_levels is not None:
             from .hash_grid import level_weights
