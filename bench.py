@@ -40,14 +40,16 @@ RAYS = 8192
 SAMPLES = 128
 LATTICE = 320
 # Algorithmic bytes per sample (no cache credit), per kernel of the fused render launch:
-#   hash_features  16 levels x 8 corners x 2 features = 256 table entries = 1024 B FP32 /
-#                  512 B BF16 gathered, 16 feature pairs written (128 B / 64 B);
-#   render_kernel  grid forward 32 words (8 corners + 24 face neighbours) = 128 B, the 16
-#                  feature pairs read back (128 B / 64 B), per-ray streaming ~9 B;
+#   hash_features  (BF16 form) 16 levels x 8 corners x 2 features = 256 bf16 table entries
+#                  = 512 B gathered, 16 feature pairs written (64 B);
+#   render_kernel  grid forward 32 words (8 corners + 24 face neighbours) = 128 B, the hash
+#                  features (BF16: 64 B read back; FP32: 1024 B gathered in the kernel),
+#                  per-ray streaming ~9 B;
 #   hash_scatter   256 FP32 read-modify-writes = 2048 B;
 #   grid_scatter   32 words read-modify-write = 256 B.
 KERNEL_BYTES = {
-    "fp32": {"hash_features": 1024 + 128, "render_kernel": 128 + 128 + 9, "hash_scatter": 2048, "grid_scatter": 256},
+    # FP32 form: the render kernel gathers the hash features itself (no hash_features launch)
+    "fp32": {"render_kernel": 128 + 1024 + 9, "hash_scatter": 2048, "grid_scatter": 256},
     "bf16": {"hash_features": 512 + 64, "render_kernel": 128 + 64 + 9, "hash_scatter": 2048, "grid_scatter": 256},
 }
 # event index pairs bracketing each kernel in vxn_render_args.kernel_events
@@ -329,6 +331,8 @@ def main():
         log(f"gpu {form}: {r}")
         kernels = {}
         for k, ms in r["kernel_ms"].items():
+            if k not in KERNEL_BYTES[form] and k != "reductions":
+                continue
             bps = KERNEL_BYTES[form].get(k)
             ach = None if bps is None else bps * ns / (ms * 1e-3) / 1e9
             kernels[k] = {"ms": ms, "bytes_per_sample": bps, "achieved_GBps": ach}

@@ -136,16 +136,22 @@ __global__ void __launch_bounds__(256) voxel_corners_kernel(GridDesc g, const fl
 
 // Lattice query: one thread per lattice point; consecutive threads walk z, so the
 // 8-corner and 24-neighbour stencil loads of a wavefront fall on a few cache lines.
+// One lattice row (ix, iy) per work-group, z across the threads.  XCD-aware: work-groups
+// are dispatched to the 8 XCDs round-robin (work-group b on XCD b % 8), and XCD k walks
+// the rows of its own slab of ix in order, so each XCD's L2 streams one eighth of the
+// grid (plus a one-slab halo) instead of every XCD re-reading all of it.
 __global__ void __launch_bounds__(256) lattice_query_kernel(GridDesc g, int NX, int NY, int NZ, float lx0, float ly0,
                                                             float lz0, float lsx, float lsy, float lsz,
                                                             float* __restrict__ sdf, float* __restrict__ normal) {
-  const int64_t n = (int64_t)NX * NY * NZ;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const int iz = (int)(i % NZ);
-    const int64_t r = i / NZ;
-    const int iy = (int)(r % NY);
-    const int ix = (int)(r / NY);
-    const float px = lx0 + (float)ix * lsx, py = ly0 + (float)iy * lsy, pz = lz0 + (float)iz * lsz;
+  const int xcd = blockIdx.x & 7, j = blockIdx.x >> 3;
+  const int slab = (NX + 7) / 8;
+  const int ix = xcd * slab + j / NY, iy = j - (j / NY) * NY;
+  if (ix >= NX || j / NY >= slab) return;
+  const float px = lx0 + (float)ix * lsx, py = ly0 + (float)iy * lsy;
+  const int64_t row = ((int64_t)ix * NY + iy) * NZ;
+  for (int iz = threadIdx.x; iz < NZ; iz += blockDim.x) {
+    const int64_t i = row + iz;
+    const float pz = lz0 + (float)iz * lsz;
     const GsStencil s = gs_stencil(g, px, py, pz);
     sdf[i] = gs_eval(g, s);
     const Corners c = corner_decomp(g, px, py, pz);
@@ -292,8 +298,9 @@ extern "C" int vxn_lattice_query(const float* grid, int X, int Y, int Z, const f
   VXN_REQUIRE(NX >= 2 && NY >= 2 && NZ >= 2, VXN_ERR_SHAPE, "lattice must be >= 2 per axis");
   const GridDesc g = make_grid_desc(grid, X, Y, Z, box6);
   const float* lb = lattice_box6;
-  const int64_t n = (int64_t)NX * NY * NZ;
-  hipLaunchKernelGGL(lattice_query_kernel, dim3(blocks_for(n, 256, 1 << 20)), dim3(256), 0, (hipStream_t)stream, g,
+  VXN_REQUIRE((int64_t)8 * ((NX + 7) / 8) * NY < ((int64_t)1 << 31), VXN_ERR_SHAPE, "lattice too large");
+  const unsigned nb = (unsigned)(8 * ((NX + 7) / 8) * NY);
+  hipLaunchKernelGGL(lattice_query_kernel, dim3(nb), dim3(256), 0, (hipStream_t)stream, g,
                      NX, NY, NZ, lb[0], lb[1], lb[2], (lb[3] - lb[0]) / (float)(NX - 1), (lb[4] - lb[1]) / (float)(NY - 1),
                      (lb[5] - lb[2]) / (float)(NZ - 1), sdf, normal);
   VXN_LAUNCH_CHECK();

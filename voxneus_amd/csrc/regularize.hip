@@ -383,38 +383,56 @@ __global__ void __launch_bounds__(kGatherThreads) reg_vr_kernel(GridDesc g, cons
      for (int64_t ww = w0 + threadIdx.x; ww <= w1; ww += kGatherThreads) marked = marked || bits[ww] != 0u;
    }
    if (!__syncthreads_or(marked)) continue;
-   for (int wz = threadIdx.x; wz < Z; wz += kGatherThreads) {
+   for (int z0 = threadIdx.x & ~63; z0 < Z; z0 += kGatherThreads) {
+    // one wavefront covers z0 .. z0 + 63 of the row
+    const int lane = threadIdx.x & 63, wz = z0 + lane;
     const int64_t w = (int64_t)row * Z + wz;
-    if (!member(w, wx, wy, wz)) continue;
-    const float fc = f[w];
-    const float fxp = f[w + YZ], fxm = f[w - YZ], fyp = f[w + Z], fym = f[w - Z], fzp = f[w + 1], fzm = f[w - 1];
-    // eikonal (eikonal.py:181-218): v = scale (1 - 1/(|n| + eps)) n; +v/d to w+e, -v/d to w-e
-    const float nx = (fxp - fxm) / dx, ny = (fyp - fym) / dy, nz = (fzp - fzm) / dz;
-    const float nn = sqrtf(nx * nx + ny * ny + nz * nz);
-    // curvature (curvature.py:351-384)
-    const float Lx = ((fxp + fxm) - 2.0f * fc) / ex2;
-    const float Ly = ((fyp + fym) - 2.0f * fc) / ey2;
-    const float Lz = ((fzp + fzm) - 2.0f * fc) / ez2;
-    const double de = (double)nn - 1.0;
-    se += de * de;
-    sc += (double)Lx * Lx + (double)Ly * Ly + (double)Lz * Lz;
-    cnt += 1.0;
-    const float sv = scale * (1.0f - 1.0f / (nn + 1e-8f));
-    const float ex = w_eik != 0.0f ? ((sv * nx) / dx) * w_eik : 0.0f;
-    const float ey = w_eik != 0.0f ? ((sv * ny) / dy) * w_eik : 0.0f;
-    const float ez = w_eik != 0.0f ? ((sv * nz) / dz) * w_eik : 0.0f;
-    const float gx = w_curv != 0.0f ? (scale * (Lx / ex2)) * w_curv : 0.0f;
-    const float gy = w_curv != 0.0f ? (scale * (Ly / ey2)) * w_curv : 0.0f;
-    const float gz = w_curv != 0.0f ? (scale * (Lz / ez2)) * w_curv : 0.0f;
-    // one atomic per target: consecutive lanes hold consecutive z, so every instruction
-    // touches a few contiguous segments
-    if (w_curv != 0.0f) atomicAdd(gg + w, (scale * ((-2.0f * Lx / ex2 - 2.0f * Ly / ey2) - 2.0f * Lz / ez2)) * w_curv);
-    atomicAdd(gg + w + YZ, ex + gx);
-    atomicAdd(gg + w - YZ, -ex + gx);
-    atomicAdd(gg + w + Z, ey + gy);
-    atomicAdd(gg + w - Z, -ey + gy);
-    atomicAdd(gg + w + 1, ez + gz);
-    atomicAdd(gg + w - 1, -ez + gz);
+    const bool mem = wz < Z && member(w, wx, wy, wz);
+    float cz = 0.0f, zp = 0.0f, zm = 0.0f, xp = 0.0f, xm = 0.0f, yp = 0.0f, ym = 0.0f;
+    if (mem) {
+      const float fc = f[w];
+      const float fxp = f[w + YZ], fxm = f[w - YZ], fyp = f[w + Z], fym = f[w - Z], fzp = f[w + 1], fzm = f[w - 1];
+      // eikonal (eikonal.py:181-218): v = scale (1 - 1/(|n| + eps)) n; +v/d to w+e, -v/d to w-e
+      const float nx = (fxp - fxm) / dx, ny = (fyp - fym) / dy, nz = (fzp - fzm) / dz;
+      const float nn = sqrtf(nx * nx + ny * ny + nz * nz);
+      // curvature (curvature.py:351-384)
+      const float Lx = ((fxp + fxm) - 2.0f * fc) / ex2;
+      const float Ly = ((fyp + fym) - 2.0f * fc) / ey2;
+      const float Lz = ((fzp + fzm) - 2.0f * fc) / ez2;
+      const double de = (double)nn - 1.0;
+      se += de * de;
+      sc += (double)Lx * Lx + (double)Ly * Ly + (double)Lz * Lz;
+      cnt += 1.0;
+      const float sv = scale * (1.0f - 1.0f / (nn + 1e-8f));
+      const float ex = w_eik != 0.0f ? ((sv * nx) / dx) * w_eik : 0.0f;
+      const float ey = w_eik != 0.0f ? ((sv * ny) / dy) * w_eik : 0.0f;
+      const float ez = w_eik != 0.0f ? ((sv * nz) / dz) * w_eik : 0.0f;
+      const float gx = w_curv != 0.0f ? (scale * (Lx / ex2)) * w_curv : 0.0f;
+      const float gy = w_curv != 0.0f ? (scale * (Ly / ey2)) * w_curv : 0.0f;
+      const float gz = w_curv != 0.0f ? (scale * (Lz / ez2)) * w_curv : 0.0f;
+      cz = w_curv != 0.0f ? (scale * ((-2.0f * Lx / ex2 - 2.0f * Ly / ey2) - 2.0f * Lz / ez2)) * w_curv : 0.0f;
+      xp = ex + gx;
+      xm = -ex + gx;
+      yp = ey + gy;
+      ym = -ey + gy;
+      zp = ez + gz;
+      zm = -ez + gz;
+    }
+    // the z line: w's own term plus its neighbours' -z / +z terms, summed across lanes so
+    // one atomic per position covers all three (the wave's two end terms go on their own)
+    const bool mp = __shfl_up((int)mem, 1, 64) != 0, mn = __shfl_down((int)mem, 1, 64) != 0;
+    const float from_below = __shfl_up(zp, 1, 64), from_above = __shfl_down(zm, 1, 64);
+    const bool below = lane > 0 && mp, above = lane < 63 && mn;
+    const float line = (cz + (below ? from_below : 0.0f)) + (above ? from_above : 0.0f);
+    if (mem || below || above) atomicAdd(gg + w, line);
+    if (mem && lane == 63) atomicAdd(gg + w + 1, zp);
+    if (mem && lane == 0) atomicAdd(gg + w - 1, zm);
+    if (mem) {
+      atomicAdd(gg + w + YZ, xp);
+      atomicAdd(gg + w - YZ, xm);
+      atomicAdd(gg + w + Z, yp);
+      atomicAdd(gg + w - Z, ym);
+    }
    }
   }
   se = wave_sum_d(se);
