@@ -123,6 +123,37 @@ def run_gpu(sc, batches, form, steps, warmup):
     return {"ms_per_step": dt / steps * 1e3, "rays_per_s": RAYS * steps / dt, "kernel_ms": kern, "loss": loss}
 
 
+def run_full_render(form):
+    """Config 4: every pixel of the 49 hemisphere views at 800x600 (f = 1200), 64 + 4x16
+    samples per ray, on one resident scene (320^3 FP32 grid, 16 x 2^19 hash), through the
+    serving kernels with all views admitted at once."""
+    import torch
+
+    from voxneus_amd import _native as N
+    from voxneus_amd import scenes as S
+    from voxneus_amd.serving import RenderRequest, Server, make_resident_scene
+
+    dev = torch.device("cuda:0")
+    sc = make_resident_scene(LATTICE, 19, 4, dev, table_dtype=torch.bfloat16 if form == "bf16" else torch.float32)
+    mode = N.VXN_MLP_BF16_MFMA if form == "bf16" else N.VXN_MLP_FP32
+    srv = Server([sc], mlp_mode=mode, ray_capacity=1 << 17, point_capacity=1 << 10, ray_arena=49 * 800 * 600,
+                 point_arena=1 << 10, max_segs=64)
+    cams = S.hemisphere_cameras(49)
+    reqs = [RenderRequest(0, cams[v], 800, 600, 1200.0, 1200.0, 400.0, 300.0) for v in range(49)]
+    srv.serve(reqs[:1], realtime=False)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = srv.serve(reqs, realtime=False)
+    dt = time.perf_counter() - t0
+    rays = sum(r.rows for r in res)
+    out = {"views": 49, "image": "800x600", "rays": rays, "samples_per_ray": SAMPLES, "seconds": dt,
+           "rays_per_s": rays / dt, "samples_per_s": rays * SAMPLES / dt, "grid": f"{LATTICE}^3 FP32",
+           "hash": "16 x 2^19 " + ("bf16" if form == "bf16" else "fp32")}
+    del srv, sc
+    torch.cuda.empty_cache()
+    return out
+
+
 def run_lattice(reps=10):
     """SDF + normal at every point of a 320^3 lattice over the box of a 320^3 grid."""
     import torch
@@ -300,7 +331,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-rays", type=int, default=128)
     ap.add_argument("--no-error-check", action="store_true", help="skip the FP64 RGB error check (profiling runs)")
-    ap.add_argument("--no-lattice", action="store_true", help="skip the 320^3 lattice-query measurement")
+    ap.add_argument("--no-lattice", action="store_true",
+                    help="skip the 320^3 measurements (lattice query, 49-view 800x600 render)")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop latency measurement")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB device copy (3 dispatches): the known-size kernel that scales "
@@ -349,6 +381,10 @@ def main():
     lat = None if a.no_lattice else run_lattice()
     if lat:
         log(f"lattice: {lat}")
+    full = None
+    if not a.no_lattice:
+        full = {form: run_full_render(form) for form in forms}
+        log(f"full-scale render: {full}")
     serving = None
     if not a.no_serving:
         serving = {}
@@ -376,6 +412,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(dom, main_form),
                      "kernel": dom, "kernel_ms": kms[dom], "bytes_per_launch": nbytes},
         "grid_gather": lat,
+        "full_scale_render": full,
         "serving": serving,
         "cpu_baseline": cpu,
     }

@@ -43,24 +43,32 @@ enum { VXN_MLP_FP32 = 0, VXN_MLP_BF16_MFMA = 1 };
 #define VXN_MLP_HIDDEN 64
 #define VXN_MLP_PARAMS (64 * 51 + 64 + 3 * 64 + 3) /* W0, b0, W1, b1 flattened in that order */
 
+/* Surface gradient of the dense grid (DenseSDFGrid.query_normals gradient_mode,
+ * models/sdf_grid.py:122-160): the Eq. 7 interpolated central difference or the Eq. 4
+ * derivative of the trilinear interpolant (the ablation baseline).                   */
+#define VXN_GRADIENT_INTERPOLATED 0
+#define VXN_GRADIENT_ANALYTICAL 1
+
 const char* vxn_version(void);
 const char* vxn_last_error(void);
 
 /* ---- Dense SDF grid queries -------------------------------------------------------
  * grid: [X,Y,Z] float32, z fastest.  box6 (host): xyz_min[3], xyz_max[3].            */
 
-/* Trilinear SDF (F.grid_sample, border, align_corners) + Eq. 7 interpolated gradient,
- * its normalisation, and optionally the 8 corner indices [n,8,3] int64 and trilinear
- * weights [n,8].  Replaces DenseSDFGrid.query_sdf + query_normals
- * (models/sdf_grid.py:90-160, geometry/gradients.py:103-139). Outputs are nullable.  */
+/* Trilinear SDF (F.grid_sample, border, align_corners) + the surface gradient in
+ * gradient_mode (VXN_GRADIENT_*), its normalisation, and optionally the 8 corner indices
+ * [n,8,3] int64 and trilinear weights [n,8].  Replaces DenseSDFGrid.query_sdf +
+ * query_normals (models/sdf_grid.py:90-160, geometry/gradients.py:103-220). Outputs are
+ * nullable.                                                                           */
 int vxn_grid_query_fwd(const float* grid, int X, int Y, int Z, const float* box6, const float* xyz, int64_t n,
                        float* sdf, float* grad, float* normal, int64_t* corners, float* corner_weights,
-                       void* stream);
+                       int32_t gradient_mode, void* stream);
 
 /* Backward of the above w.r.t. the grid: d_sdf [n] and/or d_grad [n,3] (dL w.r.t. the
  * un-normalised gradient) are scattered into grad_grid (accumulated).               */
 int vxn_grid_query_bwd(const float* grid, int X, int Y, int Z, const float* box6, const float* xyz, int64_t n,
-                       const float* d_sdf, const float* d_grad, float* grad_grid, void* stream);
+                       const float* d_sdf, const float* d_grad, float* grad_grid, int32_t gradient_mode,
+                       void* stream);
 
 /* Eq. 7 on continuous grid coordinates [n,3] (already in [0,res-1]); eps3 (host) is the
  * voxel size.  Replaces _backend.gradient_interp_forward / _backward
@@ -71,17 +79,26 @@ int vxn_gradient_interp_fwd(const float* grid, int X, int Y, int Z, const float*
 int vxn_gradient_interp_bwd(const float* grad_out, const float* coords, int64_t n, float* grad_grid, int X, int Y,
                             int Z, const float* eps3, void* stream);
 
+/* Eq. 4 analytical gradient [n,3] and its unit normal [n,3] (both nullable) on continuous
+ * grid coordinates.  Replaces compute_analytical_gradient (geometry/gradients.py:142-220);
+ * the backward accumulates dL/dgrad [n,3] into grad_grid, as autograd through its corner
+ * gather does.                                                                        */
+int vxn_analytical_gradient_fwd(const float* grid, int X, int Y, int Z, const float* coords, int64_t n,
+                                const float* eps3, float* grad, float* normal, void* stream);
+int vxn_analytical_gradient_bwd(const float* grad_out, const float* coords, int64_t n, float* grad_grid, int X,
+                                int Y, int Z, const float* eps3, void* stream);
+
 /* Corner indices [n,8,3] int64 and trilinear weights [n,8] (both nullable) of continuous
  * grid coordinates [n,3].  Replaces get_voxel_corners_and_weights
  * (geometry/coordinates.py:55-124).                                                   */
 int vxn_voxel_corners(const float* coords, int64_t n, int X, int Y, int Z, int64_t* corners, float* corner_weights,
                       void* stream);
 
-/* SDF value and interpolated unit normal at every point of an NX*NY*NZ lattice spanning
- * lattice_box6 (host), written [NX,NY,NZ] and [NX,NY,NZ,3].  The mesh-extraction query
- * of geometry/mesh.py:80-114 (compute_mesh_normals) over a full lattice.             */
+/* SDF value and unit normal (in gradient_mode) at every point of an NX*NY*NZ lattice
+ * spanning lattice_box6 (host), written [NX,NY,NZ] and [NX,NY,NZ,3].  The mesh-extraction
+ * query of geometry/mesh.py:80-114 (compute_mesh_normals) over a full lattice.       */
 int vxn_lattice_query(const float* grid, int X, int Y, int Z, const float* box6, int NX, int NY, int NZ,
-                      const float* lattice_box6, float* sdf, float* normal, void* stream);
+                      const float* lattice_box6, float* sdf, float* normal, int32_t gradient_mode, void* stream);
 
 /* ---- Multiresolution hash encoding ------------------------------------------------
  * table: [L, 2^log2T, F] float32 (dtype 0) or bfloat16 (dtype 1); F must be 2.
@@ -187,6 +204,7 @@ typedef struct vxn_render_args {
      gather and after it, the render kernel, the hash scatter, the grid scatter and the
      reductions (forward: only the first three) */
   void* const* kernel_events;
+  int32_t gradient_mode; /* VXN_GRADIENT_*: the normal of every sample (VoxNeuS gradient_mode) */
 } vxn_render_args;
 
 int vxn_render_workspace_bytes(const vxn_render_args* a, int64_t* bytes);
@@ -247,6 +265,53 @@ int vxn_reg_eik_curv_vr(const float* grid, int X, int Y, int Z, const float* eps
  * (training/scheduler.py:14-33) without its host read-back.                           */
 int vxn_s_amplify(float* grad_v, float k, int32_t* amplified, void* stream);
 
+/* ---- Optimizer updates ------------------------------------------------------------
+ * float32 tensors of n elements, 16-byte aligned; step is the update count including
+ * this one (the reference's state["step"] after its increment).                      */
+
+/* Dense and masked (grad != 0 only) Adam of the native backend: _backend.adam_upd /
+ * masked_adam_upd (cuda/adam_upd.cpp:48-72, cuda/adam_upd_kernel.cu:6-82), with
+ * step_size = lr sqrt(1 - beta2^t) / (1 - beta1^t) formed in float as there.  The masked
+ * form is SparseGridAdam.step on a HIP tensor (training/optimizer.py:75-90); n_updated
+ * (device, nullable) accumulates the number of elements updated (its
+ * last_updated_count, without the host read-back).                                   */
+int vxn_adam_upd(float* param, const float* grad, float* exp_avg, float* exp_avg_sq, int64_t n, int32_t step,
+                 float beta1, float beta2, float lr, float eps, void* stream);
+int vxn_masked_adam_upd(float* param, const float* grad, float* exp_avg, float* exp_avg_sq, int64_t n, int32_t step,
+                        float beta1, float beta2, float lr, float eps, int32_t* n_updated, void* stream);
+
+/* SparseGridAdam.step with weight decay (training/optimizer.py:92-113: the masked
+ * PyTorch formula, eps added after dividing by sqrt(1 - beta2^t)).  The scalars are the
+ * Python floats (double), rounded to float where they meet a tensor, as PyTorch does.  */
+int vxn_sparse_adam_decay(float* param, const float* grad, float* exp_avg, float* exp_avg_sq, int64_t n,
+                          int64_t step, double beta1, double beta2, double lr, double eps, double weight_decay,
+                          int32_t* n_updated, void* stream);
+
+/* torch.optim.Adam (no weight decay, no amsgrad) over up to 8 tensors in one launch: the
+ * network optimizer of create_optimizer (training/optimizer.py:143-150; hash table, MLP
+ * and variance, each its own parameter group and learning rate), in the order of
+ * PyTorch's multi-tensor update.  A tensor whose lr_flag (device int32, nullable) is
+ * nonzero uses lr * lr_flag_scale: the s-amplified variance learning rate
+ * (trainer.py:332) chosen on the device.  param_min clamps the updated parameter from
+ * below (the variance floor, trainer.py:346-353; -INFINITY for none).  param_bf16
+ * (nullable) receives the updated parameter rounded to bfloat16 (a bf16 working copy of
+ * an FP32 master).                                                                     */
+typedef struct vxn_adam_tensor {
+  float* param;
+  const float* grad;
+  float* exp_avg;
+  float* exp_avg_sq;
+  int64_t n;
+  double lr;
+  const int32_t* lr_flag;
+  double lr_flag_scale;
+  float param_min;
+  void* param_bf16;
+} vxn_adam_tensor;
+
+int vxn_adam_step(const vxn_adam_tensor* tensors, int32_t n_tensors, int64_t step, double beta1, double beta2,
+                  double eps, void* stream);
+
 /* ---- Serving: many resident scenes, packed request batches -------------------------
  * The reference renders one scene in 2048-ray chunks (scripts/render.py:35-54:
  * near_far_from_sphere + VoxNeuS.render, no_grad, det sampler) and queries SDF and
@@ -271,6 +336,7 @@ typedef struct vxn_scene_desc {
   const float* mlp;           /* [VXN_MLP_PARAMS] float32 */
   float s;                    /* NeuS sharpness */
   float sphere_radius;        /* compositing inside-sphere radius (1.0 in the reference) */
+  int32_t gradient_mode;      /* VXN_GRADIENT_* of the scene's model */
 } vxn_scene_desc;
 
 /* Size of one packed scene record of the device table. */

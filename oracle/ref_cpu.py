@@ -123,14 +123,23 @@ def interpolate_gradients(grid5, coords, voxel_size, grid_res):
     return grad, grad / (norm + 1e-8), corners, weights
 
 
-def query_normals(grid5, box: GridBox, xyz):
-    """sdf_grid.py:122-149 (interpolated mode)."""
+def query_normals(grid5, box: GridBox, xyz, gradient_mode="interpolated"):
+    """sdf_grid.py:122-160: Eq. 7 ("interpolated") or Eq. 4 ("analytical"), with the
+    corner indices and trilinear weights of either; any other mode is a ValueError."""
     coords = world_to_grid_index(xyz, box.xyz_min, box.xyz_max, box.res.to(xyz.dtype))
-    return interpolate_gradients(grid5, coords, box.voxel_size, box.res)
+    mode = gradient_mode.lower()
+    if mode == "interpolated":
+        return interpolate_gradients(grid5, coords, box.voxel_size, box.res)
+    if mode == "analytical":
+        grad, normal = compute_analytical_gradient(grid5, coords, box.voxel_size, box.res)
+        corners, weights = get_voxel_corners_and_weights(coords, box.res)
+        return grad, normal, corners, weights
+    raise ValueError(f"Unknown gradient mode: {gradient_mode}")
 
 
 def compute_analytical_gradient(grid5, coords, voxel_size, grid_res):
-    """gradients.py:142-220 (Eq. 4), used to pin the continuity property."""
+    """gradients.py:142-220 (Eq. 4): d/dx of the trilinear interpolant of the cell's 8
+    corner values, the ablation baseline of VoxNeuS(gradient_mode="analytical")."""
     corners, w = get_voxel_corners_and_weights(coords, grid_res)
     data = grid5.squeeze(0).squeeze(0)
     rx, ry, rz = int(grid_res[0]), int(grid_res[1]), int(grid_res[2])
@@ -584,6 +593,66 @@ def amplify_s_gradient(grad_v, k=5.0):
 
 
 # ----------------------------------------------------------------------------------------
+# Optimizers (Code/voxneus/training/optimizer.py, cuda/adam_upd_kernel.cu, torch.optim.Adam)
+# Every elementwise step is its own PyTorch op, so each rounds to float32 once.
+# ----------------------------------------------------------------------------------------
+
+def adam_native_step(p, g, m, v, step, beta1, beta2, lr, eps, masked):
+    """adam_upd_kernel.cu:6-38 per element, with the host step size of :50 / :70 formed in
+    float: step_size = lr * sqrtf(1 - powf(b2, t)) / (1 - powf(b1, t)).  Updates p, m, v
+    (float32 tensors) in place; ``masked`` leaves elements with g == 0 untouched."""
+    import numpy as np
+    f = np.float32
+    b1, b2, lr32, eps32 = f(beta1), f(beta2), f(lr), f(eps)
+    one = f(1.0)
+    step_size = lr32 * np.sqrt(one - np.power(b2, f(step))) / (one - np.power(b1, f(step)))
+    sel = (g != 0.0) if masked else torch.ones_like(g, dtype=torch.bool)
+    omb1, omb2 = one - b1, one - b2
+    mn = torch.mul(m, float(b1)) + torch.mul(g, float(omb1))
+    vn = torch.mul(v, float(b2)) + torch.mul(torch.mul(g, float(omb2)), g)
+    upd = torch.div(torch.mul(mn, float(step_size)), torch.sqrt(vn) + float(eps32))
+    pn = p - upd
+    m.copy_(torch.where(sel, mn, m))
+    v.copy_(torch.where(sel, vn, v))
+    p.copy_(torch.where(sel, pn, p))
+    return int(sel.sum())
+
+
+def sparse_adam_masked_step(p, g, m, v, step, lr, beta1, beta2, eps, weight_decay):
+    """optimizer.py:92-113 (the masked PyTorch formula, the path taken with weight decay):
+    only elements with g != 0 change; eps is added after dividing by sqrt(1 - b2^t)."""
+    mask = g != 0.0
+    n = int(mask.sum())
+    if n > 0:
+        ga = g[mask]
+        if weight_decay != 0.0:
+            ga = ga + weight_decay * p[mask]
+        ma = m[mask] * beta1 + (1.0 - beta1) * ga
+        m[mask] = ma
+        va = v[mask] * beta2 + (1.0 - beta2) * (ga * ga)
+        v[mask] = va
+        step_size = lr / (1.0 - beta1 ** step)
+        denom = (torch.sqrt(va) / math.sqrt(1.0 - beta2 ** step)) + eps
+        p[mask] -= step_size * (ma / denom)
+    return n
+
+
+def torch_adam_step(p, g, m, v, step, lr, beta1=0.9, beta2=0.999, eps=1e-8):
+    """torch.optim.Adam (no weight decay, no amsgrad) in the order of its multi-tensor
+    update, the one used for device tensors (torch/optim/adam.py, _multi_tensor_adam):
+    m = lerp(m, g, 1 - b1); v = v * b2 + (1 - b2) * (g * g);
+    p = p + (-lr / bc1) * (m / (sqrt(v) / sqrt(bc2) + eps)), scalars from Python floats."""
+    w = 1.0 - beta1
+    m.copy_(m + torch.mul(g - m, w))  # lerp with weight < 0.5
+    v.copy_(torch.mul(v, beta2))
+    v.copy_(v + torch.mul(torch.mul(g, g), 1.0 - beta2))
+    bc1 = 1.0 - beta1 ** step
+    bc2 = 1.0 - beta2 ** step
+    denom = torch.div(torch.sqrt(v), bc2 ** 0.5) + eps
+    p.copy_(p + torch.mul(torch.div(m, denom), (lr / bc1) * -1))
+
+
+# ----------------------------------------------------------------------------------------
 # Assembled model: forward_points, render, train step (voxneus.py, trainer.py)
 # ----------------------------------------------------------------------------------------
 
@@ -593,10 +662,11 @@ class RefModel:
     def __init__(self, grid, embeddings, W0, b0, W1, b1, variance=0.3, s_max=300.0,
                  xyz_min=(-1.0, -1.0, -1.0), xyz_max=(1.0, 1.0, 1.0), hash_base_resolution=16,
                  hash_max_resolution=2048.0, occupancy_mask=None, occupancy_res=128, dtype=torch.float32,
-                 mlp_form="fp32"):
+                 mlp_form="fp32", gradient_mode="interpolated"):
         t = lambda a: torch.as_tensor(a).to(dtype).clone()
         self.dtype = dtype
         self.mlp_form = mlp_form
+        self.gradient_mode = gradient_mode
         self.grid = t(grid).reshape(1, 1, *grid.shape[-3:]).requires_grad_(True)
         self.embeddings = t(embeddings).requires_grad_(True)
         self.W0, self.b0 = t(W0).requires_grad_(True), t(b0).requires_grad_(True)
@@ -624,7 +694,7 @@ class RefModel:
     def forward_points(self, xyz, view_dirs, active_levels=None):
         """voxneus.py:115-167."""
         sdf = self.query_sdf(xyz)
-        grad, normal, corners, weights = query_normals(self.grid, self.box, xyz)
+        grad, normal, corners, weights = query_normals(self.grid, self.box, xyz, self.gradient_mode)
         norm_xyz = (xyz - self.box.xyz_min) / (self.box.xyz_max - self.box.xyz_min + 1e-8)
         feats = hash_encode(torch.clamp(norm_xyz, min=0.0, max=1.0), self.embeddings, self.level_res, active_levels)
         mlp = radiance_mlp_bf16 if self.mlp_form == "bf16" else radiance_mlp

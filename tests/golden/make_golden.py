@@ -3,7 +3,9 @@
 Run once, in the build container only (the reference checkout is not present on GPU
 hosts):
 
-    python tests/golden/make_golden.py /root/reference/Code
+    python tests/golden/make_golden.py /root/reference/Code [fixture-name ...]
+
+(with fixture names, only those fixtures are written)
 
 It imports the reference's pure-PyTorch path (``force_pytorch=True``; no compiled backend
 exists on this host) on the CPU, feeds it the seeded synthetic scenes from
@@ -15,6 +17,8 @@ replaced by a no-op so the gradients it produced can be read back).
 Fixtures:
   * ``unit_grid.npz``    - query_sdf / query_normals (sdf, grad, normal, corners, weights)
                            and the grid gradient of a random linear loss, 64^3 sphere grid;
+  * ``unit_grid_analytical.npz`` - the same with gradient_mode="analytical" (Eq. 4:
+                           compute_analytical_gradient) and its grid gradient;
   * ``unit_hash.npz``    - hash encoding (16 levels, T=2^15, per-level scale 1.5) with and
                            without the lambda mask, plus embedding gradients;
   * ``unit_mlp.npz``     - SH embedding + radiance MLP outputs and parameter gradients;
@@ -68,7 +72,7 @@ def sparse(a):
     return nz, a[nz].astype(a.dtype)
 
 
-def main(ref_code: str):
+def main(ref_code: str, only=()):
     sys.path.insert(0, ref_code)
     torch.set_num_threads(1)
     from voxneus.models.sdf_grid import DenseSDFGrid
@@ -101,9 +105,20 @@ def main(ref_code: str):
     loss = (grad * torch.from_numpy(r_g)).sum() + (sdf * torch.from_numpy(r_s)).sum()
     loss.backward()
     gi, gv = sparse(det(g.grid.grad))
-    save("unit_grid.npz", inputs_sha=sha(sc.grid, pts), pts=pts, r_g=r_g, r_s=r_s, sdf=det(sdf), grad=det(grad),
-         normal=det(normal), corners=det(corners).astype(np.int16), weights=det(weights),
-         grid_grad_idx=gi, grid_grad_val=gv)
+    if not only or "unit_grid.npz" in only:
+        save("unit_grid.npz", inputs_sha=sha(sc.grid, pts), pts=pts, r_g=r_g, r_s=r_s, sdf=det(sdf),
+             grad=det(grad), normal=det(normal), corners=det(corners).astype(np.int16), weights=det(weights),
+             grid_grad_idx=gi, grid_grad_val=gv)
+    # the Eq. 4 analytical gradient (gradient_mode="analytical", ablation E1)
+    g.grid.grad = None
+    grad_a, normal_a, _, _ = g.query_normals(xyz, gradient_mode="analytical")
+    (grad_a * torch.from_numpy(r_g)).sum().backward()
+    ai, av = sparse(det(g.grid.grad))
+    if not only or "unit_grid_analytical.npz" in only:
+        save("unit_grid_analytical.npz", inputs_sha=sha(sc.grid, pts), pts=pts, r_g=r_g, grad=det(grad_a),
+             normal=det(normal_a), grid_grad_idx=ai, grid_grad_val=av)
+    if only and not any(n not in ("unit_grid.npz", "unit_grid_analytical.npz") for n in only):
+        return
 
     # ------------------------------------------------------------------ unit_hash
     maxres = scenes.hash_max_resolution(16, 1.5, 16)
@@ -341,4 +356,4 @@ def main(ref_code: str):
 
 
 if __name__ == "__main__":
-    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference/Code")
+    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference/Code", tuple(sys.argv[2:]))

@@ -49,15 +49,22 @@ def test_struct_layouts_match_header_field_order():
 def test_host_validation_errors():
     L = N.lib()
     box = N.box6((-1, -1, -1), (1, 1, 1))
-    rc = L.vxn_grid_query_fwd(None, 8, 8, 8, box, None, 1, None, None, None, None, None, None)
+    rc = L.vxn_grid_query_fwd(None, 8, 8, 8, box, None, 1, None, None, None, None, None, 0, None)
     assert rc == -1 and b"grid" in L.vxn_last_error()
     rc = L.vxn_grid_query_fwd(ctypes.c_void_p(16), 1, 8, 8, box, ctypes.c_void_p(16), 1, None, None, None, None, None,
-                              None)
+                              0, None)
     assert rc == -2
     bad = N.box6((1, -1, -1), (-1, 1, 1))
     rc = L.vxn_grid_query_fwd(ctypes.c_void_p(16), 8, 8, 8, bad, ctypes.c_void_p(16), 1, None, None, None, None, None,
-                              None)
+                              0, None)
     assert rc == -3
+    # gradient_mode outside {0 interpolated, 1 analytical}: a value error before any launch
+    rc = L.vxn_grid_query_fwd(ctypes.c_void_p(16), 8, 8, 8, box, ctypes.c_void_p(16), 1, None, None, None, None, None,
+                              2, None)
+    assert rc == -3 and b"gradient_mode" in L.vxn_last_error()
+    rc = L.vxn_lattice_query(ctypes.c_void_p(16), 8, 8, 8, box, 4, 4, 4, box, ctypes.c_void_p(16),
+                             ctypes.c_void_p(16), -1, None)
+    assert rc == -3 and b"gradient_mode" in L.vxn_last_error()
     res = (ctypes.c_int32 * 16)(*([16] * 16))
     rc = L.vxn_hash_fwd(ctypes.c_void_p(16), 0, 16, 15, 4, res, None, ctypes.c_void_p(16), 1, ctypes.c_void_p(16),
                         None)

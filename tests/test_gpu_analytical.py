@@ -1,0 +1,180 @@
+"""GPU parity of gradient_mode="analytical" (Eq. 4, the derivative of the trilinear
+interpolant; geometry/gradients.py:142-220) against the reference's golden vectors and
+the CPU oracle: the standalone queries, the fused render forward and backward, the fused
+training step and the mesh lattice query.  Gradients of the forward pass are bit-exact
+(same operation order as the reference's PyTorch CPU ops); grid gradients are held to
+1e-5 of the largest reference magnitude (atomics reorder the sums).
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import ref_cpu as R
+from tests.conftest import load_golden
+from voxneus_amd import _native as N
+from voxneus_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+DEV = torch.device("cuda")
+BOX = ((-1.0, -1.0, -1.0), (1.0, 1.0, 1.0))
+BOX6 = (-1.0, -1.0, -1.0, 1.0, 1.0, 1.0)
+ANA = N.VXN_GRADIENT_ANALYTICAL
+
+
+def rel_err(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-30))
+
+
+@pytest.fixture(scope="module")
+def Fn():
+    from voxneus_amd import functional
+    return functional
+
+
+def _mlp_flat(sc):
+    m = sc.mlp
+    return torch.cat([torch.from_numpy(m[k]).reshape(-1) for k in ("W0", "b0", "W1", "b1")])
+
+
+def test_grid_query_analytical_matches_reference(Fn):
+    g = load_golden("unit_grid_analytical.npz")
+    sc = scenes.config1_scene(seed=0)
+    grid = torch.from_numpy(sc.grid).to(DEV).requires_grad_(True)
+    xyz = torch.from_numpy(g["pts"]).to(DEV)
+    o = Fn.grid_query_raw(grid.detach(), BOX, xyz, False, True, True, gradient_mode=ANA)
+    assert np.array_equal(o["grad"].cpu().numpy(), g["grad"])
+    assert np.abs(o["normal"].cpu().numpy() - g["normal"]).max() < 1e-6
+    _, gv = Fn.GridQueryFunction.apply(grid, xyz, BOX, ANA)
+    (gv * torch.from_numpy(g["r_g"]).to(DEV)).sum().backward()
+    ref = np.zeros(grid.numel(), np.float32)
+    ref[g["grid_grad_idx"]] = g["grid_grad_val"]
+    assert rel_err(grid.grad.cpu().numpy().reshape(-1), ref) < 1e-5
+
+
+def test_sdf_grid_query_normals_modes(Fn):
+    """DenseSDFGrid.query_normals(gradient_mode=...) as the reference's module exposes it."""
+    from voxneus_amd.sdf_grid import DenseSDFGrid
+    sc = scenes.config1_scene(seed=0)
+    sg = DenseSDFGrid(64).to(DEV)
+    with torch.no_grad():
+        sg.grid.copy_(torch.from_numpy(sc.grid).reshape(sg.grid.shape))
+    xyz = torch.from_numpy(load_golden("unit_grid_analytical.npz")["pts"]).to(DEV)
+    grad, normal, corners, weights = sg.query_normals(xyz, gradient_mode="Analytical")
+    rg, rn, rc, rw = R.query_normals(torch.from_numpy(sc.grid)[None, None], R.GridBox(64), xyz.cpu(), "analytical")
+    assert np.array_equal(grad.detach().cpu().numpy(), rg.numpy())
+    assert torch.equal(corners.cpu(), rc) and torch.equal(weights.cpu(), rw)
+    assert (normal.detach().cpu() - rn).abs().max().item() < 1e-6
+    with pytest.raises(ValueError):
+        sg.query_normals(xyz, gradient_mode="central")
+
+
+def test_analytical_gradient_reference_shape(Fn):
+    """compute_analytical_gradient on a random 32^3 grid, coords in [0, res-1.001]
+    (the pattern of test_cuda_parity.py:31-70), forward and grid gradient."""
+    torch.manual_seed(43)
+    res = 32
+    grid = torch.randn(1, 1, res, res, res)
+    coords = torch.rand(4096, 3) * (res - 1.001)
+    coords[:4] = torch.tensor([[0.0, 0.0, 0.0], [res - 1.0, res - 1.0, res - 1.0], [-3.0, 40.0, 7.5],
+                               [15.0, 16.0, 17.0]])
+    vs = torch.tensor([2.0 / (res - 1), 3.0 / (res - 1), 0.5 / (res - 1)])
+    gres = torch.tensor([res] * 3, dtype=torch.float32)
+    gr = grid.clone().requires_grad_(True)
+    ref, ref_n = R.compute_analytical_gradient(gr, coords, vs, gres)
+    gg = grid.to(DEV).requires_grad_(True)
+    out, nrm = Fn.compute_analytical_gradient(gg, coords.to(DEV), vs)
+    assert np.array_equal(out.detach().cpu().numpy(), ref.detach().numpy())
+    assert (nrm.detach().cpu() - ref_n.detach()).abs().max().item() < 1e-6
+    w = torch.randn(4096, 3)
+    (ref * w).sum().backward()
+    (out * w.to(DEV)).sum().backward()
+    assert rel_err(gg.grad.cpu().numpy(), gr.grad.numpy()) < 1e-5
+
+
+def test_render_analytical_matches_oracle(Fn):
+    """The fused render with gradient_mode=analytical vs the oracle's RefModel in the same
+    mode on the same z values: per-sample gradients bit-exact, composited outputs and all
+    parameter gradients to 1e-5."""
+    g = load_golden("step_c1.npz")
+    sc = scenes.config1_scene(seed=0)
+    m = R.RefModel(sc.grid, sc.embeddings, **sc.mlp, hash_max_resolution=sc.hash_max_resolution,
+                   gradient_mode="analytical")
+    o, d = torch.from_numpy(g["rays_o"]), torch.from_numpy(g["rays_d"])
+    zs = tuple(torch.from_numpy(g[k]) for k in ("z_vals", "dists", "mid"))
+    rng = np.random.default_rng(7)
+    rr = [torch.from_numpy(rng.normal(size=sh).astype(np.float32)) for sh in ((64, 3), (64, 1), (64, 3), (64, 1))]
+    out = m.render(o, d, None, None, zs=zs)
+    L = (out["rgb"] * rr[0]).sum() + (out["depth"] * rr[1]).sum() + (out["normal"] * rr[2]).sum() + (
+        out["opacity"] * rr[3]).sum()
+    L.backward()
+    cfg = Fn.RenderConfig(BOX6, m.level_res, gradient_mode=ANA)
+    grid = torch.from_numpy(sc.grid).to(DEV).requires_grad_(True)
+    emb = torch.from_numpy(sc.embeddings).to(DEV).requires_grad_(True)
+    mlp = _mlp_flat(sc).to(DEV).requires_grad_(True)
+    v = torch.tensor(0.3, device=DEV, requires_grad=True)
+    s = torch.clamp(torch.exp(v * 10.0), min=1e-4, max=300.0)
+    outs = Fn.RenderFunction.apply(grid, emb, mlp, s, o.to(DEV), d.to(DEV), zs[2].to(DEV), zs[1].to(DEV), cfg, True,
+                                   None)
+    rgb, depth, normal, opac, w, alpha, T, sdf_s, grad_s, nrm_s, corners = outs
+    assert np.array_equal(grad_s.detach().cpu().numpy(), out["grad_samples"].detach().numpy())
+    assert (nrm_s.detach().cpu() - out["normal_samples"].detach()).abs().max().item() < 1e-6
+    assert torch.equal(corners.cpu(), out["corner_indices"])
+    for k, v_ in (("rgb", rgb), ("depth", depth), ("opacity", opac), ("weights", w)):
+        assert (v_.detach().cpu() - out[k].detach().reshape(v_.shape)).abs().max().item() < 1e-5, k
+    # the analytical normals differ from the interpolated ones on this scene
+    o7 = Fn.RenderFunction.apply(grid.detach(), emb.detach(), mlp.detach(), s.detach(), o.to(DEV), d.to(DEV),
+                                 zs[2].to(DEV), zs[1].to(DEV), Fn.RenderConfig(BOX6, m.level_res), False, None)
+    assert (o7[9] - nrm_s).abs().max().item() > 1e-3
+    Lg = sum((outs[i] * rr[k].to(DEV)).sum() for k, i in enumerate((0, 1, 2, 3)))
+    Lg.backward()
+    assert rel_err(grid.grad.cpu().numpy(), m.grid.grad.numpy()[0, 0]) < 1e-5
+    assert rel_err(emb.grad.cpu().numpy(), m.embeddings.grad.numpy()) < 1e-5
+    ref_mlp = torch.cat([m.W0.grad.reshape(-1), m.b0.grad, m.W1.grad.reshape(-1), m.b1.grad]).numpy()
+    assert rel_err(mlp.grad.cpu().numpy(), ref_mlp) < 1e-5
+    assert abs(v.grad.item() - m.variance.grad.item()) <= 1e-5 * max(1e-3, abs(m.variance.grad.item()))
+
+
+@pytest.mark.parametrize("name", ["step_c1.npz", "step_c2.npz"])
+def test_fused_train_step_analytical_matches_oracle(name):
+    """The whole training step (sampler, render, Huber loss, backward, V_R, eikonal and
+    curvature, s-amplification) with VoxNeuS(gradient_mode="analytical")."""
+    from voxneus_amd.training import FusedTrainStep, StepConfig
+    g = load_golden(name)
+    sc = scenes.config1_scene(seed=0) if name == "step_c1.npz" else scenes.make_scene("multi", 128, 17, seed=2)
+    res = R.hash_level_resolutions(16, 16, sc.hash_max_resolution)
+    m = R.RefModel(sc.grid, sc.embeddings, **sc.mlp, hash_max_resolution=sc.hash_max_resolution,
+                   gradient_mode="analytical")
+    o, d, tgt = (torch.from_numpy(g[k]) for k in ("rays_o", "rays_d", "target"))
+    zs = tuple(torch.from_numpy(g[k]) for k in ("z_vals", "dists", "mid"))
+    ref = m.train_step(o, d, tgt, w_eik=0.1, w_curv=5e-4, zs=zs)
+    st = FusedTrainStep(torch.from_numpy(sc.grid).to(DEV), torch.from_numpy(sc.embeddings).to(DEV),
+                        _mlp_flat(sc).to(DEV), torch.tensor(0.3, device=DEV), res,
+                        cfg=StepConfig(w_eik=0.1, w_curv=5e-4, gradient_mode="analytical"))
+    out = st.step(o.to(DEV), d.to(DEV), tgt.to(DEV))
+    assert np.array_equal(out["z_vals"].cpu().numpy(), g["z_vals"])
+    assert np.array_equal(st.active_linear().cpu().numpy(), ref["active_linear"].numpy())
+    assert abs(out["loss"].item() - ref["loss"].item()) <= 1e-6 * abs(ref["loss"].item())
+    le, lc = out["reg_losses"].tolist()
+    assert abs(le - float(ref["loss_eik"])) <= 1e-5 * abs(float(ref["loss_eik"]))
+    assert abs(lc - float(ref["loss_curv"])) <= 1e-5 * abs(float(ref["loss_curv"]))
+    assert rel_err(st.grad_grid.cpu().numpy(), m.grid.grad.numpy()[0, 0]) < 1e-5
+    assert rel_err(st.grad_table.cpu().numpy(), m.embeddings.grad.numpy()) < 1e-5
+    ref_mlp = torch.cat([m.W0.grad.reshape(-1), m.b0.grad, m.W1.grad.reshape(-1), m.b1.grad]).numpy()
+    assert rel_err(st.grad_mlp.cpu().numpy(), ref_mlp) < 1e-5
+    assert bool(out["amplified"].item()) == bool(ref["amplified"])
+    assert abs(st.grad_v.item() - m.variance.grad.item()) <= 1e-5 * abs(m.variance.grad.item())
+
+
+def test_lattice_query_analytical_equals_point_query(Fn):
+    sc = scenes.config1_scene(seed=0)
+    grid = torch.from_numpy(sc.grid).to(DEV)
+    n = 19
+    sdf, nrm = Fn.lattice_query(grid, BOX, n, n, n, BOX6, gradient_mode=ANA)
+    pts = -1.0 + torch.arange(n, dtype=torch.float32) * torch.tensor(2.0 / (n - 1), dtype=torch.float32)
+    P = torch.stack(torch.meshgrid(pts, pts, pts, indexing="ij"), -1).reshape(-1, 3).to(DEV)
+    o = Fn.grid_query_raw(grid, BOX, P, True, False, True, gradient_mode=ANA)
+    assert torch.equal(sdf.reshape(-1), o["sdf"]) and torch.equal(nrm.reshape(-1, 3), o["normal"])

@@ -18,8 +18,10 @@ DEV = torch.device("cuda")
 
 @pytest.fixture(scope="module")
 def scenes():
-    spec = [(64, 15, 1, torch.float32), (96, 16, 2, torch.bfloat16), (48, 15, 3, torch.float32)]
-    return [make_resident_scene(r, t, seed, DEV, table_dtype=dt) for r, t, seed, dt in spec]
+    # the third scene's model uses the Eq. 4 analytical gradient
+    spec = [(64, 15, 1, torch.float32, "interpolated"), (96, 16, 2, torch.bfloat16, "interpolated"),
+            (48, 15, 3, torch.float32, "analytical")]
+    return [make_resident_scene(r, t, seed, DEV, table_dtype=dt, gradient_mode=gm) for r, t, seed, dt, gm in spec]
 
 
 def _requests(n, n_scenes, seed, sizes=((24, 16), (40, 30), (7, 5)), arrival_gap=0.0):
@@ -42,12 +44,13 @@ def _requests(n, n_scenes, seed, sizes=((24, 16), (40, 30), (7, 5)), arrival_gap
 
 def _standalone(sc, req, mlp_mode):
     if isinstance(req, QueryRequest):
-        out = Fn.grid_query_raw(sc.grid, sc.box, req.points, want_sdf=True, want_grad=False, want_normal=True)
+        out = Fn.grid_query_raw(sc.grid, sc.box, req.points, want_sdf=True, want_grad=False, want_normal=True,
+                                gradient_mode=sc.gradient_mode)
         return {"sdf": out["sdf"].reshape(-1, 1), "normal": out["normal"]}
     o, d = pinhole_rays(req, DEV)
     z, dists, mid = Fn.upsample(o, d, None, None, sc.grid, sc.box, 64, 64, 4, False, sc.occ_mask, sc.occ_box,
                                 bound_radius=1.0)
-    cfg = Fn.RenderConfig(sc.box, sc.level_res, mlp_mode=mlp_mode)
+    cfg = Fn.RenderConfig(sc.box, sc.level_res, mlp_mode=mlp_mode, gradient_mode=sc.gradient_mode)
     out = Fn.RenderFunction.apply(sc.grid, sc.table, sc.mlp, torch.tensor(sc.s, device=DEV), o, d, mid, dists, cfg,
                                   False, None)
     return {"rgb": out[0], "depth": out[1]}

@@ -120,6 +120,72 @@ def test_interpolated_gradient_continuity():
     assert di < 1e-3 and da > 50 * di
 
 
+def test_unit_grid_analytical_matches_reference():
+    """gradient_mode="analytical" (Eq. 4) against the reference's own outputs on the same
+    grid and points, and its grid gradient through the corner gather."""
+    g = load_golden("unit_grid_analytical.npz")
+    sc = scenes.config1_scene(seed=0)
+    assert sha(sc.grid, g["pts"]) == str(g["inputs_sha"])
+    grid = _grid5(sc.grid).requires_grad_(True)
+    box = R.GridBox(64)
+    xyz = torch.from_numpy(g["pts"])
+    grad, normal, corners, weights = R.query_normals(grid, box, xyz, "analytical")
+    assert np.array_equal(grad.detach().numpy(), g["grad"])
+    assert np.array_equal(normal.detach().numpy(), g["normal"])
+    _, _, ci, wi = R.query_normals(grid, box, xyz, "interpolated")
+    assert torch.equal(corners, ci) and torch.equal(weights, wi)
+    (grad * torch.from_numpy(g["r_g"])).sum().backward()
+    gg = grid.grad.numpy().reshape(-1)
+    ref = np.zeros_like(gg)
+    ref[g["grid_grad_idx"]] = g["grid_grad_val"]
+    assert np.array_equal(np.flatnonzero(gg), np.sort(g["grid_grad_idx"][g["grid_grad_val"] != 0]))
+    np.testing.assert_allclose(gg, ref, rtol=1e-6, atol=1e-6)
+    with pytest.raises(ValueError):
+        R.query_normals(grid, box, xyz, "central")
+
+
+def test_analytical_gradient_of_trilinear_field_is_exact():
+    """A trilinear field f = c0 + c1 x + c2 y + c3 z + c4 xy + c5 yz + c6 xz + c7 xyz is its
+    own interpolant, so Eq. 4 must equal its exact derivative everywhere (a swapped corner,
+    sign or axis fails this)."""
+    c = torch.tensor([0.1, 0.3, -0.5, 0.8, 0.7, -0.4, 0.25, 1.3], dtype=torch.float64)
+    res = 9
+    box = R.GridBox(res, dtype=torch.float64)
+    ax = torch.linspace(-1, 1, res, dtype=torch.float64)
+    X, Y, Z = torch.meshgrid(ax, ax, ax, indexing="ij")
+    f = lambda x, y, z: (c[0] + c[1] * x + c[2] * y + c[3] * z + c[4] * x * y + c[5] * y * z + c[6] * x * z
+                         + c[7] * x * y * z)
+    grid = f(X, Y, Z)[None, None]
+    gen = torch.Generator().manual_seed(5)
+    xyz = torch.rand(500, 3, dtype=torch.float64, generator=gen) * 1.98 - 0.99
+    x, y, z = xyz.unbind(-1)
+    exact = torch.stack([c[1] + c[4] * y + c[6] * z + c[7] * y * z, c[2] + c[4] * x + c[5] * z + c[7] * x * z,
+                         c[3] + c[5] * y + c[6] * x + c[7] * x * y], -1)
+    grad, normal, _, _ = R.query_normals(grid, box, xyz, "analytical")
+    assert torch.allclose(grad, exact, atol=1e-10)
+    assert torch.allclose(normal, exact / (exact.norm(dim=-1, keepdim=True) + 1e-8), atol=1e-10)
+
+
+def test_analytical_gradient_is_the_derivative_of_grid_sample():
+    """Brute force: inside a cell, Eq. 4 is the derivative of F.grid_sample's trilinear SDF,
+    taken here by central differences of the SDF query in float64."""
+    gen = torch.Generator().manual_seed(9)
+    res = 7
+    box = R.GridBox(res, dtype=torch.float64)
+    grid = torch.randn(1, 1, res, res, res, dtype=torch.float64, generator=gen)
+    cell = torch.randint(0, res - 1, (64, 3), generator=gen).to(torch.float64)
+    frac = torch.rand(64, 3, dtype=torch.float64, generator=gen) * 0.8 + 0.1
+    xyz = -1.0 + (cell + frac) * box.voxel_size
+    grad, _, _, _ = R.query_normals(grid, box, xyz, "analytical")
+    h = 1e-6
+    fd = torch.zeros_like(xyz)
+    for a in range(3):
+        e = torch.zeros(3, dtype=torch.float64)
+        e[a] = h
+        fd[:, a] = (R.query_sdf(grid, box, xyz + e) - R.query_sdf(grid, box, xyz - e)).squeeze(-1) / (2 * h)
+    assert torch.allclose(grad, fd, rtol=1e-6, atol=1e-6)
+
+
 def test_unit_mlp_matches_reference():
     g = load_golden("unit_mlp.npz")
     sc = scenes.config1_scene(seed=0)

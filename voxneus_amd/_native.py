@@ -20,6 +20,17 @@ VXN_DTYPE_F32 = 0
 VXN_DTYPE_BF16 = 1
 VXN_MLP_FP32 = 0
 VXN_MLP_BF16_MFMA = 1
+VXN_GRADIENT_INTERPOLATED = 0
+VXN_GRADIENT_ANALYTICAL = 1
+GRADIENT_MODES = {"interpolated": VXN_GRADIENT_INTERPOLATED, "analytical": VXN_GRADIENT_ANALYTICAL}
+
+
+def gradient_mode_code(mode: str) -> int:
+    """models/sdf_grid.py:142-160: the mode name, case-insensitive; anything else is a ValueError."""
+    try:
+        return GRADIENT_MODES[mode.lower()]
+    except (KeyError, AttributeError):
+        raise ValueError(f"Unknown gradient mode: {mode}") from None
 MAX_LEVELS = 32
 MLP_PARAMS = 64 * 51 + 64 + 3 * 64 + 3
 
@@ -55,7 +66,7 @@ class RenderArgs(ctypes.Structure):
         ("target_rgb", c_vp), ("huber_delta", c_f),
         ("grad_grid", c_vp), ("grad_table", c_vp), ("grad_mlp", c_vp), ("grad_s", c_vp), ("loss", c_vp),
         ("vr_bitmask", c_vp), ("workspace", c_vp), ("workspace_bytes", c_i64),
-        ("kernel_events", c_vp),
+        ("kernel_events", c_vp), ("gradient_mode", c_i32),
     ]
 
 
@@ -65,7 +76,7 @@ class SceneDesc(ctypes.Structure):
         ("occ_mask", c_vp), ("occ_res", c_i32), ("occ_box", c_f * 6),
         ("table", c_vp), ("table_dtype", c_i32), ("n_levels", c_i32), ("log2_T", c_i32), ("n_features", c_i32),
         ("level_res", c_i32 * MAX_LEVELS), ("level_w", c_f * MAX_LEVELS),
-        ("mlp", c_vp), ("s", c_f), ("sphere_radius", c_f),
+        ("mlp", c_vp), ("s", c_f), ("sphere_radius", c_f), ("gradient_mode", c_i32),
     ]
 
 
@@ -81,16 +92,33 @@ class ServeRenderArgs(ctypes.Structure):
     ]
 
 
+class AdamTensor(ctypes.Structure):
+    _fields_ = [
+        ("param", c_vp), ("grad", c_vp), ("exp_avg", c_vp), ("exp_avg_sq", c_vp), ("n", c_i64),
+        ("lr", ctypes.c_double), ("lr_flag", c_vp), ("lr_flag_scale", ctypes.c_double), ("param_min", c_f),
+        ("param_bf16", c_vp),
+    ]
+
+
 # name -> (restype, argtypes); every entry point returns int except the two strings
 _SIGS = {
     "vxn_version": (ctypes.c_char_p, []),
     "vxn_last_error": (ctypes.c_char_p, []),
-    "vxn_grid_query_fwd": (c_i32, [c_vp, c_i32, c_i32, c_i32, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
-    "vxn_grid_query_bwd": (c_i32, [c_vp, c_i32, c_i32, c_i32, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp]),
+    "vxn_grid_query_fwd": (c_i32, [c_vp, c_i32, c_i32, c_i32, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_i32,
+                                   c_vp]),
+    "vxn_grid_query_bwd": (c_i32, [c_vp, c_i32, c_i32, c_i32, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_i32, c_vp]),
     "vxn_gradient_interp_fwd": (c_i32, [c_vp, c_i32, c_i32, c_i32, c_vp, c_i64, c_vp, c_vp, c_vp]),
     "vxn_gradient_interp_bwd": (c_i32, [c_vp, c_vp, c_i64, c_vp, c_i32, c_i32, c_i32, c_vp, c_vp]),
+    "vxn_analytical_gradient_fwd": (c_i32, [c_vp, c_i32, c_i32, c_i32, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp]),
+    "vxn_analytical_gradient_bwd": (c_i32, [c_vp, c_vp, c_i64, c_vp, c_i32, c_i32, c_i32, c_vp, c_vp]),
     "vxn_voxel_corners": (c_i32, [c_vp, c_i64, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp]),
-    "vxn_lattice_query": (c_i32, [c_vp, c_i32, c_i32, c_i32, c_vp, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp]),
+    "vxn_lattice_query": (c_i32, [c_vp, c_i32, c_i32, c_i32, c_vp, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_i32,
+                                  c_vp]),
+    "vxn_adam_upd": (c_i32, [c_vp, c_vp, c_vp, c_vp, c_i64, c_i32, c_f, c_f, c_f, c_f, c_vp]),
+    "vxn_masked_adam_upd": (c_i32, [c_vp, c_vp, c_vp, c_vp, c_i64, c_i32, c_f, c_f, c_f, c_f, c_vp, c_vp]),
+    "vxn_sparse_adam_decay": (c_i32, [c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, ctypes.c_double, ctypes.c_double,
+                                      ctypes.c_double, ctypes.c_double, ctypes.c_double, c_vp, c_vp]),
+    "vxn_adam_step": (c_i32, [c_vp, c_i32, c_i64, ctypes.c_double, ctypes.c_double, ctypes.c_double, c_vp]),
     "vxn_hash_fwd": (c_i32, [c_vp, c_i32, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_i64, c_vp, c_vp]),
     "vxn_hash_bwd": (c_i32, [c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp]),
     "vxn_upsample": (c_i32, [ctypes.POINTER(SamplerArgs), c_vp]),

@@ -19,7 +19,7 @@ __global__ void __launch_bounds__(256) grid_query_fwd_kernel(GridDesc g, const f
     const Corners c = corner_decomp(g, px, py, pz);
     if (grad || normal) {
       float gv[3];
-      eq7_gradient(g, c, gv);
+      grid_gradient(g, c, gv);
       if (grad) {
         grad[3 * i] = gv[0];
         grad[3 * i + 1] = gv[1];
@@ -59,7 +59,7 @@ __global__ void __launch_bounds__(256) grid_query_bwd_kernel(GridDesc g, const f
     if (d_grad) {
       const Corners c = corner_decomp(g, px, py, pz);
       const float dg[3] = {d_grad[3 * i], d_grad[3 * i + 1], d_grad[3 * i + 2]};
-      eq7_scatter(g, c, dg, grad_grid);
+      grid_gradient_scatter(g, c, dg, grad_grid);
     }
   }
 }
@@ -80,6 +80,7 @@ VXN_DEV Corners corners_from_coords(const GridDesc& g, float cx, float cy, float
     c.c0[a] = i0;
     c.c1[a] = i1;
     d[a] = cc - (float)i0;
+    c.d[a] = d[a];
   }
   const float ox = 1.0f - d[0], oy = 1.0f - d[1], oz = 1.0f - d[2];
   c.w[0] = ox * oy * oz;
@@ -93,25 +94,36 @@ VXN_DEV Corners corners_from_coords(const GridDesc& g, float cx, float cy, float
   return c;
 }
 
-__global__ void __launch_bounds__(256) gradient_interp_fwd_kernel(GridDesc g, const float* __restrict__ coords,
-                                                                  int64_t n, float* __restrict__ out) {
+// Gradient (and optionally its unit normal) on continuous grid coordinates, in the mode
+// the descriptor names.
+__global__ void __launch_bounds__(256) coords_gradient_fwd_kernel(GridDesc g, const float* __restrict__ coords,
+                                                                  int64_t n, float* __restrict__ out,
+                                                                  float* __restrict__ normal) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     const Corners c = corners_from_coords(g, coords[3 * i], coords[3 * i + 1], coords[3 * i + 2]);
     float gv[3];
-    eq7_gradient(g, c, gv);
-    out[3 * i] = gv[0];
-    out[3 * i + 1] = gv[1];
-    out[3 * i + 2] = gv[2];
+    grid_gradient(g, c, gv);
+    if (out) {
+      out[3 * i] = gv[0];
+      out[3 * i + 1] = gv[1];
+      out[3 * i + 2] = gv[2];
+    }
+    if (normal) {
+      const float r = norm3(gv) + 1e-8f;
+      normal[3 * i] = gv[0] / r;
+      normal[3 * i + 1] = gv[1] / r;
+      normal[3 * i + 2] = gv[2] / r;
+    }
   }
 }
 
-__global__ void __launch_bounds__(256) gradient_interp_bwd_kernel(GridDesc g, const float* __restrict__ coords,
+__global__ void __launch_bounds__(256) coords_gradient_bwd_kernel(GridDesc g, const float* __restrict__ coords,
                                                                   int64_t n, const float* __restrict__ gout,
                                                                   float* __restrict__ grad_grid) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     const Corners c = corners_from_coords(g, coords[3 * i], coords[3 * i + 1], coords[3 * i + 2]);
     const float dg[3] = {gout[3 * i], gout[3 * i + 1], gout[3 * i + 2]};
-    eq7_scatter(g, c, dg, grad_grid);
+    grid_gradient_scatter(g, c, dg, grad_grid);
   }
 }
 
@@ -156,7 +168,7 @@ __global__ void __launch_bounds__(256) lattice_query_kernel(GridDesc g, int NX, 
     sdf[i] = gs_eval(g, s);
     const Corners c = corner_decomp(g, px, py, pz);
     float gv[3];
-    eq7_gradient(g, c, gv);
+    grid_gradient(g, c, gv);
     const float rr = norm3(gv) + 1e-8f;
     normal[3 * i] = gv[0] / rr;
     normal[3 * i + 1] = gv[1] / rr;
@@ -178,6 +190,7 @@ static GridDesc coords_desc(const float* grid, int X, int Y, int Z, const float*
     g.bound[a] = (float)((double)res[a] - 1.0001);
     g.eps[a] = eps3[a];
   }
+  g.analytical = 0;
   return g;
 }
 
@@ -200,7 +213,7 @@ __global__ void __launch_bounds__(256) serve_query_kernel(const SceneDev* __rest
     sdf[i] = gs_eval(g, gs_stencil(g, px, py, pz));
     if (normal) {
       float gv[3];
-      eq7_gradient(g, corner_decomp(g, px, py, pz), gv);
+      grid_gradient(g, corner_decomp(g, px, py, pz), gv);
       const float r = norm3(gv) + 1e-8f;
       normal[3 * i] = gv[0] / r;
       normal[3 * i + 1] = gv[1] / r;
@@ -228,12 +241,14 @@ extern "C" int vxn_serve_query(const void* scenes, int32_t n_scenes, const int32
 
 extern "C" int vxn_grid_query_fwd(const float* grid, int X, int Y, int Z, const float* box6, const float* xyz,
                                   int64_t n, float* sdf, float* grad, float* normal, int64_t* corners,
-                                  float* corner_weights, void* stream) {
+                                  float* corner_weights, int32_t gradient_mode, void* stream) {
   if (int e = check_grid(grid, X, Y, Z, box6)) return e;
+  if (int e = check_gradient_mode(gradient_mode)) return e;
   VXN_REQUIRE(n >= 0, VXN_ERR_SHAPE, "n must be >= 0");
   if (n == 0) return 0;
   VXN_REQUIRE(xyz != nullptr, VXN_ERR_NULL, "xyz is null");
-  const GridDesc g = make_grid_desc(grid, X, Y, Z, box6);
+  GridDesc g = make_grid_desc(grid, X, Y, Z, box6);
+  g.analytical = gradient_mode;
   hipLaunchKernelGGL(grid_query_fwd_kernel, dim3(blocks_for(n, 256, 65536)), dim3(256), 0, (hipStream_t)stream, g,
                      xyz, n, sdf, grad, normal, corners, corner_weights);
   VXN_LAUNCH_CHECK();
@@ -242,12 +257,14 @@ extern "C" int vxn_grid_query_fwd(const float* grid, int X, int Y, int Z, const 
 
 extern "C" int vxn_grid_query_bwd(const float* grid, int X, int Y, int Z, const float* box6, const float* xyz,
                                   int64_t n, const float* d_sdf, const float* d_grad, float* grad_grid,
-                                  void* stream) {
+                                  int32_t gradient_mode, void* stream) {
   if (int e = check_grid(grid, X, Y, Z, box6)) return e;
+  if (int e = check_gradient_mode(gradient_mode)) return e;
   VXN_REQUIRE(n >= 0, VXN_ERR_SHAPE, "n must be >= 0");
   if (n == 0 || (!d_sdf && !d_grad)) return 0;
   VXN_REQUIRE(xyz != nullptr && grad_grid != nullptr, VXN_ERR_NULL, "xyz or grad_grid is null");
-  const GridDesc g = make_grid_desc(grid, X, Y, Z, box6);
+  GridDesc g = make_grid_desc(grid, X, Y, Z, box6);
+  g.analytical = gradient_mode;
   hipLaunchKernelGGL(grid_query_bwd_kernel, dim3(blocks_for(n, 256, 65536)), dim3(256), 0, (hipStream_t)stream, g,
                      xyz, n, d_sdf, d_grad, grad_grid);
   VXN_LAUNCH_CHECK();
@@ -260,8 +277,8 @@ extern "C" int vxn_gradient_interp_fwd(const float* grid, int X, int Y, int Z, c
   VXN_REQUIRE(X >= 2 && Y >= 2 && Z >= 2 && n >= 0, VXN_ERR_SHAPE, "bad shape");
   if (n == 0) return 0;
   const GridDesc g = coords_desc(grid, X, Y, Z, eps3);
-  hipLaunchKernelGGL(gradient_interp_fwd_kernel, dim3(blocks_for(n, 256, 65536)), dim3(256), 0, (hipStream_t)stream,
-                     g, coords, n, out);
+  hipLaunchKernelGGL(coords_gradient_fwd_kernel, dim3(blocks_for(n, 256, 65536)), dim3(256), 0, (hipStream_t)stream,
+                     g, coords, n, out, nullptr);
   VXN_LAUNCH_CHECK();
   return 0;
 }
@@ -272,7 +289,33 @@ extern "C" int vxn_gradient_interp_bwd(const float* grad_out, const float* coord
   VXN_REQUIRE(X >= 2 && Y >= 2 && Z >= 2 && n >= 0, VXN_ERR_SHAPE, "bad shape");
   if (n == 0) return 0;
   const GridDesc g = coords_desc(nullptr, X, Y, Z, eps3);
-  hipLaunchKernelGGL(gradient_interp_bwd_kernel, dim3(blocks_for(n, 256, 65536)), dim3(256), 0, (hipStream_t)stream,
+  hipLaunchKernelGGL(coords_gradient_bwd_kernel, dim3(blocks_for(n, 256, 65536)), dim3(256), 0, (hipStream_t)stream,
+                     g, coords, n, grad_out, grad_grid);
+  VXN_LAUNCH_CHECK();
+  return 0;
+}
+
+extern "C" int vxn_analytical_gradient_fwd(const float* grid, int X, int Y, int Z, const float* coords, int64_t n,
+                                           const float* eps3, float* grad, float* normal, void* stream) {
+  VXN_REQUIRE(grid && coords && eps3, VXN_ERR_NULL, "null argument");
+  VXN_REQUIRE(X >= 2 && Y >= 2 && Z >= 2 && n >= 0, VXN_ERR_SHAPE, "bad shape");
+  if (n == 0 || (!grad && !normal)) return 0;
+  GridDesc g = coords_desc(grid, X, Y, Z, eps3);
+  g.analytical = 1;
+  hipLaunchKernelGGL(coords_gradient_fwd_kernel, dim3(blocks_for(n, 256, 65536)), dim3(256), 0, (hipStream_t)stream,
+                     g, coords, n, grad, normal);
+  VXN_LAUNCH_CHECK();
+  return 0;
+}
+
+extern "C" int vxn_analytical_gradient_bwd(const float* grad_out, const float* coords, int64_t n, float* grad_grid,
+                                           int X, int Y, int Z, const float* eps3, void* stream) {
+  VXN_REQUIRE(grad_out && coords && eps3 && grad_grid, VXN_ERR_NULL, "null argument");
+  VXN_REQUIRE(X >= 2 && Y >= 2 && Z >= 2 && n >= 0, VXN_ERR_SHAPE, "bad shape");
+  if (n == 0) return 0;
+  GridDesc g = coords_desc(nullptr, X, Y, Z, eps3);
+  g.analytical = 1;
+  hipLaunchKernelGGL(coords_gradient_bwd_kernel, dim3(blocks_for(n, 256, 65536)), dim3(256), 0, (hipStream_t)stream,
                      g, coords, n, grad_out, grad_grid);
   VXN_LAUNCH_CHECK();
   return 0;
@@ -292,11 +335,14 @@ extern "C" int vxn_voxel_corners(const float* coords, int64_t n, int X, int Y, i
 }
 
 extern "C" int vxn_lattice_query(const float* grid, int X, int Y, int Z, const float* box6, int NX, int NY, int NZ,
-                                 const float* lattice_box6, float* sdf, float* normal, void* stream) {
+                                 const float* lattice_box6, float* sdf, float* normal, int32_t gradient_mode,
+                                 void* stream) {
   if (int e = check_grid(grid, X, Y, Z, box6)) return e;
+  if (int e = check_gradient_mode(gradient_mode)) return e;
   VXN_REQUIRE(lattice_box6 && sdf && normal, VXN_ERR_NULL, "null argument");
   VXN_REQUIRE(NX >= 2 && NY >= 2 && NZ >= 2, VXN_ERR_SHAPE, "lattice must be >= 2 per axis");
-  const GridDesc g = make_grid_desc(grid, X, Y, Z, box6);
+  GridDesc g = make_grid_desc(grid, X, Y, Z, box6);
+  g.analytical = gradient_mode;
   const float* lb = lattice_box6;
   VXN_REQUIRE((int64_t)8 * ((NX + 7) / 8) * NY < ((int64_t)1 << 31), VXN_ERR_SHAPE, "lattice too large");
   const unsigned nb = (unsigned)(8 * ((NX + 7) / 8) * NY);

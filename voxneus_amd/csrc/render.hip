@@ -286,7 +286,7 @@ __global__ void __launch_bounds__(kRT, 2) render_kernel(RenderDev a) {
     const float sdf = gs_eval(G, gs);
     const Corners cn = corner_decomp(G, px, py, pz);
     float gv[3];
-    eq7_gradient(G, cn, gv);
+    grid_gradient(G, cn, gv);
     const float gr = norm3(gv);
     const float nrm[3] = {gv[0] / (gr + 1e-8f), gv[1] / (gr + 1e-8f), gv[2] / (gr + 1e-8f)};
     const float x0 = fminf(fmaxf((px - G.mn[0]) / G.den[0], 0.0f), 1.0f);
@@ -925,19 +925,29 @@ __global__ void __launch_bounds__(256) grid_scatter_kernel(RenderDev a) {
   const Corners cn = corner_decomp(a.g, px, py, pz);
   const GsStencil gs = gs_stencil(a.g, px, py, pz);
   const GridDesc& g = a.g;
-  // interior cells: the whole 4x4x4 brick is inside the grid and the divisors are 2*eps
-  const bool interior = cn.c0[0] >= 1 && cn.c0[0] + 2 <= g.X - 1 && cn.c0[1] >= 1 && cn.c0[1] + 2 <= g.Y - 1 &&
-                        cn.c0[2] >= 1 && cn.c0[2] + 2 <= g.Z - 1;
+  const bool ana = g.analytical != 0;
+  // Eq. 7 bricks: the whole 4x4x4 brick is inside the grid and the divisors are 2*eps.
+  // Eq. 4 touches only the 8 cell corners, distinct whenever c1 = c0 + 1 on every axis.
+  const bool interior =
+      ana ? (cn.c1[0] == cn.c0[0] + 1 && cn.c1[1] == cn.c0[1] + 1 && cn.c1[2] == cn.c0[2] + 1)
+          : (cn.c0[0] >= 1 && cn.c0[0] + 2 <= g.X - 1 && cn.c0[1] >= 1 && cn.c0[1] + 2 <= g.Y - 1 &&
+             cn.c0[2] >= 1 && cn.c0[2] + 2 <= g.Z - 1);
   const bool gs_same = gs.x0 == cn.c0[0] && gs.y0 == cn.c0[1] && gs.z0 == cn.c0[2] && gs.valid == 0xffu;
   if (live && !interior) {
-    eq7_scatter(g, cn, dg, a.grad_grid);
+    grid_gradient_scatter(g, cn, dg, a.grad_grid);
     if (dsdf != 0.0f) gs_scatter(g, gs, dsdf, a.grad_grid);
   }
   const bool brick = live && interior;
   float v[32];
 #pragma unroll
   for (int k = 0; k < 32; ++k) v[k] = 0.0f;
-  if (brick) {
+  if (brick && ana) {
+    float cg[8];
+    eq4_corner_grads(g, cn, dg, cg);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = cg[k];
+  }
+  if (brick && !ana) {
     // interior divisor of every corner: max((i_p - i_m) * eps, eps) = 2 * eps
     const float dvx = fmaxf(2.0f * g.eps[0], g.eps[0]), dvy = fmaxf(2.0f * g.eps[1], g.eps[1]),
                 dvz = fmaxf(2.0f * g.eps[2], g.eps[2]);
@@ -955,13 +965,13 @@ __global__ void __launch_bounds__(256) grid_scatter_kernel(RenderDev a) {
       v[oz ? slot_face(2, 1, ox, oy) : slot_corner(ox, oy, 1)] += vz;
       v[oz ? slot_corner(ox, oy, 0) : slot_face(2, 0, ox, oy)] -= vz;
     }
-    if (dsdf != 0.0f) {
-      if (gs_same) {
+  }
+  if (brick && dsdf != 0.0f) {
+    if (gs_same) {
 #pragma unroll
-        for (int k = 0; k < 8; ++k) v[k] += gs.w[k] * dsdf;
-      } else {
-        gs_scatter(g, gs, dsdf, a.grad_grid);
-      }
+      for (int k = 0; k < 8; ++k) v[k] += gs.w[k] * dsdf;
+    } else {
+      gs_scatter(g, gs, dsdf, a.grad_grid);
     }
   }
   const uint64_t key = brick ? (((uint64_t)cn.c0[0] << 42) | ((uint64_t)cn.c0[1] << 21) | (uint64_t)cn.c0[2])
@@ -971,6 +981,7 @@ __global__ void __launch_bounds__(256) grid_scatter_kernel(RenderDev a) {
   if (tb == 0) return;
   // Flush the runs' bricks column by column: the 32 slots form 12 (x, y) columns of
   // contiguous z (4 of length 4 through the cell, 8 of length 2 on the x and y faces).
+  // An Eq. 4 brick holds only the cell corners: z = 0, 1 of the through-cell columns.
   // Passes of 16 runs; lane quad q takes the q-th run of the pass and lane 4q + e its
   // e-th z, so each atomic instruction writes whole columns and merges into few requests.
   const int64_t sx = (int64_t)g.Y * g.Z, sy = g.Z;
@@ -999,8 +1010,9 @@ __global__ void __launch_bounds__(256) grid_scatter_kernel(RenderDev a) {
       const float s0 = __shfl(v[slot_face(2, 0, ox, oy)], src, 64), s1 = __shfl(v[slot_corner(ox, oy, 0)], src, 64);
       const float s2 = __shfl(v[slot_corner(ox, oy, 1)], src, 64), s3 = __shfl(v[slot_face(2, 1, ox, oy)], src, 64);
       const float val = e == 0 ? s0 : (e == 1 ? s1 : (e == 2 ? s2 : s3));
-      if (act) atomicAdd(gp + ox * sx + oy * sy + (e - 1), val);
+      if (act && (!ana || e == 1 || e == 2)) atomicAdd(gp + ox * sx + oy * sy + (e - 1), val);
     }
+    if (ana) continue;
     // face columns: x = -1 / 2 at y = u, y = -1 / 2 at x = u; z = 0, 1
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
@@ -1092,6 +1104,7 @@ static RenderWs render_ws(int64_t R, int S) {
 
 static int build_render_dev(const vxn_render_args& A, RenderDev& a, int mode) {
   if (int e = check_grid(A.grid, A.X, A.Y, A.Z, A.box)) return e;
+  if (int e = check_gradient_mode(A.gradient_mode)) return e;
   VXN_REQUIRE(A.n_rays >= 0, VXN_ERR_SHAPE, "n_rays must be >= 0");
   VXN_REQUIRE(A.n_samples >= 1 && A.n_samples <= kRT, VXN_ERR_UNSUPPORTED, "n_samples must be in [1, %d] (got %d)",
               kRT, A.n_samples);
@@ -1111,6 +1124,7 @@ static int build_render_dev(const vxn_render_args& A, RenderDev& a, int mode) {
   a.R = A.n_rays;
   a.S = A.n_samples;
   a.g = make_grid_desc(A.grid, A.X, A.Y, A.Z, A.box);
+  a.g.analytical = A.gradient_mode;
   a.h = h;
   a.mlp = A.mlp;
   a.s = A.s;
@@ -1250,12 +1264,14 @@ extern "C" int vxn_serve_pack_scene(const vxn_scene_desc* d, void* out, int64_t 
   VXN_REQUIRE(out_bytes >= (int64_t)sizeof(SceneDev), VXN_ERR_WORKSPACE, "a scene record needs %lld bytes",
               (long long)sizeof(SceneDev));
   if (int e = check_grid(d->grid, d->X, d->Y, d->Z, d->box)) return e;
+  if (int e = check_gradient_mode(d->gradient_mode)) return e;
   VXN_REQUIRE(d->n_levels == 16 && d->n_features == 2, VXN_ERR_UNSUPPORTED,
               "the fused render implements the 16-level x 2-feature hash feeding the 51-input MLP");
   VXN_REQUIRE(d->mlp != nullptr && d->table != nullptr, VXN_ERR_NULL, "null table or mlp");
   if (d->occ_mask) VXN_REQUIRE(d->occ_res >= 1, VXN_ERR_SHAPE, "occ_res must be >= 1");
   SceneDev sc{};
   sc.g = make_grid_desc(d->grid, d->X, d->Y, d->Z, d->box);
+  sc.g.analytical = d->gradient_mode;
   if (int e = make_hash_desc(sc.h, d->table, d->table_dtype, d->n_levels, d->log2_T, d->n_features, d->level_res,
                              d->level_w))
     return e;

@@ -25,6 +25,7 @@ struct GridDesc {
   float rm1[3];    // float(res) - 1.0f                      (coordinates.py:28)
   float bound[3];  // float32(res - 1.0001)                  (coordinates.py:73)
   float eps[3];    // (xyz_max - xyz_min) / (res - 1)        (sdf_grid.py:83-88)
+  int analytical;  // gradient mode: 0 = Eq. 7 interpolated, 1 = Eq. 4 analytical (sdf_grid.py:142-160)
 };
 
 VXN_DEV int64_t lin3(const GridDesc& g, int x, int y, int z) {
@@ -132,6 +133,7 @@ VXN_DEV float gs_eval(const GridDesc& g, const GsStencil& s) {
 struct Corners {
   int c0[3], c1[3];
   float w[8];
+  float d[3];  // fractional offsets in the cell
 };
 
 VXN_DEV Corners corner_decomp(const GridDesc& g, float px, float py, float pz) {
@@ -150,6 +152,7 @@ VXN_DEV Corners corner_decomp(const GridDesc& g, float px, float py, float pz) {
     c.c0[a] = i0;
     c.c1[a] = i1;
     d[a] = cc - (float)i0;
+    c.d[a] = d[a];
   }
   const float ox = 1.0f - d[0], oy = 1.0f - d[1], oz = 1.0f - d[2];
   c.w[0] = ox * oy * oz;
@@ -224,6 +227,72 @@ VXN_DEV void eq7_scatter(const GridDesc& g, const Corners& c, const float dg[3],
     atomicAdd(grad_grid + lin3(g, x, y, ip), v);
     atomicAdd(grad_grid + lin3(g, x, y, im), -v);
   }
+}
+
+// Eq. 4 (gradients.py:142-220): the derivative of the trilinear interpolant itself,
+// grad_a = sum_k (d w_k / d x_a) f[c_k].  For corner k = 4*ox + 2*oy + oz,
+// d w_k / d x = (ox ? +1 : -1) * Y_oy * Z_oz / eps_x with X_0 = 1 - dx, X_1 = dx (and
+// likewise for y, z); the 8 terms are summed left to right, as PyTorch's CPU sum folds
+// an 8-long contiguous row.
+VXN_DEV void eq4_weights(const GridDesc& g, const Corners& c, float dw[3][8]) {
+  const float dx = c.d[0], dy = c.d[1], dz = c.d[2];
+  const float X[2] = {1.0f - dx, dx}, Y[2] = {1.0f - dy, dy}, Z[2] = {1.0f - dz, dz};
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int ox = (k >> 2) & 1, oy = (k >> 1) & 1, oz = k & 1;
+    const float px = Y[oy] * Z[oz], py = X[ox] * Z[oz], pz = X[ox] * Y[oy];
+    dw[0][k] = (ox ? px : -px) / g.eps[0];
+    dw[1][k] = (oy ? py : -py) / g.eps[1];
+    dw[2][k] = (oz ? pz : -pz) / g.eps[2];
+  }
+}
+
+VXN_DEV void eq4_gradient(const GridDesc& g, const Corners& c, float out[3]) {
+  float dw[3][8], f[8];
+  eq4_weights(g, c, dw);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    int x, y, z;
+    corner_xyz(c, k, x, y, z);
+    f[k] = g.grid[lin3(g, x, y, z)];
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    float acc = dw[a][0] * f[0];
+#pragma unroll
+    for (int k = 1; k < 8; ++k) acc = acc + dw[a][k] * f[k];
+    out[a] = acc;
+  }
+}
+
+// d f[c_k] of the Eq. 4 gradient for dL/dgrad = dg.
+VXN_DEV void eq4_corner_grads(const GridDesc& g, const Corners& c, const float dg[3], float v[8]) {
+  float dw[3][8];
+  eq4_weights(g, c, dw);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) v[k] = (dg[2] * dw[2][k] + dg[1] * dw[1][k]) + dg[0] * dw[0][k];
+}
+
+VXN_DEV void eq4_scatter(const GridDesc& g, const Corners& c, const float dg[3], float* grad_grid) {
+  float v[8];
+  eq4_corner_grads(g, c, dg, v);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    int x, y, z;
+    corner_xyz(c, k, x, y, z);
+    atomicAdd(grad_grid + lin3(g, x, y, z), v[k]);
+  }
+}
+
+// The grid's gradient in its configured mode, and its backward.
+VXN_DEV void grid_gradient(const GridDesc& g, const Corners& c, float out[3]) {
+  if (g.analytical) eq4_gradient(g, c, out);
+  else eq7_gradient(g, c, out);
+}
+
+VXN_DEV void grid_gradient_scatter(const GridDesc& g, const Corners& c, const float dg[3], float* grad_grid) {
+  if (g.analytical) eq4_scatter(g, c, dg, grad_grid);
+  else eq7_scatter(g, c, dg, grad_grid);
 }
 
 VXN_DEV void gs_scatter(const GridDesc& g, const GsStencil& s, float dsdf, float* grad_grid) {

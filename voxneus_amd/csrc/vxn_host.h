@@ -57,7 +57,15 @@ inline GridDesc make_grid_desc(const float* grid, int X, int Y, int Z, const flo
     g.bound[a] = (float)((double)res[a] - 1.0001);
     g.eps[a] = (mx - mn) / ((float)res[a] - 1.0f);
   }
+  g.analytical = 0;
   return g;
+}
+
+inline int check_gradient_mode(int mode) {
+  VXN_REQUIRE(mode == VXN_GRADIENT_INTERPOLATED || mode == VXN_GRADIENT_ANALYTICAL, VXN_ERR_VALUE,
+              "gradient_mode must be %d (interpolated) or %d (analytical), got %d", VXN_GRADIENT_INTERPOLATED,
+              VXN_GRADIENT_ANALYTICAL, mode);
+  return 0;
 }
 
 inline int check_grid(const float* grid, int X, int Y, int Z, const float* box6) {

@@ -31,8 +31,10 @@ def _grid3(grid: torch.Tensor) -> torch.Tensor:
 # ----------------------------------------------------------------------------------------
 
 def grid_query_raw(grid3: torch.Tensor, box, xyz: torch.Tensor, want_sdf=True, want_grad=True,
-                   want_normal=False, want_corners=False, want_weights=False):
-    """Run the fused SDF + Eq. 7 query on flattened points; returns a dict of tensors."""
+                   want_normal=False, want_corners=False, want_weights=False,
+                   gradient_mode: int = N.VXN_GRADIENT_INTERPOLATED):
+    """Run the fused SDF + gradient query (Eq. 7, or Eq. 4 with ``VXN_GRADIENT_ANALYTICAL``)
+    on flattened points; returns a dict of tensors."""
     grid3 = grid3.contiguous()
     xyz = xyz.reshape(-1, 3).contiguous()
     n = xyz.shape[0]
@@ -52,7 +54,7 @@ def grid_query_raw(grid3: torch.Tensor, box, xyz: torch.Tensor, want_sdf=True, w
     N.check(N.lib().vxn_grid_query_fwd(
         N.dptr(grid3, torch.float32, "grid"), X, Y, Z, N.box6(*box), N.dptr(xyz, torch.float32, "xyz"), n,
         N.dptr(out.get("sdf")), N.dptr(out.get("grad")), N.dptr(out.get("normal")), N.dptr(out.get("corners")),
-        N.dptr(out.get("weights")), _stream(xyz)), "grid_query_fwd")
+        N.dptr(out.get("weights")), int(gradient_mode), _stream(xyz)), "grid_query_fwd")
     return out
 
 
@@ -60,11 +62,12 @@ class GridQueryFunction(torch.autograd.Function):
     """(grid, xyz) -> (sdf [n], grad [n,3]); backward scatters into the grid."""
 
     @staticmethod
-    def forward(ctx, grid, xyz, box):
+    def forward(ctx, grid, xyz, box, gradient_mode=N.VXN_GRADIENT_INTERPOLATED):
         g3 = _grid3(grid)
-        o = grid_query_raw(g3, box, xyz, want_sdf=True, want_grad=True)
+        o = grid_query_raw(g3, box, xyz, want_sdf=True, want_grad=True, gradient_mode=gradient_mode)
         ctx.save_for_backward(g3, xyz)
         ctx.box = box
+        ctx.mode = int(gradient_mode)
         ctx.grid_shape = grid.shape
         return o["sdf"], o["grad"]
 
@@ -80,9 +83,9 @@ class GridQueryFunction(torch.autograd.Function):
             dg = d_grad.contiguous() if d_grad is not None else None
             N.check(N.lib().vxn_grid_query_bwd(
                 N.dptr(g3), X, Y, Z, N.box6(*ctx.box), N.dptr(xf), xf.shape[0], N.dptr(ds), N.dptr(dg),
-                N.dptr(gg), _stream(gg)), "grid_query_bwd")
+                N.dptr(gg), ctx.mode, _stream(gg)), "grid_query_bwd")
             gg = gg.reshape(ctx.grid_shape)
-        return gg, None, None
+        return gg, None, None, None
 
 
 class GradientInterpFunction(torch.autograd.Function):
@@ -115,6 +118,45 @@ class GradientInterpFunction(torch.autograd.Function):
         return gg, None, None
 
 
+class AnalyticalGradientFunction(torch.autograd.Function):
+    """Eq. 4 on continuous grid coordinates: (grid, coords, voxel_size) -> grad [n,3]
+    (geometry/gradients.py:142-220); backward scatters dL/dgrad into the 8 cell corners."""
+
+    @staticmethod
+    def forward(ctx, grid, coords, voxel_size):
+        g3 = _grid3(grid).contiguous()
+        c = coords.reshape(-1, 3).contiguous()
+        eps = [float(v) for v in voxel_size.detach().cpu().tolist()]
+        out = torch.empty(c.shape[0], 3, device=c.device, dtype=torch.float32)
+        X, Y, Z = g3.shape
+        N.check(N.lib().vxn_analytical_gradient_fwd(N.dptr(g3, torch.float32, "grid"), X, Y, Z,
+                                                    N.dptr(c, torch.float32, "coords"), c.shape[0], N.f3(eps),
+                                                    N.dptr(out), None, _stream(c)), "analytical_gradient_fwd")
+        ctx.save_for_backward(c)
+        ctx.eps, ctx.shape3, ctx.grid_shape = eps, g3.shape, grid.shape
+        return out
+
+    @staticmethod
+    def backward(ctx, gout):
+        (c,) = ctx.saved_tensors
+        gg = None
+        if ctx.needs_input_grad[0]:
+            X, Y, Z = ctx.shape3
+            gg = torch.zeros(ctx.shape3, device=c.device, dtype=torch.float32)
+            go = gout.contiguous()
+            N.check(N.lib().vxn_analytical_gradient_bwd(N.dptr(go), N.dptr(c), c.shape[0], N.dptr(gg), X, Y, Z,
+                                                        N.f3(ctx.eps), _stream(gg)), "analytical_gradient_bwd")
+            gg = gg.reshape(ctx.grid_shape)
+        return gg, None, None
+
+
+def compute_analytical_gradient(grid, continuous_coords, voxel_size, grid_res=None):
+    """geometry/gradients.py:142-220: (grad, normal) of Eq. 4; grad is differentiable in grid."""
+    shape = continuous_coords.shape[:-1]
+    grad = AnalyticalGradientFunction.apply(grid, continuous_coords, voxel_size).reshape(*shape, 3)
+    return grad, grad / (torch.linalg.norm(grad, dim=-1, keepdim=True) + 1e-8)
+
+
 def voxel_corners(coords: torch.Tensor, grid_shape):
     """-> (corners [n,8,3] int64, weights [n,8]) of continuous grid coordinates."""
     c = coords.reshape(-1, 3).contiguous()
@@ -127,13 +169,14 @@ def voxel_corners(coords: torch.Tensor, grid_shape):
     return corners, weights
 
 
-def lattice_query(grid3, box, NX, NY, NZ, lattice_box):
+def lattice_query(grid3, box, NX, NY, NZ, lattice_box, gradient_mode: int = N.VXN_GRADIENT_INTERPOLATED):
     g3 = grid3.contiguous()
     sdf = torch.empty(NX, NY, NZ, device=g3.device, dtype=torch.float32)
     nrm = torch.empty(NX, NY, NZ, 3, device=g3.device, dtype=torch.float32)
     X, Y, Z = g3.shape
     N.check(N.lib().vxn_lattice_query(N.dptr(g3, torch.float32, "grid"), X, Y, Z, N.box6(*box), NX, NY, NZ,
-                                      N.box6(*lattice_box), N.dptr(sdf), N.dptr(nrm), _stream(g3)), "lattice_query")
+                                      N.box6(*lattice_box), N.dptr(sdf), N.dptr(nrm), int(gradient_mode),
+                                      _stream(g3)), "lattice_query")
     return sdf, nrm
 
 
@@ -266,7 +309,8 @@ class RenderConfig:
     """Static settings of one fused render call."""
 
     def __init__(self, box, level_res, level_w=None, sphere_radius=1.0, cos_anneal_ratio=0.0,
-                 use_iter_cos=False, mlp_mode=N.VXN_MLP_FP32, table_grad_out=None):
+                 use_iter_cos=False, mlp_mode=N.VXN_MLP_FP32, table_grad_out=None,
+                 gradient_mode=N.VXN_GRADIENT_INTERPOLATED):
         # table_grad_out: optional FP32 [L,T,F] tensor that the backward accumulates the
         # hash-table gradient into (for bf16 tables with an FP32 master copy); the table
         # input then receives no gradient through autograd
@@ -278,6 +322,7 @@ class RenderConfig:
         self.cos_anneal_ratio = float(cos_anneal_ratio)
         self.use_iter_cos = bool(use_iter_cos)
         self.mlp_mode = mlp_mode
+        self.gradient_mode = int(gradient_mode)
 
 
 def fill_render_args(a: "N.RenderArgs", cfg: RenderConfig, grid3, table, mlp_flat, s: float, rays_o, rays_d,
@@ -301,6 +346,7 @@ def fill_render_args(a: "N.RenderArgs", cfg: RenderConfig, grid3, table, mlp_fla
     a.sphere_radius = cfg.sphere_radius
     a.cos_anneal_ratio = cfg.cos_anneal_ratio
     a.use_iter_cos = int(cfg.use_iter_cos)
+    a.gradient_mode = cfg.gradient_mode
 
 
 def render_workspace(a: "N.RenderArgs", device) -> torch.Tensor:

@@ -15,6 +15,7 @@ import numpy as np
 import torch
 import torch.nn as nn
 
+from . import _native as N
 from . import functional as Fn
 
 
@@ -77,15 +78,14 @@ class DenseSDFGrid(nn.Module):
         return sdf.reshape(*xyz.shape[:-1], 1)
 
     def query_normals(self, xyz: torch.Tensor, gradient_mode: str = "interpolated"):
-        """Eq. 7 gradient, unit normal, corner indices (..., 8, 3) int64 and weights (..., 8)
-        (sdf_grid.py:122-160)."""
-        if gradient_mode.lower() != "interpolated":
-            raise NotImplementedError("only the Eq. 7 interpolated gradient runs on the gfx950 path "
-                                      "(the analytical Eq. 4 mode is an ablation)")
+        """Gradient (Eq. 7 "interpolated" or Eq. 4 "analytical"), unit normal, corner indices
+        (..., 8, 3) int64 and weights (..., 8) (sdf_grid.py:122-160); any other mode is a
+        ValueError."""
+        mode = N.gradient_mode_code(gradient_mode)
         shape = xyz.shape[:-1]
         raw = Fn.grid_query_raw(self.grid.detach().reshape(self.grid.shape[-3:]), self.box, xyz, want_sdf=False,
                                 want_grad=False, want_corners=True, want_weights=True)
-        _, grad = Fn.GridQueryFunction.apply(self.grid, xyz, self.box)
+        _, grad = Fn.GridQueryFunction.apply(self.grid, xyz, self.box, mode)
         grad = grad.reshape(*shape, 3)
         normal = grad / (torch.linalg.norm(grad, dim=-1, keepdim=True) + 1e-8)
         return grad, normal, raw["corners"].reshape(*shape, 8, 3), raw["weights"].reshape(*shape, 8)
@@ -93,12 +93,13 @@ class DenseSDFGrid(nn.Module):
     def query_features(self, xyz: torch.Tensor) -> Optional[torch.Tensor]:
         return None
 
-    def query_lattice(self, NX: int, NY: int, NZ: int, lattice_min=None, lattice_max=None):
+    def query_lattice(self, NX: int, NY: int, NZ: int, lattice_min=None, lattice_max=None,
+                      gradient_mode: str = "interpolated"):
         """SDF and unit normal on a full NX*NY*NZ lattice (mesh-extraction query)."""
         lo = self.xyz_min.tolist() if lattice_min is None else list(lattice_min)
         hi = self.xyz_max.tolist() if lattice_max is None else list(lattice_max)
         return Fn.lattice_query(self.grid.detach().reshape(self.grid.shape[-3:]), self.box, NX, NY, NZ,
-                                tuple(lo + hi))
+                                tuple(lo + hi), N.gradient_mode_code(gradient_mode))
 
     def get_numpy_grid(self) -> np.ndarray:
         return self.grid.detach().squeeze().cpu().numpy()

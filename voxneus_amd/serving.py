@@ -39,11 +39,13 @@ from . import _native as N
 
 class ResidentScene:
     """One trained scene kept in HBM: grid [X,Y,Z] f32, hash table [16,T,2] f32/bf16,
-    flat MLP [3523] f32, NeuS sharpness s, occupancy mask (bool [r,r,r] or None)."""
+    flat MLP [3523] f32, NeuS sharpness s, occupancy mask (bool [r,r,r] or None) and the
+    model's gradient mode ("interpolated" Eq. 7 or "analytical" Eq. 4)."""
 
     def __init__(self, grid: torch.Tensor, table: torch.Tensor, mlp: torch.Tensor, level_res: Sequence[int],
                  s: float, box=(-1.0, -1.0, -1.0, 1.0, 1.0, 1.0), occ_mask: Optional[torch.Tensor] = None,
-                 occ_box=None, level_w: Optional[Sequence[float]] = None, sphere_radius: float = 1.0):
+                 occ_box=None, level_w: Optional[Sequence[float]] = None, sphere_radius: float = 1.0,
+                 gradient_mode: str = "interpolated"):
         if grid.dim() != 3 or grid.dtype != torch.float32 or not grid.is_cuda:
             raise ValueError("grid must be a float32 [X,Y,Z] HIP tensor")
         if table.dim() != 3 or table.shape[0] != 16 or table.shape[2] != 2 or not table.is_cuda:
@@ -58,6 +60,7 @@ class ResidentScene:
         self.occ_mask = None if occ_mask is None else occ_mask.contiguous().to(torch.bool)
         self.occ_box = self.box if occ_box is None else tuple(float(v) for v in occ_box)
         self.sphere_radius = float(sphere_radius)
+        self.gradient_mode = N.gradient_mode_code(gradient_mode)
 
     @property
     def nbytes(self) -> int:
@@ -81,11 +84,13 @@ class ResidentScene:
             d.level_w[l] = 1.0 if self.level_w is None else self.level_w[l]
         d.mlp = self.mlp.data_ptr()
         d.s, d.sphere_radius = self.s, self.sphere_radius
+        d.gradient_mode = self.gradient_mode
         return d
 
 
 def make_resident_scene(grid_res: int, log2_T: int, seed: int, device, table_dtype=torch.float32,
-                        occ_res: int = 128, per_level_scale: float = 1.5, variance: float = 0.3) -> ResidentScene:
+                        occ_res: int = 128, per_level_scale: float = 1.5, variance: float = 0.3,
+                        gradient_mode: str = "interpolated") -> ResidentScene:
     """A seeded multi-primitive scene built on the device: the SDF of
     ``scenes.MultiPrimitive.seeded(seed)`` at the grid vertices (float64, stored float32),
     a U(-1e-4, 1e-4) hash table, a kaiming-uniform MLP, an all-True occupancy grid."""
@@ -111,7 +116,7 @@ def make_resident_scene(grid_res: int, log2_T: int, seed: int, device, table_dty
     s = min(max(math.exp(variance * 10.0), 1e-4), 300.0)
     occ = torch.ones((occ_res,) * 3, dtype=torch.bool, device=device)
     res = level_resolutions(16, 16, S.hash_max_resolution(16, per_level_scale, 16))
-    return ResidentScene(grid, table, flat, res, s, occ_mask=occ)
+    return ResidentScene(grid, table, flat, res, s, occ_mask=occ, gradient_mode=gradient_mode)
 
 
 def _smooth_min(a, b, k):

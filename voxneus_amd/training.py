@@ -64,6 +64,7 @@ class StepConfig:
     relax_radius: float = 1.2
     mlp_mode: int = N.VXN_MLP_FP32
     active_levels: Optional[float] = None
+    gradient_mode: str = "interpolated"  # or "analytical" (Eq. 4), as VoxNeuS(gradient_mode=...)
 
 
 class FusedTrainStep:
@@ -113,7 +114,8 @@ class FusedTrainStep:
 
     def _render_args(self, rays_o, rays_d, mid, dists, target):
         cfg = self.cfg
-        rc = Fn.RenderConfig(self.box, self.level_res, self.level_w, cfg.sphere_radius, mlp_mode=cfg.mlp_mode)
+        rc = Fn.RenderConfig(self.box, self.level_res, self.level_w, cfg.sphere_radius, mlp_mode=cfg.mlp_mode,
+                             gradient_mode=N.gradient_mode_code(cfg.gradient_mode))
         a = N.RenderArgs()
         s = torch.clamp(torch.exp(self.variance.detach() * 10.0), min=1e-4, max=self.s_max)
         self.s_value = s

@@ -30,8 +30,7 @@ class VoxNeuS(nn.Module):
                  hash_base_resolution: int = 16, hash_max_resolution: float = 2048.0,
                  hash_dtype: torch.dtype = torch.float32, mlp_mode: int = N.VXN_MLP_FP32):
         super().__init__()
-        if gradient_mode.lower() != "interpolated":
-            raise NotImplementedError("the fused path implements the Eq. 7 interpolated gradient")
+        N.gradient_mode_code(gradient_mode)  # ValueError on an unknown mode (sdf_grid.py:159-160)
         if view_embed_type.lower() != "sh":
             raise NotImplementedError("the fused path implements the SH view embedding")
         self.gradient_mode = gradient_mode
@@ -58,7 +57,7 @@ class VoxNeuS(nn.Module):
     def forward_points(self, xyz, view_dirs, active_levels: Optional[float] = None) -> Dict[str, torch.Tensor]:
         """voxneus.py:115-167."""
         sdf = self.sdf_grid.query_sdf(xyz)
-        grad, normal, corners, weights = self.sdf_grid.query_normals(xyz)
+        grad, normal, corners, weights = self.sdf_grid.query_normals(xyz, gradient_mode=self.gradient_mode)
         g = self.sdf_grid
         norm_xyz = (xyz - g.xyz_min) / (g.xyz_max - g.xyz_min + 1e-8)
         feats = self.radiance_hash(norm_xyz, active_levels=active_levels)
@@ -69,7 +68,8 @@ class VoxNeuS(nn.Module):
     def render_config(self, active_levels=None, sphere_radius=1.0, cos_anneal_ratio=0.0, use_iter_cos=False):
         enc = self.radiance_hash.encoder
         return Fn.RenderConfig(self.sdf_grid.box6, enc.level_res, level_weights(enc.n_levels, active_levels),
-                               sphere_radius, cos_anneal_ratio, use_iter_cos, self.mlp_mode)
+                               sphere_radius, cos_anneal_ratio, use_iter_cos, self.mlp_mode,
+                               gradient_mode=N.gradient_mode_code(self.gradient_mode))
 
     def render(self, rays_o, rays_d, sampler: RaySampler, near, far, background_rgb=None, cos_anneal_ratio=0.0,
                use_iter_cos_annealing=False, active_levels=None, up_sample_steps=4, sphere_radius=1.0,
