@@ -594,23 +594,36 @@ def amplify_s_gradient(grad_v, k=5.0):
 
 # ----------------------------------------------------------------------------------------
 # Optimizers (Code/voxneus/training/optimizer.py, cuda/adam_upd_kernel.cu, torch.optim.Adam)
-# Every elementwise step is its own PyTorch op, so each rounds to float32 once.
+# Every elementwise step is its own PyTorch op, so each rounds to float32 once.  Square
+# roots are correctly rounded (IEEE sqrt, as sqrtf on the GPU): PyTorch's vectorised CPU
+# float32 sqrt is not, so it is taken in float64 and rounded once to float32, which for
+# a square root gives the correctly rounded float32 result.
 # ----------------------------------------------------------------------------------------
+
+def _sqrt32(x):
+    return torch.sqrt(x.double()).to(torch.float32)
+
 
 def adam_native_step(p, g, m, v, step, beta1, beta2, lr, eps, masked):
     """adam_upd_kernel.cu:6-38 per element, with the host step size of :50 / :70 formed in
     float: step_size = lr * sqrtf(1 - powf(b2, t)) / (1 - powf(b1, t)).  Updates p, m, v
     (float32 tensors) in place; ``masked`` leaves elements with g == 0 untouched."""
+    import ctypes
+    import ctypes.util
+
     import numpy as np
     f = np.float32
+    libm = ctypes.CDLL(ctypes.util.find_library("m"))
+    libm.powf.restype, libm.powf.argtypes = ctypes.c_float, [ctypes.c_float, ctypes.c_float]
+    powf = lambda a, b: f(libm.powf(float(a), float(b)))  # the C library's powf, as the host code calls
     b1, b2, lr32, eps32 = f(beta1), f(beta2), f(lr), f(eps)
     one = f(1.0)
-    step_size = lr32 * np.sqrt(one - np.power(b2, f(step))) / (one - np.power(b1, f(step)))
+    step_size = lr32 * np.sqrt(one - powf(b2, f(step))) / (one - powf(b1, f(step)))
     sel = (g != 0.0) if masked else torch.ones_like(g, dtype=torch.bool)
     omb1, omb2 = one - b1, one - b2
     mn = torch.mul(m, float(b1)) + torch.mul(g, float(omb1))
     vn = torch.mul(v, float(b2)) + torch.mul(torch.mul(g, float(omb2)), g)
-    upd = torch.div(torch.mul(mn, float(step_size)), torch.sqrt(vn) + float(eps32))
+    upd = torch.div(torch.mul(mn, float(step_size)), _sqrt32(vn) + float(eps32))
     pn = p - upd
     m.copy_(torch.where(sel, mn, m))
     v.copy_(torch.where(sel, vn, v))
@@ -632,7 +645,7 @@ def sparse_adam_masked_step(p, g, m, v, step, lr, beta1, beta2, eps, weight_deca
         va = v[mask] * beta2 + (1.0 - beta2) * (ga * ga)
         v[mask] = va
         step_size = lr / (1.0 - beta1 ** step)
-        denom = (torch.sqrt(va) / math.sqrt(1.0 - beta2 ** step)) + eps
+        denom = (_sqrt32(va) / math.sqrt(1.0 - beta2 ** step)) + eps
         p[mask] -= step_size * (ma / denom)
     return n
 
@@ -641,14 +654,17 @@ def torch_adam_step(p, g, m, v, step, lr, beta1=0.9, beta2=0.999, eps=1e-8):
     """torch.optim.Adam (no weight decay, no amsgrad) in the order of its multi-tensor
     update, the one used for device tensors (torch/optim/adam.py, _multi_tensor_adam):
     m = lerp(m, g, 1 - b1); v = v * b2 + (1 - b2) * (g * g);
-    p = p + (-lr / bc1) * (m / (sqrt(v) / sqrt(bc2) + eps)), scalars from Python floats."""
-    w = 1.0 - beta1
-    m.copy_(m + torch.mul(g - m, w))  # lerp with weight < 0.5
+    p = p + (-lr / bc1) * (m / (sqrt(v) / sqrt(bc2) + eps)), scalars from Python floats.
+    PyTorch evaluates lerp with weight < 0.5 as one fused multiply-add, m + w * (g - m)
+    rounded once; it is emulated here in float64 (w * (g - m) is exact there, so only a
+    float64 sum within 2^-29 ulp of a float32 rounding midpoint could round twice)."""
+    w = torch.tensor(1.0 - beta1, dtype=torch.float32).double()
+    m.copy_((m.double() + w * (g - m).double()).to(torch.float32))
     v.copy_(torch.mul(v, beta2))
     v.copy_(v + torch.mul(torch.mul(g, g), 1.0 - beta2))
     bc1 = 1.0 - beta1 ** step
     bc2 = 1.0 - beta2 ** step
-    denom = torch.div(torch.sqrt(v), bc2 ** 0.5) + eps
+    denom = torch.div(_sqrt32(v), bc2 ** 0.5) + eps
     p.copy_(p + torch.mul(torch.div(m, denom), (lr / bc1) * -1))
 
 

@@ -106,7 +106,7 @@ __global__ void __launch_bounds__(kAdamThreads) grid_adam_kernel(float* __restri
 }
 
 // torch.optim.Adam over several parameters in one launch (its multi-tensor CUDA order):
-//   m = lerp(m, g, 1 - b1) = m + (1 - b1) (g - m);  v = v b2 + (1 - b2) (g g);
+//   m = lerp(m, g, 1 - b1) = fma(1 - b1, g - m, m);  v = v b2 + (1 - b2) (g g);
 //   p = p + (-lr / (1 - b1^t)) (m / (sqrt(v) / sqrt(1 - b2^t) + eps)).
 struct AdamTensorDev {
   float* p;
@@ -146,7 +146,7 @@ __global__ void __launch_bounds__(kAdamThreads) torch_adam_kernel(AdamMultiDev a
   load4(T.v, i, T.n, vv);
 #pragma unroll
   for (int e = 0; e < 4; ++e) {
-    mv[e] = mv[e] + a.w1 * (gv[e] - mv[e]);
+    mv[e] = fmaf(a.w1, gv[e] - mv[e], mv[e]);  // lerp, weight < 0.5: one rounding, as PyTorch
     vv[e] = vv[e] * a.b2;
     vv[e] = vv[e] + a.omb2 * (gv[e] * gv[e]);
     const float denom = sqrtf(vv[e]) / a.bc2_sqrt + a.eps;

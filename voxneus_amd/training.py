@@ -173,3 +173,65 @@ class FusedTrainStep:
         if n_in > 0:
             lst = lst[lst >= 0]
         return lst & 0x7FFFFFFF
+
+
+class FusedOptimizer:
+    """The reference's optimizer step (trainer.py:328-353, training/optimizer.py:15-154)
+    on a ``FusedTrainStep``'s device buffers, with no host synchronisation:
+
+    * SparseGridAdam (lr_sdf, betas 0.9/0.99, eps 1e-8) on the grid through the native
+      masked kernel; ``n_updated`` counts the updated voxels on the device;
+    * Adam (betas 0.9/0.999, eps 1e-8) on the hash table, the MLP and the variance in one
+      launch, the hash and MLP rates under the cosine decay of trainer.py:329, the
+      variance rate multiplied by k on the device when the step's s-gradient was
+      amplified (trainer.py:332), and an optional variance floor (trainer.py:346-353).
+
+    A bf16 hash table keeps an FP32 master here; each update writes its bf16 rounding
+    back into the step's table."""
+
+    def __init__(self, st: "FusedTrainStep", lr_sdf: float = 0.1, lr_hash: float = 0.01, lr_mlp: float = 1e-3,
+                 lr_variance: float = 5e-4, total_steps: int = 40000):
+        self.st = st
+        self.lr_sdf, self.lr_hash, self.lr_mlp, self.lr_variance = lr_sdf, lr_hash, lr_mlp, lr_variance
+        self.total_steps = total_steps
+        z = lambda t: torch.zeros(t.shape, device=t.device, dtype=torch.float32)
+        self.grid_m, self.grid_v = z(st.grid), z(st.grid)
+        self.table_master = st.table.float().contiguous() if st.table.dtype == torch.bfloat16 else None
+        self.table_m, self.table_v = z(st.table), z(st.table)
+        self.mlp_m, self.mlp_v = z(st.mlp), z(st.mlp)
+        self.var_m, self.var_v = z(st.variance), z(st.variance)
+        self.n_updated = torch.zeros(1, device=st.grid.device, dtype=torch.int32)
+        self.sdf_step = 0
+        self.net_step = 0
+
+    def step(self, it: int, variance_floor: Optional[float] = None):
+        """Apply both optimizers after ``FusedTrainStep.step`` of training iteration ``it``."""
+        import math
+
+        from .optim import cosine_lr_factor
+        st = self.st
+        L = N.lib()
+        s = Fn._stream(st.grid)
+        f = cosine_lr_factor(it, self.total_steps)
+        self.sdf_step += 1
+        self.n_updated.zero_()
+        N.check(L.vxn_masked_adam_upd(st.grid.data_ptr(), st.grad_grid.data_ptr(), self.grid_m.data_ptr(),
+                                      self.grid_v.data_ptr(), st.grid.numel(), self.sdf_step, 0.9, 0.99,
+                                      self.lr_sdf, 1e-8, self.n_updated.data_ptr(), s), "masked_adam_upd")
+        self.net_step += 1
+        master = self.table_master
+        ts = (N.AdamTensor * 3)()
+        rows = (((master if master is not None else st.table), st.grad_table, self.table_m, self.table_v,
+                 self.lr_hash * f, None, 1.0, -math.inf, st.table if master is not None else None),
+                (st.mlp, st.grad_mlp, self.mlp_m, self.mlp_v, self.lr_mlp * f, None, 1.0, -math.inf, None),
+                (st.variance, st.grad_v, self.var_m, self.var_v, self.lr_variance,
+                 st.amplified if st.cfg.amplify else None, float(st.cfg.s_amp_k),
+                 -math.inf if variance_floor is None else float(variance_floor), None))
+        for d, (p, g, m, v, lr, flag, scale, pmin, bf) in zip(ts, rows):
+            d.param, d.grad, d.exp_avg, d.exp_avg_sq = p.data_ptr(), g.data_ptr(), m.data_ptr(), v.data_ptr()
+            d.n = p.numel()
+            d.lr, d.lr_flag, d.lr_flag_scale, d.param_min = lr, N.dptr(flag), scale, pmin
+            d.param_bf16 = N.dptr(bf)
+        N.check(L.vxn_adam_step(ctypes.cast(ts, ctypes.c_void_p), 3, self.net_step, 0.9, 0.999, 1e-8, s),
+                "adam_step")
+
