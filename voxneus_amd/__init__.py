@@ -1,7 +1,7 @@
 """MI355X-native VoxNeuS voxel-SDF rendering and regularisation path.
 
 The public names mirror the reference's ``voxneus`` package (models, rendering,
-regularization, training.scheduler), so the hot path can be switched by import.  The
+geometry, regularization, training.scheduler, training.optimizer), so the hot path can be switched by import.  The
 compute runs in hand-written gfx950 kernels (``libvoxneus_hip.so``, C ABI in
 ``include/voxneus_hip.h``); importing a name that needs the library raises if it has not
 been built.
@@ -20,6 +20,8 @@ _LAZY = {
     "extract_active_vertices": "regularization", "compute_eikonal_loss_and_gradient": "regularization",
     "compute_curvature_loss_and_gradient": "regularization",
     "amplify_s_gradient": "training", "FusedTrainStep": "training", "StepConfig": "training",
+    "FusedOptimizer": "training", "compute_analytical_gradient": "sdf_grid",
+    "SparseGridAdam": "optim", "NetAdam": "optim", "create_optimizer": "optim",
 }
 
 

@@ -119,3 +119,9 @@ def interpolate_gradients(grid, continuous_coords, voxel_size, grid_res):
     g = Fn.GradientInterpFunction.apply(grid, continuous_coords, voxel_size)
     g = g.reshape(*continuous_coords.shape[:-1], 3)
     return g, g / (torch.linalg.norm(g, dim=-1, keepdim=True) + 1e-8), corners, weights
+
+
+def compute_analytical_gradient(grid, continuous_coords, voxel_size, grid_res):
+    """gradients.py:142-220: the Eq. 4 gradient and its unit normal on continuous grid
+    coordinates; the gradient is differentiable in the grid."""
+    return Fn.compute_analytical_gradient(grid, continuous_coords, voxel_size, grid_res)
