@@ -100,6 +100,13 @@ int vxn_voxel_corners(const float* coords, int64_t n, int X, int Y, int Z, int64
 int vxn_lattice_query(const float* grid, int X, int Y, int Z, const float* box6, int NX, int NY, int NZ,
                       const float* lattice_box6, float* sdf, float* normal, int32_t gradient_mode, void* stream);
 
+/* Progressive grid super-resolution: grid [X,Y,Z] -> out [NX,NY,NZ] by trilinear
+ * interpolation with aligned corners (F.interpolate, mode="trilinear",
+ * align_corners=True), in PyTorch's CPU operation order.  Replaces upsample_sdf_grid
+ * (geometry/upsample.py:12-39) under DenseSDFGrid.scale_volume_grid
+ * (models/sdf_grid.py:183-195); out must not overlap grid.                           */
+int vxn_upsample_grid(const float* grid, int X, int Y, int Z, float* out, int NX, int NY, int NZ, void* stream);
+
 /* ---- Multiresolution hash encoding ------------------------------------------------
  * table: [L, 2^log2T, F] float32 (dtype 0) or bfloat16 (dtype 1); F must be 2.
  * level_res (host, L entries): floor(base * b^l) as computed by the reference;
@@ -264,6 +271,24 @@ int vxn_reg_eik_curv_vr(const float* grid, int X, int Y, int Z, const float* eps
  * (device, nullable) receives 1 or 0.  Replaces amplify_s_gradient
  * (training/scheduler.py:14-33) without its host read-back.                           */
 int vxn_s_amplify(float* grad_v, float k, int32_t* amplified, void* stream);
+
+/* ---- Occupancy grid -----------------------------------------------------------------
+ * OccupancyGrid.update_from_sdf (rendering/occupancy_grid.py:62-119), called every 16
+ * steps by the trainer (trainer.py:171-173) through VoxNeuS.update_occupancy_grid
+ * (models/voxneus.py:105-113): at cell centres of the res^3 occupancy grid over
+ * occ_box6 (host), the SDF grid's logistic density s sigmoid(s d)(1 - sigmoid(s d)),
+ * raised to 2 threshold within 2 |voxel step| of the surface, replaces the cell's
+ * density by max(decay * previous, new) (the new value alone when first_update) and
+ * sets mask = density > threshold.  density [res^3] float32, mask [res^3] uint8.
+ * With idx_x/y/z (device int64 [n_cells] each, as the reference's torch.randint draws)
+ * only those cells are updated (cells drawn twice get the same value; indices outside
+ * [0, res) are skipped) and a workspace of vxn_occupancy_update_workspace_bytes is
+ * needed; with null indices every cell is updated once (a full sweep, no workspace).  */
+int vxn_occupancy_update_workspace_bytes(int64_t n_cells, int64_t* bytes);
+int vxn_occupancy_update(float* density, uint8_t* mask, int32_t res, const float* occ_box6, const float* grid,
+                         int32_t X, int32_t Y, int32_t Z, const float* box6, float s, float decay, float threshold,
+                         int32_t first_update, const int64_t* idx_x, const int64_t* idx_y, const int64_t* idx_z,
+                         int64_t n_cells, void* workspace, int64_t workspace_bytes, void* stream);
 
 /* ---- Optimizer updates ------------------------------------------------------------
  * float32 tensors of n elements, 16-byte aligned; step is the update count including

@@ -137,6 +137,12 @@ def query_normals(grid5, box: GridBox, xyz, gradient_mode="interpolated"):
     raise ValueError(f"Unknown gradient mode: {gradient_mode}")
 
 
+def upsample_sdf_grid(grid5, new_res):
+    """upsample.py:12-39: F.interpolate(trilinear, align_corners=True) to the new shape."""
+    shape = (new_res,) * 3 if isinstance(new_res, int) else tuple(new_res)
+    return F.interpolate(grid5.detach(), size=shape, mode="trilinear", align_corners=True)
+
+
 def compute_analytical_gradient(grid5, coords, voxel_size, grid_res):
     """gradients.py:142-220 (Eq. 4): d/dx of the trilinear interpolant of the cell's 8
     corner values, the ablation baseline of VoxNeuS(gradient_mode="analytical")."""
@@ -316,6 +322,34 @@ def is_occupied(mask, xyz_min, xyz_max, pts):
     c = torch.clamp(norm * float(res), 0.0, float(res - 1.0001)).to(torch.int64)
     occ = mask[c[:, 0], c[:, 1], c[:, 2]] & inb
     return occ.reshape(shape)
+
+
+def logistic_density(sdf, s):
+    """occupancy_grid.py:15-22: phi_s(d) = s * sigmoid(s d) * (1 - sigmoid(s d))."""
+    sig = torch.sigmoid(s * sdf)
+    return s * sig * (1.0 - sig)
+
+
+def occupancy_update(density, mask, xyz_min, xyz_max, grid5, box: "GridBox", s, idx_x, idx_y, idx_z,
+                     first_update, decay=0.95, threshold=0.01):
+    """occupancy_grid.py:62-119 with the cells given (the reference draws them with
+    torch.randint): density and mask ([res]^3, float32 / bool) are updated in place."""
+    res = density.shape[0]
+    step = (xyz_max - xyz_min) / float(res)
+    nx = (idx_x.float() + 0.5) / float(res)
+    ny = (idx_y.float() + 0.5) / float(res)
+    nz = (idx_z.float() + 0.5) / float(res)
+    pts = torch.stack([xyz_min[0] + nx * (xyz_max[0] - xyz_min[0]), xyz_min[1] + ny * (xyz_max[1] - xyz_min[1]),
+                       xyz_min[2] + nz * (xyz_max[2] - xyz_min[2])], dim=-1)
+    sdf = query_sdf(grid5, box, pts).squeeze(-1)
+    dens = logistic_density(sdf, s)
+    voxel_diag = torch.linalg.norm(step).item()
+    near = (torch.abs(sdf) < 2.0 * voxel_diag).float()
+    eff = torch.maximum(dens, near * (threshold * 2.0))
+    prev = density[idx_x, idx_y, idx_z]
+    upd = eff if first_update else torch.maximum(prev * decay, eff)
+    density[idx_x, idx_y, idx_z] = upd
+    mask[idx_x, idx_y, idx_z] = upd > threshold
 
 
 def sample_pdf(bins, weights, n_samples, det=False, eps=1e-5, u=None, record=None):

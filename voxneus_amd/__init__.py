@@ -22,6 +22,7 @@ _LAZY = {
     "amplify_s_gradient": "training", "FusedTrainStep": "training", "StepConfig": "training",
     "FusedOptimizer": "training", "compute_analytical_gradient": "sdf_grid",
     "SparseGridAdam": "optim", "NetAdam": "optim", "create_optimizer": "optim",
+    "upsample_sdf_grid": "sdf_grid", "logistic_density": "rendering",
 }
 
 

@@ -194,6 +194,51 @@ static GridDesc coords_desc(const float* grid, int X, int Y, int Z, const float*
   return g;
 }
 
+// Progressive super-resolution (geometry/upsample.py:12-39): F.interpolate(trilinear,
+// align_corners=True) of the SDF grid to a new resolution, with PyTorch's CPU kernel
+// arithmetic: per axis, ratio = (in - 1) / (out - 1) and src = ratio * o in float,
+// i0 = floor(src), l1 = clamp(src - i0, 0, 1), l0 = 1 - l1 (a copy when in == out);
+// the 8 corners are combined z first, then y, then x, each pair as fma(t0, l0, t1 * l1).
+VXN_DEV void upsample_axis(int64_t o, int64_t n_in, int64_t n_out, int64_t& i0, int64_t& i1, float& l0, float& l1) {
+  if (n_in == n_out) {
+    i0 = i1 = o;
+    l0 = 1.0f;
+    l1 = 0.0f;
+    return;
+  }
+  const float ratio = n_out > 1 ? (float)(n_in - 1) / (float)(n_out - 1) : 0.0f;
+  const float src = ratio * (float)o;
+  i0 = min((int64_t)floorf(src), n_in - 1);
+  l1 = fminf(fmaxf(src - (float)i0, 0.0f), 1.0f);
+  i1 = i0 + (i0 < n_in - 1 ? 1 : 0);
+  l0 = 1.0f - l1;
+}
+
+__global__ void __launch_bounds__(256) upsample_grid_kernel(const float* __restrict__ in, int X, int Y, int Z,
+                                                            float* __restrict__ out, int NX, int NY, int NZ) {
+  const int64_t n = (int64_t)NX * NY * NZ;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t ox = i / ((int64_t)NY * NZ), oy = (i / NZ) % NY, oz = i % NZ;
+    int64_t x[2], y[2], z[2];
+    float a[2], b[2], c[2];
+    upsample_axis(ox, X, NX, x[0], x[1], a[0], a[1]);
+    upsample_axis(oy, Y, NY, y[0], y[1], b[0], b[1]);
+    upsample_axis(oz, Z, NZ, z[0], z[1], c[0], c[1]);
+    float t[2];
+#pragma unroll
+    for (int dx = 0; dx < 2; ++dx) {
+      float u[2];
+#pragma unroll
+      for (int dy = 0; dy < 2; ++dy) {
+        const float* row = in + (x[dx] * Y + y[dy]) * (int64_t)Z;
+        u[dy] = fmaf(row[z[0]], c[0], row[z[1]] * c[1]);
+      }
+      t[dx] = fmaf(u[0], b[0], u[1] * b[1]);
+    }
+    out[i] = fmaf(t[0], a[0], t[1] * a[1]);
+  }
+}
+
 // SDF + unit normal over a packed batch of points of many scenes (the serving query).
 __global__ void __launch_bounds__(256) serve_query_kernel(const SceneDev* __restrict__ scenes, int n_scenes,
                                                           const int32_t* __restrict__ pt_scene,
@@ -349,6 +394,17 @@ extern "C" int vxn_lattice_query(const float* grid, int X, int Y, int Z, const f
   hipLaunchKernelGGL(lattice_query_kernel, dim3(nb), dim3(256), 0, (hipStream_t)stream, g,
                      NX, NY, NZ, lb[0], lb[1], lb[2], (lb[3] - lb[0]) / (float)(NX - 1), (lb[4] - lb[1]) / (float)(NY - 1),
                      (lb[5] - lb[2]) / (float)(NZ - 1), sdf, normal);
+  VXN_LAUNCH_CHECK();
+  return 0;
+}
+
+extern "C" int vxn_upsample_grid(const float* grid, int X, int Y, int Z, float* out, int NX, int NY, int NZ,
+                                 void* stream) {
+  VXN_REQUIRE(grid != nullptr && out != nullptr, VXN_ERR_NULL, "null grid");
+  VXN_REQUIRE(X >= 1 && Y >= 1 && Z >= 1 && NX >= 1 && NY >= 1 && NZ >= 1, VXN_ERR_SHAPE, "sizes must be >= 1");
+  const int64_t n = (int64_t)NX * NY * NZ;
+  hipLaunchKernelGGL(upsample_grid_kernel, dim3(blocks_for(n, 256, 65536)), dim3(256), 0, (hipStream_t)stream, grid,
+                     X, Y, Z, out, NX, NY, NZ);
   VXN_LAUNCH_CHECK();
   return 0;
 }

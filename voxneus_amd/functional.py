@@ -511,3 +511,53 @@ def curvature_grad_ref(grid3, active, grad3, eps3, weight):
 def s_amplify(grad_v: torch.Tensor, k: float, amplified: Optional[torch.Tensor] = None):
     N.check(N.lib().vxn_s_amplify(N.dptr(grad_v, torch.float32, "grad_v"), float(k), N.dptr(amplified),
                                   _stream(grad_v)), "s_amplify")
+
+
+# ----------------------------------------------------------------------------------------
+# Occupancy grid EMA update
+# ----------------------------------------------------------------------------------------
+
+def occupancy_update(density: torch.Tensor, mask: torch.Tensor, occ_box, grid3: torch.Tensor, box, s: float,
+                     decay: float, threshold: float, first_update: bool, idx=None,
+                     workspace: Optional[torch.Tensor] = None):
+    """OccupancyGrid.update_from_sdf (occupancy_grid.py:62-119) on the device.  ``idx`` =
+    (idx_x, idx_y, idx_z) int64 device tensors of the cells to update, or None for every
+    cell.  ``mask`` is the bool occupancy mask, updated in place with ``density``."""
+    res = density.shape[0]
+    if tuple(density.shape) != (res, res, res) or tuple(mask.shape) != (res, res, res):
+        raise ValueError("density and mask must be [res, res, res]")
+    if mask.dtype != torch.bool or not mask.is_contiguous() or not density.is_contiguous():
+        raise ValueError("mask must be a contiguous bool tensor, density contiguous float32")
+    g3 = grid3.contiguous()
+    X, Y, Z = g3.shape
+    ix = iy = iz = None
+    n = 0
+    ws = None
+    if idx is not None:
+        ix, iy, iz = (t.contiguous() for t in idx)
+        n = ix.numel()
+        if iy.numel() != n or iz.numel() != n:
+            raise ValueError("the three index arrays must have the same length")
+        nb = ctypes.c_int64(0)
+        N.check(N.lib().vxn_occupancy_update_workspace_bytes(n, ctypes.byref(nb)))
+        ws = workspace
+        if ws is None or ws.numel() * ws.element_size() < nb.value:
+            ws = torch.empty(max(int(nb.value), 1), device=density.device, dtype=torch.uint8)
+    N.check(N.lib().vxn_occupancy_update(
+        N.dptr(density, torch.float32, "density"), mask.data_ptr(), res, N.box6(*occ_box),
+        N.dptr(g3, torch.float32, "grid"), X, Y, Z, N.box6(*box), float(s), float(decay), float(threshold),
+        int(bool(first_update)), N.dptr(ix, torch.int64, "idx_x"), N.dptr(iy, torch.int64, "idx_y"),
+        N.dptr(iz, torch.int64, "idx_z"), n, N.dptr(ws), 0 if ws is None else ws.numel() * ws.element_size(),
+        _stream(density)), "occupancy_update")
+
+
+def upsample_grid(grid: torch.Tensor, new_shape) -> torch.Tensor:
+    """Trilinear, aligned-corner resampling of an [X,Y,Z] (or [1,1,X,Y,Z]) grid to
+    ``new_shape`` (upsample.py:12-39); returns a new [NX,NY,NZ] tensor."""
+    g3 = _grid3(grid).contiguous()
+    NX, NY, NZ = (int(v) for v in new_shape)
+    out = torch.empty(NX, NY, NZ, device=g3.device, dtype=torch.float32)
+    X, Y, Z = g3.shape
+    N.check(N.lib().vxn_upsample_grid(N.dptr(g3, torch.float32, "grid"), X, Y, Z, N.dptr(out), NX, NY, NZ,
+                                      _stream(g3)), "upsample_grid")
+    return out

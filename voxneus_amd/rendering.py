@@ -21,10 +21,17 @@ from . import functional as Fn
 from .training import near_far_from_sphere  # noqa: F401  (re-exported, rays.py:102-128)
 
 
+def logistic_density(sdf: torch.Tensor, s: torch.Tensor) -> torch.Tensor:
+    """occupancy_grid.py:15-22: phi_s(x) = s * sigmoid(s x) * (1 - sigmoid(s x)) (elementwise,
+    on the tensors' device; the occupancy update evaluates it inside its kernel)."""
+    sig = torch.sigmoid(s * sdf)
+    return s * sig * (1.0 - sig)
+
+
 class OccupancyGrid(nn.Module):
     """Occupancy mask consumed by the upsampler (occupancy_grid.py:33-147).  A fresh grid
-    is all True; the EMA update (``update_from_sdf``) runs before the path and is not part
-    of it."""
+    is all True; ``update_from_sdf`` is the EMA update from the SDF grid's logistic density
+    (occupancy_grid.py:62-119) on the device."""
 
     def __init__(self, resolution: int = 128, xyz_min=(-1.0, -1.0, -1.0), xyz_max=(1.0, 1.0, 1.0),
                  decay: float = 0.95, threshold: float = 0.01):
@@ -45,6 +52,28 @@ class OccupancyGrid(nn.Module):
     @property
     def occupied_percentage(self) -> float:
         return (self.occupancy_mask.sum().item() / self.occupancy_mask.numel()) * 100.0
+
+    @torch.no_grad()
+    def update_from_sdf(self, query_sdf_fn, s_val, subsample_ratio: float = 0.25, generator=None,
+                        full_sweep: bool = False) -> None:
+        """occupancy_grid.py:62-119.  ``query_sdf_fn`` must be a DenseSDFGrid's ``query_sdf``
+        (the kernel evaluates the grid itself).  The cells are drawn as the reference draws
+        them (three torch.randint vectors of max(res^3 * ratio, 1024) cells, with
+        replacement, on the grid's device); ``full_sweep`` updates every cell once instead."""
+        grid = getattr(query_sdf_fn, "__self__", None)
+        if grid is None or not hasattr(grid, "grid"):
+            raise NotImplementedError("the occupancy update evaluates a DenseSDFGrid directly")
+        dev = self.density_grid.device
+        res = self.resolution
+        idx = None
+        if not full_sweep:
+            n = max(int(res * res * res * subsample_ratio), 1024)
+            idx = tuple(torch.randint(0, res, (n,), device=dev, generator=generator) for _ in range(3))
+        s = float(s_val.detach()) if torch.is_tensor(s_val) else float(s_val)
+        Fn.occupancy_update(self.density_grid, self.occupancy_mask, self.box6,
+                            grid.grid.detach().reshape(grid.grid.shape[-3:]), grid.box6, s, self.decay,
+                            self.threshold, self.total_updates == 0, idx)
+        self.total_updates += 1
 
 
 class RaySampler(nn.Module):

@@ -101,6 +101,15 @@ class DenseSDFGrid(nn.Module):
         return Fn.lattice_query(self.grid.detach().reshape(self.grid.shape[-3:]), self.box, NX, NY, NZ,
                                 tuple(lo + hi), N.gradient_mode_code(gradient_mode))
 
+    def scale_volume_grid(self, new_resolution: Union[int, List[int], Tuple[int, int, int]]) -> None:
+        """sdf_grid.py:183-195: replace the grid by its trilinear upsampling (aligned corners)
+        to ``new_resolution``, on the device."""
+        self.grid = nn.Parameter(upsample_sdf_grid(self.grid, new_resolution))
+        if isinstance(new_resolution, int):
+            self.res = torch.tensor([new_resolution] * 3, dtype=torch.float32)
+        else:
+            self.res = torch.tensor(list(new_resolution), dtype=torch.float32)
+
     def get_numpy_grid(self) -> np.ndarray:
         return self.grid.detach().squeeze().cpu().numpy()
 
@@ -125,3 +134,10 @@ def compute_analytical_gradient(grid, continuous_coords, voxel_size, grid_res):
     """gradients.py:142-220: the Eq. 4 gradient and its unit normal on continuous grid
     coordinates; the gradient is differentiable in the grid."""
     return Fn.compute_analytical_gradient(grid, continuous_coords, voxel_size, grid_res)
+
+
+def upsample_sdf_grid(grid_tensor: torch.Tensor, new_res) -> torch.Tensor:
+    """upsample.py:12-39: (1, 1, X, Y, Z) -> (1, 1, NX, NY, NZ), trilinear with aligned
+    corners (``vxn_upsample_grid``)."""
+    shape = (new_res,) * 3 if isinstance(new_res, int) else tuple(int(v) for v in new_res)
+    return Fn.upsample_grid(grid_tensor.detach(), shape).reshape(1, 1, *shape)

@@ -90,6 +90,8 @@ class FusedTrainStep:
             # (occupancy_grid.py:51-54), i.e. factor 1 inside the box and 0.1 outside it
             occ_mask = torch.ones((128, 128, 128), device=dev, dtype=torch.bool)
         self.occ_mask = occ_mask
+        self.occ_density = torch.zeros(occ_mask.shape, device=dev, dtype=torch.float32)
+        self.occ_updates = 0
         X, Y, Z = grid.shape
         self.grad_grid = torch.zeros_like(grid)
         self.grad_table = torch.zeros(table.shape, device=dev, dtype=torch.float32)
@@ -163,6 +165,22 @@ class FusedTrainStep:
             Fn.s_amplify(self.grad_v, cfg.s_amp_k, self.amplified)
         return {"loss": self.loss, "reg_losses": self.reg_losses, "vr_counts": self.vr_counts,
                 "amplified": self.amplified, "z_vals": z, "dists": dists, "mid_z_vals": mid}
+
+    def update_occupancy(self, subsample_ratio: float = 0.25, generator=None, full_sweep: bool = False,
+                         decay: float = 0.95, threshold: float = 0.01):
+        """The trainer's periodic occupancy update (trainer.py:171-173, every 16 steps;
+        occupancy_grid.py:62-119) of this step's occupancy mask over its box, at the
+        current s."""
+        res = self.occ_mask.shape[0]
+        idx = None
+        if not full_sweep:
+            n = max(int(res ** 3 * subsample_ratio), 1024)
+            idx = tuple(torch.randint(0, res, (n,), device=self.grid.device, generator=generator)
+                        for _ in range(3))
+        s = float(torch.clamp(torch.exp(self.variance.detach() * 10.0), min=1e-4, max=self.s_max))
+        Fn.occupancy_update(self.occ_density, self.occ_mask, self.box, self.grid, self.box, s, decay, threshold,
+                            self.occ_updates == 0, idx)
+        self.occ_updates += 1
 
     def active_linear(self) -> torch.Tensor:
         """Sorted V_R linear indices of the last step after the relaxed-sphere filter

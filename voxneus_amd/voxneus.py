@@ -54,6 +54,10 @@ class VoxNeuS(nn.Module):
     def std(self) -> torch.Tensor:
         return self.variance_net.std
 
+    def update_occupancy_grid(self, subsample_ratio: float = 0.25) -> None:
+        """voxneus.py:105-113: the occupancy EMA update at the current s."""
+        self.occupancy_grid.update_from_sdf(self.sdf_grid.query_sdf, self.s.detach(), subsample_ratio)
+
     def forward_points(self, xyz, view_dirs, active_levels: Optional[float] = None) -> Dict[str, torch.Tensor]:
         """voxneus.py:115-167."""
         sdf = self.sdf_grid.query_sdf(xyz)
