@@ -107,6 +107,26 @@ int vxn_lattice_query(const float* grid, int X, int Y, int Z, const float* box6,
  * (models/sdf_grid.py:183-195); out must not overlap grid.                           */
 int vxn_upsample_grid(const float* grid, int X, int Y, int Z, float* out, int NX, int NY, int NZ, void* stream);
 
+/* ---- Mesh extraction -----------------------------------------------------------------
+ * Marching cubes of the grid at `level` (value < level is inside), replacing the
+ * scikit-image call of extract_mesh_marching_cubes (geometry/mesh.py:34-77): one vertex
+ * per grid edge whose end values straddle the level, at the linear-interpolation
+ * crossing, mapped to world space as index * (xyz_max - xyz_min) / (res - 1) + xyz_min
+ * (mesh.py:71-73); triangles oriented with normals towards increasing values, from a
+ * case table built on faces so the mesh is crack-free (vxn_mc_case_table returns it:
+ * tri [256][16] edge triples, -1 terminated, ntri [256]; host only).
+ * Two calls: vxn_mc_count writes counts[0] = vertices and counts[1] = triangles (device
+ * int32) and leaves offsets in the workspace; vxn_mc_emit, on the same grid and
+ * workspace, writes verts [V,3] float32 and faces [F,3] int32 (entries beyond the
+ * capacities are dropped).                                                            */
+int vxn_mc_case_table(int8_t* tri, uint8_t* ntri);
+int vxn_mc_workspace_bytes(int X, int Y, int Z, int64_t* bytes);
+int vxn_mc_count(const float* grid, int X, int Y, int Z, float level, void* workspace, int64_t workspace_bytes,
+                 int32_t* counts, void* stream);
+int vxn_mc_emit(const float* grid, int X, int Y, int Z, const float* box6, float level, void* workspace,
+                int64_t workspace_bytes, float* verts, int64_t max_verts, int32_t* faces, int64_t max_faces,
+                void* stream);
+
 /* ---- Multiresolution hash encoding ------------------------------------------------
  * table: [L, 2^log2T, F] float32 (dtype 0) or bfloat16 (dtype 1); F must be 2.
  * level_res (host, L entries): floor(base * b^l) as computed by the reference;

@@ -812,3 +812,177 @@ class RefModel:
             self.variance.grad = g
         return {"loss": loss.detach(), "loss_eik": le, "loss_curv": lc, "active_linear": lin,
                 "active_vertices": verts, "amplified": amplified, "render": out, "near": near, "far": far}
+
+
+# ----------------------------------------------------------------------------------------
+# Mesh extraction (Code/voxneus/geometry/mesh.py:34-77)
+# The reference calls scikit-image's marching_cubes, which is not installed here; this is
+# marching cubes with one vertex per straddling grid edge at the linear-interpolation
+# crossing (the vertex set scikit-image's Lewiner variant also produces) and triangles
+# from a face-consistent case table.  Pinned by the mesh's properties (closed, oriented,
+# Euler characteristic, enclosed volume and area, vertices on the zero crossing), not by
+# scikit-image's triangulation.
+# ----------------------------------------------------------------------------------------
+
+def _mc_edge(c0, c1):
+    d = c0 ^ c1
+    a = {4: 0, 2: 1, 1: 2}[d]
+    lo = c0 & ~d
+    u, v = [(lo >> (2 - b)) & 1 for b in range(3) if b != a]
+    return 4 * a + 2 * u + v
+
+
+def _mc_edge_corners(e):
+    a, uv = e // 4, ((e >> 1) & 1, e & 1)
+    base = [0, 0, 0]
+    for b, x in zip([b for b in range(3) if b != a], uv):
+        base[b] = x
+    c0 = (base[0] << 2) | (base[1] << 1) | base[2]
+    return c0, c0 | (1 << (2 - a))
+
+
+def _mc_edge_faces(e):
+    """The two cube faces (axis, side) containing edge e."""
+    c0, c1 = _mc_edge_corners(e)
+    a = e // 4
+    return {(b, (c0 >> (2 - b)) & 1) for b in range(3) if b != a}
+
+
+def mc_case_table():
+    """For every case (bit k set = corner k = 4ox+2oy+oz inside), the triangles as edge
+    triples: on each face, walked counter-clockwise about its outward normal, every run of
+    inside corners is cut off by a segment from the crossing leaving the run to the
+    crossing entering it; segments chain into loops (edge by edge across faces), each
+    loop oriented with its normal from inside to outside corners and fanned from its
+    first edge."""
+    import numpy as np
+    pos = lambda k: np.array([(k >> 2) & 1, (k >> 1) & 1, k & 1], float)
+    faces = []
+    for a in range(3):
+        for s in (0, 1):
+            cs = [k for k in range(8) if ((k >> (2 - a)) & 1) == s]
+            n = np.zeros(3)
+            n[a] = 1.0 if s else -1.0
+            t1 = np.zeros(3)
+            t1[(a + 1) % 3] = 1.0
+            t2 = np.cross(n, t1)
+            ctr = sum(pos(k) for k in cs) / 4.0
+            cs.sort(key=lambda k: math.atan2(float(np.dot(pos(k) - ctr, t2)), float(np.dot(pos(k) - ctr, t1))))
+            faces.append(cs)
+    table = []
+    for case in range(256):
+        inside = [(case >> k) & 1 for k in range(8)]
+        succ = {}
+        for cs in faces:
+            for i in range(4):
+                if inside[cs[i]] and not inside[cs[(i + 1) % 4]]:
+                    j = i
+                    while inside[cs[(j - 1) % 4]]:
+                        j = (j - 1) % 4
+                    succ[_mc_edge(cs[i], cs[(i + 1) % 4])] = _mc_edge(cs[(j - 1) % 4], cs[j])
+        tris, seen = [], set()
+        for e0 in range(12):
+            if e0 not in succ or e0 in seen:
+                continue
+            loop, e = [e0], succ[e0]
+            seen.add(e0)
+            while e != e0:
+                loop.append(e)
+                seen.add(e)
+                e = succ[e]
+            mids = [sum(pos(c) for c in _mc_edge_corners(x)) / 2.0 for x in loop]
+            nrm = sum(np.cross(mids[i], mids[(i + 1) % len(mids)]) for i in range(len(mids)))
+            cor = [c for x in loop for c in _mc_edge_corners(x)]
+            ins = np.mean([pos(c) for c in cor if inside[c]], 0)
+            outs = np.mean([pos(c) for c in cor if not inside[c]], 0)
+            if float(np.dot(nrm, outs - ins)) < 0:
+                loop = loop[::-1]
+            # fan from the first loop vertex whose fan has no triangle lying in a cube face
+            # (two cells sharing the face would both cover it)
+            m = len(loop)
+            for k in range(m):
+                cand = loop[k:] + loop[:k]
+                fan = [(cand[0], cand[i], cand[i + 1]) for i in range(1, m - 1)]
+                if not any(_mc_edge_faces(a) & _mc_edge_faces(b) & _mc_edge_faces(c) for a, b, c in fan):
+                    break
+            else:
+                raise AssertionError(f"case {case}: no fan avoids the cube faces")
+            tris += fan
+        table.append(tris)
+    return table
+
+
+def marching_cubes(volume, level=0.0, table=None):
+    """Vertices in grid-index space (float64 [V,3]; numbered by owning grid vertex, the
+    edge's low end, in grid order, then by axis) and triangles (int32 [F,3]; by cell in
+    grid order, then table order).  value < level is inside."""
+    import numpy as np
+    vol = np.asarray(volume, dtype=np.float32)
+    X, Y, Z = vol.shape
+    table = mc_case_table() if table is None else table
+    inside = vol < np.float32(level)
+    vmask = np.zeros(vol.shape, np.int64)
+    vmask[:-1, :, :] |= (inside[:-1] != inside[1:]).astype(np.int64)
+    vmask[:, :-1, :] |= (inside[:, :-1] != inside[:, 1:]).astype(np.int64) << 1
+    vmask[:, :, :-1] |= (inside[:, :, :-1] != inside[:, :, 1:]).astype(np.int64) << 2
+    flat = vmask.reshape(-1)
+    cnt = ((flat & 1) + ((flat >> 1) & 1) + ((flat >> 2) & 1)).astype(np.int64)
+    voff = np.concatenate([[0], np.cumsum(cnt)[:-1]])
+    owners, axes = [], []
+    for a in range(3):
+        idx = np.nonzero((flat >> a) & 1)[0]
+        owners.append(idx)
+        axes.append(np.full(idx.shape, a))
+    owners, axes = np.concatenate(owners), np.concatenate(axes)
+    order = np.lexsort((axes, owners))
+    owners, axes = owners[order], axes[order]
+    p = np.stack(np.unravel_index(owners, vol.shape), -1).astype(np.float64)
+    step = np.array([Y * Z, Z, 1])[axes]
+    v0 = vol.reshape(-1)[owners].astype(np.float64)
+    v1 = vol.reshape(-1)[owners + step].astype(np.float64)
+    t = (float(np.float32(level)) - v0) / (v1 - v0)
+    p[np.arange(p.shape[0]), axes] += t
+    cx, cy, cz = np.meshgrid(np.arange(X - 1), np.arange(Y - 1), np.arange(Z - 1), indexing="ij")
+    base = (cx * Y * Z + cy * Z + cz).reshape(-1)
+    case = np.zeros(base.shape, np.int64)
+    for k in range(8):
+        ox, oy, oz = (k >> 2) & 1, (k >> 1) & 1, k & 1
+        case |= inside[ox:X - 1 + ox, oy:Y - 1 + oy, oz:Z - 1 + oz].reshape(-1).astype(np.int64) << k
+    keys, faces = [], []
+    for c in range(256):
+        tris = table[c]
+        if not tris:
+            continue
+        cells = base[case == c]
+        if cells.size == 0:
+            continue
+        for ti, tri in enumerate(tris):
+            ids = []
+            for e in tri:
+                a, u, v = e // 4, (e >> 1) & 1, e & 1
+                off = [0, 0, 0]
+                for b, x in zip([b for b in range(3) if b != a], (u, v)):
+                    off[b] = x
+                own = cells + off[0] * Y * Z + off[1] * Z + off[2]
+                below = flat[own] & ((1 << a) - 1)
+                ids.append(voff[own] + (below & 1) + ((below >> 1) & 1))
+            faces.append(np.stack(ids, -1))
+            keys.append(np.stack([cells, np.full(cells.shape, ti)], -1))
+    if faces:
+        faces, keys = np.concatenate(faces), np.concatenate(keys)
+        faces = faces[np.lexsort((keys[:, 1], keys[:, 0]))].astype(np.int32)
+    else:
+        faces = np.zeros((0, 3), np.int32)
+    return p, faces
+
+
+def extract_mesh_marching_cubes(sdf_grid, xyz_min, xyz_max, level=0.0):
+    """mesh.py:34-77 (without the optional Gaussian pre-filter): world-space float32
+    vertices, index * (xyz_max - xyz_min) / (res - 1) + xyz_min, and int32 faces."""
+    import numpy as np
+    verts, faces = marching_cubes(sdf_grid, level)
+    res = np.array(np.asarray(sdf_grid).shape, dtype=np.float64)
+    mn = np.asarray(xyz_min, dtype=np.float32)
+    mx = np.asarray(xyz_max, dtype=np.float32)
+    scale = (mx - mn) / (res - 1.0)
+    return (verts * scale + mn).astype(np.float32), faces

@@ -22,7 +22,9 @@ _LAZY = {
     "amplify_s_gradient": "training", "FusedTrainStep": "training", "StepConfig": "training",
     "FusedOptimizer": "training", "compute_analytical_gradient": "sdf_grid",
     "SparseGridAdam": "optim", "NetAdam": "optim", "create_optimizer": "optim",
-    "upsample_sdf_grid": "sdf_grid", "logistic_density": "rendering",
+    "upsample_sdf_grid": "sdf_grid", "get_resolution_for_step": "sdf_grid", "logistic_density": "rendering",
+    "extract_mesh_marching_cubes": "mesh", "marching_cubes": "mesh",
+    "save_checkpoint": "checkpoint", "load_checkpoint": "checkpoint", "load_resident_scene": "checkpoint",
 }
 
 

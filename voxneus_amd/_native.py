@@ -114,6 +114,10 @@ _SIGS = {
     "vxn_voxel_corners": (c_i32, [c_vp, c_i64, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp]),
     "vxn_lattice_query": (c_i32, [c_vp, c_i32, c_i32, c_i32, c_vp, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_i32,
                                   c_vp]),
+    "vxn_mc_case_table": (c_i32, [c_vp, c_vp]),
+    "vxn_mc_workspace_bytes": (c_i32, [c_i32, c_i32, c_i32, c_vp]),
+    "vxn_mc_count": (c_i32, [c_vp, c_i32, c_i32, c_i32, c_f, c_vp, c_i64, c_vp, c_vp]),
+    "vxn_mc_emit": (c_i32, [c_vp, c_i32, c_i32, c_i32, c_vp, c_f, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp]),
     "vxn_upsample_grid": (c_i32, [c_vp, c_i32, c_i32, c_i32, c_vp, c_i32, c_i32, c_i32, c_vp]),
     "vxn_occupancy_update_workspace_bytes": (c_i32, [c_i64, c_vp]),
     "vxn_occupancy_update": (c_i32, [c_vp, c_vp, c_i32, c_vp, c_vp, c_i32, c_i32, c_i32, c_vp, c_f, c_f, c_f, c_i32,

@@ -141,3 +141,13 @@ def upsample_sdf_grid(grid_tensor: torch.Tensor, new_res) -> torch.Tensor:
     corners (``vxn_upsample_grid``)."""
     shape = (new_res,) * 3 if isinstance(new_res, int) else tuple(int(v) for v in new_res)
     return Fn.upsample_grid(grid_tensor.detach(), shape).reshape(1, 1, *shape)
+
+
+def get_resolution_for_step(step: int, milestones=(10000, 30000), resolutions=(96, 160, 320)) -> int:
+    """upsample.py:42-63: the grid resolution of a training step (96^3 until 10k, 160^3 until
+    30k, then 320^3)."""
+    if step < milestones[0]:
+        return resolutions[0]
+    if step < milestones[1]:
+        return resolutions[1]
+    return resolutions[2]
