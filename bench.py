@@ -118,7 +118,12 @@ def run_gpu(sc, batches, form, steps, warmup):
         out = one(i, True)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    kern = {k: float(np.mean([e.elapsed_ms(i, j) for e in ev])) for k, (i, j) in KERNEL_EVENTS.items()}
+    pairs = dict(KERNEL_EVENTS)
+    if st.side is not None:
+        # the hash scatter runs on the side stream (events 2 -> 3); the grid scatter follows
+        # the render kernel directly on the main stream (events 2 -> 4)
+        pairs["grid_scatter"] = (2, 4)
+    kern = {k: float(np.mean([e.elapsed_ms(i, j) for e in ev])) for k, (i, j) in pairs.items()}
     loss = out["loss"].item()
     return {"ms_per_step": dt / steps * 1e3, "rays_per_s": RAYS * steps / dt, "kernel_ms": kern, "loss": loss}
 
@@ -182,6 +187,88 @@ def run_lattice(reps=10):
             "bytes_per_point": LATTICE_BYTES_PER_POINT, "achieved": achieved, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "compulsory_GBps": (grid.numel() * 4 + pts * 16) / (ms * 1e-3) / 1e9}
+
+
+def _event_ms(fn, reps=5):
+    """Mean HIP-event time (ms) of ``fn`` on the current stream over ``reps`` calls after
+    one warm-up call."""
+    import torch
+
+    from voxneus_amd import _native as N
+    fn()
+    evs = []
+    for _ in range(reps):
+        e = N.KernelEvents(2)
+        st = torch.cuda.current_stream().cuda_stream
+        e.record(0, st)
+        fn()
+        e.record(1, st)
+        evs.append(e)
+    torch.cuda.synchronize()
+    return float(np.mean([e.elapsed_ms(0, 1) for e in evs]))
+
+
+def run_pipeline_stages(sc, batches, form):
+    """The steps around the training step and after training, at the bench sizes: both
+    optimizers after a fused step (256^3 grid, 16 x 2^19 hash), the occupancy update (128^3:
+    a full sweep and the trainer's quarter of random cells), progressive super-resolution
+    160^3 -> 320^3, and marching cubes on a 320^3 grid."""
+    import torch
+
+    from voxneus_amd import _native as N
+    from voxneus_amd import functional as Fn
+    from voxneus_amd import scenes
+    from voxneus_amd.hash_grid import level_resolutions
+    from voxneus_amd.mesh import marching_cubes
+    from voxneus_amd.training import FusedOptimizer, FusedTrainStep, StepConfig
+
+    dev = torch.device("cuda:0")
+    table = torch.from_numpy(sc.embeddings).to(dev)
+    mlp = mlp_flat(sc).to(dev)
+    if form == "bf16":
+        table = table.to(torch.bfloat16)
+    st = FusedTrainStep(torch.from_numpy(sc.grid).to(dev), table, mlp, torch.tensor(0.3, device=dev),
+                        level_resolutions(16, 16, sc.hash_max_resolution),
+                        cfg=StepConfig(perturb=True, mlp_mode=N.VXN_MLP_BF16_MFMA if form == "bf16" else N.VXN_MLP_FP32))
+    o, d, tgt = (torch.from_numpy(a).to(dev) for a in batches[0])
+    st.step(o, d, tgt)
+    opt = FusedOptimizer(st, total_steps=40000)
+    it = [0]
+
+    def opt_step():
+        opt.step(it[0])
+        it[0] += 1
+
+    out = {}
+    ms = _event_ms(opt_step)
+    n_grid, n_tab = st.grid.numel(), st.table.numel()
+    upd = int(opt.n_updated.item())
+    # masked grid Adam: gradient read for every voxel, 28 B per updated voxel; the network
+    # Adam: 28 B per element (+2 B bf16 copy)
+    nbytes = 4 * n_grid + 28 * upd + (28 + (2 if form == "bf16" else 0)) * n_tab + 28 * (mlp.numel() + 1)
+    out["optimizers"] = {"ms": ms, "grid_voxels": n_grid, "grid_updated": upd, "table_entries": n_tab,
+                         "algorithmic_GBps": nbytes / (ms * 1e-3) / 1e9}
+    out["occupancy_sweep_128"] = {"ms": _event_ms(lambda: st.update_occupancy(full_sweep=True))}
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(0)
+    out["occupancy_quarter_128"] = {"ms": _event_ms(lambda: st.update_occupancy(0.25, generator=gen))}
+    g160 = torch.from_numpy(scenes.sdf_grid_from(scenes.MultiPrimitive.seeded(3).sdf, 160)).to(dev)
+    ms = _event_ms(lambda: Fn.upsample_grid(g160, (LATTICE,) * 3))
+    out["upsample_160_to_320"] = {"ms": ms, "algorithmic_GBps": (g160.numel() * 4 + LATTICE ** 3 * 4) / (ms * 1e-3) / 1e9}
+    g320 = Fn.upsample_grid(g160, (LATTICE,) * 3)
+    box6 = (-1.0, -1.0, -1.0, 1.0, 1.0, 1.0)
+    v, f = marching_cubes(g320, box6)
+    torch.cuda.synchronize()
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        v, f = marching_cubes(g320, box6)
+    torch.cuda.synchronize()
+    out["marching_cubes_320"] = {"ms_wall_incl_count_read": (time.perf_counter() - t0) / reps * 1e3,
+                                 "vertices": int(v.shape[0]), "faces": int(f.shape[0])}
+    del st, opt, g160, g320, v, f
+    torch.cuda.empty_cache()
+    return out
 
 
 SERVE_RENDER_SIZES = ((64, 64), (200, 150), (400, 300), (800, 600))
@@ -379,6 +466,9 @@ def main():
     nbytes = KERNEL_BYTES[main_form][dom] * ns
     achieved = r["kernels"][dom]["achieved_GBps"]
     lat = None if a.no_lattice else run_lattice()
+    stages = None if a.no_lattice else run_pipeline_stages(sc, batches, main_form)
+    if stages:
+        log(f"pipeline stages: {stages}")
     if lat:
         log(f"lattice: {lat}")
     full = None
@@ -412,6 +502,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(dom, main_form),
                      "kernel": dom, "kernel_ms": kms[dom], "bytes_per_launch": nbytes},
         "grid_gather": lat,
+        "pipeline_stages": stages,
         "full_scale_render": full,
         "serving": serving,
         "cpu_baseline": cpu,

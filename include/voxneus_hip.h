@@ -232,6 +232,13 @@ typedef struct vxn_render_args {
      reductions (forward: only the first three) */
   void* const* kernel_events;
   int32_t gradient_mode; /* VXN_GRADIENT_*: the normal of every sample (VoxNeuS gradient_mode) */
+  /* nullable: a second hipStream_t for the backward's hash-table scatter, which then runs
+     concurrently with the grid scatter, the reductions and whatever the caller queues
+     next on `stream`.  side_events (hipEvent_t[2], required with it): [0] is recorded on
+     `stream` when the scatter may start, [1] on side_stream when it has finished; the
+     caller makes `stream` wait for [1] before it reads grad_table. */
+  void* side_stream;
+  void* const* side_events;
 } vxn_render_args;
 
 int vxn_render_workspace_bytes(const vxn_render_args* a, int64_t* bytes);

@@ -16,7 +16,7 @@ _LAZY = {
     "RadianceMLP": "radiance", "SphericalHarmonicsEmbedder": "radiance", "SingleVarianceNetwork": "radiance",
     "RaySampler": "rendering", "OccupancyGrid": "rendering", "compute_neus_alpha": "rendering",
     "composite_weights": "rendering", "volume_render_from_samples": "rendering",
-    "compute_huber_color_loss": "rendering", "near_far_from_sphere": "training",
+    "compute_huber_color_loss": "rendering", "near_far_from_sphere": "training", "get_rays": "training",
     "extract_active_vertices": "regularization", "compute_eikonal_loss_and_gradient": "regularization",
     "compute_curvature_loss_and_gradient": "regularization",
     "amplify_s_gradient": "training", "FusedTrainStep": "training", "StepConfig": "training",

@@ -66,7 +66,7 @@ class RenderArgs(ctypes.Structure):
         ("target_rgb", c_vp), ("huber_delta", c_f),
         ("grad_grid", c_vp), ("grad_table", c_vp), ("grad_mlp", c_vp), ("grad_s", c_vp), ("loss", c_vp),
         ("vr_bitmask", c_vp), ("workspace", c_vp), ("workspace_bytes", c_i64),
-        ("kernel_events", c_vp), ("gradient_mode", c_i32),
+        ("kernel_events", c_vp), ("gradient_mode", c_i32), ("side_stream", c_vp), ("side_events", c_vp),
     ]
 
 
@@ -179,6 +179,37 @@ def lib():
 
 class HipError(RuntimeError):
     pass
+
+
+class SideStream:
+    """A second stream (default priority) on ``device`` and a pair of untimed HIP events for
+    ``vxn_render_args.side_stream`` / ``side_events``; ``join(stream)`` makes ``stream``
+    wait for the side work queued so far."""
+
+    _hip = None
+
+    def __init__(self, device):
+        if SideStream._hip is None:
+            h = ctypes.CDLL("libamdhip64.so.7")
+            h.hipEventCreateWithFlags.argtypes = [ctypes.POINTER(c_vp), ctypes.c_uint]
+            h.hipStreamWaitEvent.argtypes = [c_vp, c_vp, ctypes.c_uint]
+            h.hipEventDestroy.argtypes = [c_vp]
+            SideStream._hip = h
+        self.stream = torch.cuda.Stream(device=device)
+        self.events = (c_vp * 2)()
+        for i in range(2):
+            ev = c_vp()
+            if SideStream._hip.hipEventCreateWithFlags(ctypes.byref(ev), 2) != 0:  # hipEventDisableTiming
+                raise HipError("hipEventCreateWithFlags failed")
+            self.events[i] = ev
+
+    @property
+    def handle(self) -> int:
+        return self.stream.cuda_stream
+
+    def join(self, stream: int):
+        if SideStream._hip.hipStreamWaitEvent(c_vp(stream), self.events[1], 0) != 0:
+            raise HipError("hipStreamWaitEvent failed")
 
 
 class KernelEvents:

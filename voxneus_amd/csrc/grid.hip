@@ -196,47 +196,59 @@ static GridDesc coords_desc(const float* grid, int X, int Y, int Z, const float*
 
 // Progressive super-resolution (geometry/upsample.py:12-39): F.interpolate(trilinear,
 // align_corners=True) of the SDF grid to a new resolution, with PyTorch's CPU kernel
-// arithmetic: per axis, ratio = (in - 1) / (out - 1) and src = ratio * o in float,
-// i0 = floor(src), l1 = clamp(src - i0, 0, 1), l0 = 1 - l1 (a copy when in == out);
-// the 8 corners are combined z first, then y, then x, each pair as fma(t0, l0, t1 * l1).
-VXN_DEV void upsample_axis(int64_t o, int64_t n_in, int64_t n_out, int64_t& i0, int64_t& i1, float& l0, float& l1) {
-  if (n_in == n_out) {
+// arithmetic: per axis, ratio = (in - 1) / (out - 1) (formed on the host, in float) and
+// src = ratio * o in float, i0 = floor(src), l1 = clamp(src - i0, 0, 1), l0 = 1 - l1 (a
+// copy when in == out); the 8 corners are combined z first, then y, then x, each pair as
+// fma(t0, l0, t1 * l1).  One output row (ox, oy) per work-group, z across the threads.
+struct UpAxis {
+  int n_in, n_out;
+  float ratio;
+};
+
+VXN_DEV void upsample_axis(int o, const UpAxis& ax, int& i0, int& i1, float& l0, float& l1) {
+  if (ax.n_in == ax.n_out) {
     i0 = i1 = o;
     l0 = 1.0f;
     l1 = 0.0f;
     return;
   }
-  const float ratio = n_out > 1 ? (float)(n_in - 1) / (float)(n_out - 1) : 0.0f;
-  const float src = ratio * (float)o;
-  i0 = min((int64_t)floorf(src), n_in - 1);
+  const float src = ax.ratio * (float)o;
+  i0 = min((int)floorf(src), ax.n_in - 1);
   l1 = fminf(fmaxf(src - (float)i0, 0.0f), 1.0f);
-  i1 = i0 + (i0 < n_in - 1 ? 1 : 0);
+  i1 = i0 + (i0 < ax.n_in - 1 ? 1 : 0);
   l0 = 1.0f - l1;
 }
 
-__global__ void __launch_bounds__(256) upsample_grid_kernel(const float* __restrict__ in, int X, int Y, int Z,
-                                                            float* __restrict__ out, int NX, int NY, int NZ) {
-  const int64_t n = (int64_t)NX * NY * NZ;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t ox = i / ((int64_t)NY * NZ), oy = (i / NZ) % NY, oz = i % NZ;
-    int64_t x[2], y[2], z[2];
-    float a[2], b[2], c[2];
-    upsample_axis(ox, X, NX, x[0], x[1], a[0], a[1]);
-    upsample_axis(oy, Y, NY, y[0], y[1], b[0], b[1]);
-    upsample_axis(oz, Z, NZ, z[0], z[1], c[0], c[1]);
-    float t[2];
-#pragma unroll
-    for (int dx = 0; dx < 2; ++dx) {
-      float u[2];
-#pragma unroll
-      for (int dy = 0; dy < 2; ++dy) {
-        const float* row = in + (x[dx] * Y + y[dy]) * (int64_t)Z;
-        u[dy] = fmaf(row[z[0]], c[0], row[z[1]] * c[1]);
-      }
-      t[dx] = fmaf(u[0], b[0], u[1] * b[1]);
-    }
-    out[i] = fmaf(t[0], a[0], t[1] * a[1]);
+__global__ void __launch_bounds__(256) upsample_grid_kernel(const float* __restrict__ in, float* __restrict__ out,
+                                                            UpAxis ux, UpAxis uy, UpAxis uz) {
+  const int ox = blockIdx.x / uy.n_out, oy = blockIdx.x - ox * uy.n_out;
+  int x[2], y[2];
+  float a[2], b[2];
+  upsample_axis(ox, ux, x[0], x[1], a[0], a[1]);
+  upsample_axis(oy, uy, y[0], y[1], b[0], b[1]);
+  const int Y = uy.n_in, Z = uz.n_in;
+  const float* r00 = in + ((int64_t)x[0] * Y + y[0]) * Z;
+  const float* r01 = in + ((int64_t)x[0] * Y + y[1]) * Z;
+  const float* r10 = in + ((int64_t)x[1] * Y + y[0]) * Z;
+  const float* r11 = in + ((int64_t)x[1] * Y + y[1]) * Z;
+  float* orow = out + ((int64_t)ox * uy.n_out + oy) * uz.n_out;
+  for (int oz = threadIdx.x; oz < uz.n_out; oz += blockDim.x) {
+    int z0, z1;
+    float c0, c1;
+    upsample_axis(oz, uz, z0, z1, c0, c1);
+    const float u00 = fmaf(r00[z0], c0, r00[z1] * c1), u01 = fmaf(r01[z0], c0, r01[z1] * c1);
+    const float u10 = fmaf(r10[z0], c0, r10[z1] * c1), u11 = fmaf(r11[z0], c0, r11[z1] * c1);
+    const float t0 = fmaf(u00, b[0], u01 * b[1]), t1 = fmaf(u10, b[0], u11 * b[1]);
+    orow[oz] = fmaf(t0, a[0], t1 * a[1]);
   }
+}
+
+static UpAxis up_axis(int n_in, int n_out) {
+  UpAxis a;
+  a.n_in = n_in;
+  a.n_out = n_out;
+  a.ratio = n_out > 1 ? (float)(n_in - 1) / (float)(n_out - 1) : 0.0f;
+  return a;
 }
 
 // SDF + unit normal over a packed batch of points of many scenes (the serving query).
@@ -402,9 +414,9 @@ extern "C" int vxn_upsample_grid(const float* grid, int X, int Y, int Z, float* 
                                  void* stream) {
   VXN_REQUIRE(grid != nullptr && out != nullptr, VXN_ERR_NULL, "null grid");
   VXN_REQUIRE(X >= 1 && Y >= 1 && Z >= 1 && NX >= 1 && NY >= 1 && NZ >= 1, VXN_ERR_SHAPE, "sizes must be >= 1");
-  const int64_t n = (int64_t)NX * NY * NZ;
-  hipLaunchKernelGGL(upsample_grid_kernel, dim3(blocks_for(n, 256, 65536)), dim3(256), 0, (hipStream_t)stream, grid,
-                     X, Y, Z, out, NX, NY, NZ);
+  VXN_REQUIRE((int64_t)NX * NY < ((int64_t)1 << 31), VXN_ERR_SHAPE, "output too large");
+  hipLaunchKernelGGL(upsample_grid_kernel, dim3((unsigned)(NX * NY)), dim3(256), 0, (hipStream_t)stream, grid, out,
+                     up_axis(X, NX), up_axis(Y, NY), up_axis(Z, NZ));
   VXN_LAUNCH_CHECK();
   return 0;
 }

@@ -1215,9 +1215,19 @@ static int launch_render(const vxn_render_args* p, void* stream) {
   if (MODE != kFwd) {
     const int64_t ns = p->n_rays * (int64_t)p->n_samples;
     const int sb = (int)((ns + 255) / 256);
-    hipLaunchKernelGGL(hash_scatter_kernel, dim3(sb), dim3(256), 0, s, a, dedup_levels(a.h));
+    hipStream_t hs = s;
+    hipEvent_t const* se = (hipEvent_t const*)p->side_events;
+    if (p->side_stream) {
+      VXN_REQUIRE(se != nullptr && se[0] != nullptr && se[1] != nullptr, VXN_ERR_NULL,
+                  "side_stream needs side_events[2]");
+      hs = (hipStream_t)p->side_stream;
+      VXN_HIP(hipEventRecord(se[0], s));
+      VXN_HIP(hipStreamWaitEvent(hs, se[0], 0));
+    }
+    hipLaunchKernelGGL(hash_scatter_kernel, dim3(sb), dim3(256), 0, hs, a, dedup_levels(a.h));
     VXN_LAUNCH_CHECK();
-    if (ev) VXN_HIP(hipEventRecord(ev[3], s));
+    if (ev) VXN_HIP(hipEventRecord(ev[3], hs));
+    if (p->side_stream) VXN_HIP(hipEventRecord(se[1], hs));
     hipLaunchKernelGGL(grid_scatter_kernel, dim3(sb), dim3(256), 0, s, a);
     VXN_LAUNCH_CHECK();
     if (ev) VXN_HIP(hipEventRecord(ev[4], s));

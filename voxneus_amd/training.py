@@ -32,6 +32,24 @@ def near_far_from_sphere(rays_o, rays_d, radius=1.0, default_near=0.05, default_
     return near, far
 
 
+def get_rays(H: int, W: int, K, c2w: torch.Tensor, normalize_dir: bool = True, convention: str = "opencv"):
+    """rendering/rays.py:11-69: rays through the pixel centres (+0.5) of a pinhole camera,
+    (H, W, 3) origins and directions on c2w's device (ray setup, upstream of the path)."""
+    dev = c2w.device
+    K = torch.as_tensor(K, dtype=torch.float32, device=dev)
+    i, j = torch.meshgrid(torch.arange(W, dtype=torch.float32, device=dev) + 0.5,
+                          torch.arange(H, dtype=torch.float32, device=dev) + 0.5, indexing="xy")
+    fx, fy, cx, cy = K[0, 0], K[1, 1], K[0, 2], K[1, 2]
+    if convention.lower() == "opencv":
+        dirs = torch.stack([(i - cx) / fx, (j - cy) / fy, torch.ones_like(i)], dim=-1)
+    else:
+        dirs = torch.stack([(i - cx) / fx, -(j - cy) / fy, -torch.ones_like(i)], dim=-1)
+    rays_d = torch.matmul(dirs, c2w[:3, :3].T)
+    if normalize_dir:
+        rays_d = rays_d / torch.linalg.norm(rays_d, dim=-1, keepdim=True)
+    return c2w[:3, 3].expand(rays_d.shape), rays_d
+
+
 def amplify_s_gradient(variance_param: torch.nn.Parameter, k: float = 5.0) -> bool:
     """training/scheduler.py:14-33: grad <- k*grad when grad < 0; returns whether it fired."""
     if variance_param.grad is None:
@@ -65,6 +83,8 @@ class StepConfig:
     mlp_mode: int = N.VXN_MLP_FP32
     active_levels: Optional[float] = None
     gradient_mode: str = "interpolated"  # or "analytical" (Eq. 4), as VoxNeuS(gradient_mode=...)
+    # run the hash-table scatter on a second stream, beside the grid scatter and regularisers
+    concurrent_scatter: bool = True
 
 
 class FusedTrainStep:
@@ -109,6 +129,7 @@ class FusedTrainStep:
         self.eps3 = Fn.voxel_size_of(self.box, (X, Y, Z))
         self._render_ws = None
         self.kernel_events = None  # optional N.KernelEvents(6) recorded between the render kernels
+        self.side = N.SideStream(dev) if cfg.concurrent_scatter else None
         self.level_w = None
         if cfg.active_levels is not None:
             from .hash_grid import level_weights
@@ -134,6 +155,9 @@ class FusedTrainStep:
         a.workspace, a.workspace_bytes = self._render_ws.data_ptr(), self._render_ws.numel()
         if self.kernel_events is not None:
             a.kernel_events = ctypes.cast(self.kernel_events.array, ctypes.c_void_p)
+        if self.side is not None:
+            a.side_stream = self.side.handle
+            a.side_events = ctypes.cast(self.side.events, ctypes.c_void_p)
         return a
 
     def sample(self, rays_o, rays_d, t_rand=None, u_rand=None):
@@ -163,6 +187,8 @@ class FusedTrainStep:
         self.grad_v = s_grad_to_v(self.grad_s, self.variance, self.s_max).reshape(1).contiguous()
         if cfg.amplify:
             Fn.s_amplify(self.grad_v, cfg.s_amp_k, self.amplified)
+        if self.side is not None:
+            self.side.join(Fn._stream(mid))  # grad_table is complete past this point
         return {"loss": self.loss, "reg_losses": self.reg_losses, "vr_counts": self.vr_counts,
                 "amplified": self.amplified, "z_vals": z, "dists": dists, "mid_z_vals": mid}
 
