@@ -118,12 +118,7 @@ def run_gpu(sc, batches, form, steps, warmup):
         out = one(i, True)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    pairs = dict(KERNEL_EVENTS)
-    if st.side is not None:
-        # the hash scatter runs on the side stream (events 2 -> 3); the grid scatter follows
-        # the render kernel directly on the main stream (events 2 -> 4)
-        pairs["grid_scatter"] = (2, 4)
-    kern = {k: float(np.mean([e.elapsed_ms(i, j) for e in ev])) for k, (i, j) in pairs.items()}
+    kern = {k: float(np.mean([e.elapsed_ms(i, j) for e in ev])) for k, (i, j) in KERNEL_EVENTS.items()}
     loss = out["loss"].item()
     return {"ms_per_step": dt / steps * 1e3, "rays_per_s": RAYS * steps / dt, "kernel_ms": kern, "loss": loss}
 

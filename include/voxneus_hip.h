@@ -166,6 +166,8 @@ typedef struct vxn_sampler_args {
   int32_t* inds;       /* nullable [steps, R, per_step]: searchsorted indices */
   float bound_radius;  /* when near and far are both null: near/far from this bounding sphere
                           (near_far_from_sphere, rendering/rays.py:102-128), computed in-kernel */
+  uint32_t* vr_bitmask; /* nullable [ceil(X*Y*Z/32)]: bit set for all 8 corners of the cell of
+                           every final sample (at o + d * mid_z), as vxn_render_args.vr_bitmask */
 } vxn_sampler_args;
 
 int vxn_upsample(const vxn_sampler_args* a, void* stream);
@@ -232,13 +234,10 @@ typedef struct vxn_render_args {
      reductions (forward: only the first three) */
   void* const* kernel_events;
   int32_t gradient_mode; /* VXN_GRADIENT_*: the normal of every sample (VoxNeuS gradient_mode) */
-  /* nullable: a second hipStream_t for the backward's hash-table scatter, which then runs
-     concurrently with the grid scatter, the reductions and whatever the caller queues
-     next on `stream`.  side_events (hipEvent_t[2], required with it): [0] is recorded on
-     `stream` when the scatter may start, [1] on side_stream when it has finished; the
-     caller makes `stream` wait for [1] before it reads grad_table. */
-  void* side_stream;
-  void* const* side_events;
+  /* nullable hipEvent_t, recorded on `stream` after the hash-feature gather, just before
+     the render kernel: a caller can start independent work on another stream from it
+     (the fused training step starts its regularisers there) */
+  void* fork_event;
 } vxn_render_args;
 
 int vxn_render_workspace_bytes(const vxn_render_args* a, int64_t* bytes);

@@ -48,6 +48,7 @@ class SamplerArgs(ctypes.Structure):
         ("n_samples", c_i32), ("n_importance", c_i32), ("up_sample_steps", c_i32), ("perturb", c_i32),
         ("t_lin", c_vp), ("u_det", c_vp), ("t_rand", c_vp), ("u_rand", c_vp),
         ("z_vals", c_vp), ("dists", c_vp), ("mid_z", c_vp), ("inds", c_vp), ("bound_radius", c_f),
+        ("vr_bitmask", c_vp),
     ]
 
 
@@ -66,7 +67,7 @@ class RenderArgs(ctypes.Structure):
         ("target_rgb", c_vp), ("huber_delta", c_f),
         ("grad_grid", c_vp), ("grad_table", c_vp), ("grad_mlp", c_vp), ("grad_s", c_vp), ("loss", c_vp),
         ("vr_bitmask", c_vp), ("workspace", c_vp), ("workspace_bytes", c_i64),
-        ("kernel_events", c_vp), ("gradient_mode", c_i32), ("side_stream", c_vp), ("side_events", c_vp),
+        ("kernel_events", c_vp), ("gradient_mode", c_i32), ("fork_event", c_vp),
     ]
 
 
@@ -182,9 +183,10 @@ class HipError(RuntimeError):
 
 
 class SideStream:
-    """A second stream (default priority) on ``device`` and a pair of untimed HIP events for
-    ``vxn_render_args.side_stream`` / ``side_events``; ``join(stream)`` makes ``stream``
-    wait for the side work queued so far."""
+    """A second stream (default priority) on ``device`` and a pair of untimed HIP events:
+    ``fork(stream)`` starts side work after what ``stream`` has queued (``events[0]`` can
+    also be recorded by a render launch as its ``fork_event``), ``finish()`` marks its end
+    and ``join(stream)`` makes ``stream`` wait for that mark."""
 
     _hip = None
 
@@ -194,6 +196,7 @@ class SideStream:
             h.hipEventCreateWithFlags.argtypes = [ctypes.POINTER(c_vp), ctypes.c_uint]
             h.hipStreamWaitEvent.argtypes = [c_vp, c_vp, ctypes.c_uint]
             h.hipEventDestroy.argtypes = [c_vp]
+            h.hipEventRecord.argtypes = [c_vp, c_vp]
             SideStream._hip = h
         self.stream = torch.cuda.Stream(device=device)
         self.events = (c_vp * 2)()
@@ -206,6 +209,20 @@ class SideStream:
     @property
     def handle(self) -> int:
         return self.stream.cuda_stream
+
+    def fork(self, stream: Optional[int] = None):
+        """Make the side stream wait for the work queued on ``stream`` so far (or, with
+        None, for the last record of ``events[0]``, e.g. by a render launch)."""
+        h = SideStream._hip
+        if stream is not None and h.hipEventRecord(self.events[0], c_vp(stream)) != 0:
+            raise HipError("hipEventRecord failed")
+        if h.hipStreamWaitEvent(c_vp(self.handle), self.events[0], 0) != 0:
+            raise HipError("hipStreamWaitEvent failed")
+
+    def finish(self):
+        """Mark the end of the side work queued so far (``join`` waits for this mark)."""
+        if SideStream._hip.hipEventRecord(self.events[1], c_vp(self.handle)) != 0:
+            raise HipError("hipEventRecord failed")
 
     def join(self, stream: int):
         if SideStream._hip.hipStreamWaitEvent(c_vp(stream), self.events[1], 0) != 0:

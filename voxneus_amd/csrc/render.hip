@@ -466,25 +466,10 @@ __global__ void __launch_bounds__(kRT, 2) render_kernel(RenderDev a) {
       }
     }
     if (a.vr) {
-      // V_R marks: a sample in the same cell as the previous lane's marks nothing new;
-      // the two z-corners of each (x, y) column share a word unless z0 ends one.
+      // V_R marks: a sample in the same cell as the previous lane's marks nothing new.
       const int key = valid ? (int)lin3(G, cn.c0[0], cn.c0[1], cn.c0[2]) : -1;
       const int kp = __shfl_up(key, 1, 64);
-      if (valid && (lane == 0 || kp != key)) {
-#pragma unroll
-        for (int k = 0; k < 8; k += 2) {
-          int cx, cy, cz0, cz1;
-          corner_xyz(cn, k, cx, cy, cz0);
-          corner_xyz(cn, k + 1, cx, cy, cz1);
-          const int64_t l0 = lin3(G, cx, cy, cz0), l1 = lin3(G, cx, cy, cz1);
-          if ((l0 >> 5) == (l1 >> 5)) {
-            atomicOr(a.vr + (l0 >> 5), (1u << (l0 & 31)) | (1u << (l1 & 31)));
-          } else {
-            atomicOr(a.vr + (l0 >> 5), 1u << (l0 & 31));
-            atomicOr(a.vr + (l1 >> 5), 1u << (l1 & 31));
-          }
-        }
-      }
+      if (valid && (lane == 0 || kp != key)) vr_mark_cell(G, cn, a.vr);
     }
     if (MODE == kFwd) continue;
 
@@ -1206,6 +1191,7 @@ static int launch_render(const vxn_render_args* p, void* stream) {
     VXN_LAUNCH_CHECK();
   }
   if (ev) VXN_HIP(hipEventRecord(ev[1], s));
+  if (p->fork_event) VXN_HIP(hipEventRecord((hipEvent_t)p->fork_event, s));
   if (p->mlp_mode == VXN_MLP_BF16_MFMA)
     hipLaunchKernelGGL((render_kernel<MODE, kMlpBf16, 0>), dim3(nb), dim3(kRT), 0, s, a);
   else
@@ -1215,19 +1201,9 @@ static int launch_render(const vxn_render_args* p, void* stream) {
   if (MODE != kFwd) {
     const int64_t ns = p->n_rays * (int64_t)p->n_samples;
     const int sb = (int)((ns + 255) / 256);
-    hipStream_t hs = s;
-    hipEvent_t const* se = (hipEvent_t const*)p->side_events;
-    if (p->side_stream) {
-      VXN_REQUIRE(se != nullptr && se[0] != nullptr && se[1] != nullptr, VXN_ERR_NULL,
-                  "side_stream needs side_events[2]");
-      hs = (hipStream_t)p->side_stream;
-      VXN_HIP(hipEventRecord(se[0], s));
-      VXN_HIP(hipStreamWaitEvent(hs, se[0], 0));
-    }
-    hipLaunchKernelGGL(hash_scatter_kernel, dim3(sb), dim3(256), 0, hs, a, dedup_levels(a.h));
+    hipLaunchKernelGGL(hash_scatter_kernel, dim3(sb), dim3(256), 0, s, a, dedup_levels(a.h));
     VXN_LAUNCH_CHECK();
-    if (ev) VXN_HIP(hipEventRecord(ev[3], hs));
-    if (p->side_stream) VXN_HIP(hipEventRecord(se[1], hs));
+    if (ev) VXN_HIP(hipEventRecord(ev[3], s));
     hipLaunchKernelGGL(grid_scatter_kernel, dim3(sb), dim3(256), 0, s, a);
     VXN_LAUNCH_CHECK();
     if (ev) VXN_HIP(hipEventRecord(ev[4], s));

@@ -26,6 +26,7 @@ struct SamplerDev {
   const float *t_lin, *u_det, *t_rand, *u_rand;
   float *z_out, *dists_out, *mid_out;
   int32_t* inds;
+  uint32_t* vr;  // nullable V_R bitmask: the corners of every final sample's cell
   float bound_r;
   // serving batches: per-ray scene of a packed batch; R is then the capacity and the
   // batch size is read from n_rays_dev
@@ -264,12 +265,27 @@ __global__ void __launch_bounds__(kSampThreads) upsample_kernel(SamplerDev a) {
       n += a.per;
     }
     // 5. final intervals and midpoints (sampler.py:197-202)
+    float midv = 0.0f;
     if (t < n) {
       const float zz = zs[t];
       const float dist = t < n - 1 ? zs[t + 1] - zz : zs[n - 1] - zs[n - 2];
+      midv = zz + dist * 0.5f;
       a.z_out[ray * n + t] = zz;
       a.dists_out[ray * n + t] = dist;
-      a.mid_out[ray * n + t] = zz + dist * 0.5f;
+      a.mid_out[ray * n + t] = midv;
+    }
+    if (!MULTI && a.vr) {
+      // V_R of the render's samples (their positions are o + d * mid, as the render forms
+      // them), marked here so that the regularisers need not wait for the render: a
+      // sample in the same cell as the previous lane's marks nothing new
+      Corners cn{};
+      int key = -1;
+      if (t < n) {
+        cn = corner_decomp(G, ox + dx * midv, oy + dy * midv, oz + dz * midv);
+        key = (int)lin3(G, cn.c0[0], cn.c0[1], cn.c0[2]);
+      }
+      const int kp = __shfl_up(key, 1, 64);
+      if (t < n && ((t & 63) == 0 || kp != key)) vr_mark_cell(G, cn, a.vr);
     }
     __syncthreads();
   }
@@ -300,6 +316,7 @@ int launch_serve_upsample(const SceneDev* scenes, int n_scenes, const int32_t* r
   a.ray_scene = ray_scene;
   a.n_rays_dev = n_dev;
   a.n_scenes = n_scenes;
+  a.vr = nullptr;
   const int nb = cap < 8192 ? (int)cap : 8192;
   hipLaunchKernelGGL(upsample_kernel<1>, dim3(nb), dim3(kSampThreads), 0, stream, a);
   VXN_LAUNCH_CHECK();
@@ -351,6 +368,7 @@ extern "C" int vxn_upsample(const vxn_sampler_args* p, void* stream) {
   a.dists_out = A.dists;
   a.mid_out = A.mid_z;
   a.inds = A.inds;
+  a.vr = A.vr_bitmask;
   a.bound_r = A.bound_radius;
   a.scenes = nullptr;
   a.ray_scene = nullptr;

@@ -295,6 +295,25 @@ VXN_DEV void grid_gradient_scatter(const GridDesc& g, const Corners& c, const fl
   else eq7_scatter(g, c, dg, grad_grid);
 }
 
+// V_R marks of one sample (regularization/active_vertices.py:12-50): the 8 corners of its
+// cell, in a bitmask of the grid's vertices.  The two z-corners of each (x, y) column share
+// a word unless the first ends one.
+VXN_DEV void vr_mark_cell(const GridDesc& g, const Corners& cn, uint32_t* vr) {
+#pragma unroll
+  for (int k = 0; k < 8; k += 2) {
+    int cx, cy, cz0, cz1;
+    corner_xyz(cn, k, cx, cy, cz0);
+    corner_xyz(cn, k + 1, cx, cy, cz1);
+    const int64_t l0 = lin3(g, cx, cy, cz0), l1 = lin3(g, cx, cy, cz1);
+    if ((l0 >> 5) == (l1 >> 5)) {
+      atomicOr(vr + (l0 >> 5), (1u << (l0 & 31)) | (1u << (l1 & 31)));
+    } else {
+      atomicOr(vr + (l0 >> 5), 1u << (l0 & 31));
+      atomicOr(vr + (l1 >> 5), 1u << (l1 & 31));
+    }
+  }
+}
+
 VXN_DEV void gs_scatter(const GridDesc& g, const GsStencil& s, float dsdf, float* grad_grid) {
 #pragma unroll
   for (int k = 0; k < 8; ++k) {

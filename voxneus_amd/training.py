@@ -83,8 +83,11 @@ class StepConfig:
     mlp_mode: int = N.VXN_MLP_FP32
     active_levels: Optional[float] = None
     gradient_mode: str = "interpolated"  # or "analytical" (Eq. 4), as VoxNeuS(gradient_mode=...)
-    # run the hash-table scatter on a second stream, beside the grid scatter and regularisers
-    concurrent_scatter: bool = True
+    # mark V_R in the sampler and run the regularisers on a second stream, beside the
+    # hash-feature gather, the render and the scatters
+    concurrent_regularisers: bool = True
+    # start them once the hash-feature gather is done (else right after the sampler)
+    fork_after_features: bool = False
 
 
 class FusedTrainStep:
@@ -129,7 +132,7 @@ class FusedTrainStep:
         self.eps3 = Fn.voxel_size_of(self.box, (X, Y, Z))
         self._render_ws = None
         self.kernel_events = None  # optional N.KernelEvents(6) recorded between the render kernels
-        self.side = N.SideStream(dev) if cfg.concurrent_scatter else None
+        self.side = N.SideStream(dev) if cfg.concurrent_regularisers else None
         self.level_w = None
         if cfg.active_levels is not None:
             from .hash_grid import level_weights
@@ -155,40 +158,59 @@ class FusedTrainStep:
         a.workspace, a.workspace_bytes = self._render_ws.data_ptr(), self._render_ws.numel()
         if self.kernel_events is not None:
             a.kernel_events = ctypes.cast(self.kernel_events.array, ctypes.c_void_p)
-        if self.side is not None:
-            a.side_stream = self.side.handle
-            a.side_events = ctypes.cast(self.side.events, ctypes.c_void_p)
         return a
 
-    def sample(self, rays_o, rays_d, t_rand=None, u_rand=None):
+    def sample(self, rays_o, rays_d, t_rand=None, u_rand=None, vr_bitmask=None):
         cfg = self.cfg
         # ray bounds from the unit sphere (trainer.py:188) are computed inside the sampler
         return Fn.upsample(rays_o, rays_d, None, None, self.grid, self.box, cfg.n_samples, cfg.n_importance,
                            cfg.up_sample_steps, cfg.perturb, self.occ_mask, self.box, t_rand, u_rand,
-                           bound_radius=1.0)
+                           bound_radius=1.0, vr_bitmask=vr_bitmask)
+
+    def _regularisers(self):
+        # V_R (relaxed-sphere filtered) and both regularisers straight from the bitmask
+        cfg = self.cfg
+        Fn.reg_eik_curv_vr(self.grid, self.eps3, self.box, cfg.relax_radius, self.bitmask, cfg.w_eik, cfg.w_curv,
+                           self.grad_grid, self.reg_ws, self.reg_losses, self.vr_counts)
 
     def step(self, rays_o: torch.Tensor, rays_d: torch.Tensor, target: torch.Tensor, t_rand=None, u_rand=None,
              zs=None) -> Dict[str, torch.Tensor]:
         """One fwd+bwd step; gradients are overwritten.  Returns device tensors."""
         cfg = self.cfg
         rays_o, rays_d, target = rays_o.contiguous(), rays_d.contiguous(), target.contiguous()
-        if zs is None:
-            z, dists, mid = self.sample(rays_o, rays_d, t_rand, u_rand)
-        else:
-            z, dists, mid = zs
         self.grad_grid.zero_()
         self.grad_table.zero_()
         self.bitmask.zero_()
+        # with the sampler in this step, it marks V_R and the regularisers run on the side
+        # stream beside the render kernel and the scatters; both only add into grad_grid
+        early = self.side is not None and zs is None
+        if zs is None:
+            z, dists, mid = self.sample(rays_o, rays_d, t_rand, u_rand, self.bitmask if early else None)
+        else:
+            z, dists, mid = zs
+        main = Fn._stream(mid)
         a = self._render_args(rays_o, rays_d, mid.contiguous(), dists.contiguous(), target)
-        N.check(N.lib().vxn_render_train(ctypes.byref(a), Fn._stream(mid)), "render_train")
-        # V_R (relaxed-sphere filtered) and both regularisers straight from the bitmask
-        Fn.reg_eik_curv_vr(self.grid, self.eps3, self.box, cfg.relax_radius, self.bitmask, cfg.w_eik, cfg.w_curv,
-                           self.grad_grid, self.reg_ws, self.reg_losses, self.vr_counts)
+        if early:
+            a.vr_bitmask = None
+            if cfg.fork_after_features:
+                a.fork_event = self.side.events[0]  # recorded just before the render kernel
+            else:
+                self.side.fork(main)  # right after the sampler
+        N.check(N.lib().vxn_render_train(ctypes.byref(a), main), "render_train")
+        if early:
+            # queued after the render launch, so the render's kernels are dispatched first
+            if cfg.fork_after_features:
+                self.side.fork(None)
+            with torch.cuda.stream(self.side.stream):
+                self._regularisers()
+            self.side.finish()
+        else:
+            self._regularisers()
         self.grad_v = s_grad_to_v(self.grad_s, self.variance, self.s_max).reshape(1).contiguous()
         if cfg.amplify:
             Fn.s_amplify(self.grad_v, cfg.s_amp_k, self.amplified)
-        if self.side is not None:
-            self.side.join(Fn._stream(mid))  # grad_table is complete past this point
+        if early:
+            self.side.join(main)  # grad_grid, the losses and V_R counts are complete past this point
         return {"loss": self.loss, "reg_losses": self.reg_losses, "vr_counts": self.vr_counts,
                 "amplified": self.amplified, "z_vals": z, "dists": dists, "mid_z_vals": mid}
 
