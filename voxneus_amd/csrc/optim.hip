@@ -61,46 +61,73 @@ struct AdamConsts {
 //   p -= step_size (m / (sqrt(v) / sqrt(1 - b2^t) + eps))  with step_size = lr / (1 - b1^t).
 // MASKED: only elements with g != 0 change (and are counted).
 template <int FORM, bool MASKED>
+VXN_DEV void grid_adam_update(float pv[4], float mv[4], float vv[4], const float gv[4], int64_t i, int64_t n,
+                              const AdamConsts& c, int& cnt) {
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    if (MASKED && gv[e] == 0.0f) continue;
+    if (i + e >= n) continue;
+    ++cnt;
+    if constexpr (FORM == 0) {
+      const float gr = gv[e];
+      mv[e] = c.b1 * mv[e] + c.omb1 * gr;
+      vv[e] = c.b2 * vv[e] + c.omb2 * gr * gr;
+      pv[e] = pv[e] - c.step_size * mv[e] / (sqrtf(vv[e]) + c.eps);
+    } else {
+      const float gr = c.wd != 0.0f ? gv[e] + c.wd * pv[e] : gv[e];
+      mv[e] = mv[e] * c.b1 + c.omb1 * gr;
+      vv[e] = vv[e] * c.b2 + c.omb2 * (gr * gr);
+      const float denom = sqrtf(vv[e]) / c.bc2_sqrt + c.eps;
+      pv[e] = pv[e] - c.step_size * (mv[e] / denom);
+    }
+  }
+}
+
+// Each thread takes kGroups float4 groups a work-group stride apart: their gradient loads
+// are all issued before any moment is read, so the dependent loads of the masked form
+// overlap across groups instead of serialising.
+constexpr int kGroups = 4;
+
+template <int FORM, bool MASKED>
 __global__ void __launch_bounds__(kAdamThreads) grid_adam_kernel(float* __restrict__ p, const float* __restrict__ g,
                                                                   float* __restrict__ m, float* __restrict__ v,
                                                                   int64_t n, AdamConsts c,
                                                                   int32_t* __restrict__ n_updated) {
   int cnt = 0;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x * 4;
-  for (int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4; i < n; i += stride) {
-    float gv[4];
-    load4(g, i, n, gv);
-    bool any = !MASKED;
-    if constexpr (MASKED) {
+  const int64_t tile = (int64_t)kAdamThreads * 4 * kGroups;
+  for (int64_t base = (int64_t)blockIdx.x * tile; base < n; base += (int64_t)gridDim.x * tile) {
+    float gv[kGroups][4];
+    bool act[kGroups];
 #pragma unroll
-      for (int e = 0; e < 4; ++e) any = any || gv[e] != 0.0f;
-    }
-    if (!any) continue;
-    float pv[4], mv[4], vv[4];
-    load4(p, i, n, pv);
-    load4(m, i, n, mv);
-    load4(v, i, n, vv);
+    for (int q = 0; q < kGroups; ++q) {
+      const int64_t i = base + ((int64_t)q * kAdamThreads + threadIdx.x) * 4;
+      act[q] = i < n;
+      if (act[q]) load4(g, i, n, gv[q]);
+      if constexpr (MASKED) {
+        bool any = false;
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      if (MASKED && gv[e] == 0.0f) continue;
-      if (i + e >= n) continue;
-      ++cnt;
-      if constexpr (FORM == 0) {
-        const float gr = gv[e];
-        mv[e] = c.b1 * mv[e] + c.omb1 * gr;
-        vv[e] = c.b2 * vv[e] + c.omb2 * gr * gr;
-        pv[e] = pv[e] - c.step_size * mv[e] / (sqrtf(vv[e]) + c.eps);
-      } else {
-        const float gr = c.wd != 0.0f ? gv[e] + c.wd * pv[e] : gv[e];
-        mv[e] = mv[e] * c.b1 + c.omb1 * gr;
-        vv[e] = vv[e] * c.b2 + c.omb2 * (gr * gr);
-        const float denom = sqrtf(vv[e]) / c.bc2_sqrt + c.eps;
-        pv[e] = pv[e] - c.step_size * (mv[e] / denom);
+        for (int e = 0; e < 4; ++e) any = any || (act[q] && gv[q][e] != 0.0f);
+        act[q] = any;
       }
     }
-    store4(p, i, n, pv);
-    store4(m, i, n, mv);
-    store4(v, i, n, vv);
+    float pv[kGroups][4], mv[kGroups][4], vv[kGroups][4];
+#pragma unroll
+    for (int q = 0; q < kGroups; ++q) {
+      if (!act[q]) continue;
+      const int64_t i = base + ((int64_t)q * kAdamThreads + threadIdx.x) * 4;
+      load4(p, i, n, pv[q]);
+      load4(m, i, n, mv[q]);
+      load4(v, i, n, vv[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < kGroups; ++q) {
+      if (!act[q]) continue;
+      const int64_t i = base + ((int64_t)q * kAdamThreads + threadIdx.x) * 4;
+      grid_adam_update<FORM, MASKED>(pv[q], mv[q], vv[q], gv[q], i, n, c, cnt);
+      store4(p, i, n, pv[q]);
+      store4(m, i, n, mv[q]);
+      store4(v, i, n, vv[q]);
+    }
   }
   if (n_updated) count_add(n_updated, cnt);
 }
@@ -170,7 +197,7 @@ static float native_step_size(int step, float b1, float b2, float lr) {
 
 static int grid_adam_launch(int form, bool masked, float* param, const float* grad, float* exp_avg,
                             float* exp_avg_sq, int64_t n, const AdamConsts& c, int32_t* n_updated, hipStream_t s) {
-  const int nb = blocks_for((n + 3) / 4, kAdamThreads, 8192);
+  const int nb = blocks_for((n + 4 * kGroups - 1) / (4 * kGroups), kAdamThreads, 4096);
   if (form == 0 && masked)
     hipLaunchKernelGGL((grid_adam_kernel<0, true>), dim3(nb), dim3(kAdamThreads), 0, s, param, grad, exp_avg,
                        exp_avg_sq, n, c, n_updated);
