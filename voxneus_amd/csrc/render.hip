@@ -176,8 +176,9 @@ template <int MODE, int MLPM, int MULTI>
 __global__ void __launch_bounds__(kRT, 2) render_kernel(RenderDev a) {
   constexpr bool kBF = MLPM == kMlpBf16;
   constexpr bool kBFB = kBF && MODE != kFwd;
-  // BF16 backward / training: the hash features come from hash_features_kernel (a.hfeat)
-  constexpr bool kPre = kBF && MODE != kFwd && !MULTI;
+  // BF16 backward / training and BF16 serving: the hash features come from
+  // hash_features_kernel (a.hfeat)
+  constexpr bool kPre = kBF && (MODE != kFwd || MULTI);
   __shared__ Red red;
   __shared__ __align__(16) float preh[64];  // b0 + W0[:, 32:48] . SH(d): the per-ray part of the hidden layer
   __shared__ int nlive_s[2];  // samples per wavefront with a non-zero output gradient
@@ -803,20 +804,33 @@ VXN_DEV void sample_pos(const RenderDev& a, int64_t i, float& px, float& py, flo
 // one level's table at a time (2^19 entries: 2 MB bf16) instead of every XCD streaming
 // all 16 tables through its L2.  The values are those of hash_level_fwd at the render
 // kernel's sample position (same operations), rounded to bf16 as the BF16 MLP's inputs.
+template <int MULTI>
 __global__ void __launch_bounds__(256) hash_features_kernel(RenderDev a, int64_t chunks) {
   const int64_t NS = a.R * (int64_t)a.S;
   const int64_t r = blockIdx.x >> 3;
   const int l = (int)(blockIdx.x & 7) + 8 * (int)(r / chunks);
-  if (l >= a.h.L) return;
+  if (l >= 16) return;
   const int64_t i = (r % chunks) * (int64_t)blockDim.x + threadIdx.x;
   if (i >= NS) return;
+  const GridDesc* g = &a.g;
+  const HashDesc* h = &a.h;
+  if constexpr (MULTI) {  // serving batch: the ray's scene, rays past the batch size skipped
+    const int64_t ray = i / a.S;
+    if (ray >= min((int64_t)max(*a.n_rays_dev, 0), a.R)) return;
+    const int si = a.ray_scene[ray];
+    if (si < 0 || si >= a.n_scenes) return;
+    g = &a.scenes[si].g;
+    h = &a.scenes[si].h;
+  } else if (l >= a.h.L) {
+    return;
+  }
   float px, py, pz;
   sample_pos(a, i, px, py, pz);
-  const float x0 = fminf(fmaxf((px - a.g.mn[0]) / a.g.den[0], 0.0f), 1.0f);
-  const float x1 = fminf(fmaxf((py - a.g.mn[1]) / a.g.den[1], 0.0f), 1.0f);
-  const float x2 = fminf(fmaxf((pz - a.g.mn[2]) / a.g.den[2], 0.0f), 1.0f);
+  const float x0 = fminf(fmaxf((px - g->mn[0]) / g->den[0], 0.0f), 1.0f);
+  const float x1 = fminf(fmaxf((py - g->mn[1]) / g->den[1], 0.0f), 1.0f);
+  const float x2 = fminf(fmaxf((pz - g->mn[2]) / g->den[2], 0.0f), 1.0f);
   float f0, f1;
-  hash_level_fwd(a.h, l, x0, x1, x2, f0, f1);
+  hash_level_fwd(*h, l, x0, x1, x2, f0, f1);
   ((uint32_t*)a.hfeat)[l * NS + i] = pk_bf16(f0, f1);
 }
 
@@ -1187,7 +1201,7 @@ static int launch_render(const vxn_render_args* p, void* stream) {
     const int64_t ns = p->n_rays * (int64_t)p->n_samples;
     const int64_t chunks = (ns + 255) / 256;
     const unsigned nb_hf = (unsigned)(8 * ((a.h.L + 7) / 8) * chunks);
-    hipLaunchKernelGGL(hash_features_kernel, dim3(nb_hf), dim3(256), 0, s, a, chunks);
+    hipLaunchKernelGGL(hash_features_kernel<0>, dim3(nb_hf), dim3(256), 0, s, a, chunks);
     VXN_LAUNCH_CHECK();
   }
   if (ev) VXN_HIP(hipEventRecord(ev[1], s));
@@ -1285,7 +1299,8 @@ extern "C" int vxn_serve_render_workspace_bytes(const vxn_serve_render_args* a, 
   VXN_REQUIRE(a != nullptr && bytes != nullptr, VXN_ERR_NULL, "null argument");
   int steps, per, S;
   if (int e = serve_shape(*a, steps, per, S)) return e;
-  *bytes = 3 * align256(a->capacity * (int64_t)S * (int64_t)sizeof(float));
+  const int64_t plane = align256(a->capacity * (int64_t)S * (int64_t)sizeof(float));
+  *bytes = (a->mlp_mode == VXN_MLP_BF16_MFMA ? 3 + 16 : 3) * plane;  // z, dists, mid (+ 16 feature planes)
   return 0;
 }
 
@@ -1303,8 +1318,10 @@ extern "C" int vxn_serve_render(const vxn_serve_render_args* p, void* stream) {
   VXN_REQUIRE(A.mlp_mode == VXN_MLP_FP32 || A.mlp_mode == VXN_MLP_BF16_MFMA, VXN_ERR_VALUE, "unknown mlp_mode %d",
               A.mlp_mode);
   const int64_t plane = align256(A.capacity * (int64_t)S * (int64_t)sizeof(float));
-  VXN_REQUIRE(A.workspace != nullptr && A.workspace_bytes >= 3 * plane, VXN_ERR_WORKSPACE,
-              "serve workspace needs %lld bytes", (long long)(3 * plane));
+  const bool bf = A.mlp_mode == VXN_MLP_BF16_MFMA;
+  const int64_t need = (bf ? 3 + 16 : 3) * plane;
+  VXN_REQUIRE(A.workspace != nullptr && A.workspace_bytes >= need, VXN_ERR_WORKSPACE,
+              "serve workspace needs %lld bytes", (long long)need);
   float* z = (float*)A.workspace;
   float* dists = (float*)((char*)A.workspace + plane);
   float* mid = (float*)((char*)A.workspace + 2 * plane);
@@ -1331,7 +1348,15 @@ extern "C" int vxn_serve_render(const vxn_serve_render_args* p, void* stream) {
   a.n_rays_dev = A.n_rays;
   a.n_scenes = A.n_scenes;
   const int nb = render_blocks(A.capacity);
-  if (A.mlp_mode == VXN_MLP_BF16_MFMA)
+  if (bf) {
+    // the BF16 form gathers the hash features level-major, XCD-aware, ahead of the render
+    a.hfeat = (char*)A.workspace + 3 * plane;
+    const int64_t chunks = (A.capacity * (int64_t)S + 255) / 256;
+    VXN_REQUIRE(16 * chunks < ((int64_t)1 << 31), VXN_ERR_SHAPE, "batch too large");
+    hipLaunchKernelGGL(hash_features_kernel<1>, dim3((unsigned)(16 * chunks)), dim3(256), 0, s, a, chunks);
+    VXN_LAUNCH_CHECK();
+  }
+  if (bf)
     hipLaunchKernelGGL((render_kernel<kFwd, kMlpBf16, 1>), dim3(nb), dim3(kRT), 0, s, a);
   else
     hipLaunchKernelGGL((render_kernel<kFwd, kMlpF32, 1>), dim3(nb), dim3(kRT), 0, s, a);
