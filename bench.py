@@ -116,11 +116,13 @@ def run_gpu(sc, batches, form, steps, warmup):
     t0 = time.perf_counter()
     for i in range(steps):
         out = one(i, True)
+    t_host = time.perf_counter() - t0  # host time to queue the steps
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     kern = {k: float(np.mean([e.elapsed_ms(i, j) for e in ev])) for k, (i, j) in KERNEL_EVENTS.items()}
     loss = out["loss"].item()
-    return {"ms_per_step": dt / steps * 1e3, "rays_per_s": RAYS * steps / dt, "kernel_ms": kern, "loss": loss}
+    return {"ms_per_step": dt / steps * 1e3, "rays_per_s": RAYS * steps / dt, "kernel_ms": kern, "loss": loss,
+            "host_ms_per_step": t_host / steps * 1e3}
 
 
 def run_full_render(form):
@@ -453,7 +455,8 @@ def main():
         err = None if a.no_error_check else rgb_error_vs_fp64(sc, batches[0], form)
         log(f"rgb max-abs err vs fp64 ({form}): {err}")
         res[form] = {"rays_per_s": r["rays_per_s"], "samples_per_s": r["rays_per_s"] * SAMPLES,
-                     "ms_per_step": r["ms_per_step"], "rgb_max_abs_err_vs_fp64": err, "kernels": kernels}
+                     "ms_per_step": r["ms_per_step"], "host_ms_per_step": r["host_ms_per_step"],
+                     "rgb_max_abs_err_vs_fp64": err, "kernels": kernels}
     main_form = forms[0]
     r = res[main_form]
     kms = {k: v["ms"] for k, v in r["kernels"].items()}

@@ -238,6 +238,10 @@ typedef struct vxn_render_args {
      the render kernel: a caller can start independent work on another stream from it
      (the fused training step starts its regularisers there) */
   void* fork_event;
+  /* nullable device float: the NeuS sharpness s read on the device in place of `s`, so a
+     caller that keeps s on the device (s = clamp(exp(10 v)), models/variance.py:31) needs
+     no host read */
+  const float* s_dev;
 } vxn_render_args;
 
 int vxn_render_workspace_bytes(const vxn_render_args* a, int64_t* bytes);

@@ -67,7 +67,7 @@ class RenderArgs(ctypes.Structure):
         ("target_rgb", c_vp), ("huber_delta", c_f),
         ("grad_grid", c_vp), ("grad_table", c_vp), ("grad_mlp", c_vp), ("grad_s", c_vp), ("loss", c_vp),
         ("vr_bitmask", c_vp), ("workspace", c_vp), ("workspace_bytes", c_i64),
-        ("kernel_events", c_vp), ("gradient_mode", c_i32), ("fork_event", c_vp),
+        ("kernel_events", c_vp), ("gradient_mode", c_i32), ("fork_event", c_vp), ("s_dev", c_vp),
     ]
 
 

@@ -14,7 +14,7 @@ __global__ void __launch_bounds__(256) hash_fwd_kernel(HashDesc h, const float* 
     const float z = fminf(fmaxf(x01[3 * i + 2], 0.0f), 1.0f);
     for (int l = 0; l < L; ++l) {
       float f0, f1;
-      hash_level_fwd(h, l, x, y, z, f0, f1);
+      hash_level_fwd<true>(h, l, x, y, z, f0, f1);
       feats[i * 2 * L + 2 * l] = f0;
       feats[i * 2 * L + 2 * l + 1] = f1;
     }

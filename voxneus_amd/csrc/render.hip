@@ -58,6 +58,7 @@ struct RenderDev {
   HashDesc h;
   const float* mlp;
   float s, sphere_r, cos_anneal;
+  const float* s_dev;  // nullable: s read from device memory
   int iter_cos;
   float *rgb, *depth, *normal, *opacity, *weights, *alpha, *trans, *sdf_s, *grad_s, *normal_s;
   int64_t* corners;
@@ -256,7 +257,7 @@ __global__ void __launch_bounds__(kRT, 2) render_kernel(RenderDev a) {
     }
     const GridDesc& G = MULTI ? scn->g : a.g;
     const HashDesc& Hd = MULTI ? scn->h : a.h;
-    const float sv = MULTI ? scn->s : a.s;
+    const float sv = MULTI ? scn->s : (a.s_dev ? *a.s_dev : a.s);
     const float sr = MULTI ? scn->sphere_r : a.sphere_r;
     const float* W0 = MULTI ? scn->mlp : a.mlp;
     const float* b0 = W0 + kOffB0;
@@ -296,7 +297,7 @@ __global__ void __launch_bounds__(kRT, 2) render_kernel(RenderDev a) {
     float x[35];  // 32 hash features, then the unit normal (MLP input columns 0-31, 48-50)
     if constexpr (!kPre) {
 #pragma unroll
-      for (int l = 0; l < 16; ++l) hash_level_fwd(Hd, l, x0, x1, x2, x[2 * l], x[2 * l + 1]);
+      for (int l = 0; l < 16; ++l) hash_level_fwd<kBF>(Hd, l, x0, x1, x2, x[2 * l], x[2 * l + 1]);
     }
     x[32] = nrm[0];
     x[33] = nrm[1];
@@ -830,7 +831,7 @@ __global__ void __launch_bounds__(256) hash_features_kernel(RenderDev a, int64_t
   const float x1 = fminf(fmaxf((py - g->mn[1]) / g->den[1], 0.0f), 1.0f);
   const float x2 = fminf(fmaxf((pz - g->mn[2]) / g->den[2], 0.0f), 1.0f);
   float f0, f1;
-  hash_level_fwd(*h, l, x0, x1, x2, f0, f1);
+  hash_level_fwd<true>(*h, l, x0, x1, x2, f0, f1);
   ((uint32_t*)a.hfeat)[l * NS + i] = pk_bf16(f0, f1);
 }
 
@@ -1127,6 +1128,7 @@ static int build_render_dev(const vxn_render_args& A, RenderDev& a, int mode) {
   a.h = h;
   a.mlp = A.mlp;
   a.s = A.s;
+  a.s_dev = A.s_dev;
   a.sphere_r = A.sphere_radius;
   a.cos_anneal = A.cos_anneal_ratio;
   a.iter_cos = A.use_iter_cos;

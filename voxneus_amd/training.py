@@ -131,6 +131,9 @@ class FusedTrainStep:
         self.reg_losses = torch.zeros(2, device=dev, dtype=torch.float64)
         self.eps3 = Fn.voxel_size_of(self.box, (X, Y, Z))
         self._render_ws = None
+        self._args = None
+        self._args_key = None
+        self.s_value = torch.zeros((), device=dev, dtype=torch.float32)  # s of the last step, on the device
         self.kernel_events = None  # optional N.KernelEvents(6) recorded between the render kernels
         self.side = N.SideStream(dev) if cfg.concurrent_regularisers else None
         self.level_w = None
@@ -139,25 +142,36 @@ class FusedTrainStep:
             self.level_w = level_weights(len(self.level_res), cfg.active_levels)
 
     def _render_args(self, rays_o, rays_d, mid, dists, target):
+        """The render arguments of this step.  The static part is built once; s stays on the
+        device (``s_dev``), so building the arguments reads nothing back from the GPU."""
         cfg = self.cfg
-        rc = Fn.RenderConfig(self.box, self.level_res, self.level_w, cfg.sphere_radius, mlp_mode=cfg.mlp_mode,
-                             gradient_mode=N.gradient_mode_code(cfg.gradient_mode))
-        a = N.RenderArgs()
-        s = torch.clamp(torch.exp(self.variance.detach() * 10.0), min=1e-4, max=self.s_max)
-        self.s_value = s
-        Fn.fill_render_args(a, rc, self.grid, self.table, self.mlp, float(s), rays_o, rays_d, mid, dists)
+        # s = clamp(exp(10 v), 1e-4, s_max) (variance.py:31), computed on the device
+        torch.clamp(torch.exp(self.variance.detach() * 10.0), min=1e-4, max=self.s_max, out=self.s_value)
+        R, S = mid.shape
+        key = (R, S, mid.device, self.grid.data_ptr(), self.table.data_ptr(), self.mlp.data_ptr())
+        a = self._args
+        if a is None or self._args_key != key:
+            rc = Fn.RenderConfig(self.box, self.level_res, self.level_w, cfg.sphere_radius, mlp_mode=cfg.mlp_mode,
+                                 gradient_mode=N.gradient_mode_code(cfg.gradient_mode))
+            a = N.RenderArgs()
+            Fn.fill_render_args(a, rc, self.grid, self.table, self.mlp, 0.0, rays_o, rays_d, mid, dists)
+            a.s_dev = N.dptr(self.s_value)
+            a.huber_delta = cfg.huber_delta
+            a.grad_grid, a.grad_table = N.dptr(self.grad_grid), N.dptr(self.grad_table)
+            a.grad_mlp, a.grad_s, a.loss = N.dptr(self.grad_mlp), N.dptr(self.grad_s), N.dptr(self.loss)
+            need = ctypes.c_int64(0)
+            N.check(N.lib().vxn_render_workspace_bytes(ctypes.byref(a), ctypes.byref(need)))
+            if self._render_ws is None or self._render_ws.numel() < need.value:
+                self._render_ws = torch.empty(int(need.value), device=mid.device, dtype=torch.uint8)
+            a.workspace, a.workspace_bytes = self._render_ws.data_ptr(), self._render_ws.numel()
+            self._args, self._args_key = a, key
+        a.rays_o, a.rays_d = N.dptr(rays_o, torch.float32, "rays_o"), N.dptr(rays_d, torch.float32, "rays_d")
+        a.mid_z, a.dists = N.dptr(mid, torch.float32, "mid_z"), N.dptr(dists, torch.float32, "dists")
         a.target_rgb = N.dptr(target, torch.float32, "target")
-        a.huber_delta = cfg.huber_delta
-        a.grad_grid, a.grad_table = N.dptr(self.grad_grid), N.dptr(self.grad_table)
-        a.grad_mlp, a.grad_s, a.loss = N.dptr(self.grad_mlp), N.dptr(self.grad_s), N.dptr(self.loss)
         a.vr_bitmask = N.dptr(self.bitmask)
-        need = ctypes.c_int64(0)
-        N.check(N.lib().vxn_render_workspace_bytes(ctypes.byref(a), ctypes.byref(need)))
-        if self._render_ws is None or self._render_ws.numel() < need.value:
-            self._render_ws = torch.empty(int(need.value), device=mid.device, dtype=torch.uint8)
-        a.workspace, a.workspace_bytes = self._render_ws.data_ptr(), self._render_ws.numel()
-        if self.kernel_events is not None:
-            a.kernel_events = ctypes.cast(self.kernel_events.array, ctypes.c_void_p)
+        a.fork_event = None
+        a.kernel_events = (ctypes.cast(self.kernel_events.array, ctypes.c_void_p)
+                           if self.kernel_events is not None else None)
         return a
 
     def sample(self, rays_o, rays_d, t_rand=None, u_rand=None, vr_bitmask=None):
