@@ -271,7 +271,8 @@ class KernelEvents:
         return float(ms.value)
 
     def __del__(self):
-        h = KernelEvents._hip
+        # at interpreter shutdown the class attribute may already be gone
+        h = getattr(KernelEvents, "_hip", None) if KernelEvents is not None else None
         if h is not None:
             for ev in self.array:
                 if ev:
