@@ -1,5 +1,6 @@
 // 4-step progressive importance upsampler (NeuS up_sample + sample_pdf + cat_z_vals),
-// one ray per 128-thread work-group, all per-ray state in LDS.
+// one ray per one-wavefront work-group (each lane holds two samples), all per-ray state in
+// LDS.
 //
 // Rounding follows the reference's PyTorch CPU evaluation so that the searchsorted
 // indices, and therefore the z values, come out bit-identical:
@@ -13,7 +14,7 @@
 
 namespace vxn {
 
-constexpr int kSampThreads = 128;
+constexpr int kSampThreads = 64;
 constexpr int kMaxS = 128;
 constexpr int kMaxPer = 64;
 
@@ -38,6 +39,7 @@ struct SamplerDev {
 
 template <int MULTI>
 __global__ void __launch_bounds__(kSampThreads) upsample_kernel(SamplerDev a) {
+  // One wavefront per ray: lane t holds samples t and t + 64 (kMaxS = 128).
   __shared__ float zs[kMaxS], sd[kMaxS], wv[kMaxS], cdf[kMaxS + 1], cosv[kMaxS];
   __shared__ float nz[kMaxPer], nzs[kMaxPer];
   __shared__ unsigned char ins[kMaxS];
@@ -69,59 +71,80 @@ __global__ void __launch_bounds__(kSampThreads) upsample_kernel(SamplerDev a) {
     }
     int n = a.n0;
     // 1. stratified samples (sampler.py:85-125)
-    float z = 0.0f;
-    if (t < n) {
-      const float tl = a.t_lin[t];
-      z = nr * (1.0f - tl) + fr * tl;
-      zs[t] = z;
+    float z[2] = {0.0f, 0.0f};
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int i = t + 64 * h;
+      if (i < n) {
+        const float tl = a.t_lin[i];
+        z[h] = nr * (1.0f - tl) + fr * tl;
+        zs[i] = z[h];
+      }
     }
     __syncthreads();
     if (a.perturb) {
-      if (t < n) {
-        const float lower = t == 0 ? zs[0] : 0.5f * (zs[t] + zs[t - 1]);
-        const float upper = t == n - 1 ? zs[n - 1] : 0.5f * (zs[t + 1] + zs[t]);
-        z = lower + (upper - lower) * a.t_rand[ray * n + t];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int i = t + 64 * h;
+        if (i < n) {
+          const float lower = i == 0 ? zs[0] : 0.5f * (zs[i] + zs[i - 1]);
+          const float upper = i == n - 1 ? zs[n - 1] : 0.5f * (zs[i + 1] + zs[i]);
+          z[h] = lower + (upper - lower) * a.t_rand[ray * n + i];
+        }
       }
       __syncthreads();
-      if (t < n) zs[t] = z;
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+        if (t + 64 * h < n) zs[t + 64 * h] = z[h];
       __syncthreads();
     }
     // 2. SDF at the stratified points
-    if (t < n) {
-      const float px = ox + dx * z, py = oy + dy * z, pz = oz + dz * z;
-      sd[t] = gs_eval(G, gs_stencil(G, px, py, pz));
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int i = t + 64 * h;
+      if (i < n) {
+        const float px = ox + dx * z[h], py = oy + dy * z[h], pz = oz + dz * z[h];
+        sd[i] = gs_eval(G, gs_stencil(G, px, py, pz));
+      }
     }
     __syncthreads();
     for (int step = 0; step < a.steps; ++step) {
       const float inv_s = 64.0f * (float)(1 << step);
       const int m = n - 1;
       const bool last = step + 1 == a.steps;
-      if (t < n) {
-        const float zz = zs[t];
-        const float px = ox + dx * zz, py = oy + dy * zz, pz = oz + dz * zz;
-        ins[t] = sqrtf(px * px + py * py + pz * pz) < 1.0f;
-      }
-      if (t < m) cosv[t] = (sd[t + 1] - sd[t]) / ((zs[t + 1] - zs[t]) + 1e-5f);
-      __syncthreads();
-      float alpha = 0.0f;
-      if (t < m) {
-        const float pc = t == 0 ? 0.0f : cosv[t - 1];
-        float c = fminf(pc, cosv[t]);
-        c = fminf(fmaxf(c, -1e3f), 0.0f) * ((ins[t] | ins[t + 1]) ? 1.0f : 0.0f);
-        const float dist = zs[t + 1] - zs[t];
-        const float mid = (sd[t] + sd[t + 1]) * 0.5f;
-        const float pe = mid - c * dist * 0.5f;
-        const float ne = mid + c * dist * 0.5f;
-        const float pcdf = sigmoid_ref(pe * inv_s);
-        const float ncdf = sigmoid_ref(ne * inv_s);
-        alpha = ((pcdf - ncdf) + 1e-5f) / (pcdf + 1e-5f);
-        wv[t] = (1.0f - alpha) + 1e-7f;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int i = t + 64 * h;
+        if (i < n) {
+          const float zz = zs[i];
+          const float px = ox + dx * zz, py = oy + dy * zz, pz = oz + dz * zz;
+          ins[i] = sqrtf(px * px + py * py + pz * pz) < 1.0f;
+        }
+        if (i < m) cosv[i] = (sd[i + 1] - sd[i]) / ((zs[i + 1] - zs[i]) + 1e-5f);
       }
       __syncthreads();
-      if (t < 64) {
-        // exclusive cumprod in double, sequential (as torch.cumprod on the CPU): wave 0
-        // holds the operands in registers and walks them lane by lane
-        const float w0 = t < m ? wv[t] : 0.0f, w1 = t + 64 < m ? wv[t + 64] : 0.0f;
+      float alpha[2] = {0.0f, 0.0f};
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int i = t + 64 * h;
+        if (i < m) {
+          const float pc = i == 0 ? 0.0f : cosv[i - 1];
+          float c = fminf(pc, cosv[i]);
+          c = fminf(fmaxf(c, -1e3f), 0.0f) * ((ins[i] | ins[i + 1]) ? 1.0f : 0.0f);
+          const float dist = zs[i + 1] - zs[i];
+          const float mid = (sd[i] + sd[i + 1]) * 0.5f;
+          const float pe = mid - c * dist * 0.5f;
+          const float ne = mid + c * dist * 0.5f;
+          const float pcdf = sigmoid_ref(pe * inv_s);
+          const float ncdf = sigmoid_ref(ne * inv_s);
+          alpha[h] = ((pcdf - ncdf) + 1e-5f) / (pcdf + 1e-5f);
+        }
+      }
+      {
+        // exclusive cumprod of (1 - alpha + 1e-7) in double, sequential (as torch.cumprod
+        // on the CPU), walked lane by lane through the registers
+        const float w0 = t < m ? (1.0f - alpha[0]) + 1e-7f : 0.0f;
+        const float w1 = t + 64 < m ? (1.0f - alpha[1]) + 1e-7f : 0.0f;
         double T = 1.0;
         float c0 = 0.0f, c1 = 0.0f;
         for (int j = 0; j < m; ++j) {
@@ -132,17 +155,19 @@ __global__ void __launch_bounds__(kSampThreads) upsample_kernel(SamplerDev a) {
           }
           T *= (double)wj;
         }
-        if (t < m) cdf[t] = c0;
-        if (t + 64 < m) cdf[t + 64] = c1;
-      }
-      __syncthreads();
-      if (t < m) {
-        float w = alpha * cdf[t];
-        if (OC.occ) {
-          const float mz = (zs[t] + zs[t + 1]) * 0.5f;
-          w = w * occ_factor(OC, ox + dx * mz, oy + dy * mz, oz + dz * mz);
+        const float cc[2] = {c0, c1};
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int i = t + 64 * h;
+          if (i < m) {
+            float w = alpha[h] * cc[h];
+            if (OC.occ) {
+              const float mz = (zs[i] + zs[i + 1]) * 0.5f;
+              w = w * occ_factor(OC, ox + dx * mz, oy + dy * mz, oz + dz * mz);
+            }
+            wv[i] = w + 1e-5f;
+          }
         }
-        wv[t] = w + 1e-5f;
       }
       __syncthreads();
       // 3. sample_pdf: row sum in the 8-lane cascade order
@@ -156,7 +181,7 @@ __global__ void __launch_bounds__(kSampThreads) upsample_kernel(SamplerDev a) {
         acc8[t] = ((acc[0] + acc[1]) + acc[2]) + acc[3];
       }
       __syncthreads();
-      if (t < 64) {
+      {
         float tot = 0.0f;
         if (t == 0) {
           if (nvec == 0) {  // rows shorter than one vector: scalar 4-accumulator cascade
@@ -221,20 +246,24 @@ __global__ void __launch_bounds__(kSampThreads) upsample_kernel(SamplerDev a) {
         nzs[r] = v;
       }
       __syncthreads();
-      float zo = 0.0f, so = 0.0f, zn = 0.0f, sn = 0.0f;
-      int po = -1, pn = -1;
-      if (t < n) {
-        zo = zs[t];
-        so = sd[t];
-        int lo = 0, hi = a.per;  // count of new < zo
-        while (lo < hi) {
-          const int md = (lo + hi) >> 1;
-          if (nzs[md] < zo)
-            lo = md + 1;
-          else
-            hi = md;
+      float zo[2] = {0.0f, 0.0f}, so[2] = {0.0f, 0.0f}, zn = 0.0f, sn = 0.0f;
+      int po[2] = {-1, -1}, pn = -1;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int i = t + 64 * h;
+        if (i < n) {
+          zo[h] = zs[i];
+          so[h] = sd[i];
+          int lo = 0, hi = a.per;  // count of new < zo
+          while (lo < hi) {
+            const int md = (lo + hi) >> 1;
+            if (nzs[md] < zo[h])
+              lo = md + 1;
+            else
+              hi = md;
+          }
+          po[h] = i + lo;
         }
-        po = t + lo;
       }
       if (t < a.per) {
         zn = nzs[t];
@@ -253,9 +282,12 @@ __global__ void __launch_bounds__(kSampThreads) upsample_kernel(SamplerDev a) {
         }
       }
       __syncthreads();
-      if (po >= 0) {
-        zs[po] = zo;
-        sd[po] = so;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        if (po[h] >= 0) {
+          zs[po[h]] = zo[h];
+          sd[po[h]] = so[h];
+        }
       }
       if (pn >= 0) {
         zs[pn] = zn;
@@ -265,27 +297,38 @@ __global__ void __launch_bounds__(kSampThreads) upsample_kernel(SamplerDev a) {
       n += a.per;
     }
     // 5. final intervals and midpoints (sampler.py:197-202)
-    float midv = 0.0f;
-    if (t < n) {
-      const float zz = zs[t];
-      const float dist = t < n - 1 ? zs[t + 1] - zz : zs[n - 1] - zs[n - 2];
-      midv = zz + dist * 0.5f;
-      a.z_out[ray * n + t] = zz;
-      a.dists_out[ray * n + t] = dist;
-      a.mid_out[ray * n + t] = midv;
+    float midv[2] = {0.0f, 0.0f};
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int i = t + 64 * h;
+      if (i < n) {
+        const float zz = zs[i];
+        const float dist = i < n - 1 ? zs[i + 1] - zz : zs[n - 1] - zs[n - 2];
+        midv[h] = zz + dist * 0.5f;
+        a.z_out[ray * n + i] = zz;
+        a.dists_out[ray * n + i] = dist;
+        a.mid_out[ray * n + i] = midv[h];
+      }
     }
     if (!MULTI && a.vr) {
       // V_R of the render's samples (their positions are o + d * mid, as the render forms
       // them), marked here so that the regularisers need not wait for the render: a
-      // sample in the same cell as the previous lane's marks nothing new
-      Corners cn{};
-      int key = -1;
-      if (t < n) {
-        cn = corner_decomp(G, ox + dx * midv, oy + dy * midv, oz + dz * midv);
-        key = (int)lin3(G, cn.c0[0], cn.c0[1], cn.c0[2]);
+      // sample in the same cell as the sample before it marks nothing new
+      Corners cn[2];
+      int key[2] = {-1, -1};
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int i = t + 64 * h;
+        if (i < n) {
+          cn[h] = corner_decomp(G, ox + dx * midv[h], oy + dy * midv[h], oz + dz * midv[h]);
+          key[h] = (int)lin3(G, cn[h].c0[0], cn[h].c0[1], cn[h].c0[2]);
+        }
       }
-      const int kp = __shfl_up(key, 1, 64);
-      if (t < n && ((t & 63) == 0 || kp != key)) vr_mark_cell(G, cn, a.vr);
+      const int kp0 = __shfl_up(key[0], 1, 64);
+      const int k63 = __shfl(key[0], 63, 64);
+      const int kp1 = t == 0 ? k63 : __shfl_up(key[1], 1, 64);
+      if (t < n && (t == 0 || kp0 != key[0])) vr_mark_cell(G, cn[0], a.vr);
+      if (t + 64 < n && kp1 != key[1]) vr_mark_cell(G, cn[1], a.vr);
     }
     __syncthreads();
   }
