@@ -286,9 +286,11 @@ int vxn_curvature_grad(const float* grid, int X, int Y, int Z, const int64_t* ac
 
 /* The same regularisers straight from the V_R bitmask the render marks, for the fused
  * step: counts[2] (device) receives {|V_R|, |V_R inside the relaxed sphere|} (box6 and
- * relax_radius as vxn_vr_compact), then every grid vertex gathers its gradient from the
- * V_R vertices whose stencils hold it (no atomics, deterministic).  Same losses, gradient
- * and |V_R| normalisation as vxn_reg_eik_curv on the compacted list; replaces
+ * relax_radius as vxn_vr_compact), then work-groups walk the grid's (x, y) rows, test
+ * V_R membership in the bitmask and add each member's eikonal and curvature terms with
+ * one atomic per target (a z-line's own and neighbour terms merged across lanes).  Same
+ * losses, gradient and |V_R| normalisation as vxn_reg_eik_curv on the compacted list;
+ * replaces
  * extract_active_vertices + the relaxed-sphere filter + both regularisers
  * (regularization/active_vertices.py:12-50, trainer.py:260-266, eikonal.py:13-171,
  * curvature.py:13-165).  Workspace: vxn_reg_vr_workspace_bytes.                         */
@@ -301,6 +303,18 @@ int vxn_reg_eik_curv_vr(const float* grid, int X, int Y, int Z, const float* eps
  * (device, nullable) receives 1 or 0.  Replaces amplify_s_gradient
  * (training/scheduler.py:14-33) without its host read-back.                           */
 int vxn_s_amplify(float* grad_v, float k, int32_t* amplified, void* stream);
+
+/* Post-upsample eikonal clamp: dst[i] += g[i] * c with g = src * weight and
+ * c = (1 / max|g|) * clamp when max|g| > clamp, else c = 1 (a NaN maximum leaves c = 1),
+ * in the reference's float32 operation order.  src is the eikonal gradient computed with
+ * weight 1 into its own buffer; dst is grid.grad.  dst is read and written without
+ * atomics, so nothing may write it concurrently; dst and src must not overlap.  Replaces
+ * the cooldown branch of VoxNeuSTrainer.train_step (training/trainer.py:309-316), which
+ * trainer.py:131-164 arms for 500 steps after each grid super-resolution.  Workspace:
+ * vxn_eik_clamp_add_workspace_bytes.                                                   */
+int vxn_eik_clamp_add_workspace_bytes(int64_t* bytes);
+int vxn_eik_clamp_add(float* dst, const float* src, int64_t n, float weight, float clamp, void* workspace,
+                      int64_t workspace_bytes, void* stream);
 
 /* ---- Occupancy grid -----------------------------------------------------------------
  * OccupancyGrid.update_from_sdf (rendering/occupancy_grid.py:62-119), called every 16

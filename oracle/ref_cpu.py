@@ -626,6 +626,19 @@ def amplify_s_gradient(grad_v, k=5.0):
     return grad_v, False
 
 
+def eik_clamp_add(grad, g_eik, w_eik, clamp, cooldown=True):
+    """trainer.py:309-316: grid.grad + w_eik * g_eik, the weighted eikonal gradient scaled
+    to max-abs ``clamp`` while the post-upsample cooldown runs.  Float32 tensor ops in the
+    reference's order; ``clamp / norm`` is a Python float over a 0-d tensor, which PyTorch
+    evaluates as ``norm.reciprocal() * clamp``."""
+    g = g_eik * w_eik
+    if cooldown:
+        norm = g.abs().max()
+        if norm > clamp:
+            g = g * (norm.reciprocal() * clamp)
+    return grad + g
+
+
 # ----------------------------------------------------------------------------------------
 # Optimizers (Code/voxneus/training/optimizer.py, cuda/adam_upd_kernel.cu, torch.optim.Adam)
 # Every elementwise step is its own PyTorch op, so each rounds to float32 once.  Square

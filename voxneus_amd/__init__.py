@@ -20,7 +20,7 @@ _LAZY = {
     "extract_active_vertices": "regularization", "compute_eikonal_loss_and_gradient": "regularization",
     "compute_curvature_loss_and_gradient": "regularization",
     "amplify_s_gradient": "training", "FusedTrainStep": "training", "StepConfig": "training",
-    "FusedOptimizer": "training", "compute_analytical_gradient": "sdf_grid",
+    "FusedOptimizer": "training", "SuperResolutionSchedule": "training", "compute_analytical_gradient": "sdf_grid",
     "SparseGridAdam": "optim", "NetAdam": "optim", "create_optimizer": "optim",
     "upsample_sdf_grid": "sdf_grid", "get_resolution_for_step": "sdf_grid", "logistic_density": "rendering",
     "extract_mesh_marching_cubes": "mesh", "marching_cubes": "mesh",

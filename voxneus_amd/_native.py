@@ -144,6 +144,8 @@ _SIGS = {
     "vxn_eikonal_grad": (c_i32, [c_vp, c_i32, c_i32, c_i32, c_vp, c_i64, c_vp, c_vp, c_f, c_vp]),
     "vxn_curvature_grad": (c_i32, [c_vp, c_i32, c_i32, c_i32, c_vp, c_i64, c_vp, c_vp, c_f, c_vp]),
     "vxn_s_amplify": (c_i32, [c_vp, c_f, c_vp, c_vp]),
+    "vxn_eik_clamp_add_workspace_bytes": (c_i32, [ctypes.POINTER(c_i64)]),
+    "vxn_eik_clamp_add": (c_i32, [c_vp, c_vp, c_i64, c_f, c_f, c_vp, c_i64, c_vp]),
     "vxn_reg_vr_workspace_bytes": (c_i32, [c_i32, c_i32, c_i32, ctypes.POINTER(c_i64)]),
     "vxn_reg_eik_curv_vr": (c_i32, [c_vp, c_i32, c_i32, c_i32, c_vp, c_vp, c_f, c_vp, c_f, c_f, c_vp, c_vp, c_vp,
                                     c_vp, c_i64, c_vp]),

@@ -459,6 +459,95 @@ __global__ void __launch_bounds__(kGatherThreads) reg_vr_kernel(GridDesc g, cons
   }
 }
 
+// Post-upsample eikonal clamp (trainer.py:309-316): max |src| per work-group, then the
+// scale factor, then dst += (src * w) * c.  NaN propagates through the maximum as
+// torch's max does, and a NaN maximum leaves c = 1 (the reference's comparison is false).
+constexpr int kClampBlocks = 1024;
+constexpr int kClampThreads = 256;
+
+VXN_DEV float nan_max(float a, float b) { return (a > b || a != a) ? a : b; }
+
+VXN_DEV float block_nan_max(float m) {
+  __shared__ float red[kClampThreads / 64];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = nan_max(m, __shfl_xor(m, o, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  m = red[0];
+#pragma unroll
+  for (int k = 1; k < kClampThreads / 64; ++k) m = nan_max(m, red[k]);
+  return m;
+}
+
+__global__ void __launch_bounds__(kClampThreads) absmax_partial_kernel(const float* __restrict__ src, int64_t n,
+                                                                       int vec, float* __restrict__ part) {
+  float m = 0.0f;
+  const int64_t stride = (int64_t)gridDim.x * kClampThreads;
+  const int64_t t0 = blockIdx.x * (int64_t)kClampThreads + threadIdx.x;
+  int64_t head = 0;
+  if (vec) {
+    const int64_t nv = n >> 2;
+    const float4* s4 = (const float4*)src;
+    for (int64_t i = t0; i < nv; i += stride) {
+      const float4 v = s4[i];
+      m = nan_max(m, nan_max(nan_max(fabsf(v.x), fabsf(v.y)), nan_max(fabsf(v.z), fabsf(v.w))));
+    }
+    head = nv << 2;
+  }
+  for (int64_t i = head + t0; i < n; i += stride) m = nan_max(m, fabsf(src[i]));
+  m = block_nan_max(m);
+  if (threadIdx.x == 0) part[blockIdx.x] = m;
+}
+
+// part[nparts] <- the factor c of max |src * w| = max|src| * |w| (rounding is monotone)
+__global__ void __launch_bounds__(kClampThreads) clamp_factor_kernel(float* __restrict__ part, int nparts, float w,
+                                                                     float clamp) {
+  float m = 0.0f;
+  for (int i = threadIdx.x; i < nparts; i += kClampThreads) m = nan_max(m, part[i]);
+  m = block_nan_max(m);
+  if (threadIdx.x == 0) {
+    const float mw = m * fabsf(w);
+    part[nparts] = mw > clamp ? (1.0f / mw) * clamp : 1.0f;
+  }
+}
+
+__global__ void __launch_bounds__(kClampThreads) clamped_add_kernel(float* __restrict__ dst,
+                                                                    const float* __restrict__ src, int64_t n, int vec,
+                                                                    const float* __restrict__ factor, float w) {
+  const float c = *factor;
+  const bool sc = c != 1.0f;
+  const int64_t stride = (int64_t)gridDim.x * kClampThreads;
+  const int64_t t0 = blockIdx.x * (int64_t)kClampThreads + threadIdx.x;
+  int64_t head = 0;
+  if (vec) {
+    const int64_t nv = n >> 2;
+    const float4* s4 = (const float4*)src;
+    float4* d4 = (float4*)dst;
+    for (int64_t i = t0; i < nv; i += stride) {
+      const float4 v = s4[i];
+      float4 o = d4[i];
+      float g0 = v.x * w, g1 = v.y * w, g2 = v.z * w, g3 = v.w * w;
+      if (sc) {
+        g0 = g0 * c;
+        g1 = g1 * c;
+        g2 = g2 * c;
+        g3 = g3 * c;
+      }
+      o.x = o.x + g0;
+      o.y = o.y + g1;
+      o.z = o.z + g2;
+      o.w = o.w + g3;
+      d4[i] = o;
+    }
+    head = nv << 2;
+  }
+  for (int64_t i = head + t0; i < n; i += stride) {
+    float g = src[i] * w;
+    if (sc) g = g * c;
+    dst[i] = dst[i] + g;
+  }
+}
+
 __global__ void s_amplify_kernel(float* grad_v, float k, int32_t* amplified) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     const float gv = grad_v[0];
@@ -641,6 +730,34 @@ extern "C" int vxn_curvature_grad(const float* grid, int X, int Y, int Z, const 
 extern "C" int vxn_s_amplify(float* grad_v, float k, int32_t* amplified, void* stream) {
   VXN_REQUIRE(grad_v != nullptr, VXN_ERR_NULL, "grad_v is null");
   hipLaunchKernelGGL(s_amplify_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, grad_v, k, amplified);
+  VXN_LAUNCH_CHECK();
+  return 0;
+}
+
+extern "C" int vxn_eik_clamp_add_workspace_bytes(int64_t* bytes) {
+  VXN_REQUIRE(bytes != nullptr, VXN_ERR_NULL, "bytes is null");
+  *bytes = (int64_t)(kClampBlocks + 1) * 4;
+  return 0;
+}
+
+extern "C" int vxn_eik_clamp_add(float* dst, const float* src, int64_t n, float weight, float clamp, void* workspace,
+                                 int64_t workspace_bytes, void* stream) {
+  VXN_REQUIRE(n >= 0, VXN_ERR_SHAPE, "n must be >= 0");
+  if (n == 0) return 0;
+  VXN_REQUIRE(dst && src, VXN_ERR_NULL, "null argument");
+  VXN_REQUIRE(!(dst < src + n && src < dst + n), VXN_ERR_VALUE, "dst and src must not overlap");
+  VXN_REQUIRE(workspace != nullptr && workspace_bytes >= (int64_t)(kClampBlocks + 1) * 4, VXN_ERR_WORKSPACE,
+              "clamp workspace needs %lld bytes", (long long)(kClampBlocks + 1) * 4);
+  const int vec = (((uintptr_t)dst | (uintptr_t)src) & 15) == 0 ? 1 : 0;
+  float* part = (float*)workspace;
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(absmax_partial_kernel, dim3(kClampBlocks), dim3(kClampThreads), 0, s, src, n, vec, part);
+  VXN_LAUNCH_CHECK();
+  hipLaunchKernelGGL(clamp_factor_kernel, dim3(1), dim3(kClampThreads), 0, s, part, kClampBlocks, weight, clamp);
+  VXN_LAUNCH_CHECK();
+  const int64_t units = vec ? (n >> 2) : n;
+  hipLaunchKernelGGL(clamped_add_kernel, dim3(blocks_for(units, kClampThreads, 8192)), dim3(kClampThreads), 0, s,
+                     dst, src, n, vec, (const float*)(part + kClampBlocks), weight);
   VXN_LAUNCH_CHECK();
   return 0;
 }

@@ -515,6 +515,25 @@ def s_amplify(grad_v: torch.Tensor, k: float, amplified: Optional[torch.Tensor] 
                                   _stream(grad_v)), "s_amplify")
 
 
+def eik_clamp_workspace(device) -> torch.Tensor:
+    nb = ctypes.c_int64(0)
+    N.check(N.lib().vxn_eik_clamp_add_workspace_bytes(ctypes.byref(nb)), "eik_clamp_add_workspace_bytes")
+    return torch.empty(int(nb.value), device=device, dtype=torch.uint8)
+
+
+def eik_clamp_add(dst: torch.Tensor, src: torch.Tensor, weight: float, clamp: float,
+                  workspace: Optional[torch.Tensor] = None) -> None:
+    """dst += src * weight, scaled down to max-abs ``clamp`` when it exceeds it
+    (trainer.py:309-316, the post-upsample cooldown); on dst's current stream."""
+    if dst.shape != src.shape:
+        raise ValueError(f"dst {tuple(dst.shape)} and src {tuple(src.shape)} differ")
+    if workspace is None:
+        workspace = eik_clamp_workspace(dst.device)
+    N.check(N.lib().vxn_eik_clamp_add(N.dptr(dst, torch.float32, "dst"), N.dptr(src, torch.float32, "src"),
+                                      dst.numel(), float(weight), float(clamp), workspace.data_ptr(),
+                                      workspace.numel(), _stream(dst)), "eik_clamp_add")
+
+
 # ----------------------------------------------------------------------------------------
 # Occupancy grid EMA update
 # ----------------------------------------------------------------------------------------

@@ -88,6 +88,9 @@ class StepConfig:
     concurrent_regularisers: bool = True
     # start them once the hash-feature gather is done (else right after the sampler)
     fork_after_features: bool = False
+    # max-abs of the weighted eikonal gradient during a post-upsample cooldown
+    # (trainer.py:118, scheduler.eik_grad_clamp)
+    eik_grad_clamp: float = 0.1
 
 
 class FusedTrainStep:
@@ -136,6 +139,11 @@ class FusedTrainStep:
         self.s_value = torch.zeros((), device=dev, dtype=torch.float32)  # s of the last step, on the device
         self.kernel_events = None  # optional N.KernelEvents(6) recorded between the render kernels
         self.side = N.SideStream(dev) if cfg.concurrent_regularisers else None
+        # steps left in the post-upsample cooldown (trainer.py:119, 163): while > 0 the
+        # eikonal gradient goes to its own buffer and is clamped before it is added
+        self.post_upsample_cooldown = 0
+        self._g_eik = None
+        self._clamp_ws = None
         self.level_w = None
         if cfg.active_levels is not None:
             from .hash_grid import level_weights
@@ -181,17 +189,38 @@ class FusedTrainStep:
                            cfg.up_sample_steps, cfg.perturb, self.occ_mask, self.box, t_rand, u_rand,
                            bound_radius=1.0, vr_bitmask=vr_bitmask)
 
+    def _cooldown(self) -> bool:
+        return self.post_upsample_cooldown > 0 and self.cfg.w_eik > 0
+
     def _regularisers(self):
         # V_R (relaxed-sphere filtered) and both regularisers straight from the bitmask
         cfg = self.cfg
+        if self._cooldown():
+            # trainer.py:296-318: the eikonal gradient (weight 1) into its own buffer, the
+            # curvature term into grad_grid; the clamped eikonal add follows the join
+            self._g_eik.zero_()
+            Fn.reg_eik_curv_vr(self.grid, self.eps3, self.box, cfg.relax_radius, self.bitmask, 1.0, 0.0, self._g_eik,
+                               self.reg_ws, self.reg_losses, self.vr_counts)
+            Fn.reg_eik_curv_vr(self.grid, self.eps3, self.box, cfg.relax_radius, self.bitmask, 0.0, cfg.w_curv,
+                               self.grad_grid, self.reg_ws, self.reg_losses, self.vr_counts)
+            return
         Fn.reg_eik_curv_vr(self.grid, self.eps3, self.box, cfg.relax_radius, self.bitmask, cfg.w_eik, cfg.w_curv,
                            self.grad_grid, self.reg_ws, self.reg_losses, self.vr_counts)
+
+    def _eik_clamp(self):
+        """trainer.py:310-316 on grad_grid, once the step's other writers of it are done."""
+        Fn.eik_clamp_add(self.grad_grid, self._g_eik, self.cfg.w_eik, self.cfg.eik_grad_clamp, self._clamp_ws)
+        self.post_upsample_cooldown -= 1
 
     def step(self, rays_o: torch.Tensor, rays_d: torch.Tensor, target: torch.Tensor, t_rand=None, u_rand=None,
              zs=None) -> Dict[str, torch.Tensor]:
         """One fwd+bwd step; gradients are overwritten.  Returns device tensors."""
         cfg = self.cfg
         rays_o, rays_d, target = rays_o.contiguous(), rays_d.contiguous(), target.contiguous()
+        cooldown = self._cooldown()
+        if cooldown and (self._g_eik is None or self._g_eik.shape != self.grid.shape):
+            self._g_eik = torch.zeros_like(self.grid)
+            self._clamp_ws = Fn.eik_clamp_workspace(self.grid.device)
         self.grad_grid.zero_()
         self.grad_table.zero_()
         self.bitmask.zero_()
@@ -225,8 +254,27 @@ class FusedTrainStep:
             Fn.s_amplify(self.grad_v, cfg.s_amp_k, self.amplified)
         if early:
             self.side.join(main)  # grad_grid, the losses and V_R counts are complete past this point
+        if cooldown:
+            self._eik_clamp()
         return {"loss": self.loss, "reg_losses": self.reg_losses, "vr_counts": self.vr_counts,
                 "amplified": self.amplified, "z_vals": z, "dists": dists, "mid_z_vals": mid}
+
+    def rescale_grid(self, new_resolution) -> None:
+        """DenseSDFGrid.scale_volume_grid (sdf_grid.py:183-195) on this step's grid: the
+        grid is replaced by its trilinear upsampling (aligned corners, on the device) and
+        the gradient, V_R and regulariser buffers follow the new shape."""
+        shape = (new_resolution,) * 3 if isinstance(new_resolution, int) else tuple(int(v) for v in new_resolution)
+        dev = self.grid.device
+        self.grid = Fn.upsample_grid(self.grid, shape)
+        X, Y, Z = shape
+        self.grad_grid = torch.zeros_like(self.grid)
+        self.bitmask = torch.zeros(Fn.vr_bitmask_words((X, Y, Z)), device=dev, dtype=torch.int32)
+        nb = ctypes.c_int64(0)
+        N.check(N.lib().vxn_reg_vr_workspace_bytes(X, Y, Z, ctypes.byref(nb)))
+        self.reg_ws = torch.empty(int(nb.value), device=dev, dtype=torch.uint8)
+        self.eps3 = Fn.voxel_size_of(self.box, (X, Y, Z))
+        self._g_eik = None
+        self._args = None  # the render arguments hold the old grid
 
     def update_occupancy(self, subsample_ratio: float = 0.25, generator=None, full_sweep: bool = False,
                          decay: float = 0.95, threshold: float = 0.01):
@@ -284,6 +332,14 @@ class FusedOptimizer:
         self.sdf_step = 0
         self.net_step = 0
 
+    def reset_sdf(self, lr: float) -> None:
+        """A fresh SparseGridAdam on the step's (rescaled) grid at rate ``lr``
+        (trainer.py:150-159): zero moments, step count from 0."""
+        z = torch.zeros(self.st.grid.shape, device=self.st.grid.device, dtype=torch.float32)
+        self.grid_m, self.grid_v = z, torch.zeros_like(z)
+        self.lr_sdf = float(lr)
+        self.sdf_step = 0
+
     def step(self, it: int, variance_floor: Optional[float] = None):
         """Apply both optimizers after ``FusedTrainStep.step`` of training iteration ``it``."""
         import math
@@ -293,6 +349,9 @@ class FusedOptimizer:
         L = N.lib()
         s = Fn._stream(st.grid)
         f = cosine_lr_factor(it, self.total_steps)
+        if self.grid_m.shape != st.grid.shape:
+            raise ValueError(f"the grid is {tuple(st.grid.shape)} but the SDF moments are "
+                             f"{tuple(self.grid_m.shape)}: call reset_sdf after rescaling the grid")
         self.sdf_step += 1
         self.n_updated.zero_()
         N.check(L.vxn_masked_adam_upd(st.grid.data_ptr(), st.grad_grid.data_ptr(), self.grid_m.data_ptr(),
@@ -315,3 +374,29 @@ class FusedOptimizer:
         N.check(L.vxn_adam_step(ctypes.cast(ts, ctypes.c_void_p), 3, self.net_step, 0.9, 0.999, 1e-8, s),
                 "adam_step")
 
+
+
+@dataclass
+class SuperResolutionSchedule:
+    """Progressive grid super-resolution of the trainer (trainer.py:107-109, 131-164):
+    at ``milestones[i]`` the grid goes to ``resolutions[i + 1]``, the SDF optimizer
+    restarts at ``lr_sdf_stages[i + 1]``, and the eikonal gradient is clamped for
+    ``cooldown_steps`` steps (the clamp itself is ``StepConfig.eik_grad_clamp``)."""
+
+    milestones: tuple = (10000, 30000)
+    resolutions: tuple = (96, 160, 320)
+    lr_sdf_stages: tuple = (0.1, 0.05, 0.02)
+    cooldown_steps: int = 500
+
+    def apply(self, step: int, st: FusedTrainStep, opt: Optional[FusedOptimizer] = None) -> bool:
+        """check_and_apply_upsampling: True if ``step`` is a milestone (and the grid was
+        rescaled)."""
+        ms = list(self.milestones)
+        if step not in ms:
+            return False
+        idx = ms.index(step) + 1
+        st.rescale_grid(int(self.resolutions[idx]))
+        if opt is not None:
+            opt.reset_sdf(float(self.lr_sdf_stages[min(idx, len(self.lr_sdf_stages) - 1)]))
+        st.post_upsample_cooldown = int(self.cooldown_steps)
+        return True
