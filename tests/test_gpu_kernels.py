@@ -201,6 +201,31 @@ def test_render_backward_matches_oracle_autograd(Fn):
     assert abs(v.grad.item() - m.variance.grad.item()) <= 1e-5 * max(1e-3, abs(m.variance.grad.item()))
 
 
+@pytest.mark.parametrize("log2_T", [19, 21])
+def test_render_backward_hash_gradient_large_tables(Fn, log2_T):
+    """The hash-table gradient with 2^19 entries per level (binned: 4096 bins of 2048
+    entries) and 2^21 (16384 bins, past the binned path's limit: the atomic scatter)."""
+    g = load_golden("step_c1.npz")
+    sc = scenes.config1_scene(seed=0)
+    emb = (torch.rand((16, 1 << log2_T, 2), generator=torch.Generator().manual_seed(log2_T)) - 0.5) * 0.2
+    m = R.RefModel(sc.grid, emb.numpy(), **sc.mlp, hash_max_resolution=sc.hash_max_resolution)
+    o, d = torch.from_numpy(g["rays_o"]), torch.from_numpy(g["rays_d"])
+    zs = tuple(torch.from_numpy(g[k]) for k in ("z_vals", "dists", "mid"))
+    rr = torch.from_numpy(np.random.default_rng(5).normal(size=(64, 3)).astype(np.float32))
+    out = m.render(o, d, None, None, zs=zs)
+    (out["rgb"] * rr).sum().backward()
+    cfg = Fn.RenderConfig(BOX6, m.level_res)
+    ed = emb.to(DEV).requires_grad_(True)
+    s = torch.clamp(torch.exp(torch.tensor(0.3, device=DEV) * 10.0), min=1e-4, max=300.0)
+    outs = Fn.RenderFunction.apply(torch.from_numpy(sc.grid).to(DEV), ed, _mlp_flat(sc).to(DEV), s, o.to(DEV),
+                                   d.to(DEV), zs[2].to(DEV), zs[1].to(DEV), cfg, False, None)
+    (outs[0] * rr.to(DEV)).sum().backward()
+    ref = m.embeddings.grad.numpy()
+    got = ed.grad.cpu().numpy()
+    assert np.array_equal(got != 0, ref != 0)
+    assert rel_err(got, ref) < 1e-5
+
+
 @pytest.mark.parametrize("name", ["step_c1.npz", "step_c2.npz"])
 def test_fused_train_step_matches_reference(name):
     from voxneus_amd.training import FusedTrainStep, StepConfig

@@ -901,6 +901,295 @@ __global__ void __launch_bounds__(256) hash_scatter_kernel(RenderDev a, int dedu
   }
 }
 
+// ------------------------------------------------------------------------------------
+// Binned hash-table gradient.  Device-scope float atomics sustain about 21 G requests/s
+// however they are spread, which bounds hash_scatter_kernel.  Here the same records are
+// binned by table region instead of added in place: (1) each work-group counts, per bin
+// of 2048 consecutive entries of one level, the records its samples produce (LDS
+// histogram); (2) an exclusive scan over [bin][work-group] gives every work-group its own
+// contiguous range in every bin; (3) the work-groups write their records there; (4) one
+// wavefront per bin sums the bin's records in LDS and adds the sums to grad_table.  A
+// record holds the two corners of a cell that differ only in x (their entries differ by
+// x ^ (x + 1) in the low bits, so they share a bin unless x ends in eleven 1 bits; then
+// each corner gets a record of its own, its second half adding zero to the same entry):
+// two entries within the bin and both features' weighted gradients of each.  Per live
+// sample and level that is 4 records, with runs of consecutive samples in one cell summed
+// first on the levels below `dedup_levels`, as in the atomic scatter.
+// ------------------------------------------------------------------------------------
+constexpr int kBinBits = 11;
+constexpr int kBinEntries = 1 << kBinBits;  // 16 KB of float2 sums + 8 KB of tags in LDS
+constexpr int kBinChunks = 4;               // 256-sample chunks per work-group (passes 1, 3)
+constexpr int kMaxBins = 8192;              // LDS counters of passes 1 and 3
+
+struct BinDev {
+  int nbl;          // bins per level: ceil(T / kBinEntries)
+  int nbins;        // L * nbl
+  int nwg;          // work-groups of passes 1 and 3
+  int32_t* off;     // [nbins * nwg + 1]: counts, then their exclusive scan and total
+  uint32_t* ridx;   // [records] entries within the bin: x corner | (x + 1 corner) << 16
+  float4* rval;     // [records] dL/dfeature 0, 1 of the x corner, then of the x + 1 corner
+};
+
+// Walks the hash levels of sample i (lanes of a wavefront hold consecutive samples) and
+// calls level(l, tail, cell, v) with every lane of the wavefront, on each level with a live
+// lane: tail marks the lanes that emit records (the last lane of each run of samples in
+// one cell on the levels below `dedup_levels`, every live lane above), v their 8 corners'
+// weighted gradients (run sums).  VALUES = false leaves v unset: only the tails are needed.
+template <bool VALUES, class F>
+VXN_DEV void hash_levels(const RenderDev& a, int64_t i, int lane, int dedup_levels, F&& level) {
+  const int64_t NS = a.R * (int64_t)a.S;
+  const bool in = i < NS;
+  float x0 = 0.0f, x1 = 0.0f, x2 = 0.0f;
+  if (in) {
+    float px, py, pz;
+    sample_pos(a, i, px, py, pz);
+    x0 = fminf(fmaxf((px - a.g.mn[0]) / a.g.den[0], 0.0f), 1.0f);
+    x1 = fminf(fmaxf((py - a.g.mn[1]) / a.g.den[1], 0.0f), 1.0f);
+    x2 = fminf(fmaxf((pz - a.g.mn[2]) / a.g.den[2], 0.0f), 1.0f);
+  }
+  for (int l = 0; l < a.h.L; ++l) {
+    float d0 = in ? a.gfeat[(2 * l) * NS + i] : 0.0f;
+    float d1 = in ? a.gfeat[(2 * l + 1) * NS + i] : 0.0f;
+    const float lw = a.h.lw[l];
+    if (lw == 0.0f) continue;
+    if (lw < 1.0f) {
+      d0 = d0 * lw;
+      d1 = d1 * lw;
+    }
+    const bool live = d0 != 0.0f || d1 != 0.0f;
+    if (__ballot(live) == 0) continue;
+    const HashCell c = hash_cell(x0, x1, x2, a.h.res[l], a.h.mask);
+    float v[16];
+    if constexpr (VALUES) {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        v[2 * k] = d0 * c.w[k];
+        v[2 * k + 1] = d1 * c.w[k];
+      }
+    }
+    bool tail = live;
+    if (l < dedup_levels) {
+      const float res = a.h.res[l];
+      const uint64_t cx = (uint64_t)(int)floorf(x0 * res), cy = (uint64_t)(int)floorf(x1 * res),
+                     cz = (uint64_t)(int)floorf(x2 * res);
+      const uint64_t key = live ? ((cx << 42) | (cy << 21) | cz) : (~0ull - (uint64_t)lane);
+      if constexpr (VALUES) {
+        tail = seg_sum<16>(key, v, lane) && live;
+      } else {
+        const uint64_t kn = __shfl_down(key, 1, 64);
+        tail = (lane == 63 || kn != key) && live;
+      }
+    }
+    level(l, tail, c, v);
+  }
+}
+
+VXN_DEV void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__global__ void __launch_bounds__(256) hash_bin_count_kernel(RenderDev a, BinDev b, int dedup_levels) {
+  __shared__ int32_t cnt[kMaxBins];
+  for (int k = threadIdx.x; k < b.nbins; k += 256) cnt[k] = 0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  for (int c = 0; c < kBinChunks; ++c) {
+    const int64_t i = ((int64_t)blockIdx.x * kBinChunks + c) * 256 + threadIdx.x;
+    hash_levels<false>(a, i, lane, dedup_levels, [&](int l, bool tail, const HashCell& cell, const float(&)[16]) {
+      if (tail) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int ba = (int)(cell.idx[q] >> kBinBits), bb = (int)(cell.idx[q + 4] >> kBinBits);
+          atomicAdd(&cnt[l * b.nbl + ba], 1);
+          if (bb != ba) atomicAdd(&cnt[l * b.nbl + bb], 1);
+        }
+      }
+    });
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < b.nbins; k += 256) b.off[(int64_t)k * b.nwg + blockIdx.x] = cnt[k];
+}
+
+// Records straight to their reserved positions (any number of bins per level).
+__global__ void __launch_bounds__(256) hash_bin_write_kernel(RenderDev a, BinDev b, int dedup_levels) {
+  __shared__ int32_t pos[kMaxBins];
+  for (int k = threadIdx.x; k < b.nbins; k += 256) pos[k] = b.off[(int64_t)k * b.nwg + blockIdx.x];
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  for (int c = 0; c < kBinChunks; ++c) {
+    const int64_t i = ((int64_t)blockIdx.x * kBinChunks + c) * 256 + threadIdx.x;
+    hash_levels<true>(a, i, lane, dedup_levels, [&](int l, bool tail, const HashCell& cell, const float(&v)[16]) {
+      if (tail) {
+        constexpr uint32_t m = kBinEntries - 1;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const uint32_t ia = cell.idx[q], ib = cell.idx[q + 4];
+          const int ba = (int)(ia >> kBinBits), bb = (int)(ib >> kBinBits);
+          const int32_t p = atomicAdd(&pos[l * b.nbl + ba], 1);
+          if (bb == ba) {
+            b.ridx[p] = (ia & m) | ((ib & m) << 16);
+            b.rval[p] = make_float4(v[2 * q], v[2 * q + 1], v[2 * q + 8], v[2 * q + 9]);
+          } else {
+            b.ridx[p] = (ia & m) * 0x10001u;
+            b.rval[p] = make_float4(v[2 * q], v[2 * q + 1], 0.0f, 0.0f);
+            const int32_t p2 = atomicAdd(&pos[l * b.nbl + bb], 1);
+            b.ridx[p2] = (ib & m) * 0x10001u;
+            b.rval[p2] = make_float4(v[2 * q + 8], v[2 * q + 9], 0.0f, 0.0f);
+          }
+        }
+      }
+    });
+  }
+}
+
+// One wavefront per bin.  LDS float atomics run at a few cycles per lane, so the bin is
+// summed with plain LDS read-add-write instead: in each round every pending half-record
+// writes its lane id to its entry's tag, and a lane adds a half whose tag reads back its
+// own id (one lane per entry per round; a lane whose two halves share the entry adds both
+// in turn); the others retry in the next round.  A single wavefront owns the bin's LDS
+// and its LDS instructions complete in order, so no update is lost.
+__global__ void __launch_bounds__(64) hash_bin_sum_kernel(BinDev b, float* __restrict__ grad_table, int64_t T) {
+  __shared__ float2 acc[kBinEntries];
+  __shared__ uint8_t tag[kBinEntries];
+  const int lane = threadIdx.x;
+  const int bin = blockIdx.x;
+  const int l = bin / b.nbl;
+  const int64_t base = (int64_t)(bin - l * b.nbl) << kBinBits;
+  const int ne = (int)min((int64_t)kBinEntries, T - base);
+  for (int e = lane; e < ne; e += 64) acc[e] = make_float2(0.0f, 0.0f);
+  wave_sync();
+  const int32_t r0 = b.off[(int64_t)bin * b.nwg], r1 = b.off[(int64_t)(bin + 1) * b.nwg];
+  constexpr int U = 8;  // 64-record groups loaded ahead
+  for (int32_t k0 = r0; k0 < r1; k0 += U * 64) {
+    uint32_t r[U];
+    float4 v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int32_t k = k0 + u * 64 + lane;
+      r[u] = k < r1 ? b.ridx[k] : 0u;
+      v[u] = k < r1 ? b.rval[k] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int e0 = (int)(r[u] & 0xFFFFu), e1 = (int)(r[u] >> 16);
+      bool p0 = k0 + u * 64 + lane < r1, p1 = p0;
+      while (__ballot(p0 || p1) != 0) {
+        if (p0) tag[e0] = (uint8_t)lane;
+        if (p1) tag[e1] = (uint8_t)lane;
+        wave_sync();
+        const bool w0 = p0 && tag[e0] == lane;
+        const bool w1 = p1 && tag[e1] == lane;
+        if (w0) {
+          float2 o = acc[e0];
+          o.x = o.x + v[u].x;
+          o.y = o.y + v[u].y;
+          acc[e0] = o;
+          p0 = false;
+        }
+        if (w1) {
+          float2 o = acc[e1];
+          o.x = o.x + v[u].z;
+          o.y = o.y + v[u].w;
+          acc[e1] = o;
+          p1 = false;
+        }
+        wave_sync();
+      }
+    }
+  }
+  float2* g = (float2*)(grad_table + ((int64_t)l * T + base) * 2);
+  for (int e = lane; e < ne; e += 64) {
+    const float2 sum = acc[e];
+    if (sum.x != 0.0f || sum.y != 0.0f) {
+      float2 o = g[e];
+      o.x = o.x + sum.x;
+      o.y = o.y + sum.y;
+      g[e] = o;
+    }
+  }
+}
+
+// Exclusive scan of n int32 in place (x has n + 1 entries; x[n] receives the total):
+// tile sums, one work-group over the tile sums, then each tile with its offset.
+constexpr int kScanThreads = 256;
+constexpr int kScanPer = 16;
+constexpr int kScanTile = kScanThreads * kScanPer;
+
+VXN_DEV int32_t block_excl_scan_i32(int32_t v, int32_t& total) {
+  __shared__ int32_t ws[kScanThreads / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int32_t inc = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int32_t u = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += u;
+  }
+  if (lane == 63) ws[wid] = inc;
+  __syncthreads();
+  int32_t pre = 0;
+  total = 0;
+#pragma unroll
+  for (int w = 0; w < kScanThreads / 64; ++w) {
+    if (w < wid) pre += ws[w];
+    total += ws[w];
+  }
+  return pre + inc - v;
+}
+
+__global__ void __launch_bounds__(kScanThreads) scan_tile_sums_kernel(const int32_t* __restrict__ x, int64_t n,
+                                                                      int32_t* __restrict__ tsum) {
+  const int64_t base = (int64_t)blockIdx.x * kScanTile;
+  int32_t s = 0;
+  for (int k = threadIdx.x; k < kScanTile; k += kScanThreads) {
+    const int64_t i = base + k;
+    if (i < n) s += x[i];
+  }
+  int32_t total;
+  block_excl_scan_i32(s, total);
+  if (threadIdx.x == 0) tsum[blockIdx.x] = total;
+}
+
+// One work-group: exclusive scan of nt tile sums in place, total at tsum[nt].
+__global__ void __launch_bounds__(kScanThreads) scan_tile_offsets_kernel(int32_t* __restrict__ tsum, int nt) {
+  const int per = (nt + kScanThreads - 1) / kScanThreads;
+  const int b0 = threadIdx.x * per;
+  int32_t s = 0;
+  for (int k = 0; k < per; ++k)
+    if (b0 + k < nt) s += tsum[b0 + k];
+  int32_t total;
+  int32_t run = block_excl_scan_i32(s, total);
+  for (int k = 0; k < per; ++k) {
+    if (b0 + k < nt) {
+      const int32_t v = tsum[b0 + k];
+      tsum[b0 + k] = run;
+      run += v;
+    }
+  }
+  if (threadIdx.x == 0) tsum[nt] = total;
+}
+
+__global__ void __launch_bounds__(kScanThreads) scan_tiles_kernel(int32_t* __restrict__ x, int64_t n,
+                                                                  const int32_t* __restrict__ tsum, int nt) {
+  const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanPer;
+  int32_t v[kScanPer];
+  int32_t s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanPer; ++k) {
+    v[k] = base + k < n ? x[base + k] : 0;
+    s += v[k];
+  }
+  int32_t total;
+  int32_t run = block_excl_scan_i32(s, total) + tsum[blockIdx.x];
+#pragma unroll
+  for (int k = 0; k < kScanPer; ++k) {
+    if (base + k < n) x[base + k] = run;
+    run += v[k];
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) x[n] = tsum[nt];
+}
+
 // Brick slots of the Eq. 7 stencil around a cell with base corner c0: 8 cell corners
 // (ox,oy,oz) then the 24 face neighbours at offset -1 / +2 along one axis.
 VXN_DEV int slot_corner(int ox, int oy, int oz) { return 4 * ox + 2 * oy + oz; }
@@ -1084,12 +1373,40 @@ static int render_blocks(int64_t R) {
 
 static int64_t align256(int64_t b) { return (b + 255) & ~(int64_t)255; }
 
-// Workspace layout: MLP partials | scalar partials | reduction rows | gfeat | ggeo | hfeat.
-struct RenderWs {
-  int64_t off_sc, off_rows, off_feat, off_geo, off_hf, total;
+// Shape of the binned hash gradient for ns samples over L levels of T = 2^log2_T
+// entries; use = false selects the atomic scatter (bins or records beyond int32 / LDS).
+struct BinPlan {
+  bool use;
+  int nbl, nbins, nwg, nt;
+  int64_t n_off, max_rec;
 };
 
-static RenderWs render_ws(int64_t R, int S) {
+// Records: 4 per sample and level, 8 on levels whose x can end in eleven 1 bits (res >=
+// 2047: the x corner pair may straddle two bins).
+static BinPlan bin_plan(int64_t ns, int L, int log2_T, const int32_t* level_res) {
+  BinPlan p{};
+  if (ns <= 0 || L <= 0 || L > VXN_MAX_LEVELS || log2_T < 0 || log2_T > 30) return p;
+  const int64_t T = (int64_t)1 << log2_T;
+  p.nbl = (int)((T + kBinEntries - 1) / kBinEntries);
+  p.nbins = L * p.nbl;
+  p.nwg = (int)((ns + 256 * kBinChunks - 1) / (256 * kBinChunks));
+  int per = 0;
+  for (int l = 0; l < L; ++l) per += level_res[l] >= kBinEntries - 1 ? 8 : 4;
+  p.max_rec = ns * per;
+  p.n_off = (int64_t)p.nbins * p.nwg;
+  p.nt = (int)((p.n_off + kScanTile - 1) / kScanTile);
+  p.use = p.nbins <= kMaxBins && p.max_rec < ((int64_t)1 << 31) && p.n_off + 1 < ((int64_t)1 << 31);
+  return p;
+}
+
+// Workspace layout: MLP partials | scalar partials | reduction rows | gfeat | ggeo | hfeat
+// | bin offsets | scan tile sums | record entries | record values.
+struct RenderWs {
+  int64_t off_sc, off_rows, off_feat, off_geo, off_hf, off_bin, off_tsum, off_ridx, off_rval, total;
+  BinPlan bins;
+};
+
+static RenderWs render_ws(int64_t R, int S, int L, int log2_T, const int32_t* level_res) {
   RenderWs w;
   const int64_t nb = render_blocks(R);
   const int64_t ns = R * (int64_t)S;
@@ -1098,8 +1415,22 @@ static RenderWs render_ws(int64_t R, int S) {
   w.off_feat = align256(w.off_rows + (int64_t)kRedRows * kNP * (int64_t)sizeof(float));
   w.off_geo = align256(w.off_feat + 32 * ns * (int64_t)sizeof(float));
   w.off_hf = align256(w.off_geo + kGeo * ns * (int64_t)sizeof(float));
-  w.total = align256(w.off_hf + 16 * ns * (int64_t)sizeof(uint32_t));
+  w.off_bin = align256(w.off_hf + 16 * ns * (int64_t)sizeof(uint32_t));
+  w.bins = bin_plan(ns, L, log2_T, level_res);
+  if (w.bins.use) {
+    w.off_tsum = align256(w.off_bin + (w.bins.n_off + 1) * 4);
+    w.off_ridx = align256(w.off_tsum + ((int64_t)w.bins.nt + 1) * 4);
+    w.off_rval = align256(w.off_ridx + w.bins.max_rec * 4);
+    w.total = align256(w.off_rval + w.bins.max_rec * 16);
+  } else {
+    w.off_tsum = w.off_ridx = w.off_rval = w.off_bin;
+    w.total = w.off_bin;
+  }
   return w;
+}
+
+static RenderWs render_ws(const vxn_render_args& A) {
+  return render_ws(A.n_rays, A.n_samples, A.n_levels, A.log2_T, A.level_res);
 }
 
 static int build_render_dev(const vxn_render_args& A, RenderDev& a, int mode) {
@@ -1165,7 +1496,7 @@ static int build_render_dev(const vxn_render_args& A, RenderDev& a, int mode) {
   a.n_scenes = 0;
   if (mode != kFwd) {
     VXN_REQUIRE(A.grad_grid && A.grad_table && A.grad_mlp && A.grad_s, VXN_ERR_NULL, "null gradient output");
-    const RenderWs w = render_ws(A.n_rays, A.n_samples);
+    const RenderWs w = render_ws(A);
     VXN_REQUIRE(A.workspace != nullptr && A.workspace_bytes >= w.total, VXN_ERR_WORKSPACE,
                 "render workspace needs %lld bytes", (long long)w.total);
     char* ws = (char*)A.workspace;
@@ -1217,13 +1548,41 @@ static int launch_render(const vxn_render_args* p, void* stream) {
   if (MODE != kFwd) {
     const int64_t ns = p->n_rays * (int64_t)p->n_samples;
     const int sb = (int)((ns + 255) / 256);
-    hipLaunchKernelGGL(hash_scatter_kernel, dim3(sb), dim3(256), 0, s, a, dedup_levels(a.h));
-    VXN_LAUNCH_CHECK();
+    const RenderWs w = render_ws(*p);
+    if (w.bins.use && ((uintptr_t)p->grad_table & 7) == 0) {
+      char* ws = (char*)p->workspace;
+      BinDev b;
+      b.nbl = w.bins.nbl;
+      b.nbins = w.bins.nbins;
+      b.nwg = w.bins.nwg;
+      b.off = (int32_t*)(ws + w.off_bin);
+      b.ridx = (uint32_t*)(ws + w.off_ridx);
+      b.rval = (float4*)(ws + w.off_rval);
+      int32_t* tsum = (int32_t*)(ws + w.off_tsum);
+      const int dl = dedup_levels(a.h);
+      hipLaunchKernelGGL(hash_bin_count_kernel, dim3(b.nwg), dim3(256), 0, s, a, b, dl);
+      VXN_LAUNCH_CHECK();
+      hipLaunchKernelGGL(scan_tile_sums_kernel, dim3(w.bins.nt), dim3(kScanThreads), 0, s, (const int32_t*)b.off,
+                         w.bins.n_off, tsum);
+      VXN_LAUNCH_CHECK();
+      hipLaunchKernelGGL(scan_tile_offsets_kernel, dim3(1), dim3(kScanThreads), 0, s, tsum, w.bins.nt);
+      VXN_LAUNCH_CHECK();
+      hipLaunchKernelGGL(scan_tiles_kernel, dim3(w.bins.nt), dim3(kScanThreads), 0, s, b.off, w.bins.n_off,
+                         (const int32_t*)tsum, w.bins.nt);
+      VXN_LAUNCH_CHECK();
+      hipLaunchKernelGGL(hash_bin_write_kernel, dim3(b.nwg), dim3(256), 0, s, a, b, dl);
+      VXN_LAUNCH_CHECK();
+      hipLaunchKernelGGL(hash_bin_sum_kernel, dim3(b.nbins), dim3(64), 0, s, b, p->grad_table,
+                         (int64_t)1 << p->log2_T);
+      VXN_LAUNCH_CHECK();
+    } else {
+      hipLaunchKernelGGL(hash_scatter_kernel, dim3(sb), dim3(256), 0, s, a, dedup_levels(a.h));
+      VXN_LAUNCH_CHECK();
+    }
     if (ev) VXN_HIP(hipEventRecord(ev[3], s));
     hipLaunchKernelGGL(grid_scatter_kernel, dim3(sb), dim3(256), 0, s, a);
     VXN_LAUNCH_CHECK();
     if (ev) VXN_HIP(hipEventRecord(ev[4], s));
-    const RenderWs w = render_ws(p->n_rays, p->n_samples);
     float* rows = (float*)((char*)p->workspace + w.off_rows);
     hipLaunchKernelGGL(render_reduce1_kernel, dim3((kNP + 255) / 256, kRedRows), dim3(256), 0, s, a.part_mlp, nb,
                        rows);
@@ -1243,7 +1602,7 @@ using namespace vxn;
 extern "C" int vxn_render_workspace_bytes(const vxn_render_args* a, int64_t* bytes) {
   VXN_REQUIRE(a != nullptr && bytes != nullptr, VXN_ERR_NULL, "null argument");
   VXN_REQUIRE(a->n_rays >= 0 && a->n_samples >= 0, VXN_ERR_SHAPE, "bad shape");
-  *bytes = render_ws(a->n_rays, a->n_samples).total;
+  *bytes = render_ws(*a).total;
   return 0;
 }
 
