@@ -45,15 +45,17 @@ LATTICE = 320
 #   render_kernel  grid forward 32 words (8 corners + 24 face neighbours) = 128 B, the hash
 #                  features (BF16: 64 B read back; FP32: 1024 B gathered in the kernel),
 #                  per-ray streaming ~9 B;
-#   hash_scatter   256 FP32 read-modify-writes = 2048 B;
+#   hash_gradient  the hash-table gradient stage (binned: count, scan, record write, per-bin
+#                  sums; 6 launches between the same two events), accounted as the 256 FP32
+#                  read-modify-writes it replaces = 2048 B;
 #   grid_scatter   32 words read-modify-write = 256 B.
 KERNEL_BYTES = {
     # FP32 form: the render kernel gathers the hash features itself (no hash_features launch)
-    "fp32": {"render_kernel": 128 + 1024 + 9, "hash_scatter": 2048, "grid_scatter": 256},
-    "bf16": {"hash_features": 512 + 64, "render_kernel": 128 + 64 + 9, "hash_scatter": 2048, "grid_scatter": 256},
+    "fp32": {"render_kernel": 128 + 1024 + 9, "hash_gradient": 2048, "grid_scatter": 256},
+    "bf16": {"hash_features": 512 + 64, "render_kernel": 128 + 64 + 9, "hash_gradient": 2048, "grid_scatter": 256},
 }
 # event index pairs bracketing each kernel in vxn_render_args.kernel_events
-KERNEL_EVENTS = {"hash_features": (0, 1), "render_kernel": (1, 2), "hash_scatter": (2, 3), "grid_scatter": (3, 4),
+KERNEL_EVENTS = {"hash_features": (0, 1), "render_kernel": (1, 2), "hash_gradient": (2, 3), "grid_scatter": (3, 4),
                  "reductions": (4, 5)}
 # lattice query: 32-word stencil (128 B) in, SDF + normal (16 B) out per lattice point
 LATTICE_BYTES_PER_POINT = 128 + 16

@@ -230,8 +230,9 @@ typedef struct vxn_render_args {
   void* workspace;
   int64_t workspace_bytes;
   /* nullable profiling hook: hipEvent_t[6] recorded on the stream before the hash-feature
-     gather and after it, the render kernel, the hash scatter, the grid scatter and the
-     reductions (forward: only the first three) */
+     gather and after it, the render kernel, the hash-table gradient (its binned passes or
+     the atomic scatter), the grid scatter and the reductions (forward: only the first
+     three) */
   void* const* kernel_events;
   int32_t gradient_mode; /* VXN_GRADIENT_*: the normal of every sample (VoxNeuS gradient_mode) */
   /* nullable hipEvent_t, recorded on `stream` after the hash-feature gather, just before

@@ -38,7 +38,7 @@ struct SamplerDev {
 };
 
 template <int MULTI>
-__global__ void __launch_bounds__(kSampThreads) upsample_kernel(SamplerDev a) {
+__global__ void __launch_bounds__(kSampThreads, 8) upsample_kernel(SamplerDev a) {
   // One wavefront per ray: lane t holds samples t and t + 64 (kMaxS = 128).
   __shared__ float zs[kMaxS], sd[kMaxS], wv[kMaxS], cdf[kMaxS + 1], cosv[kMaxS];
   __shared__ float nz[kMaxPer], nzs[kMaxPer];
