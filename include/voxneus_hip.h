@@ -248,7 +248,7 @@ typedef struct vxn_render_args {
 /* Workspace of the backward and training entries (the forward needs none): per-sample
  * records of the hash-feature and geometry gradients, and the binned hash-table gradient
  * (4 records of 20 bytes per sample and level, 8 on levels with res >= 2047, plus the
- * per-bin offsets): about 1.4 GB for 2^20 samples over 16 levels.  Tables beyond 2^20
+ * per-bin offsets): about 1.4 GB for 2^20 samples over 16 levels.  Tables beyond 2^21
  * entries per level (16 levels) take the atomic scatter and no record space.  The size
  * depends on n_rays, n_samples, n_levels, log2_T and level_res.                        */
 int vxn_render_workspace_bytes(const vxn_render_args* a, int64_t* bytes);

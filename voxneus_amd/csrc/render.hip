@@ -917,9 +917,9 @@ __global__ void __launch_bounds__(256) hash_scatter_kernel(RenderDev a, int dedu
 // first on the levels below `dedup_levels`, as in the atomic scatter.
 // ------------------------------------------------------------------------------------
 constexpr int kBinBits = 11;
-constexpr int kBinEntries = 1 << kBinBits;  // 16 KB of float2 sums + 8 KB of tags in LDS
+constexpr int kBinEntries = 1 << kBinBits;  // 16 KB of float2 sums + 2 KB of tags in LDS
 constexpr int kBinChunks = 4;               // 256-sample chunks per work-group (passes 1, 3)
-constexpr int kMaxBins = 8192;              // LDS counters of passes 1 and 3
+constexpr int kMaxBins = 16384;             // LDS counters of passes 1 and 3 (nbins x 4 B)
 
 struct BinDev {
   int nbl;          // bins per level: ceil(T / kBinEntries)
@@ -991,7 +991,7 @@ VXN_DEV void wave_sync() {
 }
 
 __global__ void __launch_bounds__(256) hash_bin_count_kernel(RenderDev a, BinDev b, int dedup_levels) {
-  __shared__ int32_t cnt[kMaxBins];
+  extern __shared__ int32_t cnt[];  // [nbins]
   for (int k = threadIdx.x; k < b.nbins; k += 256) cnt[k] = 0;
   __syncthreads();
   const int lane = threadIdx.x & 63;
@@ -1014,7 +1014,7 @@ __global__ void __launch_bounds__(256) hash_bin_count_kernel(RenderDev a, BinDev
 
 // Records straight to their reserved positions (any number of bins per level).
 __global__ void __launch_bounds__(256) hash_bin_write_kernel(RenderDev a, BinDev b, int dedup_levels) {
-  __shared__ int32_t pos[kMaxBins];
+  extern __shared__ int32_t pos[];  // [nbins]
   for (int k = threadIdx.x; k < b.nbins; k += 256) pos[k] = b.off[(int64_t)k * b.nwg + blockIdx.x];
   __syncthreads();
   const int lane = threadIdx.x & 63;
@@ -1381,8 +1381,8 @@ struct BinPlan {
   int64_t n_off, max_rec;
 };
 
-// Records: 4 per sample and level, 8 on levels whose x can end in eleven 1 bits (res >=
-// 2047: the x corner pair may straddle two bins).
+// Records: 4 per sample and level, 8 on levels whose x can end in kBinBits 1 bits (res >=
+// kBinEntries - 1: the x corner pair may straddle two bins).
 static BinPlan bin_plan(int64_t ns, int L, int log2_T, const int32_t* level_res) {
   BinPlan p{};
   if (ns <= 0 || L <= 0 || L > VXN_MAX_LEVELS || log2_T < 0 || log2_T > 30) return p;
@@ -1560,7 +1560,8 @@ static int launch_render(const vxn_render_args* p, void* stream) {
       b.rval = (float4*)(ws + w.off_rval);
       int32_t* tsum = (int32_t*)(ws + w.off_tsum);
       const int dl = dedup_levels(a.h);
-      hipLaunchKernelGGL(hash_bin_count_kernel, dim3(b.nwg), dim3(256), 0, s, a, b, dl);
+      const size_t lds = (size_t)b.nbins * sizeof(int32_t);
+      hipLaunchKernelGGL(hash_bin_count_kernel, dim3(b.nwg), dim3(256), lds, s, a, b, dl);
       VXN_LAUNCH_CHECK();
       hipLaunchKernelGGL(scan_tile_sums_kernel, dim3(w.bins.nt), dim3(kScanThreads), 0, s, (const int32_t*)b.off,
                          w.bins.n_off, tsum);
@@ -1570,7 +1571,7 @@ static int launch_render(const vxn_render_args* p, void* stream) {
       hipLaunchKernelGGL(scan_tiles_kernel, dim3(w.bins.nt), dim3(kScanThreads), 0, s, b.off, w.bins.n_off,
                          (const int32_t*)tsum, w.bins.nt);
       VXN_LAUNCH_CHECK();
-      hipLaunchKernelGGL(hash_bin_write_kernel, dim3(b.nwg), dim3(256), 0, s, a, b, dl);
+      hipLaunchKernelGGL(hash_bin_write_kernel, dim3(b.nwg), dim3(256), lds, s, a, b, dl);
       VXN_LAUNCH_CHECK();
       hipLaunchKernelGGL(hash_bin_sum_kernel, dim3(b.nbins), dim3(64), 0, s, b, p->grad_table,
                          (int64_t)1 << p->log2_T);
