@@ -107,6 +107,18 @@ int vxn_lattice_query(const float* grid, int X, int Y, int Z, const float* box6,
  * (models/sdf_grid.py:183-195); out must not overlap grid.                           */
 int vxn_upsample_grid(const float* grid, int X, int Y, int Z, float* out, int NX, int NY, int NZ, void* stream);
 
+/* Post-training smoothing of the grid before mesh extraction: a separable Gaussian
+ * filter with edge replication, one 1-D correlation per axis (x, then y, then z), each
+ * accumulated in double and rounded to float32 (scipy.ndimage.gaussian_filter with
+ * mode="nearest").  weights (host, 2*radius+1 doubles, symmetric, radius <= 32) are the
+ * correlation weights (scipy's normalised Gaussian of radius int(4 sigma + 0.5)).
+ * Replaces apply_gaussian_filter_to_grid (geometry/mesh.py:17-31), the optional
+ * pre-filter of extract_mesh_marching_cubes (mesh.py:56-57); out must not overlap grid.
+ * Workspace: vxn_gaussian_filter_workspace_bytes (one grid of float32).               */
+int vxn_gaussian_filter_workspace_bytes(int X, int Y, int Z, int64_t* bytes);
+int vxn_gaussian_filter(const float* grid, int X, int Y, int Z, const double* weights, int radius, float* out,
+                        void* workspace, int64_t workspace_bytes, void* stream);
+
 /* ---- Mesh extraction -----------------------------------------------------------------
  * Marching cubes of the grid at `level` (value < level is inside), replacing the
  * scikit-image call of extract_mesh_marching_cubes (geometry/mesh.py:34-77): one vertex

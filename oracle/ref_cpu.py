@@ -989,10 +989,31 @@ def marching_cubes(volume, level=0.0, table=None):
     return p, faces
 
 
-def extract_mesh_marching_cubes(sdf_grid, xyz_min, xyz_max, level=0.0):
-    """mesh.py:34-77 (without the optional Gaussian pre-filter): world-space float32
-    vertices, index * (xyz_max - xyz_min) / (res - 1) + xyz_min, and int32 faces."""
+def apply_gaussian_filter_to_grid(sdf_grid, sigma=1.0):
+    """mesh.py:17-31: scipy.ndimage.gaussian_filter(sdf_grid, sigma, mode="nearest"), the
+    library routine the reference calls (scipy is installed here; a float32 grid is
+    filtered axis by axis, each 1-D pass accumulated in double and stored as float32)."""
+    from scipy.ndimage import gaussian_filter
+    return gaussian_filter(sdf_grid, sigma=sigma, mode="nearest")
+
+
+def gaussian_weights(sigma, truncate=4.0):
+    """scipy's 1-D Gaussian correlation weights (scipy/ndimage/_filters.py
+    gaussian_filter1d and _gaussian_kernel1d, order 0): radius int(truncate*sigma + 0.5),
+    exp(-0.5/sigma^2 * x^2) over x = -r..r, divided by its sum, in float64."""
     import numpy as np
+    r = int(truncate * float(sigma) + 0.5)
+    x = np.arange(-r, r + 1)
+    phi = np.exp(-0.5 / (float(sigma) * float(sigma)) * x ** 2)
+    return phi / phi.sum(), r
+
+
+def extract_mesh_marching_cubes(sdf_grid, xyz_min, xyz_max, level=0.0, post_filter_sigma=None):
+    """mesh.py:34-77: world-space float32 vertices, index * (xyz_max - xyz_min) / (res - 1)
+    + xyz_min, and int32 faces, after the optional Gaussian pre-filter (mesh.py:56-57)."""
+    import numpy as np
+    if post_filter_sigma is not None and post_filter_sigma > 0:
+        sdf_grid = apply_gaussian_filter_to_grid(np.asarray(sdf_grid).copy(), post_filter_sigma)
     verts, faces = marching_cubes(sdf_grid, level)
     res = np.array(np.asarray(sdf_grid).shape, dtype=np.float64)
     mn = np.asarray(xyz_min, dtype=np.float32)

@@ -1,7 +1,9 @@
-"""Marching cubes and grid super-resolution on the GPU: meshes equal the CPU oracle's
-(vertices bit-exact, same triangles in the same order) on closed shapes, random fields
-full of ambiguous cells and a 128^3 scene grid; the reference's own upsampling checks
-(test_grid_upsampling.py) hold on the device."""
+"""Marching cubes, the Gaussian pre-filter and grid super-resolution on the GPU: meshes
+equal the CPU oracle's (vertices bit-exact, same triangles in the same order) on closed
+shapes, random fields full of ambiguous cells and a 128^3 scene grid, with and without
+the pre-filter; the filter equals scipy.ndimage.gaussian_filter(mode="nearest") bit for
+bit; the reference's own upsampling checks (test_grid_upsampling.py) hold on the
+device."""
 
 import numpy as np
 import pytest
@@ -20,10 +22,10 @@ def _lattice(res):
     return np.meshgrid(ax, ax, ax, indexing="ij")
 
 
-def _check(vol, level, lo=(-1.0, -1.0, -1.0), hi=(1.0, 1.0, 1.0)):
+def _check(vol, level, lo=(-1.0, -1.0, -1.0), hi=(1.0, 1.0, 1.0), sigma=None):
     from voxneus_amd.mesh import extract_mesh_marching_cubes
-    v, f = extract_mesh_marching_cubes(vol, np.array(lo, np.float32), np.array(hi, np.float32), level)
-    rv, rf = R.extract_mesh_marching_cubes(vol, lo, hi, level)
+    v, f = extract_mesh_marching_cubes(vol, np.array(lo, np.float32), np.array(hi, np.float32), level, sigma)
+    rv, rf = R.extract_mesh_marching_cubes(vol, lo, hi, level, sigma)
     assert v.dtype == np.float32 and f.dtype == np.int32
     assert np.array_equal(v, rv) and np.array_equal(f, rf)
     return v, f
@@ -63,3 +65,28 @@ def test_reference_upsampling_checks():
     pts = torch.tensor([[0.5, 0.0, 0.0], [0.0, 0.5, 0.0], [0.0, 0.0, 0.5], [-0.5, 0.0, 0.0]], device=DEV)
     sdf = grid.query_sdf(pts)
     assert (sdf.abs() < 0.005).all()
+
+
+@pytest.mark.parametrize("shape", [(17, 23, 29), (64, 64, 64), (1, 9, 40), (2, 30, 3)])
+@pytest.mark.parametrize("sigma", [0.1, 0.7, 1.0, 1.5, 2.3])
+def test_gaussian_filter_matches_scipy(shape, sigma):
+    from voxneus_amd.mesh import apply_gaussian_filter_to_grid
+    rng = np.random.default_rng(sum(shape) + int(sigma * 10))
+    vol = rng.normal(size=shape).astype(np.float32)
+    got = apply_gaussian_filter_to_grid(vol, sigma)
+    assert got.dtype == np.float32 and got.shape == vol.shape
+    assert np.array_equal(got, R.apply_gaussian_filter_to_grid(vol, sigma))
+    # a device tensor stays on the device
+    t = apply_gaussian_filter_to_grid(torch.from_numpy(vol).to(DEV), sigma)
+    assert t.is_cuda and np.array_equal(t.cpu().numpy(), got)
+
+
+def test_filtered_meshes_match_oracle():
+    X, Y, Z = _lattice(48)
+    rng = np.random.default_rng(9)
+    noisy = (np.sqrt(X * X + Y * Y + Z * Z) - 0.6 + 0.02 * rng.normal(size=X.shape)).astype(np.float32)
+    v, f = _check(noisy, 0.0, sigma=1.0)
+    r = np.linalg.norm(v, axis=1)
+    assert f.shape[0] > 1000 and abs(float(r.mean()) - 0.6) < 0.02
+    sc = scenes.make_scene("multi", 128, 17, seed=2)
+    _check(np.ascontiguousarray(sc.grid, np.float32), 0.0, sigma=1.5)

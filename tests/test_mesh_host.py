@@ -103,3 +103,53 @@ def test_library_table_equals_the_oracle():
         assert int(nt[c]) == len(tris)
         assert tri[c][:3 * nt[c]].reshape(-1, 3).tolist() == [list(t) for t in tris]
         assert (tri[c][3 * nt[c]:] == -1).all()
+
+
+@pytest.mark.parametrize("sigma", [0.1, 0.5, 1.0, 1.5, 2.3, 7.9])
+def test_gaussian_weights_are_scipys(sigma):
+    """The oracle's and the library's weights are scipy's own (its private kernel builder,
+    read here only as the checker)."""
+    from scipy.ndimage import _filters
+    from voxneus_amd.functional import gaussian_weights
+    w, r = R.gaussian_weights(sigma)
+    assert r == int(4.0 * sigma + 0.5)
+    ref = _filters._gaussian_kernel1d(sigma, 0, r)
+    assert np.array_equal(w, ref)
+    lw, lr = gaussian_weights(sigma)
+    assert lr == r and np.array_equal(lw, ref[::-1]) and np.array_equal(lw, lw[::-1])
+
+
+def _separable(vol, sigma):
+    """The device kernel's algorithm in numpy: per axis (x, y, z), a double-precision
+    correlation with edge replication, the centre term first and then the symmetric pairs
+    from the outermost inwards, rounded to float32 after each axis."""
+    w, r = R.gaussian_weights(sigma)
+    out = vol.astype(np.float32)
+    for ax in range(3):
+        a = np.moveaxis(out.astype(np.float64), ax, -1)
+        n = a.shape[-1]
+        pad = np.concatenate([np.repeat(a[..., :1], r, -1), a, np.repeat(a[..., -1:], r, -1)], -1)
+        acc = pad[..., r:r + n] * w[r]
+        for j in range(r, 0, -1):
+            acc = acc + (pad[..., r - j:r - j + n] + pad[..., r + j:r + j + n]) * w[r - j]
+        out = np.moveaxis(acc.astype(np.float32), -1, ax)
+    return out
+
+
+@pytest.mark.parametrize("sigma", [0.7, 1.0, 1.5, 2.3])
+def test_separable_double_accumulation_is_scipys_filter(sigma):
+    rng = np.random.default_rng(int(sigma * 10))
+    for shape in ((23, 19, 31), (1, 7, 12), (40, 3, 2)):
+        vol = rng.normal(size=shape).astype(np.float32)
+        assert np.array_equal(_separable(vol, sigma), R.apply_gaussian_filter_to_grid(vol, sigma))
+
+
+def test_gaussian_filter_closed_forms():
+    """Constants stay constant; away from the edges a linear field is unchanged (the
+    kernel is symmetric and sums to 1); at the edges replication bends it."""
+    c = np.full((12, 13, 14), 0.375, np.float32)
+    assert np.array_equal(R.apply_gaussian_filter_to_grid(c, 1.3), c)
+    X = np.broadcast_to(np.arange(30, dtype=np.float32)[:, None, None], (30, 8, 8)).copy()
+    out = R.apply_gaussian_filter_to_grid(X, 1.0)
+    assert np.abs(out[4:-4] - X[4:-4]).max() < 1e-5
+    assert out[0, 0, 0] > X[0, 0, 0]

@@ -23,7 +23,7 @@ _LAZY = {
     "FusedOptimizer": "training", "SuperResolutionSchedule": "training", "compute_analytical_gradient": "sdf_grid",
     "SparseGridAdam": "optim", "NetAdam": "optim", "create_optimizer": "optim",
     "upsample_sdf_grid": "sdf_grid", "get_resolution_for_step": "sdf_grid", "logistic_density": "rendering",
-    "extract_mesh_marching_cubes": "mesh", "marching_cubes": "mesh",
+    "extract_mesh_marching_cubes": "mesh", "marching_cubes": "mesh", "apply_gaussian_filter_to_grid": "mesh",
     "save_checkpoint": "checkpoint", "load_checkpoint": "checkpoint", "load_resident_scene": "checkpoint",
 }
 

@@ -120,6 +120,8 @@ _SIGS = {
     "vxn_mc_count": (c_i32, [c_vp, c_i32, c_i32, c_i32, c_f, c_vp, c_i64, c_vp, c_vp]),
     "vxn_mc_emit": (c_i32, [c_vp, c_i32, c_i32, c_i32, c_vp, c_f, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp]),
     "vxn_upsample_grid": (c_i32, [c_vp, c_i32, c_i32, c_i32, c_vp, c_i32, c_i32, c_i32, c_vp]),
+    "vxn_gaussian_filter_workspace_bytes": (c_i32, [c_i32, c_i32, c_i32, ctypes.POINTER(c_i64)]),
+    "vxn_gaussian_filter": (c_i32, [c_vp, c_i32, c_i32, c_i32, c_vp, c_i32, c_vp, c_vp, c_i64, c_vp]),
     "vxn_occupancy_update_workspace_bytes": (c_i32, [c_i64, c_vp]),
     "vxn_occupancy_update": (c_i32, [c_vp, c_vp, c_i32, c_vp, c_vp, c_i32, c_i32, c_i32, c_vp, c_f, c_f, c_f, c_i32,
                                      c_vp, c_vp, c_vp, c_i64, c_vp, c_i64, c_vp]),

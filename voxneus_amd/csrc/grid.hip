@@ -279,6 +279,36 @@ __global__ void __launch_bounds__(256) serve_query_kernel(const SceneDev* __rest
   }
 }
 
+// ------------------------------------------------------------------------------------
+// Separable Gaussian filter with edge replication (scipy.ndimage.gaussian_filter,
+// mode="nearest", as apply_gaussian_filter_to_grid calls it): one 1-D correlation per
+// axis, x then y then z, each accumulated in double and rounded to float, as scipy's
+// correlate1d does line by line.
+// ------------------------------------------------------------------------------------
+constexpr int kGaussMaxRadius = 32;
+
+struct GaussTaps {
+  double w[2 * kGaussMaxRadius + 1];  // correlation weights, centre at w[r]
+  int r;
+};
+
+__global__ void __launch_bounds__(256) gauss1d_kernel(const float* __restrict__ in, float* __restrict__ out, int X,
+                                                      int Y, int Z, int axis, GaussTaps k) {
+  const int64_t n = (int64_t)X * Y * Z;
+  const int64_t stride = axis == 0 ? (int64_t)Y * Z : (axis == 1 ? (int64_t)Z : 1);
+  const int len = axis == 0 ? X : (axis == 1 ? Y : Z);
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int p = (int)((i / stride) % len);
+    const float* c = in + i - (int64_t)p * stride;  // start of the line through i
+    double acc = (double)in[i] * k.w[k.r];
+    for (int j = k.r; j >= 1; --j) {
+      const int lo = max(p - j, 0), hi = min(p + j, len - 1);
+      acc = acc + ((double)c[(int64_t)lo * stride] + (double)c[(int64_t)hi * stride]) * k.w[k.r - j];
+    }
+    out[i] = (float)acc;
+  }
+}
+
 }  // namespace vxn
 
 using namespace vxn;
@@ -417,6 +447,41 @@ extern "C" int vxn_upsample_grid(const float* grid, int X, int Y, int Z, float* 
   VXN_REQUIRE((int64_t)NX * NY < ((int64_t)1 << 31), VXN_ERR_SHAPE, "output too large");
   hipLaunchKernelGGL(upsample_grid_kernel, dim3((unsigned)(NX * NY)), dim3(256), 0, (hipStream_t)stream, grid, out,
                      up_axis(X, NX), up_axis(Y, NY), up_axis(Z, NZ));
+  VXN_LAUNCH_CHECK();
+  return 0;
+}
+
+extern "C" int vxn_gaussian_filter_workspace_bytes(int X, int Y, int Z, int64_t* bytes) {
+  VXN_REQUIRE(bytes != nullptr, VXN_ERR_NULL, "bytes is null");
+  VXN_REQUIRE(X >= 1 && Y >= 1 && Z >= 1, VXN_ERR_SHAPE, "sizes must be >= 1");
+  *bytes = (int64_t)X * Y * Z * (int64_t)sizeof(float);
+  return 0;
+}
+
+extern "C" int vxn_gaussian_filter(const float* grid, int X, int Y, int Z, const double* weights, int radius,
+                                   float* out, void* workspace, int64_t workspace_bytes, void* stream) {
+  VXN_REQUIRE(grid && weights && out, VXN_ERR_NULL, "null argument");
+  VXN_REQUIRE(X >= 1 && Y >= 1 && Z >= 1, VXN_ERR_SHAPE, "sizes must be >= 1");
+  VXN_REQUIRE(radius >= 0 && radius <= kGaussMaxRadius, VXN_ERR_UNSUPPORTED, "radius must be in [0, %d] (got %d)",
+              kGaussMaxRadius, radius);
+  const int64_t n = (int64_t)X * Y * Z;
+  VXN_REQUIRE(workspace != nullptr && workspace_bytes >= n * 4, VXN_ERR_WORKSPACE,
+              "Gaussian filter workspace needs %lld bytes", (long long)(n * 4));
+  VXN_REQUIRE(!((const char*)out < (const char*)(grid + n) && (const char*)grid < (const char*)(out + n)),
+              VXN_ERR_VALUE, "out must not overlap grid");
+  GaussTaps k;
+  k.r = radius;
+  for (int j = 0; j <= 2 * radius; ++j) k.w[j] = weights[j];
+  for (int j = 0; j < radius; ++j)
+    VXN_REQUIRE(weights[j] == weights[2 * radius - j], VXN_ERR_VALUE, "the weights must be symmetric");
+  float* tmp = (float*)workspace;
+  hipStream_t s = (hipStream_t)stream;
+  const int nb = blocks_for(n, 256, 65536);
+  hipLaunchKernelGGL(gauss1d_kernel, dim3(nb), dim3(256), 0, s, grid, out, X, Y, Z, 0, k);
+  VXN_LAUNCH_CHECK();
+  hipLaunchKernelGGL(gauss1d_kernel, dim3(nb), dim3(256), 0, s, (const float*)out, tmp, X, Y, Z, 1, k);
+  VXN_LAUNCH_CHECK();
+  hipLaunchKernelGGL(gauss1d_kernel, dim3(nb), dim3(256), 0, s, (const float*)tmp, out, X, Y, Z, 2, k);
   VXN_LAUNCH_CHECK();
   return 0;
 }

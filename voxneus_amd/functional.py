@@ -10,6 +10,7 @@ import ctypes
 import math
 from typing import Dict, List, Optional, Sequence, Tuple
 
+import numpy as np
 import torch
 
 from . import _native as N
@@ -581,4 +582,33 @@ def upsample_grid(grid: torch.Tensor, new_shape) -> torch.Tensor:
     X, Y, Z = g3.shape
     N.check(N.lib().vxn_upsample_grid(N.dptr(g3, torch.float32, "grid"), X, Y, Z, N.dptr(out), NX, NY, NZ,
                                       _stream(g3)), "upsample_grid")
+    return out
+
+
+def gaussian_weights(sigma: float, truncate: float = 4.0):
+    """The correlation weights scipy.ndimage.gaussian_filter1d uses (order 0): radius
+    int(truncate * sigma + 0.5), exp(-0.5 / sigma^2 * x^2) normalised to sum 1, in
+    float64 -> (weights [2r+1] numpy float64, r)."""
+    sd = float(sigma)
+    r = int(truncate * sd + 0.5)
+    x = np.arange(-r, r + 1)
+    phi = np.exp(-0.5 / (sd * sd) * x ** 2)
+    return np.ascontiguousarray((phi / phi.sum())[::-1]), r
+
+
+def gaussian_filter_grid(grid: torch.Tensor, sigma: float, truncate: float = 4.0) -> torch.Tensor:
+    """Separable Gaussian smoothing of an [X,Y,Z] (or [1,1,X,Y,Z]) grid with edge
+    replication (mesh.py:17-31, scipy's mode="nearest"); returns a new [X,Y,Z] tensor."""
+    g3 = _grid3(grid).contiguous()
+    X, Y, Z = g3.shape
+    if float(sigma) <= 1e-15:  # scipy skips every axis
+        return g3.clone()
+    w, r = gaussian_weights(sigma, truncate)
+    out = torch.empty_like(g3)
+    nb = ctypes.c_int64(0)
+    N.check(N.lib().vxn_gaussian_filter_workspace_bytes(X, Y, Z, ctypes.byref(nb)), "gaussian_filter_workspace")
+    ws = torch.empty(int(nb.value), device=g3.device, dtype=torch.uint8)
+    N.check(N.lib().vxn_gaussian_filter(N.dptr(g3, torch.float32, "grid"), X, Y, Z,
+                                        w.ctypes.data_as(ctypes.c_void_p), r, N.dptr(out), ws.data_ptr(), ws.numel(),
+                                        _stream(g3)), "gaussian_filter")
     return out

@@ -2,7 +2,8 @@
 SDF grid by the gfx950 kernels (``vxn_mc_count`` / ``vxn_mc_emit``), in place of the
 reference's host scikit-image call.  One vertex per grid edge whose end values straddle
 the level, at the linear-interpolation crossing; triangles from a face-consistent case
-table (closed where the surface is, normals towards increasing values)."""
+table (closed where the surface is, normals towards increasing values).  The optional
+Gaussian pre-filter (mesh.py:17-31) runs on the device too (``vxn_gaussian_filter``)."""
 
 from __future__ import annotations
 
@@ -41,16 +42,29 @@ def marching_cubes(grid: torch.Tensor, box6, level: float = 0.0,
     return verts, faces
 
 
+def _device_grid(sdf_grid) -> torch.Tensor:
+    g = sdf_grid if torch.is_tensor(sdf_grid) else torch.from_numpy(np.ascontiguousarray(sdf_grid, np.float32))
+    if not g.is_cuda:
+        g = g.to("cuda")
+    return g.to(torch.float32)
+
+
+def apply_gaussian_filter_to_grid(sdf_grid, sigma: float = 1.0):
+    """mesh.py:17-31: the post-training 3-D Gaussian smoothing of the SDF volume
+    (scipy.ndimage.gaussian_filter with mode="nearest") on the device.  A numpy input
+    gives a numpy result, a tensor a device tensor."""
+    out = Fn.gaussian_filter_grid(_device_grid(sdf_grid), sigma)
+    return out if torch.is_tensor(sdf_grid) else out.cpu().numpy()
+
+
 def extract_mesh_marching_cubes(sdf_grid, xyz_min, xyz_max, level: float = 0.0,
                                 post_filter_sigma: Optional[float] = None) -> Tuple[np.ndarray, np.ndarray]:
     """mesh.py:34-77: world-space vertices (float32) and faces (int32) as numpy arrays.
     ``sdf_grid`` may be a numpy array (copied to the current HIP device) or a device
-    tensor.  The optional Gaussian pre-filter is not part of this path."""
+    tensor; ``post_filter_sigma`` > 0 smooths it first (mesh.py:56-57)."""
+    g = _device_grid(sdf_grid)
     if post_filter_sigma is not None and post_filter_sigma > 0:
-        raise NotImplementedError("the Gaussian pre-filter (mesh.py:17-31) is not implemented on the device")
-    g = sdf_grid if torch.is_tensor(sdf_grid) else torch.from_numpy(np.ascontiguousarray(sdf_grid, np.float32))
-    if not g.is_cuda:
-        g = g.to("cuda")
+        g = Fn.gaussian_filter_grid(g, post_filter_sigma)
     box6 = tuple(float(v) for v in list(np.asarray(xyz_min, np.float32)) + list(np.asarray(xyz_max, np.float32)))
     verts, faces = marching_cubes(g.to(torch.float32), box6, level)
     return verts.cpu().numpy(), faces.cpu().numpy()
