@@ -42,6 +42,12 @@ enum { VXN_MLP_FP32 = 0, VXN_MLP_BF16_MFMA = 1 };
 #define VXN_MLP_IN 51
 #define VXN_MLP_HIDDEN 64
 #define VXN_MLP_PARAMS (64 * 51 + 64 + 3 * 64 + 3) /* W0, b0, W1, b1 flattened in that order */
+/* View embedding of the radiance MLP (VoxNeuS view_embed_type, models/voxneus.py:78-82):
+ * degree-4 spherical harmonics (16 values, W0 [64,51]) or the NeRF positional encoding
+ * with 4 frequencies and the input (27 values, W0 [64,62], VXN_MLP_PARAMS_PE). */
+#define VXN_VIEW_SH 0
+#define VXN_VIEW_PE 1
+#define VXN_MLP_PARAMS_PE (64 * 62 + 64 + 3 * 64 + 3)
 
 /* Surface gradient of the dense grid (DenseSDFGrid.query_normals gradient_mode,
  * models/sdf_grid.py:122-160): the Eq. 7 interpolated central difference or the Eq. 4
@@ -185,9 +191,9 @@ typedef struct vxn_sampler_args {
 int vxn_upsample(const vxn_sampler_args* a, void* stream);
 
 /* ---- Fused per-ray render (forward, backward, training step) ----------------------
- * One ray per work-group: per-sample SDF + Eq. 7 normal, hash features, SH view
- * embedding, the 51->64->3 radiance MLP, NeuS Eq. 2 alpha, front-to-back transmittance
- * and compositing.  Replaces VoxNeuS.render passes 2-3 with forward_points and
+ * One ray per work-group: per-sample SDF + Eq. 7 normal, hash features, the SH (or
+ * positional-encoding) view embedding, the 51->64->3 (62->64->3) radiance MLP, NeuS Eq. 2
+ * alpha, front-to-back transmittance and compositing.  Serving scenes use the SH form.  Replaces VoxNeuS.render passes 2-3 with forward_points and
  * volume_render_from_samples (models/voxneus.py:115-167, 226-258;
  * rendering/neus_alpha.py:12-81; rendering/volume_render.py:12-153).                 */
 typedef struct vxn_render_args {
@@ -205,7 +211,8 @@ typedef struct vxn_render_args {
   int32_t n_levels, log2_T, n_features;
   int32_t level_res[VXN_MAX_LEVELS];
   float level_w[VXN_MAX_LEVELS];
-  const float* mlp; /* [VXN_MLP_PARAMS] float32: W0 [64,51], b0 [64], W1 [3,64], b1 [3] */
+  const float* mlp; /* [VXN_MLP_PARAMS] float32: W0 [64,51], b0 [64], W1 [3,64], b1 [3]; with
+                       view_embed = VXN_VIEW_PE [VXN_MLP_PARAMS_PE], W0 [64,62] */
   int32_t mlp_mode;
   float s;
   float sphere_radius;
@@ -235,7 +242,7 @@ typedef struct vxn_render_args {
   /* gradient outputs */
   float* grad_grid;  /* [X,Y,Z], accumulated */
   float* grad_table; /* [L,T,F] float32, accumulated */
-  float* grad_mlp;   /* [VXN_MLP_PARAMS], overwritten */
+  float* grad_mlp;   /* [VXN_MLP_PARAMS] (PE: [VXN_MLP_PARAMS_PE]), overwritten */
   float* grad_s;     /* [1] dL/ds, overwritten */
   double* loss;      /* [1] Huber loss (training step only), overwritten */
   uint32_t* vr_bitmask; /* nullable [ceil(X*Y*Z/32)]: bit set for all 8 corners of every sample */
@@ -255,6 +262,9 @@ typedef struct vxn_render_args {
      caller that keeps s on the device (s = clamp(exp(10 v)), models/variance.py:31) needs
      no host read */
   const float* s_dev;
+  /* VXN_VIEW_SH (0) or VXN_VIEW_PE: the view embedding the MLP's W0 columns 32.. expect;
+     grad_mlp then has VXN_MLP_PARAMS or VXN_MLP_PARAMS_PE entries */
+  int32_t view_embed;
 } vxn_render_args;
 
 /* Workspace of the backward and training entries (the forward needs none): per-sample

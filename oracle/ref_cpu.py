@@ -252,6 +252,27 @@ def sh_embed(dirs):
     return torch.cat(r, -1)
 
 
+def pe_embed(dirs):
+    """embedder.py:12-44 with num_freqs 4, include_input, log sampling (voxneus.py:81):
+    [d, sin(d f), cos(d f) for f in 2 ** linspace(0, 3, 4)], 27 outputs."""
+    freq_bands = 2.0 ** torch.linspace(0.0, 3.0, 4)
+    outputs = [dirs]
+    for freq in freq_bands:
+        outputs.append(torch.sin(dirs * freq))
+        outputs.append(torch.cos(dirs * freq))
+    return torch.cat(outputs, dim=-1)
+
+
+def view_embed(dirs, view_embed_type="sh"):
+    """voxneus.py:78-82: the model's view embedder."""
+    t = view_embed_type.lower()
+    if t == "sh":
+        return sh_embed(dirs)
+    if t == "pe":
+        return pe_embed(dirs)
+    raise ValueError(f"unknown view_embed_type {view_embed_type!r}")
+
+
 def radiance_mlp(hash_feats, view_embed, normals, W0, b0, W1, b1):
     """radiance_mlp.py:81-99 with the 2-layer net of :36-53."""
     x = torch.cat([hash_feats, view_embed, normals], -1)
@@ -266,15 +287,17 @@ class _RadianceMlpBF16(torch.autograd.Function):
     """The BF16 matrix-core form of the radiance MLP (this build's definition, not the
     reference's): bf16 operands and FP32 accumulation at the points the device kernel
     rounds.  Forward: the 35 per-sample inputs (hash features, normal) and W0's columns
-    for them are rounded to bf16; the SH part b0 + W0[:, 32:48] . sh stays FP32; relu(h)
-    and W1 are rounded to bf16.  Backward: the output gradient and dL/dh are rounded to
-    bf16; dW0's SH columns use bf16 SH values; db1 sums the unrounded output gradient."""
+    for them are rounded to bf16; the view-embedding part b0 + W0[:, 32:32+V] . view stays
+    FP32 (V = 16 SH or 27 positional-encoding values); relu(h) and W1 are rounded to bf16.
+    Backward: the output gradient and dL/dh are rounded to bf16; dW0's view columns use
+    bf16 view values; db1 sums the unrounded output gradient."""
 
     @staticmethod
     def forward(ctx, feats, sh, nrm, W0, b0, W1, b1):
+        V = sh.shape[-1]
         x = _bf16(torch.cat([feats, nrm], -1))
-        W0x = _bf16(torch.cat([W0[:, :32], W0[:, 48:51]], 1))
-        hid = (b0 + sh @ W0[:, 32:48].T) + x @ W0x.T
+        W0x = _bf16(torch.cat([W0[:, :32], W0[:, 32 + V:35 + V]], 1))
+        hid = (b0 + sh @ W0[:, 32:32 + V].T) + x @ W0x.T
         hb = _bf16(torch.relu(hid))
         W1b = _bf16(W1)
         rgb = torch.sigmoid(b1 + hb @ W1b.T)
@@ -289,11 +312,12 @@ class _RadianceMlpBF16(torch.autograd.Function):
         doutb = _bf16(dout)
         dh = _bf16((doutb @ W1b) * mask.reshape(-1, 64))
         dx = dh @ W0x
-        xf, shf = x.reshape(-1, 35), _bf16(sh.reshape(-1, 16))
-        dW0 = torch.zeros(64, 51, dtype=g.dtype)
+        V = sh.shape[-1]
+        xf, shf = x.reshape(-1, 35), _bf16(sh.reshape(-1, V))
+        dW0 = torch.zeros(64, 35 + V, dtype=g.dtype)
         dW0[:, :32] = dh.T @ xf[:, :32]
-        dW0[:, 32:48] = dh.T @ shf
-        dW0[:, 48:51] = dh.T @ xf[:, 32:35]
+        dW0[:, 32:32 + V] = dh.T @ shf
+        dW0[:, 32 + V:35 + V] = dh.T @ xf[:, 32:35]
         return (dx[:, :32].reshape(*shp, 32), None, dx[:, 32:35].reshape(*shp, 3), dW0, dh.sum(0), doutb.T @ hb.reshape(-1, 64),
                 dout.sum(0))
 
@@ -725,11 +749,12 @@ class RefModel:
     def __init__(self, grid, embeddings, W0, b0, W1, b1, variance=0.3, s_max=300.0,
                  xyz_min=(-1.0, -1.0, -1.0), xyz_max=(1.0, 1.0, 1.0), hash_base_resolution=16,
                  hash_max_resolution=2048.0, occupancy_mask=None, occupancy_res=128, dtype=torch.float32,
-                 mlp_form="fp32", gradient_mode="interpolated"):
+                 mlp_form="fp32", gradient_mode="interpolated", view_embed_type="sh"):
         t = lambda a: torch.as_tensor(a).to(dtype).clone()
         self.dtype = dtype
         self.mlp_form = mlp_form
         self.gradient_mode = gradient_mode
+        self.view_embed_type = view_embed_type
         self.grid = t(grid).reshape(1, 1, *grid.shape[-3:]).requires_grad_(True)
         self.embeddings = t(embeddings).requires_grad_(True)
         self.W0, self.b0 = t(W0).requires_grad_(True), t(b0).requires_grad_(True)
@@ -761,7 +786,7 @@ class RefModel:
         norm_xyz = (xyz - self.box.xyz_min) / (self.box.xyz_max - self.box.xyz_min + 1e-8)
         feats = hash_encode(torch.clamp(norm_xyz, min=0.0, max=1.0), self.embeddings, self.level_res, active_levels)
         mlp = radiance_mlp_bf16 if self.mlp_form == "bf16" else radiance_mlp
-        rgb = mlp(feats, sh_embed(view_dirs), normal, self.W0, self.b0, self.W1, self.b1)
+        rgb = mlp(feats, view_embed(view_dirs, self.view_embed_type), normal, self.W0, self.b0, self.W1, self.b1)
         return {"sdf": sdf, "grad": grad, "normal": normal, "corner_indices": corners, "weights": weights,
                 "rgb": rgb, "hash_feats": feats}
 

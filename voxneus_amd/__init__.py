@@ -14,6 +14,7 @@ _LAZY = {
     "interpolate_gradients": "sdf_grid", "world_to_grid_index": "sdf_grid",
     "RadianceHashGrid": "hash_grid", "PyTorchHashGrid": "hash_grid", "HashGridEncoder": "hash_grid",
     "RadianceMLP": "radiance", "SphericalHarmonicsEmbedder": "radiance", "SingleVarianceNetwork": "radiance",
+    "PositionalEmbedder": "radiance",
     "RaySampler": "rendering", "OccupancyGrid": "rendering", "compute_neus_alpha": "rendering",
     "composite_weights": "rendering", "volume_render_from_samples": "rendering",
     "compute_huber_color_loss": "rendering", "near_far_from_sphere": "training", "get_rays": "training",

@@ -33,6 +33,23 @@ def gradient_mode_code(mode: str) -> int:
         raise ValueError(f"Unknown gradient mode: {mode}") from None
 MAX_LEVELS = 32
 MLP_PARAMS = 64 * 51 + 64 + 3 * 64 + 3
+VXN_VIEW_SH = 0
+VXN_VIEW_PE = 1
+MLP_PARAMS_PE = 64 * 62 + 64 + 3 * 64 + 3
+
+
+def view_embed_code(view_embed_type: str) -> int:
+    """VoxNeuS view_embed_type ("sh" or "pe", models/voxneus.py:78-82) -> VXN_VIEW_*."""
+    t = str(view_embed_type).lower()
+    if t == "sh":
+        return VXN_VIEW_SH
+    if t == "pe":
+        return VXN_VIEW_PE
+    raise ValueError(f"unknown view_embed_type {view_embed_type!r} (expected 'sh' or 'pe')")
+
+
+def mlp_params(view_embed: int) -> int:
+    return MLP_PARAMS_PE if view_embed == VXN_VIEW_PE else MLP_PARAMS
 
 c_f = ctypes.c_float
 c_i32 = ctypes.c_int32
@@ -68,6 +85,7 @@ class RenderArgs(ctypes.Structure):
         ("grad_grid", c_vp), ("grad_table", c_vp), ("grad_mlp", c_vp), ("grad_s", c_vp), ("loss", c_vp),
         ("vr_bitmask", c_vp), ("workspace", c_vp), ("workspace_bytes", c_i64),
         ("kernel_events", c_vp), ("gradient_mode", c_i32), ("fork_event", c_vp), ("s_dev", c_vp),
+        ("view_embed", c_i32),
     ]
 
 

@@ -95,6 +95,9 @@ def load_resident_scene(source: Union[str, Dict[str, Any]], device, table_dtype:
     table = st[KEY_TABLE]
     L, T, F = table.shape
     W0, b0, W1, b1 = (st[k] for k in KEY_MLP)
+    if tuple(W0.shape) == (64, 62):
+        raise NotImplementedError("serving scenes use the SH view embedding (51 -> 64 -> 3 MLP); this "
+                                  "checkpoint's MLP takes the positional encoding")
     if tuple(W0.shape) != (64, 51) or tuple(W1.shape) != (3, 64):
         raise NotImplementedError("the fused render implements the 51 -> 64 -> 3 radiance MLP")
     mn = st.get("sdf_grid.xyz_min", torch.tensor([-1.0, -1.0, -1.0]))

@@ -32,6 +32,28 @@ constexpr int kNP = VXN_MLP_PARAMS;
 constexpr int kOffB0 = 64 * 51;
 constexpr int kOffW1 = kOffB0 + 64;
 constexpr int kOffB1 = kOffW1 + 3 * 64;
+
+// Parameter layout of the radiance MLP with a view embedding of NV values (16 SH, 27
+// positional encoding): W0 [64][32 + NV + 3] (hash features, view embedding, normal, the
+// order RadianceMLP concatenates them), b0 [64], W1 [3][64], b1 [3], flattened in turn.
+template <int NV>
+struct MlpLayout {
+  static constexpr int WS = 35 + NV;  // W0 row stride
+  static constexpr int B0 = 64 * WS;
+  static constexpr int W1 = B0 + 64;
+  static constexpr int B1 = W1 + 3 * 64;
+  static constexpr int NP = B1 + 3;
+};
+constexpr int kNPMax = MlpLayout<27>::NP;
+static_assert(MlpLayout<16>::NP == kNP, "SH layout");
+
+template <int NV>
+VXN_DEV void view_embedding(float x, float y, float z, float (&o)[NV]) {
+  if constexpr (NV == 16)
+    sh16(x, y, z, o);
+  else
+    pe27(x, y, z, o);
+}
 constexpr int kXS = 37;  // LDS row stride (floats): 32 hash features + 3 normal components
 constexpr int kCS = 17;  // LDS row stride (floats) for a 16-hidden-unit chunk
 constexpr int kStage = kRT * kXS + kRT * 3 + 2 * kRT * kCS;
@@ -138,24 +160,27 @@ VXN_DEV void block_sums(float (&v)[N], Red& red, int lane, int wid) {
 }
 
 // W0 restricted to the 35 per-sample inputs: k < 32 hash features, 32..34 the normal
-// (W0 columns 48..50), zero beyond.
+// (W0 columns 32 + NV .. 34 + NV), zero beyond.
+template <int NV>
 VXN_DEV float w0x(const float* W0, int h, int k) {
-  return k < 32 ? W0[h * 51 + k] : (k < 35 ? W0[h * 51 + 16 + k] : 0.0f);
+  constexpr int WS = MlpLayout<NV>::WS;
+  return k < 32 ? W0[h * WS + k] : (k < 35 ? W0[h * WS + NV + k] : 0.0f);
 }
 
 // Builds the BF16 weight fragments of the matrix-core MLP in LDS (all threads).
+template <int NV>
 VXN_DEV void build_mlp_fragments(const float* W0, const float* W1, u32x4_t* frag, int t) {
   for (int idx = t; idx < kFragWords; idx += kRT) {
     float v[8];
     if (idx < kFragDX) {  // A = W0x: row h = 32mt + j, K = k = 16ks + 8g + e
       const int f = idx >> 6, l = idx & 63, mt = f / 3, ks = f % 3;
-      for (int e = 0; e < 8; ++e) v[e] = w0x(W0, 32 * mt + (l & 31), 16 * ks + 8 * (l >> 5) + e);
+      for (int e = 0; e < 8; ++e) v[e] = w0x<NV>(W0, 32 * mt + (l & 31), 16 * ks + 8 * (l >> 5) + e);
     } else if (idx < kFragDXN) {  // A = W0x^T: row k = j, K = hidden kperm(mt, q, g, e)
       const int f = (idx - kFragDX) >> 6, l = (idx - kFragDX) & 63;
-      for (int e = 0; e < 8; ++e) v[e] = w0x(W0, kperm(f >> 1, f & 1, l >> 5, e), l & 31);
+      for (int e = 0; e < 8; ++e) v[e] = w0x<NV>(W0, kperm(f >> 1, f & 1, l >> 5, e), l & 31);
     } else if (idx < kFragL2) {  // rows k = 32 + j (j < 3) of W0x^T
       const int f = (idx - kFragDXN) / 6, r = (idx - kFragDXN) % 6;
-      for (int e = 0; e < 8; ++e) v[e] = w0x(W0, kperm(f >> 1, f & 1, r / 3, e), 32 + r % 3);
+      for (int e = 0; e < 8; ++e) v[e] = w0x<NV>(W0, kperm(f >> 1, f & 1, r / 3, e), 32 + r % 3);
     } else if (idx < kFragDH) {  // A = W1: row c = j (j < 3), K = hidden kperm(mt, q, g, e)
       const int f = (idx - kFragL2) / 6, r = (idx - kFragL2) % 6;
       for (int e = 0; e < 8; ++e) v[e] = W1[(r % 3) * 64 + kperm(f >> 1, f & 1, r / 3, e)];
@@ -173,31 +198,33 @@ VXN_DEV void build_mlp_fragments(const float* W0, const float* W1, u32x4_t* frag
   }
 }
 
-template <int MODE, int MLPM, int MULTI>
+template <int MODE, int MLPM, int MULTI, int NV>
 __global__ void __launch_bounds__(kRT, 2) render_kernel(RenderDev a) {
+  using ML = MlpLayout<NV>;
+  constexpr int VM = (NV + 7) / 8;  // view-embedding columns per thread in the FP32 weight gradient
   constexpr bool kBF = MLPM == kMlpBf16;
   constexpr bool kBFB = kBF && MODE != kFwd;
   // BF16 backward / training and BF16 serving: the hash features come from
   // hash_features_kernel (a.hfeat)
   constexpr bool kPre = kBF && (MODE != kFwd || MULTI);
   __shared__ Red red;
-  __shared__ __align__(16) float preh[64];  // b0 + W0[:, 32:48] . SH(d): the per-ray part of the hidden layer
+  __shared__ __align__(16) float preh[64];  // b0 + W0[:, 32:32+NV] . view(d): the per-ray part of the hidden layer
   __shared__ int nlive_s[2];  // samples per wavefront with a non-zero output gradient
   __shared__ float stage[(MODE == kFwd || kBF) ? 1 : kStage];
   // BF16 form: weight fragments, the ray's SH values, and the transposed bf16 tiles of the
   // parameter-gradient products: inputs xt[35][s], hidden (then dL/dhidden) hdt[64][s],
   // output gradients ot[3][s]; s = sample of the ray (0..127).
   __shared__ u32x4_t bfrag[kBF ? kFragWords : 1];
-  __shared__ float shl[16];
+  __shared__ float shl[32];
   __shared__ uint64_t live_s[2];
   __shared__ __align__(16) uint16_t xt[kBFB ? 35 * kTS : 8];
   __shared__ __align__(16) uint16_t hdt[kBFB ? 64 * kTS : 8];
   __shared__ __align__(16) uint16_t ot[kBFB ? 3 * kTS : 8];
   const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
   const float* W0 = a.mlp;
-  const float* b0 = a.mlp + kOffB0;
-  const float* W1 = a.mlp + kOffW1;
-  const float* b1 = a.mlp + kOffB1;
+  const float* b0 = a.mlp + ML::B0;
+  const float* W1 = a.mlp + ML::W1;
+  const float* b1 = a.mlp + ML::B1;
   const int64_t NS = a.R * (int64_t)a.S;
   // LDS staging for the parameter gradients: inputs Xs[128][kXS] (hash features and
   // normal), output grads Os[128][3], and per 16-hidden-unit chunk Hs/Ds[128][kCS].
@@ -207,14 +234,16 @@ __global__ void __launch_bounds__(kRT, 2) render_kernel(RenderDev a) {
   float* Ds = Hs + kRT * kCS;
   // Per-thread ownership of parameter-gradient outputs within each hidden chunk:
   // hidden j = 16*chunk + (t & 15); input columns {g, g+8, ...} of the 35 staged ones
-  // (g = t >> 4) and SH columns {g, g+8}; W1 (c, j) for t < 48; b1[c] for t < 3.
+  // (g = t >> 4) and view-embedding columns {g, g+8, ...}; W1 (c, j) for t < 48; b1[c]
+  // for t < 3.
   const int jl = t & 15, grp = t >> 4;
-  float accX[4][5], accSH[4][2], accW1[4], accB0[4];
+  float accX[4][5], accSH[4][VM], accW1[4], accB0[4];
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
 #pragma unroll
     for (int k = 0; k < 5; ++k) accX[c][k] = 0.0f;
-    accSH[c][0] = accSH[c][1] = 0.0f;
+#pragma unroll
+    for (int m = 0; m < VM; ++m) accSH[c][m] = 0.0f;
     accW1[c] = 0.0f;
     accB0[c] = 0.0f;
   }
@@ -226,7 +255,7 @@ __global__ void __launch_bounds__(kRT, 2) render_kernel(RenderDev a) {
   f32x16_t gw0[2] = {}, gw1 = {};
   float accb[3] = {0.0f, 0.0f, 0.0f};
   if constexpr (kBF && !MULTI) {
-    build_mlp_fragments(W0, W1, bfrag, t);
+    build_mlp_fragments<NV>(W0, W1, bfrag, t);
     __syncthreads();
   }
   int frag_scene = -1;  // serving batches: scene whose MLP the fragments hold
@@ -250,7 +279,7 @@ __global__ void __launch_bounds__(kRT, 2) render_kernel(RenderDev a) {
       if constexpr (kBF) {
         if (sx != frag_scene) {
           __syncthreads();
-          build_mlp_fragments(scn->mlp, scn->mlp + kOffW1, bfrag, t);
+          build_mlp_fragments<NV>(scn->mlp, scn->mlp + ML::W1, bfrag, t);
           frag_scene = sx;
         }
       }
@@ -260,27 +289,29 @@ __global__ void __launch_bounds__(kRT, 2) render_kernel(RenderDev a) {
     const float sv = MULTI ? scn->s : (a.s_dev ? *a.s_dev : a.s);
     const float sr = MULTI ? scn->sphere_r : a.sphere_r;
     const float* W0 = MULTI ? scn->mlp : a.mlp;
-    const float* b0 = W0 + kOffB0;
-    const float* W1 = W0 + kOffW1;
-    const float* b1 = W0 + kOffB1;
+    const float* b0 = W0 + ML::B0;
+    const float* W1 = W0 + ML::W1;
+    const float* b1 = W0 + ML::B1;
     const float ox = a.o[3 * ray], oy = a.o[3 * ray + 1], oz = a.o[3 * ray + 2];
     const float dx = a.d[3 * ray], dy = a.d[3 * ray + 1], dz = a.d[3 * ray + 2];
     const float mid = valid ? a.mid[si] : 0.0f;
     const float dist = valid ? a.dist[si] : 0.0f;
-    // per-ray SH embedding and its contribution to the hidden pre-activations
-    float sh[16];
-    sh16(dx, dy, dz, sh);
+    // per-ray view embedding and its contribution to the hidden pre-activations
+    float sh[NV];
+    view_embedding<NV>(dx, dy, dz, sh);
     if (t < 64) {
-      const float* wr = W0 + t * 51 + 32;
+      const float* wr = W0 + t * ML::WS + 32;
       float acc = b0[t];
 #pragma unroll
-      for (int i = 0; i < 16; ++i) acc = fmaf(wr[i], sh[i], acc);
+      for (int i = 0; i < NV; ++i) acc = fmaf(wr[i], sh[i], acc);
       preh[t] = acc;
     }
-    const float shA = sh[grp & 7], shB = sh[(grp & 7) + 8];
+    float shv[VM];  // this thread's view-embedding columns grp + 8m
+#pragma unroll
+    for (int m = 0; m < VM; ++m) shv[m] = (grp & 7) + 8 * m < NV ? sh[(grp & 7) + 8 * m] : 0.0f;
     if (kBF && t == 0) {
 #pragma unroll
-      for (int i = 0; i < 16; ++i) shl[i] = sh[i];
+      for (int i = 0; i < NV; ++i) shl[i] = sh[i];
     }
     // ---------------------------------------------------------------- sample forward
     const float px = ox + dx * mid, py = oy + dy * mid, pz = oz + dz * mid;
@@ -309,12 +340,12 @@ __global__ void __launch_bounds__(kRT, 2) render_kernel(RenderDev a) {
       // MLP forward, streaming the hidden units (one W0 row per iteration)
 #pragma unroll 2
       for (int j = 0; j < 64; ++j) {
-        const float* wr = W0 + j * 51;
+        const float* wr = W0 + j * ML::WS;
         float acc = preh[j];
 #pragma unroll
         for (int i = 0; i < 32; ++i) acc = fmaf(wr[i], x[i], acc);
 #pragma unroll
-        for (int k = 0; k < 3; ++k) acc = fmaf(wr[48 + k], x[32 + k], acc);
+        for (int k = 0; k < 3; ++k) acc = fmaf(wr[32 + NV + k], x[32 + k], acc);
         const float hj = fmaxf(acc, 0.0f);
         o3[0] = fmaf(W1[j], hj, o3[0]);
         o3[1] = fmaf(W1[64 + j], hj, o3[1]);
@@ -565,19 +596,19 @@ __global__ void __launch_bounds__(kRT, 2) render_kernel(RenderDev a) {
 #pragma unroll 2
         for (int jj = 0; jj < 16; ++jj) {
           const int j = 16 * cc + jj;
-          const float* wr = W0 + j * 51;
+          const float* wr = W0 + j * ML::WS;
           float acc = preh[j];
 #pragma unroll
           for (int i = 0; i < 32; ++i) acc = fmaf(wr[i], x[i], acc);
 #pragma unroll
-          for (int k = 0; k < 3; ++k) acc = fmaf(wr[48 + k], x[32 + k], acc);
+          for (int k = 0; k < 3; ++k) acc = fmaf(wr[32 + NV + k], x[32 + k], acc);
           const float hj = fmaxf(acc, 0.0f);
           const float dh = (W1[j] * dout[0] + W1[64 + j] * dout[1]) + W1[128 + j] * dout[2];
           const float dzj = hj > 0.0f ? dh : 0.0f;
 #pragma unroll
           for (int i = 0; i < 32; ++i) dxa[i] = fmaf(wr[i], dzj, dxa[i]);
 #pragma unroll
-          for (int k = 0; k < 3; ++k) dxa[32 + k] = fmaf(wr[48 + k], dzj, dxa[32 + k]);
+          for (int k = 0; k < 3; ++k) dxa[32 + k] = fmaf(wr[32 + NV + k], dzj, dxa[32 + k]);
           if (mlive) {
             Hs[slot * kCS + jj] = hj;
             Ds[slot * kCS + jj] = dzj;
@@ -600,8 +631,8 @@ __global__ void __launch_bounds__(kRT, 2) render_kernel(RenderDev a) {
       }
 #pragma unroll
       for (int k = 0; k < 5; ++k) accX[cc][k] += aX[k];
-      accSH[cc][0] = fmaf(shA, dsum, accSH[cc][0]);
-      accSH[cc][1] = fmaf(shB, dsum, accSH[cc][1]);
+#pragma unroll
+      for (int m = 0; m < VM; ++m) accSH[cc][m] = fmaf(shv[m], dsum, accSH[cc][m]);
       accB0[cc] += dsum;
       accW1[cc] += aW1;
       __syncthreads();
@@ -692,11 +723,12 @@ __global__ void __launch_bounds__(kRT, 2) render_kernel(RenderDev a) {
       for (int k = 0; k < 3; ++k)
         dn[k] += __uint_as_float(half_swap(__float_as_uint(dxn[0][k]), __float_as_uint(dxn[1][k]))[0]);
       __syncthreads();  // dL/dhidden staged
-      // [dW0x | db0 | dW0 SH] [32 h x 64] += dL/dhidden . [X | 1 | SH]; columns 32 + j of the
-      // second tile: j < 3 the normal, 3 the bias, 4..19 the SH inputs, the rest zero
+      // [dW0x | db0 | dW0 view] [32 h x 64] += dL/dhidden . [X | 1 | view]; columns 32 + j of
+      // the second tile: j < 3 the normal, 3 the bias, 4 .. 3 + NV the view embedding, the
+      // rest zero
       uint32_t bc = 0u;
       if (jq == 3) bc = pk_bf16(1.0f, 1.0f);
-      else if (jq >= 4 && jq < 20) bc = pk_bf16(shl[jq - 4], shl[jq - 4]);
+      else if (jq >= 4 && jq < 4 + NV) bc = pk_bf16(shl[jq - 4], shl[jq - 4]);
       const u32x4_t Bc = {bc, bc, bc, bc};
 #pragma unroll 2
       for (int ks = 0; ks < 8; ++ks) {
@@ -722,39 +754,41 @@ __global__ void __launch_bounds__(kRT, 2) render_kernel(RenderDev a) {
 
   if (MODE == kFwd) return;
   // ---------------------------------------------------------------- write partials
-  float* P = a.part_mlp + (int64_t)blockIdx.x * kNP;
+  float* P = a.part_mlp + (int64_t)blockIdx.x * ML::NP;
   if constexpr (kBF) {
 #pragma unroll
     for (int kt = 0; kt < 2; ++kt)
 #pragma unroll
       for (int v = 0; v < 16; ++v) {
         const int h = 32 * wid + crow(v, gq), k = 32 * kt + jq;
-        // k: 0..31 hash features, 32..34 normal (W0 columns 48..50), 35 bias, 36..51 SH (columns 32..47)
-        if (k < 32) P[h * 51 + k] = gw0[kt][v];
-        else if (k < 35) P[h * 51 + 16 + k] = gw0[kt][v];
-        else if (k == 35) P[kOffB0 + h] = gw0[kt][v];
-        else if (k < 52) P[h * 51 + k - 4] = gw0[kt][v];
+        // k: 0..31 hash features, 32..34 normal (W0 columns 32 + NV ..), 35 bias,
+        // 36 .. 35 + NV the view embedding (columns 32 .. 31 + NV)
+        if (k < 32) P[h * ML::WS + k] = gw0[kt][v];
+        else if (k < 35) P[h * ML::WS + NV + k] = gw0[kt][v];
+        else if (k == 35) P[ML::B0 + h] = gw0[kt][v];
+        else if (k < 36 + NV) P[h * ML::WS + k - 4] = gw0[kt][v];
       }
 #pragma unroll
     for (int v = 0; v < 16; ++v)
-      if (jq < 3) P[kOffW1 + jq * 64 + 32 * wid + crow(v, gq)] = gw1[v];
+      if (jq < 3) P[ML::W1 + jq * 64 + 32 * wid + crow(v, gq)] = gw1[v];
     block_sums<3>(accb, red, lane, wid);
-    if (t < 3) P[kOffB1 + t] = accb[t];
+    if (t < 3) P[ML::B1 + t] = accb[t];
   } else {
 #pragma unroll
   for (int cc = 0; cc < 4; ++cc) {
     const int j = 16 * cc + jl;
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
-      const int col = grp + 8 * k;  // 0..31 hash features, 32..34 normal -> W0 columns 48..50
-      if (col < 35) P[j * 51 + (col < 32 ? col : col + 16)] = accX[cc][k];
+      const int col = grp + 8 * k;  // 0..31 hash features, 32..34 normal -> W0 columns 32 + NV ..
+      if (col < 35) P[j * ML::WS + (col < 32 ? col : col + NV)] = accX[cc][k];
     }
-    P[j * 51 + 32 + grp] = accSH[cc][0];
-    P[j * 51 + 32 + grp + 8] = accSH[cc][1];
-    if (grp == 0) P[kOffB0 + j] = accB0[cc];
-    if (t < 48) P[kOffW1 + (t >> 4) * 64 + j] = accW1[cc];
+#pragma unroll
+    for (int m = 0; m < VM; ++m)
+      if (grp + 8 * m < NV) P[j * ML::WS + 32 + grp + 8 * m] = accSH[cc][m];
+    if (grp == 0) P[ML::B0 + j] = accB0[cc];
+    if (t < 48) P[ML::W1 + (t >> 4) * 64 + j] = accW1[cc];
   }
-  if (t < 3) P[kOffB1 + t] = accb1;
+  if (t < 3) P[ML::B1 + t] = accb1;
   }
   double dsw = wave_sum_d(ds_acc);
   if (lane == 0) red.d[wid] = dsw;
@@ -1320,23 +1354,23 @@ __global__ void __launch_bounds__(256) grid_scatter_kernel(RenderDev a) {
 constexpr int kRedRows = 64;
 
 __global__ void __launch_bounds__(256) render_reduce1_kernel(const float* __restrict__ part_mlp, int nb,
-                                                             float* __restrict__ rows) {
+                                                             float* __restrict__ rows, int np) {
   const int k = blockIdx.x * 256 + threadIdx.x;
   const int r = blockIdx.y;
-  if (k >= kNP) return;
+  if (k >= np) return;
   float acc = 0.0f;
-  for (int b = r; b < nb; b += kRedRows) acc += part_mlp[(int64_t)b * kNP + k];
-  rows[(int64_t)r * kNP + k] = acc;
+  for (int b = r; b < nb; b += kRedRows) acc += part_mlp[(int64_t)b * np + k];
+  rows[(int64_t)r * np + k] = acc;
 }
 
 __global__ void __launch_bounds__(256) render_reduce2_kernel(const float* __restrict__ rows,
                                                              const double* __restrict__ part_sc, int nb,
                                                              float* __restrict__ grad_mlp, float* __restrict__ grad_s,
-                                                             double* __restrict__ loss, float inv_n3) {
+                                                             double* __restrict__ loss, float inv_n3, int np) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < kNP) {
+  if (k < np) {
     double acc = 0.0;
-    for (int r = 0; r < kRedRows; ++r) acc += (double)rows[(int64_t)r * kNP + k];
+    for (int r = 0; r < kRedRows; ++r) acc += (double)rows[(int64_t)r * np + k];
     grad_mlp[k] = (float)acc;
   }
   if (blockIdx.x == gridDim.x - 1 && threadIdx.x < 64) {  // one wavefront: fixed-order strided sums
@@ -1406,13 +1440,13 @@ struct RenderWs {
   BinPlan bins;
 };
 
-static RenderWs render_ws(int64_t R, int S, int L, int log2_T, const int32_t* level_res) {
+static RenderWs render_ws(int64_t R, int S, int L, int log2_T, const int32_t* level_res, int np) {
   RenderWs w;
   const int64_t nb = render_blocks(R);
   const int64_t ns = R * (int64_t)S;
-  w.off_sc = align256(nb * kNP * (int64_t)sizeof(float));
+  w.off_sc = align256(nb * np * (int64_t)sizeof(float));
   w.off_rows = align256(w.off_sc + nb * 2 * (int64_t)sizeof(double));
-  w.off_feat = align256(w.off_rows + (int64_t)kRedRows * kNP * (int64_t)sizeof(float));
+  w.off_feat = align256(w.off_rows + (int64_t)kRedRows * np * (int64_t)sizeof(float));
   w.off_geo = align256(w.off_feat + 32 * ns * (int64_t)sizeof(float));
   w.off_hf = align256(w.off_geo + kGeo * ns * (int64_t)sizeof(float));
   w.off_bin = align256(w.off_hf + 16 * ns * (int64_t)sizeof(uint32_t));
@@ -1430,7 +1464,8 @@ static RenderWs render_ws(int64_t R, int S, int L, int log2_T, const int32_t* le
 }
 
 static RenderWs render_ws(const vxn_render_args& A) {
-  return render_ws(A.n_rays, A.n_samples, A.n_levels, A.log2_T, A.level_res);
+  return render_ws(A.n_rays, A.n_samples, A.n_levels, A.log2_T, A.level_res,
+                   A.view_embed == VXN_VIEW_PE ? MlpLayout<27>::NP : MlpLayout<16>::NP);
 }
 
 static int build_render_dev(const vxn_render_args& A, RenderDev& a, int mode) {
@@ -1440,9 +1475,11 @@ static int build_render_dev(const vxn_render_args& A, RenderDev& a, int mode) {
   VXN_REQUIRE(A.n_samples >= 1 && A.n_samples <= kRT, VXN_ERR_UNSUPPORTED, "n_samples must be in [1, %d] (got %d)",
               kRT, A.n_samples);
   VXN_REQUIRE(A.n_levels == 16 && A.n_features == 2, VXN_ERR_UNSUPPORTED,
-              "the fused render implements the 16-level x 2-feature hash feeding the 51-input MLP");
+              "the fused render implements the 16-level x 2-feature hash feeding the radiance MLP");
   VXN_REQUIRE(A.mlp_mode == VXN_MLP_FP32 || A.mlp_mode == VXN_MLP_BF16_MFMA, VXN_ERR_VALUE, "unknown mlp_mode %d",
               A.mlp_mode);
+  VXN_REQUIRE(A.view_embed == VXN_VIEW_SH || A.view_embed == VXN_VIEW_PE, VXN_ERR_VALUE, "unknown view_embed %d",
+              A.view_embed);
   VXN_REQUIRE(A.rays_o && A.rays_d && A.mid_z && A.dists && A.table && A.mlp, VXN_ERR_NULL, "null input");
   VXN_REQUIRE(A.n_rays * (int64_t)A.n_samples < ((int64_t)1 << 40), VXN_ERR_SHAPE, "too many samples");
   HashDesc h;
@@ -1520,8 +1557,8 @@ static int dedup_levels(const HashDesc& h) {
   return n;
 }
 
-template <int MODE>
-static int launch_render(const vxn_render_args* p, void* stream) {
+template <int MODE, int NV>
+static int launch_render_nv(const vxn_render_args* p, void* stream) {
   VXN_REQUIRE(p != nullptr, VXN_ERR_NULL, "args is null");
   RenderDev a;
   if (int e = build_render_dev(*p, a, MODE)) return e;
@@ -1540,9 +1577,9 @@ static int launch_render(const vxn_render_args* p, void* stream) {
   if (ev) VXN_HIP(hipEventRecord(ev[1], s));
   if (p->fork_event) VXN_HIP(hipEventRecord((hipEvent_t)p->fork_event, s));
   if (p->mlp_mode == VXN_MLP_BF16_MFMA)
-    hipLaunchKernelGGL((render_kernel<MODE, kMlpBf16, 0>), dim3(nb), dim3(kRT), 0, s, a);
+    hipLaunchKernelGGL((render_kernel<MODE, kMlpBf16, 0, NV>), dim3(nb), dim3(kRT), 0, s, a);
   else
-    hipLaunchKernelGGL((render_kernel<MODE, kMlpF32, 0>), dim3(nb), dim3(kRT), 0, s, a);
+    hipLaunchKernelGGL((render_kernel<MODE, kMlpF32, 0, NV>), dim3(nb), dim3(kRT), 0, s, a);
   VXN_LAUNCH_CHECK();
   if (ev) VXN_HIP(hipEventRecord(ev[2], s));
   if (MODE != kFwd) {
@@ -1585,15 +1622,22 @@ static int launch_render(const vxn_render_args* p, void* stream) {
     VXN_LAUNCH_CHECK();
     if (ev) VXN_HIP(hipEventRecord(ev[4], s));
     float* rows = (float*)((char*)p->workspace + w.off_rows);
-    hipLaunchKernelGGL(render_reduce1_kernel, dim3((kNP + 255) / 256, kRedRows), dim3(256), 0, s, a.part_mlp, nb,
-                       rows);
+    constexpr int np = MlpLayout<NV>::NP;
+    hipLaunchKernelGGL(render_reduce1_kernel, dim3((np + 255) / 256, kRedRows), dim3(256), 0, s, a.part_mlp, nb,
+                       rows, np);
     VXN_LAUNCH_CHECK();
-    hipLaunchKernelGGL(render_reduce2_kernel, dim3((kNP + 1 + 255) / 256), dim3(256), 0, s, rows, a.part_sc, nb,
-                       p->grad_mlp, p->grad_s, MODE == kTrain ? p->loss : nullptr, a.inv_n3);
+    hipLaunchKernelGGL(render_reduce2_kernel, dim3((np + 1 + 255) / 256), dim3(256), 0, s, rows, a.part_sc, nb,
+                       p->grad_mlp, p->grad_s, MODE == kTrain ? p->loss : nullptr, a.inv_n3, np);
     VXN_LAUNCH_CHECK();
     if (ev) VXN_HIP(hipEventRecord(ev[5], s));
   }
   return 0;
+}
+
+template <int MODE>
+static int launch_render(const vxn_render_args* p, void* stream) {
+  VXN_REQUIRE(p != nullptr, VXN_ERR_NULL, "args is null");
+  return p->view_embed == VXN_VIEW_PE ? launch_render_nv<MODE, 27>(p, stream) : launch_render_nv<MODE, 16>(p, stream);
 }
 
 }  // namespace vxn
@@ -1719,9 +1763,9 @@ extern "C" int vxn_serve_render(const vxn_serve_render_args* p, void* stream) {
     VXN_LAUNCH_CHECK();
   }
   if (bf)
-    hipLaunchKernelGGL((render_kernel<kFwd, kMlpBf16, 1>), dim3(nb), dim3(kRT), 0, s, a);
+    hipLaunchKernelGGL((render_kernel<kFwd, kMlpBf16, 1, 16>), dim3(nb), dim3(kRT), 0, s, a);
   else
-    hipLaunchKernelGGL((render_kernel<kFwd, kMlpF32, 1>), dim3(nb), dim3(kRT), 0, s, a);
+    hipLaunchKernelGGL((render_kernel<kFwd, kMlpF32, 1, 16>), dim3(nb), dim3(kRT), 0, s, a);
   VXN_LAUNCH_CHECK();
   return 0;
 }

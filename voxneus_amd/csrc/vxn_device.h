@@ -362,6 +362,25 @@ VXN_DEV HashCell hash_cell(float x0, float x1, float x2, float res, uint32_t mas
   return h;
 }
 
+// NeRF positional encoding of a view direction (embedder.py:12-44 with num_freqs 4,
+// include_input, log-spaced bands 1, 2, 4, 8): [d, sin(d), cos(d), sin(2d), cos(2d), ...,
+// sin(8d), cos(8d)], each a 3-vector: 27 values.  The band products are exact.
+VXN_DEV void pe27(float x, float y, float z, float o[27]) {
+  o[0] = x;
+  o[1] = y;
+  o[2] = z;
+#pragma unroll
+  for (int f = 0; f < 4; ++f) {
+    const float b = (float)(1 << f);
+    o[3 + 6 * f] = sinf(x * b);
+    o[4 + 6 * f] = sinf(y * b);
+    o[5 + 6 * f] = sinf(z * b);
+    o[6 + 6 * f] = cosf(x * b);
+    o[7 + 6 * f] = cosf(y * b);
+    o[8 + 6 * f] = cosf(z * b);
+  }
+}
+
 // Real spherical harmonics, degree 0-3 (embedder.py:55-87), same operation order.
 VXN_DEV void sh16(float x, float y, float z, float o[16]) {
   o[0] = 0.28209479177387814f;

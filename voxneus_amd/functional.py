@@ -313,7 +313,7 @@ class RenderConfig:
 
     def __init__(self, box, level_res, level_w=None, sphere_radius=1.0, cos_anneal_ratio=0.0,
                  use_iter_cos=False, mlp_mode=N.VXN_MLP_FP32, table_grad_out=None,
-                 gradient_mode=N.VXN_GRADIENT_INTERPOLATED):
+                 gradient_mode=N.VXN_GRADIENT_INTERPOLATED, view_embed=N.VXN_VIEW_SH):
         # table_grad_out: optional FP32 [L,T,F] tensor that the backward accumulates the
         # hash-table gradient into (for bf16 tables with an FP32 master copy); the table
         # input then receives no gradient through autograd
@@ -326,6 +326,7 @@ class RenderConfig:
         self.use_iter_cos = bool(use_iter_cos)
         self.mlp_mode = mlp_mode
         self.gradient_mode = int(gradient_mode)
+        self.view_embed = int(view_embed)  # VXN_VIEW_SH or VXN_VIEW_PE
 
 
 def fill_render_args(a: "N.RenderArgs", cfg: RenderConfig, grid3, table, mlp_flat, s: float, rays_o, rays_d,
@@ -343,8 +344,12 @@ def fill_render_args(a: "N.RenderArgs", cfg: RenderConfig, grid3, table, mlp_fla
     for i, r in enumerate(cfg.level_res):
         a.level_res[i] = int(r)
         a.level_w[i] = 1.0 if cfg.level_w is None else float(cfg.level_w[i])
+    if mlp_flat.numel() != N.mlp_params(cfg.view_embed):
+        raise ValueError(f"the MLP has {mlp_flat.numel()} parameters; the view embedding needs "
+                         f"{N.mlp_params(cfg.view_embed)}")
     a.mlp = N.dptr(mlp_flat, torch.float32, "mlp")
     a.mlp_mode = cfg.mlp_mode
+    a.view_embed = cfg.view_embed
     a.s = float(s)
     a.sphere_radius = cfg.sphere_radius
     a.cos_anneal_ratio = cfg.cos_anneal_ratio
@@ -412,7 +417,7 @@ class RenderFunction(torch.autograd.Function):
             gt = torch.zeros(tb.shape, device=dev, dtype=torch.float32)
         elif gt.shape != tb.shape or gt.dtype != torch.float32 or not gt.is_contiguous():
             raise ValueError("table_grad_out must be a contiguous float32 tensor of the table's shape")
-        gm = torch.empty(N.MLP_PARAMS, device=dev, dtype=torch.float32)
+        gm = torch.empty(mf.numel(), device=dev, dtype=torch.float32)
         gs = torch.empty(1, device=dev, dtype=torch.float32)
         a.grad_grid, a.grad_table, a.grad_mlp, a.grad_s = N.dptr(gg), N.dptr(gt), N.dptr(gm), N.dptr(gs)
         ws = render_workspace(a, dev)

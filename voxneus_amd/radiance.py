@@ -1,10 +1,11 @@
-"""Appearance modules with the reference's parameter layout: SH view embedding
-(models/embedder.py:47-87), the 2-layer radiance MLP (models/radiance_mlp.py:11-99) and
-the single variance parameter (models/variance.py:11-46).
+"""Appearance modules with the reference's parameter layout: the SH and positional-
+encoding view embeddings (models/embedder.py:12-87), the 2-layer radiance MLP
+(models/radiance_mlp.py:11-99) and the single variance parameter (models/variance.py:11-46).
 
 On the fused path these are evaluated inside the gfx950 render kernel; the modules hold
-the parameters (``net.0.weight`` [64,51], ``net.0.bias``, ``net.2.weight`` [3,64],
-``net.2.bias``, ``variance``) under the reference's state-dict names.
+the parameters (``net.0.weight`` [64,51] with the SH embedding or [64,62] with the
+positional encoding, ``net.0.bias``, ``net.2.weight`` [3,64], ``net.2.bias``,
+``variance``) under the reference's state-dict names.
 """
 
 from __future__ import annotations
@@ -41,22 +42,48 @@ class SphericalHarmonicsEmbedder(nn.Module):
                           -C[5] * x * (x * x - 3.0 * y * y)], -1)
 
 
+class PositionalEmbedder(nn.Module):
+    """gamma(x) = [x, sin(x), cos(x), sin(2x), cos(2x), ...] (embedder.py:12-44); the fused
+    render evaluates it per ray for num_freqs 4 with the input (27 outputs)."""
+
+    def __init__(self, input_dims: int = 3, num_freqs: int = 4, include_input: bool = True,
+                 log_sampling: bool = True):
+        super().__init__()
+        if (input_dims, num_freqs, include_input, log_sampling) != (3, 4, True, True):
+            raise NotImplementedError("the fused path implements 4 log-spaced frequencies with the input "
+                                      "(27 outputs)")
+        self.input_dims, self.num_freqs, self.include_input = input_dims, num_freqs, include_input
+        self.register_buffer("freq_bands", 2.0 ** torch.linspace(0.0, num_freqs - 1, num_freqs))
+        self.out_dim = input_dims + input_dims * num_freqs * 2
+
+    def forward(self, x: torch.Tensor) -> torch.Tensor:
+        outputs = [x]
+        for freq in self.freq_bands:
+            outputs.append(torch.sin(x * freq))
+            outputs.append(torch.cos(x * freq))
+        return torch.cat(outputs, dim=-1)
+
+
 class RadianceMLP(nn.Module):
-    """sigmoid(W1 relu(W0 [hash32, sh16, normal3] + b0) + b1), kaiming-uniform (ReLU) init."""
+    """sigmoid(W1 relu(W0 [hash32, view, normal3] + b0) + b1), kaiming-uniform (ReLU) init;
+    view = the 16 SH values or the 27 positional-encoding values."""
 
     def __init__(self, hash_feature_dim: int = 32, view_embed_dim: int = 16, normal_dim: int = 3,
                  geom_feature_dim: int = 0, hidden_dim: int = 64, num_layers: int = 2, weight_norm: bool = False,
                  entangled: bool = False):
         super().__init__()
-        if (hash_feature_dim, view_embed_dim, normal_dim, geom_feature_dim, hidden_dim, num_layers) != (
-                32, 16, 3, 0, 64, 2) or weight_norm or entangled:
-            raise NotImplementedError("the fused gfx950 MLP is the reference default 51 -> 64 -> 3 "
-                                      "(no geometry features, no weight norm, not entangled)")
+        if (hash_feature_dim, normal_dim, geom_feature_dim, hidden_dim, num_layers) != (
+                32, 3, 0, 64, 2) or view_embed_dim not in (16, 27) or weight_norm or entangled:
+            raise NotImplementedError("the fused gfx950 MLP is the reference default 51 -> 64 -> 3 (SH view "
+                                      "embedding) or 62 -> 64 -> 3 (positional encoding), with no geometry "
+                                      "features, no weight norm, not entangled")
         self.entangled = False
         self.geom_feature_dim = 0
         self.num_layers = 2
         self.weight_norm = False
-        self.net = nn.Sequential(nn.Linear(51, 64), nn.ReLU(inplace=True), nn.Linear(64, 3), nn.Sigmoid())
+        self.view_embed_dim = view_embed_dim
+        self.in_dim = 35 + view_embed_dim
+        self.net = nn.Sequential(nn.Linear(self.in_dim, 64), nn.ReLU(inplace=True), nn.Linear(64, 3), nn.Sigmoid())
         self.reset_parameters()
 
     def reset_parameters(self) -> None:
@@ -71,8 +98,9 @@ class RadianceMLP(nn.Module):
                           self.net[2].bias])
 
     def unflatten_grad(self, g: torch.Tensor) -> None:
-        """Write a flat [3523] gradient into the four parameters' .grad (accumulating)."""
-        sizes = (64 * 51, 64, 3 * 64, 3)
+        """Write a flat gradient (the ``flat()`` layout) into the four parameters' .grad
+        (accumulating)."""
+        sizes = (64 * self.in_dim, 64, 3 * 64, 3)
         params = (self.net[0].weight, self.net[0].bias, self.net[2].weight, self.net[2].bias)
         off = 0
         for p, n in zip(params, sizes):

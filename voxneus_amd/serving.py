@@ -52,6 +52,9 @@ class ResidentScene:
             raise ValueError("table must be a [16,T,2] HIP tensor")
         self.grid = grid.contiguous()
         self.table = table.contiguous()
+        if mlp.numel() != N.MLP_PARAMS:
+            raise ValueError(f"a serving scene's MLP is the SH-embedding 51 -> 64 -> 3 net ({N.MLP_PARAMS} "
+                             f"parameters; got {mlp.numel()})")
         self.mlp = mlp.contiguous().to(torch.float32)
         self.level_res = [int(r) for r in level_res]
         self.level_w = None if level_w is None else [float(w) for w in level_w]

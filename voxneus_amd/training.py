@@ -83,6 +83,7 @@ class StepConfig:
     mlp_mode: int = N.VXN_MLP_FP32
     active_levels: Optional[float] = None
     gradient_mode: str = "interpolated"  # or "analytical" (Eq. 4), as VoxNeuS(gradient_mode=...)
+    view_embed_type: str = "sh"  # or "pe" (positional encoding; the MLP's W0 is [64, 62])
     # mark V_R in the sampler and run the regularisers on a second stream, beside the
     # hash-feature gather, the render and the scatters
     concurrent_regularisers: bool = True
@@ -97,7 +98,8 @@ class FusedTrainStep:
     """Owns the device workspaces of one scene's training step.
 
     Parameters are plain device tensors: ``grid`` [X,Y,Z] f32, ``table`` [L,T,2] f32 or
-    bf16, ``mlp`` [VXN_MLP_PARAMS] f32 (W0, b0, W1, b1 flattened), ``variance`` scalar.
+    bf16, ``mlp`` [VXN_MLP_PARAMS] f32 (W0, b0, W1, b1 flattened; [VXN_MLP_PARAMS_PE] with
+    the positional-encoding view embedding), ``variance`` scalar.
     Gradients land in ``grad_grid`` [X,Y,Z], ``grad_table`` [L,T,2] f32, ``grad_mlp``,
     ``grad_v`` (after amplification); ``loss`` / ``reg_losses`` are FP64 device scalars.
     """
@@ -121,7 +123,10 @@ class FusedTrainStep:
         X, Y, Z = grid.shape
         self.grad_grid = torch.zeros_like(grid)
         self.grad_table = torch.zeros(table.shape, device=dev, dtype=torch.float32)
-        self.grad_mlp = torch.zeros(N.MLP_PARAMS, device=dev, dtype=torch.float32)
+        if mlp.numel() != N.mlp_params(N.view_embed_code(cfg.view_embed_type)):
+            raise ValueError(f"mlp has {mlp.numel()} parameters; view_embed_type {cfg.view_embed_type!r} needs "
+                             f"{N.mlp_params(N.view_embed_code(cfg.view_embed_type))}")
+        self.grad_mlp = torch.zeros(mlp.numel(), device=dev, dtype=torch.float32)
         self.grad_s = torch.zeros(1, device=dev, dtype=torch.float32)
         self.grad_v = torch.zeros((), device=dev, dtype=torch.float32)
         self.loss = torch.zeros(1, device=dev, dtype=torch.float64)
@@ -160,7 +165,8 @@ class FusedTrainStep:
         a = self._args
         if a is None or self._args_key != key:
             rc = Fn.RenderConfig(self.box, self.level_res, self.level_w, cfg.sphere_radius, mlp_mode=cfg.mlp_mode,
-                                 gradient_mode=N.gradient_mode_code(cfg.gradient_mode))
+                                 gradient_mode=N.gradient_mode_code(cfg.gradient_mode),
+                                 view_embed=N.view_embed_code(cfg.view_embed_type))
             a = N.RenderArgs()
             Fn.fill_render_args(a, rc, self.grid, self.table, self.mlp, 0.0, rays_o, rays_d, mid, dists)
             a.s_dev = N.dptr(self.s_value)

@@ -16,7 +16,7 @@ import torch.nn as nn
 from . import _native as N
 from . import functional as Fn
 from .hash_grid import RadianceHashGrid, level_weights
-from .radiance import RadianceMLP, SingleVarianceNetwork, SphericalHarmonicsEmbedder
+from .radiance import PositionalEmbedder, RadianceMLP, SingleVarianceNetwork, SphericalHarmonicsEmbedder
 from .rendering import OccupancyGrid, RaySampler, apply_background
 from .sdf_grid import DenseSDFGrid
 
@@ -31,8 +31,8 @@ class VoxNeuS(nn.Module):
                  hash_dtype: torch.dtype = torch.float32, mlp_mode: int = N.VXN_MLP_FP32):
         super().__init__()
         N.gradient_mode_code(gradient_mode)  # ValueError on an unknown mode (sdf_grid.py:159-160)
-        if view_embed_type.lower() != "sh":
-            raise NotImplementedError("the fused path implements the SH view embedding")
+        self.view_embed = N.view_embed_code(view_embed_type)  # ValueError on an unknown type
+        self.view_embed_type = view_embed_type.lower()
         self.gradient_mode = gradient_mode
         self.entangled = entangled
         self.geom_feature_dim = geom_feature_dim
@@ -42,9 +42,11 @@ class VoxNeuS(nn.Module):
         self.occupancy_grid = OccupancyGrid(occupancy_res, xyz_min, xyz_max)
         self.radiance_hash = RadianceHashGrid(hash_n_levels, hash_n_features, hash_log2_size, hash_base_resolution,
                                               hash_max_resolution, dtype=hash_dtype)
-        self.view_embedder = SphericalHarmonicsEmbedder(4)
-        self.radiance_mlp = RadianceMLP(self.radiance_hash.out_dim, 16, 3, geom_feature_dim, mlp_hidden_dim,
-                                        entangled=entangled)
+        # voxneus.py:78-82: degree-4 SH (16 values) or the positional encoding (27)
+        self.view_embedder = (SphericalHarmonicsEmbedder(4) if self.view_embed == N.VXN_VIEW_SH
+                              else PositionalEmbedder(num_freqs=4, include_input=True))
+        self.radiance_mlp = RadianceMLP(self.radiance_hash.out_dim, self.view_embedder.out_dim, 3, geom_feature_dim,
+                                        mlp_hidden_dim, entangled=entangled)
 
     @property
     def s(self) -> torch.Tensor:
@@ -73,7 +75,7 @@ class VoxNeuS(nn.Module):
         enc = self.radiance_hash.encoder
         return Fn.RenderConfig(self.sdf_grid.box6, enc.level_res, level_weights(enc.n_levels, active_levels),
                                sphere_radius, cos_anneal_ratio, use_iter_cos, self.mlp_mode,
-                               gradient_mode=N.gradient_mode_code(self.gradient_mode))
+                               gradient_mode=N.gradient_mode_code(self.gradient_mode), view_embed=self.view_embed)
 
     def render(self, rays_o, rays_d, sampler: RaySampler, near, far, background_rgb=None, cos_anneal_ratio=0.0,
                use_iter_cos_annealing=False, active_levels=None, up_sample_steps=4, sphere_radius=1.0,
