@@ -48,6 +48,9 @@ enum { VXN_MLP_FP32 = 0, VXN_MLP_BF16_MFMA = 1 };
 #define VXN_VIEW_SH 0
 #define VXN_VIEW_PE 1
 #define VXN_MLP_PARAMS_PE (64 * 62 + 64 + 3 * 64 + 3)
+/* Colour loss of the training step (trainer.py:213-236, loss_type). */
+#define VXN_LOSS_HUBER 0
+#define VXN_LOSS_MSE 1
 
 /* Surface gradient of the dense grid (DenseSDFGrid.query_normals gradient_mode,
  * models/sdf_grid.py:122-160): the Eq. 7 interpolated central difference or the Eq. 4
@@ -265,6 +268,16 @@ typedef struct vxn_render_args {
   /* VXN_VIEW_SH (0) or VXN_VIEW_PE: the view embedding the MLP's W0 columns 32.. expect;
      grad_mlp then has VXN_MLP_PARAMS or VXN_MLP_PARAMS_PE entries */
   int32_t view_embed;
+  /* training step only (trainer.py:213-245): the colour loss VXN_LOSS_HUBER (delta
+     huber_delta) or VXN_LOSS_MSE, a mean over the rays' 3 channels; with target_mask
+     (nullable device [R], the foreground mask) the mask-weighted sum over 3 * sum(mask)
+     (the plain mean when the mask sums to 0), plus mask_weight * the mean binary cross
+     entropy of clamp(opacity, 1e-6, 1 - 1e-6) against the mask.  loss receives the total;
+     loss_parts (nullable device [2]) the colour loss and the mask BCE. */
+  int32_t loss_type;
+  const float* target_mask;
+  float mask_weight;
+  double* loss_parts;
 } vxn_render_args;
 
 /* Workspace of the backward and training entries (the forward needs none): per-sample

@@ -540,6 +540,37 @@ def huber(pred, target, delta=0.1):
     return torch.mean(torch.where(diff < delta, 0.5 * (diff ** 2) / delta, diff - 0.5 * delta))
 
 
+def color_loss(pred_rgb, target_rgb, loss_type="Huber", use_mask=False, target_mask=None, pred_opacity=None,
+               mask_weight=0.1):
+    """trainer.py:213-245: (loss_total, loss_rgb, loss_mask or None).  Huber (delta 0.1) or
+    MSE; with use_mask and a mask, the foreground-weighted colour loss (plain mean when the
+    mask sums to zero) plus mask_weight * BCE(clamp(opacity, 1e-6, 1 - 1e-6), mask)."""
+    if use_mask and target_mask is not None:
+        if loss_type.lower() == "huber":
+            diff = torch.abs(pred_rgb - target_rgb)
+            is_small = diff < 0.1
+            per_ray_loss = torch.where(is_small, 0.5 * (diff ** 2) / 0.1, diff - 0.05)
+        else:
+            per_ray_loss = (pred_rgb - target_rgb) ** 2
+        fg_weight = target_mask.expand_as(pred_rgb)
+        fg_sum = fg_weight.sum()
+        if fg_sum > 0:
+            loss_rgb = (per_ray_loss * fg_weight).sum() / fg_sum
+        else:
+            loss_rgb = per_ray_loss.mean()
+    else:
+        if loss_type.lower() == "huber":
+            loss_rgb = huber(pred_rgb, target_rgb, delta=0.1)
+        else:
+            loss_rgb = F.mse_loss(pred_rgb, target_rgb)
+    loss_total = loss_rgb
+    loss_mask = None
+    if use_mask and target_mask is not None:
+        loss_mask = F.binary_cross_entropy(torch.clamp(pred_opacity, 1e-6, 1.0 - 1e-6), target_mask)
+        loss_total = loss_total + mask_weight * loss_mask
+    return loss_total, loss_rgb, loss_mask
+
+
 def volume_render_from_samples(colors, sdf, normals, ray_dirs, z_vals, dists, s, pts=None, sphere_radius=1.0,
                                background_rgb=None, cos_anneal_ratio=0.0, use_iter_cos=False):
     """volume_render.py:57-153."""

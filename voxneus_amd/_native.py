@@ -35,6 +35,13 @@ MAX_LEVELS = 32
 MLP_PARAMS = 64 * 51 + 64 + 3 * 64 + 3
 VXN_VIEW_SH = 0
 VXN_VIEW_PE = 1
+VXN_LOSS_HUBER = 0
+VXN_LOSS_MSE = 1
+
+
+def loss_type_code(loss_type: str) -> int:
+    """The trainer's loss_type ("Huber" or anything else = MSE, trainer.py:59,216,233)."""
+    return VXN_LOSS_HUBER if str(loss_type).lower() == "huber" else VXN_LOSS_MSE
 MLP_PARAMS_PE = 64 * 62 + 64 + 3 * 64 + 3
 
 
@@ -85,7 +92,8 @@ class RenderArgs(ctypes.Structure):
         ("grad_grid", c_vp), ("grad_table", c_vp), ("grad_mlp", c_vp), ("grad_s", c_vp), ("loss", c_vp),
         ("vr_bitmask", c_vp), ("workspace", c_vp), ("workspace_bytes", c_i64),
         ("kernel_events", c_vp), ("gradient_mode", c_i32), ("fork_event", c_vp), ("s_dev", c_vp),
-        ("view_embed", c_i32),
+        ("view_embed", c_i32), ("loss_type", c_i32), ("target_mask", c_vp), ("mask_weight", c_f),
+        ("loss_parts", c_vp),
     ]
 
 

@@ -88,11 +88,19 @@ struct RenderDev {
   const float* target;
   float huber_delta;
   float inv_n3;
+  // training colour loss: VXN_LOSS_*; target_mask [R] (nullable) weights it by the
+  // foreground and adds mask_weight * BCE(opacity, mask); loss_norm (device, in the
+  // workspace): [0] the colour loss's normalisation, [1] 1 if the mask weights it
+  int loss_type;
+  const float* target_mask;
+  float mask_weight;
+  float inv_r;
+  const float* loss_norm;
   float* grad_grid;
   float* grad_table;
   uint32_t* vr;
   float* part_mlp;  // [gridDim.x][kNP]
-  double* part_sc;  // [gridDim.x][2]: dL/ds, Huber sum
+  double* part_sc;  // [gridDim.x][3]: dL/ds, colour-loss sum, mask-BCE sum
   float* gfeat;     // [32][R*S] dL/d(hash features), feature-major
   float* ggeo;      // [4][R*S]  dL/dsdf, dL/dgrad
   // BF16 backward / training: hash features gathered beforehand by hash_features_kernel,
@@ -248,7 +256,7 @@ __global__ void __launch_bounds__(kRT, 2) render_kernel(RenderDev a) {
     accB0[c] = 0.0f;
   }
   float accb1 = 0.0f;
-  double ds_acc = 0.0, loss_acc = 0.0;
+  double ds_acc = 0.0, loss_acc = 0.0, bce_acc = 0.0;
   // BF16 form: parameter-gradient accumulators; wave wid owns hidden rows 32*wid + crow.
   //   gw0[kt]: columns k = 32kt + j of [W0x | b0 | W0 SH columns]; gw1: W1^T columns c = j.
   const int jq = lane & 31, gq = lane >> 5;
@@ -509,17 +517,40 @@ __global__ void __launch_bounds__(kRT, 2) render_kernel(RenderDev a) {
     // ---------------------------------------------------------------- upstream gradients
     float dr[3] = {0.0f, 0.0f, 0.0f}, dd = 0.0f, dop = 0.0f, dN[3] = {0.0f, 0.0f, 0.0f};
     if (MODE == kTrain) {
+      // colour loss (trainer.py:213-236): Huber or MSE per channel, plain mean over the
+      // batch, or with a foreground mask the mask-weighted sum over the mask's sum (the
+      // plain mean when the mask sums to zero)
+      const float* mk = a.target_mask;
+      const float fw = (mk && a.loss_norm[1] != 0.0f) ? mk[ray] : 1.0f;
+      const float nrm_l = mk ? a.loss_norm[0] : a.inv_n3;
       double ls = 0.0;
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         const float df = comp[c] - a.target[3 * ray + c];
-        const float ad = fabsf(df);
-        const bool small = ad < a.huber_delta;
-        ls += small ? (double)(0.5f * (ad * ad) / a.huber_delta) : (double)(ad - 0.5f * a.huber_delta);
-        const float sg = df > 0.0f ? 1.0f : (df < 0.0f ? -1.0f : 0.0f);
-        dr[c] = (small ? (ad / a.huber_delta) : 1.0f) * sg * a.inv_n3;
+        float l, dl;
+        if (a.loss_type == VXN_LOSS_MSE) {
+          l = df * df;
+          dl = 2.0f * df;
+        } else {
+          const float ad = fabsf(df);
+          const bool small = ad < a.huber_delta;
+          l = small ? 0.5f * (ad * ad) / a.huber_delta : ad - 0.5f * a.huber_delta;
+          const float sg = df > 0.0f ? 1.0f : (df < 0.0f ? -1.0f : 0.0f);
+          dl = (small ? (ad / a.huber_delta) : 1.0f) * sg;
+        }
+        ls += (double)(l * fw);
+        dr[c] = (dl * fw) * nrm_l;
       }
       if (t == 0) loss_acc += ls;
+      // mask BCE (trainer.py:242-245): mask_weight * mean BCE(clamp(opacity, 1e-6, 1 - 1e-6))
+      if (mk && a.mask_weight > 0.0f) {
+        const float op = comp[4], m = mk[ray];
+        const float xo = fminf(fmaxf(op, 1e-6f), 1.0f - 1e-6f);
+        if (t == 0)
+          bce_acc -= (double)(m * fmaxf(logf(xo), -100.0f) + (1.0f - m) * fmaxf(logf(1.0f - xo), -100.0f));
+        const bool pass = op >= 1e-6f && op <= 1.0f - 1e-6f;
+        dop = pass ? a.mask_weight * ((xo - m) / fmaxf((1.0f - xo) * xo, 1e-12f)) * a.inv_r : 0.0f;
+      }
     } else {
       if (a.d_rgb) {
         dr[0] = a.d_rgb[3 * ray];
@@ -794,8 +825,9 @@ __global__ void __launch_bounds__(kRT, 2) render_kernel(RenderDev a) {
   if (lane == 0) red.d[wid] = dsw;
   __syncthreads();
   if (t == 0) {
-    a.part_sc[2 * blockIdx.x] = red.d[0] + red.d[1];
-    a.part_sc[2 * blockIdx.x + 1] = loss_acc;
+    a.part_sc[3 * blockIdx.x] = red.d[0] + red.d[1];
+    a.part_sc[3 * blockIdx.x + 1] = loss_acc;
+    a.part_sc[3 * blockIdx.x + 2] = bce_acc;
   }
 }
 
@@ -1366,7 +1398,9 @@ __global__ void __launch_bounds__(256) render_reduce1_kernel(const float* __rest
 __global__ void __launch_bounds__(256) render_reduce2_kernel(const float* __restrict__ rows,
                                                              const double* __restrict__ part_sc, int nb,
                                                              float* __restrict__ grad_mlp, float* __restrict__ grad_s,
-                                                             double* __restrict__ loss, float inv_n3, int np) {
+                                                             double* __restrict__ loss, float inv_n3, int np,
+                                                             const float* __restrict__ loss_norm, float mask_weight,
+                                                             float inv_r, double* __restrict__ loss_parts) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k < np) {
     double acc = 0.0;
@@ -1374,17 +1408,42 @@ __global__ void __launch_bounds__(256) render_reduce2_kernel(const float* __rest
     grad_mlp[k] = (float)acc;
   }
   if (blockIdx.x == gridDim.x - 1 && threadIdx.x < 64) {  // one wavefront: fixed-order strided sums
-    double ds = 0.0, ls = 0.0;
+    double ds = 0.0, ls = 0.0, lb = 0.0;
     for (int b = threadIdx.x; b < nb; b += 64) {
-      ds += part_sc[2 * b];
-      ls += part_sc[2 * b + 1];
+      ds += part_sc[3 * b];
+      ls += part_sc[3 * b + 1];
+      lb += part_sc[3 * b + 2];
     }
     ds = wave_sum_d(ds);
     ls = wave_sum_d(ls);
+    lb = wave_sum_d(lb);
     if (threadIdx.x == 0) {
       if (grad_s) grad_s[0] = (float)ds;
-      if (loss) loss[0] = ls * (double)inv_n3;
+      const double lc = ls * (double)(loss_norm ? loss_norm[0] : inv_n3);
+      const double lm = lb * (double)inv_r;
+      if (loss) loss[0] = lc + (double)mask_weight * lm;
+      if (loss_parts) {
+        loss_parts[0] = lc;
+        loss_parts[1] = lm;
+      }
     }
+  }
+}
+
+// The colour loss's normalisation with a foreground mask (trainer.py:223-231): out[0] =
+// 1 / (3 * sum(mask)) and out[1] = 1 when the mask has weight, else 1 / (3 R) and 0.
+__global__ void __launch_bounds__(256) mask_norm_kernel(const float* __restrict__ mask, int64_t R,
+                                                        float* __restrict__ out) {
+  double acc = 0.0;
+  for (int64_t i = threadIdx.x; i < R; i += 256) acc += (double)mask[i];
+  acc = wave_sum_d(acc);
+  __shared__ double red[4];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double fg = 3.0 * (((red[0] + red[1]) + red[2]) + red[3]);
+    out[0] = fg > 0.0 ? (float)(1.0 / fg) : (float)(1.0 / (3.0 * (double)R));
+    out[1] = fg > 0.0 ? 1.0f : 0.0f;
   }
 }
 
@@ -1436,7 +1495,7 @@ static BinPlan bin_plan(int64_t ns, int L, int log2_T, const int32_t* level_res)
 // Workspace layout: MLP partials | scalar partials | reduction rows | gfeat | ggeo | hfeat
 // | bin offsets | scan tile sums | record entries | record values.
 struct RenderWs {
-  int64_t off_sc, off_rows, off_feat, off_geo, off_hf, off_bin, off_tsum, off_ridx, off_rval, total;
+  int64_t off_sc, off_norm, off_rows, off_feat, off_geo, off_hf, off_bin, off_tsum, off_ridx, off_rval, total;
   BinPlan bins;
 };
 
@@ -1445,7 +1504,8 @@ static RenderWs render_ws(int64_t R, int S, int L, int log2_T, const int32_t* le
   const int64_t nb = render_blocks(R);
   const int64_t ns = R * (int64_t)S;
   w.off_sc = align256(nb * np * (int64_t)sizeof(float));
-  w.off_rows = align256(w.off_sc + nb * 2 * (int64_t)sizeof(double));
+  w.off_norm = align256(w.off_sc + nb * 3 * (int64_t)sizeof(double));
+  w.off_rows = align256(w.off_norm + 2 * (int64_t)sizeof(float));
   w.off_feat = align256(w.off_rows + (int64_t)kRedRows * np * (int64_t)sizeof(float));
   w.off_geo = align256(w.off_feat + 32 * ns * (int64_t)sizeof(float));
   w.off_hf = align256(w.off_geo + kGeo * ns * (int64_t)sizeof(float));
@@ -1519,6 +1579,11 @@ static int build_render_dev(const vxn_render_args& A, RenderDev& a, int mode) {
   a.target = A.target_rgb;
   a.huber_delta = A.huber_delta;
   a.inv_n3 = A.n_rays > 0 ? (float)(1.0 / (3.0 * (double)A.n_rays)) : 0.0f;
+  a.inv_r = A.n_rays > 0 ? (float)(1.0 / (double)A.n_rays) : 0.0f;
+  a.loss_type = A.loss_type;
+  a.target_mask = mode == kTrain ? A.target_mask : nullptr;
+  a.mask_weight = A.mask_weight;
+  a.loss_norm = nullptr;
   a.grad_grid = A.grad_grid;
   a.grad_table = A.grad_table;
   a.vr = A.vr_bitmask;
@@ -1539,6 +1604,7 @@ static int build_render_dev(const vxn_render_args& A, RenderDev& a, int mode) {
     char* ws = (char*)A.workspace;
     a.part_mlp = (float*)ws;
     a.part_sc = (double*)(ws + w.off_sc);
+    a.loss_norm = (const float*)(ws + w.off_norm);
     a.gfeat = (float*)(ws + w.off_feat);
     a.ggeo = (float*)(ws + w.off_geo);
     a.hfeat = ws + w.off_hf;
@@ -1546,6 +1612,9 @@ static int build_render_dev(const vxn_render_args& A, RenderDev& a, int mode) {
   if (mode == kTrain) {
     VXN_REQUIRE(A.target_rgb != nullptr, VXN_ERR_NULL, "target_rgb is null");
     VXN_REQUIRE(A.huber_delta > 0.0f, VXN_ERR_VALUE, "huber_delta must be > 0");
+    VXN_REQUIRE(A.loss_type == VXN_LOSS_HUBER || A.loss_type == VXN_LOSS_MSE, VXN_ERR_VALUE,
+                "unknown loss_type %d", A.loss_type);
+    VXN_REQUIRE(A.mask_weight >= 0.0f, VXN_ERR_VALUE, "mask_weight must be >= 0");
   }
   return 0;
 }
@@ -1575,6 +1644,10 @@ static int launch_render_nv(const vxn_render_args* p, void* stream) {
     VXN_LAUNCH_CHECK();
   }
   if (ev) VXN_HIP(hipEventRecord(ev[1], s));
+  if (MODE == kTrain && a.target_mask) {
+    hipLaunchKernelGGL(mask_norm_kernel, dim3(1), dim3(256), 0, s, a.target_mask, p->n_rays, (float*)a.loss_norm);
+    VXN_LAUNCH_CHECK();
+  }
   if (p->fork_event) VXN_HIP(hipEventRecord((hipEvent_t)p->fork_event, s));
   if (p->mlp_mode == VXN_MLP_BF16_MFMA)
     hipLaunchKernelGGL((render_kernel<MODE, kMlpBf16, 0, NV>), dim3(nb), dim3(kRT), 0, s, a);
@@ -1627,7 +1700,9 @@ static int launch_render_nv(const vxn_render_args* p, void* stream) {
                        rows, np);
     VXN_LAUNCH_CHECK();
     hipLaunchKernelGGL(render_reduce2_kernel, dim3((np + 1 + 255) / 256), dim3(256), 0, s, rows, a.part_sc, nb,
-                       p->grad_mlp, p->grad_s, MODE == kTrain ? p->loss : nullptr, a.inv_n3, np);
+                       p->grad_mlp, p->grad_s, MODE == kTrain ? p->loss : nullptr, a.inv_n3, np,
+                       a.target_mask ? a.loss_norm : nullptr, a.target_mask ? a.mask_weight : 0.0f, a.inv_r,
+                       MODE == kTrain ? p->loss_parts : nullptr);
     VXN_LAUNCH_CHECK();
     if (ev) VXN_HIP(hipEventRecord(ev[5], s));
   }

@@ -75,6 +75,11 @@ class StepConfig:
     perturb: bool = False
     sphere_radius: float = 1.0
     huber_delta: float = 0.1
+    # trainer.py:56-59, 213-245: "Huber" or "MSE" colour loss; with use_mask and a target
+    # mask passed to step(), the foreground-weighted colour loss plus mask_weight * BCE
+    loss_type: str = "Huber"
+    use_mask: bool = False
+    mask_weight: float = 0.1
     w_eik: float = 0.1
     w_curv: float = 5e-4
     s_amp_k: float = 5.0
@@ -130,6 +135,7 @@ class FusedTrainStep:
         self.grad_s = torch.zeros(1, device=dev, dtype=torch.float32)
         self.grad_v = torch.zeros((), device=dev, dtype=torch.float32)
         self.loss = torch.zeros(1, device=dev, dtype=torch.float64)
+        self.loss_parts = torch.zeros(2, device=dev, dtype=torch.float64)  # colour loss, mask BCE
         self.amplified = torch.zeros(1, device=dev, dtype=torch.int32)
         self.bitmask = torch.zeros(Fn.vr_bitmask_words((X, Y, Z)), device=dev, dtype=torch.int32)
         self.vr_counts = torch.zeros(2, device=dev, dtype=torch.int32)
@@ -154,7 +160,7 @@ class FusedTrainStep:
             from .hash_grid import level_weights
             self.level_w = level_weights(len(self.level_res), cfg.active_levels)
 
-    def _render_args(self, rays_o, rays_d, mid, dists, target):
+    def _render_args(self, rays_o, rays_d, mid, dists, target, target_mask=None):
         """The render arguments of this step.  The static part is built once; s stays on the
         device (``s_dev``), so building the arguments reads nothing back from the GPU."""
         cfg = self.cfg
@@ -171,6 +177,9 @@ class FusedTrainStep:
             Fn.fill_render_args(a, rc, self.grid, self.table, self.mlp, 0.0, rays_o, rays_d, mid, dists)
             a.s_dev = N.dptr(self.s_value)
             a.huber_delta = cfg.huber_delta
+            a.loss_type = N.loss_type_code(cfg.loss_type)
+            a.mask_weight = float(cfg.mask_weight)
+            a.loss_parts = N.dptr(self.loss_parts)
             a.grad_grid, a.grad_table = N.dptr(self.grad_grid), N.dptr(self.grad_table)
             a.grad_mlp, a.grad_s, a.loss = N.dptr(self.grad_mlp), N.dptr(self.grad_s), N.dptr(self.loss)
             need = ctypes.c_int64(0)
@@ -182,6 +191,7 @@ class FusedTrainStep:
         a.rays_o, a.rays_d = N.dptr(rays_o, torch.float32, "rays_o"), N.dptr(rays_d, torch.float32, "rays_d")
         a.mid_z, a.dists = N.dptr(mid, torch.float32, "mid_z"), N.dptr(dists, torch.float32, "dists")
         a.target_rgb = N.dptr(target, torch.float32, "target")
+        a.target_mask = N.dptr(target_mask, torch.float32, "target_mask")
         a.vr_bitmask = N.dptr(self.bitmask)
         a.fork_event = None
         a.kernel_events = (ctypes.cast(self.kernel_events.array, ctypes.c_void_p)
@@ -219,10 +229,17 @@ class FusedTrainStep:
         self.post_upsample_cooldown -= 1
 
     def step(self, rays_o: torch.Tensor, rays_d: torch.Tensor, target: torch.Tensor, t_rand=None, u_rand=None,
-             zs=None) -> Dict[str, torch.Tensor]:
-        """One fwd+bwd step; gradients are overwritten.  Returns device tensors."""
+             zs=None, target_mask: Optional[torch.Tensor] = None) -> Dict[str, torch.Tensor]:
+        """One fwd+bwd step; gradients are overwritten.  Returns device tensors.
+        ``target_mask`` ([R] or [R,1] foreground mask) is used when ``cfg.use_mask``."""
         cfg = self.cfg
         rays_o, rays_d, target = rays_o.contiguous(), rays_d.contiguous(), target.contiguous()
+        if cfg.use_mask and target_mask is not None:
+            target_mask = target_mask.reshape(-1).to(torch.float32).contiguous()
+            if target_mask.numel() != rays_o.shape[0]:
+                raise ValueError(f"target_mask has {target_mask.numel()} entries for {rays_o.shape[0]} rays")
+        else:
+            target_mask = None
         cooldown = self._cooldown()
         if cooldown and (self._g_eik is None or self._g_eik.shape != self.grid.shape):
             self._g_eik = torch.zeros_like(self.grid)
@@ -238,7 +255,7 @@ class FusedTrainStep:
         else:
             z, dists, mid = zs
         main = Fn._stream(mid)
-        a = self._render_args(rays_o, rays_d, mid.contiguous(), dists.contiguous(), target)
+        a = self._render_args(rays_o, rays_d, mid.contiguous(), dists.contiguous(), target, target_mask)
         if early:
             a.vr_bitmask = None
             if cfg.fork_after_features:
@@ -262,7 +279,8 @@ class FusedTrainStep:
             self.side.join(main)  # grad_grid, the losses and V_R counts are complete past this point
         if cooldown:
             self._eik_clamp()
-        return {"loss": self.loss, "reg_losses": self.reg_losses, "vr_counts": self.vr_counts,
+        return {"loss": self.loss, "loss_parts": self.loss_parts, "reg_losses": self.reg_losses,
+                "vr_counts": self.vr_counts,
                 "amplified": self.amplified, "z_vals": z, "dists": dists, "mid_z_vals": mid}
 
     def rescale_grid(self, new_resolution) -> None:
