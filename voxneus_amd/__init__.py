@@ -25,6 +25,7 @@ _LAZY = {
     "SparseGridAdam": "optim", "NetAdam": "optim", "create_optimizer": "optim",
     "upsample_sdf_grid": "sdf_grid", "get_resolution_for_step": "sdf_grid", "logistic_density": "rendering",
     "extract_mesh_marching_cubes": "mesh", "marching_cubes": "mesh", "apply_gaussian_filter_to_grid": "mesh",
+    "VoxNeuSScheduler": "scheduler", "compute_neus2_lambda": "scheduler", "FusedTrainer": "trainer",
     "save_checkpoint": "checkpoint", "load_checkpoint": "checkpoint", "load_resident_scene": "checkpoint",
 }
 

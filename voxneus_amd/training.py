@@ -13,6 +13,7 @@ needs one device read; ``FusedTrainStep`` uses the device-side form instead).
 from __future__ import annotations
 
 import ctypes
+import dataclasses
 from dataclasses import dataclass
 from typing import Dict, Optional
 
@@ -97,6 +98,10 @@ class StepConfig:
     # max-abs of the weighted eikonal gradient during a post-upsample cooldown
     # (trainer.py:118, scheduler.eik_grad_clamp)
     eik_grad_clamp: float = 0.1
+    # NeuS iter_cos annealing of the alpha's cosine (neus_alpha.py:46-57; the trainer
+    # passes min(step / 10000, 1) as the ratio, trainer.py:192)
+    use_iter_cos: bool = False
+    cos_anneal_ratio: float = 0.0
 
 
 class FusedTrainStep:
@@ -111,12 +116,16 @@ class FusedTrainStep:
 
     def __init__(self, grid: torch.Tensor, table: torch.Tensor, mlp: torch.Tensor, variance: torch.Tensor,
                  level_res, box=(-1.0, -1.0, -1.0, 1.0, 1.0, 1.0), s_max: float = 300.0,
-                 occ_mask: Optional[torch.Tensor] = None, cfg: StepConfig = StepConfig()):
+                 occ_mask: Optional[torch.Tensor] = None, cfg: Optional[StepConfig] = None, occ_box=None):
         self.grid, self.table, self.mlp, self.variance = grid, table, mlp, variance
         self.level_res = list(level_res)
         self.box = tuple(float(v) for v in box)
+        # the occupancy grid's own bounds (the model's, occupancy_grid.py:33-47)
+        self.occ_box = self.box if occ_box is None else tuple(float(v) for v in occ_box)
         self.s_max = float(s_max)
-        self.cfg = cfg
+        # a private copy: the schedules change the weights of this step's configuration
+        self.cfg = dataclasses.replace(cfg) if cfg is not None else StepConfig()
+        cfg = self.cfg
         dev = grid.device
         if occ_mask is None:
             # the reference always samples with an occupancy grid; a fresh one is all True
@@ -156,9 +165,14 @@ class FusedTrainStep:
         self._g_eik = None
         self._clamp_ws = None
         self.level_w = None
-        if cfg.active_levels is not None:
-            from .hash_grid import level_weights
-            self.level_w = level_weights(len(self.level_res), cfg.active_levels)
+        self.set_active_levels(cfg.active_levels)
+
+    def set_active_levels(self, active_levels: Optional[float]) -> None:
+        """The NeuS2 bandwidth lambda of the following steps (radiance_hash.py:90-102); None
+        keeps every level.  The level weights go into each launch's arguments."""
+        from .hash_grid import level_weights
+        self.cfg.active_levels = active_levels
+        self.level_w = level_weights(len(self.level_res), active_levels)
 
     def _render_args(self, rays_o, rays_d, mid, dists, target, target_mask=None):
         """The render arguments of this step.  The static part is built once; s stays on the
@@ -170,7 +184,8 @@ class FusedTrainStep:
         key = (R, S, mid.device, self.grid.data_ptr(), self.table.data_ptr(), self.mlp.data_ptr())
         a = self._args
         if a is None or self._args_key != key:
-            rc = Fn.RenderConfig(self.box, self.level_res, self.level_w, cfg.sphere_radius, mlp_mode=cfg.mlp_mode,
+            rc = Fn.RenderConfig(self.box, self.level_res, self.level_w, cfg.sphere_radius, cfg.cos_anneal_ratio,
+                                 cfg.use_iter_cos, mlp_mode=cfg.mlp_mode,
                                  gradient_mode=N.gradient_mode_code(cfg.gradient_mode),
                                  view_embed=N.view_embed_code(cfg.view_embed_type))
             a = N.RenderArgs()
@@ -188,6 +203,11 @@ class FusedTrainStep:
                 self._render_ws = torch.empty(int(need.value), device=mid.device, dtype=torch.uint8)
             a.workspace, a.workspace_bytes = self._render_ws.data_ptr(), self._render_ws.numel()
             self._args, self._args_key = a, key
+        # per-step settings of the schedules
+        for i in range(len(self.level_res)):
+            a.level_w[i] = 1.0 if self.level_w is None else float(self.level_w[i])
+        a.cos_anneal_ratio = float(cfg.cos_anneal_ratio)
+        a.use_iter_cos = int(cfg.use_iter_cos)
         a.rays_o, a.rays_d = N.dptr(rays_o, torch.float32, "rays_o"), N.dptr(rays_d, torch.float32, "rays_d")
         a.mid_z, a.dists = N.dptr(mid, torch.float32, "mid_z"), N.dptr(dists, torch.float32, "dists")
         a.target_rgb = N.dptr(target, torch.float32, "target")
@@ -202,7 +222,7 @@ class FusedTrainStep:
         cfg = self.cfg
         # ray bounds from the unit sphere (trainer.py:188) are computed inside the sampler
         return Fn.upsample(rays_o, rays_d, None, None, self.grid, self.box, cfg.n_samples, cfg.n_importance,
-                           cfg.up_sample_steps, cfg.perturb, self.occ_mask, self.box, t_rand, u_rand,
+                           cfg.up_sample_steps, cfg.perturb, self.occ_mask, self.occ_box, t_rand, u_rand,
                            bound_radius=1.0, vr_bitmask=vr_bitmask)
 
     def _cooldown(self) -> bool:
@@ -288,9 +308,16 @@ class FusedTrainStep:
         grid is replaced by its trilinear upsampling (aligned corners, on the device) and
         the gradient, V_R and regulariser buffers follow the new shape."""
         shape = (new_resolution,) * 3 if isinstance(new_resolution, int) else tuple(int(v) for v in new_resolution)
+        self.adopt_grid(Fn.upsample_grid(self.grid, shape))
+
+    def adopt_grid(self, grid: torch.Tensor) -> None:
+        """Train ``grid`` ([X,Y,Z] f32 on the step's device) from now on; the gradient, V_R
+        and regulariser buffers follow its shape."""
         dev = self.grid.device
-        self.grid = Fn.upsample_grid(self.grid, shape)
-        X, Y, Z = shape
+        if grid.dim() != 3 or grid.dtype != torch.float32 or grid.device != dev or not grid.is_contiguous():
+            raise ValueError("the grid must be a contiguous float32 [X,Y,Z] tensor on the step's device")
+        self.grid = grid
+        X, Y, Z = grid.shape
         self.grad_grid = torch.zeros_like(self.grid)
         self.bitmask = torch.zeros(Fn.vr_bitmask_words((X, Y, Z)), device=dev, dtype=torch.int32)
         nb = ctypes.c_int64(0)
@@ -312,7 +339,7 @@ class FusedTrainStep:
             idx = tuple(torch.randint(0, res, (n,), device=self.grid.device, generator=generator)
                         for _ in range(3))
         s = float(torch.clamp(torch.exp(self.variance.detach() * 10.0), min=1e-4, max=self.s_max))
-        Fn.occupancy_update(self.occ_density, self.occ_mask, self.box, self.grid, self.box, s, decay, threshold,
+        Fn.occupancy_update(self.occ_density, self.occ_mask, self.occ_box, self.grid, self.box, s, decay, threshold,
                             self.occ_updates == 0, idx)
         self.occ_updates += 1
 
