@@ -278,6 +278,14 @@ typedef struct vxn_render_args {
   const float* target_mask;
   float mask_weight;
   double* loss_parts;
+  /* backward and training entries: nullable second stream (default priority) for the
+     grid scatter, which then runs beside the hash-table gradient.  scatter_events
+     (hipEvent_t[2], required with it) order it after the render kernel and make `stream`
+     wait for it before the call returns, so every output is complete on `stream` as
+     without it.  kernel_events[4] is then recorded on scatter_stream after the grid
+     scatter (its span is kernel_events[2] -> [4]). */
+  void* scatter_stream;
+  void* const* scatter_events;
 } vxn_render_args;
 
 /* Workspace of the backward and training entries (the forward needs none): per-sample

@@ -229,3 +229,30 @@ def test_fused_trainer_schedules_over_milestones():
     assert bool(torch.isfinite(out["rgb"]).all())
     with pytest.raises(ValueError):
         tr.train_step(9)  # no batch and no dataset
+
+
+def test_concurrent_grid_scatter_equals_serial():
+    """The grid scatter on its own stream (vxn_render_args.scatter_stream) gives the serial
+    step's outputs: loss, MLP gradient and regulariser losses bit for bit, the grid and
+    hash-table gradients up to the order of their adds (grid atomics; record positions
+    in the hash bins are taken with LDS atomics, so a bin's summation order varies from
+    run to run)."""
+    from voxneus_amd.training import FusedTrainStep, StepConfig
+    g = load_golden("step_c1.npz")
+    sc = scenes.config1_scene(seed=0)
+    o, d, tgt = (torch.from_numpy(g[k]).to(DEV) for k in ("rays_o", "rays_d", "target"))
+    res = []
+    for conc in (False, True):
+        st = FusedTrainStep(torch.from_numpy(sc.grid).to(DEV), torch.from_numpy(sc.embeddings).to(DEV),
+                            _flat(sc.mlp).to(DEV), torch.tensor(0.3, device=DEV),
+                            R.hash_level_resolutions(16, 16, sc.hash_max_resolution),
+                            cfg=StepConfig(concurrent_scatter=conc))
+        assert (st.scatter is not None) == conc
+        for _ in range(2):
+            out = st.step(o, d, tgt)
+        res.append((float(out["loss"]), st.grad_grid.cpu().numpy(), st.grad_table.cpu().numpy(),
+                    st.grad_mlp.cpu().numpy(), out["reg_losses"].cpu().numpy()))
+    a, b = res
+    assert a[0] == b[0]
+    assert np.array_equal(a[3], b[3]) and np.array_equal(a[4], b[4])
+    assert rel_err(b[1], a[1]) < 1e-6 and rel_err(b[2], a[2]) < 1e-6

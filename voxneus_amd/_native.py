@@ -93,7 +93,7 @@ class RenderArgs(ctypes.Structure):
         ("vr_bitmask", c_vp), ("workspace", c_vp), ("workspace_bytes", c_i64),
         ("kernel_events", c_vp), ("gradient_mode", c_i32), ("fork_event", c_vp), ("s_dev", c_vp),
         ("view_embed", c_i32), ("loss_type", c_i32), ("target_mask", c_vp), ("mask_weight", c_f),
-        ("loss_parts", c_vp),
+        ("loss_parts", c_vp), ("scatter_stream", c_vp), ("scatter_events", c_vp),
     ]
 
 

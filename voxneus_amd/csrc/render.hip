@@ -1616,6 +1616,11 @@ static int build_render_dev(const vxn_render_args& A, RenderDev& a, int mode) {
                 "unknown loss_type %d", A.loss_type);
     VXN_REQUIRE(A.mask_weight >= 0.0f, VXN_ERR_VALUE, "mask_weight must be >= 0");
   }
+  if (mode != kFwd && A.scatter_stream) {
+    hipEvent_t const* se = (hipEvent_t const*)A.scatter_events;
+    VXN_REQUIRE(se != nullptr && se[0] != nullptr && se[1] != nullptr, VXN_ERR_NULL,
+                "scatter_stream needs scatter_events[2]");
+  }
   return 0;
 }
 
@@ -1658,6 +1663,19 @@ static int launch_render_nv(const vxn_render_args* p, void* stream) {
   if (MODE != kFwd) {
     const int64_t ns = p->n_rays * (int64_t)p->n_samples;
     const int sb = (int)((ns + 255) / 256);
+    // the grid scatter needs only the render kernel's per-sample geometry gradients: on a
+    // second stream it runs beside the hash-table gradient (atomics against plain stores
+    // and LDS work)
+    hipStream_t ss = (hipStream_t)p->scatter_stream;
+    hipEvent_t const* se = (hipEvent_t const*)p->scatter_events;
+    if (ss) {
+      VXN_HIP(hipEventRecord(se[0], s));
+      VXN_HIP(hipStreamWaitEvent(ss, se[0], 0));
+      hipLaunchKernelGGL(grid_scatter_kernel, dim3(sb), dim3(256), 0, ss, a);
+      VXN_LAUNCH_CHECK();
+      if (ev) VXN_HIP(hipEventRecord(ev[4], ss));
+      VXN_HIP(hipEventRecord(se[1], ss));
+    }
     const RenderWs w = render_ws(*p);
     if (w.bins.use && ((uintptr_t)p->grad_table & 7) == 0) {
       char* ws = (char*)p->workspace;
@@ -1691,9 +1709,11 @@ static int launch_render_nv(const vxn_render_args* p, void* stream) {
       VXN_LAUNCH_CHECK();
     }
     if (ev) VXN_HIP(hipEventRecord(ev[3], s));
-    hipLaunchKernelGGL(grid_scatter_kernel, dim3(sb), dim3(256), 0, s, a);
-    VXN_LAUNCH_CHECK();
-    if (ev) VXN_HIP(hipEventRecord(ev[4], s));
+    if (!ss) {
+      hipLaunchKernelGGL(grid_scatter_kernel, dim3(sb), dim3(256), 0, s, a);
+      VXN_LAUNCH_CHECK();
+      if (ev) VXN_HIP(hipEventRecord(ev[4], s));
+    }
     float* rows = (float*)((char*)p->workspace + w.off_rows);
     constexpr int np = MlpLayout<NV>::NP;
     hipLaunchKernelGGL(render_reduce1_kernel, dim3((np + 255) / 256, kRedRows), dim3(256), 0, s, a.part_mlp, nb,
@@ -1704,6 +1724,7 @@ static int launch_render_nv(const vxn_render_args* p, void* stream) {
                        a.target_mask ? a.loss_norm : nullptr, a.target_mask ? a.mask_weight : 0.0f, a.inv_r,
                        MODE == kTrain ? p->loss_parts : nullptr);
     VXN_LAUNCH_CHECK();
+    if (ss) VXN_HIP(hipStreamWaitEvent(s, se[1], 0));
     if (ev) VXN_HIP(hipEventRecord(ev[5], s));
   }
   return 0;
