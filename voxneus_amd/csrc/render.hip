@@ -980,7 +980,11 @@ __global__ void __launch_bounds__(256) hash_scatter_kernel(RenderDev a, int dedu
 // each corner gets a record of its own, its second half adding zero to the same entry):
 // two entries within the bin and both features' weighted gradients of each.  Per live
 // sample and level that is 4 records, with runs of consecutive samples in one cell summed
-// first on the levels below `dedup_levels`, as in the atomic scatter.
+// first on the levels below `dedup_levels`, as in the atomic scatter.  Above them, where
+// runs are short, a record is factored into one 16-byte store: the entries and flags in
+// the first word, then x's cell fraction fx and the two features' gradients times the
+// pair's y-z weight (the x corner takes (1 - fx) of them, the x + 1 corner fx), so those
+// records need no separate entry array.
 // ------------------------------------------------------------------------------------
 constexpr int kBinBits = 11;
 constexpr int kBinEntries = 1 << kBinBits;  // 16 KB of float2 sums + 2 KB of tags in LDS
@@ -994,13 +998,18 @@ struct BinDev {
   int32_t* off;     // [nbins * nwg + 1]: counts, then their exclusive scan and total
   uint32_t* ridx;   // [records] entries within the bin: x corner | (x + 1 corner) << 16
   float4* rval;     // [records] dL/dfeature 0, 1 of the x corner, then of the x + 1 corner
+  int fact_level;   // first level with factored records: rval only, {bits, fx, g0, g1} with
+                    // bits = x corner | (x + 1 corner) << 11 | kFactFirst/kFactSecond
 };
+constexpr uint32_t kFactFirst = 1u << 22;   // split pair: only the x corner, (1 - fx) g
+constexpr uint32_t kFactSecond = 2u << 22;  // split pair: only the x + 1 corner, fx g
 
 // Walks the hash levels of sample i (lanes of a wavefront hold consecutive samples) and
 // calls level(l, tail, cell, v) with every lane of the wavefront, on each level with a live
 // lane: tail marks the lanes that emit records (the last lane of each run of samples in
 // one cell on the levels below `dedup_levels`, every live lane above), v their 8 corners'
-// weighted gradients (run sums).  VALUES = false leaves v unset: only the tails are needed.
+// weighted gradients (run sums) and d0, d1 the level's feature gradients of the lane's own
+// sample.  VALUES = false leaves v unset: only the tails are needed.
 template <bool VALUES, class F>
 VXN_DEV void hash_levels(const RenderDev& a, int64_t i, int lane, int dedup_levels, F&& level) {
   const int64_t NS = a.R * (int64_t)a.S;
@@ -1046,7 +1055,7 @@ VXN_DEV void hash_levels(const RenderDev& a, int64_t i, int lane, int dedup_leve
         tail = (lane == 63 || kn != key) && live;
       }
     }
-    level(l, tail, c, v);
+    level(l, tail, c, v, d0, d1);
   }
 }
 
@@ -1063,7 +1072,8 @@ __global__ void __launch_bounds__(256) hash_bin_count_kernel(RenderDev a, BinDev
   const int lane = threadIdx.x & 63;
   for (int c = 0; c < kBinChunks; ++c) {
     const int64_t i = ((int64_t)blockIdx.x * kBinChunks + c) * 256 + threadIdx.x;
-    hash_levels<false>(a, i, lane, dedup_levels, [&](int l, bool tail, const HashCell& cell, const float(&)[16]) {
+    hash_levels<false>(a, i, lane, dedup_levels,
+                       [&](int l, bool tail, const HashCell& cell, const float(&)[16], float, float) {
       if (tail) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -1086,8 +1096,28 @@ __global__ void __launch_bounds__(256) hash_bin_write_kernel(RenderDev a, BinDev
   const int lane = threadIdx.x & 63;
   for (int c = 0; c < kBinChunks; ++c) {
     const int64_t i = ((int64_t)blockIdx.x * kBinChunks + c) * 256 + threadIdx.x;
-    hash_levels<true>(a, i, lane, dedup_levels, [&](int l, bool tail, const HashCell& cell, const float(&v)[16]) {
-      if (tail) {
+    hash_levels<true>(a, i, lane, dedup_levels,
+                      [&](int l, bool tail, const HashCell& cell, const float(&v)[16], float d0, float d1) {
+      if (tail && l >= b.fact_level) {
+        constexpr uint32_t m = kBinEntries - 1;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const uint32_t ia = cell.idx[q], ib = cell.idx[q + 4];
+          const int ba = (int)(ia >> kBinBits), bb = (int)(ib >> kBinBits);
+          const float wy = (q & 2) ? cell.dy : 1.0f - cell.dy;
+          const float wz = (q & 1) ? cell.dz : 1.0f - cell.dz;
+          const float wyz = wy * wz;
+          const float g0 = d0 * wyz, g1 = d1 * wyz;
+          const int32_t p = atomicAdd(&pos[l * b.nbl + ba], 1);
+          if (bb == ba) {
+            b.rval[p] = make_float4(__uint_as_float((ia & m) | ((ib & m) << 11)), cell.dx, g0, g1);
+          } else {
+            b.rval[p] = make_float4(__uint_as_float((ia & m) * 0x801u | kFactFirst), cell.dx, g0, g1);
+            const int32_t p2 = atomicAdd(&pos[l * b.nbl + bb], 1);
+            b.rval[p2] = make_float4(__uint_as_float((ib & m) * 0x801u | kFactSecond), cell.dx, g0, g1);
+          }
+        }
+      } else if (tail) {
         constexpr uint32_t m = kBinEntries - 1;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -1127,6 +1157,7 @@ __global__ void __launch_bounds__(64) hash_bin_sum_kernel(BinDev b, float* __res
   for (int e = lane; e < ne; e += 64) acc[e] = make_float2(0.0f, 0.0f);
   wave_sync();
   const int32_t r0 = b.off[(int64_t)bin * b.nwg], r1 = b.off[(int64_t)(bin + 1) * b.nwg];
+  const bool fact = l >= b.fact_level;
   constexpr int U = 8;  // 64-record groups loaded ahead
   for (int32_t k0 = r0; k0 < r1; k0 += U * 64) {
     uint32_t r[U];
@@ -1134,8 +1165,19 @@ __global__ void __launch_bounds__(64) hash_bin_sum_kernel(BinDev b, float* __res
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int32_t k = k0 + u * 64 + lane;
-      r[u] = k < r1 ? b.ridx[k] : 0u;
-      v[u] = k < r1 ? b.rval[k] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      if (fact) {
+        const float4 f = k < r1 ? b.rval[k] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        const uint32_t bits = __float_as_uint(f.x);
+        const float ox = 1.0f - f.y;
+        const float a0 = ox * f.z, a1 = ox * f.w, c0 = f.y * f.z, c1 = f.y * f.w;
+        r[u] = (bits & 0x7FFu) | (((bits >> 11) & 0x7FFu) << 16);
+        const uint32_t fl = bits & (kFactFirst | kFactSecond);
+        v[u] = fl == 0u ? make_float4(a0, a1, c0, c1)
+                        : (fl == kFactFirst ? make_float4(a0, a1, 0.0f, 0.0f) : make_float4(c0, c1, 0.0f, 0.0f));
+      } else {
+        r[u] = k < r1 ? b.ridx[k] : 0u;
+        v[u] = k < r1 ? b.rval[k] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      }
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -1688,6 +1730,7 @@ static int launch_render_nv(const vxn_render_args* p, void* stream) {
       b.rval = (float4*)(ws + w.off_rval);
       int32_t* tsum = (int32_t*)(ws + w.off_tsum);
       const int dl = dedup_levels(a.h);
+      b.fact_level = dl;
       const size_t lds = (size_t)b.nbins * sizeof(int32_t);
       hipLaunchKernelGGL(hash_bin_count_kernel, dim3(b.nwg), dim3(256), lds, s, a, b, dl);
       VXN_LAUNCH_CHECK();

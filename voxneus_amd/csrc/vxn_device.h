@@ -339,6 +339,7 @@ VXN_DEV uint32_t hash3(uint32_t x, uint32_t y, uint32_t z, uint32_t mask) {
 struct HashCell {
   uint32_t idx[8];
   float w[8];
+  float dx, dy, dz;  // position in the cell: w[k] = (x weight * y weight) * z weight
 };
 
 VXN_DEV HashCell hash_cell(float x0, float x1, float x2, float res, uint32_t mask) {
@@ -348,6 +349,9 @@ VXN_DEV HashCell hash_cell(float x0, float x1, float x2, float res, uint32_t mas
   const float dx = s0 - f0, dy = s1 - f1, dz = s2 - f2;
   const float ox = 1.0f - dx, oy = 1.0f - dy, oz = 1.0f - dz;
   HashCell h;
+  h.dx = dx;
+  h.dy = dy;
+  h.dz = dz;
   h.w[0] = ox * oy * oz;
   h.w[1] = ox * oy * dz;
   h.w[2] = ox * dy * oz;
