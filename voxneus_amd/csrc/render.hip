@@ -1348,12 +1348,13 @@ __global__ void __launch_bounds__(256) grid_scatter_kernel(RenderDev a) {
     // interior divisor of every corner: max((i_p - i_m) * eps, eps) = 2 * eps
     const float dvx = fmaxf(2.0f * g.eps[0], g.eps[0]), dvy = fmaxf(2.0f * g.eps[1], g.eps[1]),
                 dvz = fmaxf(2.0f * g.eps[2], g.eps[2]);
+    const float rx = 0.5f * (1.0f / g.eps[0]), ry = 0.5f * (1.0f / g.eps[1]), rz = 0.5f * (1.0f / g.eps[2]);
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
       const int ox = (k >> 2) & 1, oy = (k >> 1) & 1, oz = k & 1;
-      const float vx = (cn.w[k] * dg[0]) / dvx;
-      const float vy = (cn.w[k] * dg[1]) / dvy;
-      const float vz = (cn.w[k] * dg[2]) / dvz;
+      const float vx = div_rn(cn.w[k] * dg[0], dvx, rx);
+      const float vy = div_rn(cn.w[k] * dg[1], dvy, ry);
+      const float vz = div_rn(cn.w[k] * dg[2], dvz, rz);
       // x stencil of corner k: +v at ox+1, -v at ox-1
       v[ox ? slot_face(0, 1, oy, oz) : slot_corner(1, oy, oz)] += vx;
       v[ox ? slot_corner(0, oy, oz) : slot_face(0, 0, oy, oz)] -= vx;

@@ -180,22 +180,49 @@ VXN_DEV void fd_axis(int i, int res, float eps, int& ip, int& im, float& dv) {
   dv = fmaxf((float)(ip - im) * eps, eps);
 }
 
+// The same, with y = RN(1 / dv) from reps = RN(1 / eps): dv is 2 eps inside the grid
+// (ip - im = 2) and eps at a border, and halving is exact, so y = reps / 2 or reps.
+VXN_DEV void fd_axis(int i, int res, float eps, float reps, int& ip, int& im, float& dv, float& y) {
+  fd_axis(i, res, eps, ip, im, dv);
+  y = (ip - im == 2) ? 0.5f * reps : reps;
+}
+
+// Correctly rounded a / b from y = RN(1 / b) (b > 0), by Markstein's residual correction
+// (P. Markstein, "IA-64 and Elementary Functions", 2000, ch. 8; Muller et al., "Handbook
+// of Floating-Point Arithmetic", the fma-based division theorems): with y correctly
+// rounded and q within one ulp of a / b, the residual e = a - b q is exact in one fma and
+// RN(q + e y) = RN(a / b).  q = RN(a y) is within 1.5 ulp, so one correction first brings
+// it within one ulp and a second rounds correctly (an exact q gets zero residuals and is
+// kept).  Without branches: the quotient takes the sign of a y (the division's sign,
+// signed zeros included), and an infinite a y is the quotient of an infinite a (a NaN a
+// stays NaN).  Quotients in the subnormal range may differ from the division in the last
+// place.
+VXN_DEV float div_rn(float a, float b, float y) {
+  const float q0 = a * y;
+  const float e0 = fmaf(-q0, b, a);
+  const float q1 = fmaf(e0, y, q0);
+  const float e1 = fmaf(-q1, b, a);
+  const float q2 = copysignf(fmaf(e1, y, q1), q0);
+  return __builtin_isinf(q0) ? q0 : q2;
+}
+
 // Eq. 7: grad = sum_k w_k n_k, summed in PyTorch's CPU order for an 8-long reduction
 // over a strided dimension: ((t0 + t4) + (t1 + t5)) + (t2 + t6)) + (t3 + t7).
 VXN_DEV void eq7_gradient(const GridDesc& g, const Corners& c, float out[3]) {
+  const float r0 = 1.0f / g.eps[0], r1 = 1.0f / g.eps[1], r2 = 1.0f / g.eps[2];
   float t[8][3];
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
     int x, y, z;
     corner_xyz(c, k, x, y, z);
     int ip, im;
-    float dv;
-    fd_axis(x, g.X, g.eps[0], ip, im, dv);
-    const float nx = (g.grid[lin3(g, ip, y, z)] - g.grid[lin3(g, im, y, z)]) / dv;
-    fd_axis(y, g.Y, g.eps[1], ip, im, dv);
-    const float ny = (g.grid[lin3(g, x, ip, z)] - g.grid[lin3(g, x, im, z)]) / dv;
-    fd_axis(z, g.Z, g.eps[2], ip, im, dv);
-    const float nz = (g.grid[lin3(g, x, y, ip)] - g.grid[lin3(g, x, y, im)]) / dv;
+    float dv, ry;
+    fd_axis(x, g.X, g.eps[0], r0, ip, im, dv, ry);
+    const float nx = div_rn(g.grid[lin3(g, ip, y, z)] - g.grid[lin3(g, im, y, z)], dv, ry);
+    fd_axis(y, g.Y, g.eps[1], r1, ip, im, dv, ry);
+    const float ny = div_rn(g.grid[lin3(g, x, ip, z)] - g.grid[lin3(g, x, im, z)], dv, ry);
+    fd_axis(z, g.Z, g.eps[2], r2, ip, im, dv, ry);
+    const float nz = div_rn(g.grid[lin3(g, x, y, ip)] - g.grid[lin3(g, x, y, im)], dv, ry);
     t[k][0] = c.w[k] * nx;
     t[k][1] = c.w[k] * ny;
     t[k][2] = c.w[k] * nz;
@@ -208,22 +235,23 @@ VXN_DEV void eq7_gradient(const GridDesc& g, const Corners& c, float out[3]) {
 // Scatter dL/dgrad (the un-normalised Eq. 7 gradient) into the grid gradient:
 // d n_k,a / d f[i_p] = +1/dv, d f[i_m] = -1/dv, weighted by w_k.
 VXN_DEV void eq7_scatter(const GridDesc& g, const Corners& c, const float dg[3], float* grad_grid) {
+  const float r0 = 1.0f / g.eps[0], r1 = 1.0f / g.eps[1], r2 = 1.0f / g.eps[2];
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
     int x, y, z;
     corner_xyz(c, k, x, y, z);
     int ip, im;
-    float dv;
-    fd_axis(x, g.X, g.eps[0], ip, im, dv);
-    float v = (c.w[k] * dg[0]) / dv;
+    float dv, ry;
+    fd_axis(x, g.X, g.eps[0], r0, ip, im, dv, ry);
+    float v = div_rn(c.w[k] * dg[0], dv, ry);
     atomicAdd(grad_grid + lin3(g, ip, y, z), v);
     atomicAdd(grad_grid + lin3(g, im, y, z), -v);
-    fd_axis(y, g.Y, g.eps[1], ip, im, dv);
-    v = (c.w[k] * dg[1]) / dv;
+    fd_axis(y, g.Y, g.eps[1], r1, ip, im, dv, ry);
+    v = div_rn(c.w[k] * dg[1], dv, ry);
     atomicAdd(grad_grid + lin3(g, x, ip, z), v);
     atomicAdd(grad_grid + lin3(g, x, im, z), -v);
-    fd_axis(z, g.Z, g.eps[2], ip, im, dv);
-    v = (c.w[k] * dg[2]) / dv;
+    fd_axis(z, g.Z, g.eps[2], r2, ip, im, dv, ry);
+    v = div_rn(c.w[k] * dg[2], dv, ry);
     atomicAdd(grad_grid + lin3(g, x, y, ip), v);
     atomicAdd(grad_grid + lin3(g, x, y, im), -v);
   }
