@@ -329,16 +329,22 @@ def test_s_amplify_device_side(Fn):
     assert g.item() == 1.25 and f.item() == 0
 
 
-def test_lattice_query_equals_point_query(Fn):
+@pytest.mark.parametrize("shape,lbox", [((17, 17, 17), BOX6), ((64, 64, 64), BOX6),
+                                        ((23, 19, 31), (-1.3, -0.7, -1.1, 1.2, 0.9, 1.4))])
+def test_lattice_query_equals_point_query(Fn, shape, lbox):
+    """The lattice kernel (stencil rows staged in LDS) equals the point query at the same
+    points bit for bit: a lattice between the grid's vertices, one on them, and one whose
+    box reaches past the grid's (clamped border cells)."""
     sc = scenes.config1_scene(seed=0)
     grid = torch.from_numpy(sc.grid).to(DEV)
-    n = 17
-    sdf, nrm = Fn.lattice_query(grid, BOX, n, n, n, BOX6)
-    ax = torch.linspace(-1.0, 1.0, n)
-    lsx = (1.0 - -1.0) / (n - 1)
-    pts = torch.stack(torch.meshgrid(ax, ax, ax, indexing="ij"), -1).reshape(-1, 3)
-    pts = (-1.0 + torch.arange(n, dtype=torch.float32) * torch.tensor(lsx, dtype=torch.float32))
-    P = torch.stack(torch.meshgrid(pts, pts, pts, indexing="ij"), -1).reshape(-1, 3).to(DEV)
+    nx, ny, nz = shape
+    sdf, nrm = Fn.lattice_query(grid, BOX, nx, ny, nz, lbox)
+    ax = []
+    for a, n in enumerate(shape):
+        lo, hi = np.float32(lbox[a]), np.float32(lbox[a + 3])
+        step = np.float32((hi - lo) / np.float32(n - 1))
+        ax.append(torch.from_numpy(lo + np.arange(n, dtype=np.float32) * step))
+    P = torch.stack(torch.meshgrid(*ax, indexing="ij"), -1).reshape(-1, 3).to(DEV)
     o = Fn.grid_query_raw(grid, BOX, P, True, False, True)
     assert torch.equal(sdf.reshape(-1), o["sdf"]) and torch.equal(nrm.reshape(-1, 3), o["normal"])
 

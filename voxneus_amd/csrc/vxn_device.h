@@ -114,16 +114,22 @@ VXN_DEV GsStencil gs_stencil(const GridDesc& g, float px, float py, float pz) {
   return s;
 }
 
-VXN_DEV float gs_eval(const GridDesc& g, const GsStencil& s) {
+// ld(x, y, z) returns the grid value at a vertex (global memory, or a staged copy).
+template <class Ld>
+VXN_DEV float gs_eval(const GsStencil& s, Ld&& ld) {
   float out = 0.0f;
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
     if (s.valid & (1u << k)) {
       const int x = s.x0 + ((k >> 2) & 1), y = s.y0 + ((k >> 1) & 1), z = s.z0 + (k & 1);
-      out = out + g.grid[lin3(g, x, y, z)] * s.w[k];
+      out = out + ld(x, y, z) * s.w[k];
     }
   }
   return out;
+}
+
+VXN_DEV float gs_eval(const GridDesc& g, const GsStencil& s) {
+  return gs_eval(s, [&](int x, int y, int z) { return g.grid[lin3(g, x, y, z)]; });
 }
 
 // ------------------------------------------------------------------------------------
@@ -208,7 +214,8 @@ VXN_DEV float div_rn(float a, float b, float y) {
 
 // Eq. 7: grad = sum_k w_k n_k, summed in PyTorch's CPU order for an 8-long reduction
 // over a strided dimension: ((t0 + t4) + (t1 + t5)) + (t2 + t6)) + (t3 + t7).
-VXN_DEV void eq7_gradient(const GridDesc& g, const Corners& c, float out[3]) {
+template <class Ld>
+VXN_DEV void eq7_gradient(const GridDesc& g, const Corners& c, float out[3], Ld&& ld) {
   const float r0 = 1.0f / g.eps[0], r1 = 1.0f / g.eps[1], r2 = 1.0f / g.eps[2];
   float t[8][3];
 #pragma unroll
@@ -218,11 +225,11 @@ VXN_DEV void eq7_gradient(const GridDesc& g, const Corners& c, float out[3]) {
     int ip, im;
     float dv, ry;
     fd_axis(x, g.X, g.eps[0], r0, ip, im, dv, ry);
-    const float nx = div_rn(g.grid[lin3(g, ip, y, z)] - g.grid[lin3(g, im, y, z)], dv, ry);
+    const float nx = div_rn(ld(ip, y, z) - ld(im, y, z), dv, ry);
     fd_axis(y, g.Y, g.eps[1], r1, ip, im, dv, ry);
-    const float ny = div_rn(g.grid[lin3(g, x, ip, z)] - g.grid[lin3(g, x, im, z)], dv, ry);
+    const float ny = div_rn(ld(x, ip, z) - ld(x, im, z), dv, ry);
     fd_axis(z, g.Z, g.eps[2], r2, ip, im, dv, ry);
-    const float nz = div_rn(g.grid[lin3(g, x, y, ip)] - g.grid[lin3(g, x, y, im)], dv, ry);
+    const float nz = div_rn(ld(x, y, ip) - ld(x, y, im), dv, ry);
     t[k][0] = c.w[k] * nx;
     t[k][1] = c.w[k] * ny;
     t[k][2] = c.w[k] * nz;
@@ -230,6 +237,10 @@ VXN_DEV void eq7_gradient(const GridDesc& g, const Corners& c, float out[3]) {
 #pragma unroll
   for (int a = 0; a < 3; ++a)
     out[a] = (((t[0][a] + t[4][a]) + (t[1][a] + t[5][a])) + (t[2][a] + t[6][a])) + (t[3][a] + t[7][a]);
+}
+
+VXN_DEV void eq7_gradient(const GridDesc& g, const Corners& c, float out[3]) {
+  eq7_gradient(g, c, out, [&](int x, int y, int z) { return g.grid[lin3(g, x, y, z)]; });
 }
 
 // Scatter dL/dgrad (the un-normalised Eq. 7 gradient) into the grid gradient:
