@@ -1140,20 +1140,28 @@ __global__ void __launch_bounds__(256) hash_bin_write_kernel(RenderDev a, BinDev
   }
 }
 
-// One wavefront per bin.  LDS float atomics run at a few cycles per lane, so the bin is
-// summed with plain LDS read-add-write instead: in each round every pending half-record
-// writes its lane id to its entry's tag, and a lane adds a half whose tag reads back its
-// own id (one lane per entry per round; a lane whose two halves share the entry adds both
-// in turn); the others retry in the next round.  A single wavefront owns the bin's LDS
-// and its LDS instructions complete in order, so no update is lost.
+// kSumSplit wavefronts per bin, each owning a quarter of its entries (reading all the
+// bin's records and adding the halves that fall in its quarter): 4.5 KB of LDS per
+// wavefront instead of 18, so more wavefronts per SIMD hide the LDS latency of the tag
+// rounds (the records are read four times, mostly from L2).  LDS float atomics run at a
+// few cycles per lane, so a quarter is summed with plain LDS read-add-write instead: in
+// each round every pending half-record writes its lane id to its entry's tag, and a lane
+// adds a half whose tag reads back its own id (one lane per entry per round; a lane whose
+// two halves share the entry adds both in turn); the others retry in the next round.  A
+// single wavefront owns the quarter's LDS and its LDS instructions complete in order, so
+// no update is lost.
+constexpr int kSumSplit = 4;
+constexpr int kSumEntries = kBinEntries / kSumSplit;
+
 __global__ void __launch_bounds__(64) hash_bin_sum_kernel(BinDev b, float* __restrict__ grad_table, int64_t T) {
-  __shared__ float2 acc[kBinEntries];
-  __shared__ uint8_t tag[kBinEntries];
+  __shared__ float2 acc[kSumEntries];
+  __shared__ uint8_t tag[kSumEntries];
   const int lane = threadIdx.x;
-  const int bin = blockIdx.x;
+  const int bin = blockIdx.x / kSumSplit;
+  const int part = blockIdx.x - bin * kSumSplit;
   const int l = bin / b.nbl;
-  const int64_t base = (int64_t)(bin - l * b.nbl) << kBinBits;
-  const int ne = (int)min((int64_t)kBinEntries, T - base);
+  const int64_t base = ((int64_t)(bin - l * b.nbl) << kBinBits) + (int64_t)part * kSumEntries;
+  const int ne = (int)max((int64_t)0, min((int64_t)kSumEntries, T - base));
   for (int e = lane; e < ne; e += 64) acc[e] = make_float2(0.0f, 0.0f);
   wave_sync();
   const int32_t r0 = b.off[(int64_t)bin * b.nwg], r1 = b.off[(int64_t)(bin + 1) * b.nwg];
@@ -1181,8 +1189,9 @@ __global__ void __launch_bounds__(64) hash_bin_sum_kernel(BinDev b, float* __res
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const int e0 = (int)(r[u] & 0xFFFFu), e1 = (int)(r[u] >> 16);
-      bool p0 = k0 + u * 64 + lane < r1, p1 = p0;
+      const int e0 = (int)(r[u] & 0xFFFFu) - part * kSumEntries, e1 = (int)(r[u] >> 16) - part * kSumEntries;
+      const bool ok = k0 + u * 64 + lane < r1;
+      bool p0 = ok && e0 >= 0 && e0 < kSumEntries, p1 = ok && e1 >= 0 && e1 < kSumEntries;
       while (__ballot(p0 || p1) != 0) {
         if (p0) tag[e0] = (uint8_t)lane;
         if (p1) tag[e1] = (uint8_t)lane;
@@ -1745,7 +1754,7 @@ static int launch_render_nv(const vxn_render_args* p, void* stream) {
       VXN_LAUNCH_CHECK();
       hipLaunchKernelGGL(hash_bin_write_kernel, dim3(b.nwg), dim3(256), lds, s, a, b, dl);
       VXN_LAUNCH_CHECK();
-      hipLaunchKernelGGL(hash_bin_sum_kernel, dim3(b.nbins), dim3(64), 0, s, b, p->grad_table,
+      hipLaunchKernelGGL(hash_bin_sum_kernel, dim3(b.nbins * kSumSplit), dim3(64), 0, s, b, p->grad_table,
                          (int64_t)1 << p->log2_T);
       VXN_LAUNCH_CHECK();
     } else {
