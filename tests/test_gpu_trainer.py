@@ -256,3 +256,29 @@ def test_concurrent_grid_scatter_equals_serial():
     assert a[0] == b[0]
     assert np.array_equal(a[3], b[3]) and np.array_equal(a[4], b[4])
     assert rel_err(b[1], a[1]) < 1e-6 and rel_err(b[2], a[2]) < 1e-6
+
+
+@pytest.mark.parametrize("form", ["bf16", "pe"])
+def test_fused_trainer_other_forms(form):
+    """The trainer on the BF16 form (bf16 hash table with an FP32 master in the optimizer,
+    BF16 MLP on the matrix cores) and on the positional-encoding view embedding: steps run,
+    losses stay finite, and the model's own parameters (the bf16 table included) move."""
+    from voxneus_amd import _native as N
+    from voxneus_amd.trainer import FusedTrainer
+    from voxneus_amd.voxneus import VoxNeuS
+    torch.manual_seed(7)
+    if form == "bf16":
+        model = VoxNeuS(grid_res=32, hash_log2_size=14, hash_dtype=torch.bfloat16, mlp_mode=N.VXN_MLP_BF16_MFMA)
+    else:
+        model = VoxNeuS(grid_res=32, hash_log2_size=14, view_embed_type="pe")
+    tr = FusedTrainer(model, _trainer_config(), device=DEV)
+    emb0 = model.radiance_hash.encoder.embeddings.detach().float().cpu().clone()
+    w0 = model.radiance_mlp.net[0].weight.detach().cpu().clone()
+    b = _batch(256)
+    for step in range(4):
+        h = tr.read_metrics(tr.train_step(step, b))
+        assert np.isfinite(h["loss_total"]) and np.isfinite(h["loss_eik"]), (form, step)
+    assert model.radiance_hash.encoder.embeddings.data_ptr() == tr.step_engine.table.data_ptr()
+    assert not torch.equal(model.radiance_hash.encoder.embeddings.detach().float().cpu(), emb0)
+    assert not torch.equal(model.radiance_mlp.net[0].weight.detach().cpu(), w0)
+    assert tuple(model.sdf_grid.grid.shape) == (1, 1, 40, 40, 40)
