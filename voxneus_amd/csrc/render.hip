@@ -324,10 +324,17 @@ __global__ void __launch_bounds__(kRT, 2) render_kernel(RenderDev a) {
     // ---------------------------------------------------------------- sample forward
     const float px = ox + dx * mid, py = oy + dy * mid, pz = oz + dz * mid;
     const GsStencil gs = gs_stencil(G, px, py, pz);
-    const float sdf = gs_eval(G, gs);
     const Corners cn = corner_decomp(G, px, py, pz);
-    float gv[3];
-    grid_gradient(G, cn, gv);
+    float sdf, gv[3];
+    if (!G.analytical && brick_interior(G, cn)) {
+      Brick bk;
+      load_brick(G, cn, bk);
+      if (!gs_eval_brick(gs, cn, bk, sdf)) sdf = gs_eval(G, gs);
+      eq7_gradient_brick(G, cn, bk, gv);
+    } else {
+      sdf = gs_eval(G, gs);
+      grid_gradient(G, cn, gv);
+    }
     const float gr = norm3(gv);
     const float nrm[3] = {gv[0] / (gr + 1e-8f), gv[1] / (gr + 1e-8f), gv[2] / (gr + 1e-8f)};
     const float x0 = fminf(fmaxf((px - G.mn[0]) / G.den[0], 0.0f), 1.0f);

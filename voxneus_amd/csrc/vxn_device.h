@@ -243,6 +243,69 @@ VXN_DEV void eq7_gradient(const GridDesc& g, const Corners& c, float out[3]) {
   eq7_gradient(g, c, out, [&](int x, int y, int z) { return g.grid[lin3(g, x, y, z)]; });
 }
 
+// The 32 distinct values of an interior cell's Eq. 7 stencil, read as z-columns with
+// vector loads: col[2 ox + oy][j] = f(x0 + ox, y0 + oy, z0 - 1 + j), the cell's four
+// columns over four z; xf[2 side + oy][j] = f(x0 - 1 + 3 side, y0 + oy, z0 + j) and
+// yf[2 side + ox][j] = f(x0 + ox, y0 - 1 + 3 side, z0 + j), the face neighbours over two.
+struct Brick {
+  float col[4][4];
+  float xf[4][2];
+  float yf[4][2];
+};
+
+// Interior: the whole stencil lies inside the grid (c1 = c0 + 1 on every axis, and one
+// more vertex on each side), so every fd_axis divisor is 2 eps.
+VXN_DEV bool brick_interior(const GridDesc& g, const Corners& c) {
+  return c.c0[0] >= 1 && c.c0[0] + 2 <= g.X - 1 && c.c0[1] >= 1 && c.c0[1] + 2 <= g.Y - 1 && c.c0[2] >= 1 &&
+         c.c0[2] + 2 <= g.Z - 1;
+}
+
+VXN_DEV void load_brick(const GridDesc& g, const Corners& c, Brick& b) {
+  const int x0 = c.c0[0], y0 = c.c0[1], z0 = c.c0[2];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int ox = q >> 1, oy = q & 1;
+    __builtin_memcpy(b.col[q], g.grid + lin3(g, x0 + ox, y0 + oy, z0 - 1), 16);
+    __builtin_memcpy(b.xf[q], g.grid + lin3(g, x0 - 1 + 3 * (q >> 1), y0 + (q & 1), z0), 8);
+    __builtin_memcpy(b.yf[q], g.grid + lin3(g, x0 + (q & 1), y0 - 1 + 3 * (q >> 1), z0), 8);
+  }
+}
+
+// eq7_gradient on an interior cell from its brick: the same operations in the same order
+// (divisor 2 eps, reciprocal RN(1 / eps) / 2).
+VXN_DEV void eq7_gradient_brick(const GridDesc& g, const Corners& c, const Brick& b, float out[3]) {
+  const float r0 = 0.5f * (1.0f / g.eps[0]), r1 = 0.5f * (1.0f / g.eps[1]), r2 = 0.5f * (1.0f / g.eps[2]);
+  const float d0 = fmaxf(2.0f * g.eps[0], g.eps[0]), d1 = fmaxf(2.0f * g.eps[1], g.eps[1]),
+              d2 = fmaxf(2.0f * g.eps[2], g.eps[2]);
+  float t[8][3];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int ox = (k >> 2) & 1, oy = (k >> 1) & 1, oz = k & 1;
+    const float xp = ox ? b.xf[2 + oy][oz] : b.col[2 + oy][oz + 1];
+    const float xm = ox ? b.col[oy][oz + 1] : b.xf[oy][oz];
+    const float yp = oy ? b.yf[2 + ox][oz] : b.col[2 * ox + 1][oz + 1];
+    const float ym = oy ? b.col[2 * ox][oz + 1] : b.yf[ox][oz];
+    const float zp = b.col[2 * ox + oy][oz + 2], zm = b.col[2 * ox + oy][oz];
+    t[k][0] = c.w[k] * div_rn(xp - xm, d0, r0);
+    t[k][1] = c.w[k] * div_rn(yp - ym, d1, r1);
+    t[k][2] = c.w[k] * div_rn(zp - zm, d2, r2);
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+    out[a] = (((t[0][a] + t[4][a]) + (t[1][a] + t[5][a])) + (t[2][a] + t[6][a])) + (t[3][a] + t[7][a]);
+}
+
+// gs_eval from the brick when grid_sample's cell is the Eq. 7 cell with all 8 corners in
+// the grid (the same products and sum order); false otherwise.
+VXN_DEV bool gs_eval_brick(const GsStencil& s, const Corners& c, const Brick& b, float& out) {
+  if (s.x0 != c.c0[0] || s.y0 != c.c0[1] || s.z0 != c.c0[2] || s.valid != 0xFFu) return false;
+  float o = 0.0f;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) o = o + b.col[(k >> 1) & 3][(k & 1) + 1] * s.w[k];
+  out = o;
+  return true;
+}
+
 // Scatter dL/dgrad (the un-normalised Eq. 7 gradient) into the grid gradient:
 // d n_k,a / d f[i_p] = +1/dv, d f[i_m] = -1/dv, weighted by w_k.
 VXN_DEV void eq7_scatter(const GridDesc& g, const Corners& c, const float dg[3], float* grad_grid) {
