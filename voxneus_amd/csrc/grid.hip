@@ -147,7 +147,9 @@ __global__ void __launch_bounds__(256) voxel_corners_kernel(GridDesc g, const fl
 }
 
 // Lattice query: one thread per lattice point; consecutive threads walk z, so the
-// 8-corner and 24-neighbour stencil loads of a wavefront fall on a few cache lines.
+// 8-corner and 24-neighbour stencil loads of a wavefront fall on a few cache lines.  An
+// interior cell's stencil is read as 12 z-columns with vector loads (load_brick) and
+// evaluated with the generic code's operations (bit-identical results).
 // One lattice row (ix, iy) per work-group, z across the threads.  XCD-aware: work-groups
 // are dispatched to the 8 XCDs round-robin (work-group b on XCD b % 8), and XCD k walks
 // the rows of its own slab of ix in order, so each XCD's L2 streams one eighth of the
@@ -165,56 +167,18 @@ __global__ void __launch_bounds__(256) lattice_query_kernel(GridDesc g, int NX, 
     const int64_t i = row + iz;
     const float pz = lz0 + (float)iz * lsz;
     const GsStencil s = gs_stencil(g, px, py, pz);
-    sdf[i] = gs_eval(g, s);
     const Corners c = corner_decomp(g, px, py, pz);
-    float gv[3];
-    grid_gradient(g, c, gv);
-    const float rr = norm3(gv) + 1e-8f;
-    normal[3 * i] = gv[0] / rr;
-    normal[3 * i + 1] = gv[1] / rr;
-    normal[3 * i + 2] = gv[2] / rr;
-  }
-}
-
-// Eq. 7 lattice query with the row's stencil staged in LDS.  Along a lattice row x and y
-// are fixed, so both cell decompositions (the Eq. 7 corners and grid_sample's) have one x
-// and y cell per row: the work-group loads the 4 x 4 (x, y) grid rows around the Eq. 7
-// cell, every z, with coalesced loads, and the 8-corner and 24-neighbour reads of all its
-// points come from LDS (global memory for grid_sample's cell when it falls outside those
-// rows).  The arithmetic is that of the unstaged kernel.
-constexpr int kLatZMax = 1024;  // 16 rows of up to this many z values: 64 KB of LDS
-
-__global__ void __launch_bounds__(256) lattice_query_lds_kernel(GridDesc g, int NX, int NY, int NZ, float lx0,
-                                                                float ly0, float lz0, float lsx, float lsy,
-                                                                float lsz, float* __restrict__ sdf,
-                                                                float* __restrict__ normal) {
-  extern __shared__ float rows[];  // [4 x][4 y][Z]
-  const int xcd = blockIdx.x & 7, j = blockIdx.x >> 3;
-  const int slab = (NX + 7) / 8;
-  const int ix = xcd * slab + j / NY, iy = j - (j / NY) * NY;
-  if (ix >= NX || j / NY >= slab) return;
-  const float px = lx0 + (float)ix * lsx, py = ly0 + (float)iy * lsy;
-  const Corners c0 = corner_decomp(g, px, py, lz0);
-  const int bx = c0.c0[0] - 1, by = c0.c0[1] - 1;
-  const int Z = g.Z;
-  for (int t = threadIdx.x; t < 16 * Z; t += blockDim.x) {
-    const int r = t / Z, z = t - r * Z;
-    const int x = bx + (r >> 2), y = by + (r & 3);
-    if (x >= 0 && x < g.X && y >= 0 && y < g.Y) rows[t] = g.grid[lin3(g, x, y, z)];
-  }
-  __syncthreads();
-  auto lds = [&](int x, int y, int z) { return rows[((x - bx) * 4 + (y - by)) * Z + z]; };
-  auto glb = [&](int x, int y, int z) { return g.grid[lin3(g, x, y, z)]; };
-  const int64_t row = ((int64_t)ix * NY + iy) * NZ;
-  for (int iz = threadIdx.x; iz < NZ; iz += blockDim.x) {
-    const int64_t i = row + iz;
-    const float pz = lz0 + (float)iz * lsz;
-    const GsStencil s = gs_stencil(g, px, py, pz);
-    const bool in = s.x0 >= bx && s.x0 + 1 <= bx + 3 && s.y0 >= by && s.y0 + 1 <= by + 3;
-    sdf[i] = in ? gs_eval(s, lds) : gs_eval(s, glb);
-    const Corners c = corner_decomp(g, px, py, pz);
-    float gv[3];
-    eq7_gradient(g, c, gv, lds);
+    float sd, gv[3];
+    if (!g.analytical && brick_interior(g, c)) {
+      Brick bk;
+      load_brick(g, c, bk);
+      if (!gs_eval_brick(s, c, bk, sd)) sd = gs_eval(g, s);
+      eq7_gradient_brick(g, c, bk, gv);
+    } else {
+      sd = gs_eval(g, s);
+      grid_gradient(g, c, gv);
+    }
+    sdf[i] = sd;
     const float rr = norm3(gv) + 1e-8f;
     normal[3 * i] = gv[0] / rr;
     normal[3 * i + 1] = gv[1] / rr;
@@ -481,12 +445,8 @@ extern "C" int vxn_lattice_query(const float* grid, int X, int Y, int Z, const f
   const unsigned nb = (unsigned)(8 * ((NX + 7) / 8) * NY);
   const float sx = (lb[3] - lb[0]) / (float)(NX - 1), sy = (lb[4] - lb[1]) / (float)(NY - 1),
               sz = (lb[5] - lb[2]) / (float)(NZ - 1);
-  if (!g.analytical && Z <= kLatZMax)
-    hipLaunchKernelGGL(lattice_query_lds_kernel, dim3(nb), dim3(256), (size_t)16 * Z * sizeof(float),
-                       (hipStream_t)stream, g, NX, NY, NZ, lb[0], lb[1], lb[2], sx, sy, sz, sdf, normal);
-  else
-    hipLaunchKernelGGL(lattice_query_kernel, dim3(nb), dim3(256), 0, (hipStream_t)stream, g, NX, NY, NZ, lb[0],
-                       lb[1], lb[2], sx, sy, sz, sdf, normal);
+  hipLaunchKernelGGL(lattice_query_kernel, dim3(nb), dim3(256), 0, (hipStream_t)stream, g, NX, NY, NZ, lb[0], lb[1],
+                     lb[2], sx, sy, sz, sdf, normal);
   VXN_LAUNCH_CHECK();
   return 0;
 }
