@@ -293,38 +293,43 @@ class _RadianceMlpBF16(torch.autograd.Function):
     bf16 view values; db1 sums the unrounded output gradient."""
 
     @staticmethod
-    def forward(ctx, feats, sh, nrm, W0, b0, W1, b1):
+    def forward(ctx, feats, sh, nrm, W0, b0, W1, b1, rounding=True):
+        rnd = _bf16 if rounding else (lambda t: t)
         V = sh.shape[-1]
-        x = _bf16(torch.cat([feats, nrm], -1))
-        W0x = _bf16(torch.cat([W0[:, :32], W0[:, 32 + V:35 + V]], 1))
+        x = rnd(torch.cat([feats, nrm], -1))
+        W0x = rnd(torch.cat([W0[:, :32], W0[:, 32 + V:35 + V]], 1))
         hid = (b0 + sh @ W0[:, 32:32 + V].T) + x @ W0x.T
-        hb = _bf16(torch.relu(hid))
-        W1b = _bf16(W1)
+        hb = rnd(torch.relu(hid))
+        W1b = rnd(W1)
         rgb = torch.sigmoid(b1 + hb @ W1b.T)
         ctx.save_for_backward(x, W0x, hid > 0, hb, W1b, sh, rgb)
+        ctx.rounding = rounding
         return rgb
 
     @staticmethod
     def backward(ctx, g):
         x, W0x, mask, hb, W1b, sh, rgb = ctx.saved_tensors
+        rnd = _bf16 if ctx.rounding else (lambda t: t)
         shp = x.shape[:-1]
         dout = (g * (1.0 - rgb) * rgb).reshape(-1, 3)
-        doutb = _bf16(dout)
-        dh = _bf16((doutb @ W1b) * mask.reshape(-1, 64))
+        doutb = rnd(dout)
+        dh = rnd((doutb @ W1b) * mask.reshape(-1, 64))
         dx = dh @ W0x
         V = sh.shape[-1]
-        xf, shf = x.reshape(-1, 35), _bf16(sh.reshape(-1, V))
+        xf, shf = x.reshape(-1, 35), rnd(sh.reshape(-1, V))
         dW0 = torch.zeros(64, 35 + V, dtype=g.dtype)
         dW0[:, :32] = dh.T @ xf[:, :32]
         dW0[:, 32:32 + V] = dh.T @ shf
         dW0[:, 32 + V:35 + V] = dh.T @ xf[:, 32:35]
         return (dx[:, :32].reshape(*shp, 32), None, dx[:, 32:35].reshape(*shp, 3), dW0, dh.sum(0), doutb.T @ hb.reshape(-1, 64),
-                dout.sum(0))
+                dout.sum(0), None)
 
 
-def radiance_mlp_bf16(hash_feats, view_embed, normals, W0, b0, W1, b1):
-    """The BF16 form of ``radiance_mlp`` (see ``_RadianceMlpBF16``)."""
-    return _RadianceMlpBF16.apply(hash_feats, view_embed, normals, W0, b0, W1, b1)
+def radiance_mlp_bf16(hash_feats, view_embed, normals, W0, b0, W1, b1, rounding=True):
+    """The BF16 form of ``radiance_mlp`` (see ``_RadianceMlpBF16``).  ``rounding=False``
+    keeps every operand at the input precision: the same contractions as the BF16 form
+    with the bf16 rounding points removed, which must equal ``radiance_mlp``."""
+    return _RadianceMlpBF16.apply(hash_feats, view_embed, normals, W0, b0, W1, b1, rounding)
 
 
 def variance_s(v, s_max=300.0):

@@ -168,7 +168,17 @@ def test_render_forward_matches_reference(Fn, name):
     for k, v in (("alpha", alpha), ("transmittance", T), ("weights", w), ("rgb", rgb), ("depth", depth),
                  ("opacity", opac)):
         assert np.abs(v - g[k]).max() < 1e-5, k
-    assert np.abs(normal - g["normal"]).max() < 1e-4
+    # normal = normalize(sum w n) (volume_render.py:123-133): its rounding error grows as
+    # 1 / |sum w n|, so the unnormalised composite is held to 1e-5 relative, and the unit
+    # normal itself to 1e-5 on the rays that hit a surface (|sum w n| >= 0.1)
+    m = R.RefModel(sc.grid, sc.embeddings, **sc.mlp, hash_max_resolution=sc.hash_max_resolution)
+    with torch.no_grad():
+        ro = m.render(torch.from_numpy(g["rays_o"]), torch.from_numpy(g["rays_d"]), None, None,
+                      zs=tuple(torch.from_numpy(g[k]) for k in ("z_vals", "dists", "mid")))
+    nsum = (ro["weights"][..., None] * ro["normal_samples"]).sum(1).norm(dim=-1).numpy()
+    err = np.abs(normal - g["normal"]).max(-1)
+    assert (err * nsum).max() <= 1e-5 * nsum.max()
+    assert err[nsum >= 0.1].max() <= 1e-5
 
 
 def test_render_backward_matches_oracle_autograd(Fn):

@@ -1,6 +1,9 @@
 """The serving loop on the GPU: every request's result equals the standalone
 single-scene call on the same rays / points, and is the same alone and in a mix
-(bit-exact: per-ray work never depends on the rest of a batch)."""
+(bit-exact: per-ray work never depends on the rest of a batch); served renders and
+queries against the CPU oracle's render.py / mesh.py arithmetic (render.py:35-54:
+near/far from the unit sphere, deterministic sampler; mesh.py:80-114: query_sdf and
+query_normals)."""
 
 import numpy as np
 import pytest
@@ -91,3 +94,26 @@ def test_serving_realtime_admission_and_latency(scenes):
         assert r.done >= r.arrival and r.admitted >= r.arrival - 1e-3
         assert all(torch.isfinite(v).all() for v in r.outputs.values())
     assert srv.ray_pool.free_rows == 3000 and srv.pt_pool.free_rows == 3000
+
+
+@pytest.mark.parametrize("mode", [N.VXN_MLP_FP32, N.VXN_MLP_BF16_MFMA])
+def test_serving_matches_oracle(scenes, mode):
+    """Served renders on 48 sampled pixels per request against the oracle's render of the
+    same rays (FP32 form within 1e-5; BF16 form within 5e-3 RGB of the FP64 oracle) and
+    served queries on 300 points per request against query_sdf (bit-exact) and
+    query_normals (1e-6); the scenes cover FP32 and bf16 tables and the Eq. 4 gradient."""
+    from tests.test_gpu_configs import check_query, check_render
+
+    reqs = _requests(12, len(scenes), seed=13, sizes=((40, 30), (64, 48)))
+    srv = Server(scenes, mlp_mode=mode, ray_capacity=2048, point_capacity=2048, ray_arena=40000,
+                 point_arena=40000)
+    res = srv.serve(reqs, realtime=False)
+    kinds = set()
+    for i, (r, q) in enumerate(zip(res, reqs)):
+        sc = scenes[q.scene]
+        kinds.add((r.kind, q.scene))
+        if isinstance(q, RenderRequest):
+            check_render(sc, q, r, mode, 48, seed=70 + i)
+        else:
+            check_query(sc, q.points, r, 300, seed=80 + i)
+    assert {k for k, _ in kinds} == {"render", "query"}
