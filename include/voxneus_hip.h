@@ -349,6 +349,18 @@ int vxn_reg_eik_curv_vr(const float* grid, int X, int Y, int Z, const float* eps
                         float relax_radius, const uint32_t* bitmask, float w_eik, float w_curv, float* grad_grid,
                         double* losses, int32_t* counts, void* workspace, int64_t workspace_bytes, void* stream);
 
+/* No counterpart in the reference.  vxn_reg_eik_curv_vr for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving or training loops, harnesses
+ * that forbid hipMalloc): the same kernels and results, with the partial sums in a static
+ * device array of 176 KiB per GPU (held until the process ends) instead of a workspace.
+ * Grids of at most VXN_REG_NOALLOC_MAX_VOXELS vertices (512^3); larger ones return
+ * VXN_ERR_SHAPE.  Allocates nothing; every call on one GPU shares the array, so calls
+ * that overlap on one GPU (other streams, other threads) race.                          */
+#define VXN_REG_NOALLOC_MAX_VOXELS (1ll << 27)
+int vxn_reg_eik_curv_vr_noalloc(const float* grid, int X, int Y, int Z, const float* eps3, const float* box6,
+                                float relax_radius, const uint32_t* bitmask, float w_eik, float w_curv,
+                                float* grad_grid, double* losses, int32_t* counts, void* stream);
+
 /* s-gradient amplification on the device: grad_v *= k if grad_v < 0.  amplified
  * (device, nullable) receives 1 or 0.  Replaces amplify_s_gradient
  * (training/scheduler.py:14-33) without its host read-back.                           */

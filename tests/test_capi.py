@@ -117,3 +117,14 @@ def test_serving_validation_errors():
     assert L.vxn_serve_move(None, None, 8, 4, 4, 1, None, None, 3, None, None, 0, None, None, None) == -1
     assert L.vxn_serve_query(None, 1, None, None, 4, None, None, None, None) == -1
 
+
+
+def test_noalloc_regulariser_refuses_grids_past_its_scratch():
+    """The entry point without a workspace holds partial sums for grids of up to 2^27
+    vertices; a larger grid is refused on the host before anything is launched."""
+    L = N.lib()
+    p = ctypes.c_void_p(16)
+    eps = N.f3((2.0 / 599,) * 3)
+    rc = L.vxn_reg_eik_curv_vr_noalloc(p, 600, 600, 600, eps, N.box6((-1, -1, -1), (1, 1, 1)), 1.2, p, 0.1, 5e-4, p,
+                                       p, p, None)
+    assert rc == -2 and b"at most" in L.vxn_last_error()

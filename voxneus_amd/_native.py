@@ -177,6 +177,8 @@ _SIGS = {
     "vxn_reg_vr_workspace_bytes": (c_i32, [c_i32, c_i32, c_i32, ctypes.POINTER(c_i64)]),
     "vxn_reg_eik_curv_vr": (c_i32, [c_vp, c_i32, c_i32, c_i32, c_vp, c_vp, c_f, c_vp, c_f, c_f, c_vp, c_vp, c_vp,
                                     c_vp, c_i64, c_vp]),
+    "vxn_reg_eik_curv_vr_noalloc": (c_i32, [c_vp, c_i32, c_i32, c_i32, c_vp, c_vp, c_f, c_vp, c_f, c_f, c_vp, c_vp,
+                                            c_vp, c_vp]),
     "vxn_serve_scene_bytes": (c_i32, [ctypes.POINTER(c_i64)]),
     "vxn_serve_pack_scene": (c_i32, [ctypes.POINTER(SceneDesc), c_vp, c_i64]),
     "vxn_serve_render_workspace_bytes": (c_i32, [ctypes.POINTER(ServeRenderArgs), ctypes.POINTER(c_i64)]),

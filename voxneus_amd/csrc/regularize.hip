@@ -1,6 +1,8 @@
 // Active vertex set V_R (bitmask -> ascending compaction), fused closed-form eikonal +
 // curvature gradients with FP64 loss reduction, reference-shaped single regularisers,
 // and the device-side s-gradient amplification.
+#include <mutex>
+
 #include "vxn_host.h"
 
 namespace vxn {
@@ -650,18 +652,21 @@ extern "C" int vxn_reg_vr_workspace_bytes(int X, int Y, int Z, int64_t* bytes) {
   return 0;
 }
 
-extern "C" int vxn_reg_eik_curv_vr(const float* grid, int X, int Y, int Z, const float* eps3, const float* box6,
-                                   float relax_radius, const uint32_t* bitmask, float w_eik, float w_curv,
-                                   float* grad_grid, double* losses, int32_t* counts, void* workspace,
-                                   int64_t workspace_bytes, void* stream) {
+static int check_reg_vr(const float* grid, int X, int Y, int Z, const float* eps3, const float* box6,
+                        const uint32_t* bitmask, float* grad_grid, double* losses, int32_t* counts) {
   VXN_REQUIRE(grid && eps3 && box6 && bitmask && grad_grid && losses && counts, VXN_ERR_NULL, "null argument");
   VXN_REQUIRE(X >= 3 && Y >= 3 && Z >= 3, VXN_ERR_SHAPE, "grid resolution must be >= 3 per axis");
   VXN_REQUIRE((int64_t)X * Y * Z < ((int64_t)1 << 31), VXN_ERR_SHAPE, "grid too large for int32 V_R entries");
   for (int a = 0; a < 3; ++a) VXN_REQUIRE(eps3[a] > 0.0f, VXN_ERR_VALUE, "voxel size must be > 0");
-  const int nb = vr_nblocks(X, Y, Z);
-  const int64_t need = (int64_t)kGatherBlocks * 3 * 8 + (int64_t)nb * 2 * 4;
-  VXN_REQUIRE(workspace != nullptr && workspace_bytes >= need, VXN_ERR_WORKSPACE,
-              "regulariser workspace needs %lld bytes", (long long)need);
+  return 0;
+}
+
+// The regulariser pass over the V_R bitmask with its partial sums at `partial` (double
+// [kGatherBlocks][3]) and `cpart` (int32 [vr_nblocks][2]); every block of the count and
+// gather kernels overwrites its own partials, so the buffers need no clearing.
+static int launch_reg_vr(const float* grid, int X, int Y, int Z, const float* eps3, const float* box6,
+                         float relax_radius, const uint32_t* bitmask, float w_eik, float w_curv, float* grad_grid,
+                         double* losses, int32_t* counts, double* partial, int32_t* cpart, hipStream_t s) {
   GridDesc g = {};
   g.grid = grid;
   g.X = X;
@@ -675,9 +680,7 @@ extern "C" int vxn_reg_eik_curv_vr(const float* grid, int X, int Y, int Z, const
     v.e[a] = (box6[3 + a] - box6[a]) / ((float)res[a] - 1.0f);
   }
   v.relax = relax_radius;
-  double* partial = (double*)workspace;
-  int32_t* cpart = (int32_t*)((char*)workspace + (int64_t)kGatherBlocks * 3 * 8);
-  hipStream_t s = (hipStream_t)stream;
+  const int nb = vr_nblocks(X, Y, Z);
   hipLaunchKernelGGL(vr_count2_kernel, dim3(nb), dim3(kCompactThreads), 0, s, bitmask, vr_nwords(X, Y, Z), X, Y, Z, v,
                      cpart);
   VXN_LAUNCH_CHECK();
@@ -689,6 +692,74 @@ extern "C" int vxn_reg_eik_curv_vr(const float* grid, int X, int Y, int Z, const
   hipLaunchKernelGGL(reg_finish_kernel, dim3(1), dim3(64), 0, s, (const double*)partial, kGatherBlocks, losses);
   VXN_LAUNCH_CHECK();
   return 0;
+}
+
+extern "C" int vxn_reg_eik_curv_vr(const float* grid, int X, int Y, int Z, const float* eps3, const float* box6,
+                                   float relax_radius, const uint32_t* bitmask, float w_eik, float w_curv,
+                                   float* grad_grid, double* losses, int32_t* counts, void* workspace,
+                                   int64_t workspace_bytes, void* stream) {
+  if (int e = check_reg_vr(grid, X, Y, Z, eps3, box6, bitmask, grad_grid, losses, counts)) return e;
+  const int nb = vr_nblocks(X, Y, Z);
+  const int64_t need = (int64_t)kGatherBlocks * 3 * 8 + (int64_t)nb * 2 * 4;
+  VXN_REQUIRE(workspace != nullptr && workspace_bytes >= need, VXN_ERR_WORKSPACE,
+              "regulariser workspace needs %lld bytes", (long long)need);
+  double* partial = (double*)workspace;
+  int32_t* cpart = (int32_t*)((char*)workspace + (int64_t)kGatherBlocks * 3 * 8);
+  return launch_reg_vr(grid, X, Y, Z, eps3, box6, relax_radius, bitmask, w_eik, w_curv, grad_grid, losses, counts,
+                       partial, cpart, (hipStream_t)stream);
+}
+
+// ------------------------------------------------------------------------------------
+// The same pass without a workspace argument: the partials live in a file-scope device
+// array (one copy per GPU, held until the process ends).  Host code must not pass the
+// array's name to a kernel (it is a host-side stand-in, not a device address): its
+// address on each device is taken once with hipGetSymbolAddress, with that device
+// current, and kept in reg_scratch_ptr[].  Calls that overlap on one GPU share the array
+// and race on it.
+// ------------------------------------------------------------------------------------
+constexpr int kNoAllocMaxCountBlocks = (int)(((int64_t)VXN_REG_NOALLOC_MAX_VOXELS / 32 + kCompactThreads - 1) /
+                                             kCompactThreads);
+constexpr int64_t kNoAllocPartialBytes = (int64_t)kGatherBlocks * 3 * 8;
+constexpr int64_t kNoAllocScratchBytes = kNoAllocPartialBytes + (int64_t)kNoAllocMaxCountBlocks * 2 * 4;
+static_assert(kNoAllocScratchBytes == 180224, "176 KiB of scratch per GPU");
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex reg_scratch_mu;
+static void* reg_scratch_ptr[kMaxDevices];
+
+// The device address of `scratch` on the current device (looked up once per device).
+static int reg_scratch(void** out) {
+  int dev = -1;
+  VXN_HIP(hipGetDevice(&dev));
+  VXN_REQUIRE(dev >= 0 && dev < kMaxDevices, VXN_ERR_VALUE, "device %d is beyond the %d supported", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lock(reg_scratch_mu);
+  if (reg_scratch_ptr[dev] == nullptr) {
+    void* p = nullptr;
+    VXN_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    VXN_REQUIRE(p != nullptr, VXN_ERR_HIP, "no device address for the regulariser scratch on device %d", dev);
+    reg_scratch_ptr[dev] = p;
+  }
+  *out = reg_scratch_ptr[dev];
+  return 0;
+}
+
+extern "C" int vxn_reg_eik_curv_vr_noalloc(const float* grid, int X, int Y, int Z, const float* eps3,
+                                           const float* box6, float relax_radius, const uint32_t* bitmask,
+                                           float w_eik, float w_curv, float* grad_grid, double* losses,
+                                           int32_t* counts, void* stream) {
+  if (int e = check_reg_vr(grid, X, Y, Z, eps3, box6, bitmask, grad_grid, losses, counts)) return e;
+  VXN_REQUIRE((int64_t)X * Y * Z <= (int64_t)VXN_REG_NOALLOC_MAX_VOXELS, VXN_ERR_SHAPE,
+              "grid has %lld vertices; the entry point without a workspace takes at most %lld "
+              "(use vxn_reg_eik_curv_vr with a workspace)",
+              (long long)X * Y * Z, (long long)VXN_REG_NOALLOC_MAX_VOXELS);
+  void* p = nullptr;
+  if (int e = reg_scratch(&p)) return e;
+  double* partial = (double*)p;
+  int32_t* cpart = (int32_t*)((char*)p + kNoAllocPartialBytes);
+  return launch_reg_vr(grid, X, Y, Z, eps3, box6, relax_radius, bitmask, w_eik, w_curv, grad_grid, losses, counts,
+                       partial, cpart, (hipStream_t)stream);
 }
 
 extern "C" int vxn_eikonal_grad(const float* grid, int X, int Y, int Z, const int64_t* active, int64_t V,
