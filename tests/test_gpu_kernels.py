@@ -213,9 +213,9 @@ def test_render_backward_matches_oracle_autograd(Fn):
 
 @pytest.mark.parametrize("log2_T", [19, 21, 22])
 def test_render_backward_hash_gradient_large_tables(Fn, log2_T):
-    """The hash-table gradient with 2^19 entries per level (binned: 4096 bins of 2048
-    entries), 2^21 (16384 bins, the binned path's limit: 64 KB of LDS counters) and 2^22
-    (past it: the atomic scatter)."""
+    """The hash-table gradient with 2^19 entries per level (binned: 256 bins of 2048
+    entries per level), 2^21 (1024 bins per level, the binned path's limit: 4 KB of LDS
+    counters per wavefront) and 2^22 (past it: the atomic scatter)."""
     g = load_golden("step_c1.npz")
     sc = scenes.config1_scene(seed=0)
     emb = (torch.rand((16, 1 << log2_T, 2), generator=torch.Generator().manual_seed(log2_T)) - 0.5) * 0.2

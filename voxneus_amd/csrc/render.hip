@@ -977,32 +977,43 @@ __global__ void __launch_bounds__(256) hash_scatter_kernel(RenderDev a, int dedu
 // ------------------------------------------------------------------------------------
 // Binned hash-table gradient.  Device-scope float atomics sustain about 21 G requests/s
 // however they are spread, which bounds hash_scatter_kernel.  Here the same records are
-// binned by table region instead of added in place: (1) each work-group counts, per bin
-// of 2048 consecutive entries of one level, the records its samples produce (LDS
-// histogram); (2) an exclusive scan over [bin][work-group] gives every work-group its own
-// contiguous range in every bin; (3) the work-groups write their records there; (4) one
-// wavefront per bin sums the bin's records in LDS and adds the sums to grad_table.  A
-// record holds the two corners of a cell that differ only in x (their entries differ by
-// x ^ (x + 1) in the low bits, so they share a bin unless x ends in eleven 1 bits; then
-// each corner gets a record of its own, its second half adding zero to the same entry):
-// two entries within the bin and both features' weighted gradients of each.  Per live
-// sample and level that is 4 records, with runs of consecutive samples in one cell summed
-// first on the levels below `dedup_levels`, as in the atomic scatter.  Above them, where
-// runs are short, a record is factored into one 16-byte store: the entries and flags in
-// the first word, then x's cell fraction fx and the two features' gradients times the
-// pair's y-z weight (the x corner takes (1 - fx) of them, the x + 1 corner fx), so those
-// records need no separate entry array.
+// binned by table region instead of added in place: (1) each wavefront counts, for one
+// level and 1024 consecutive samples, the records it produces per bin of 2048 consecutive
+// entries (LDS counters of its own), one row of counts per wavefront; (2) a column scan
+// over [level][row][bin] turns the counts into each row's first position in every bin;
+// (3) the wavefronts write their records there; (4) four wavefronts per bin, each owning a
+// quarter of its entries, sum the bin's records in LDS and add the sums to grad_table
+// without float atomics.  A record holds the
+// two corners of a cell that differ only in x (their entries differ by x ^ (x + 1) in the
+// low bits, so they share a bin unless x ends in eleven 1 bits; then each corner gets a
+// record of its own, its second half adding zero to the same entry): two entries within
+// the bin and both features' weighted gradients of each.  Per live sample and level that
+// is 4 records, with runs of consecutive samples in one cell summed first on the levels
+// below `dedup_levels`, as in the atomic scatter.  Above them, where runs are short, a
+// record is factored into one 16-byte store: the entries and flags in the first word, then
+// x's cell fraction fx and the two features' gradients times the pair's y-z weight (the x
+// corner takes (1 - fx) of them, the x + 1 corner fx), so those records need no separate
+// entry array.  Every position comes from a wavefront's own row and LDS counters, so the
+// records' order in a bin, and the gradient, are the same on every run.
 // ------------------------------------------------------------------------------------
 constexpr int kBinBits = 11;
-constexpr int kBinEntries = 1 << kBinBits;  // 16 KB of float2 sums + 2 KB of tags in LDS
-constexpr int kBinChunks = 4;               // 256-sample chunks per work-group (passes 1, 3)
-constexpr int kMaxBins = 16384;             // LDS counters of passes 1 and 3 (nbins x 4 B)
+constexpr int kBinEntries = 1 << kBinBits;
+constexpr int kBinTileWaves = 4;            // wavefronts per work-group of the count / write passes
+constexpr int kBinWaveChunks = 16;          // 64-sample chunks per wavefront
+constexpr int kBinWaveSamples = 64 * kBinWaveChunks;
+constexpr int kBinTile = kBinWaveSamples * kBinTileWaves;  // 4096 samples per work-group
+constexpr int kMaxBinsPerLevel = 1024;      // LDS counters per wavefront: 4 KB (tables up to 2^21)
 
 struct BinDev {
   int nbl;          // bins per level: ceil(T / kBinEntries)
-  int nbins;        // L * nbl
-  int nwg;          // work-groups of passes 1 and 3
-  int32_t* off;     // [nbins * nwg + 1]: counts, then their exclusive scan and total
+  int nbins;        // L * nbl, in (level, bin) order
+  int ntile;        // sample tiles of kBinTile: work-groups per level of passes 1 and 3
+  int nrows;        // count rows per level: one per wavefront, ntile * kBinTileWaves
+  int nchunk;       // row chunks of the column scan
+  int32_t* cnt;     // [L][nrows][nbl]: each wavefront's record count per bin, then (scanned in
+                    // place) the position of its first record in each bin
+  int32_t* csum;    // [L][nchunk][nbl]: column sums of row chunks, then their exclusive scan
+  int32_t* base;    // [nbins + 1]: first record of each bin; base[nbins] = all records
   uint32_t* ridx;   // [records] entries within the bin: x corner | (x + 1 corner) << 16
   float4* rval;     // [records] dL/dfeature 0, 1 of the x corner, then of the x + 1 corner
   int fact_level;   // first level with factored records: rval only, {bits, fx, g0, g1} with
@@ -1011,16 +1022,26 @@ struct BinDev {
 constexpr uint32_t kFactFirst = 1u << 22;   // split pair: only the x corner, (1 - fx) g
 constexpr uint32_t kFactSecond = 2u << 22;  // split pair: only the x + 1 corner, fx g
 
-// Walks the hash levels of sample i (lanes of a wavefront hold consecutive samples) and
-// calls level(l, tail, cell, v) with every lane of the wavefront, on each level with a live
-// lane: tail marks the lanes that emit records (the last lane of each run of samples in
-// one cell on the levels below `dedup_levels`, every live lane above), v their 8 corners'
-// weighted gradients (run sums) and d0, d1 the level's feature gradients of the lane's own
-// sample.  VALUES = false leaves v unset: only the tails are needed.
+// Level l of the 64 consecutive samples a wavefront holds (lane = sample i): calls
+// emit(tail, cell, v, d0, d1) with every lane of the wavefront when some lane is live:
+// tail marks the lanes that emit records (the last lane of each run of samples in one
+// cell below `dedup_levels`, every live lane above), v their 8 corners' weighted
+// gradients (run sums) and d0, d1 the level's feature gradients of the lane's own sample.
+// VALUES = false leaves v unset: only the tails are needed.
 template <bool VALUES, class F>
-VXN_DEV void hash_levels(const RenderDev& a, int64_t i, int lane, int dedup_levels, F&& level) {
+VXN_DEV void hash_level_records(const RenderDev& a, int64_t i, int lane, int l, int dedup_levels, F&& emit) {
   const int64_t NS = a.R * (int64_t)a.S;
   const bool in = i < NS;
+  const float lw = a.h.lw[l];
+  if (lw == 0.0f) return;
+  float d0 = in ? a.gfeat[(2 * l) * NS + i] : 0.0f;
+  float d1 = in ? a.gfeat[(2 * l + 1) * NS + i] : 0.0f;
+  if (lw < 1.0f) {
+    d0 = d0 * lw;
+    d1 = d1 * lw;
+  }
+  const bool live = d0 != 0.0f || d1 != 0.0f;
+  if (__ballot(live) == 0) return;
   float x0 = 0.0f, x1 = 0.0f, x2 = 0.0f;
   if (in) {
     float px, py, pz;
@@ -1029,41 +1050,29 @@ VXN_DEV void hash_levels(const RenderDev& a, int64_t i, int lane, int dedup_leve
     x1 = fminf(fmaxf((py - a.g.mn[1]) / a.g.den[1], 0.0f), 1.0f);
     x2 = fminf(fmaxf((pz - a.g.mn[2]) / a.g.den[2], 0.0f), 1.0f);
   }
-  for (int l = 0; l < a.h.L; ++l) {
-    float d0 = in ? a.gfeat[(2 * l) * NS + i] : 0.0f;
-    float d1 = in ? a.gfeat[(2 * l + 1) * NS + i] : 0.0f;
-    const float lw = a.h.lw[l];
-    if (lw == 0.0f) continue;
-    if (lw < 1.0f) {
-      d0 = d0 * lw;
-      d1 = d1 * lw;
-    }
-    const bool live = d0 != 0.0f || d1 != 0.0f;
-    if (__ballot(live) == 0) continue;
-    const HashCell c = hash_cell(x0, x1, x2, a.h.res[l], a.h.mask);
-    float v[16];
-    if constexpr (VALUES) {
+  const HashCell c = hash_cell(x0, x1, x2, a.h.res[l], a.h.mask);
+  float v[16];
+  if constexpr (VALUES) {
 #pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        v[2 * k] = d0 * c.w[k];
-        v[2 * k + 1] = d1 * c.w[k];
-      }
+    for (int k = 0; k < 8; ++k) {
+      v[2 * k] = d0 * c.w[k];
+      v[2 * k + 1] = d1 * c.w[k];
     }
-    bool tail = live;
-    if (l < dedup_levels) {
-      const float res = a.h.res[l];
-      const uint64_t cx = (uint64_t)(int)floorf(x0 * res), cy = (uint64_t)(int)floorf(x1 * res),
-                     cz = (uint64_t)(int)floorf(x2 * res);
-      const uint64_t key = live ? ((cx << 42) | (cy << 21) | cz) : (~0ull - (uint64_t)lane);
-      if constexpr (VALUES) {
-        tail = seg_sum<16>(key, v, lane) && live;
-      } else {
-        const uint64_t kn = __shfl_down(key, 1, 64);
-        tail = (lane == 63 || kn != key) && live;
-      }
-    }
-    level(l, tail, c, v, d0, d1);
   }
+  bool tail = live;
+  if (l < dedup_levels) {
+    const float res = a.h.res[l];
+    const uint64_t cx = (uint64_t)(int)floorf(x0 * res), cy = (uint64_t)(int)floorf(x1 * res),
+                   cz = (uint64_t)(int)floorf(x2 * res);
+    const uint64_t key = live ? ((cx << 42) | (cy << 21) | cz) : (~0ull - (uint64_t)lane);
+    if constexpr (VALUES) {
+      tail = seg_sum<16>(key, v, lane) && live;
+    } else {
+      const uint64_t kn = __shfl_down(key, 1, 64);
+      tail = (lane == 63 || kn != key) && live;
+    }
+  }
+  emit(tail, c, v, d0, d1);
 }
 
 VXN_DEV void wave_sync() {
@@ -1072,41 +1081,53 @@ VXN_DEV void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Passes 1 and 3 run one work-group per (level, 4096-sample tile), level-major; each of its
+// four wavefronts walks 1024 consecutive samples in 16 chunks and owns one row of counters
+// (LDS, then cnt[level][row]).  A wavefront's positions in a bin come from its own row of
+// the scan and from LDS atomics that only its own instructions touch, so every record's
+// position - and the order in which pass 4 adds it - is the same on every run.
 __global__ void __launch_bounds__(256) hash_bin_count_kernel(RenderDev a, BinDev b, int dedup_levels) {
-  extern __shared__ int32_t cnt[];  // [nbins]
-  for (int k = threadIdx.x; k < b.nbins; k += 256) cnt[k] = 0;
-  __syncthreads();
-  const int lane = threadIdx.x & 63;
-  for (int c = 0; c < kBinChunks; ++c) {
-    const int64_t i = ((int64_t)blockIdx.x * kBinChunks + c) * 256 + threadIdx.x;
-    hash_levels<false>(a, i, lane, dedup_levels,
-                       [&](int l, bool tail, const HashCell& cell, const float(&)[16], float, float) {
+  extern __shared__ int32_t cnt_lds[];  // [kBinTileWaves][nbl]
+  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int l = (int)(blockIdx.x / (unsigned)b.ntile), tile = (int)(blockIdx.x - (unsigned)l * b.ntile);
+  int32_t* mine = cnt_lds + wid * b.nbl;
+  for (int k = lane; k < b.nbl; k += 64) mine[k] = 0;
+  wave_sync();
+  const int64_t i0 = (int64_t)tile * kBinTile + (int64_t)wid * kBinWaveSamples;
+  for (int c = 0; c < kBinWaveChunks; ++c) {
+    hash_level_records<false>(a, i0 + c * 64 + lane, lane, l, dedup_levels,
+                              [&](bool tail, const HashCell& cell, const float(&)[16], float, float) {
       if (tail) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const int ba = (int)(cell.idx[q] >> kBinBits), bb = (int)(cell.idx[q + 4] >> kBinBits);
-          atomicAdd(&cnt[l * b.nbl + ba], 1);
-          if (bb != ba) atomicAdd(&cnt[l * b.nbl + bb], 1);
+          atomicAdd(&mine[ba], 1);
+          if (bb != ba) atomicAdd(&mine[bb], 1);
         }
       }
     });
   }
-  __syncthreads();
-  for (int k = threadIdx.x; k < b.nbins; k += 256) b.off[(int64_t)k * b.nwg + blockIdx.x] = cnt[k];
+  wave_sync();
+  int32_t* out = b.cnt + ((int64_t)l * b.nrows + (int64_t)tile * kBinTileWaves + wid) * b.nbl;
+  for (int k = lane; k < b.nbl; k += 64) out[k] = mine[k];
 }
 
-// Records straight to their reserved positions (any number of bins per level).
+// Records straight to the wavefront's reserved positions in each bin.
 __global__ void __launch_bounds__(256) hash_bin_write_kernel(RenderDev a, BinDev b, int dedup_levels) {
-  extern __shared__ int32_t pos[];  // [nbins]
-  for (int k = threadIdx.x; k < b.nbins; k += 256) pos[k] = b.off[(int64_t)k * b.nwg + blockIdx.x];
-  __syncthreads();
-  const int lane = threadIdx.x & 63;
-  for (int c = 0; c < kBinChunks; ++c) {
-    const int64_t i = ((int64_t)blockIdx.x * kBinChunks + c) * 256 + threadIdx.x;
-    hash_levels<true>(a, i, lane, dedup_levels,
-                      [&](int l, bool tail, const HashCell& cell, const float(&v)[16], float d0, float d1) {
-      if (tail && l >= b.fact_level) {
-        constexpr uint32_t m = kBinEntries - 1;
+  extern __shared__ int32_t pos_lds[];  // [kBinTileWaves][nbl]
+  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int l = (int)(blockIdx.x / (unsigned)b.ntile), tile = (int)(blockIdx.x - (unsigned)l * b.ntile);
+  int32_t* pos = pos_lds + wid * b.nbl;
+  const int32_t* row = b.cnt + ((int64_t)l * b.nrows + (int64_t)tile * kBinTileWaves + wid) * b.nbl;
+  for (int k = lane; k < b.nbl; k += 64) pos[k] = row[k];
+  wave_sync();
+  const bool fact = l >= b.fact_level;
+  const int64_t i0 = (int64_t)tile * kBinTile + (int64_t)wid * kBinWaveSamples;
+  for (int c = 0; c < kBinWaveChunks; ++c) {
+    hash_level_records<true>(a, i0 + c * 64 + lane, lane, l, dedup_levels,
+                             [&](bool tail, const HashCell& cell, const float(&v)[16], float d0, float d1) {
+      constexpr uint32_t m = kBinEntries - 1;
+      if (tail && fact) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const uint32_t ia = cell.idx[q], ib = cell.idx[q + 4];
@@ -1115,29 +1136,28 @@ __global__ void __launch_bounds__(256) hash_bin_write_kernel(RenderDev a, BinDev
           const float wz = (q & 1) ? cell.dz : 1.0f - cell.dz;
           const float wyz = wy * wz;
           const float g0 = d0 * wyz, g1 = d1 * wyz;
-          const int32_t p = atomicAdd(&pos[l * b.nbl + ba], 1);
+          const int32_t p = atomicAdd(&pos[ba], 1);
           if (bb == ba) {
             b.rval[p] = make_float4(__uint_as_float((ia & m) | ((ib & m) << 11)), cell.dx, g0, g1);
           } else {
             b.rval[p] = make_float4(__uint_as_float((ia & m) * 0x801u | kFactFirst), cell.dx, g0, g1);
-            const int32_t p2 = atomicAdd(&pos[l * b.nbl + bb], 1);
+            const int32_t p2 = atomicAdd(&pos[bb], 1);
             b.rval[p2] = make_float4(__uint_as_float((ib & m) * 0x801u | kFactSecond), cell.dx, g0, g1);
           }
         }
       } else if (tail) {
-        constexpr uint32_t m = kBinEntries - 1;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const uint32_t ia = cell.idx[q], ib = cell.idx[q + 4];
           const int ba = (int)(ia >> kBinBits), bb = (int)(ib >> kBinBits);
-          const int32_t p = atomicAdd(&pos[l * b.nbl + ba], 1);
+          const int32_t p = atomicAdd(&pos[ba], 1);
           if (bb == ba) {
             b.ridx[p] = (ia & m) | ((ib & m) << 16);
             b.rval[p] = make_float4(v[2 * q], v[2 * q + 1], v[2 * q + 8], v[2 * q + 9]);
           } else {
             b.ridx[p] = (ia & m) * 0x10001u;
             b.rval[p] = make_float4(v[2 * q], v[2 * q + 1], 0.0f, 0.0f);
-            const int32_t p2 = atomicAdd(&pos[l * b.nbl + bb], 1);
+            const int32_t p2 = atomicAdd(&pos[bb], 1);
             b.ridx[p2] = (ib & m) * 0x10001u;
             b.rval[p2] = make_float4(v[2 * q + 8], v[2 * q + 9], 0.0f, 0.0f);
           }
@@ -1147,31 +1167,136 @@ __global__ void __launch_bounds__(256) hash_bin_write_kernel(RenderDev a, BinDev
   }
 }
 
-// kSumSplit wavefronts per bin, each owning a quarter of its entries (reading all the
-// bin's records and adding the halves that fall in its quarter): 4.5 KB of LDS per
-// wavefront instead of 18, so more wavefronts per SIMD hide the LDS latency of the tag
-// rounds (the records are read four times, mostly from L2).  LDS float atomics run at a
-// few cycles per lane, so a quarter is summed with plain LDS read-add-write instead: in
-// each round every pending half-record writes its lane id to its entry's tag, and a lane
-// adds a half whose tag reads back its own id (one lane per entry per round; a lane whose
-// two halves share the entry adds both in turn); the others retry in the next round.  A
-// single wavefront owns the quarter's LDS and its LDS instructions complete in order, so
-// no update is lost.
+// Pass 2, the scan that turns the count rows into positions: a bin's records are stored in
+// row order (rows of one level in tile, then wavefront order), bins in (level, bin) order.
+// Column chunk sums, an exclusive scan of each column's chunks (one thread per column),
+// one work-group's scan of the column totals (base), then each chunk writes its rows'
+// positions in place.
+constexpr int kColChunk = 64;    // rows per chunk
+constexpr int kColThreads = 1024;
+
+constexpr int kColBatch = 16;  // rows loaded ahead
+
+__global__ void __launch_bounds__(256) bin_col_sums_kernel(BinDev b) {
+  const int bin = blockIdx.x * 256 + threadIdx.x, ch = blockIdx.y, l = blockIdx.z;
+  if (bin >= b.nbl) return;
+  const int r1 = min(b.nrows, (ch + 1) * kColChunk);
+  const int32_t* c = b.cnt + (int64_t)l * b.nrows * b.nbl + bin;
+  int32_t s = 0;
+  for (int r0 = ch * kColChunk; r0 < r1; r0 += kColBatch) {
+    int32_t v[kColBatch];
+#pragma unroll
+    for (int j = 0; j < kColBatch; ++j) v[j] = r0 + j < r1 ? c[(int64_t)(r0 + j) * b.nbl] : 0;
+#pragma unroll
+    for (int j = 0; j < kColBatch; ++j) s += v[j];
+  }
+  b.csum[((int64_t)l * b.nchunk + ch) * b.nbl + bin] = s;
+}
+
+// One thread per column (level, bin): exclusive scan of the column's chunk sums in place,
+// the column total into base[column].
+__global__ void __launch_bounds__(256) bin_col_scan_kernel(BinDev b) {
+  const int col = blockIdx.x * 256 + threadIdx.x;
+  if (col >= b.nbins) return;
+  const int l = col / b.nbl, bin = col - l * b.nbl;
+  int32_t* cs = b.csum + (int64_t)l * b.nchunk * b.nbl + bin;
+  int32_t run = 0;
+  for (int c0 = 0; c0 < b.nchunk; c0 += kColBatch) {
+    int32_t v[kColBatch];
+#pragma unroll
+    for (int j = 0; j < kColBatch; ++j) v[j] = c0 + j < b.nchunk ? cs[(int64_t)(c0 + j) * b.nbl] : 0;
+#pragma unroll
+    for (int j = 0; j < kColBatch; ++j) {
+      if (c0 + j < b.nchunk) cs[(int64_t)(c0 + j) * b.nbl] = run;
+      run += v[j];
+    }
+  }
+  b.base[col] = run;
+}
+
+// One work-group: exclusive scan of the column totals in base[] in place (thread t takes
+// `per` consecutive columns), the grand total at base[nbins].
+__global__ void __launch_bounds__(kColThreads) bin_base_scan_kernel(BinDev b) {
+  __shared__ int32_t ws[kColThreads / 64];
+  const int per = (b.nbins + kColThreads - 1) / kColThreads;
+  const int c0 = threadIdx.x * per;
+  int32_t tot = 0;
+  for (int k = 0; k < per; ++k)
+    if (c0 + k < b.nbins) tot += b.base[c0 + k];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int32_t inc = tot;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int32_t u = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += u;
+  }
+  if (lane == 63) ws[wid] = inc;
+  __syncthreads();
+  int32_t pre = 0, all = 0;
+  for (int w = 0; w < kColThreads / 64; ++w) {
+    if (w < wid) pre += ws[w];
+    all += ws[w];
+  }
+  int32_t run = pre + inc - tot;
+  for (int k = 0; k < per; ++k) {
+    if (c0 + k >= b.nbins) break;
+    const int32_t t = b.base[c0 + k];
+    b.base[c0 + k] = run;
+    run += t;
+  }
+  if (threadIdx.x == 0) b.base[b.nbins] = all;
+}
+
+__global__ void __launch_bounds__(256) bin_col_positions_kernel(BinDev b) {
+  const int bin = blockIdx.x * 256 + threadIdx.x, ch = blockIdx.y, l = blockIdx.z;
+  if (bin >= b.nbl) return;
+  const int r1 = min(b.nrows, (ch + 1) * kColChunk);
+  int32_t* c = b.cnt + (int64_t)l * b.nrows * b.nbl + bin;
+  int32_t run = b.base[l * b.nbl + bin] + b.csum[((int64_t)l * b.nchunk + ch) * b.nbl + bin];
+  for (int r0 = ch * kColChunk; r0 < r1; r0 += kColBatch) {
+    int32_t v[kColBatch];
+#pragma unroll
+    for (int j = 0; j < kColBatch; ++j) v[j] = r0 + j < r1 ? c[(int64_t)(r0 + j) * b.nbl] : 0;
+#pragma unroll
+    for (int j = 0; j < kColBatch; ++j) {
+      if (r0 + j < r1) c[(int64_t)(r0 + j) * b.nbl] = run;
+      run += v[j];
+    }
+  }
+}
+
+// Pass 4: kSumSplit wavefronts per bin, each owning a quarter of its entries (4.6 KB of LDS
+// with its tags): each reads all the bin's records and adds the halves that fall in its
+// quarter.  The four wavefronts of a bin are work-groups b, b + 8, b + 16, b + 24 of one
+// group of 32, which the dispatcher places on the same XCD (work-group b runs on XCD
+// b % 8), so the bin's records come from HBM once and from that XCD's L2 three times.
+// LDS float atomics run at a few cycles per lane, so a quarter is summed with plain LDS
+// read-add-write instead: in each round every pending half-record writes its lane id to
+// its entry's tag, and a lane adds a half whose tag reads back its own id (one lane per
+// entry per round; a lane whose two halves share the entry adds both, in order); the
+// others retry in the next round.  A single wavefront owns the quarter's LDS and its LDS
+// instructions complete in order, so no update is lost, and the adds to an entry follow
+// the records' order, which the count and write passes fix.  A lane with nothing to add in
+// a round reads and writes its own dummy slot (entry kSumEntries + lane), so a round is
+// straight-line LDS code: both tag reads, then both accumulator reads, issued together.
 constexpr int kSumSplit = 4;
 constexpr int kSumEntries = kBinEntries / kSumSplit;
 
 __global__ void __launch_bounds__(64) hash_bin_sum_kernel(BinDev b, float* __restrict__ grad_table, int64_t T) {
-  __shared__ float2 acc[kSumEntries];
-  __shared__ uint8_t tag[kSumEntries];
+  __shared__ float2 acc[kSumEntries + 64];
+  __shared__ uint8_t tag[kSumEntries + 64];
   const int lane = threadIdx.x;
-  const int bin = blockIdx.x / kSumSplit;
-  const int part = blockIdx.x - bin * kSumSplit;
+  const int xcd = (int)(blockIdx.x & 7), k = (int)(blockIdx.x >> 3);
+  const int part = k & (kSumSplit - 1);
+  const int bin = (k / kSumSplit) * 8 + xcd;
+  if (bin >= b.nbins) return;
   const int l = bin / b.nbl;
   const int64_t base = ((int64_t)(bin - l * b.nbl) << kBinBits) + (int64_t)part * kSumEntries;
   const int ne = (int)max((int64_t)0, min((int64_t)kSumEntries, T - base));
   for (int e = lane; e < ne; e += 64) acc[e] = make_float2(0.0f, 0.0f);
+  const int dummy = kSumEntries + lane;
   wave_sync();
-  const int32_t r0 = b.off[(int64_t)bin * b.nwg], r1 = b.off[(int64_t)(bin + 1) * b.nwg];
+  const int32_t r0 = b.base[bin], r1 = b.base[bin + 1];
   const bool fact = l >= b.fact_level;
   constexpr int U = 8;  // 64-record groups loaded ahead
   for (int32_t k0 = r0; k0 < r1; k0 += U * 64) {
@@ -1179,9 +1304,9 @@ __global__ void __launch_bounds__(64) hash_bin_sum_kernel(BinDev b, float* __res
     float4 v[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const int32_t k = k0 + u * 64 + lane;
+      const int32_t kk = k0 + u * 64 + lane;
       if (fact) {
-        const float4 f = k < r1 ? b.rval[k] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        const float4 f = kk < r1 ? b.rval[kk] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         const uint32_t bits = __float_as_uint(f.x);
         const float ox = 1.0f - f.y;
         const float a0 = ox * f.z, a1 = ox * f.w, c0 = f.y * f.z, c1 = f.y * f.w;
@@ -1190,35 +1315,37 @@ __global__ void __launch_bounds__(64) hash_bin_sum_kernel(BinDev b, float* __res
         v[u] = fl == 0u ? make_float4(a0, a1, c0, c1)
                         : (fl == kFactFirst ? make_float4(a0, a1, 0.0f, 0.0f) : make_float4(c0, c1, 0.0f, 0.0f));
       } else {
-        r[u] = k < r1 ? b.ridx[k] : 0u;
-        v[u] = k < r1 ? b.rval[k] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        r[u] = kk < r1 ? b.ridx[kk] : 0u;
+        v[u] = kk < r1 ? b.rval[kk] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
       }
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int e0 = (int)(r[u] & 0xFFFFu) - part * kSumEntries, e1 = (int)(r[u] >> 16) - part * kSumEntries;
       const bool ok = k0 + u * 64 + lane < r1;
-      bool p0 = ok && e0 >= 0 && e0 < kSumEntries, p1 = ok && e1 >= 0 && e1 < kSumEntries;
+      bool p0 = ok && e0 >= 0 && e0 < ne, p1 = ok && e1 >= 0 && e1 < ne;
       while (__ballot(p0 || p1) != 0) {
-        if (p0) tag[e0] = (uint8_t)lane;
-        if (p1) tag[e1] = (uint8_t)lane;
+        tag[p0 ? e0 : dummy] = (uint8_t)lane;
+        tag[p1 ? e1 : dummy] = (uint8_t)lane;
         wave_sync();
-        const bool w0 = p0 && tag[e0] == lane;
-        const bool w1 = p1 && tag[e1] == lane;
-        if (w0) {
-          float2 o = acc[e0];
-          o.x = o.x + v[u].x;
-          o.y = o.y + v[u].y;
-          acc[e0] = o;
-          p0 = false;
+        const int t0 = tag[p0 ? e0 : dummy], t1 = tag[p1 ? e1 : dummy];
+        const bool w0 = p0 && t0 == lane, w1 = p1 && t1 == lane;
+        // both halves on one entry: one read-add-write with the first half added first
+        const bool same = w0 && w1 && e0 == e1;
+        const int a0 = w0 ? e0 : dummy, a1 = (w1 && !same) ? e1 : dummy;
+        float2 o0 = acc[a0], o1 = acc[a1];
+        o0.x = o0.x + v[u].x;
+        o0.y = o0.y + v[u].y;
+        if (same) {
+          o0.x = o0.x + v[u].z;
+          o0.y = o0.y + v[u].w;
         }
-        if (w1) {
-          float2 o = acc[e1];
-          o.x = o.x + v[u].z;
-          o.y = o.y + v[u].w;
-          acc[e1] = o;
-          p1 = false;
-        }
+        o1.x = o1.x + v[u].z;
+        o1.y = o1.y + v[u].w;
+        acc[a0] = o0;
+        acc[a1] = o1;
+        p0 = p0 && !w0;
+        p1 = p1 && !w1;
         wave_sync();
       }
     }
@@ -1233,85 +1360,6 @@ __global__ void __launch_bounds__(64) hash_bin_sum_kernel(BinDev b, float* __res
       g[e] = o;
     }
   }
-}
-
-// Exclusive scan of n int32 in place (x has n + 1 entries; x[n] receives the total):
-// tile sums, one work-group over the tile sums, then each tile with its offset.
-constexpr int kScanThreads = 256;
-constexpr int kScanPer = 16;
-constexpr int kScanTile = kScanThreads * kScanPer;
-
-VXN_DEV int32_t block_excl_scan_i32(int32_t v, int32_t& total) {
-  __shared__ int32_t ws[kScanThreads / 64];
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  int32_t inc = v;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const int32_t u = __shfl_up(inc, o, 64);
-    if (lane >= o) inc += u;
-  }
-  if (lane == 63) ws[wid] = inc;
-  __syncthreads();
-  int32_t pre = 0;
-  total = 0;
-#pragma unroll
-  for (int w = 0; w < kScanThreads / 64; ++w) {
-    if (w < wid) pre += ws[w];
-    total += ws[w];
-  }
-  return pre + inc - v;
-}
-
-__global__ void __launch_bounds__(kScanThreads) scan_tile_sums_kernel(const int32_t* __restrict__ x, int64_t n,
-                                                                      int32_t* __restrict__ tsum) {
-  const int64_t base = (int64_t)blockIdx.x * kScanTile;
-  int32_t s = 0;
-  for (int k = threadIdx.x; k < kScanTile; k += kScanThreads) {
-    const int64_t i = base + k;
-    if (i < n) s += x[i];
-  }
-  int32_t total;
-  block_excl_scan_i32(s, total);
-  if (threadIdx.x == 0) tsum[blockIdx.x] = total;
-}
-
-// One work-group: exclusive scan of nt tile sums in place, total at tsum[nt].
-__global__ void __launch_bounds__(kScanThreads) scan_tile_offsets_kernel(int32_t* __restrict__ tsum, int nt) {
-  const int per = (nt + kScanThreads - 1) / kScanThreads;
-  const int b0 = threadIdx.x * per;
-  int32_t s = 0;
-  for (int k = 0; k < per; ++k)
-    if (b0 + k < nt) s += tsum[b0 + k];
-  int32_t total;
-  int32_t run = block_excl_scan_i32(s, total);
-  for (int k = 0; k < per; ++k) {
-    if (b0 + k < nt) {
-      const int32_t v = tsum[b0 + k];
-      tsum[b0 + k] = run;
-      run += v;
-    }
-  }
-  if (threadIdx.x == 0) tsum[nt] = total;
-}
-
-__global__ void __launch_bounds__(kScanThreads) scan_tiles_kernel(int32_t* __restrict__ x, int64_t n,
-                                                                  const int32_t* __restrict__ tsum, int nt) {
-  const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanPer;
-  int32_t v[kScanPer];
-  int32_t s = 0;
-#pragma unroll
-  for (int k = 0; k < kScanPer; ++k) {
-    v[k] = base + k < n ? x[base + k] : 0;
-    s += v[k];
-  }
-  int32_t total;
-  int32_t run = block_excl_scan_i32(s, total) + tsum[blockIdx.x];
-#pragma unroll
-  for (int k = 0; k < kScanPer; ++k) {
-    if (base + k < n) x[base + k] = run;
-    run += v[k];
-  }
-  if (blockIdx.x == 0 && threadIdx.x == 0) x[n] = tsum[nt];
 }
 
 // Brick slots of the Eq. 7 stencil around a cell with base corner c0: 8 cell corners
@@ -1526,11 +1574,12 @@ static int render_blocks(int64_t R) {
 static int64_t align256(int64_t b) { return (b + 255) & ~(int64_t)255; }
 
 // Shape of the binned hash gradient for ns samples over L levels of T = 2^log2_T
-// entries; use = false selects the atomic scatter (bins or records beyond int32 / LDS).
+// entries; use = false selects the atomic scatter (more bins per level than the LDS
+// counters hold, or records beyond int32).
 struct BinPlan {
   bool use;
-  int nbl, nbins, nwg, nt;
-  int64_t n_off, max_rec;
+  int nbl, nbins, ntile, nrows, nchunk;
+  int64_t n_cnt, n_csum, max_rec;
 };
 
 // Records: 4 per sample and level, 8 on levels whose x can end in kBinBits 1 bits (res >=
@@ -1541,20 +1590,26 @@ static BinPlan bin_plan(int64_t ns, int L, int log2_T, const int32_t* level_res)
   const int64_t T = (int64_t)1 << log2_T;
   p.nbl = (int)((T + kBinEntries - 1) / kBinEntries);
   p.nbins = L * p.nbl;
-  p.nwg = (int)((ns + 256 * kBinChunks - 1) / (256 * kBinChunks));
+  const int64_t ntile = (ns + kBinTile - 1) / kBinTile;
   int per = 0;
   for (int l = 0; l < L; ++l) per += level_res[l] >= kBinEntries - 1 ? 8 : 4;
   p.max_rec = ns * per;
-  p.n_off = (int64_t)p.nbins * p.nwg;
-  p.nt = (int)((p.n_off + kScanTile - 1) / kScanTile);
-  p.use = p.nbins <= kMaxBins && p.max_rec < ((int64_t)1 << 31) && p.n_off + 1 < ((int64_t)1 << 31);
+  p.use = p.nbl <= kMaxBinsPerLevel && p.max_rec < ((int64_t)1 << 31) && ntile * L < ((int64_t)1 << 31) &&
+          ntile * kBinTileWaves * (int64_t)p.nbins < ((int64_t)1 << 31);
+  if (!p.use) return p;
+  p.ntile = (int)ntile;
+  p.nrows = p.ntile * kBinTileWaves;
+  p.nchunk = (p.nrows + kColChunk - 1) / kColChunk;
+  p.n_cnt = (int64_t)L * p.nrows * p.nbl;
+  p.n_csum = (int64_t)L * p.nchunk * p.nbl;
   return p;
 }
 
 // Workspace layout: MLP partials | scalar partials | reduction rows | gfeat | ggeo | hfeat
-// | bin offsets | scan tile sums | record entries | record values.
+// | bin count rows | column chunk sums | bin bases | record entries | record values.
 struct RenderWs {
-  int64_t off_sc, off_norm, off_rows, off_feat, off_geo, off_hf, off_bin, off_tsum, off_ridx, off_rval, total;
+  int64_t off_sc, off_norm, off_rows, off_feat, off_geo, off_hf, off_cnt, off_csum, off_base, off_ridx, off_rval,
+      total;
   BinPlan bins;
 };
 
@@ -1568,16 +1623,17 @@ static RenderWs render_ws(int64_t R, int S, int L, int log2_T, const int32_t* le
   w.off_feat = align256(w.off_rows + (int64_t)kRedRows * np * (int64_t)sizeof(float));
   w.off_geo = align256(w.off_feat + 32 * ns * (int64_t)sizeof(float));
   w.off_hf = align256(w.off_geo + kGeo * ns * (int64_t)sizeof(float));
-  w.off_bin = align256(w.off_hf + 16 * ns * (int64_t)sizeof(uint32_t));
+  w.off_cnt = align256(w.off_hf + 16 * ns * (int64_t)sizeof(uint32_t));
   w.bins = bin_plan(ns, L, log2_T, level_res);
   if (w.bins.use) {
-    w.off_tsum = align256(w.off_bin + (w.bins.n_off + 1) * 4);
-    w.off_ridx = align256(w.off_tsum + ((int64_t)w.bins.nt + 1) * 4);
+    w.off_csum = align256(w.off_cnt + w.bins.n_cnt * 4);
+    w.off_base = align256(w.off_csum + w.bins.n_csum * 4);
+    w.off_ridx = align256(w.off_base + ((int64_t)w.bins.nbins + 1) * 4);
     w.off_rval = align256(w.off_ridx + w.bins.max_rec * 4);
     w.total = align256(w.off_rval + w.bins.max_rec * 16);
   } else {
-    w.off_tsum = w.off_ridx = w.off_rval = w.off_bin;
-    w.total = w.off_bin;
+    w.off_csum = w.off_base = w.off_ridx = w.off_rval = w.off_cnt;
+    w.total = w.off_cnt;
   }
   return w;
 }
@@ -1741,27 +1797,33 @@ static int launch_render_nv(const vxn_render_args* p, void* stream) {
       BinDev b;
       b.nbl = w.bins.nbl;
       b.nbins = w.bins.nbins;
-      b.nwg = w.bins.nwg;
-      b.off = (int32_t*)(ws + w.off_bin);
+      b.ntile = w.bins.ntile;
+      b.nrows = w.bins.nrows;
+      b.nchunk = w.bins.nchunk;
+      b.cnt = (int32_t*)(ws + w.off_cnt);
+      b.csum = (int32_t*)(ws + w.off_csum);
+      b.base = (int32_t*)(ws + w.off_base);
       b.ridx = (uint32_t*)(ws + w.off_ridx);
       b.rval = (float4*)(ws + w.off_rval);
-      int32_t* tsum = (int32_t*)(ws + w.off_tsum);
       const int dl = dedup_levels(a.h);
       b.fact_level = dl;
-      const size_t lds = (size_t)b.nbins * sizeof(int32_t);
-      hipLaunchKernelGGL(hash_bin_count_kernel, dim3(b.nwg), dim3(256), lds, s, a, b, dl);
+      const size_t lds = (size_t)kBinTileWaves * b.nbl * sizeof(int32_t);
+      const unsigned nwg = (unsigned)(a.h.L * b.ntile);
+      hipLaunchKernelGGL(hash_bin_count_kernel, dim3(nwg), dim3(256), lds, s, a, b, dl);
       VXN_LAUNCH_CHECK();
-      hipLaunchKernelGGL(scan_tile_sums_kernel, dim3(w.bins.nt), dim3(kScanThreads), 0, s, (const int32_t*)b.off,
-                         w.bins.n_off, tsum);
+      const dim3 cgrid((unsigned)((b.nbl + 255) / 256), (unsigned)b.nchunk, (unsigned)a.h.L);
+      hipLaunchKernelGGL(bin_col_sums_kernel, cgrid, dim3(256), 0, s, b);
       VXN_LAUNCH_CHECK();
-      hipLaunchKernelGGL(scan_tile_offsets_kernel, dim3(1), dim3(kScanThreads), 0, s, tsum, w.bins.nt);
+      hipLaunchKernelGGL(bin_col_scan_kernel, dim3((unsigned)((b.nbins + 255) / 256)), dim3(256), 0, s, b);
       VXN_LAUNCH_CHECK();
-      hipLaunchKernelGGL(scan_tiles_kernel, dim3(w.bins.nt), dim3(kScanThreads), 0, s, b.off, w.bins.n_off,
-                         (const int32_t*)tsum, w.bins.nt);
+      hipLaunchKernelGGL(bin_base_scan_kernel, dim3(1), dim3(kColThreads), 0, s, b);
       VXN_LAUNCH_CHECK();
-      hipLaunchKernelGGL(hash_bin_write_kernel, dim3(b.nwg), dim3(256), lds, s, a, b, dl);
+      hipLaunchKernelGGL(bin_col_positions_kernel, cgrid, dim3(256), 0, s, b);
       VXN_LAUNCH_CHECK();
-      hipLaunchKernelGGL(hash_bin_sum_kernel, dim3(b.nbins * kSumSplit), dim3(64), 0, s, b, p->grad_table,
+      hipLaunchKernelGGL(hash_bin_write_kernel, dim3(nwg), dim3(256), lds, s, a, b, dl);
+      VXN_LAUNCH_CHECK();
+      const unsigned nsum = (unsigned)(((b.nbins + 7) / 8) * 8 * kSumSplit);
+      hipLaunchKernelGGL(hash_bin_sum_kernel, dim3(nsum), dim3(64), 0, s, b, p->grad_table,
                          (int64_t)1 << p->log2_T);
       VXN_LAUNCH_CHECK();
     } else {
