@@ -840,25 +840,64 @@ __global__ void __launch_bounds__(kRT, 2) render_kernel(RenderDev a) {
 
 // ------------------------------------------------------------------------------------
 // Wavefront segmented sum: lanes with equal keys in consecutive runs; every lane gets
-// the sum over [run start, itself]; returns true on the run's last lane.
+// the sum over [run start, itself]; returns true on the run's last lane.  The lane
+// shifts are DPP moves (one VALU instruction each, no LDS round trip): a Hillis-Steele
+// scan over each 16-lane row (row_shr 1, 2, 4, 8), then the row carries of the GFX9 scan
+// (row_bcast:15 adds lane 15 of rows 0 / 2 to rows 1 / 3, row_bcast:31 adds lane 31 to
+// rows 2 and 3).  A lane takes a shifted value only when its run reaches back to the
+// source lane, so a carried value is itself a sum over the same run.
 // ------------------------------------------------------------------------------------
+constexpr int kDppRowShr = 0x110;      // + n: row_shr:n
+constexpr int kDppWaveShl1 = 0x130;    // lane j reads lane j + 1
+constexpr int kDppWaveShr1 = 0x138;    // lane j reads lane j - 1
+constexpr int kDppRowBcast15 = 0x142;  // lane 15 of each row to the next row
+constexpr int kDppRowBcast31 = 0x143;  // lane 31 to rows 2 and 3
+
+// The DPP move of v; lanes without a source, or in rows outside ROW_MASK, read 0.
+template <int CTRL, int ROW_MASK>
+VXN_DEV int dpp_i(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, ROW_MASK, 0xF, true);
+}
+template <int CTRL, int ROW_MASK>
+VXN_DEV float dpp_f(float v) {
+  return __int_as_float(dpp_i<CTRL, ROW_MASK>(__float_as_int(v)));
+}
+
 template <int NV>
 VXN_DEV bool seg_sum(uint64_t key, float (&v)[NV], int lane) {
-  const uint64_t kp = __shfl_up(key, 1, 64);
-  const uint64_t kn = __shfl_down(key, 1, 64);
+  const int klo = (int)(uint32_t)key, khi = (int)(uint32_t)(key >> 32);
+  const uint64_t kp = (uint64_t)(uint32_t)dpp_i<kDppWaveShr1, 0xF>(klo) |
+                      ((uint64_t)(uint32_t)dpp_i<kDppWaveShr1, 0xF>(khi) << 32);
+  const uint64_t kn = (uint64_t)(uint32_t)dpp_i<kDppWaveShl1, 0xF>(klo) |
+                      ((uint64_t)(uint32_t)dpp_i<kDppWaveShl1, 0xF>(khi) << 32);
+  // run start of every lane: a max-scan of the starts (lanes without a source read 0)
   int st = (lane == 0 || kp != key) ? lane : 0;
+  st = max(st, dpp_i<kDppRowShr + 1, 0xF>(st));
+  st = max(st, dpp_i<kDppRowShr + 2, 0xF>(st));
+  st = max(st, dpp_i<kDppRowShr + 4, 0xF>(st));
+  st = max(st, dpp_i<kDppRowShr + 8, 0xF>(st));
+  st = max(st, dpp_i<kDppRowBcast15, 0xA>(st));
+  st = max(st, dpp_i<kDppRowBcast31, 0xC>(st));
+  const int r = lane & 15;
+  const bool c1 = r >= 1 && lane - 1 >= st, c2 = r >= 2 && lane - 2 >= st, c4 = r >= 4 && lane - 4 >= st,
+             c8 = r >= 8 && lane - 8 >= st;
+  const bool c15 = ((lane >> 4) & 1) != 0 && st <= (lane & ~15) - 1, c31 = lane >= 32 && st <= 31;
 #pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const int u = __shfl_up(st, o, 64);
-    if (lane >= o) st = max(st, u);
-  }
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-#pragma unroll
-    for (int k = 0; k < NV; ++k) {
-      const float u = __shfl_up(v[k], o, 64);
-      if (lane - o >= st) v[k] += u;
-    }
+  for (int k = 0; k < NV; ++k) {
+    float x = v[k];
+    float u = dpp_f<kDppRowShr + 1, 0xF>(x);
+    x = c1 ? x + u : x;
+    u = dpp_f<kDppRowShr + 2, 0xF>(x);
+    x = c2 ? x + u : x;
+    u = dpp_f<kDppRowShr + 4, 0xF>(x);
+    x = c4 ? x + u : x;
+    u = dpp_f<kDppRowShr + 8, 0xF>(x);
+    x = c8 ? x + u : x;
+    u = dpp_f<kDppRowBcast15, 0xA>(x);
+    x = c15 ? x + u : x;
+    u = dpp_f<kDppRowBcast31, 0xC>(x);
+    x = c31 ? x + u : x;
+    v[k] = x;
   }
   return lane == 63 || kn != key;
 }
