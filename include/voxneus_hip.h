@@ -361,6 +361,84 @@ int vxn_reg_eik_curv_vr_noalloc(const float* grid, int X, int Y, int Z, const fl
                                 float relax_radius, const uint32_t* bitmask, float w_eik, float w_curv,
                                 float* grad_grid, double* losses, int32_t* counts, void* stream);
 
+/* ---- Alpha and compositing of per-sample values the caller holds ------------------ */
+
+/* volume_render_from_samples (rendering/volume_render.py:57-153): per ray of n_samples
+ * (<= 128) samples, the Eq. 2 alpha from sdf / normals / ray_dirs / dists / s (neus_alpha.py:
+ * 12-81; iter_cos annealing with use_iter_cos), the unit-sphere mask from pts (nullable),
+ * transmittance, the T < 1e-4 cutoff and the composites rgb [R,3], depth [R,1] (from
+ * z_vals), normal [R,3], opacity [R,1], weights / alpha / transmittance [R,S] (outputs
+ * nullable).  With alpha_in [R,S] (and sdf = null) the alpha is an input: composite_weights
+ * (volume_render.py:12-40).  colors, normals and z_vals are nullable (read as zero).
+ * The backward takes the upstream gradients (nullable = zero) of every output and writes
+ * d_colors, d_sdf, d_normals, d_alpha_in (nullable) and d_s (one float, overwritten, summed
+ * in a fixed order; workspace vxn_composite_bwd_workspace_bytes).                       */
+typedef struct vxn_composite_args {
+  int64_t n_rays;
+  int32_t n_samples;
+  const float* colors;   /* [R,S,3] */
+  const float* sdf;      /* [R,S] */
+  const float* normals;  /* [R,S,3] */
+  const float* ray_dirs; /* [R,3], or [R,S,3] with dirs_per_sample */
+  int32_t dirs_per_sample;
+  const float* z_vals;   /* [R,S] */
+  const float* dists;    /* [R,S] */
+  const float* s;        /* device scalar */
+  const float* pts;      /* [R,S,3] */
+  const float* alpha_in; /* [R,S] */
+  float sphere_radius;
+  float cos_anneal_ratio;
+  int32_t use_iter_cos;
+  float* rgb;
+  float* depth;
+  float* normal;
+  float* opacity;
+  float* weights;
+  float* alpha;
+  float* transmittance;
+  const float* d_rgb;
+  const float* d_depth;
+  const float* d_normal;
+  const float* d_opacity;
+  const float* d_weights;
+  const float* d_alpha;
+  const float* d_transmittance;
+  float* d_colors;
+  float* d_sdf;
+  float* d_normals;
+  float* d_alpha_in;
+  float* d_s;
+} vxn_composite_args;
+int vxn_composite_fwd(const vxn_composite_args* a, void* stream);
+int vxn_composite_bwd_workspace_bytes(int64_t n_rays, int64_t* bytes);
+int vxn_composite_bwd(const vxn_composite_args* a, void* workspace, int64_t workspace_bytes, void* stream);
+
+/* compute_neus_alpha (rendering/neus_alpha.py:12-81) elementwise over n samples: sdf [n],
+ * normals / dirs [n,3], dists [n], s (device scalar); the backward writes d_sdf, d_normals
+ * (nullable) and d_s (nullable, one float, overwritten; workspace
+ * vxn_neus_alpha_workspace_bytes).                                                      */
+int vxn_neus_alpha_fwd(const float* sdf, const float* normals, const float* dirs, const float* dists, const float* s,
+                       int64_t n, float cos_anneal_ratio, int32_t use_iter_cos, float* alpha, void* stream);
+int vxn_neus_alpha_workspace_bytes(int64_t n, int64_t* bytes);
+int vxn_neus_alpha_bwd(const float* sdf, const float* normals, const float* dirs, const float* dists, const float* s,
+                       int64_t n, float cos_anneal_ratio, int32_t use_iter_cos, const float* d_alpha, float* d_sdf,
+                       float* d_normals, float* d_s, void* workspace, int64_t workspace_bytes, void* stream);
+
+/* The radiance MLP one point at a time: rgb [n,3] = sigmoid(W1 relu(W0 x + b0) + b1) with
+ * x = [feats (n,32), view (n,view_dim: 16 SH or 27 positional encoding), normals (n,3)]
+ * and mlp = W0 [64][35 + view_dim], b0, W1 [3][64], b1 flattened; mlp_mode
+ * VXN_MLP_BF16_MFMA takes the BF16 form's rounding points.  Replaces RadianceMLP.forward
+ * (models/radiance_mlp.py:81-99) on VoxNeuS.forward_points (models/voxneus.py:115-167).
+ * The backward writes dL/dfeats, dL/dview, dL/dnormals (each nullable) and overwrites
+ * d_mlp with the gradient summed over the points in a fixed order; workspace
+ * vxn_point_mlp_bwd_workspace_bytes.                                                    */
+int vxn_point_mlp_fwd(const float* feats, const float* view, int view_dim, const float* normals, const float* mlp,
+                      int64_t n, int mlp_mode, float* rgb, void* stream);
+int vxn_point_mlp_bwd_workspace_bytes(int64_t n, int view_dim, int64_t* bytes);
+int vxn_point_mlp_bwd(const float* feats, const float* view, int view_dim, const float* normals, const float* mlp,
+                      int64_t n, int mlp_mode, const float* d_rgb, float* d_feats, float* d_view, float* d_normals,
+                      float* d_mlp, void* workspace, int64_t workspace_bytes, void* stream);
+
 /* s-gradient amplification on the device: grad_v *= k if grad_v < 0.  amplified
  * (device, nullable) receives 1 or 0.  Replaces amplify_s_gradient
  * (training/scheduler.py:14-33) without its host read-back.                           */

@@ -32,7 +32,8 @@ def test_library_exports_every_declared_symbol():
 def test_struct_layouts_match_header_field_order():
     text = open(os.path.join(REPO, "include", "voxneus_hip.h")).read()
     for struct, cls in (("vxn_sampler_args", N.SamplerArgs), ("vxn_render_args", N.RenderArgs),
-                        ("vxn_scene_desc", N.SceneDesc), ("vxn_serve_render_args", N.ServeRenderArgs)):
+                        ("vxn_scene_desc", N.SceneDesc), ("vxn_serve_render_args", N.ServeRenderArgs),
+                        ("vxn_composite_args", N.CompositeArgs)):
         body = re.search(r"typedef struct %s \{(.*?)\} %s;" % (struct, struct), text, re.S).group(1)
         body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
         names = []

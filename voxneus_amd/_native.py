@@ -119,6 +119,20 @@ class ServeRenderArgs(ctypes.Structure):
     ]
 
 
+class CompositeArgs(ctypes.Structure):
+    _fields_ = [
+        ("n_rays", c_i64), ("n_samples", c_i32), ("colors", c_vp), ("sdf", c_vp), ("normals", c_vp),
+        ("ray_dirs", c_vp), ("dirs_per_sample", c_i32), ("z_vals", c_vp), ("dists", c_vp), ("s", c_vp),
+        ("pts", c_vp), ("alpha_in", c_vp), ("sphere_radius", c_f), ("cos_anneal_ratio", c_f),
+        ("use_iter_cos", c_i32),
+        ("rgb", c_vp), ("depth", c_vp), ("normal", c_vp), ("opacity", c_vp), ("weights", c_vp), ("alpha", c_vp),
+        ("transmittance", c_vp),
+        ("d_rgb", c_vp), ("d_depth", c_vp), ("d_normal", c_vp), ("d_opacity", c_vp), ("d_weights", c_vp),
+        ("d_alpha", c_vp), ("d_transmittance", c_vp),
+        ("d_colors", c_vp), ("d_sdf", c_vp), ("d_normals", c_vp), ("d_alpha_in", c_vp), ("d_s", c_vp),
+    ]
+
+
 class AdamTensor(ctypes.Structure):
     _fields_ = [
         ("param", c_vp), ("grad", c_vp), ("exp_avg", c_vp), ("exp_avg_sq", c_vp), ("n", c_i64),
@@ -177,6 +191,17 @@ _SIGS = {
     "vxn_reg_vr_workspace_bytes": (c_i32, [c_i32, c_i32, c_i32, ctypes.POINTER(c_i64)]),
     "vxn_reg_eik_curv_vr": (c_i32, [c_vp, c_i32, c_i32, c_i32, c_vp, c_vp, c_f, c_vp, c_f, c_f, c_vp, c_vp, c_vp,
                                     c_vp, c_i64, c_vp]),
+    "vxn_composite_fwd": (c_i32, [ctypes.POINTER(CompositeArgs), c_vp]),
+    "vxn_composite_bwd_workspace_bytes": (c_i32, [c_i64, ctypes.POINTER(c_i64)]),
+    "vxn_composite_bwd": (c_i32, [ctypes.POINTER(CompositeArgs), c_vp, c_i64, c_vp]),
+    "vxn_neus_alpha_fwd": (c_i32, [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_f, c_i32, c_vp, c_vp]),
+    "vxn_neus_alpha_workspace_bytes": (c_i32, [c_i64, ctypes.POINTER(c_i64)]),
+    "vxn_neus_alpha_bwd": (c_i32, [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_f, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp,
+                                   c_i64, c_vp]),
+    "vxn_point_mlp_fwd": (c_i32, [c_vp, c_vp, c_i32, c_vp, c_vp, c_i64, c_i32, c_vp, c_vp]),
+    "vxn_point_mlp_bwd_workspace_bytes": (c_i32, [c_i64, c_i32, ctypes.POINTER(c_i64)]),
+    "vxn_point_mlp_bwd": (c_i32, [c_vp, c_vp, c_i32, c_vp, c_vp, c_i64, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
+                                  c_i64, c_vp]),
     "vxn_reg_eik_curv_vr_noalloc": (c_i32, [c_vp, c_i32, c_i32, c_i32, c_vp, c_vp, c_f, c_vp, c_f, c_f, c_vp, c_vp,
                                             c_vp, c_vp]),
     "vxn_serve_scene_bytes": (c_i32, [ctypes.POINTER(c_i64)]),

@@ -233,6 +233,170 @@ class HashEncodeFunction(torch.autograd.Function):
 
 
 # ----------------------------------------------------------------------------------------
+# NeuS alpha and compositing of per-sample values (volume_render.py, neus_alpha.py)
+# ----------------------------------------------------------------------------------------
+
+def _s_tensor(s, device) -> torch.Tensor:
+    """The sharpness s as a one-element float32 device tensor (a Python number or a tensor
+    holding one value)."""
+    if torch.is_tensor(s):
+        if s.numel() != 1:
+            raise NotImplementedError("the compositing kernels take one s for every sample")
+        return s.reshape(1).to(device=device, dtype=torch.float32)
+    return torch.tensor([float(s)], device=device, dtype=torch.float32)
+
+
+def _f32c(t):
+    return None if t is None else t.to(torch.float32).contiguous()
+
+
+class CompositeFunction(torch.autograd.Function):
+    """volume_render_from_samples (volume_render.py:57-153) on [R,S] samples (S <= 128):
+    returns rgb [R,3], depth [R,1], normal [R,3], opacity [R,1], weights, alpha and
+    transmittance [R,S].  With ``alpha_in`` the alpha is an input (composite_weights)."""
+
+    @staticmethod
+    def forward(ctx, colors, sdf, normals, ray_dirs, z_vals, dists, s, alpha_in, pts, sphere_radius,
+                cos_anneal_ratio, use_iter_cos):
+        ref = alpha_in if alpha_in is not None else sdf
+        R, S = ref.shape[0], ref.shape[1]
+        dev = ref.device
+        a = N.CompositeArgs()
+        a.n_rays, a.n_samples = R, S
+        keep = [_f32c(colors), _f32c(sdf), _f32c(normals), _f32c(ray_dirs), _f32c(z_vals), _f32c(dists),
+                None if alpha_in is not None else _s_tensor(s, dev), _f32c(pts), _f32c(alpha_in)]
+        c, f, n, d, z, dt, sv, p, ai = keep
+        a.colors, a.sdf, a.normals, a.ray_dirs = N.dptr(c), N.dptr(f), N.dptr(n), N.dptr(d)
+        a.dirs_per_sample = int(d is not None and d.dim() == 3)
+        a.z_vals, a.dists, a.s, a.pts, a.alpha_in = N.dptr(z), N.dptr(dt), N.dptr(sv), N.dptr(p), N.dptr(ai)
+        a.sphere_radius = float(sphere_radius) if (sphere_radius is not None and p is not None) else 0.0
+        a.cos_anneal_ratio, a.use_iter_cos = float(cos_anneal_ratio), int(bool(use_iter_cos))
+        e = lambda *sh: torch.empty(*sh, device=dev, dtype=torch.float32)
+        outs = [e(R, 3), e(R, 1), e(R, 3), e(R, 1), e(R, S), e(R, S), e(R, S)]
+        a.rgb, a.depth, a.normal, a.opacity, a.weights, a.alpha, a.transmittance = (N.dptr(o) for o in outs)
+        N.check(N.lib().vxn_composite_fwd(ctypes.byref(a), _stream(ref)), "composite_fwd")
+        ctx.save_for_backward(*[t if t is not None else torch.empty(0) for t in keep])
+        ctx.present = [t is not None for t in keep]
+        ctx.meta = (R, S, a.dirs_per_sample, a.sphere_radius, a.cos_anneal_ratio, a.use_iter_cos)
+        ctx.s_is_tensor = torch.is_tensor(s)
+        return tuple(outs)
+
+    @staticmethod
+    def backward(ctx, d_rgb, d_depth, d_normal, d_opacity, d_weights, d_alpha, d_trans):
+        saved = [t if pr else None for t, pr in zip(ctx.saved_tensors, ctx.present)]
+        c, f, n, d, z, dt, sv, p, ai = saved
+        R, S, dps, sr, car, uic = ctx.meta
+        ref = ai if ai is not None else f
+        dev = ref.device
+        a = N.CompositeArgs()
+        a.n_rays, a.n_samples = R, S
+        a.colors, a.sdf, a.normals, a.ray_dirs = N.dptr(c), N.dptr(f), N.dptr(n), N.dptr(d)
+        a.dirs_per_sample = dps
+        a.z_vals, a.dists, a.s, a.pts, a.alpha_in = N.dptr(z), N.dptr(dt), N.dptr(sv), N.dptr(p), N.dptr(ai)
+        a.sphere_radius, a.cos_anneal_ratio, a.use_iter_cos = sr, car, uic
+        g = [_f32c(t) for t in (d_rgb, d_depth, d_normal, d_opacity, d_weights, d_alpha, d_trans)]
+        (a.d_rgb, a.d_depth, a.d_normal, a.d_opacity, a.d_weights, a.d_alpha,
+         a.d_transmittance) = (N.dptr(t) for t in g)
+        need = ctx.needs_input_grad
+        dc = torch.empty(R, S, 3, device=dev) if (need[0] and c is not None) else None
+        df = torch.empty(R, S, device=dev) if (need[1] and f is not None) else None
+        dn = torch.empty(R, S, 3, device=dev) if (need[2] and n is not None) else None
+        da = torch.empty(R, S, device=dev) if (need[7] and ai is not None) else None
+        ds = torch.empty(1, device=dev) if (need[6] and sv is not None) else None
+        a.d_colors, a.d_sdf, a.d_normals, a.d_alpha_in, a.d_s = (N.dptr(t) for t in (dc, df, dn, da, ds))
+        nb = ctypes.c_int64(0)
+        N.check(N.lib().vxn_composite_bwd_workspace_bytes(R, ctypes.byref(nb)), "composite_bwd_workspace_bytes")
+        ws = torch.empty(max(int(nb.value), 8), device=dev, dtype=torch.uint8)
+        N.check(N.lib().vxn_composite_bwd(ctypes.byref(a), ws.data_ptr(), ws.numel(), _stream(ref)), "composite_bwd")
+        d_s = None if ds is None else (ds.reshape(()) if ctx.s_is_tensor else None)
+        return dc, df, dn, None, None, None, d_s, da, None, None, None, None
+
+
+class NeusAlphaFunction(torch.autograd.Function):
+    """compute_neus_alpha (neus_alpha.py:12-81) elementwise on flattened samples."""
+
+    @staticmethod
+    def forward(ctx, sdf, normals, dirs, dists, s, cos_anneal_ratio, use_iter_cos):
+        f, n, d, dt = _f32c(sdf.reshape(-1)), _f32c(normals.reshape(-1, 3)), _f32c(dirs.reshape(-1, 3)), \
+            _f32c(dists.reshape(-1))
+        if not (f.shape[0] == n.shape[0] == d.shape[0] == dt.shape[0]):
+            raise ValueError("sdf, normals, ray_dirs and dists must hold the same number of samples")
+        sv = _s_tensor(s, f.device)
+        alpha = torch.empty_like(f)
+        N.check(N.lib().vxn_neus_alpha_fwd(N.dptr(f), N.dptr(n), N.dptr(d), N.dptr(dt), N.dptr(sv), f.shape[0],
+                                           float(cos_anneal_ratio), int(bool(use_iter_cos)), N.dptr(alpha),
+                                           _stream(f)), "neus_alpha_fwd")
+        ctx.save_for_backward(f, n, d, dt, sv)
+        ctx.meta = (sdf.shape, normals.shape, float(cos_anneal_ratio), int(bool(use_iter_cos)), torch.is_tensor(s))
+        return alpha.reshape(sdf.shape)
+
+    @staticmethod
+    def backward(ctx, d_alpha):
+        f, n, d, dt, sv = ctx.saved_tensors
+        sshape, nshape, car, uic, s_tensor = ctx.meta
+        cnt = f.shape[0]
+        need = ctx.needs_input_grad
+        df = torch.empty_like(f) if need[0] else None
+        dn = torch.empty_like(n) if need[1] else None
+        ds = torch.empty(1, device=f.device) if (need[4] and s_tensor) else None
+        nb = ctypes.c_int64(0)
+        N.check(N.lib().vxn_neus_alpha_workspace_bytes(cnt, ctypes.byref(nb)), "neus_alpha_workspace_bytes")
+        ws = torch.empty(max(int(nb.value), 8), device=f.device, dtype=torch.uint8)
+        da = _f32c(d_alpha.reshape(-1))
+        N.check(N.lib().vxn_neus_alpha_bwd(N.dptr(f), N.dptr(n), N.dptr(d), N.dptr(dt), N.dptr(sv), cnt, car, uic,
+                                           N.dptr(da), N.dptr(df), N.dptr(dn), N.dptr(ds), ws.data_ptr(), ws.numel(),
+                                           _stream(f)), "neus_alpha_bwd")
+        return (None if df is None else df.reshape(sshape), None if dn is None else dn.reshape(nshape), None, None,
+                None if ds is None else ds.reshape(()), None, None)
+
+
+# ----------------------------------------------------------------------------------------
+# Radiance MLP, one point at a time (forward_points)
+# ----------------------------------------------------------------------------------------
+
+class PointMlpFunction(torch.autograd.Function):
+    """RadianceMLP.forward (radiance_mlp.py:81-99) on [..., 32] hash features, [..., V]
+    view embeddings (V = 16 or 27) and [..., 3] normals with the flat MLP ``mlp`` (W0, b0,
+    W1, b1); ``mlp_mode`` VXN_MLP_BF16_MFMA takes the BF16 form's rounding points."""
+
+    @staticmethod
+    def forward(ctx, feats, view, normals, mlp, mlp_mode):
+        lead = feats.shape[:-1]
+        V = view.shape[-1]
+        f = feats.reshape(-1, 32).to(torch.float32).contiguous()
+        v = view.reshape(-1, V).to(torch.float32).contiguous()
+        n = normals.reshape(-1, 3).to(torch.float32).contiguous()
+        w = mlp.to(torch.float32).contiguous()
+        if not (f.shape[0] == v.shape[0] == n.shape[0]):
+            raise ValueError("feats, view and normals must have the same leading shape")
+        rgb = torch.empty(f.shape[0], 3, device=f.device, dtype=torch.float32)
+        N.check(N.lib().vxn_point_mlp_fwd(N.dptr(f), N.dptr(v), V, N.dptr(n), N.dptr(w), f.shape[0], int(mlp_mode),
+                                          N.dptr(rgb), _stream(f)), "point_mlp_fwd")
+        ctx.save_for_backward(f, v, n, w)
+        ctx.meta = (lead, V, int(mlp_mode))
+        return rgb.reshape(*lead, 3)
+
+    @staticmethod
+    def backward(ctx, d_rgb):
+        f, v, n, w = ctx.saved_tensors
+        lead, V, mode = ctx.meta
+        cnt = f.shape[0]
+        dr = d_rgb.reshape(-1, 3).to(torch.float32).contiguous()
+        df = torch.empty_like(f) if ctx.needs_input_grad[0] else None
+        dv = torch.empty_like(v) if ctx.needs_input_grad[1] else None
+        dn = torch.empty_like(n) if ctx.needs_input_grad[2] else None
+        dw = torch.empty_like(w)
+        nb = ctypes.c_int64(0)
+        N.check(N.lib().vxn_point_mlp_bwd_workspace_bytes(cnt, V, ctypes.byref(nb)), "point_mlp_bwd_workspace_bytes")
+        ws = torch.empty(max(int(nb.value), 1), device=f.device, dtype=torch.uint8)
+        N.check(N.lib().vxn_point_mlp_bwd(N.dptr(f), N.dptr(v), V, N.dptr(n), N.dptr(w), cnt, mode, N.dptr(dr),
+                                          N.dptr(df), N.dptr(dv), N.dptr(dn), N.dptr(dw), ws.data_ptr(), ws.numel(),
+                                          _stream(f)), "point_mlp_bwd")
+        r = lambda t, k: None if t is None else t.reshape(*lead, k)
+        return r(df, 32), r(dv, V), r(dn, 3), (dw if ctx.needs_input_grad[3] else None), None
+
+
+# ----------------------------------------------------------------------------------------
 # Progressive upsampler
 # ----------------------------------------------------------------------------------------
 

@@ -5,7 +5,8 @@ encoding view embeddings (models/embedder.py:12-87), the 2-layer radiance MLP
 On the fused path these are evaluated inside the gfx950 render kernel; the modules hold
 the parameters (``net.0.weight`` [64,51] with the SH embedding or [64,62] with the
 positional encoding, ``net.0.bias``, ``net.2.weight`` [3,64], ``net.2.bias``,
-``variance``) under the reference's state-dict names.
+``variance``) under the reference's state-dict names.  Called on its own (as
+``VoxNeuS.forward_points`` does) the MLP runs the per-point gfx950 kernel.
 """
 
 from __future__ import annotations
@@ -14,6 +15,9 @@ from typing import Optional, Union
 
 import torch
 import torch.nn as nn
+
+from . import _native as N
+from . import functional as Fn
 
 SH_C = (0.28209479177387814, 0.4886025119029199, 1.0925484305920792, 0.31539156525252005,
         0.5462742152960396, 0.5900435899266435, 2.890611442640554, 0.4570457994644658,
@@ -84,6 +88,7 @@ class RadianceMLP(nn.Module):
         self.view_embed_dim = view_embed_dim
         self.in_dim = 35 + view_embed_dim
         self.net = nn.Sequential(nn.Linear(self.in_dim, 64), nn.ReLU(inplace=True), nn.Linear(64, 3), nn.Sigmoid())
+        self.mlp_mode = N.VXN_MLP_FP32  # the owning VoxNeuS sets its form
         self.reset_parameters()
 
     def reset_parameters(self) -> None:
@@ -112,7 +117,9 @@ class RadianceMLP(nn.Module):
             off += n
 
     def forward(self, hash_features, view_embeddings, normals, geom_features=None, sdf=None):
-        return self.net(torch.cat([hash_features, view_embeddings, normals], -1))
+        """radiance_mlp.py:81-99 on the device: the per-point MLP kernel (FP32, or the BF16
+        form's rounding points when ``mlp_mode`` is VXN_MLP_BF16_MFMA)."""
+        return Fn.PointMlpFunction.apply(hash_features, view_embeddings, normals, self.flat(), self.mlp_mode)
 
 
 class SingleVarianceNetwork(nn.Module):

@@ -4,9 +4,9 @@ volume_render,occupancy_grid,rays}.py.
 The multi-step upsampler runs in the gfx950 sampler kernel; ``VoxNeuS.render`` composes
 it with the fused render kernel.  The standalone ``compute_neus_alpha`` /
 ``composite_weights`` / ``volume_render_from_samples`` entry points keep the reference's
-signatures for callers that already hold per-sample colours, SDF and normals; they
-evaluate the same formulas with elementwise device ops (their fused form is inside the
-render kernel).
+signatures for callers that already hold per-sample colours, SDF and normals; they run
+the alpha and compositing kernels (csrc/composite.hip), each with its hand-written
+backward, in the fused render kernel's arithmetic.
 """
 
 from __future__ import annotations
@@ -15,7 +15,6 @@ from typing import Any, Dict, Optional, Tuple
 
 import torch
 import torch.nn as nn
-import torch.nn.functional as F
 
 from . import functional as Fn
 from .training import near_far_from_sphere  # noqa: F401  (re-exported, rays.py:102-128)
@@ -76,6 +75,43 @@ class OccupancyGrid(nn.Module):
         self.total_updates += 1
 
 
+def sdf_grid_of(sdf_fn):
+    """The DenseSDFGrid that ``sdf_fn`` evaluates: a bound ``query_sdf``, or a closure over
+    a grid (or over a model with an ``sdf_grid``) whose values on probe points equal that
+    grid's ``query_sdf`` bit for bit.  Anything else raises: the sampler kernel cannot call
+    back into Python."""
+    from .sdf_grid import DenseSDFGrid
+
+    owner = getattr(sdf_fn, "__self__", None)
+    if isinstance(owner, DenseSDFGrid):
+        return owner
+    found = None
+    for cell in getattr(sdf_fn, "__closure__", None) or ():
+        try:
+            c = cell.cell_contents
+        except ValueError:
+            continue
+        g = c if isinstance(c, DenseSDFGrid) else getattr(c, "sdf_grid", None)
+        if isinstance(g, DenseSDFGrid):
+            found = g
+            break
+    if found is None:
+        raise NotImplementedError("the gfx950 sampler evaluates a DenseSDFGrid directly: pass its query_sdf, a "
+                                  "closure over the grid or over a model holding it as sdf_grid, or grid=")
+    dev = found.grid.device
+    lo = torch.tensor(found.box6[:3], device=dev)
+    hi = torch.tensor(found.box6[3:], device=dev)
+    gen = torch.Generator(device="cpu").manual_seed(0)
+    probe = (lo + (hi - lo) * torch.rand(64, 3, generator=gen).to(dev)).contiguous()
+    with torch.no_grad():
+        got = sdf_fn(probe).reshape(-1)
+        ref = found.query_sdf(probe).reshape(-1)
+    if got.shape != ref.shape or not torch.equal(got, ref):
+        raise NotImplementedError("sdf_fn does not evaluate the DenseSDFGrid it refers to (its values differ on "
+                                  "probe points); the gfx950 sampler only samples the grid itself")
+    return found
+
+
 class RaySampler(nn.Module):
     """sampler.py:69-204 with the upsampler on the GPU."""
 
@@ -89,12 +125,13 @@ class RaySampler(nn.Module):
                              n_importance: Optional[int] = None, up_sample_steps: int = 4,
                              perturb: Optional[bool] = None, occupancy_grid: Optional[Any] = None,
                              grid=None, t_rand=None, u_rand=None):
-        """``sdf_fn`` must be a DenseSDFGrid's ``query_sdf`` (or pass ``grid=`` the module):
-        the sampler evaluates the grid itself."""
+        """sampler.py:127-204.  The gfx950 sampler evaluates the dense grid itself, so
+        ``sdf_fn`` must evaluate one of this package's DenseSDFGrids: its ``query_sdf``, or a
+        closure over the grid or over a model holding it as ``sdf_grid`` (what
+        ``VoxNeuS.render`` passes, voxneus.py:210-224).  The closure's values are checked
+        against the grid's own on probe points; ``grid=`` names the grid directly."""
         if grid is None:
-            grid = getattr(sdf_fn, "__self__", None)
-        if grid is None or not hasattr(grid, "grid"):
-            raise NotImplementedError("the gfx950 sampler evaluates a DenseSDFGrid directly; pass grid=")
+            grid = sdf_grid_of(sdf_fn)
         n_samples = n_samples or self.n_samples
         n_importance = self.n_importance if n_importance is None else n_importance
         perturb = self.perturb if perturb is None else perturb
@@ -108,33 +145,31 @@ class RaySampler(nn.Module):
 
 def compute_neus_alpha(sdf, normals, ray_dirs, dists, s, cos_anneal_ratio=0.0, use_iter_cos_annealing=False,
                        eps=1e-5):
-    """neus_alpha.py:12-81."""
+    """neus_alpha.py:12-81 on the device (vxn_neus_alpha_fwd / _bwd, one thread per sample):
+    sdf and dists (..., 1) or (...), normals and ray_dirs (..., 3), broadcast together; s a
+    number or a one-element tensor.  Returns alpha (...)."""
+    if eps != 1e-5:
+        raise NotImplementedError("the alpha kernel uses the reference's eps = 1e-5")
     if sdf.ndim < normals.ndim:
         sdf = sdf.unsqueeze(-1)
     if dists.ndim < normals.ndim:
         dists = dists.unsqueeze(-1)
-    true_cos = torch.sum(normals * ray_dirs, dim=-1, keepdim=True)
-    if use_iter_cos_annealing:
-        cos_val = -(F.relu(-true_cos * 0.5 + 0.5) * (1.0 - cos_anneal_ratio) + F.relu(-true_cos) * cos_anneal_ratio)
-    else:
-        cos_val = true_cos
-    off = 0.5 * dists * cos_val
-    pp = torch.sigmoid(s * (sdf - off))
-    pn = torch.sigmoid(s * (sdf + off))
-    alpha = torch.clamp((pp - pn) / torch.clamp(pp, min=eps), min=0.0, max=1.0)
-    if alpha.shape[-1] == 1 and sdf.ndim == normals.ndim:
-        alpha = alpha.squeeze(-1)
-    return alpha
+    lead = torch.broadcast_shapes(sdf.shape[:-1], dists.shape[:-1], normals.shape[:-1], ray_dirs.shape[:-1])
+    sdf = sdf.expand(*lead, 1)
+    dists = dists.expand(*lead, 1)
+    normals = normals.expand(*lead, 3)
+    ray_dirs = ray_dirs.expand(*lead, 3)
+    return Fn.NeusAlphaFunction.apply(sdf.reshape(lead), normals, ray_dirs, dists.reshape(lead), s,
+                                      cos_anneal_ratio, use_iter_cos_annealing)
 
 
 def composite_weights(alpha, eps=1e-7):
-    """volume_render.py:12-40."""
-    B = alpha.shape[0]
-    oma = torch.clamp(1.0 - alpha + eps, min=0.0, max=1.0)
-    cp = torch.cumprod(torch.cat([torch.ones([B, 1], device=alpha.device, dtype=alpha.dtype), oma], -1), -1)
-    T = cp[:, :-1]
-    w = alpha * T
-    return torch.where(T < 1e-4, torch.zeros_like(w), w), T
+    """volume_render.py:12-40 on the device (vxn_composite_* with the alpha as input):
+    alpha [B, S] (S <= 128) -> (weights, transmittance) [B, S]."""
+    if eps != 1e-7:
+        raise NotImplementedError("the compositing kernel uses the reference's eps = 1e-7")
+    out = Fn.CompositeFunction.apply(None, None, None, None, None, None, None, alpha, None, None, 0.0, False)
+    return out[4], out[6]
 
 
 def compute_huber_color_loss(pred_rgb, target_rgb, delta=0.1):
@@ -145,17 +180,20 @@ def compute_huber_color_loss(pred_rgb, target_rgb, delta=0.1):
 
 def volume_render_from_samples(colors, sdf, normals, ray_dirs, z_vals, dists, s, pts=None, sphere_radius=1.0,
                                background_rgb=None, cos_anneal_ratio=0.0, use_iter_cos_annealing=False):
-    """volume_render.py:57-153."""
-    rde = ray_dirs.unsqueeze(1).expand(-1, colors.shape[1], -1) if ray_dirs.ndim == 2 else ray_dirs
-    alpha = compute_neus_alpha(sdf, normals, rde, dists, s, cos_anneal_ratio, use_iter_cos_annealing)
-    if pts is not None and sphere_radius is not None and sphere_radius > 0:
-        alpha = alpha * (torch.linalg.norm(pts, ord=2, dim=-1) < sphere_radius).float().detach()
-    w, T = composite_weights(alpha)
-    rgb = torch.sum(w.unsqueeze(-1) * colors, dim=1)
-    depth = torch.sum(w * z_vals, dim=-1, keepdim=True)
-    nrm = torch.sum(w.unsqueeze(-1) * normals, dim=1)
-    nrm = nrm / (torch.linalg.norm(nrm, dim=-1, keepdim=True) + 1e-8)
-    opacity = torch.sum(w, dim=-1, keepdim=True)
+    """volume_render.py:57-153 on the device (vxn_composite_fwd / _bwd: one ray per work-group,
+    FP64 transmittance, the fused render kernel's arithmetic): colors / normals (R, S, 3),
+    sdf and dists (R, S) or (R, S, 1), ray_dirs (R, 3) or (R, S, 3), z_vals (R, S), pts
+    (R, S, 3) for the unit-sphere mask; S <= 128.  Gradients reach colors, sdf, normals and
+    a tensor s."""
+    R, S = colors.shape[0], colors.shape[1]
+    sdf = sdf.reshape(R, S)
+    dists = dists.reshape(R, S)
+    if ray_dirs.ndim == 3:
+        ray_dirs = ray_dirs.expand(R, S, 3)
+    rgb, depth, nrm, opacity, w, alpha, T = Fn.CompositeFunction.apply(
+        colors, sdf, normals, ray_dirs, z_vals, dists, s, None, pts,
+        sphere_radius if (pts is not None and sphere_radius is not None and sphere_radius > 0) else None,
+        cos_anneal_ratio, use_iter_cos_annealing)
     rgb = apply_background(rgb, opacity, background_rgb)
     return {"rgb": rgb, "depth": depth, "normal": nrm, "opacity": opacity, "weights": w, "alpha": alpha,
             "transmittance": T}

@@ -3,7 +3,8 @@
 ``render`` runs pass 1 (the multi-step upsampler) and passes 2-3 (per-sample geometry,
 hash features, SH, MLP, Eq. 2 alpha, transmittance and compositing) as two gfx950
 kernels; autograd runs the fused backward kernel.  ``forward_points`` keeps the
-reference's per-point interface (grid-query and hash kernels, MLP on the device).
+reference's per-point interface: the grid-query, hash and per-point MLP kernels, each
+with its hand-written backward.
 """
 
 from __future__ import annotations
@@ -47,6 +48,7 @@ class VoxNeuS(nn.Module):
                               else PositionalEmbedder(num_freqs=4, include_input=True))
         self.radiance_mlp = RadianceMLP(self.radiance_hash.out_dim, self.view_embedder.out_dim, 3, geom_feature_dim,
                                         mlp_hidden_dim, entangled=entangled)
+        self.radiance_mlp.mlp_mode = mlp_mode
 
     @property
     def s(self) -> torch.Tensor:
