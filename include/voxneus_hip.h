@@ -149,7 +149,8 @@ int vxn_mc_emit(const float* grid, int X, int Y, int Z, const float* box6, float
                 void* stream);
 
 /* ---- Multiresolution hash encoding ------------------------------------------------
- * table: [L, 2^log2T, F] float32 (dtype 0) or bfloat16 (dtype 1); F must be 2.
+ * table: [L, 2^log2T, F] float32 (dtype 0, 8-byte aligned) or bfloat16 (dtype 1, 16-byte
+ * aligned; VXN_ERR_VALUE otherwise); F must be 2.
  * level_res (host, L entries): floor(base * b^l) as computed by the reference;
  * level_w (host, nullable): the NeuS2 lambda weights (1 = active).
  * x01: [n,3] normalised coordinates, clamped to [0,1] inside.  Replaces
@@ -209,7 +210,7 @@ typedef struct vxn_render_args {
   const float* grid;
   int32_t X, Y, Z;
   float box[6];
-  const void* table; /* [L,T,F] */
+  const void* table; /* [L,T,F]; 8-byte aligned (float32) or 16-byte aligned (bfloat16) */
   int32_t table_dtype;
   int32_t n_levels, log2_T, n_features;
   int32_t level_res[VXN_MAX_LEVELS];
@@ -537,7 +538,7 @@ typedef struct vxn_scene_desc {
   const uint8_t* occ_mask;    /* nullable [r,r,r] bool occupancy (sampler modulation) */
   int32_t occ_res;
   float occ_box[6];
-  const void* table;          /* [L,T,2] float32 or bfloat16 */
+  const void* table;          /* [L,T,2] float32 (8-byte aligned) or bfloat16 (16-byte aligned) */
   int32_t table_dtype;
   int32_t n_levels, log2_T, n_features;
   int32_t level_res[VXN_MAX_LEVELS];

@@ -23,6 +23,7 @@ Fixtures:
                            without the lambda mask, plus embedding gradients;
   * ``unit_mlp.npz``     - SH embedding + radiance MLP outputs and parameter gradients;
   * ``unit_alpha.npz``   - NeuS alpha, compositing and their input gradients;
+  * ``unit_occupancy.npz`` - three OccupancyGrid.update_from_sdf updates (seeded draws);
   * ``unit_sampler.npz`` - 4-step upsampler (deterministic and injected-uniform), with a
                            seeded occupancy mask: per-step weights and new samples;
   * ``step_c1.npz`` / ``step_c2.npz`` - full training-step composition (render, Huber
@@ -117,7 +118,26 @@ def main(ref_code: str, only=()):
     if not only or "unit_grid_analytical.npz" in only:
         save("unit_grid_analytical.npz", inputs_sha=sha(sc.grid, pts), pts=pts, r_g=r_g, grad=det(grad_a),
              normal=det(normal_a), grid_grad_idx=ai, grid_grad_val=av)
-    if only and not any(n not in ("unit_grid.npz", "unit_grid_analytical.npz") for n in only):
+    # ------------------------------------------------------------------ unit_occupancy
+    # OccupancyGrid.update_from_sdf over the config-1 grid: three updates with seeded
+    # torch.randint draws (replayed from the same seeds to store them), s = exp(3), exp(3),
+    # 60; the density grid and mask after each update
+    if not only or "unit_occupancy.npz" in only:
+        occ = OccupancyGrid(resolution=32)
+        s_vals = (float(torch.exp(torch.tensor(3.0))), float(torch.exp(torch.tensor(3.0))), 60.0)
+        rec = {}
+        for k, s_v in enumerate(s_vals):
+            torch.manual_seed(100 + k)
+            occ.update_from_sdf(g.query_sdf, torch.tensor(s_v), subsample_ratio=0.25)
+            torch.manual_seed(100 + k)
+            n = max(int(32 ** 3 * 0.25), 1024)
+            draws = [torch.randint(0, 32, (n,)) for _ in range(3)]
+            rec[f"idx{k}"] = torch.stack(draws).numpy().astype(np.int16)
+            rec[f"density{k}"] = det(occ.density_grid).copy()
+            rec[f"mask{k}"] = np.packbits(det(occ.occupancy_mask).reshape(-1))
+        save("unit_occupancy.npz", inputs_sha=sha(sc.grid), s_vals=np.array(s_vals, np.float32), **rec)
+    if only and not any(n not in ("unit_grid.npz", "unit_grid_analytical.npz", "unit_occupancy.npz")
+                        for n in only):
         return
 
     # ------------------------------------------------------------------ unit_hash

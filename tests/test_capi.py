@@ -107,6 +107,13 @@ def test_serving_validation_errors():
         d.level_res[l] = 16 + l
     assert L.vxn_serve_pack_scene(ctypes.byref(d), buf, nb.value - 1) == -5  # record too small
     assert L.vxn_serve_pack_scene(ctypes.byref(d), buf, nb.value) == 0
+    d.table = 24  # an FP32 table needs 8-byte alignment, a BF16 one 16-byte
+    assert L.vxn_serve_pack_scene(ctypes.byref(d), buf, nb.value) == 0
+    d.table_dtype = N.VXN_DTYPE_BF16
+    assert L.vxn_serve_pack_scene(ctypes.byref(d), buf, nb.value) == -3
+    d.table, d.table_dtype = 20, N.VXN_DTYPE_F32
+    assert L.vxn_serve_pack_scene(ctypes.byref(d), buf, nb.value) == -3
+    d.table = 16
     a = N.ServeRenderArgs()
     a.capacity, a.n_samples, a.n_importance, a.up_sample_steps = 4, 64, 64, 4
     assert L.vxn_serve_render_workspace_bytes(ctypes.byref(a), ctypes.byref(nb)) == 0

@@ -85,3 +85,19 @@ def test_model_update_draws_cells_like_the_reference(Fn):
                        torch.tensor(25.0), *idx, first_update=True)
     assert torch.equal(occ.density_grid.cpu(), dens) and torch.equal(occ.occupancy_mask.cpu(), mask)
     assert occ.total_updates == 1 and 0.0 < occ.occupied_percentage < 100.0
+
+
+def test_update_matches_reference_fixture(Fn):
+    """The device update, given the reference's own draws, reproduces the density grid and
+    mask the reference wrote after each of three updates (unit_occupancy.npz) bit for bit."""
+    from tests.conftest import load_golden
+
+    g = load_golden("unit_occupancy.npz")
+    grid_d = _grid().to(DEV)
+    dd, md = torch.zeros((32,) * 3, device=DEV), torch.ones((32,) * 3, dtype=torch.bool, device=DEV)
+    for k, s in enumerate(g["s_vals"].tolist()):
+        idx = tuple(torch.from_numpy(g[f"idx{k}"][a].astype(np.int64)).to(DEV) for a in range(3))
+        Fn.occupancy_update(dd, md, BOX6, grid_d, BOX6, s, 0.95, 0.01, k == 0, idx)
+        ref_mask = np.unpackbits(g[f"mask{k}"]).astype(bool).reshape(32, 32, 32)
+        assert np.array_equal(dd.cpu().numpy(), g[f"density{k}"]), k
+        assert np.array_equal(md.cpu().numpy(), ref_mask), k

@@ -183,3 +183,34 @@ def test_fused_optimizer_step_matches_oracle():
         assert int(opt.n_updated.item()) == int((gr["grid"] != 0).sum())
         for k, dv in (("grid", st.grid), ("table", st.table), ("mlp", st.mlp), ("var", st.variance)):
             assert torch.equal(dv.cpu(), host[k]), (it, k)
+
+
+def test_sdf_optimizer_after_a_milestone_takes_the_default_betas():
+    """At a super-resolution milestone the reference replaces the grid's optimizer with
+    SparseGridAdam(sdf_params, lr=new_lr) (trainer.py:157), i.e. the class default betas
+    (0.9, 0.999) (optimizer.py:24), not create_optimizer's (0.9, 0.99): after
+    SuperResolutionSchedule.apply the grid's updates equal the oracle's native Adam with
+    beta2 = 0.999 at the stage rate, from step 1 with fresh moments."""
+    from oracle.ref_cpu import hash_level_resolutions
+    from voxneus_amd import scenes
+    from voxneus_amd.training import FusedOptimizer, FusedTrainStep, StepConfig, SuperResolutionSchedule
+    sc = scenes.config1_scene(seed=0)
+    mlp = torch.cat([torch.from_numpy(sc.mlp[k]).reshape(-1) for k in ("W0", "b0", "W1", "b1")])
+    st = FusedTrainStep(torch.from_numpy(sc.grid).to(DEV), torch.from_numpy(sc.embeddings).to(DEV), mlp.to(DEV),
+                        torch.tensor(0.3, device=DEV), hash_level_resolutions(16, 16, sc.hash_max_resolution),
+                        cfg=StepConfig())
+    opt = FusedOptimizer(st, total_steps=100)
+    o, d, tgt = (torch.from_numpy(x).to(DEV) for x in scenes.config_rays("sphere", 256, seed=3))
+    st.step(o, d, tgt)
+    opt.step(0)
+    sched = SuperResolutionSchedule(milestones=(1,), resolutions=(64, 80), lr_sdf_stages=(0.1, 0.05),
+                                    cooldown_steps=0)
+    assert sched.apply(1, st, opt) and tuple(st.grid.shape) == (80, 80, 80)
+    host = st.grid.cpu()
+    m, v = torch.zeros_like(host), torch.zeros_like(host)
+    for it in range(1, 4):
+        st.step(o, d, tgt)
+        g = st.grad_grid.cpu()
+        opt.step(it)
+        R.adam_native_step(host, g, m, v, it, 0.9, 0.999, 0.05, 1e-8, True)
+        assert torch.equal(st.grid.cpu(), host), it

@@ -5,6 +5,8 @@ queries against the CPU oracle's render.py / mesh.py arithmetic (render.py:35-54
 near/far from the unit sphere, deterministic sampler; mesh.py:80-114: query_sdf and
 query_normals)."""
 
+import ctypes
+
 import numpy as np
 import pytest
 import torch
@@ -117,3 +119,26 @@ def test_serving_matches_oracle(scenes, mode):
         else:
             check_query(sc, q.points, r, 300, seed=80 + i)
     assert {k for k, _ in kinds} == {"render", "query"}
+
+
+def test_resident_scene_rejects_tables_the_kernels_cannot_index(scenes):
+    """The hash index is masked with T - 1, so T must be a power of two; the fused MLP
+    takes 16 levels of 2 features; a misaligned BF16 table is refused by the C-ABI."""
+    sc = scenes[0]
+    with pytest.raises(ValueError, match="power of two"):
+        type(sc)(sc.grid, torch.zeros(16, 3000, 2, device=DEV), sc.mlp, sc.level_res, sc.s)
+    with pytest.raises(ValueError, match="one value per level"):
+        type(sc)(sc.grid, sc.table, sc.mlp, sc.level_res[:8], sc.s)
+    with pytest.raises(ValueError, match="float32 or bfloat16"):
+        type(sc)(sc.grid, sc.table.half(), sc.mlp, sc.level_res, sc.s)
+    raw = torch.zeros(16 * 1024 * 2 + 8, dtype=torch.bfloat16, device=DEV)
+    odd = raw[2:2 + 16 * 1024 * 2].view(16, 1024, 2)  # 4 bytes past a 16-byte boundary
+    bad = type(sc)(sc.grid, odd, sc.mlp, sc.level_res, sc.s)
+    assert bad.table.data_ptr() % 16 == 4
+    n = ctypes.c_int64(0)
+    assert N.lib().vxn_serve_scene_bytes(ctypes.byref(n)) == 0
+    buf = (ctypes.c_uint8 * n.value)()
+    d = bad.desc()
+    assert N.lib().vxn_serve_pack_scene(ctypes.byref(d), buf, n.value) == -3
+    d = sc.desc()
+    assert N.lib().vxn_serve_pack_scene(ctypes.byref(d), buf, n.value) == 0

@@ -60,3 +60,25 @@ def test_update_rules_on_a_plane():
     want2 = torch.maximum(prev[ix, iy, iz] * 0.95, phi2)
     assert torch.allclose(dens[ix, iy, iz], want2, atol=1e-12)
     assert bool(((prev[ix, iy, iz] * 0.95) > phi2).any()) and bool(((prev[ix, iy, iz] * 0.95) < phi2).any())
+
+
+def test_oracle_update_matches_reference_fixture():
+    """Three OccupancyGrid.update_from_sdf updates written by the reference itself (32^3
+    occupancy grid over the config-1 SDF grid, its seeded torch.randint draws, s = e^3, e^3,
+    60): the oracle given the same draws reproduces the density grid and the mask after
+    every update bit for bit (duplicate draws included: PyTorch's index_put keeps the last)."""
+    from tests.conftest import load_golden
+    from voxneus_amd import scenes
+
+    g = load_golden("unit_occupancy.npz")
+    grid = torch.from_numpy(scenes.config1_scene(seed=0).grid)
+    dens, mask = torch.zeros((32,) * 3), torch.ones((32,) * 3, dtype=torch.bool)
+    mn, mx = torch.full((3,), -1.0), torch.full((3,), 1.0)
+    for k, s in enumerate(g["s_vals"].tolist()):
+        idx = [torch.from_numpy(g[f"idx{k}"][a].astype(np.int64)) for a in range(3)]
+        R.occupancy_update(dens, mask, mn, mx, grid[None, None], R.GridBox(64), torch.tensor(s), *idx,
+                           first_update=k == 0)
+        ref_mask = np.unpackbits(g[f"mask{k}"]).astype(bool).reshape(32, 32, 32)
+        assert np.array_equal(dens.numpy(), g[f"density{k}"]), k
+        assert np.array_equal(mask.numpy(), ref_mask), k
+    assert 0 < int(mask.sum()) < mask.numel()

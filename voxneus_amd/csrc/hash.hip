@@ -41,6 +41,10 @@ int make_hash_desc(HashDesc& h, const void* table, int dtype, int L, int log2T, 
   VXN_REQUIRE(log2T >= 1 && log2T <= 24, VXN_ERR_SHAPE, "log2 table size must be in [1, 24]");
   VXN_REQUIRE(F == 2, VXN_ERR_UNSUPPORTED, "only 2 features per level are implemented (got %d)", F);
   VXN_REQUIRE(dtype == VXN_DTYPE_F32 || dtype == VXN_DTYPE_BF16, VXN_ERR_VALUE, "unknown table dtype %d", dtype);
+  // entries are read as one 8-byte (FP32 pair) or 4-byte (BF16 pair) load, and the BF16
+  // table's gather loads 16 bytes at once where four entries sit together
+  VXN_REQUIRE(((uintptr_t)table & (dtype == VXN_DTYPE_BF16 ? 15u : 7u)) == 0, VXN_ERR_VALUE,
+              "table must be %d-byte aligned", dtype == VXN_DTYPE_BF16 ? 16 : 8);
   h.table = table;
   h.dtype = dtype;
   h.L = L;

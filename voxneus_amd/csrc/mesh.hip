@@ -3,8 +3,9 @@
 // values straddle the level (value < level is inside), at the linear-interpolation
 // crossing, shared by the cells around that edge; triangles per cell from a case table.
 //
-// The case table is built on the host from a rule on the cube's faces, so that the two
-// cells sharing a face always cut it the same way and the mesh has no cracks: walking
+// The case table (mc_table.h, compiled into constant memory) follows a rule on the cube's
+// faces, so that the two cells sharing a face always cut it the same way and the mesh has
+// no cracks: walking
 // each face's corners counter-clockwise about its outward normal, every run of inside
 // corners is cut off by one segment from the crossing where the walk leaves the run to
 // the crossing where it entered it (diagonal inside corners on a face stay separated).
@@ -20,11 +21,9 @@
 // vertex owns the edges leaving it along +x, +y, +z; vertices are numbered by owner in
 // grid order, then by axis.  Triangles are numbered by cell in grid order, then in table
 // order.
-#include <math.h>
 #include <string.h>
 
-#include <vector>
-
+#include "mc_table.h"
 #include "vxn_host.h"
 
 namespace vxn {
@@ -33,186 +32,12 @@ constexpr int kMcThreads = 256;
 constexpr int kScanItems = 4;                          // per thread
 constexpr int kScanBlock = kMcThreads * kScanItems;    // items per work-group
 
-__constant__ int8_t c_mc_tri[256][16];  // edge triples, -1 terminated
-__constant__ uint8_t c_mc_ntri[256];
-
-// ------------------------------------------------------------------------------------
-// Host: the case table
-// ------------------------------------------------------------------------------------
-static int mc_edge_id(int c0, int c1) {
-  const int d = c0 ^ c1;
-  const int a = d == 4 ? 0 : (d == 2 ? 1 : 2);
-  const int lo = c0 & ~d;
-  int bits[2], n = 0;
-  for (int b = 0; b < 3; ++b)
-    if (b != a) bits[n++] = (lo >> (2 - b)) & 1;
-  return 4 * a + 2 * bits[0] + bits[1];
-}
-
-static void mc_edge_corners(int e, int& c0, int& c1) {
-  const int a = e / 4, u = (e >> 1) & 1, v = e & 1;
-  int base[3] = {0, 0, 0}, n = 0;
-  const int uv[2] = {u, v};
-  for (int b = 0; b < 3; ++b)
-    if (b != a) base[b] = uv[n++];
-  c0 = (base[0] << 2) | (base[1] << 1) | base[2];
-  c1 = c0 | (1 << (2 - a));
-}
-
-// The two cube faces containing edge e, as bits 2 b + side of face (axis b, side).
-static unsigned mc_edge_faces(int e) {
-  int c0, c1;
-  mc_edge_corners(e, c0, c1);
-  const int a = e / 4;
-  unsigned f = 0;
-  for (int b = 0; b < 3; ++b)
-    if (b != a) f |= 1u << (2 * b + ((c0 >> (2 - b)) & 1));
-  return f;
-}
-
-static void mc_corner_pos(int k, double p[3]) {
-  p[0] = (k >> 2) & 1;
-  p[1] = (k >> 1) & 1;
-  p[2] = k & 1;
-}
-
-static void mc_build_table(int8_t tri[256][16], uint8_t ntri[256]) {
-  // faces: corners of each face in counter-clockwise order about the outward normal
-  int faces[6][4];
-  for (int a = 0, f = 0; a < 3; ++a) {
-    for (int s = 0; s < 2; ++s, ++f) {
-      int cs[4], m = 0;
-      for (int k = 0; k < 8; ++k)
-        if (((k >> (2 - a)) & 1) == s) cs[m++] = k;
-      double n[3] = {0, 0, 0}, t1[3] = {0, 0, 0}, t2[3], ctr[3] = {0, 0, 0};
-      n[a] = s ? 1.0 : -1.0;
-      t1[(a + 1) % 3] = 1.0;
-      t2[0] = n[1] * t1[2] - n[2] * t1[1];
-      t2[1] = n[2] * t1[0] - n[0] * t1[2];
-      t2[2] = n[0] * t1[1] - n[1] * t1[0];
-      for (int i = 0; i < 4; ++i) {
-        double p[3];
-        mc_corner_pos(cs[i], p);
-        for (int j = 0; j < 3; ++j) ctr[j] += 0.25 * p[j];
-      }
-      double ang[4];
-      for (int i = 0; i < 4; ++i) {
-        double p[3];
-        mc_corner_pos(cs[i], p);
-        double x = 0, y = 0;
-        for (int j = 0; j < 3; ++j) {
-          x += (p[j] - ctr[j]) * t1[j];
-          y += (p[j] - ctr[j]) * t2[j];
-        }
-        ang[i] = atan2(y, x);
-      }
-      for (int i = 0; i < 4; ++i)  // insertion sort by angle
-        for (int j = i; j > 0 && ang[j] < ang[j - 1]; --j) {
-          const double ta = ang[j];
-          ang[j] = ang[j - 1];
-          ang[j - 1] = ta;
-          const int tc = cs[j];
-          cs[j] = cs[j - 1];
-          cs[j - 1] = tc;
-        }
-      for (int i = 0; i < 4; ++i) faces[f][i] = cs[i];
-    }
-  }
-  for (int c = 0; c < 256; ++c) {
-    int succ[12];
-    for (int e = 0; e < 12; ++e) succ[e] = -1;
-    auto in = [c](int k) { return (c >> k) & 1; };
-    for (int f = 0; f < 6; ++f) {
-      const int* cs = faces[f];
-      for (int i = 0; i < 4; ++i) {
-        if (in(cs[i]) && !in(cs[(i + 1) & 3])) {
-          const int exit_e = mc_edge_id(cs[i], cs[(i + 1) & 3]);
-          int j = i;
-          while (in(cs[(j + 3) & 3])) j = (j + 3) & 3;
-          succ[exit_e] = mc_edge_id(cs[(j + 3) & 3], cs[j]);
-        }
-      }
-    }
-    int nt = 0;
-    bool seen[12] = {false};
-    for (int e0 = 0; e0 < 12; ++e0) {
-      if (succ[e0] < 0 || seen[e0]) continue;
-      std::vector<int> loop{e0};
-      seen[e0] = true;
-      for (int e = succ[e0]; e != e0; e = succ[e]) {
-        loop.push_back(e);
-        seen[e] = true;
-      }
-      // orientation: normal from the edge midpoints vs inside -> outside corners
-      double nrm[3] = {0, 0, 0}, ins[3] = {0, 0, 0}, outs[3] = {0, 0, 0};
-      int nin = 0, nout = 0;
-      const int m = (int)loop.size();
-      std::vector<double> mid(3 * m);
-      for (int i = 0; i < m; ++i) {
-        int c0, c1;
-        mc_edge_corners(loop[i], c0, c1);
-        double p0[3], p1[3];
-        mc_corner_pos(c0, p0);
-        mc_corner_pos(c1, p1);
-        for (int j = 0; j < 3; ++j) mid[3 * i + j] = 0.5 * (p0[j] + p1[j]);
-        const int cc[2] = {c0, c1};
-        for (int q = 0; q < 2; ++q) {
-          double p[3];
-          mc_corner_pos(cc[q], p);
-          double* acc = in(cc[q]) ? ins : outs;
-          (in(cc[q]) ? nin : nout) += 1;
-          for (int j = 0; j < 3; ++j) acc[j] += p[j];
-        }
-      }
-      for (int i = 0; i < m; ++i) {
-        const double* a = &mid[3 * i];
-        const double* b = &mid[3 * ((i + 1) % m)];
-        nrm[0] += a[1] * b[2] - a[2] * b[1];
-        nrm[1] += a[2] * b[0] - a[0] * b[2];
-        nrm[2] += a[0] * b[1] - a[1] * b[0];
-      }
-      double dot = 0;
-      for (int j = 0; j < 3; ++j) dot += nrm[j] * (outs[j] / nout - ins[j] / nin);
-      if (dot < 0) std::vector<int>(loop.rbegin(), loop.rend()).swap(loop);
-      // fan from the first loop vertex whose fan has no triangle lying in a cube face
-      // (the two cells sharing that face would both cover it)
-      int apex = -1;
-      for (int k = 0; k < m && apex < 0; ++k) {
-        bool ok = true;
-        for (int i = 1; i + 1 < m && ok; ++i) {
-          const int e0 = loop[k], e1 = loop[(k + i) % m], e2 = loop[(k + i + 1) % m];
-          ok = (mc_edge_faces(e0) & mc_edge_faces(e1) & mc_edge_faces(e2)) == 0;
-        }
-        if (ok) apex = k;
-      }
-      if (apex < 0) apex = 0;  // no such case arises (the table tests check every case)
-      for (int i = 1; i + 1 < m; ++i) {
-        tri[c][3 * nt] = (int8_t)loop[apex];
-        tri[c][3 * nt + 1] = (int8_t)loop[(apex + i) % m];
-        tri[c][3 * nt + 2] = (int8_t)loop[(apex + i + 1) % m];
-        ++nt;
-      }
-    }
-    for (int i = 3 * nt; i < 16; ++i) tri[c][i] = -1;
-    ntri[c] = (uint8_t)nt;
-  }
-}
-
-static bool g_table_ready[64];
-
-static int mc_upload_table() {
-  int dev = 0;
-  VXN_HIP(hipGetDevice(&dev));
-  VXN_REQUIRE(dev >= 0 && dev < 64, VXN_ERR_UNSUPPORTED, "device index %d out of range", dev);
-  if (g_table_ready[dev]) return 0;
-  int8_t tri[256][16];
-  uint8_t ntri[256];
-  mc_build_table(tri, ntri);
-  VXN_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_mc_tri), tri, sizeof(tri)));
-  VXN_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_mc_ntri), ntri, sizeof(ntri)));
-  g_table_ready[dev] = true;
-  return 0;
-}
+// The table in constant memory, initialised when the code object loads: no upload, no
+// state to track per device.
+__constant__ int8_t c_mc_tri[256][16] = VXN_MC_TRI_INIT;  // edge triples, -1 terminated
+__constant__ uint8_t c_mc_ntri[256] = VXN_MC_NTRI_INIT;
+static const int8_t kMcTri[256][16] = VXN_MC_TRI_INIT;
+static const uint8_t kMcNtri[256] = VXN_MC_NTRI_INIT;
 
 // ------------------------------------------------------------------------------------
 // Device
@@ -451,11 +276,8 @@ using namespace vxn;
 
 extern "C" int vxn_mc_case_table(int8_t* tri, uint8_t* ntri) {
   VXN_REQUIRE(tri != nullptr && ntri != nullptr, VXN_ERR_NULL, "null output");
-  int8_t t[256][16];
-  uint8_t nt[256];
-  mc_build_table(t, nt);
-  memcpy(tri, t, sizeof(t));
-  memcpy(ntri, nt, sizeof(nt));
+  memcpy(tri, kMcTri, sizeof(kMcTri));
+  memcpy(ntri, kMcNtri, sizeof(kMcNtri));
   return 0;
 }
 
@@ -471,7 +293,6 @@ extern "C" int vxn_mc_count(const float* grid, int X, int Y, int Z, float level,
   McDev m{};
   if (int e = mc_dev(grid, X, Y, Z, level, workspace, workspace_bytes, m)) return e;
   VXN_REQUIRE(counts != nullptr, VXN_ERR_NULL, "counts is null");
-  if (int e = mc_upload_table()) return e;
   hipStream_t s = (hipStream_t)stream;
   const int64_t n = (int64_t)X * Y * Z;
   hipLaunchKernelGGL(mc_classify_kernel, dim3(blocks_for(n, kMcThreads, 65536)), dim3(kMcThreads), 0, s, m);
@@ -493,7 +314,6 @@ extern "C" int vxn_mc_emit(const float* grid, int X, int Y, int Z, const float* 
   VXN_REQUIRE(box6 != nullptr, VXN_ERR_NULL, "box is null");
   VXN_REQUIRE(max_verts >= 0 && max_faces >= 0, VXN_ERR_SHAPE, "negative capacity");
   VXN_REQUIRE((verts || max_verts == 0) && (faces || max_faces == 0), VXN_ERR_NULL, "null output");
-  if (int e = mc_upload_table()) return e;
   // world = index * (xyz_max - xyz_min) / (res - 1) + xyz_min, the extent in float32
   // and the rest in float64 (mesh.py:71-73)
   const int res[3] = {X, Y, Z};

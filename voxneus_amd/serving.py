@@ -50,6 +50,12 @@ class ResidentScene:
             raise ValueError("grid must be a float32 [X,Y,Z] HIP tensor")
         if table.dim() != 3 or table.shape[0] != 16 or table.shape[2] != 2 or not table.is_cuda:
             raise ValueError("table must be a [16,T,2] HIP tensor")
+        if table.shape[1] < 2 or table.shape[1] & (table.shape[1] - 1):
+            raise ValueError(f"the hash table size T must be a power of two (got {table.shape[1]})")
+        if table.dtype not in (torch.float32, torch.bfloat16):
+            raise ValueError(f"table must be float32 or bfloat16 (got {table.dtype})")
+        if len(level_res) != 16 or (level_w is not None and len(level_w) != 16):
+            raise ValueError("level_res (and level_w when given) must hold one value per level (16)")
         self.grid = grid.contiguous()
         self.table = table.contiguous()
         if mlp.numel() != N.MLP_PARAMS:

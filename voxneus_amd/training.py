@@ -368,7 +368,7 @@ class FusedOptimizer:
     """The reference's optimizer step (trainer.py:328-353, training/optimizer.py:15-154)
     on a ``FusedTrainStep``'s device buffers, with no host synchronisation:
 
-    * SparseGridAdam (lr_sdf, betas 0.9/0.99, eps 1e-8) on the grid through the native
+    * SparseGridAdam (lr_sdf, betas 0.9/0.99, after reset_sdf 0.9/0.999, eps 1e-8) on the grid through the native
       masked kernel; ``n_updated`` counts the updated voxels on the device;
     * Adam (betas 0.9/0.999, eps 1e-8) on the hash table, the MLP and the variance in one
       launch, the hash and MLP rates under the cosine decay of trainer.py:329, the
@@ -392,14 +392,20 @@ class FusedOptimizer:
         self.n_updated = torch.zeros(1, device=st.grid.device, dtype=torch.int32)
         self.sdf_step = 0
         self.net_step = 0
+        # create_optimizer builds the grid's SparseGridAdam with betas (0.9, 0.99)
+        # (optimizer.py:139); the one rebuilt at a super-resolution milestone takes the
+        # class default (0.9, 0.999) (trainer.py:157, optimizer.py:24)
+        self.sdf_beta2 = 0.99
 
     def reset_sdf(self, lr: float) -> None:
         """A fresh SparseGridAdam on the step's (rescaled) grid at rate ``lr``
-        (trainer.py:150-159): zero moments, step count from 0."""
+        (trainer.py:150-159): zero moments, step count from 0, and the class's default
+        betas (0.9, 0.999), as ``SparseGridAdam(sdf_params, lr=new_lr)`` takes them."""
         z = torch.zeros(self.st.grid.shape, device=self.st.grid.device, dtype=torch.float32)
         self.grid_m, self.grid_v = z, torch.zeros_like(z)
         self.lr_sdf = float(lr)
         self.sdf_step = 0
+        self.sdf_beta2 = 0.999
 
     def step(self, it: int, variance_floor: Optional[float] = None):
         """Apply both optimizers after ``FusedTrainStep.step`` of training iteration ``it``."""
@@ -416,7 +422,7 @@ class FusedOptimizer:
         self.sdf_step += 1
         self.n_updated.zero_()
         N.check(L.vxn_masked_adam_upd(st.grid.data_ptr(), st.grad_grid.data_ptr(), self.grid_m.data_ptr(),
-                                      self.grid_v.data_ptr(), st.grid.numel(), self.sdf_step, 0.9, 0.99,
+                                      self.grid_v.data_ptr(), st.grid.numel(), self.sdf_step, 0.9, self.sdf_beta2,
                                       self.lr_sdf, 1e-8, self.n_updated.data_ptr(), s), "masked_adam_upd")
         self.net_step += 1
         master = self.table_master
