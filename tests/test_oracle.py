@@ -234,15 +234,16 @@ def test_unit_alpha_matches_reference():
                                        torch.from_numpy(g["dists"]), s, pts=torch.from_numpy(g["pts"]))
     for k in ("alpha", "transmittance", "weights"):
         assert np.array_equal(out[k].detach().numpy(), g[k]), k
+    # the oracle runs the reference's float32 torch ops in the same order: bit-identical
+    # (tolerances of 1e-7 of the largest magnitude only absorb a different CPU's libm)
     for k in ("rgb", "depth", "normal", "opacity"):
-        np.testing.assert_allclose(out[k].detach().numpy(), g[k], rtol=0, atol=1e-6)
+        np.testing.assert_allclose(out[k].detach().numpy(), g[k], rtol=0, atol=1e-7)
     L = ((out["rgb"] * torch.from_numpy(g["r_rgb"])).sum() + (out["depth"] * torch.from_numpy(g["r_d"])).sum()
          + (out["opacity"] * torch.from_numpy(g["r_o"])).sum() + (out["normal"] * torch.from_numpy(g["r_n"])).sum())
     L.backward()
-    np.testing.assert_allclose(ts[0].grad.numpy(), g["d_sdf"], rtol=1e-4, atol=1e-5)
-    np.testing.assert_allclose(ts[1].grad.numpy(), g["d_nrm"], rtol=1e-4, atol=1e-5)
-    np.testing.assert_allclose(ts[2].grad.numpy(), g["d_col"], rtol=1e-5, atol=1e-6)
-    assert abs(s.grad.item() - float(g["d_s"])) <= 1e-4 * max(1.0, abs(float(g["d_s"])))
+    for t, k in zip(ts, ("d_sdf", "d_nrm", "d_col")):
+        np.testing.assert_allclose(t.grad.numpy(), g[k], rtol=0, atol=1e-7 * np.abs(g[k]).max(), err_msg=k)
+    assert abs(s.grad.item() - float(g["d_s"])) <= 1e-7 * abs(float(g["d_s"]))
 
 
 def test_alpha_closed_forms():
@@ -304,22 +305,24 @@ def test_train_step_matches_reference(name):
     assert np.array_equal(r["corner_indices"][..., 0, :].numpy().astype(np.int16), g["corner_c0"])
     assert np.array_equal(out["active_linear"].numpy().astype(np.int32), g["vr_lin"])
     for k in ("rgb", "depth", "opacity", "weights", "alpha", "transmittance", "sdf_samples", "grad_samples"):
-        np.testing.assert_allclose(r[k].detach().numpy(), g[k], rtol=1e-5, atol=1e-6, err_msg=k)
+        np.testing.assert_allclose(r[k].detach().numpy(), g[k], rtol=1e-6, atol=1e-7, err_msg=k)
     assert abs(out["loss"].item() - float(g["loss_rgb"])) < 1e-7
     assert abs(out["loss_eik"].item() - float(g["loss_eik"])) <= 1e-5 * abs(float(g["loss_eik"]))
     assert abs(out["loss_curv"].item() - float(g["loss_curv"])) <= 1e-5 * abs(float(g["loss_curv"]))
     gg = m.grid.grad.numpy().reshape(-1)
     ref = np.zeros_like(gg)
     ref[g["grid_grad_idx"]] = g["grid_grad_val"]
-    np.testing.assert_allclose(gg, ref, rtol=1e-4, atol=1e-4 * np.abs(ref).max())
+    # gradients within 1e-6 of the largest magnitude (measured: grid <= 3.6e-7, W0/W1 <= 2.7e-7,
+    # the hash table, biases and s bit-identical): only the summation order differs
+    np.testing.assert_allclose(gg, ref, rtol=0, atol=1e-6 * np.abs(ref).max())
     eg = m.embeddings.grad.numpy().reshape(-1)
     assert int(np.count_nonzero(eg)) == int(g["emb_grad_nnz"])
-    np.testing.assert_allclose(eg[g["emb_grad_idx"]], g["emb_grad_val"], rtol=1e-4, atol=1e-4 * np.abs(eg).max())
+    np.testing.assert_allclose(eg[g["emb_grad_idx"]], g["emb_grad_val"], rtol=0, atol=1e-6 * np.abs(eg).max())
     for k, gk in (("W0", "dW0"), ("b0", "db0"), ("W1", "dW1"), ("b1", "db1")):
         t = getattr(m, k).grad.numpy()
-        np.testing.assert_allclose(t, g[gk], rtol=1e-4, atol=1e-4 * np.abs(g[gk]).max())
+        np.testing.assert_allclose(t, g[gk], rtol=0, atol=1e-6 * np.abs(g[gk]).max())
     assert bool(out["amplified"]) == bool(g["amplified"])
-    assert abs(m.variance.grad.item() - float(g["dv_post"])) <= 1e-4 * abs(float(g["dv_post"]))
+    assert abs(m.variance.grad.item() - float(g["dv_post"])) <= 1e-6 * abs(float(g["dv_post"]))
 
 
 def test_eikonal_curvature_vs_fp64_autograd():
