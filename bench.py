@@ -25,6 +25,7 @@ of the same workload.
 from __future__ import annotations
 
 import argparse
+import gc
 import json
 import os
 import sys
@@ -118,12 +119,17 @@ def run_gpu(sc, batches, form, steps, warmup, concurrent_scatter=False):
     for i in range(warmup):
         one(i, False)
     torch.cuda.synchronize()
+    # a full collection of the Python heap takes tens of ms: run it now, not inside the
+    # timed steps, whose host side is only the launch sequence
+    gc.collect()
+    gc.disable()
     t0 = time.perf_counter()
     for i in range(steps):
         out = one(i, True)
     t_host = time.perf_counter() - t0  # host time to queue the steps
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    gc.enable()
     kev = KERNEL_EVENTS_CONCURRENT if st.cfg.concurrent_scatter else KERNEL_EVENTS
     kern = {k: float(np.mean([e.elapsed_ms(i, j) for e in ev])) for k, (i, j) in kev.items()}
     loss = out["loss"].item()

@@ -287,6 +287,22 @@ typedef struct vxn_render_args {
      scatter (its span is kernel_events[2] -> [4]). */
   void* scatter_stream;
   void* const* scatter_events;
+  /* nullable device float: the variance v of SingleVarianceNetwork (models/variance.py:
+     31-41).  With it the kernels evaluate s = clamp(exp(10 v), 1e-4, s_max) themselves (in
+     place of s and s_dev); backward and training entries then also write s_out (nullable,
+     that s) and d_variance (nullable) = dL/ds * e * [1e-4 <= e <= s_max] * 10 with
+     e = exp(10 v), and with amplified (nullable) apply amplify_s_gradient
+     (training/scheduler.py:14-33): a negative d_variance is multiplied by s_amp_k and
+     amplified[0] records whether it was. */
+  const float* variance;
+  float s_max;
+  float* s_out;
+  float* d_variance;
+  float s_amp_k;
+  int32_t* amplified;
+  /* backward and training entries: 1 = grad_table is overwritten (every entry, zero where
+     no sample reaches it) instead of accumulated, so the caller need not clear it */
+  int32_t grad_table_overwrite;
 } vxn_render_args;
 
 /* Workspace of the backward and training entries (the forward needs none): per-sample

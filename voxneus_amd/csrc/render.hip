@@ -82,6 +82,8 @@ struct RenderDev {
   const float* mlp;
   float s, sphere_r, cos_anneal;
   const float* s_dev;  // nullable: s read from device memory
+  const float* var_dev;  // nullable: s = clamp(exp(10 v), 1e-4, s_max) from the variance v
+  float s_max;
   int iter_cos;
   float *rgb, *depth, *normal, *opacity, *weights, *alpha, *trans, *sdf_s, *grad_s, *normal_s;
   int64_t* corners;
@@ -242,7 +244,7 @@ __global__ void __launch_bounds__(kRT, 2) render_kernel(RenderDev a) {
     }
     const GridDesc& G = MULTI ? scn->g : a.g;
     const HashDesc& Hd = MULTI ? scn->h : a.h;
-    const float sv = MULTI ? scn->s : (a.s_dev ? *a.s_dev : a.s);
+    const float sv = MULTI ? scn->s : (a.var_dev ? s_of_variance(*a.var_dev, a.s_max) : (a.s_dev ? *a.s_dev : a.s));
     const float sr = MULTI ? scn->sphere_r : a.sphere_r;
     const float* W0 = MULTI ? scn->mlp : a.mlp;
     const float* b0 = W0 + ML::B0;
@@ -1269,7 +1271,8 @@ __global__ void __launch_bounds__(256) bin_col_positions_kernel(BinDev b) {
 constexpr int kSumSplit = 4;
 constexpr int kSumEntries = kBinEntries / kSumSplit;
 
-__global__ void __launch_bounds__(64) hash_bin_sum_kernel(BinDev b, float* __restrict__ grad_table, int64_t T) {
+__global__ void __launch_bounds__(64) hash_bin_sum_kernel(BinDev b, float* __restrict__ grad_table, int64_t T,
+                                                         int overwrite) {
   __shared__ float2 acc[kSumEntries + 64];
   __shared__ uint8_t tag[kSumEntries + 64];
   const int lane = threadIdx.x;
@@ -1340,7 +1343,9 @@ __global__ void __launch_bounds__(64) hash_bin_sum_kernel(BinDev b, float* __res
   float2* g = (float2*)(grad_table + ((int64_t)l * T + base) * 2);
   for (int e = lane; e < ne; e += 64) {
     const float2 sum = acc[e];
-    if (sum.x != 0.0f || sum.y != 0.0f) {
+    if (overwrite) {
+      g[e] = sum;
+    } else if (sum.x != 0.0f || sum.y != 0.0f) {
       float2 o = g[e];
       o.x = o.x + sum.x;
       o.y = o.y + sum.y;
@@ -1489,12 +1494,30 @@ __global__ void __launch_bounds__(256) render_reduce1_kernel(const float* __rest
   rows[(int64_t)r * np + k] = acc;
 }
 
+// Clears n floats (the atomic hash scatter's table gradient when it is to be overwritten;
+// that path also takes tables that are not 8-byte aligned).
+__global__ void __launch_bounds__(256) zero_f32_kernel(float* __restrict__ p, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0.0f;
+}
+
+// The variance outputs of the reductions (nullable var: none).
+struct VarianceOut {
+  const float* var;
+  float s_max;
+  float* s_out;
+  float* d_var;
+  float amp_k;
+  int32_t* amplified;
+};
+
 __global__ void __launch_bounds__(256) render_reduce2_kernel(const float* __restrict__ rows,
                                                              const double* __restrict__ part_sc, int nb,
                                                              float* __restrict__ grad_mlp, float* __restrict__ grad_s,
                                                              double* __restrict__ loss, float inv_n3, int np,
                                                              const float* __restrict__ loss_norm, float mask_weight,
-                                                             float inv_r, double* __restrict__ loss_parts) {
+                                                             float inv_r, double* __restrict__ loss_parts,
+                                                             VarianceOut vo) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k < np) {
     double acc = 0.0;
@@ -1512,7 +1535,24 @@ __global__ void __launch_bounds__(256) render_reduce2_kernel(const float* __rest
     ls = wave_sum_d(ls);
     lb = wave_sum_d(lb);
     if (threadIdx.x == 0) {
-      if (grad_s) grad_s[0] = (float)ds;
+      const float gs = (float)ds;
+      if (grad_s) grad_s[0] = gs;
+      if (vo.var) {
+        // dL/dv of s = clamp(exp(10 v), 1e-4, s_max) (models/variance.py:38-41), then
+        // amplify_s_gradient (training/scheduler.py:14-33), in PyTorch's operation order
+        const float e = expf(vo.var[0] * 10.0f);
+        if (vo.s_out) vo.s_out[0] = fminf(fmaxf(e, 1e-4f), vo.s_max);
+        if (vo.d_var) {
+          const float inside = (e >= 1e-4f && e <= vo.s_max) ? 1.0f : 0.0f;
+          float dv = ((gs * e) * inside) * 10.0f;
+          if (vo.amplified) {
+            const bool neg = dv < 0.0f;
+            if (neg) dv = dv * vo.amp_k;
+            vo.amplified[0] = neg ? 1 : 0;
+          }
+          vo.d_var[0] = dv;
+        }
+      }
       const double lc = ls * (double)(loss_norm ? loss_norm[0] : inv_n3);
       const double lm = lb * (double)inv_r;
       if (loss) loss[0] = lc + (double)mask_weight * lm;
@@ -1659,6 +1699,8 @@ static int build_render_dev(const vxn_render_args& A, RenderDev& a, int mode) {
   a.mlp = A.mlp;
   a.s = A.s;
   a.s_dev = A.s_dev;
+  a.var_dev = A.variance;
+  a.s_max = A.s_max;
   a.sphere_r = A.sphere_radius;
   a.cos_anneal = A.cos_anneal_ratio;
   a.iter_cos = A.use_iter_cos;
@@ -1698,6 +1740,7 @@ static int build_render_dev(const vxn_render_args& A, RenderDev& a, int mode) {
   a.ray_scene = nullptr;
   a.n_rays_dev = nullptr;
   a.n_scenes = 0;
+  VXN_REQUIRE(A.variance == nullptr || A.s_max > 0.0f, VXN_ERR_VALUE, "s_max must be > 0 with a variance");
   if (mode != kFwd) {
     VXN_REQUIRE(A.grad_grid && A.grad_table && A.grad_mlp && A.grad_s, VXN_ERR_NULL, "null gradient output");
     const RenderWs w = render_ws(A);
@@ -1738,9 +1781,17 @@ static int launch_render_nv(const vxn_render_args* p, void* stream) {
   VXN_REQUIRE(p != nullptr, VXN_ERR_NULL, "args is null");
   RenderDev a;
   if (int e = build_render_dev(*p, a, MODE)) return e;
-  if (p->n_rays == 0) return 0;
-  const int nb = render_blocks(p->n_rays);
   hipStream_t s = (hipStream_t)stream;
+  if (p->n_rays == 0) {
+    if (MODE != kFwd && p->grad_table_overwrite) {
+      const int64_t nf = ((int64_t)a.h.L << p->log2_T) * 2;
+      const unsigned nz = (unsigned)((nf + 255) / 256 < 65536 ? (nf + 255) / 256 : 65536);
+      hipLaunchKernelGGL(zero_f32_kernel, dim3(nz), dim3(256), 0, s, p->grad_table, nf);
+      VXN_LAUNCH_CHECK();
+    }
+    return 0;
+  }
+  const int nb = render_blocks(p->n_rays);
   hipEvent_t const* ev = (hipEvent_t const*)p->kernel_events;
   if (ev) VXN_HIP(hipEventRecord(ev[0], s));
   if (MODE != kFwd && p->mlp_mode == VXN_MLP_BF16_MFMA) {
@@ -1811,9 +1862,15 @@ static int launch_render_nv(const vxn_render_args* p, void* stream) {
       VXN_LAUNCH_CHECK();
       const unsigned nsum = (unsigned)(((b.nbins + 7) / 8) * 8 * kSumSplit);
       hipLaunchKernelGGL(hash_bin_sum_kernel, dim3(nsum), dim3(64), 0, s, b, p->grad_table,
-                         (int64_t)1 << p->log2_T);
+                         (int64_t)1 << p->log2_T, p->grad_table_overwrite ? 1 : 0);
       VXN_LAUNCH_CHECK();
     } else {
+      if (p->grad_table_overwrite) {
+        const int64_t nf = ((int64_t)a.h.L << p->log2_T) * 2;
+        const unsigned nz = (unsigned)((nf + 255) / 256 < 65536 ? (nf + 255) / 256 : 65536);
+        hipLaunchKernelGGL(zero_f32_kernel, dim3(nz), dim3(256), 0, s, p->grad_table, nf);
+        VXN_LAUNCH_CHECK();
+      }
       hipLaunchKernelGGL(hash_scatter_kernel, dim3(sb), dim3(256), 0, s, a, dedup_levels(a.h));
       VXN_LAUNCH_CHECK();
     }
@@ -1831,7 +1888,8 @@ static int launch_render_nv(const vxn_render_args* p, void* stream) {
     hipLaunchKernelGGL(render_reduce2_kernel, dim3((np + 1 + 255) / 256), dim3(256), 0, s, rows, a.part_sc, nb,
                        p->grad_mlp, p->grad_s, MODE == kTrain ? p->loss : nullptr, a.inv_n3, np,
                        a.target_mask ? a.loss_norm : nullptr, a.target_mask ? a.mask_weight : 0.0f, a.inv_r,
-                       MODE == kTrain ? p->loss_parts : nullptr);
+                       MODE == kTrain ? p->loss_parts : nullptr,
+                       VarianceOut{p->variance, p->s_max, p->s_out, p->d_variance, p->s_amp_k, p->amplified});
     VXN_LAUNCH_CHECK();
     if (ss) VXN_HIP(hipStreamWaitEvent(s, se[1], 0));
     if (ev) VXN_HIP(hipEventRecord(ev[5], s));

@@ -68,6 +68,10 @@ VXN_DEV float sigmoid_ref(float a) {
   return 1.0f / (1.0f + e);
 }
 
+// NeuS sharpness from the variance parameter, s = clamp(exp(10 v), 1e-4, s_max)
+// (models/variance.py:31), with the device expf that PyTorch's GPU exp also calls.
+VXN_DEV float s_of_variance(float v, float s_max) { return fminf(fmaxf(expf(v * 10.0f), 1e-4f), s_max); }
+
 // ------------------------------------------------------------------------------------
 // Trilinear SDF as F.grid_sample(mode="bilinear", padding_mode="border",
 // align_corners=True) on (z,y,x)-flipped [-1,1] coordinates (sdf_grid.py:90-120),

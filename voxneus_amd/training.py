@@ -146,7 +146,7 @@ class FusedTrainStep:
                              f"{N.mlp_params(N.view_embed_code(cfg.view_embed_type))}")
         self.grad_mlp = torch.zeros(mlp.numel(), device=dev, dtype=torch.float32)
         self.grad_s = torch.zeros(1, device=dev, dtype=torch.float32)
-        self.grad_v = torch.zeros((), device=dev, dtype=torch.float32)
+        self.grad_v = torch.zeros(1, device=dev, dtype=torch.float32)
         self.loss = torch.zeros(1, device=dev, dtype=torch.float64)
         self.loss_parts = torch.zeros(2, device=dev, dtype=torch.float64)  # colour loss, mask BCE
         self.amplified = torch.zeros(1, device=dev, dtype=torch.int32)
@@ -180,13 +180,14 @@ class FusedTrainStep:
         self.level_w = level_weights(len(self.level_res), active_levels)
 
     def _render_args(self, rays_o, rays_d, mid, dists, target, target_mask=None):
-        """The render arguments of this step.  The static part is built once; s stays on the
-        device (``s_dev``), so building the arguments reads nothing back from the GPU."""
+        """The render arguments of this step.  The static part is built once.  The kernels
+        read the variance v on the device and evaluate s = clamp(exp(10 v), 1e-4, s_max)
+        (variance.py:31) themselves; the reductions write s, dL/dv and the amplification
+        flag, so building the arguments reads nothing back from the GPU."""
         cfg = self.cfg
-        # s = clamp(exp(10 v), 1e-4, s_max) (variance.py:31), computed on the device
-        torch.clamp(torch.exp(self.variance.detach() * 10.0), min=1e-4, max=self.s_max, out=self.s_value)
         R, S = mid.shape
-        key = (R, S, mid.device, self.grid.data_ptr(), self.table.data_ptr(), self.mlp.data_ptr())
+        key = (R, S, mid.device, self.grid.data_ptr(), self.table.data_ptr(), self.mlp.data_ptr(),
+               self.variance.data_ptr())
         a = self._args
         if a is None or self._args_key != key:
             rc = Fn.RenderConfig(self.box, self.level_res, self.level_w, cfg.sphere_radius, cfg.cos_anneal_ratio,
@@ -195,7 +196,10 @@ class FusedTrainStep:
                                  view_embed=N.view_embed_code(cfg.view_embed_type))
             a = N.RenderArgs()
             Fn.fill_render_args(a, rc, self.grid, self.table, self.mlp, 0.0, rays_o, rays_d, mid, dists)
-            a.s_dev = N.dptr(self.s_value)
+            a.variance = N.dptr(self.variance, torch.float32, "variance")
+            a.s_max, a.s_out = float(self.s_max), N.dptr(self.s_value)
+            a.d_variance = N.dptr(self.grad_v)
+            a.grad_table_overwrite = 1
             a.huber_delta = cfg.huber_delta
             a.loss_type = N.loss_type_code(cfg.loss_type)
             a.mask_weight = float(cfg.mask_weight)
@@ -218,6 +222,8 @@ class FusedTrainStep:
         a.target_rgb = N.dptr(target, torch.float32, "target")
         a.target_mask = N.dptr(target_mask, torch.float32, "target_mask")
         a.vr_bitmask = N.dptr(self.bitmask)
+        # amplify_s_gradient (scheduler.py:14-33) on dL/dv in the reductions
+        a.s_amp_k, a.amplified = float(cfg.s_amp_k), (N.dptr(self.amplified) if cfg.amplify else None)
         if self.scatter is not None:
             a.scatter_stream = self.scatter.handle
             a.scatter_events = ctypes.cast(self.scatter.events, ctypes.c_void_p)
@@ -274,8 +280,7 @@ class FusedTrainStep:
         if cooldown and (self._g_eik is None or self._g_eik.shape != self.grid.shape):
             self._g_eik = torch.zeros_like(self.grid)
             self._clamp_ws = Fn.eik_clamp_workspace(self.grid.device)
-        self.grad_grid.zero_()
-        self.grad_table.zero_()
+        self.grad_grid.zero_()  # the render call overwrites grad_table
         self.bitmask.zero_()
         # with the sampler in this step, it marks V_R and the regularisers run on the side
         # stream beside the render kernel and the scatters; both only add into grad_grid
@@ -302,9 +307,6 @@ class FusedTrainStep:
             self.side.finish()
         else:
             self._regularisers()
-        self.grad_v = s_grad_to_v(self.grad_s, self.variance, self.s_max).reshape(1).contiguous()
-        if cfg.amplify:
-            Fn.s_amplify(self.grad_v, cfg.s_amp_k, self.amplified)
         if early:
             self.side.join(main)  # grad_grid, the losses and V_R counts are complete past this point
         if cooldown:
