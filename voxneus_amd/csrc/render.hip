@@ -982,8 +982,9 @@ __global__ void __launch_bounds__(256) hash_scatter_kernel(RenderDev a, int dedu
 // record is factored into one 16-byte store: the entries and flags in the first word, then
 // x's cell fraction fx and the two features' gradients times the pair's y-z weight (the x
 // corner takes (1 - fx) of them, the x + 1 corner fx), so those records need no separate
-// entry array.  Every position comes from a wavefront's own row and LDS counters, so the
-// records' order in a bin, and the gradient, are the same on every run.
+// entry array.  Every position comes from a tile's count row, a scan and LDS counters that
+// only one wavefront touches, so the records' order in a bin, and the gradient, are the
+// same on every run.
 // ------------------------------------------------------------------------------------
 constexpr int kBinBits = 11;
 constexpr int kBinEntries = 1 << kBinBits;
@@ -991,15 +992,15 @@ constexpr int kBinTileWaves = 4;            // wavefronts per work-group of the 
 constexpr int kBinWaveChunks = 16;          // 64-sample chunks per wavefront
 constexpr int kBinWaveSamples = 64 * kBinWaveChunks;
 constexpr int kBinTile = kBinWaveSamples * kBinTileWaves;  // 4096 samples per work-group
-constexpr int kMaxBinsPerLevel = 1024;      // LDS counters per wavefront: 4 KB (tables up to 2^21)
+constexpr int kMaxBinsPerLevel = 1024;      // tables up to 2^21 entries per level
 
 struct BinDev {
   int nbl;          // bins per level: ceil(T / kBinEntries)
   int nbins;        // L * nbl, in (level, bin) order
   int ntile;        // sample tiles of kBinTile: work-groups per level of passes 1 and 3
-  int nrows;        // count rows per level: one per wavefront, ntile * kBinTileWaves
+  int nrows;        // count rows per level: one per tile (ntile)
   int nchunk;       // row chunks of the column scan
-  int32_t* cnt;     // [L][nrows][nbl]: each wavefront's record count per bin, then (scanned in
+  int32_t* cnt;     // [L][nrows][nbl]: each tile's record count per bin, then (scanned in
                     // place) the position of its first record in each bin
   int32_t* csum;    // [L][nchunk][nbl]: column sums of row chunks, then their exclusive scan
   int32_t* base;    // [nbins + 1]: first record of each bin; base[nbins] = all records
@@ -1017,30 +1018,55 @@ constexpr uint32_t kFactSecond = 2u << 22;  // split pair: only the x + 1 corner
 // cell below `dedup_levels`, every live lane above), v their 8 corners' weighted
 // gradients (run sums) and d0, d1 the level's feature gradients of the lane's own sample.
 // VALUES = false leaves v unset: only the tails are needed.
-template <bool VALUES, class F>
-VXN_DEV void hash_level_records(const RenderDev& a, int64_t i, int lane, int l, int dedup_levels, F&& emit) {
+// The inputs of sample i at level l: its two feature gradients, mid-point depth and ray
+// (zero past the last sample), loaded ahead of their use by the write pass.
+struct LevelIn {
+  float d0, d1, m, ox, oy, oz, rx, ry, rz;
+};
+
+VXN_DEV LevelIn level_in(const RenderDev& a, int64_t i, int l) {
+  const int64_t NS = a.R * (int64_t)a.S;
+  LevelIn x{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+  if (i < NS) {
+    const int64_t ray = i / a.S;
+    x.d0 = a.gfeat[(2 * l) * NS + i];
+    x.d1 = a.gfeat[(2 * l + 1) * NS + i];
+    x.m = a.mid[i];
+    x.ox = a.o[3 * ray];
+    x.oy = a.o[3 * ray + 1];
+    x.oz = a.o[3 * ray + 2];
+    x.rx = a.d[3 * ray];
+    x.ry = a.d[3 * ray + 1];
+    x.rz = a.d[3 * ray + 2];
+  }
+  return x;
+}
+
+// hash_level_eval returns false (and sets nothing) when no lane of the wavefront emits.
+template <bool VALUES>
+VXN_DEV bool hash_level_eval(const RenderDev& a, int64_t i, const LevelIn& x, int lane, int l, int dedup_levels,
+                             HashCell& c, float (&v)[16], float& d0, float& d1, bool& tail) {
   const int64_t NS = a.R * (int64_t)a.S;
   const bool in = i < NS;
   const float lw = a.h.lw[l];
-  if (lw == 0.0f) return;
-  float d0 = in ? a.gfeat[(2 * l) * NS + i] : 0.0f;
-  float d1 = in ? a.gfeat[(2 * l + 1) * NS + i] : 0.0f;
+  if (lw == 0.0f) return false;
+  d0 = x.d0;
+  d1 = x.d1;
   if (lw < 1.0f) {
     d0 = d0 * lw;
     d1 = d1 * lw;
   }
   const bool live = d0 != 0.0f || d1 != 0.0f;
-  if (__ballot(live) == 0) return;
+  if (__ballot(live) == 0) return false;
   float x0 = 0.0f, x1 = 0.0f, x2 = 0.0f;
   if (in) {
-    float px, py, pz;
-    sample_pos(a, i, px, py, pz);
+    // sample_pos's arithmetic on the loaded ray
+    const float px = x.ox + x.rx * x.m, py = x.oy + x.ry * x.m, pz = x.oz + x.rz * x.m;
     x0 = fminf(fmaxf((px - a.g.mn[0]) / a.g.den[0], 0.0f), 1.0f);
     x1 = fminf(fmaxf((py - a.g.mn[1]) / a.g.den[1], 0.0f), 1.0f);
     x2 = fminf(fmaxf((pz - a.g.mn[2]) / a.g.den[2], 0.0f), 1.0f);
   }
-  const HashCell c = hash_cell(x0, x1, x2, a.h.res[l], a.h.mask);
-  float v[16];
+  c = hash_cell(x0, x1, x2, a.h.res[l], a.h.mask);
   if constexpr (VALUES) {
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
@@ -1048,7 +1074,7 @@ VXN_DEV void hash_level_records(const RenderDev& a, int64_t i, int lane, int l, 
       v[2 * k + 1] = d1 * c.w[k];
     }
   }
-  bool tail = live;
+  tail = live;
   if (l < dedup_levels) {
     const float res = a.h.res[l];
     const uint64_t cx = (uint64_t)(int)floorf(x0 * res), cy = (uint64_t)(int)floorf(x1 * res),
@@ -1061,7 +1087,17 @@ VXN_DEV void hash_level_records(const RenderDev& a, int64_t i, int lane, int l, 
       tail = (lane == 63 || kn != key) && live;
     }
   }
-  emit(tail, c, v, d0, d1);
+  return true;
+}
+
+template <bool VALUES, class F>
+VXN_DEV void hash_level_records(const RenderDev& a, int64_t i, int lane, int l, int dedup_levels, F&& emit) {
+  HashCell c;
+  float v[16];
+  float d0, d1;
+  bool tail;
+  if (hash_level_eval<VALUES>(a, i, level_in(a, i, l), lane, l, dedup_levels, c, v, d0, d1, tail))
+    emit(tail, c, v, d0, d1);
 }
 
 VXN_DEV void wave_sync() {
@@ -1070,18 +1106,25 @@ VXN_DEV void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Passes 1 and 3 run one work-group per (level, 4096-sample tile), level-major; each of its
-// four wavefronts walks 1024 consecutive samples in 16 chunks and owns one row of counters
-// (LDS, then cnt[level][row]).  A wavefront's positions in a bin come from its own row of
-// the scan and from LDS atomics that only its own instructions touch, so every record's
-// position - and the order in which pass 4 adds it - is the same on every run.
+// Passes 1 and 3 run one work-group per (level, 4096-sample tile), level-major: its four
+// wavefronts walk 1024 consecutive samples each, in 16 chunks of 64, and the tile owns one
+// row of counters (LDS, then cnt[level][tile]).  Pass 3 takes the chunks in lockstep: the
+// records of chunk c of all four wavefronts (at most kStageSlots) are ranked in each bin,
+// sorted by bin through LDS, and stored as runs that continue the tile's previous run in
+// that bin, so consecutive lanes store to consecutive slots.  A record's place in its bin
+// follows (chunk, wavefront, lane, corner pair): ranks come from LDS atomics on a counter
+// row that only the wavefront's own instructions touch, offsets from a scan, so every
+// position - and the order in which pass 4 adds the records - is the same on every run.
+// Staged records per chunk: 4 wavefronts x 64 lanes x 4 pairs; the rare split pairs beyond
+// that are stored directly.
+constexpr int kStageSlots = kBinTileWaves * 64 * 4;
+
 __global__ void __launch_bounds__(256) hash_bin_count_kernel(RenderDev a, BinDev b, int dedup_levels) {
-  extern __shared__ int32_t cnt_lds[];  // [kBinTileWaves][nbl]
+  extern __shared__ int32_t cnt_lds[];  // [nbl]
   const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int l = (int)(blockIdx.x / (unsigned)b.ntile), tile = (int)(blockIdx.x - (unsigned)l * b.ntile);
-  int32_t* mine = cnt_lds + wid * b.nbl;
-  for (int k = lane; k < b.nbl; k += 64) mine[k] = 0;
-  wave_sync();
+  for (int k = threadIdx.x; k < b.nbl; k += 256) cnt_lds[k] = 0;
+  __syncthreads();
   const int64_t i0 = (int64_t)tile * kBinTile + (int64_t)wid * kBinWaveSamples;
   for (int c = 0; c < kBinWaveChunks; ++c) {
     hash_level_records<false>(a, i0 + c * 64 + lane, lane, l, dedup_levels,
@@ -1090,74 +1133,195 @@ __global__ void __launch_bounds__(256) hash_bin_count_kernel(RenderDev a, BinDev
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const int ba = (int)(cell.idx[q] >> kBinBits), bb = (int)(cell.idx[q + 4] >> kBinBits);
-          atomicAdd(&mine[ba], 1);
-          if (bb != ba) atomicAdd(&mine[bb], 1);
+          atomicAdd(&cnt_lds[ba], 1);
+          if (bb != ba) atomicAdd(&cnt_lds[bb], 1);
         }
       }
     });
   }
-  wave_sync();
-  int32_t* out = b.cnt + ((int64_t)l * b.nrows + (int64_t)tile * kBinTileWaves + wid) * b.nbl;
-  for (int k = lane; k < b.nbl; k += 64) out[k] = mine[k];
+  __syncthreads();
+  int32_t* out = b.cnt + ((int64_t)l * b.nrows + tile) * b.nbl;
+  for (int k = threadIdx.x; k < b.nbl; k += 256) out[k] = cnt_lds[k];
 }
 
-// Records straight to the wavefront's reserved positions in each bin.
-__global__ void __launch_bounds__(256) hash_bin_write_kernel(RenderDev a, BinDev b, int dedup_levels) {
-  extern __shared__ int32_t pos_lds[];  // [kBinTileWaves][nbl]
-  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int l = (int)(blockIdx.x / (unsigned)b.ntile), tile = (int)(blockIdx.x - (unsigned)l * b.ntile);
-  int32_t* pos = pos_lds + wid * b.nbl;
-  const int32_t* row = b.cnt + ((int64_t)l * b.nrows + (int64_t)tile * kBinTileWaves + wid) * b.nbl;
-  for (int k = lane; k < b.nbl; k += 64) pos[k] = row[k];
-  wave_sync();
-  const bool fact = l >= b.fact_level;
-  const int64_t i0 = (int64_t)tile * kBinTile + (int64_t)wid * kBinWaveSamples;
+// A work-group barrier that orders LDS only: loads from global memory issued before it stay
+// in flight (the write pass prefetches the next chunk's inputs across its barriers).
+VXN_DEV void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// FACT: the launch covers levels l0.. of factored records (no run sums, no entry array).
+template <bool FACT>
+__global__ void __launch_bounds__(256) hash_bin_write_kernel(RenderDev a, BinDev b, int dedup_levels, int l0) {
+  extern __shared__ float4 wlds[];
+  float4* stage = wlds;                                          // [kStageSlots] sorted records
+  uint32_t* sridx = (uint32_t*)(stage + kStageSlots);            // [kStageSlots] their entry words
+  uint16_t* sbin = (uint16_t*)(sridx + kStageSlots);             // [kStageSlots] their bins
+  int32_t* wcnt = (int32_t*)(sbin + kStageSlots);                // [kBinTileWaves][nbl] ranks, then offsets
+  int32_t* gpos = wcnt + kBinTileWaves * b.nbl;                  // [nbl] the tile's next slot in each bin
+  int32_t* gb = gpos + b.nbl;                                    // [nbl] slot - stage index, this chunk
+  int32_t* wsum = gb + b.nbl;                                    // [4] scan of the wavefront totals
+  const int t = threadIdx.x, wid = t >> 6, lane = t & 63;
+  const int nbl = b.nbl;
+  const int lt = (int)(blockIdx.x / (unsigned)b.ntile), tile = (int)(blockIdx.x - (unsigned)lt * b.ntile);
+  const int l = l0 + lt;
+  const int32_t* row = b.cnt + ((int64_t)l * b.nrows + tile) * nbl;
+  for (int k = t; k < nbl; k += 256) gpos[k] = row[k];
+  constexpr bool fact = FACT;
+  int32_t* mine = wcnt + wid * nbl;
+  const int per = (nbl + 255) / 256;  // bins per thread in the scan (consecutive)
+  const int64_t i0 = (int64_t)tile * kBinTile + (int64_t)wid * kBinWaveSamples + lane;
+  constexpr uint32_t m = kBinEntries - 1;
+  LevelIn nxt = level_in(a, i0, l);
   for (int c = 0; c < kBinWaveChunks; ++c) {
-    hash_level_records<true>(a, i0 + c * 64 + lane, lane, l, dedup_levels,
-                             [&](bool tail, const HashCell& cell, const float(&v)[16], float d0, float d1) {
-      constexpr uint32_t m = kBinEntries - 1;
-      if (tail && fact) {
+    const LevelIn cur = nxt;
+    if (c + 1 < kBinWaveChunks) nxt = level_in(a, i0 + (c + 1) * 64, l);
+    for (int k = lane; k < nbl; k += 64) mine[k] = 0;
+    wave_sync();
+    // plain scalar arrays: the HIP vector types would keep these in scratch memory
+    float rvx[4], rvy[4], rvz[4], rvw[4], rwx[4], rwy[4];
+    uint32_t rx[4], rx2[4];
+    int rb[4], rb2[4], rk[4], rk2[4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const uint32_t ia = cell.idx[q], ib = cell.idx[q + 4];
-          const int ba = (int)(ia >> kBinBits), bb = (int)(ib >> kBinBits);
+    for (int q = 0; q < 4; ++q) {
+      rb[q] = -1;
+      rb2[q] = -1;
+    }
+    HashCell cell;
+    float v[16];
+    float d0, d1;
+    bool tail;
+    if (hash_level_eval<!FACT>(a, i0 + c * 64, cur, lane, l, dedup_levels, cell, v, d0, d1, tail) && tail) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const uint32_t ia = cell.idx[q], ib = cell.idx[q + 4];
+        const int ba = (int)(ia >> kBinBits), bb = (int)(ib >> kBinBits);
+        rb[q] = ba;
+        rk[q] = atomicAdd(&mine[ba], 1);
+        rx2[q] = 0u;
+        rwx[q] = 0.0f;
+        rwy[q] = 0.0f;
+        if constexpr (FACT) {
           const float wy = (q & 2) ? cell.dy : 1.0f - cell.dy;
           const float wz = (q & 1) ? cell.dz : 1.0f - cell.dz;
           const float wyz = wy * wz;
-          const float g0 = d0 * wyz, g1 = d1 * wyz;
-          const int32_t p = atomicAdd(&pos[ba], 1);
-          if (bb == ba) {
-            b.rval[p] = make_float4(__uint_as_float((ia & m) | ((ib & m) << 11)), cell.dx, g0, g1);
-          } else {
-            b.rval[p] = make_float4(__uint_as_float((ia & m) * 0x801u | kFactFirst), cell.dx, g0, g1);
-            const int32_t p2 = atomicAdd(&pos[bb], 1);
-            b.rval[p2] = make_float4(__uint_as_float((ib & m) * 0x801u | kFactSecond), cell.dx, g0, g1);
-          }
+          const uint32_t bits = bb == ba ? ((ia & m) | ((ib & m) << 11)) : ((ia & m) * 0x801u | kFactFirst);
+          rvx[q] = __uint_as_float(bits);
+          rvy[q] = cell.dx;
+          rvz[q] = d0 * wyz;
+          rvw[q] = d1 * wyz;
+          rx[q] = 0u;
+          rx2[q] = (ib & m) * 0x801u | kFactSecond;
+        } else if (bb == ba) {
+          rvx[q] = v[2 * q];
+          rvy[q] = v[2 * q + 1];
+          rvz[q] = v[2 * q + 8];
+          rvw[q] = v[2 * q + 9];
+          rx[q] = (ia & m) | ((ib & m) << 16);
+        } else {
+          rvx[q] = v[2 * q];
+          rvy[q] = v[2 * q + 1];
+          rvz[q] = 0.0f;
+          rvw[q] = 0.0f;
+          rx[q] = (ia & m) * 0x10001u;
+          rx2[q] = (ib & m) * 0x10001u;
+          rwx[q] = v[2 * q + 8];
+          rwy[q] = v[2 * q + 9];
         }
-      } else if (tail) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const uint32_t ia = cell.idx[q], ib = cell.idx[q + 4];
-          const int ba = (int)(ia >> kBinBits), bb = (int)(ib >> kBinBits);
-          const int32_t p = atomicAdd(&pos[ba], 1);
-          if (bb == ba) {
-            b.ridx[p] = (ia & m) | ((ib & m) << 16);
-            b.rval[p] = make_float4(v[2 * q], v[2 * q + 1], v[2 * q + 8], v[2 * q + 9]);
-          } else {
-            b.ridx[p] = (ia & m) * 0x10001u;
-            b.rval[p] = make_float4(v[2 * q], v[2 * q + 1], 0.0f, 0.0f);
-            const int32_t p2 = atomicAdd(&pos[bb], 1);
-            b.ridx[p2] = (ib & m) * 0x10001u;
-            b.rval[p2] = make_float4(v[2 * q + 8], v[2 * q + 9], 0.0f, 0.0f);
-          }
+        if (bb != ba) {
+          rb2[q] = bb;
+          rk2[q] = atomicAdd(&mine[bb], 1);
         }
       }
-    });
+    }
+    lds_barrier();
+    // offsets: bins in order, then wavefronts in order within a bin
+    int32_t tot = 0;
+    for (int j = 0; j < per; ++j) {
+      const int k = t * per + j;
+      if (k < nbl)
+#pragma unroll
+        for (int w = 0; w < kBinTileWaves; ++w) tot += wcnt[w * nbl + k];
+    }
+    int32_t inc = tot;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int32_t u = __shfl_up(inc, o, 64);
+      if (lane >= o) inc += u;
+    }
+    if (lane == 63) wsum[wid] = inc;
+    lds_barrier();
+    int32_t run = inc - tot;
+    for (int w = 0; w < wid; ++w) run += wsum[w];
+    const int32_t ntot = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    for (int j = 0; j < per; ++j) {
+      const int k = t * per + j;
+      if (k < nbl) {
+        const int32_t g = gpos[k];
+        gb[k] = g - run;
+        int32_t o = run;
+#pragma unroll
+        for (int w = 0; w < kBinTileWaves; ++w) {
+          const int32_t n = wcnt[w * nbl + k];
+          wcnt[w * nbl + k] = o;
+          o += n;
+        }
+        gpos[k] = g + (o - run);
+        run = o;
+      }
+    }
+    lds_barrier();
+    // records past the stage's capacity (many split pairs in one chunk) go straight out
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (rb[q] >= 0) {
+        const int slot = mine[rb[q]] + rk[q];
+        const float4 rec = make_float4(rvx[q], rvy[q], rvz[q], rvw[q]);
+        if (slot < kStageSlots) {
+          stage[slot] = rec;
+          sridx[slot] = rx[q];
+          sbin[slot] = (uint16_t)rb[q];
+        } else {
+          const int32_t dst = gb[rb[q]] + slot;
+          b.rval[dst] = rec;
+          if (!fact) b.ridx[dst] = rx[q];
+        }
+      }
+      if (rb2[q] >= 0) {
+        const int slot = mine[rb2[q]] + rk2[q];
+        const float4 rec = fact ? make_float4(__uint_as_float(rx2[q]), rvy[q], rvz[q], rvw[q])
+                                : make_float4(rwx[q], rwy[q], 0.0f, 0.0f);
+        if (slot < kStageSlots) {
+          stage[slot] = rec;
+          sridx[slot] = rx2[q];
+          sbin[slot] = (uint16_t)rb2[q];
+        } else {
+          const int32_t dst = gb[rb2[q]] + slot;
+          b.rval[dst] = rec;
+          if (!fact) b.ridx[dst] = rx2[q];
+        }
+      }
+    }
+    lds_barrier();
+    const int nst = ntot < kStageSlots ? ntot : kStageSlots;
+    for (int j = t; j < nst; j += 256) {
+      const int32_t dst = gb[sbin[j]] + j;
+      b.rval[dst] = stage[j];
+      if (!fact) b.ridx[dst] = sridx[j];
+    }
   }
 }
 
+// LDS of the write pass for nbl bins per level.
+static size_t bin_write_lds(int nbl) {
+  return (size_t)kStageSlots * (sizeof(float4) + sizeof(uint32_t) + sizeof(uint16_t)) +
+         ((size_t)kBinTileWaves * nbl + 2 * (size_t)nbl + 8) * sizeof(int32_t);
+}
+
 // Pass 2, the scan that turns the count rows into positions: a bin's records are stored in
-// row order (rows of one level in tile, then wavefront order), bins in (level, bin) order.
+// row (tile) order, bins in (level, bin) order.
 // Column chunk sums, an exclusive scan of each column's chunks (one thread per column),
 // one work-group's scan of the column totals (base), then each chunk writes its rows'
 // positions in place.
@@ -1254,47 +1418,43 @@ __global__ void __launch_bounds__(256) bin_col_positions_kernel(BinDev b) {
   }
 }
 
-// Pass 4: kSumSplit wavefronts per bin, each owning a quarter of its entries (4.6 KB of LDS
-// with its tags): each reads all the bin's records and adds the halves that fall in its
-// quarter.  The four wavefronts of a bin are work-groups b, b + 8, b + 16, b + 24 of one
-// group of 32, which the dispatcher places on the same XCD (work-group b runs on XCD
-// b % 8), so the bin's records come from HBM once and from that XCD's L2 three times.
-// LDS float atomics run at a few cycles per lane, so a quarter is summed with plain LDS
-// read-add-write instead: in each round every pending half-record writes its lane id to
-// its entry's tag, and a lane adds a half whose tag reads back its own id (one lane per
-// entry per round; a lane whose two halves share the entry adds both, in order); the
-// others retry in the next round.  A single wavefront owns the quarter's LDS and its LDS
-// instructions complete in order, so no update is lost, and the adds to an entry follow
-// the records' order, which the count and write passes fix.  A lane with nothing to add in
-// a round reads and writes its own dummy slot (entry kSumEntries + lane), so a round is
-// straight-line LDS code: both tag reads, then both accumulator reads, issued together.
-constexpr int kSumSplit = 4;
-constexpr int kSumEntries = kBinEntries / kSumSplit;
+// Pass 4: one work-group of kSumWaves wavefronts per bin.  Wavefront w reads the bin's
+// 64-record groups w, w + kSumWaves, ... (each record is read once) and adds them into its
+// own copy of the bin's 2048 entries in LDS; the copies are then added in wavefront order,
+// so the sum is the same on every run.  LDS float atomics run at a few cycles per lane, so a
+// copy is updated with plain LDS read-add-write instead: in each round every pending
+// half-record writes its lane id to its entry's tag, and a lane adds a half whose tag reads
+// back its own id (one lane per entry per round; a lane whose two halves share the entry
+// adds both, in order); the others retry in the next round.  A single wavefront owns a copy
+// and its LDS instructions complete in order, so no update is lost.  A lane with nothing to
+// add in a round reads and writes its own dummy slot (entry kBinEntries + lane), so a round
+// is straight-line LDS code.  The bins run on XCD bin % 8 (work-group b on XCD b % 8).
+constexpr int kSumWaves = 4;
+constexpr int kSumSlots = kBinEntries + 64;
 
-__global__ void __launch_bounds__(64) hash_bin_sum_kernel(BinDev b, float* __restrict__ grad_table, int64_t T,
-                                                         int overwrite) {
-  __shared__ float2 acc[kSumEntries + 64];
-  __shared__ uint8_t tag[kSumEntries + 64];
-  const int lane = threadIdx.x;
-  const int xcd = (int)(blockIdx.x & 7), k = (int)(blockIdx.x >> 3);
-  const int part = k & (kSumSplit - 1);
-  const int bin = (k / kSumSplit) * 8 + xcd;
+__global__ void __launch_bounds__(64 * kSumWaves) hash_bin_sum_kernel(BinDev b, float* __restrict__ grad_table,
+                                                                     int64_t T, int overwrite) {
+  extern __shared__ float2 sacc[];  // [kSumWaves][kSumSlots], then tags [kSumWaves][kSumSlots]
+  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int bin = blockIdx.x;
   if (bin >= b.nbins) return;
+  float2* acc = sacc + wid * kSumSlots;
+  uint8_t* tag = (uint8_t*)(sacc + kSumWaves * kSumSlots) + wid * kSumSlots;
   const int l = bin / b.nbl;
-  const int64_t base = ((int64_t)(bin - l * b.nbl) << kBinBits) + (int64_t)part * kSumEntries;
-  const int ne = (int)max((int64_t)0, min((int64_t)kSumEntries, T - base));
+  const int64_t base = (int64_t)(bin - l * b.nbl) << kBinBits;
+  const int ne = (int)max((int64_t)0, min((int64_t)kBinEntries, T - base));
   for (int e = lane; e < ne; e += 64) acc[e] = make_float2(0.0f, 0.0f);
-  const int dummy = kSumEntries + lane;
+  const int dummy = kBinEntries + lane;
   wave_sync();
   const int32_t r0 = b.base[bin], r1 = b.base[bin + 1];
   const bool fact = l >= b.fact_level;
-  constexpr int U = 8;  // 64-record groups loaded ahead
-  for (int32_t k0 = r0; k0 < r1; k0 += U * 64) {
+  constexpr int U = 4;  // 64-record groups loaded ahead
+  for (int32_t k0 = r0 + wid * 64; k0 < r1; k0 += U * 64 * kSumWaves) {
     uint32_t r[U];
     float4 v[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const int32_t kk = k0 + u * 64 + lane;
+      const int32_t kk = k0 + u * 64 * kSumWaves + lane;
       if (fact) {
         const float4 f = kk < r1 ? b.rval[kk] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         const uint32_t bits = __float_as_uint(f.x);
@@ -1311,9 +1471,9 @@ __global__ void __launch_bounds__(64) hash_bin_sum_kernel(BinDev b, float* __res
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const int e0 = (int)(r[u] & 0xFFFFu) - part * kSumEntries, e1 = (int)(r[u] >> 16) - part * kSumEntries;
-      const bool ok = k0 + u * 64 + lane < r1;
-      bool p0 = ok && e0 >= 0 && e0 < ne, p1 = ok && e1 >= 0 && e1 < ne;
+      const int e0 = (int)(r[u] & 0xFFFFu), e1 = (int)(r[u] >> 16);
+      const bool ok = k0 + u * 64 * kSumWaves + lane < r1;
+      bool p0 = ok, p1 = ok;
       while (__ballot(p0 || p1) != 0) {
         tag[p0 ? e0 : dummy] = (uint8_t)lane;
         tag[p1 ? e1 : dummy] = (uint8_t)lane;
@@ -1340,9 +1500,16 @@ __global__ void __launch_bounds__(64) hash_bin_sum_kernel(BinDev b, float* __res
       }
     }
   }
+  __syncthreads();
   float2* g = (float2*)(grad_table + ((int64_t)l * T + base) * 2);
-  for (int e = lane; e < ne; e += 64) {
-    const float2 sum = acc[e];
+  for (int e = threadIdx.x; e < ne; e += 64 * kSumWaves) {
+    float2 sum = sacc[e];
+#pragma unroll
+    for (int w = 1; w < kSumWaves; ++w) {
+      const float2 o = sacc[w * kSumSlots + e];
+      sum.x = sum.x + o.x;
+      sum.y = sum.y + o.y;
+    }
     if (overwrite) {
       g[e] = sum;
     } else if (sum.x != 0.0f || sum.y != 0.0f) {
@@ -1353,6 +1520,8 @@ __global__ void __launch_bounds__(64) hash_bin_sum_kernel(BinDev b, float* __res
     }
   }
 }
+
+constexpr size_t kSumLds = (size_t)kSumWaves * kSumSlots * (sizeof(float2) + 1);
 
 // Brick slots of the Eq. 7 stencil around a cell with base corner c0: 8 cell corners
 // (ox,oy,oz) then the 24 face neighbours at offset -1 / +2 along one axis.
@@ -1622,10 +1791,10 @@ static BinPlan bin_plan(int64_t ns, int L, int log2_T, const int32_t* level_res)
   for (int l = 0; l < L; ++l) per += level_res[l] >= kBinEntries - 1 ? 8 : 4;
   p.max_rec = ns * per;
   p.use = p.nbl <= kMaxBinsPerLevel && p.max_rec < ((int64_t)1 << 31) && ntile * L < ((int64_t)1 << 31) &&
-          ntile * kBinTileWaves * (int64_t)p.nbins < ((int64_t)1 << 31);
+          ntile * (int64_t)p.nbins < ((int64_t)1 << 31);
   if (!p.use) return p;
   p.ntile = (int)ntile;
-  p.nrows = p.ntile * kBinTileWaves;
+  p.nrows = p.ntile;
   p.nchunk = (p.nrows + kColChunk - 1) / kColChunk;
   p.n_cnt = (int64_t)L * p.nrows * p.nbl;
   p.n_csum = (int64_t)L * p.nchunk * p.nbl;
@@ -1845,9 +2014,8 @@ static int launch_render_nv(const vxn_render_args* p, void* stream) {
       b.rval = (float4*)(ws + w.off_rval);
       const int dl = dedup_levels(a.h);
       b.fact_level = dl;
-      const size_t lds = (size_t)kBinTileWaves * b.nbl * sizeof(int32_t);
       const unsigned nwg = (unsigned)(a.h.L * b.ntile);
-      hipLaunchKernelGGL(hash_bin_count_kernel, dim3(nwg), dim3(256), lds, s, a, b, dl);
+      hipLaunchKernelGGL(hash_bin_count_kernel, dim3(nwg), dim3(256), (size_t)b.nbl * sizeof(int32_t), s, a, b, dl);
       VXN_LAUNCH_CHECK();
       const dim3 cgrid((unsigned)((b.nbl + 255) / 256), (unsigned)b.nchunk, (unsigned)a.h.L);
       hipLaunchKernelGGL(bin_col_sums_kernel, cgrid, dim3(256), 0, s, b);
@@ -1858,10 +2026,19 @@ static int launch_render_nv(const vxn_render_args* p, void* stream) {
       VXN_LAUNCH_CHECK();
       hipLaunchKernelGGL(bin_col_positions_kernel, cgrid, dim3(256), 0, s, b);
       VXN_LAUNCH_CHECK();
-      hipLaunchKernelGGL(hash_bin_write_kernel, dim3(nwg), dim3(256), lds, s, a, b, dl);
-      VXN_LAUNCH_CHECK();
-      const unsigned nsum = (unsigned)(((b.nbins + 7) / 8) * 8 * kSumSplit);
-      hipLaunchKernelGGL(hash_bin_sum_kernel, dim3(nsum), dim3(64), 0, s, b, p->grad_table,
+      // levels below dl carry run sums (record values + entry array), the rest factored records
+      if (dl > 0) {
+        hipLaunchKernelGGL(hash_bin_write_kernel<false>, dim3((unsigned)(dl * b.ntile)), dim3(256),
+                           bin_write_lds(b.nbl), s, a, b, dl, 0);
+        VXN_LAUNCH_CHECK();
+      }
+      if (dl < a.h.L) {
+        hipLaunchKernelGGL(hash_bin_write_kernel<true>, dim3((unsigned)((a.h.L - dl) * b.ntile)), dim3(256),
+                           bin_write_lds(b.nbl), s, a, b, dl, dl);
+        VXN_LAUNCH_CHECK();
+      }
+      hipLaunchKernelGGL(hash_bin_sum_kernel, dim3((unsigned)b.nbins), dim3(64 * kSumWaves), kSumLds, s, b,
+                         p->grad_table,
                          (int64_t)1 << p->log2_T, p->grad_table_overwrite ? 1 : 0);
       VXN_LAUNCH_CHECK();
     } else {
