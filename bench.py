@@ -55,6 +55,10 @@ KERNEL_BYTES = {
     "fp32": {"render_kernel": 128 + 1024 + 9, "hash_gradient": 2048, "grid_scatter": 256},
     "bf16": {"hash_features": 512 + 64, "render_kernel": 128 + 64 + 9, "hash_gradient": 2048, "grid_scatter": 256},
 }
+# the sampler: 112 SDF queries x 8 corners x 4 B per 128-sample ray (no cache credit)
+SIDE_BYTES_PER_SAMPLE = {"upsample": 28}
+# names of the kernels in the committed rocprofv3 traffic summaries
+PROFILE_NAMES = {"upsample": "upsample_kernel", "reg_vr": "reg_vr_kernel"}
 # event index pairs bracketing each kernel in vxn_render_args.kernel_events
 KERNEL_EVENTS = {"hash_features": (0, 1), "render_kernel": (1, 2), "hash_gradient": (2, 3), "grid_scatter": (3, 4),
                  "reductions": (4, 5)}
@@ -133,8 +137,14 @@ def run_gpu(sc, batches, form, steps, warmup, concurrent_scatter=False):
     kev = KERNEL_EVENTS_CONCURRENT if st.cfg.concurrent_scatter else KERNEL_EVENTS
     kern = {k: float(np.mean([e.elapsed_ms(i, j) for e in ev])) for k, (i, j) in kev.items()}
     loss = out["loss"].item()
+    # the step's launches outside the render call, each timed alone on the last step's inputs:
+    # the sampler (with its V_R marks) and the regularisers over the V_R bitmask
+    o, d, _ = gb[(steps - 1) % len(gb)]
+    kern["upsample"] = _event_ms(lambda: st.sample(o, d, None, None, st.bitmask))
+    kern["reg_vr"] = _event_ms(st._regularisers)
+    n_vr = int(st.vr_counts[0].item())
     return {"ms_per_step": dt / steps * 1e3, "rays_per_s": RAYS * steps / dt, "kernel_ms": kern, "loss": loss,
-            "host_ms_per_step": t_host / steps * 1e3}
+            "host_ms_per_step": t_host / steps * 1e3, "n_vr": n_vr, "n_vox": st.grid.numel()}
 
 
 def run_full_render(form):
@@ -461,11 +471,17 @@ def main():
         log(f"gpu {form}: {r}")
         kernels = {}
         for k, ms in r["kernel_ms"].items():
-            if k not in KERNEL_BYTES[form] and k != "reductions":
+            if k not in KERNEL_BYTES[form] and k not in ("reductions", "upsample", "reg_vr"):
                 continue
-            bps = KERNEL_BYTES[form].get(k)
-            ach = None if bps is None else bps * ns / (ms * 1e-3) / 1e9
-            kernels[k] = {"ms": ms, "bytes_per_sample": bps, "achieved_GBps": ach}
+            bps = KERNEL_BYTES[form].get(k, SIDE_BYTES_PER_SAMPLE.get(k))
+            nbytes = None if bps is None else bps * ns
+            if k == "reg_vr":  # the bitmask, then 7 grid words read and 7 gradient words RMW per V_R vertex
+                nbytes = r["n_vox"] / 8 + r["n_vr"] * (7 * 4 + 7 * 8)
+            ach = None if nbytes is None else nbytes / (ms * 1e-3) / 1e9
+            traffic = profiled_traffic(PROFILE_NAMES.get(k, k), form)
+            kernels[k] = {"ms": ms, "bytes_per_sample": bps, "algorithmic_bytes": nbytes, "achieved_GBps": ach,
+                          "frac": None if ach is None else ach / HBM_PEAK_GBS, "traffic_bytes": traffic,
+                          "traffic_frac": None if traffic is None else traffic / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
         err = None if a.no_error_check else rgb_error_vs_fp64(sc, batches[0], form)
         log(f"rgb max-abs err vs fp64 ({form}): {err}")
         res[form] = {"rays_per_s": r["rays_per_s"], "samples_per_s": r["rays_per_s"] * SAMPLES,
@@ -474,7 +490,7 @@ def main():
     main_form = forms[0]
     r = res[main_form]
     kms = {k: v["ms"] for k, v in r["kernels"].items()}
-    dom = max(KERNEL_BYTES[main_form], key=lambda k: kms[k])
+    dom = max(KERNEL_BYTES[main_form], key=lambda k: kms[k])  # of the render call's kernels
     nbytes = KERNEL_BYTES[main_form][dom] * ns
     achieved = r["kernels"][dom]["achieved_GBps"]
     lat = None if a.no_lattice else run_lattice()
