@@ -353,9 +353,10 @@ int vxn_curvature_grad(const float* grid, int X, int Y, int Z, const int64_t* ac
 
 /* The same regularisers straight from the V_R bitmask the render marks, for the fused
  * step: counts[2] (device) receives {|V_R|, |V_R inside the relaxed sphere|} (box6 and
- * relax_radius as vxn_vr_compact), then work-groups walk the grid's (x, y) rows, test
- * V_R membership in the bitmask and add each member's eikonal and curvature terms with
- * one atomic per target (a z-line's own and neighbour terms merged across lanes).  Same
+ * relax_radius as vxn_vr_compact), then work-groups take tiles of 8 x 8 (x, y) rows by
+ * 64 z, test V_R membership in the bitmask, add each member's eikonal and curvature terms
+ * into the tile's LDS accumulator (the tile and a one-vertex border), and add that to
+ * grad_grid with one device atomic per reached vertex per tile.  Same
  * losses, gradient and |V_R| normalisation as vxn_reg_eik_curv on the compacted list;
  * replaces
  * extract_active_vertices + the relaxed-sphere filter + both regularisers

@@ -274,9 +274,8 @@ __global__ void __launch_bounds__(256) curvature_ref_kernel(GridDesc g, const in
 
 // ------------------------------------------------------------------------------------
 // V_R straight from the bitmask: per-block counts {|V_R|, |V_R inside the relaxed
-// sphere|}, then the regularisers over the marked vertices in grid order (work-groups walk
-// (x, y) rows, lanes walk z), so no vertex list is built and the atomics of a wavefront
-// land on contiguous addresses.
+// sphere|}, then the regularisers over the marked vertices tile by tile (below), so no
+// vertex list is built.
 // ------------------------------------------------------------------------------------
 
 struct VrGeom {
@@ -354,94 +353,211 @@ __global__ void __launch_bounds__(256) vr_count_finish_kernel(const int32_t* __r
 
 constexpr int kGatherThreads = 256;
 
+// Constants of the two regularisers' gradients.
+struct RegK {
+  float scale, dx, dy, dz, ex2, ey2, ez2, w_eik, w_curv;
+};
+
+// The terms a V_R vertex u adds to its six neighbours and (curvature) to itself, with the
+// values its own losses need (eikonal.py:181-218: v = scale (1 - 1/(|n| + eps)) n, +v/d to
+// u + e and -v/d to u - e; curvature.py:351-384: the second differences).
+struct RegTerms {
+  float xp, xm, yp, ym, zp, zm, cz;
+  float nn, Lx, Ly, Lz;
+};
+
+VXN_DEV RegTerms reg_terms(float fc, float fxp, float fxm, float fyp, float fym, float fzp, float fzm, const RegK& k) {
+  RegTerms t;
+  const float nx = (fxp - fxm) / k.dx, ny = (fyp - fym) / k.dy, nz = (fzp - fzm) / k.dz;
+  t.nn = sqrtf(nx * nx + ny * ny + nz * nz);
+  t.Lx = ((fxp + fxm) - 2.0f * fc) / k.ex2;
+  t.Ly = ((fyp + fym) - 2.0f * fc) / k.ey2;
+  t.Lz = ((fzp + fzm) - 2.0f * fc) / k.ez2;
+  const float sv = k.scale * (1.0f - 1.0f / (t.nn + 1e-8f));
+  const float ex = k.w_eik != 0.0f ? ((sv * nx) / k.dx) * k.w_eik : 0.0f;
+  const float ey = k.w_eik != 0.0f ? ((sv * ny) / k.dy) * k.w_eik : 0.0f;
+  const float ez = k.w_eik != 0.0f ? ((sv * nz) / k.dz) * k.w_eik : 0.0f;
+  const float gx = k.w_curv != 0.0f ? (k.scale * (t.Lx / k.ex2)) * k.w_curv : 0.0f;
+  const float gy = k.w_curv != 0.0f ? (k.scale * (t.Ly / k.ey2)) * k.w_curv : 0.0f;
+  const float gz = k.w_curv != 0.0f ? (k.scale * (t.Lz / k.ez2)) * k.w_curv : 0.0f;
+  t.cz = k.w_curv != 0.0f ? (k.scale * ((-2.0f * t.Lx / k.ex2 - 2.0f * t.Ly / k.ey2) - 2.0f * t.Lz / k.ez2)) * k.w_curv
+                          : 0.0f;
+  t.xp = ex + gx;
+  t.xm = -ex + gx;
+  t.yp = ey + gy;
+  t.ym = -ey + gy;
+  t.zp = ez + gz;
+  t.zm = -ez + gz;
+  return t;
+}
+
+// Tiles of kRegTX x kRegTY (x, y) rows by 64 z: each member vertex of the tile computes its
+// terms once and adds them, with LDS float atomics, into the tile's accumulator, which
+// spans the tile plus one vertex on every side; the accumulator is then added to grad_grid
+// line by line (64 consecutive z per instruction, zero sums skipped), so a vertex gets one
+// device atomic per tile that reaches it instead of one per term.  Tiles without a marked
+// vertex are skipped after one load of their bitmask words.
+constexpr int kRegTX = 8, kRegTY = 8;
+constexpr int kRegHY = kRegTY + 2;
+constexpr int kRegRows = (kRegTX + 2) * kRegHY;
+constexpr int kRegLine = 66;  // z0 - 1 .. z0 + 64
+constexpr int kRegGroup = 4;  // rows whose loads are in flight together
+
 __global__ void __launch_bounds__(kGatherThreads) reg_vr_kernel(GridDesc g, const uint32_t* __restrict__ bits,
                                                                     VrGeom v, const int32_t* __restrict__ counts,
                                                                     float w_eik, float w_curv,
                                                                     float* __restrict__ gg,
                                                                     double* __restrict__ partial) {
+  __shared__ float acc[kRegRows * kRegLine];
   const int n_all = counts[0], n_in = counts[1];
   const bool filt = n_in > 0;
   const int nact = filt ? n_in : n_all;
-  const float scale = (float)(2.0 / (double)(nact > 0 ? nact : 1));
-  const float dx = 2.0f * g.eps[0], dy = 2.0f * g.eps[1], dz = 2.0f * g.eps[2];
-  const float ex2 = g.eps[0] * g.eps[0], ey2 = g.eps[1] * g.eps[1], ez2 = g.eps[2] * g.eps[2];
+  RegK k;
+  k.scale = (float)(2.0 / (double)(nact > 0 ? nact : 1));
+  k.dx = 2.0f * g.eps[0];
+  k.dy = 2.0f * g.eps[1];
+  k.dz = 2.0f * g.eps[2];
+  k.ex2 = g.eps[0] * g.eps[0];
+  k.ey2 = g.eps[1] * g.eps[1];
+  k.ez2 = g.eps[2] * g.eps[2];
+  k.w_eik = w_eik;
+  k.w_curv = w_curv;
   const int X = g.X, Y = g.Y, Z = g.Z;
   const int64_t YZ = (int64_t)Y * Z;
   const float* f = g.grid;
-  // u is a V_R vertex that the losses use: marked, interior, inside the sphere if filtering
-  auto member = [&](int64_t u, int ux, int uy, int uz) -> bool {
-    if (ux <= 0 || ux >= X - 1 || uy <= 0 || uy >= Y - 1 || uz <= 0 || uz >= Z - 1) return false;
-    if (!((bits[u >> 5] >> (u & 31)) & 1u)) return false;
-    return !filt || vr_inside(v, ux, uy, uz);
-  };
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  // a vertex is in V_R for the losses when marked, interior and (filtering) inside the sphere
   double se = 0.0, sc = 0.0, cnt = 0.0;
-  // work-groups walk the (x, y) rows, threads walk z: no per-vertex index division
-  for (int row = blockIdx.x; row < X * Y; row += gridDim.x) {
-   const int wx = row / Y, wy = row - (row / Y) * Y;
-   // skip rows without a marked vertex
-   bool marked = false;
-   {
-     const int64_t w0 = ((int64_t)row * Z) >> 5, w1 = ((int64_t)row * Z + Z - 1) >> 5;
-     for (int64_t ww = w0 + threadIdx.x; ww <= w1; ww += kGatherThreads) marked = marked || bits[ww] != 0u;
-   }
-   if (!__syncthreads_or(marked)) continue;
-   for (int z0 = threadIdx.x & ~63; z0 < Z; z0 += kGatherThreads) {
-    // one wavefront covers z0 .. z0 + 63 of the row
-    const int lane = threadIdx.x & 63, wz = z0 + lane;
-    const int64_t w = (int64_t)row * Z + wz;
-    const bool mem = wz < Z && member(w, wx, wy, wz);
-    float cz = 0.0f, zp = 0.0f, zm = 0.0f, xp = 0.0f, xm = 0.0f, yp = 0.0f, ym = 0.0f;
-    if (mem) {
-      const float fc = f[w];
-      const float fxp = f[w + YZ], fxm = f[w - YZ], fyp = f[w + Z], fym = f[w - Z], fzp = f[w + 1], fzm = f[w - 1];
-      // eikonal (eikonal.py:181-218): v = scale (1 - 1/(|n| + eps)) n; +v/d to w+e, -v/d to w-e
-      const float nx = (fxp - fxm) / dx, ny = (fyp - fym) / dy, nz = (fzp - fzm) / dz;
-      const float nn = sqrtf(nx * nx + ny * ny + nz * nz);
-      // curvature (curvature.py:351-384)
-      const float Lx = ((fxp + fxm) - 2.0f * fc) / ex2;
-      const float Ly = ((fyp + fym) - 2.0f * fc) / ey2;
-      const float Lz = ((fzp + fzm) - 2.0f * fc) / ez2;
-      const double de = (double)nn - 1.0;
-      se += de * de;
-      sc += (double)Lx * Lx + (double)Ly * Ly + (double)Lz * Lz;
-      cnt += 1.0;
-      const float sv = scale * (1.0f - 1.0f / (nn + 1e-8f));
-      const float ex = w_eik != 0.0f ? ((sv * nx) / dx) * w_eik : 0.0f;
-      const float ey = w_eik != 0.0f ? ((sv * ny) / dy) * w_eik : 0.0f;
-      const float ez = w_eik != 0.0f ? ((sv * nz) / dz) * w_eik : 0.0f;
-      const float gx = w_curv != 0.0f ? (scale * (Lx / ex2)) * w_curv : 0.0f;
-      const float gy = w_curv != 0.0f ? (scale * (Ly / ey2)) * w_curv : 0.0f;
-      const float gz = w_curv != 0.0f ? (scale * (Lz / ez2)) * w_curv : 0.0f;
-      cz = w_curv != 0.0f ? (scale * ((-2.0f * Lx / ex2 - 2.0f * Ly / ey2) - 2.0f * Lz / ez2)) * w_curv : 0.0f;
-      xp = ex + gx;
-      xm = -ex + gx;
-      yp = ey + gy;
-      ym = -ey + gy;
-      zp = ez + gz;
-      zm = -ez + gz;
+  const int ntx = (X + kRegTX - 1) / kRegTX, nty = (Y + kRegTY - 1) / kRegTY, ntz = (Z + 63) / 64;
+  const int64_t ntiles = (int64_t)ntx * nty * ntz;
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int tz = (int)(tile % ntz);
+    const int64_t rest = tile / ntz;
+    const int ty = (int)(rest % nty), tx = (int)(rest / nty);
+    const int x0 = tx * kRegTX, y0 = ty * kRegTY, z0 = tz * 64;
+    // skip a tile without a marked vertex: 64 bits of a row span at most 3 words
+    bool marked = false;
+    for (int q = threadIdx.x; q < kRegTX * kRegTY * 3; q += kGatherThreads) {
+      const int r = q / 3, j = q - (q / 3) * 3;
+      const int rx = x0 + r / kRegTY, ry = y0 + r % kRegTY;
+      if (rx < X && ry < Y) {
+        const int64_t rb = ((int64_t)rx * Y + ry) * Z;
+        const int64_t b0 = rb + z0, b1 = rb + min(z0 + 63, Z - 1);
+        const int64_t wd = (b0 >> 5) + j;
+        if (wd <= (b1 >> 5)) marked = marked || bits[wd] != 0u;
+      }
     }
-    // the z line: w's own term plus its neighbours' -z / +z terms, summed across lanes so
-    // one atomic per position covers all three (the wave's two end terms go on their own)
-    const bool mp = __shfl_up((int)mem, 1, 64) != 0, mn = __shfl_down((int)mem, 1, 64) != 0;
-    const float from_below = __shfl_up(zp, 1, 64), from_above = __shfl_down(zm, 1, 64);
-    const bool below = lane > 0 && mp, above = lane < 63 && mn;
-    const float line = (cz + (below ? from_below : 0.0f)) + (above ? from_above : 0.0f);
-    if (mem || below || above) atomicAdd(gg + w, line);
-    if (mem && lane == 63) atomicAdd(gg + w + 1, zp);
-    if (mem && lane == 0) atomicAdd(gg + w - 1, zm);
-    if (mem) {
-      atomicAdd(gg + w + YZ, xp);
-      atomicAdd(gg + w - YZ, xm);
-      atomicAdd(gg + w + Z, yp);
-      atomicAdd(gg + w - Z, ym);
+    if (!__syncthreads_or(marked)) continue;
+    for (int e = threadIdx.x; e < kRegRows * kRegLine; e += kGatherThreads) acc[e] = 0.0f;
+    __syncthreads();
+    const int wz = z0 + lane;
+    // Wavefront w owns the tile's x slices 2w and 2w + 1 (rows r = 16 w + j: x = 2w + j / 8,
+    // y = j % 8), kRegGroup rows at a time with every load of a group (membership words and
+    // stencils, the stencil centre clamped into the interior) issued before any is used.  A
+    // member's terms that stay in the wavefront's own slices (and their y halo) are added at
+    // once with plain LDS read-add-writes: no other wavefront touches those rows.  The one
+    // term that leaves (-x from slice 2w, +x from slice 2w + 1) waits in a register until the
+    // barrier, after which its target row has no other writer.
+    constexpr int kRows = kRegTX * kRegTY / (kGatherThreads / 64);
+    static_assert(kRows == 2 * kRegTY, "a wavefront owns two x slices");
+    float leave[kRows];
+    uint32_t lmask = 0u;
+#pragma unroll
+    for (int g0 = 0; g0 < kRows; g0 += kRegGroup) {
+      float sv[kRegGroup][7];
+      uint32_t word[kRegGroup];
+      int64_t wl[kRegGroup];
+#pragma unroll
+      for (int j = 0; j < kRegGroup; ++j) {
+        const int jj = g0 + j;
+        const int wx = x0 + 2 * wid + jj / kRegTY, wy = y0 + jj % kRegTY;
+        const bool ok = wx < X && wy < Y && wz < Z;
+        wl[j] = ok ? ((int64_t)wx * Y + wy) * Z + wz : -1;
+        word[j] = ok ? bits[wl[j] >> 5] : 0u;
+        const int cx = min(max(wx, 1), X - 2), cy = min(max(wy, 1), Y - 2), cz = min(max(wz, 1), Z - 2);
+        const int64_t u = ((int64_t)cx * Y + cy) * Z + cz;
+        sv[j][0] = f[u];
+        sv[j][1] = f[u + YZ];
+        sv[j][2] = f[u - YZ];
+        sv[j][3] = f[u + Z];
+        sv[j][4] = f[u - Z];
+        sv[j][5] = f[u + 1];
+        sv[j][6] = f[u - 1];
+      }
+#pragma unroll
+      for (int j = 0; j < kRegGroup; ++j) {
+        const int jj = g0 + j;
+        const bool right = jj >= kRegTY;
+        const int wx = x0 + 2 * wid + jj / kRegTY, wy = y0 + jj % kRegTY;
+        const bool mem = wl[j] >= 0 && ((word[j] >> (wl[j] & 31)) & 1u) && wx > 0 && wx < X - 1 && wy > 0 &&
+                         wy < Y - 1 && wz > 0 && wz < Z - 1 && (!filt || vr_inside(v, wx, wy, wz));
+        if (__ballot(mem) == 0) {
+          leave[jj] = 0.0f;
+          continue;
+        }
+        RegTerms t = reg_terms(sv[j][0], sv[j][1], sv[j][2], sv[j][3], sv[j][4], sv[j][5], sv[j][6], k);
+        if (mem) {
+          const double de = (double)t.nn - 1.0;
+          se += de * de;
+          sc += (double)t.Lx * t.Lx + (double)t.Ly * t.Ly + (double)t.Lz * t.Lz;
+          cnt += 1.0;
+          lmask |= 1u << jj;
+        } else {
+          t.cz = t.zp = t.zm = t.yp = t.ym = t.xp = t.xm = 0.0f;
+        }
+        // every lane adds (zero when not a member); the z terms land on the adjacent lanes'
+        // entries, so each add is ordered across the wavefront before the next
+        const int h = ((wx - x0 + 1) * kRegHY + (wy - y0 + 1)) * kRegLine + lane + 1;
+        acc[h] = acc[h] + t.cz;
+        wave_sync();
+        acc[h + 1] = acc[h + 1] + t.zp;
+        wave_sync();
+        acc[h - 1] = acc[h - 1] + t.zm;
+        wave_sync();
+        acc[h + kRegLine] = acc[h + kRegLine] + t.yp;
+        wave_sync();
+        acc[h - kRegLine] = acc[h - kRegLine] + t.ym;
+        wave_sync();
+        if (right) {
+          acc[h - kRegHY * kRegLine] = acc[h - kRegHY * kRegLine] + t.xm;
+          leave[jj] = t.xp;
+        } else {
+          acc[h + kRegHY * kRegLine] = acc[h + kRegHY * kRegLine] + t.xp;
+          leave[jj] = t.xm;
+        }
+        wave_sync();
+      }
     }
-   }
+    __syncthreads();
+#pragma unroll
+    for (int jj = 0; jj < kRows; ++jj) {
+      if (lmask & (1u << jj)) {
+        const bool right = jj >= kRegTY;
+        const int hx = 2 * wid + jj / kRegTY + 1 + (right ? 1 : -1), hy = jj % kRegTY + 1;
+        const int h = (hx * kRegHY + hy) * kRegLine + lane + 1;
+        acc[h] = acc[h] + leave[jj];
+      }
+    }
+    __syncthreads();
+    for (int r = wid; r < kRegRows; r += kGatherThreads / 64) {
+      const int gx = x0 + r / kRegHY - 1, gy = y0 + r % kRegHY - 1;
+      if (gx < 0 || gx >= X || gy < 0 || gy >= Y) continue;
+      const int64_t rb = ((int64_t)gx * Y + gy) * Z;
+      const float val = acc[r * kRegLine + lane + 1];
+      if (val != 0.0f && wz < Z) atomicAdd(gg + rb + wz, val);
+      if (lane < 2) {
+        const int hz = lane == 0 ? 0 : kRegLine - 1;
+        const int gz = z0 - 1 + hz;
+        const float v2 = acc[r * kRegLine + hz];
+        if (v2 != 0.0f && gz >= 0 && gz < Z) atomicAdd(gg + rb + gz, v2);
+      }
+    }
+    __syncthreads();
   }
   se = wave_sum_d(se);
   sc = wave_sum_d(sc);
   cnt = wave_sum_d(cnt);
   __shared__ double red[3][kGatherThreads / 64];
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   if (lane == 0) {
     red[0][wid] = se;
     red[1][wid] = sc;
@@ -450,10 +566,10 @@ __global__ void __launch_bounds__(kGatherThreads) reg_vr_kernel(GridDesc g, cons
   __syncthreads();
   if (threadIdx.x == 0) {
     double a = 0, b = 0, c = 0;
-    for (int k = 0; k < kGatherThreads / 64; ++k) {
-      a += red[0][k];
-      b += red[1][k];
-      c += red[2][k];
+    for (int q = 0; q < kGatherThreads / 64; ++q) {
+      a += red[0][q];
+      b += red[1][q];
+      c += red[2][q];
     }
     partial[3 * blockIdx.x] = a;
     partial[3 * blockIdx.x + 1] = b;

@@ -1100,12 +1100,6 @@ VXN_DEV void hash_level_records(const RenderDev& a, int64_t i, int lane, int l, 
     emit(tail, c, v, d0, d1);
 }
 
-VXN_DEV void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 // Passes 1 and 3 run one work-group per (level, 4096-sample tile), level-major: its four
 // wavefronts walk 1024 consecutive samples each, in 16 chunks of 64, and the tile owns one
 // row of counters (LDS, then cnt[level][tile]).  Pass 3 takes the chunks in lockstep: the

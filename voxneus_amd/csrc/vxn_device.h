@@ -68,6 +68,16 @@ VXN_DEV float sigmoid_ref(float a) {
   return 1.0f / (1.0f + e);
 }
 
+// Orders one wavefront's memory operations across its lanes: what a lane wrote before the
+// call is visible to every lane of the wavefront after it (a lane's LDS read-add-write may
+// target the entry another lane updated just before; the compiler, which sees one lane at a
+// time, could otherwise reorder the two).
+VXN_DEV void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // NeuS sharpness from the variance parameter, s = clamp(exp(10 v), 1e-4, s_max)
 // (models/variance.py:31), with the device expf that PyTorch's GPU exp also calls.
 VXN_DEV float s_of_variance(float v, float s_max) { return fminf(fmaxf(expf(v * 10.0f), 1e-4f), s_max); }

@@ -91,10 +91,11 @@ class StepConfig:
     gradient_mode: str = "interpolated"  # or "analytical" (Eq. 4), as VoxNeuS(gradient_mode=...)
     view_embed_type: str = "sh"  # or "pe" (positional encoding; the MLP's W0 is [64, 62])
     # mark V_R in the sampler and run the regularisers on a second stream, beside the
-    # hash-feature gather, the render and the scatters
+    # render and the hash-table gradient
     concurrent_regularisers: bool = True
-    # start them once the hash-feature gather is done (else right after the sampler)
-    fork_after_features: bool = False
+    # start them just before the render kernel (else right after the sampler, beside the
+    # hash-feature gather; measured slower: the render kernel loses work-group slots to them)
+    fork_after_features: bool = True
     # run the grid scatter on a third stream, beside the hash-table gradient (off: its
     # atomics slow the binned passes by more than the overlap saves, 2.50 -> 2.48 ms BF16
     # and 3.29 -> 3.45 ms FP32 on the 8192-ray step)
