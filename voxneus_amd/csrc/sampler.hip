@@ -37,6 +37,28 @@ struct SamplerDev {
   int n_scenes;
 };
 
+// One sequential double-precision scan over the n <= 64 values v (lane j holds element j),
+// continuing from acc: the exclusive product (PROD, written before each multiply) or the
+// inclusive sum, rounded to float into lane j of out.  The order of the operations is the
+// CPU's (torch.cumprod / torch.cumsum accumulate a float tensor in double, element by
+// element).  Lane indices are constants after unrolling, so an element costs a lane read,
+// two conversions, the double operation and one select.
+template <bool PROD>
+VXN_DEV void seq_scan(float v, int n, int t, double& acc, float& out) {
+#pragma unroll
+  for (int j = 0; j < 64; ++j) {
+    if (j >= n) break;
+    const double x = (double)__int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
+    if constexpr (PROD) {
+      out = t == j ? (float)acc : out;
+      acc = acc * x;
+    } else {
+      acc = acc + x;
+      out = t == j ? (float)acc : out;
+    }
+  }
+}
+
 template <int MULTI>
 __global__ void __launch_bounds__(kSampThreads, 8) upsample_kernel(SamplerDev a) {
   // One wavefront per ray: lane t holds samples t and t + 64 (kMaxS = 128).
@@ -147,14 +169,8 @@ __global__ void __launch_bounds__(kSampThreads, 8) upsample_kernel(SamplerDev a)
         const float w1 = t + 64 < m ? (1.0f - alpha[1]) + 1e-7f : 0.0f;
         double T = 1.0;
         float c0 = 0.0f, c1 = 0.0f;
-        for (int j = 0; j < m; ++j) {
-          const float wj = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(j < 64 ? w0 : w1), j & 63));
-          if (t == (j & 63)) {
-            if (j < 64) c0 = (float)T;
-            else c1 = (float)T;
-          }
-          T *= (double)wj;
-        }
+        seq_scan<true>(w0, m, t, T, c0);
+        seq_scan<true>(w1, m - 64, t, T, c1);
         const float cc[2] = {c0, c1};
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
@@ -202,13 +218,8 @@ __global__ void __launch_bounds__(kSampThreads, 8) upsample_kernel(SamplerDev a)
         const float q0 = t < m ? wv[t] / tot : 0.0f, q1 = t + 64 < m ? wv[t + 64] / tot : 0.0f;
         double c = 0.0;
         float r0 = 0.0f, r1 = 0.0f;
-        for (int j = 0; j < m; ++j) {
-          c += (double)__int_as_float(__builtin_amdgcn_readlane(__float_as_int(j < 64 ? q0 : q1), j & 63));
-          if (t == (j & 63)) {
-            if (j < 64) r0 = (float)c;
-            else r1 = (float)c;
-          }
-        }
+        seq_scan<false>(q0, m, t, c, r0);
+        seq_scan<false>(q1, m - 64, t, c, r1);
         if (t == 0) cdf[0] = 0.0f;
         if (t < m) cdf[t + 1] = r0;
         if (t + 64 < m) cdf[t + 65] = r1;
