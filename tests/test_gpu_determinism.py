@@ -3,12 +3,13 @@ x 128 samples, both forms): the same inputs twice in a row.
 
 Bit-identical on every run: z values, V_R counts, the colour and regulariser losses, the
 s-gradient, the MLP gradient (per-work-group partials summed in a fixed order) and the
-hash-table gradient (each record's position in its bin comes from the wavefront's own
-count row and its own LDS counters, and each bin's entries are summed in record order).
+hash-table gradient (each record's position in its bin follows its tile, chunk, wavefront
+and lane through count rows, scans and LDS counters that one wavefront owns, and each
+bin's entries are summed in record order, then over the wavefronts' copies in order).
 
 Order-dependent: the SDF grid gradient.  The grid scatter flushes each run's Eq. 7 brick
-with float atomics, and the eikonal + curvature pass adds its seven terms per V_R vertex
-with float atomics, as the reference's CUDA kernels do (gradient_interp.cu:141-148,
+with float atomics, and the eikonal + curvature pass adds each tile's accumulated terms
+with float atomics, as the reference's CUDA kernels add theirs (gradient_interp.cu:141-148,
 eikonal_grad.cu:61-68); two runs differ in the last bits of some voxels, stated here and
 bounded by 1e-6 of the largest magnitude.
 """

@@ -214,8 +214,8 @@ def test_render_backward_matches_oracle_autograd(Fn):
 @pytest.mark.parametrize("log2_T", [19, 21, 22])
 def test_render_backward_hash_gradient_large_tables(Fn, log2_T):
     """The hash-table gradient with 2^19 entries per level (binned: 256 bins of 2048
-    entries per level), 2^21 (1024 bins per level, the binned path's limit: 4 KB of LDS
-    counters per wavefront) and 2^22 (past it: the atomic scatter)."""
+    entries per level), 2^21 (1024 bins per level, the binned path's limit: 68 KB of LDS
+    in the write pass) and 2^22 (past it: the atomic scatter)."""
     g = load_golden("step_c1.npz")
     sc = scenes.config1_scene(seed=0)
     emb = (torch.rand((16, 1 << log2_T, 2), generator=torch.Generator().manual_seed(log2_T)) - 0.5) * 0.2
@@ -294,7 +294,7 @@ def test_vr_compaction_and_regularisers_match_oracle(Fn):
 
 @pytest.mark.parametrize("relax", [1.2, 0.3])
 def test_bitmask_regularisers_equal_list_form(Fn, relax):
-    """The atomic-free gather over the V_R bitmask gives the list form's counts, losses and
+    """The tiled pass over the V_R bitmask gives the list form's counts, losses and
     gradient (FP32 sums in another order); relax 0.3 leaves part of V_R outside."""
     rng = np.random.default_rng(6)
     res = 40

@@ -2,7 +2,7 @@
 device array instead of a workspace) against vxn_reg_eik_curv_vr, through the C ABI, on
 two calls in a row, the second on a smaller grid: V_R counts and the FP64 losses
 bit-identical (fixed-order reductions), the gradient within 1e-6 of the largest magnitude
-(float atomics add in another order)."""
+(the tiles' float atomics add in another order)."""
 
 import ctypes
 
