@@ -214,3 +214,65 @@ def test_sdf_optimizer_after_a_milestone_takes_the_default_betas():
         opt.step(it)
         R.adam_native_step(host, g, m, v, it, 0.9, 0.999, 0.05, 1e-8, True)
         assert torch.equal(st.grid.cpu(), host), it
+
+
+@pytest.mark.parametrize("form", ["fp32", "bf16"])
+def test_fused_optimizer_resumes_from_a_checkpoint(form, tmp_path):
+    """Three steps, then FusedOptimizer.state_dict() through torch.save / torch.load
+    (weights_only=True) into a fresh step and optimizer holding copies of the parameters:
+    after the next step the hash table, MLP, variance and their moments equal the
+    uninterrupted run's bit for bit (the bf16 form's FP32 master included), the grid and
+    its moments to float rounding.  The saved layout is the reference's
+    (checkpoint.py:30-31): SparseGridAdam's and torch.optim.Adam's state dicts, which a
+    torch.optim.Adam over create_optimizer's parameter groups (optimizer.py:142-147)
+    accepts."""
+    from oracle.ref_cpu import hash_level_resolutions
+    from voxneus_amd import scenes
+    from voxneus_amd.training import FusedOptimizer, FusedTrainStep, StepConfig
+
+    sc = scenes.config1_scene(seed=0)
+    mlp = torch.cat([torch.from_numpy(sc.mlp[k]).reshape(-1) for k in ("W0", "b0", "W1", "b1")]).to(DEV)
+    table = torch.from_numpy(sc.embeddings).to(DEV)
+    mode = N.VXN_MLP_FP32
+    if form == "bf16":
+        table, mode = table.to(torch.bfloat16), N.VXN_MLP_BF16_MFMA
+    res = hash_level_resolutions(16, 16, sc.hash_max_resolution)
+
+    def make(grid, tab, w, var):
+        st = FusedTrainStep(grid, tab, w, var, res, cfg=StepConfig(mlp_mode=mode))
+        return st, FusedOptimizer(st, total_steps=100)
+
+    st, opt = make(torch.from_numpy(sc.grid).to(DEV), table, mlp, torch.tensor(0.3, device=DEV))
+    o, d, tgt = (torch.from_numpy(x).to(DEV) for x in scenes.config_rays("sphere", 256, seed=3))
+    for it in range(3):
+        st.step(o, d, tgt)
+        opt.step(it)
+    torch.save(opt.state_dict(), tmp_path / "opt.pt")
+    mlp_m_saved = opt.mlp_m.clone()
+    st2, opt2 = make(st.grid.clone(), st.table.clone(), st.mlp.clone(), st.variance.clone())
+    opt2.load_state_dict(torch.load(tmp_path / "opt.pt", weights_only=True))
+    for s_, o_ in ((st, opt), (st2, opt2)):
+        s_.step(o, d, tgt)
+        o_.step(3)
+    torch.cuda.synchronize()
+    for name in ("table", "mlp", "variance"):
+        assert torch.equal(getattr(st, name), getattr(st2, name)), name
+    for name in ("table_m", "table_v", "mlp_m", "mlp_v", "var_m", "var_v"):
+        assert torch.equal(getattr(opt, name), getattr(opt2, name)), name
+    # the grid gradient's device atomics add in another order on every run (the step's
+    # run-to-run test states it), so the grid and its moments agree to float rounding
+    for a_, b_ in ((st.grid, st2.grid), (opt.grid_m, opt2.grid_m), (opt.grid_v, opt2.grid_v)):
+        assert float((a_ - b_).abs().max()) <= 1e-5 * float(a_.abs().max())
+    if form == "bf16":
+        assert torch.equal(opt.table_master, opt2.table_master)
+    # the reference's torch.optim.Adam takes the saved network state as it is
+    sd = torch.load(tmp_path / "opt.pt", weights_only=True)["net_optimizer"]
+    params = [torch.zeros(tuple(st.table.shape)), torch.zeros(64, 51), torch.zeros(64), torch.zeros(3, 64),
+              torch.zeros(3), torch.zeros(())]
+    ref = torch.optim.Adam([{"params": params[:1], "lr": 1e-2}, {"params": params[1:5], "lr": 1e-3},
+                            {"params": params[5:], "lr": 5e-4}], betas=(0.9, 0.999), eps=1e-8)
+    ref.load_state_dict({"state": {k: {kk: vv.cpu() for kk, vv in v.items()} for k, v in sd["state"].items()},
+                         "param_groups": sd["param_groups"]})
+    assert float(ref.state[params[1]]["step"]) == 3.0
+    assert torch.equal(ref.state[params[1]]["exp_avg"], mlp_m_saved[:64 * 51].reshape(64, 51).cpu())
+    assert tuple(ref.state[params[0]]["exp_avg_sq"].shape) == tuple(st.table.shape)

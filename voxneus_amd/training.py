@@ -410,6 +410,80 @@ class FusedOptimizer:
         self.sdf_step = 0
         self.sdf_beta2 = 0.999
 
+    # ---- checkpoint state in the reference's layout (checkpoint.py:30-31, 57-62) --------
+    def _mlp_split(self, flat: torch.Tensor):
+        """The flat MLP vector as the four tensors of radiance_mlp.net (W0, b0, W1, b1)."""
+        n_in = (flat.numel() - 64 - 3 * 64 - 3) // 64
+        sizes = (64 * n_in, 64, 3 * 64, 3)
+        shapes = ((64, n_in), (64,), (3, 64), (3,))
+        return [t.reshape(sh) for t, sh in zip(torch.split(flat, sizes), shapes)]
+
+    def state_dict(self) -> Dict[str, Dict]:
+        """Both optimizers' states as the reference saves them: ``sdf_optimizer`` is
+        SparseGridAdam's ``state_dict()`` (one parameter, the grid: step, exp_avg,
+        exp_avg_sq) and ``net_optimizer`` torch.optim.Adam's over the parameter groups of
+        create_optimizer (optimizer.py:119-154): the hash table, the MLP's four tensors,
+        the variance.  A bf16 table's FP32 master rides along as ``net_optimizer``'s
+        ``fp32_master`` (torch.optim.Adam ignores it).  Tensors are copies on the device."""
+        sdf = {"state": {0: {"step": self.sdf_step, "exp_avg": self.grid_m.clone(),
+                             "exp_avg_sq": self.grid_v.clone()}},
+               "param_groups": [{"lr": self.lr_sdf, "betas": (0.9, self.sdf_beta2), "eps": 1e-8,
+                                 "weight_decay": 0.0, "params": [0]}]}
+        step = lambda: torch.tensor(float(self.net_step))
+        state = {0: {"step": step(), "exp_avg": self.table_m.clone(), "exp_avg_sq": self.table_v.clone()}}
+        for i, (m, v) in enumerate(zip(self._mlp_split(self.mlp_m), self._mlp_split(self.mlp_v))):
+            state[1 + i] = {"step": step(), "exp_avg": m.clone(), "exp_avg_sq": v.clone()}
+        state[5] = {"step": step(), "exp_avg": self.var_m.clone(), "exp_avg_sq": self.var_v.clone()}
+        common = {"betas": (0.9, 0.999), "eps": 1e-8, "weight_decay": 0.0, "amsgrad": False, "maximize": False,
+                  "foreach": None, "capturable": False, "differentiable": False, "fused": None}
+        groups = [dict(common, lr=self.lr_hash, params=[0]), dict(common, lr=self.lr_mlp, params=[1, 2, 3, 4]),
+                  dict(common, lr=self.lr_variance, params=[5])]
+        net = {"state": state, "param_groups": groups}
+        if self.table_master is not None:
+            net["fp32_master"] = self.table_master.clone()
+        return {"sdf_optimizer": sdf, "net_optimizer": net}
+
+    def load_state_dict(self, sd: Dict[str, Dict]) -> None:
+        """Restore ``state_dict()``'s output (either key may be missing); the learning
+        rates and the grid's betas come from the saved groups, the tensors are copied onto
+        the step's device.  Without ``fp32_master`` a bf16 table's master is the table."""
+        st = self.st
+        dev = st.grid.device
+        f32 = lambda t: t.to(device=dev, dtype=torch.float32).contiguous()
+        if "sdf_optimizer" in sd:
+            s0 = sd["sdf_optimizer"]["state"][0]
+            g0 = sd["sdf_optimizer"]["param_groups"][0]
+            if tuple(s0["exp_avg"].shape) != tuple(st.grid.shape):
+                raise ValueError(f"saved SDF moments are {tuple(s0['exp_avg'].shape)}, the grid {tuple(st.grid.shape)}")
+            self.grid_m, self.grid_v = f32(s0["exp_avg"]), f32(s0["exp_avg_sq"])
+            self.sdf_step = int(s0["step"])
+            self.lr_sdf = float(g0["lr"])
+            self.sdf_beta2 = float(g0["betas"][1])
+        if "net_optimizer" in sd:
+            net = sd["net_optimizer"]
+            s = net["state"]
+            if tuple(s[0]["exp_avg"].shape) != tuple(st.table.shape):
+                raise ValueError(f"saved hash moments are {tuple(s[0]['exp_avg'].shape)}, the table "
+                                 f"{tuple(st.table.shape)}")
+            self.table_m, self.table_v = f32(s[0]["exp_avg"]), f32(s[0]["exp_avg_sq"])
+            self.mlp_m = torch.cat([f32(s[i]["exp_avg"]).reshape(-1) for i in range(1, 5)])
+            self.mlp_v = torch.cat([f32(s[i]["exp_avg_sq"]).reshape(-1) for i in range(1, 5)])
+            if self.mlp_m.numel() != st.mlp.numel():
+                raise ValueError(f"saved MLP moments hold {self.mlp_m.numel()} values, the MLP {st.mlp.numel()}")
+            self.var_m = f32(s[5]["exp_avg"]).reshape(st.variance.shape)
+            self.var_v = f32(s[5]["exp_avg_sq"]).reshape(st.variance.shape)
+            self.net_step = int(float(s[0]["step"]))
+            groups = net["param_groups"]
+            self.lr_hash, self.lr_mlp, self.lr_variance = (float(g["lr"]) for g in groups[:3])
+            if self.table_master is not None:
+                self.table_master = (f32(net["fp32_master"]) if "fp32_master" in net
+                                     else st.table.float().contiguous())
+
+    def as_optimizers(self) -> Dict[str, "_StateView"]:
+        """``{"sdf_optimizer": ..., "net_optimizer": ...}`` objects with ``state_dict()`` and
+        ``load_state_dict()``, for ``save_checkpoint`` / ``load_checkpoint``'s ``optimizers``."""
+        return {"sdf_optimizer": _StateView(self, "sdf_optimizer"), "net_optimizer": _StateView(self, "net_optimizer")}
+
     def step(self, it: int, variance_floor: Optional[float] = None):
         """Apply both optimizers after ``FusedTrainStep.step`` of training iteration ``it``."""
         import math
@@ -444,6 +518,19 @@ class FusedOptimizer:
         N.check(L.vxn_adam_step(ctypes.cast(ts, ctypes.c_void_p), 3, self.net_step, 0.9, 0.999, 1e-8, s),
                 "adam_step")
 
+
+
+class _StateView:
+    """One of FusedOptimizer's two optimizers, as save_checkpoint / load_checkpoint see it."""
+
+    def __init__(self, opt: FusedOptimizer, name: str):
+        self.opt, self.name = opt, name
+
+    def state_dict(self) -> Dict:
+        return self.opt.state_dict()[self.name]
+
+    def load_state_dict(self, sd: Dict) -> None:
+        self.opt.load_state_dict({self.name: sd})
 
 
 @dataclass
