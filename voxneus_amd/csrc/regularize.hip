@@ -353,9 +353,11 @@ __global__ void __launch_bounds__(256) vr_count_finish_kernel(const int32_t* __r
 
 constexpr int kGatherThreads = 256;
 
-// Constants of the two regularisers' gradients.
+// Constants of the two regularisers' gradients, with the reciprocals of the six divisors
+// (the divisions by them are correctly rounded through div_rn).
 struct RegK {
   float scale, dx, dy, dz, ex2, ey2, ez2, w_eik, w_curv;
+  float rdx, rdy, rdz, rex2, rey2, rez2;
 };
 
 // The terms a V_R vertex u adds to its six neighbours and (curvature) to itself, with the
@@ -368,20 +370,22 @@ struct RegTerms {
 
 VXN_DEV RegTerms reg_terms(float fc, float fxp, float fxm, float fyp, float fym, float fzp, float fzm, const RegK& k) {
   RegTerms t;
-  const float nx = (fxp - fxm) / k.dx, ny = (fyp - fym) / k.dy, nz = (fzp - fzm) / k.dz;
+  const float nx = div_rn(fxp - fxm, k.dx, k.rdx), ny = div_rn(fyp - fym, k.dy, k.rdy),
+              nz = div_rn(fzp - fzm, k.dz, k.rdz);
   t.nn = sqrtf(nx * nx + ny * ny + nz * nz);
-  t.Lx = ((fxp + fxm) - 2.0f * fc) / k.ex2;
-  t.Ly = ((fyp + fym) - 2.0f * fc) / k.ey2;
-  t.Lz = ((fzp + fzm) - 2.0f * fc) / k.ez2;
+  t.Lx = div_rn((fxp + fxm) - 2.0f * fc, k.ex2, k.rex2);
+  t.Ly = div_rn((fyp + fym) - 2.0f * fc, k.ey2, k.rey2);
+  t.Lz = div_rn((fzp + fzm) - 2.0f * fc, k.ez2, k.rez2);
   const float sv = k.scale * (1.0f - 1.0f / (t.nn + 1e-8f));
-  const float ex = k.w_eik != 0.0f ? ((sv * nx) / k.dx) * k.w_eik : 0.0f;
-  const float ey = k.w_eik != 0.0f ? ((sv * ny) / k.dy) * k.w_eik : 0.0f;
-  const float ez = k.w_eik != 0.0f ? ((sv * nz) / k.dz) * k.w_eik : 0.0f;
-  const float gx = k.w_curv != 0.0f ? (k.scale * (t.Lx / k.ex2)) * k.w_curv : 0.0f;
-  const float gy = k.w_curv != 0.0f ? (k.scale * (t.Ly / k.ey2)) * k.w_curv : 0.0f;
-  const float gz = k.w_curv != 0.0f ? (k.scale * (t.Lz / k.ez2)) * k.w_curv : 0.0f;
-  t.cz = k.w_curv != 0.0f ? (k.scale * ((-2.0f * t.Lx / k.ex2 - 2.0f * t.Ly / k.ey2) - 2.0f * t.Lz / k.ez2)) * k.w_curv
-                          : 0.0f;
+  const float ex = k.w_eik != 0.0f ? div_rn(sv * nx, k.dx, k.rdx) * k.w_eik : 0.0f;
+  const float ey = k.w_eik != 0.0f ? div_rn(sv * ny, k.dy, k.rdy) * k.w_eik : 0.0f;
+  const float ez = k.w_eik != 0.0f ? div_rn(sv * nz, k.dz, k.rdz) * k.w_eik : 0.0f;
+  const float qx = div_rn(t.Lx, k.ex2, k.rex2), qy = div_rn(t.Ly, k.ey2, k.rey2), qz = div_rn(t.Lz, k.ez2, k.rez2);
+  const float gx = k.w_curv != 0.0f ? (k.scale * qx) * k.w_curv : 0.0f;
+  const float gy = k.w_curv != 0.0f ? (k.scale * qy) * k.w_curv : 0.0f;
+  const float gz = k.w_curv != 0.0f ? (k.scale * qz) * k.w_curv : 0.0f;
+  // -2 Lx / ex2 is RN(RN(-2 Lx) / ex2) = -2 RN(Lx / ex2): doubling is exact
+  t.cz = k.w_curv != 0.0f ? (k.scale * ((-2.0f * qx - 2.0f * qy) - 2.0f * qz)) * k.w_curv : 0.0f;
   t.xp = ex + gx;
   t.xm = -ex + gx;
   t.yp = ey + gy;
@@ -422,6 +426,12 @@ __global__ void __launch_bounds__(kGatherThreads) reg_vr_kernel(GridDesc g, cons
   k.ez2 = g.eps[2] * g.eps[2];
   k.w_eik = w_eik;
   k.w_curv = w_curv;
+  k.rdx = 1.0f / k.dx;
+  k.rdy = 1.0f / k.dy;
+  k.rdz = 1.0f / k.dz;
+  k.rex2 = 1.0f / k.ex2;
+  k.rey2 = 1.0f / k.ey2;
+  k.rez2 = 1.0f / k.ez2;
   const int X = g.X, Y = g.Y, Z = g.Z;
   const int64_t YZ = (int64_t)Y * Z;
   const float* f = g.grid;
