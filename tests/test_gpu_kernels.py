@@ -237,6 +237,40 @@ def test_render_backward_hash_gradient_large_tables(Fn, log2_T):
     assert rel_err(got, ref) < 1e-5
 
 
+def test_render_backward_hash_gradient_split_pairs(Fn):
+    """Every sample at x = 0.75384 (x01 = 0.87692): at levels 14 and 15 (res 4670, 7006)
+    its cells sit at x = 4095 and 6143, whose corner pairs straddle two 2048-entry bins, so
+    each sample gives 8 records per level and a chunk of 256 samples 2048, past the write
+    pass's 1024 staged slots (the rest are stored directly).  The hash-table gradient still
+    equals the oracle's autograd."""
+    sc = scenes.config1_scene(seed=0)
+    m = R.RefModel(sc.grid, sc.embeddings, **sc.mlp, hash_max_resolution=sc.hash_max_resolution)
+    assert [int(m.level_res[l] * 0.87692) for l in (14, 15)] == [4095, 6143]
+    n = 64
+    zr = torch.linspace(-0.3, 0.3, n)
+    o = torch.stack([torch.full((n,), 0.75384), torch.full((n,), -0.6), zr], 1).float()
+    d = torch.tensor([[0.0, 1.0, 0.0]]).expand(n, 3).contiguous()
+    z = torch.linspace(0.05, 1.15, 129)[None, :].expand(n, 129).contiguous()
+    mid = (0.5 * (z[:, 1:] + z[:, :-1])).contiguous()
+    dists = (z[:, 1:] - z[:, :-1]).contiguous()
+    rr = torch.from_numpy(np.random.default_rng(8).normal(size=(n, 3)).astype(np.float32))
+    out = m.render(o, d, None, None, zs=(z[:, :128].contiguous(), dists, mid))
+    (out["rgb"] * rr).sum().backward()
+    ref = m.embeddings.grad.numpy()
+    assert np.count_nonzero(ref[14]) > 0 and np.count_nonzero(ref[15]) > 0
+    cfg = Fn.RenderConfig(BOX6, m.level_res)
+    ed = torch.from_numpy(sc.embeddings).to(DEV).requires_grad_(True)
+    s = torch.clamp(torch.exp(torch.tensor(0.3, device=DEV) * 10.0), min=1e-4, max=300.0)
+    outs = Fn.RenderFunction.apply(torch.from_numpy(sc.grid).to(DEV), ed, _mlp_flat(sc).to(DEV), s, o.to(DEV),
+                                   d.to(DEV), mid.to(DEV), dists.to(DEV), cfg, False, None)
+    (outs[0] * rr.to(DEV)).sum().backward()
+    got = ed.grad.cpu().numpy()
+    assert np.array_equal(got != 0, ref != 0)
+    assert rel_err(got, ref) < 1e-5
+    for l in (14, 15):
+        assert rel_err(got[l], ref[l]) < 1e-5, l
+
+
 @pytest.mark.parametrize("name", ["step_c1.npz", "step_c2.npz"])
 def test_fused_train_step_matches_reference(name):
     from voxneus_amd.training import FusedTrainStep, StepConfig
