@@ -294,17 +294,26 @@ __global__ void __launch_bounds__(kCompactThreads) vr_count2_kernel(const uint32
                                                                     int32_t* __restrict__ part) {
   __shared__ int32_t red[2][kCompactThreads / 64];
   const int64_t w = blockIdx.x * (int64_t)kCompactThreads + threadIdx.x;
-  uint32_t word = w < nwords ? bits[w] : 0u;
+  const uint32_t word = w < nwords ? bits[w] : 0u;
   int c = __popc(word), ci = 0;
-  const int64_t YZ = (int64_t)Y * Z;
-  while (word) {
-    const int b = __ffs(word) - 1;
-    word &= word - 1;
-    const int64_t lin = w * 32 + b;
-    const int ix = (int)(lin / YZ);
-    const int64_t r = lin - (int64_t)ix * YZ;
-    const int iy = (int)(r / Z);
-    ci += vr_inside(v, ix, iy, (int)(r - (int64_t)iy * Z)) ? 1 : 0;
+  if (word) {
+    // the word's first vertex in 32-bit arithmetic (X*Y*Z < 2^31), then its 32 vertices
+    // in order, z fastest; every bit is tested, so the loop has no divergence of its own
+    const uint32_t lin0 = (uint32_t)w * 32u, YZ = (uint32_t)Y * (uint32_t)Z;
+    int ix = (int)(lin0 / YZ);
+    const uint32_t r = lin0 - (uint32_t)ix * YZ;
+    int iy = (int)(r / (uint32_t)Z), iz = (int)(r - (uint32_t)iy * (uint32_t)Z);
+#pragma unroll 8
+    for (int b = 0; b < 32; ++b) {
+      ci += ((word >> b) & 1u) && vr_inside(v, ix, iy, iz) ? 1 : 0;
+      if (++iz == Z) {
+        iz = 0;
+        if (++iy == Y) {
+          iy = 0;
+          ++ix;
+        }
+      }
+    }
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
