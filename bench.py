@@ -49,22 +49,24 @@ LATTICE = 320
 #   hash_gradient  the hash-table gradient stage (binned: count, scan, record write, per-bin
 #                  sums; 6 launches between the same two events), accounted as the 256 FP32
 #                  read-modify-writes it replaces = 2048 B;
-#   grid_scatter   32 words read-modify-write = 256 B.
+#   grid_gradient  the grid gradient (binned: count, scan, brick write, per-tile sums; or the
+#                  atomic scatter), accounted as the 32 words read-modify-write it replaces
+#                  = 256 B.
 KERNEL_BYTES = {
     # FP32 form: the render kernel gathers the hash features itself (no hash_features launch)
-    "fp32": {"render_kernel": 128 + 1024 + 9, "hash_gradient": 2048, "grid_scatter": 256},
-    "bf16": {"hash_features": 512 + 64, "render_kernel": 128 + 64 + 9, "hash_gradient": 2048, "grid_scatter": 256},
+    "fp32": {"render_kernel": 128 + 1024 + 9, "hash_gradient": 2048, "grid_gradient": 256},
+    "bf16": {"hash_features": 512 + 64, "render_kernel": 128 + 64 + 9, "hash_gradient": 2048, "grid_gradient": 256},
 }
 # the sampler: 112 SDF queries x 8 corners x 4 B per 128-sample ray (no cache credit)
 SIDE_BYTES_PER_SAMPLE = {"upsample": 28}
 # names of the kernels in the committed rocprofv3 traffic summaries
 PROFILE_NAMES = {"upsample": "upsample_kernel", "reg_vr": "reg_vr_kernel"}
 # event index pairs bracketing each kernel in vxn_render_args.kernel_events
-KERNEL_EVENTS = {"hash_features": (0, 1), "render_kernel": (1, 2), "hash_gradient": (2, 3), "grid_scatter": (3, 4),
+KERNEL_EVENTS = {"hash_features": (0, 1), "render_kernel": (1, 2), "grid_gradient": (2, 3), "hash_gradient": (3, 4),
                  "reductions": (4, 5)}
-# with the grid scatter on its own stream (StepConfig.concurrent_scatter) it starts with the
-# hash-table gradient, and the reductions' span includes the wait for it
-KERNEL_EVENTS_CONCURRENT = dict(KERNEL_EVENTS, grid_scatter=(2, 4), reductions=(3, 5))
+# with the atomic grid scatter on its own stream (StepConfig.concurrent_scatter) the
+# hash-table gradient starts with it, and the reductions' span includes the wait for it
+KERNEL_EVENTS_CONCURRENT = dict(KERNEL_EVENTS, hash_gradient=(2, 4))
 # lattice query: 32-word stencil (128 B) in, SDF + normal (16 B) out per lattice point
 LATTICE_BYTES_PER_POINT = 128 + 16
 

@@ -253,14 +253,16 @@ typedef struct vxn_render_args {
   void* workspace;
   int64_t workspace_bytes;
   /* nullable profiling hook: hipEvent_t[6] recorded on the stream before the hash-feature
-     gather and after it, the render kernel, the hash-table gradient (its binned passes or
-     the atomic scatter), the grid scatter and the reductions (forward: only the first
-     three) */
+     gather and after it, the render kernel, the grid gradient (the atomic scatter or the
+     binned passes), the hash-table gradient (its binned passes or the atomic scatter) and
+     the reductions (forward: only the first three) */
   void* const* kernel_events;
   int32_t gradient_mode; /* VXN_GRADIENT_*: the normal of every sample (VoxNeuS gradient_mode) */
   /* nullable hipEvent_t, recorded on `stream` after the hash-feature gather, just before
-     the render kernel: a caller can start independent work on another stream from it
-     (the fused training step starts its regularisers there) */
+     the render kernel (with grid_binned: after the binned grid gradient, whose plain
+     read-add-writes of grad_grid must not overlap other writers): a caller can start
+     independent work on another stream from it, such as atomic adds into grad_grid (the
+     fused training step starts its regularisers there) */
   void* fork_event;
   /* nullable device float: the NeuS sharpness s read on the device in place of `s`, so a
      caller that keeps s on the device (s = clamp(exp(10 v)), models/variance.py:31) needs
@@ -283,8 +285,8 @@ typedef struct vxn_render_args {
      grid scatter, which then runs beside the hash-table gradient.  scatter_events
      (hipEvent_t[2], required with it) order it after the render kernel and make `stream`
      wait for it before the call returns, so every output is complete on `stream` as
-     without it.  kernel_events[4] is then recorded on scatter_stream after the grid
-     scatter (its span is kernel_events[2] -> [4]). */
+     without it.  kernel_events[3] is then recorded on scatter_stream after the grid
+     scatter, and the hash-table gradient's span is kernel_events[2] -> [4]. */
   void* scatter_stream;
   void* const* scatter_events;
   /* nullable device float: the variance v of SingleVarianceNetwork (models/variance.py:
@@ -303,14 +305,24 @@ typedef struct vxn_render_args {
   /* backward and training entries: 1 = grad_table is overwritten (every entry, zero where
      no sample reaches it) instead of accumulated, so the caller need not clear it */
   int32_t grad_table_overwrite;
+  /* backward and training entries: 1 = the grid gradient is voxel-binned instead of the
+     atomic scatter (gradient_interp.cu:86-152): each run's brick of stencil weights is
+     listed under the 32 x 32 x 16-vertex grid tiles it reaches, and each tile's sums are
+     added into grad_grid with one plain read-add-write per vertex, so the grid gradient of
+     the call is the same on every run; stencils at the grid border stay atomic.  Grids up
+     to 1024 vertices per axis and 4096 tiles, without scatter_stream; otherwise (and with
+     0) the atomic scatter.  grad_grid must have no other writer during the call. */
+  int32_t grid_binned;
 } vxn_render_args;
 
 /* Workspace of the backward and training entries (the forward needs none): per-sample
  * records of the hash-feature and geometry gradients, and the binned hash-table gradient
  * (4 records of 20 bytes per sample and level, 8 on levels with res >= 2047, plus the
  * per-bin offsets): about 1.4 GB for 2^20 samples over 16 levels.  Tables beyond 2^21
- * entries per level (16 levels) take the atomic scatter and no record space.  The size
- * depends on n_rays, n_samples, n_levels, log2_T and level_res.                        */
+ * entries per level (16 levels) take the atomic scatter and no record space.  With
+ * grid_binned, 200 more bytes per sample (a brick and up to 8 list entries per run) and
+ * the tile count rows.  The size depends on n_rays, n_samples, n_levels, log2_T,
+ * level_res, the grid size, grid_binned and scatter_stream.                             */
 int vxn_render_workspace_bytes(const vxn_render_args* a, int64_t* bytes);
 int vxn_render_forward(const vxn_render_args* a, void* stream);
 int vxn_render_backward(const vxn_render_args* a, void* stream);

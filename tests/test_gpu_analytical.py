@@ -138,10 +138,12 @@ def test_render_analytical_matches_oracle(Fn):
     assert abs(v.grad.item() - m.variance.grad.item()) <= 1e-5 * max(1e-3, abs(m.variance.grad.item()))
 
 
+@pytest.mark.parametrize("binned", [False, True], ids=["atomic_grid", "binned_grid"])
 @pytest.mark.parametrize("name", ["step_c1.npz", "step_c2.npz"])
-def test_fused_train_step_analytical_matches_oracle(name):
+def test_fused_train_step_analytical_matches_oracle(name, binned):
     """The whole training step (sampler, render, Huber loss, backward, V_R, eikonal and
-    curvature, s-amplification) with VoxNeuS(gradient_mode="analytical")."""
+    curvature, s-amplification) with VoxNeuS(gradient_mode="analytical"), the grid gradient
+    from the atomic scatter and from the voxel-binned passes."""
     from voxneus_amd.training import FusedTrainStep, StepConfig
     g = load_golden(name)
     sc = scenes.config1_scene(seed=0) if name == "step_c1.npz" else scenes.make_scene("multi", 128, 17, seed=2)
@@ -153,7 +155,8 @@ def test_fused_train_step_analytical_matches_oracle(name):
     ref = m.train_step(o, d, tgt, w_eik=0.1, w_curv=5e-4, zs=zs)
     st = FusedTrainStep(torch.from_numpy(sc.grid).to(DEV), torch.from_numpy(sc.embeddings).to(DEV),
                         _mlp_flat(sc).to(DEV), torch.tensor(0.3, device=DEV), res,
-                        cfg=StepConfig(w_eik=0.1, w_curv=5e-4, gradient_mode="analytical"))
+                        cfg=StepConfig(w_eik=0.1, w_curv=5e-4, gradient_mode="analytical",
+                                       binned_grid_gradient=binned))
     out = st.step(o.to(DEV), d.to(DEV), tgt.to(DEV))
     assert np.array_equal(out["z_vals"].cpu().numpy(), g["z_vals"])
     assert np.array_equal(st.active_linear().cpu().numpy(), ref["active_linear"].numpy())

@@ -271,14 +271,18 @@ def test_render_backward_hash_gradient_split_pairs(Fn):
         assert rel_err(got[l], ref[l]) < 1e-5, l
 
 
+@pytest.mark.parametrize("binned", [False, True], ids=["atomic_grid", "binned_grid"])
 @pytest.mark.parametrize("name", ["step_c1.npz", "step_c2.npz"])
-def test_fused_train_step_matches_reference(name):
+def test_fused_train_step_matches_reference(name, binned):
+    """The fused step against the reference's own step (golden vectors), with the grid
+    gradient from the atomic scatter and from the voxel-binned passes."""
     from voxneus_amd.training import FusedTrainStep, StepConfig
     g = load_golden(name)
     sc = _golden_scene(name)
     st = FusedTrainStep(torch.from_numpy(sc.grid).to(DEV), torch.from_numpy(sc.embeddings).to(DEV),
                         _mlp_flat(sc).to(DEV), torch.tensor(0.3, device=DEV),
-                        R.hash_level_resolutions(16, 16, sc.hash_max_resolution), cfg=StepConfig(w_eik=0.1, w_curv=5e-4))
+                        R.hash_level_resolutions(16, 16, sc.hash_max_resolution),
+                        cfg=StepConfig(w_eik=0.1, w_curv=5e-4, binned_grid_gradient=binned))
     out = st.step(torch.from_numpy(g["rays_o"]).to(DEV), torch.from_numpy(g["rays_d"]).to(DEV),
                   torch.from_numpy(g["target"]).to(DEV))
     assert np.array_equal(out["z_vals"].cpu().numpy(), g["z_vals"])

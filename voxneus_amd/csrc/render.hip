@@ -1522,24 +1522,45 @@ constexpr size_t kSumLds = (size_t)kSumWaves * kSumSlots * (sizeof(float2) + 1);
 VXN_DEV int slot_corner(int ox, int oy, int oz) { return 4 * ox + 2 * oy + oz; }
 VXN_DEV int slot_face(int axis, int side, int u, int v) { return 8 + axis * 8 + side * 4 + 2 * u + v; }
 
-__global__ void __launch_bounds__(256) grid_scatter_kernel(RenderDev a) {
+// Inputs of the grid gradient for sample i (zero past the last sample), loaded ahead of
+// their use by the binned passes.
+struct GridIn {
+  float dsdf, dg0, dg1, dg2, px, py, pz;
+};
+
+VXN_DEV GridIn grid_in(const RenderDev& a, int64_t i) {
   const int64_t NS = a.R * (int64_t)a.S;
-  const int lane = threadIdx.x & 63;
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  const bool in = i < NS;
-  float dsdf = 0.0f, dg[3] = {0.0f, 0.0f, 0.0f};
-  float px = 0.0f, py = 0.0f, pz = 0.0f;
-  if (in) {
-    dsdf = a.ggeo[i];
-    dg[0] = a.ggeo[NS + i];
-    dg[1] = a.ggeo[2 * NS + i];
-    dg[2] = a.ggeo[3 * NS + i];
-    sample_pos(a, i, px, py, pz);
+  GridIn x{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+  if (i < NS) {
+    x.dsdf = a.ggeo[i];
+    x.dg0 = a.ggeo[NS + i];
+    x.dg1 = a.ggeo[2 * NS + i];
+    x.dg2 = a.ggeo[3 * NS + i];
+    sample_pos(a, i, x.px, x.py, x.pz);
   }
+  return x;
+}
+
+// The brick of one sample (called by every lane of the wavefront): when its whole Eq. 7
+// stencil (Eq. 4: its 8 cell corners) lies inside the grid, v gets the 32 stencil weights
+// of the brick around cell c0 (slot_corner / slot_face order; the grid_sample weights are
+// folded in when grid_sample picks the same cell) and the call returns true.  Stencils at
+// the border and grid_sample cells other than the brick's are added to grad_grid here with
+// atomics when EDGES (they are rare).  key is the cell (a lane-unique value without a brick).
+template <bool VALUES, bool EDGES>
+VXN_DEV bool grid_brick(const RenderDev& a, const GridIn& x, int lane, float (&v)[32], uint64_t& key, int (&c0)[3]) {
+  const float dsdf = x.dsdf;
+  const float dg[3] = {x.dg0, x.dg1, x.dg2};
   const bool live = dsdf != 0.0f || dg[0] != 0.0f || dg[1] != 0.0f || dg[2] != 0.0f;
-  if (__ballot(live) == 0) return;
-  const Corners cn = corner_decomp(a.g, px, py, pz);
-  const GsStencil gs = gs_stencil(a.g, px, py, pz);
+  key = ~0ull - (uint64_t)lane;
+  c0[0] = c0[1] = c0[2] = 0;
+  if (VALUES) {
+#pragma unroll
+    for (int k = 0; k < 32; ++k) v[k] = 0.0f;
+  }
+  if (__ballot(live) == 0) return false;
+  const Corners cn = corner_decomp(a.g, x.px, x.py, x.pz);
+  const GsStencil gs = gs_stencil(a.g, x.px, x.py, x.pz);
   const GridDesc& g = a.g;
   const bool ana = g.analytical != 0;
   // Eq. 7 bricks: the whole 4x4x4 brick is inside the grid and the divisors are 2*eps.
@@ -1549,21 +1570,18 @@ __global__ void __launch_bounds__(256) grid_scatter_kernel(RenderDev a) {
           : (cn.c0[0] >= 1 && cn.c0[0] + 2 <= g.X - 1 && cn.c0[1] >= 1 && cn.c0[1] + 2 <= g.Y - 1 &&
              cn.c0[2] >= 1 && cn.c0[2] + 2 <= g.Z - 1);
   const bool gs_same = gs.x0 == cn.c0[0] && gs.y0 == cn.c0[1] && gs.z0 == cn.c0[2] && gs.valid == 0xffu;
-  if (live && !interior) {
+  if (EDGES && live && !interior) {
     grid_gradient_scatter(g, cn, dg, a.grad_grid);
     if (dsdf != 0.0f) gs_scatter(g, gs, dsdf, a.grad_grid);
   }
   const bool brick = live && interior;
-  float v[32];
-#pragma unroll
-  for (int k = 0; k < 32; ++k) v[k] = 0.0f;
-  if (brick && ana) {
+  if (VALUES && brick && ana) {
     float cg[8];
     eq4_corner_grads(g, cn, dg, cg);
 #pragma unroll
     for (int k = 0; k < 8; ++k) v[k] = cg[k];
   }
-  if (brick && !ana) {
+  if (VALUES && brick && !ana) {
     // interior divisor of every corner: max((i_p - i_m) * eps, eps) = 2 * eps
     const float dvx = fmaxf(2.0f * g.eps[0], g.eps[0]), dvy = fmaxf(2.0f * g.eps[1], g.eps[1]),
                 dvz = fmaxf(2.0f * g.eps[2], g.eps[2]);
@@ -1585,14 +1603,33 @@ __global__ void __launch_bounds__(256) grid_scatter_kernel(RenderDev a) {
   }
   if (brick && dsdf != 0.0f) {
     if (gs_same) {
+      if (VALUES) {
 #pragma unroll
-      for (int k = 0; k < 8; ++k) v[k] += gs.w[k] * dsdf;
-    } else {
+        for (int k = 0; k < 8; ++k) v[k] += gs.w[k] * dsdf;
+      }
+    } else if (EDGES) {
       gs_scatter(g, gs, dsdf, a.grad_grid);
     }
   }
-  const uint64_t key = brick ? (((uint64_t)cn.c0[0] << 42) | ((uint64_t)cn.c0[1] << 21) | (uint64_t)cn.c0[2])
-                             : (~0ull - (uint64_t)lane);
+  if (brick) {
+    key = ((uint64_t)cn.c0[0] << 42) | ((uint64_t)cn.c0[1] << 21) | (uint64_t)cn.c0[2];
+    c0[0] = cn.c0[0];
+    c0[1] = cn.c0[1];
+    c0[2] = cn.c0[2];
+  }
+  return brick;
+}
+
+__global__ void __launch_bounds__(256) grid_scatter_kernel(RenderDev a) {
+  const int lane = threadIdx.x & 63;
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const GridIn x = grid_in(a, i);
+  float v[32];
+  uint64_t key;
+  int c0[3];
+  const bool brick = grid_brick<true, true>(a, x, lane, v, key, c0);
+  const GridDesc& g = a.g;
+  const bool ana = g.analytical != 0;
   const bool tail = seg_sum<32>(key, v, lane) && brick;
   const uint64_t tb = __ballot(tail);
   if (tb == 0) return;
@@ -1602,7 +1639,7 @@ __global__ void __launch_bounds__(256) grid_scatter_kernel(RenderDev a) {
   // Passes of 16 runs; lane quad q takes the q-th run of the pass and lane 4q + e its
   // e-th z, so each atomic instruction writes whole columns and merges into few requests.
   const int64_t sx = (int64_t)g.Y * g.Z, sy = g.Z;
-  const int64_t base_own = brick ? lin3(g, cn.c0[0], cn.c0[1], cn.c0[2]) : 0;
+  const int64_t base_own = brick ? lin3(g, c0[0], c0[1], c0[2]) : 0;
   const int e = lane & 3, q = lane >> 2;
   const int ntail = __popcll(tb);
   for (int pass = 0; 16 * pass < ntail; ++pass) {
@@ -1641,6 +1678,281 @@ __global__ void __launch_bounds__(256) grid_scatter_kernel(RenderDev a) {
       if (act && e >= 2) atomicAdd(gp + u * sx + off * sy + (e - 2), e == 2 ? ay0 : ay1);
     }
   }
+}
+
+// ------------------------------------------------------------------------------------
+// Voxel-binned grid gradient: the bricks of grid_scatter_kernel's runs, sorted by the grid
+// tile they touch and summed per tile in LDS, so every vertex of grad_grid is written by one
+// plain read-add-write (the border and grid_sample edge cases above stay atomic).
+//   1. grid_bin_count_kernel: one wavefront per count row of kGridRowSamples samples (16
+//      chunks of 64, the runs of grid_scatter_kernel): per tile, the row's (run, tile) pairs.
+//   2. the bin column scans of the hash gradient turn the rows into positions.
+//   3. grid_bin_write_kernel: the same runs again; a run's brick (32 floats) goes to run
+//      slot (chunk * 64 + rank of the run in its chunk), and {run slot, cell} to each
+//      tile's list at the row's next position (LDS atomics on a row only the wavefront
+//      touches, so the lists are the same on every run).
+//   4. grid_bin_sum_kernel: four work-groups per tile; each wavefront owns an 8 x 8 x 16
+//      block of the tile in LDS, queues the list entries whose bricks reach it (in list
+//      order) and adds their slots, then adds its block into grad_grid.  Every vertex's sum
+//      follows the list order, so grad_grid is the same on every run.
+// ------------------------------------------------------------------------------------
+constexpr int kGtX = 32, kGtY = 32, kGtZ = 16;  // vertices per grid tile (64 KB of LDS)
+constexpr int kGridRowSamples = 1024;           // samples per count row: one wavefront
+constexpr int kGridRowChunks = kGridRowSamples / 64;
+constexpr int kGridMaxBins = 4096;              // tiles of a grid the binned passes take
+constexpr int kGridMaxDim = 1024;               // cell coordinates are packed in 10 bits
+constexpr int kStageStride = 33;                // LDS floats per staged brick
+
+struct GridBinDev {
+  int tx, ty, tz;   // tiles per axis
+  int nbins;        // tx * ty * tz, tile (bx, by, bz) = bin (bx * ty + by) * tz + bz
+  int nrows;        // count rows
+  int32_t* cnt;     // [nrows][nbins] (pairs per row and tile, then the row's first position)
+  int32_t* base;    // [nbins + 1] first list entry of each tile
+  uint2* ref;       // [pairs] {run slot, cell of the run}, tile by tile
+  float* val;       // [samples][32] brick of the run in each run slot
+};
+
+// Last lane of a run of equal keys (the tail seg_sum reports).
+VXN_DEV bool run_tail(uint64_t key, int lane) {
+  const int klo = (int)(uint32_t)key, khi = (int)(uint32_t)(key >> 32);
+  const uint64_t kn = (uint64_t)(uint32_t)dpp_i<kDppWaveShl1, 0xF>(klo) |
+                      ((uint64_t)(uint32_t)dpp_i<kDppWaveShl1, 0xF>(khi) << 32);
+  return lane == 63 || kn != key;
+}
+
+// Vertex range of a brick along one axis: Eq. 7 reaches c0 - 1 .. c0 + 2, Eq. 4 c0 .. c0 + 1.
+VXN_DEV int brick_lo(bool ana, int c) { return ana ? c : c - 1; }
+VXN_DEV int brick_hi(bool ana, int c) { return ana ? c + 1 : c + 2; }
+
+// Calls f(bin) for every tile the brick at cell c reaches (1 to 8 of them).
+template <class F>
+VXN_DEV void brick_tiles(const GridBinDev& b, bool ana, const int (&c)[3], F&& f) {
+  const int x0 = brick_lo(ana, c[0]) / kGtX, x1 = brick_hi(ana, c[0]) / kGtX;
+  const int y0 = brick_lo(ana, c[1]) / kGtY, y1 = brick_hi(ana, c[1]) / kGtY;
+  const int z0 = brick_lo(ana, c[2]) / kGtZ, z1 = brick_hi(ana, c[2]) / kGtZ;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int bx = x0 + ((q >> 2) & 1), by = y0 + ((q >> 1) & 1), bz = z0 + (q & 1);
+    if (bx <= x1 && by <= y1 && bz <= z1) f((bx * b.ty + by) * b.tz + bz);
+  }
+}
+
+__global__ void __launch_bounds__(256) grid_bin_count_kernel(RenderDev a, GridBinDev b) {
+  extern __shared__ int32_t gcnt_lds[];  // [4][nbins]
+  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t r = (int64_t)blockIdx.x * 4 + wid;
+  if (r >= b.nrows) return;
+  int32_t* row = gcnt_lds + wid * b.nbins;
+  for (int k = lane; k < b.nbins; k += 64) row[k] = 0;
+  wave_sync();
+  const bool ana = a.g.analytical != 0;
+  const int64_t i0 = r * kGridRowSamples + lane;
+  GridIn nxt = grid_in(a, i0);
+  for (int c = 0; c < kGridRowChunks; ++c) {
+    const GridIn cur = nxt;
+    if (c + 1 < kGridRowChunks) nxt = grid_in(a, i0 + (c + 1) * 64);
+    float v[32];
+    uint64_t key;
+    int c0[3];
+    const bool brick = grid_brick<false, false>(a, cur, lane, v, key, c0);
+    if (run_tail(key, lane) && brick) brick_tiles(b, ana, c0, [&](int bin) { atomicAdd(&row[bin], 1); });
+  }
+  wave_sync();
+  int32_t* out = b.cnt + r * b.nbins;
+  for (int k = lane; k < b.nbins; k += 64) out[k] = row[k];
+}
+
+// LDS of the write pass: per wavefront its row of next positions and a 64-brick stage.
+static size_t grid_write_lds(int nbins) {
+  return 4 * ((size_t)nbins * sizeof(int32_t) + (size_t)64 * kStageStride * sizeof(float));
+}
+
+__global__ void __launch_bounds__(256) grid_bin_write_kernel(RenderDev a, GridBinDev b) {
+  extern __shared__ int32_t gw_lds[];
+  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t r = (int64_t)blockIdx.x * 4 + wid;
+  if (r >= b.nrows) return;
+  int32_t* pos = gw_lds + wid * b.nbins;                                   // [nbins]
+  float* stage = (float*)(gw_lds + 4 * b.nbins) + wid * 64 * kStageStride;  // [64][kStageStride]
+  const int32_t* prow = b.cnt + r * b.nbins;
+  for (int k = lane; k < b.nbins; k += 64) pos[k] = prow[k];
+  wave_sync();
+  const bool ana = a.g.analytical != 0;
+  const int64_t i0 = r * kGridRowSamples;
+  GridIn nxt = grid_in(a, i0 + lane);
+  for (int c = 0; c < kGridRowChunks; ++c) {
+    const GridIn cur = nxt;
+    if (c + 1 < kGridRowChunks) nxt = grid_in(a, i0 + (c + 1) * 64 + lane);
+    float v[32];
+    uint64_t key;
+    int c0[3];
+    const bool brick = grid_brick<true, true>(a, cur, lane, v, key, c0);
+    const bool tail = seg_sum<32>(key, v, lane) && brick;
+    const uint64_t tb = __ballot(tail);
+    if (tb == 0) continue;
+    // run slots of this chunk: run0 + rank, rank = runs before this lane's in the chunk
+    const int64_t run0 = i0 + c * 64;
+    const int rank = __popcll(tb & ((1ull << lane) - 1ull));
+    if (tail) {
+#pragma unroll
+      for (int k = 0; k < 32; ++k) stage[rank * kStageStride + k] = v[k];
+      const uint32_t cell = (uint32_t)c0[0] | ((uint32_t)c0[1] << 10) | ((uint32_t)c0[2] << 20);
+      const uint2 ent = make_uint2((uint32_t)(run0 + rank), cell);
+      brick_tiles(b, ana, c0, [&](int bin) { b.ref[atomicAdd(&pos[bin], 1)] = ent; });
+    }
+    wave_sync();
+    // the chunk's bricks are consecutive run slots: store them as whole 16-byte words
+    const int nq = __popcll(tb) * 8;
+    float4* dst = (float4*)(b.val + run0 * 32);
+    for (int q = lane; q < nq; q += 64) {
+      const float* sp = stage + (q >> 3) * kStageStride + (q & 7) * 4;
+      dst[q] = make_float4(sp[0], sp[1], sp[2], sp[3]);
+    }
+    wave_sync();
+  }
+}
+
+// Sum pass: four work-groups per tile, one per 16 x 16 x 16 quarter (x, y halves), of four
+// wavefronts that each own an 8 x 8 x 16 block in LDS.  The work-group stages the tile's
+// list 256 entries at a time (one per thread); each wavefront queues the staged entries
+// whose bricks reach its block, then adds them two bricks per round (lanes 0-31 entry i,
+// lanes 32-63 entry i + h of the queue), with the entries and brick values of kGsumBatch
+// rounds loaded together, so a round waits only for its own LDS read-add-write.  Bricks
+// closer than 4 cells on every axis may share vertices: their halves are added one after
+// the other.
+constexpr int kGsumWaves = 4;
+constexpr int kGsumChunk = 64 * kGsumWaves;
+constexpr int kGsumQueue = 128;
+constexpr int kGsumBatch = 16;
+constexpr int kGbX = 8, kGbY = 8;  // a wavefront's block: kGbX x kGbY x kGtZ vertices
+constexpr int kGqX = 16, kGqY = 16;  // a work-group's quarter of the tile
+static_assert((kGqX / kGbX) * (kGqY / kGbY) == kGsumWaves, "one block per wavefront");
+static_assert((kGtX / kGqX) * (kGtY / kGqY) == 4, "four quarters per tile");
+constexpr size_t kGsumLds = (size_t)kGsumWaves * kGbX * kGbY * kGtZ * sizeof(float) +
+                            (size_t)kGsumChunk * sizeof(uint2) + (size_t)kGsumWaves * kGsumQueue * sizeof(uint2);
+
+VXN_DEV int absdiff(int a, int b) { return a > b ? a - b : b - a; }
+
+__global__ void __launch_bounds__(64 * kGsumWaves) grid_bin_sum_kernel(GridBinDev b, GridDesc g,
+                                                                      float* __restrict__ grad_grid) {
+  extern __shared__ float gsl[];
+  constexpr int kBlock = kGbX * kGbY * kGtZ;
+  const int t = threadIdx.x, wid = t >> 6, lane = t & 63;
+  uint2* ents = (uint2*)(gsl + kGsumWaves * kBlock);    // [kGsumChunk] staged entries
+  uint2* queue = ents + kGsumChunk + wid * kGsumQueue;  // this wavefront's queue
+  const int bin = blockIdx.x >> 2, quarter = blockIdx.x & 3;
+  const int bz = bin % b.tz, by = (bin / b.tz) % b.ty, bx = bin / (b.tz * b.ty);
+  const int x0 = bx * kGtX + kGqX * (quarter >> 1) + kGbX * (wid >> 1);
+  const int y0 = by * kGtY + kGqY * (quarter & 1) + kGbY * (wid & 1);
+  const int z0 = bz * kGtZ;
+  float* acc = gsl + wid * kBlock;  // [kGbX][kGbY][kGtZ]
+  for (int k = lane; k < kBlock; k += 64) acc[k] = 0.0f;
+  const bool ana = g.analytical != 0;
+  // lane s (and s + 32) takes slot s of a brick: its offset from the brick's cell
+  const int s = lane & 31, half = lane >> 5;
+  int sdx, sdy, sdz;
+  if (s < 8) {
+    sdx = (s >> 2) & 1;
+    sdy = (s >> 1) & 1;
+    sdz = s & 1;
+  } else {
+    const int f = s - 8, axis = f >> 3, off = ((f >> 2) & 1) ? 2 : -1, u = (f >> 1) & 1, w = f & 1;
+    sdx = axis == 0 ? off : u;
+    sdy = axis == 0 ? u : (axis == 1 ? off : w);
+    sdz = axis == 2 ? off : w;
+  }
+  const int beg = b.base[bin], end = b.base[bin + 1];
+  // adds the queued bricks [0, qn) into the block
+  auto drain = [&](int qn) {
+    wave_sync();
+    const int h = (qn + 1) >> 1;
+    for (int i0 = 0; i0 < h; i0 += kGsumBatch) {
+      uint32_t km[kGsumBatch];
+      float val[kGsumBatch];
+      bool apart[kGsumBatch];
+#pragma unroll
+      for (int q = 0; q < kGsumBatch; ++q) {
+        const int i = i0 + q;
+        const bool two = i + h < qn;
+        const uint2 ea = i < h ? queue[i] : make_uint2(0u, 0u);
+        const uint2 eb = two ? queue[i + h] : make_uint2(0u, 0u);
+        const bool act = i < h && (half == 0 || two);
+        const uint2 e = half ? eb : ea;
+        km[q] = e.y;
+        val[q] = act ? b.val[(int64_t)e.x * 32 + s] : 0.0f;
+        apart[q] = !two || absdiff((int)(ea.y & 1023u), (int)(eb.y & 1023u)) > 3 ||
+                   absdiff((int)((ea.y >> 10) & 1023u), (int)((eb.y >> 10) & 1023u)) > 3 ||
+                   absdiff((int)(ea.y >> 20), (int)(eb.y >> 20)) > 3;
+      }
+#pragma unroll
+      for (int q = 0; q < kGsumBatch; ++q) {
+        if (i0 + q < h) {
+          const int lx = (int)(km[q] & 1023u) + sdx - x0, ly = (int)((km[q] >> 10) & 1023u) + sdy - y0,
+                    lz = (int)(km[q] >> 20) + sdz - z0;
+          const bool ok = val[q] != 0.0f && (unsigned)lx < (unsigned)kGbX && (unsigned)ly < (unsigned)kGbY &&
+                          (unsigned)lz < (unsigned)kGtZ;
+          const int idx = (lx * kGbY + ly) * kGtZ + lz;
+          if (apart[q]) {
+            if (ok) acc[idx] += val[q];
+          } else {
+            if (ok && half == 0) acc[idx] += val[q];
+            wave_sync();
+            if (ok && half == 1) acc[idx] += val[q];
+          }
+          wave_sync();
+        }
+      }
+    }
+  };
+  uint2 pe = beg + t < end ? b.ref[beg + t] : make_uint2(0u, 0u);
+  for (int c0 = beg; c0 < end; c0 += kGsumChunk) {
+    __syncthreads();
+    ents[t] = pe;
+    __syncthreads();
+    if (c0 + kGsumChunk + t < end) pe = b.ref[c0 + kGsumChunk + t];  // the next chunk in flight
+    const int n = min(kGsumChunk, end - c0);
+    int qn = 0;
+    for (int w0 = 0; w0 < n; w0 += 64) {
+      const int j = w0 + lane;
+      const uint2 e = j < n ? ents[j] : make_uint2(0u, 0u);
+      const int cx = (int)(e.y & 1023u), cy = (int)((e.y >> 10) & 1023u);
+      const bool touch = j < n && brick_hi(ana, cx) >= x0 && brick_lo(ana, cx) < x0 + kGbX &&
+                         brick_hi(ana, cy) >= y0 && brick_lo(ana, cy) < y0 + kGbY;
+      const uint64_t m = __ballot(touch);
+      if (touch) queue[qn + __popcll(m & ((1ull << lane) - 1ull))] = e;
+      qn += __popcll(m);
+      if (qn > kGsumQueue - 64) {
+        drain(qn);
+        qn = 0;
+      }
+    }
+    if (qn > 0) drain(qn);
+  }
+  // the block into grad_grid (z runs of kGtZ floats): every load first, then the stores
+  constexpr int kPer = kBlock / 64;
+  float v[kPer], o[kPer];
+#pragma unroll
+  for (int q = 0; q < kPer; ++q) {
+    const int k = lane + 64 * q;
+    const int vx = x0 + k / (kGbY * kGtZ), vy = y0 + (k / kGtZ) % kGbY, vz = z0 + k % kGtZ;
+    v[q] = acc[k];
+    if (!(vx < g.X && vy < g.Y && vz < g.Z)) v[q] = 0.0f;
+    o[q] = v[q] != 0.0f ? grad_grid[lin3(g, vx, vy, vz)] : 0.0f;
+  }
+#pragma unroll
+  for (int q = 0; q < kPer; ++q) {
+    const int k = lane + 64 * q;
+    const int vx = x0 + k / (kGbY * kGtZ), vy = y0 + (k / kGtZ) % kGbY, vz = z0 + k % kGtZ;
+    if (v[q] != 0.0f) grad_grid[lin3(g, vx, vy, vz)] = o[q] + v[q];
+  }
+}
+
+// Tiles of the binned grid gradient for an X x Y x Z grid, or 0 when it does not apply.
+static int grid_bins(int X, int Y, int Z) {
+  if (X > kGridMaxDim || Y > kGridMaxDim || Z > kGridMaxDim) return 0;
+  const int64_t n = (int64_t)((X + kGtX - 1) / kGtX) * ((Y + kGtY - 1) / kGtY) * ((Z + kGtZ - 1) / kGtZ);
+  return n <= kGridMaxBins ? (int)n : 0;
 }
 
 // Fixed-order reduction of the per-work-group partials: work-group b2 sums the blocks
@@ -1796,14 +2108,17 @@ static BinPlan bin_plan(int64_t ns, int L, int log2_T, const int32_t* level_res)
 }
 
 // Workspace layout: MLP partials | scalar partials | reduction rows | gfeat | ggeo | hfeat
-// | bin count rows | column chunk sums | bin bases | record entries | record values.
+// | bin count rows | column chunk sums | bin bases | record entries | record values
+// | grid tile count rows | their column chunk sums | tile bases | tile lists | bricks.
 struct RenderWs {
-  int64_t off_sc, off_norm, off_rows, off_feat, off_geo, off_hf, off_cnt, off_csum, off_base, off_ridx, off_rval,
-      total;
+  int64_t off_sc, off_norm, off_rows, off_feat, off_geo, off_hf, off_cnt, off_csum, off_base, off_ridx, off_rval;
+  int64_t off_gcnt, off_gcsum, off_gbase, off_gref, off_gval, total;
   BinPlan bins;
+  int gbins;   // grid tiles of the binned grid gradient (0: the atomic grid scatter)
+  int grows;   // its count rows
 };
 
-static RenderWs render_ws(int64_t R, int S, int L, int log2_T, const int32_t* level_res, int np) {
+static RenderWs render_ws(int64_t R, int S, int L, int log2_T, const int32_t* level_res, int np, int gbins) {
   RenderWs w;
   const int64_t nb = render_blocks(R);
   const int64_t ns = R * (int64_t)S;
@@ -1815,22 +2130,40 @@ static RenderWs render_ws(int64_t R, int S, int L, int log2_T, const int32_t* le
   w.off_hf = align256(w.off_geo + kGeo * ns * (int64_t)sizeof(float));
   w.off_cnt = align256(w.off_hf + 16 * ns * (int64_t)sizeof(uint32_t));
   w.bins = bin_plan(ns, L, log2_T, level_res);
+  int64_t end;
   if (w.bins.use) {
     w.off_csum = align256(w.off_cnt + w.bins.n_cnt * 4);
     w.off_base = align256(w.off_csum + w.bins.n_csum * 4);
     w.off_ridx = align256(w.off_base + ((int64_t)w.bins.nbins + 1) * 4);
     w.off_rval = align256(w.off_ridx + w.bins.max_rec * 4);
-    w.total = align256(w.off_rval + w.bins.max_rec * 16);
+    end = align256(w.off_rval + w.bins.max_rec * 16);
   } else {
     w.off_csum = w.off_base = w.off_ridx = w.off_rval = w.off_cnt;
-    w.total = w.off_cnt;
+    end = w.off_cnt;
   }
+  w.gbins = gbins;
+  w.grows = (int)((ns + kGridRowSamples - 1) / kGridRowSamples);
+  w.off_gcnt = end;
+  if (gbins > 0) {
+    const int64_t gchunk = (w.grows + kColChunk - 1) / kColChunk;
+    w.off_gcsum = align256(w.off_gcnt + (int64_t)w.grows * gbins * 4);
+    w.off_gbase = align256(w.off_gcsum + gchunk * gbins * 4);
+    w.off_gref = align256(w.off_gbase + ((int64_t)gbins + 1) * 4);
+    w.off_gval = align256(w.off_gref + ns * 8 * 8);  // a run reaches at most 8 tiles
+    end = align256(w.off_gval + ns * 32 * 4);
+  } else {
+    w.off_gcsum = w.off_gbase = w.off_gref = w.off_gval = end;
+  }
+  w.total = end;
   return w;
 }
 
 static RenderWs render_ws(const vxn_render_args& A) {
+  // the grid scatter on its own stream (scatter_stream) stays atomic: it runs beside the
+  // hash-table gradient and the caller's regularisers
+  const int gbins = A.grid_binned && !A.scatter_stream ? grid_bins(A.X, A.Y, A.Z) : 0;
   return render_ws(A.n_rays, A.n_samples, A.n_levels, A.log2_T, A.level_res,
-                   A.view_embed == VXN_VIEW_PE ? MlpLayout<27>::NP : MlpLayout<16>::NP);
+                   A.view_embed == VXN_VIEW_PE ? MlpLayout<27>::NP : MlpLayout<16>::NP, gbins);
 }
 
 static int build_render_dev(const vxn_render_args& A, RenderDev& a, int mode) {
@@ -1969,7 +2302,10 @@ static int launch_render_nv(const vxn_render_args* p, void* stream) {
     hipLaunchKernelGGL(mask_norm_kernel, dim3(1), dim3(256), 0, s, a.target_mask, p->n_rays, (float*)a.loss_norm);
     VXN_LAUNCH_CHECK();
   }
-  if (p->fork_event) VXN_HIP(hipEventRecord((hipEvent_t)p->fork_event, s));
+  // the binned grid gradient writes grad_grid with plain read-add-writes: work that the
+  // caller forks from fork_event (atomic adds into grad_grid) starts after it
+  const int gbins = MODE != kFwd ? render_ws(*p).gbins : 0;
+  if (p->fork_event && gbins == 0) VXN_HIP(hipEventRecord((hipEvent_t)p->fork_event, s));
   if (p->mlp_mode == VXN_MLP_BF16_MFMA)
     hipLaunchKernelGGL((render_kernel<MODE, kMlpBf16, 0, NV>), dim3(nb), dim3(kRT), 0, s, a);
   else
@@ -1979,20 +2315,64 @@ static int launch_render_nv(const vxn_render_args* p, void* stream) {
   if (MODE != kFwd) {
     const int64_t ns = p->n_rays * (int64_t)p->n_samples;
     const int sb = (int)((ns + 255) / 256);
-    // the grid scatter needs only the render kernel's per-sample geometry gradients: on a
-    // second stream it runs beside the hash-table gradient (atomics against plain stores
-    // and LDS work)
+    const RenderWs w = render_ws(*p);
     hipStream_t ss = (hipStream_t)p->scatter_stream;
     hipEvent_t const* se = (hipEvent_t const*)p->scatter_events;
-    if (ss) {
+    if (w.gbins > 0) {
+      char* ws = (char*)p->workspace;
+      GridBinDev gb;
+      gb.tx = (a.g.X + kGtX - 1) / kGtX;
+      gb.ty = (a.g.Y + kGtY - 1) / kGtY;
+      gb.tz = (a.g.Z + kGtZ - 1) / kGtZ;
+      gb.nbins = w.gbins;
+      gb.nrows = w.grows;
+      gb.cnt = (int32_t*)(ws + w.off_gcnt);
+      gb.base = (int32_t*)(ws + w.off_gbase);
+      gb.ref = (uint2*)(ws + w.off_gref);
+      gb.val = (float*)(ws + w.off_gval);
+      // the column scans of the hash gradient on the tile count rows (one "level")
+      BinDev cb = {};
+      cb.nbl = w.gbins;
+      cb.nbins = w.gbins;
+      cb.nrows = w.grows;
+      cb.nchunk = (w.grows + kColChunk - 1) / kColChunk;
+      cb.cnt = gb.cnt;
+      cb.csum = (int32_t*)(ws + w.off_gcsum);
+      cb.base = gb.base;
+      const unsigned gwg = (unsigned)((w.grows + 3) / 4);
+      hipLaunchKernelGGL(grid_bin_count_kernel, dim3(gwg), dim3(256), (size_t)4 * w.gbins * sizeof(int32_t), s, a,
+                         gb);
+      VXN_LAUNCH_CHECK();
+      const dim3 cgrid((unsigned)((cb.nbl + 255) / 256), (unsigned)cb.nchunk, 1u);
+      hipLaunchKernelGGL(bin_col_sums_kernel, cgrid, dim3(256), 0, s, cb);
+      VXN_LAUNCH_CHECK();
+      hipLaunchKernelGGL(bin_col_scan_kernel, dim3((unsigned)((cb.nbins + 255) / 256)), dim3(256), 0, s, cb);
+      VXN_LAUNCH_CHECK();
+      hipLaunchKernelGGL(bin_base_scan_kernel, dim3(1), dim3(kColThreads), 0, s, cb);
+      VXN_LAUNCH_CHECK();
+      hipLaunchKernelGGL(bin_col_positions_kernel, cgrid, dim3(256), 0, s, cb);
+      VXN_LAUNCH_CHECK();
+      hipLaunchKernelGGL(grid_bin_write_kernel, dim3(gwg), dim3(256), grid_write_lds(w.gbins), s, a, gb);
+      VXN_LAUNCH_CHECK();
+      hipLaunchKernelGGL(grid_bin_sum_kernel, dim3((unsigned)(4 * w.gbins)), dim3(64 * kGsumWaves), kGsumLds, s, gb,
+                         a.g, a.grad_grid);
+      VXN_LAUNCH_CHECK();
+      if (p->fork_event) VXN_HIP(hipEventRecord((hipEvent_t)p->fork_event, s));
+      if (ev) VXN_HIP(hipEventRecord(ev[3], s));
+    } else if (ss) {
+      // the atomic grid scatter needs only the render kernel's per-sample geometry
+      // gradients: on a second stream it runs beside the hash-table gradient
       VXN_HIP(hipEventRecord(se[0], s));
       VXN_HIP(hipStreamWaitEvent(ss, se[0], 0));
       hipLaunchKernelGGL(grid_scatter_kernel, dim3(sb), dim3(256), 0, ss, a);
       VXN_LAUNCH_CHECK();
-      if (ev) VXN_HIP(hipEventRecord(ev[4], ss));
+      if (ev) VXN_HIP(hipEventRecord(ev[3], ss));
       VXN_HIP(hipEventRecord(se[1], ss));
+    } else {
+      hipLaunchKernelGGL(grid_scatter_kernel, dim3(sb), dim3(256), 0, s, a);
+      VXN_LAUNCH_CHECK();
+      if (ev) VXN_HIP(hipEventRecord(ev[3], s));
     }
-    const RenderWs w = render_ws(*p);
     if (w.bins.use && ((uintptr_t)p->grad_table & 7) == 0) {
       char* ws = (char*)p->workspace;
       BinDev b;
@@ -2045,12 +2425,7 @@ static int launch_render_nv(const vxn_render_args* p, void* stream) {
       hipLaunchKernelGGL(hash_scatter_kernel, dim3(sb), dim3(256), 0, s, a, dedup_levels(a.h));
       VXN_LAUNCH_CHECK();
     }
-    if (ev) VXN_HIP(hipEventRecord(ev[3], s));
-    if (!ss) {
-      hipLaunchKernelGGL(grid_scatter_kernel, dim3(sb), dim3(256), 0, s, a);
-      VXN_LAUNCH_CHECK();
-      if (ev) VXN_HIP(hipEventRecord(ev[4], s));
-    }
+    if (ev) VXN_HIP(hipEventRecord(ev[4], s));
     float* rows = (float*)((char*)p->workspace + w.off_rows);
     constexpr int np = MlpLayout<NV>::NP;
     hipLaunchKernelGGL(render_reduce1_kernel, dim3((np + 255) / 256, kRedRows), dim3(256), 0, s, a.part_mlp, nb,

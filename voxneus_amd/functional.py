@@ -477,7 +477,7 @@ class RenderConfig:
 
     def __init__(self, box, level_res, level_w=None, sphere_radius=1.0, cos_anneal_ratio=0.0,
                  use_iter_cos=False, mlp_mode=N.VXN_MLP_FP32, table_grad_out=None,
-                 gradient_mode=N.VXN_GRADIENT_INTERPOLATED, view_embed=N.VXN_VIEW_SH):
+                 gradient_mode=N.VXN_GRADIENT_INTERPOLATED, view_embed=N.VXN_VIEW_SH, grid_binned=False):
         # table_grad_out: optional FP32 [L,T,F] tensor that the backward accumulates the
         # hash-table gradient into (for bf16 tables with an FP32 master copy); the table
         # input then receives no gradient through autograd
@@ -491,6 +491,9 @@ class RenderConfig:
         self.mlp_mode = mlp_mode
         self.gradient_mode = int(gradient_mode)
         self.view_embed = int(view_embed)  # VXN_VIEW_SH or VXN_VIEW_PE
+        # the backward's grid gradient voxel-binned (vxn_render_args.grid_binned) instead of
+        # the atomic scatter
+        self.grid_binned = bool(grid_binned)
 
 
 def fill_render_args(a: "N.RenderArgs", cfg: RenderConfig, grid3, table, mlp_flat, s: float, rays_o, rays_d,
@@ -519,6 +522,7 @@ def fill_render_args(a: "N.RenderArgs", cfg: RenderConfig, grid3, table, mlp_fla
     a.cos_anneal_ratio = cfg.cos_anneal_ratio
     a.use_iter_cos = int(cfg.use_iter_cos)
     a.gradient_mode = cfg.gradient_mode
+    a.grid_binned = int(cfg.grid_binned)
 
 
 def render_workspace(a: "N.RenderArgs", device) -> torch.Tensor:

@@ -100,6 +100,10 @@ class StepConfig:
     # atomics slow the binned passes by more than the overlap saves, 2.50 -> 2.48 ms BF16
     # and 3.29 -> 3.45 ms FP32 on the 8192-ray step)
     concurrent_scatter: bool = False
+    # the grid gradient voxel-binned (vxn_render_args.grid_binned: the same on every run, one
+    # plain read-add-write per vertex) instead of the atomic scatter; measured 0.25 ms
+    # against 0.21 ms on the 8192-ray step, so off by default
+    binned_grid_gradient: bool = False
     # max-abs of the weighted eikonal gradient during a post-upsample cooldown
     # (trainer.py:118, scheduler.eik_grad_clamp)
     eik_grad_clamp: float = 0.1
@@ -188,13 +192,14 @@ class FusedTrainStep:
         cfg = self.cfg
         R, S = mid.shape
         key = (R, S, mid.device, self.grid.data_ptr(), self.table.data_ptr(), self.mlp.data_ptr(),
-               self.variance.data_ptr())
+               self.variance.data_ptr(), cfg.binned_grid_gradient)
         a = self._args
         if a is None or self._args_key != key:
             rc = Fn.RenderConfig(self.box, self.level_res, self.level_w, cfg.sphere_radius, cfg.cos_anneal_ratio,
                                  cfg.use_iter_cos, mlp_mode=cfg.mlp_mode,
                                  gradient_mode=N.gradient_mode_code(cfg.gradient_mode),
-                                 view_embed=N.view_embed_code(cfg.view_embed_type))
+                                 view_embed=N.view_embed_code(cfg.view_embed_type),
+                                 grid_binned=cfg.binned_grid_gradient)
             a = N.RenderArgs()
             Fn.fill_render_args(a, rc, self.grid, self.table, self.mlp, 0.0, rays_o, rays_d, mid, dists)
             a.variance = N.dptr(self.variance, torch.float32, "variance")
@@ -295,7 +300,9 @@ class FusedTrainStep:
         if early:
             a.vr_bitmask = None
             if cfg.fork_after_features:
-                a.fork_event = self.side.events[0]  # recorded just before the render kernel
+                # recorded just before the render kernel (after the binned grid gradient,
+                # whose plain read-add-writes of grad_grid the regularisers' atomics follow)
+                a.fork_event = self.side.events[0]
             else:
                 self.side.fork(main)  # right after the sampler
         N.check(N.lib().vxn_render_train(ctypes.byref(a), main), "render_train")
