@@ -64,7 +64,7 @@ PROFILE_NAMES = {"upsample": "upsample_kernel", "reg_vr": "reg_vr_kernel"}
 # event index pairs bracketing each kernel in vxn_render_args.kernel_events
 KERNEL_EVENTS = {"hash_features": (0, 1), "render_kernel": (1, 2), "grid_gradient": (2, 3), "hash_gradient": (3, 4),
                  "reductions": (4, 5)}
-# with the atomic grid scatter on its own stream (StepConfig.concurrent_scatter) the
+# with the grid gradient on its own stream (StepConfig.concurrent_scatter, the default) the
 # hash-table gradient starts with it, and the reductions' span includes the wait for it
 KERNEL_EVENTS_CONCURRENT = dict(KERNEL_EVENTS, hash_gradient=(2, 4))
 # lattice query: 32-word stencil (128 B) in, SDF + normal (16 B) out per lattice point
@@ -94,7 +94,7 @@ def mlp_flat(sc):
     return torch.cat([torch.from_numpy(sc.mlp[k]).reshape(-1) for k in ("W0", "b0", "W1", "b1")])
 
 
-def run_gpu(sc, batches, form, steps, warmup, concurrent_scatter=False):
+def run_gpu(sc, batches, form, steps, warmup, serial_grid=False):
     import torch
 
     from voxneus_amd import _native as N
@@ -109,7 +109,7 @@ def run_gpu(sc, batches, form, steps, warmup, concurrent_scatter=False):
         mlp_mode = N.VXN_MLP_BF16_MFMA
     st = FusedTrainStep(torch.from_numpy(sc.grid).to(dev), table, mlp_flat(sc).to(dev),
                         torch.tensor(0.3, device=dev), level_resolutions(16, 16, sc.hash_max_resolution),
-                        cfg=StepConfig(perturb=True, mlp_mode=mlp_mode, concurrent_scatter=concurrent_scatter))
+                        cfg=StepConfig(perturb=True, mlp_mode=mlp_mode, concurrent_scatter=not serial_grid))
     gb = [tuple(torch.from_numpy(a).to(dev) for a in b) for b in batches]
     ev = []
 
@@ -136,7 +136,7 @@ def run_gpu(sc, batches, form, steps, warmup, concurrent_scatter=False):
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     gc.enable()
-    kev = KERNEL_EVENTS_CONCURRENT if st.cfg.concurrent_scatter else KERNEL_EVENTS
+    kev = KERNEL_EVENTS_CONCURRENT if st.scatter is not None else KERNEL_EVENTS
     kern = {k: float(np.mean([e.elapsed_ms(i, j) for e in ev])) for k, (i, j) in kev.items()}
     loss = out["loss"].item()
     # the step's launches outside the render call, each timed alone on the last step's inputs:
@@ -442,8 +442,9 @@ def main():
     ap.add_argument("--no-lattice", action="store_true",
                     help="skip the 320^3 measurements (lattice query, 49-view 800x600 render)")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop latency measurement")
-    ap.add_argument("--concurrent-scatter", action="store_true",
-                    help="run the grid scatter on its own stream beside the hash-table gradient")
+    ap.add_argument("--serial-grid", action="store_true",
+                    help="run the grid gradient on the main stream, after the render kernel (default: on its own "
+                         "stream beside the hash-table gradient)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB device copy (3 dispatches): the known-size kernel that scales "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes")
@@ -469,7 +470,7 @@ def main():
     res = {}
     for form in forms:
         log(f"GPU: {torch.cuda.get_device_name(0)}; form {form}; warmup {a.warmup}; steps {a.steps}")
-        r = run_gpu(sc, batches, form, a.steps, a.warmup, a.concurrent_scatter)
+        r = run_gpu(sc, batches, form, a.steps, a.warmup, a.serial_grid)
         log(f"gpu {form}: {r}")
         kernels = {}
         for k, ms in r["kernel_ms"].items():

@@ -260,9 +260,9 @@ typedef struct vxn_render_args {
   int32_t gradient_mode; /* VXN_GRADIENT_*: the normal of every sample (VoxNeuS gradient_mode) */
   /* nullable hipEvent_t, recorded on `stream` after the hash-feature gather, just before
      the render kernel (with grid_binned: after the binned grid gradient, whose plain
-     read-add-writes of grad_grid must not overlap other writers): a caller can start
-     independent work on another stream from it, such as atomic adds into grad_grid (the
-     fused training step starts its regularisers there) */
+     read-add-writes of grad_grid must not overlap other writers, on the stream that runs
+     it): a caller can start independent work on another stream from it, such as atomic
+     adds into grad_grid (the fused training step starts its regularisers there) */
   void* fork_event;
   /* nullable device float: the NeuS sharpness s read on the device in place of `s`, so a
      caller that keeps s on the device (s = clamp(exp(10 v)), models/variance.py:31) needs
@@ -282,11 +282,11 @@ typedef struct vxn_render_args {
   float mask_weight;
   double* loss_parts;
   /* backward and training entries: nullable second stream (default priority) for the
-     grid scatter, which then runs beside the hash-table gradient.  scatter_events
-     (hipEvent_t[2], required with it) order it after the render kernel and make `stream`
-     wait for it before the call returns, so every output is complete on `stream` as
-     without it.  kernel_events[3] is then recorded on scatter_stream after the grid
-     scatter, and the hash-table gradient's span is kernel_events[2] -> [4]. */
+     grid gradient (binned or atomic), which then runs beside the hash-table gradient.
+     scatter_events (hipEvent_t[2], required with it) order it after the render kernel and
+     make `stream` wait for it before the call returns, so every output is complete on
+     `stream` as without it.  kernel_events[3] is then recorded on scatter_stream after the
+     grid gradient, and the hash-table gradient's span is kernel_events[2] -> [4]. */
   void* scatter_stream;
   void* const* scatter_events;
   /* nullable device float: the variance v of SingleVarianceNetwork (models/variance.py:
@@ -308,10 +308,13 @@ typedef struct vxn_render_args {
   /* backward and training entries: 1 = the grid gradient is voxel-binned instead of the
      atomic scatter (gradient_interp.cu:86-152): each run's brick of stencil weights is
      listed under the 32 x 32 x 16-vertex grid tiles it reaches, and each tile's sums are
-     added into grad_grid with one plain read-add-write per vertex, so the grid gradient of
-     the call is the same on every run; stencils at the grid border stay atomic.  Grids up
-     to 1024 vertices per axis and 4096 tiles, without scatter_stream; otherwise (and with
-     0) the atomic scatter.  grad_grid must have no other writer during the call. */
+     added into grad_grid with one plain read-add-write per vertex; the stencils at the grid
+     border (and of the rare samples whose grid_sample cell is not their corner cell) stay
+     atomic but are added by one wavefront in sample order, so the grid gradient of the
+     call is the same on every run.  Grids up
+     to 1024 vertices per axis and 4096 tiles; otherwise (and with 0) the atomic scatter.
+     grad_grid must have no other writer during the call (fork_event marks the end of the
+     binned passes). */
   int32_t grid_binned;
 } vxn_render_args;
 
@@ -322,7 +325,7 @@ typedef struct vxn_render_args {
  * entries per level (16 levels) take the atomic scatter and no record space.  With
  * grid_binned, 200 more bytes per sample (a brick and up to 8 list entries per run) and
  * the tile count rows.  The size depends on n_rays, n_samples, n_levels, log2_T,
- * level_res, the grid size, grid_binned and scatter_stream.                             */
+ * level_res, the grid size and grid_binned.                             */
 int vxn_render_workspace_bytes(const vxn_render_args* a, int64_t* bytes);
 int vxn_render_forward(const vxn_render_args* a, void* stream);
 int vxn_render_backward(const vxn_render_args* a, void* stream);

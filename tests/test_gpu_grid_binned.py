@@ -8,7 +8,7 @@ list order and added into grad_grid once per vertex.
   magnitude (FP32 sums in another order) and is bit-identical from run to run;
 * a grid whose sides are not multiples of the tile (partial tiles at the far faces), both
   gradient modes (Eq. 7 interpolated and Eq. 4 analytical);
-* the flag is ignored with the grid scatter on its own stream (the atomic path runs).
+* on the second stream (vxn_render_args.scatter_stream) it equals the serial atomic scatter.
 The regulariser weights are 0 here so that nothing else adds into grad_grid.
 """
 
@@ -77,7 +77,9 @@ def test_binned_grid_gradient_partial_tiles(mode):
     assert _rel(b, a) <= 1e-6
 
 
-def test_binned_flag_ignored_with_scatter_stream():
+def test_binned_on_second_stream_equals_atomic():
+    """The binned grid gradient on the second stream (after the render kernel, beside the
+    hash-table gradient's record passes) against the serial atomic scatter."""
     sc = scenes.config1_scene(seed=0)
     o, d, t = scenes.config_rays("sphere", 256, seed=4)
     o, d, t = (torch.from_numpy(v).to(DEV) for v in (o, d, t))

@@ -242,14 +242,14 @@ class HipError(RuntimeError):
 
 
 class SideStream:
-    """A second stream (default priority) on ``device`` and a pair of untimed HIP events:
-    ``fork(stream)`` starts side work after what ``stream`` has queued (``events[0]`` can
-    also be recorded by a render launch as its ``fork_event``), ``finish()`` marks its end
-    and ``join(stream)`` makes ``stream`` wait for that mark."""
+    """A second stream (default priority) on ``device`` and ``n_events`` (at least 2)
+    untimed HIP events: ``fork(stream)`` starts side work after what ``stream`` has queued
+    (``events[0]`` can also be recorded by a render launch as its ``fork_event``),
+    ``finish()`` marks its end and ``join(stream)`` makes ``stream`` wait for that mark."""
 
     _hip = None
 
-    def __init__(self, device):
+    def __init__(self, device, n_events: int = 2):
         if SideStream._hip is None:
             h = ctypes.CDLL("libamdhip64.so.7")
             h.hipEventCreateWithFlags.argtypes = [ctypes.POINTER(c_vp), ctypes.c_uint]
@@ -258,8 +258,8 @@ class SideStream:
             h.hipEventRecord.argtypes = [c_vp, c_vp]
             SideStream._hip = h
         self.stream = torch.cuda.Stream(device=device)
-        self.events = (c_vp * 2)()
-        for i in range(2):
+        self.events = (c_vp * max(2, n_events))()
+        for i in range(max(2, n_events)):
             ev = c_vp()
             if SideStream._hip.hipEventCreateWithFlags(ctypes.byref(ev), 2) != 0:  # hipEventDisableTiming
                 raise HipError("hipEventCreateWithFlags failed")

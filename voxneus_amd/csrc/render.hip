@@ -813,6 +813,25 @@ VXN_DEV float dpp_f(float v) {
   return __int_as_float(dpp_i<CTRL, ROW_MASK>(__float_as_int(v)));
 }
 
+// One step of the segmented sum: every lane takes the DPP-shifted value (full EXEC: a DPP
+// source lane must be active), then the lanes of `mask` add it, under EXEC = mask; the step
+// is skipped when no lane needs it.
+constexpr int kSegCtrl[6] = {kDppRowShr + 1, kDppRowShr + 2, kDppRowShr + 4, kDppRowShr + 8, kDppRowBcast15,
+                             kDppRowBcast31};
+constexpr int kSegRows[6] = {0xF, 0xF, 0xF, 0xF, 0xA, 0xC};
+
+template <int STEP, int NV>
+VXN_DEV void seg_step(uint64_t mask, float (&v)[NV], bool c) {
+  if (mask == 0) return;
+  float u[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) u[k] = dpp_f<kSegCtrl[STEP], kSegRows[STEP]>(v[k]);
+  if (c) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] = v[k] + u[k];
+  }
+}
+
 template <int NV>
 VXN_DEV bool seg_sum(uint64_t key, float (&v)[NV], int lane) {
   const int klo = (int)(uint32_t)key, khi = (int)(uint32_t)(key >> 32);
@@ -832,23 +851,15 @@ VXN_DEV bool seg_sum(uint64_t key, float (&v)[NV], int lane) {
   const bool c1 = r >= 1 && lane - 1 >= st, c2 = r >= 2 && lane - 2 >= st, c4 = r >= 4 && lane - 4 >= st,
              c8 = r >= 8 && lane - 8 >= st;
   const bool c15 = ((lane >> 4) & 1) != 0 && st <= (lane & ~15) - 1, c31 = lane >= 32 && st <= 31;
-#pragma unroll
-  for (int k = 0; k < NV; ++k) {
-    float x = v[k];
-    float u = dpp_f<kDppRowShr + 1, 0xF>(x);
-    x = c1 ? x + u : x;
-    u = dpp_f<kDppRowShr + 2, 0xF>(x);
-    x = c2 ? x + u : x;
-    u = dpp_f<kDppRowShr + 4, 0xF>(x);
-    x = c4 ? x + u : x;
-    u = dpp_f<kDppRowShr + 8, 0xF>(x);
-    x = c8 ? x + u : x;
-    u = dpp_f<kDppRowBcast15, 0xA>(x);
-    x = c15 ? x + u : x;
-    u = dpp_f<kDppRowBcast31, 0xC>(x);
-    x = c31 ? x + u : x;
-    v[k] = x;
-  }
+  // each step adds the shifted value on the lanes whose run reaches the source lane: one
+  // DPP add per value with EXEC set to those lanes, and no step at all when no lane of the
+  // wavefront needs it (short runs skip the longer shifts)
+  seg_step<0>(__ballot(c1), v, c1);
+  seg_step<1>(__ballot(c2), v, c2);
+  seg_step<2>(__ballot(c4), v, c4);
+  seg_step<3>(__ballot(c8), v, c8);
+  seg_step<4>(__ballot(c15), v, c15);
+  seg_step<5>(__ballot(c31), v, c31);
   return lane == 63 || kn != key;
 }
 
@@ -1013,10 +1024,13 @@ constexpr uint32_t kFactFirst = 1u << 22;   // split pair: only the x corner, (1
 constexpr uint32_t kFactSecond = 2u << 22;  // split pair: only the x + 1 corner, fx g
 
 // Level l of the 64 consecutive samples a wavefront holds (lane = sample i): calls
-// emit(tail, cell, v, d0, d1) with every lane of the wavefront when some lane is live:
+// emit(tail, cell, v, d0, d1) with every lane of the wavefront when some lane is in range:
 // tail marks the lanes that emit records (the last lane of each run of samples in one
-// cell below `dedup_levels`, every live lane above), v their 8 corners' weighted
-// gradients (run sums) and d0, d1 the level's feature gradients of the lane's own sample.
+// cell below `dedup_levels`, every sample above), v their 8 corners' weighted gradients
+// (run sums) and d0, d1 the level's feature gradients of the lane's own sample.  A sample
+// whose two gradients are zero (most often one past the transmittance cutoff) is not live
+// and emits nothing; counting only live samples keeps the record passes to the samples
+// that contribute.
 // VALUES = false leaves v unset: only the tails are needed.
 // The inputs of sample i at level l: its two feature gradients, mid-point depth and ray
 // (zero past the last sample), loaded ahead of their use by the write pass.
@@ -1299,11 +1313,17 @@ __global__ void __launch_bounds__(256) hash_bin_write_kernel(RenderDev a, BinDev
       }
     }
     lds_barrier();
+    // a fixed number of store slots per thread: the next chunk's prefetched loads, issued
+    // before these stores, are then waited for without waiting for the stores themselves
     const int nst = ntot < kStageSlots ? ntot : kStageSlots;
-    for (int j = t; j < nst; j += 256) {
-      const int32_t dst = gb[sbin[j]] + j;
-      b.rval[dst] = stage[j];
-      if (!fact) b.ridx[dst] = sridx[j];
+#pragma unroll
+    for (int jj = 0; jj < kStageSlots / 256; ++jj) {
+      const int j = t + jj * 256;
+      if (j < nst) {
+        const int32_t dst = gb[sbin[j]] + j;
+        b.rval[dst] = stage[j];
+        if (!fact) b.ridx[dst] = sridx[j];
+      }
     }
   }
 }
@@ -1422,7 +1442,9 @@ __global__ void __launch_bounds__(256) bin_col_positions_kernel(BinDev b) {
 // adds both, in order); the others retry in the next round.  A single wavefront owns a copy
 // and its LDS instructions complete in order, so no update is lost.  A lane with nothing to
 // add in a round reads and writes its own dummy slot (entry kBinEntries + lane), so a round
-// is straight-line LDS code.  The bins run on XCD bin % 8 (work-group b on XCD b % 8).
+// is straight-line LDS code.  (Four groups' halves tagged together in one round, lane + 64
+// x group, took more rounds than they saved.)  The bins run on XCD bin % 8 (work-group b on
+// XCD b % 8).
 constexpr int kSumWaves = 4;
 constexpr int kSumSlots = kBinEntries + 64;
 
@@ -1442,15 +1464,38 @@ __global__ void __launch_bounds__(64 * kSumWaves) hash_bin_sum_kernel(BinDev b, 
   wave_sync();
   const int32_t r0 = b.base[bin], r1 = b.base[bin + 1];
   const bool fact = l >= b.fact_level;
-  constexpr int U = 4;  // 64-record groups loaded ahead
-  for (int32_t k0 = r0 + wid * 64; k0 < r1; k0 += U * 64 * kSumWaves) {
-    uint32_t r[U];
-    float4 v[U];
+  // U 64-record groups per iteration; the next iteration's groups are loaded before this
+  // one's are added, so the loads of a wavefront stay in flight while it works in LDS.
+  // Indices past the bin's end are clamped onto its last record (always a valid address
+  // inside the loop) and that record's use is masked.
+  constexpr int U = 4;
+  constexpr int32_t kStep = U * 64 * kSumWaves;
+  float4 nv[U];
+  uint32_t nr[U];
+  auto fetch = [&](int32_t k0) {
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const int32_t kk = k0 + u * 64 * kSumWaves + lane;
+      const int32_t kk = min(k0 + u * 64 * kSumWaves + lane, r1 - 1);
+      nv[u] = b.rval[kk];
+      nr[u] = fact ? 0u : b.ridx[kk];
+    }
+  };
+  if (r0 + wid * 64 < r1) fetch(r0 + wid * 64);
+  for (int32_t k0 = r0 + wid * 64; k0 < r1; k0 += kStep) {
+    uint32_t r[U];
+    float4 v[U];
+    float4 cv[U];
+    uint32_t cr[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      cv[u] = nv[u];
+      cr[u] = nr[u];
+    }
+    if (k0 + kStep < r1) fetch(k0 + kStep);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
       if (fact) {
-        const float4 f = kk < r1 ? b.rval[kk] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        const float4 f = cv[u];
         const uint32_t bits = __float_as_uint(f.x);
         const float ox = 1.0f - f.y;
         const float a0 = ox * f.z, a1 = ox * f.w, c0 = f.y * f.z, c1 = f.y * f.w;
@@ -1459,8 +1504,8 @@ __global__ void __launch_bounds__(64 * kSumWaves) hash_bin_sum_kernel(BinDev b, 
         v[u] = fl == 0u ? make_float4(a0, a1, c0, c1)
                         : (fl == kFactFirst ? make_float4(a0, a1, 0.0f, 0.0f) : make_float4(c0, c1, 0.0f, 0.0f));
       } else {
-        r[u] = kk < r1 ? b.ridx[kk] : 0u;
-        v[u] = kk < r1 ? b.rval[kk] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        r[u] = cr[u];
+        v[u] = cv[u];
       }
     }
 #pragma unroll
@@ -1547,13 +1592,18 @@ VXN_DEV GridIn grid_in(const RenderDev& a, int64_t i) {
 // folded in when grid_sample picks the same cell) and the call returns true.  Stencils at
 // the border and grid_sample cells other than the brick's are added to grad_grid here with
 // atomics when EDGES (they are rare).  key is the cell (a lane-unique value without a brick).
+// EDGES: the border stencils and the grid_sample stencils outside the brick are added
+// with device atomics here (the atomic scatter, and the binned passes' ordered edge pass);
+// without it they are only reported through `edge`.
 template <bool VALUES, bool EDGES>
-VXN_DEV bool grid_brick(const RenderDev& a, const GridIn& x, int lane, float (&v)[32], uint64_t& key, int (&c0)[3]) {
+VXN_DEV bool grid_brick(const RenderDev& a, const GridIn& x, int lane, float (&v)[32], uint64_t& key, int (&c0)[3],
+                        bool& edge) {
   const float dsdf = x.dsdf;
   const float dg[3] = {x.dg0, x.dg1, x.dg2};
   const bool live = dsdf != 0.0f || dg[0] != 0.0f || dg[1] != 0.0f || dg[2] != 0.0f;
   key = ~0ull - (uint64_t)lane;
   c0[0] = c0[1] = c0[2] = 0;
+  edge = false;
   if (VALUES) {
 #pragma unroll
     for (int k = 0; k < 32; ++k) v[k] = 0.0f;
@@ -1570,6 +1620,14 @@ VXN_DEV bool grid_brick(const RenderDev& a, const GridIn& x, int lane, float (&v
           : (cn.c0[0] >= 1 && cn.c0[0] + 2 <= g.X - 1 && cn.c0[1] >= 1 && cn.c0[1] + 2 <= g.Y - 1 &&
              cn.c0[2] >= 1 && cn.c0[2] + 2 <= g.Z - 1);
   const bool gs_same = gs.x0 == cn.c0[0] && gs.y0 == cn.c0[1] && gs.z0 == cn.c0[2] && gs.valid == 0xffu;
+  // grid_sample's cell one step from the corner cell along one axis (its floor rounds the
+  // other way near a cell face): its 8 corners are brick slots (cell corners and one face),
+  // so an Eq. 7 brick takes them too
+  const int gdx = gs.x0 - cn.c0[0], gdy = gs.y0 - cn.c0[1], gdz = gs.z0 - cn.c0[2];
+  const bool fold = !ana && !gs_same && gs.valid == 0xffu &&
+                    ((gdx * gdx == 1 && gdy == 0 && gdz == 0) || (gdx == 0 && gdy * gdy == 1 && gdz == 0) ||
+                     (gdx == 0 && gdy == 0 && gdz * gdz == 1));
+  edge = live && (!interior || (dsdf != 0.0f && !gs_same && !fold));
   if (EDGES && live && !interior) {
     grid_gradient_scatter(g, cn, dg, a.grad_grid);
     if (dsdf != 0.0f) gs_scatter(g, gs, dsdf, a.grad_grid);
@@ -1607,6 +1665,21 @@ VXN_DEV bool grid_brick(const RenderDev& a, const GridIn& x, int lane, float (&v
 #pragma unroll
         for (int k = 0; k < 8; ++k) v[k] += gs.w[k] * dsdf;
       }
+    } else if (fold) {
+      if (VALUES) {
+        // corner k = (ox, oy, oz) of the cell shifted by +-1 along x, y or z
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          const int ox = (k >> 2) & 1, oy = (k >> 1) & 1, oz = k & 1;
+          const float t = gs.w[k] * dsdf;
+          if (gdx == 1) v[ox ? slot_face(0, 1, oy, oz) : slot_corner(1, oy, oz)] += t;
+          if (gdx == -1) v[ox ? slot_corner(0, oy, oz) : slot_face(0, 0, oy, oz)] += t;
+          if (gdy == 1) v[oy ? slot_face(1, 1, ox, oz) : slot_corner(ox, 1, oz)] += t;
+          if (gdy == -1) v[oy ? slot_corner(ox, 0, oz) : slot_face(1, 0, ox, oz)] += t;
+          if (gdz == 1) v[oz ? slot_face(2, 1, ox, oy) : slot_corner(ox, oy, 1)] += t;
+          if (gdz == -1) v[oz ? slot_corner(ox, oy, 0) : slot_face(2, 0, ox, oy)] += t;
+        }
+      }
     } else if (EDGES) {
       gs_scatter(g, gs, dsdf, a.grad_grid);
     }
@@ -1627,7 +1700,8 @@ __global__ void __launch_bounds__(256) grid_scatter_kernel(RenderDev a) {
   float v[32];
   uint64_t key;
   int c0[3];
-  const bool brick = grid_brick<true, true>(a, x, lane, v, key, c0);
+  bool edge;
+  const bool brick = grid_brick<true, true>(a, x, lane, v, key, c0, edge);
   const GridDesc& g = a.g;
   const bool ana = g.analytical != 0;
   const bool tail = seg_sum<32>(key, v, lane) && brick;
@@ -1683,7 +1757,8 @@ __global__ void __launch_bounds__(256) grid_scatter_kernel(RenderDev a) {
 // ------------------------------------------------------------------------------------
 // Voxel-binned grid gradient: the bricks of grid_scatter_kernel's runs, sorted by the grid
 // tile they touch and summed per tile in LDS, so every vertex of grad_grid is written by one
-// plain read-add-write (the border and grid_sample edge cases above stay atomic).
+// plain read-add-write.  The border and grid_sample edge cases above stay atomic, but are
+// added by one wavefront in sample order (grid_edge_kernel), so their order is fixed too.
 //   1. grid_bin_count_kernel: one wavefront per count row of kGridRowSamples samples (16
 //      chunks of 64, the runs of grid_scatter_kernel): per tile, the row's (run, tile) pairs.
 //   2. the bin column scans of the hash gradient turn the rows into positions.
@@ -1695,6 +1770,7 @@ __global__ void __launch_bounds__(256) grid_scatter_kernel(RenderDev a) {
 //      block of the tile in LDS, queues the list entries whose bricks reach it (in list
 //      order) and adds their slots, then adds its block into grad_grid.  Every vertex's sum
 //      follows the list order, so grad_grid is the same on every run.
+//   5. grid_edge_kernel: the samples the write pass flagged, in sample order.
 // ------------------------------------------------------------------------------------
 constexpr int kGtX = 32, kGtY = 32, kGtZ = 16;  // vertices per grid tile (64 KB of LDS)
 constexpr int kGridRowSamples = 1024;           // samples per count row: one wavefront
@@ -1711,6 +1787,7 @@ struct GridBinDev {
   int32_t* base;    // [nbins + 1] first list entry of each tile
   uint2* ref;       // [pairs] {run slot, cell of the run}, tile by tile
   float* val;       // [samples][32] brick of the run in each run slot
+  uint64_t* edge;   // [nrows * kGridRowChunks] bit i of word w: sample 64 w + i has atomic stencils
 };
 
 // Last lane of a run of equal keys (the tail seg_sum reports).
@@ -1755,7 +1832,8 @@ __global__ void __launch_bounds__(256) grid_bin_count_kernel(RenderDev a, GridBi
     float v[32];
     uint64_t key;
     int c0[3];
-    const bool brick = grid_brick<false, false>(a, cur, lane, v, key, c0);
+    bool edge;
+    const bool brick = grid_brick<false, false>(a, cur, lane, v, key, c0, edge);
     if (run_tail(key, lane) && brick) brick_tiles(b, ana, c0, [&](int bin) { atomicAdd(&row[bin], 1); });
   }
   wave_sync();
@@ -1787,7 +1865,11 @@ __global__ void __launch_bounds__(256) grid_bin_write_kernel(RenderDev a, GridBi
     float v[32];
     uint64_t key;
     int c0[3];
-    const bool brick = grid_brick<true, true>(a, cur, lane, v, key, c0);
+    bool edge;
+    const bool brick = grid_brick<true, false>(a, cur, lane, v, key, c0, edge);
+    // the chunk's samples with atomic stencils, for the ordered edge pass
+    const uint64_t eb = __ballot(edge);
+    if (lane == 0) b.edge[r * kGridRowChunks + c] = eb;
     const bool tail = seg_sum<32>(key, v, lane) && brick;
     const uint64_t tb = __ballot(tail);
     if (tb == 0) continue;
@@ -1949,6 +2031,67 @@ __global__ void __launch_bounds__(64 * kGsumWaves) grid_bin_sum_kernel(GridBinDe
 }
 
 // Tiles of the binned grid gradient for an X x Y x Z grid, or 0 when it does not apply.
+// Pass 5: one wavefront lists the samples of the write pass's edge words in sample order
+// (in LDS, up to kEdgeList at a time), then adds their atomic stencils 64 samples per
+// instruction, lane k taking the k-th listed sample; so those adds reach every vertex in
+// sample order on every run.  A handful of samples per step take this path (the grid's
+// border cells, and samples whose grid_sample cell is not one of the brick's).
+constexpr int kEdgeBatch = 16;   // edge words per lane loaded at once
+constexpr int kEdgeList = 2048;  // listed samples per flush
+
+__global__ void __launch_bounds__(64) grid_edge_kernel(RenderDev a, const uint64_t* __restrict__ edge,
+                                                       int64_t nwords) {
+  __shared__ int32_t list[kEdgeList];
+  const int lane = threadIdx.x;
+  int n = 0;  // listed samples (wave-uniform)
+  auto flush = [&]() {
+    wave_sync();
+    for (int b = 0; b < n; b += 64) {
+      GridIn x{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+      if (b + lane < n) x = grid_in(a, (int64_t)list[b + lane]);
+      float v[32];
+      uint64_t key;
+      int c0[3];
+      bool e;
+      grid_brick<false, true>(a, x, lane, v, key, c0, e);
+    }
+    wave_sync();
+    n = 0;
+  };
+  for (int64_t w0 = 0; w0 < nwords; w0 += 64 * kEdgeBatch) {
+    uint64_t mine[kEdgeBatch];
+#pragma unroll
+    for (int u = 0; u < kEdgeBatch; ++u) {
+      const int64_t w = w0 + u * 64 + lane;
+      mine[u] = w < nwords ? edge[w] : 0ull;
+    }
+#pragma unroll
+    for (int u = 0; u < kEdgeBatch; ++u) {
+      if (__ballot(mine[u] != 0ull) == 0ull) continue;
+      // exclusive scan of the lanes' sample counts: lane order is sample order
+      const int c = __popcll(mine[u]);
+      int inc = c;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += t;
+      }
+      const int tot = __shfl(inc, 63, 64);
+      if (n + tot > kEdgeList) flush();
+      int pos = n + inc - c;
+      uint64_t m = mine[u];
+      const int64_t s0 = (w0 + u * 64 + lane) * 64;
+      while (m != 0ull) {
+        const int bit = __ffsll((unsigned long long)m) - 1;
+        m &= m - 1ull;
+        list[pos++] = (int32_t)(s0 + bit);
+      }
+      n += tot;
+    }
+  }
+  if (n > 0) flush();
+}
+
 static int grid_bins(int X, int Y, int Z) {
   if (X > kGridMaxDim || Y > kGridMaxDim || Z > kGridMaxDim) return 0;
   const int64_t n = (int64_t)((X + kGtX - 1) / kGtX) * ((Y + kGtY - 1) / kGtY) * ((Z + kGtZ - 1) / kGtZ);
@@ -2112,7 +2255,7 @@ static BinPlan bin_plan(int64_t ns, int L, int log2_T, const int32_t* level_res)
 // | grid tile count rows | their column chunk sums | tile bases | tile lists | bricks.
 struct RenderWs {
   int64_t off_sc, off_norm, off_rows, off_feat, off_geo, off_hf, off_cnt, off_csum, off_base, off_ridx, off_rval;
-  int64_t off_gcnt, off_gcsum, off_gbase, off_gref, off_gval, total;
+  int64_t off_gcnt, off_gcsum, off_gbase, off_gref, off_gval, off_gedge, total;
   BinPlan bins;
   int gbins;   // grid tiles of the binned grid gradient (0: the atomic grid scatter)
   int grows;   // its count rows
@@ -2150,18 +2293,17 @@ static RenderWs render_ws(int64_t R, int S, int L, int log2_T, const int32_t* le
     w.off_gbase = align256(w.off_gcsum + gchunk * gbins * 4);
     w.off_gref = align256(w.off_gbase + ((int64_t)gbins + 1) * 4);
     w.off_gval = align256(w.off_gref + ns * 8 * 8);  // a run reaches at most 8 tiles
-    end = align256(w.off_gval + ns * 32 * 4);
+    w.off_gedge = align256(w.off_gval + ns * 32 * 4);
+    end = align256(w.off_gedge + (int64_t)w.grows * kGridRowChunks * 8);
   } else {
-    w.off_gcsum = w.off_gbase = w.off_gref = w.off_gval = end;
+    w.off_gcsum = w.off_gbase = w.off_gref = w.off_gval = w.off_gedge = end;
   }
   w.total = end;
   return w;
 }
 
 static RenderWs render_ws(const vxn_render_args& A) {
-  // the grid scatter on its own stream (scatter_stream) stays atomic: it runs beside the
-  // hash-table gradient and the caller's regularisers
-  const int gbins = A.grid_binned && !A.scatter_stream ? grid_bins(A.X, A.Y, A.Z) : 0;
+  const int gbins = A.grid_binned ? grid_bins(A.X, A.Y, A.Z) : 0;
   return render_ws(A.n_rays, A.n_samples, A.n_levels, A.log2_T, A.level_res,
                    A.view_embed == VXN_VIEW_PE ? MlpLayout<27>::NP : MlpLayout<16>::NP, gbins);
 }
@@ -2289,9 +2431,53 @@ static int launch_render_nv(const vxn_render_args* p, void* stream) {
   }
   const int nb = render_blocks(p->n_rays);
   hipEvent_t const* ev = (hipEvent_t const*)p->kernel_events;
+  const int64_t ns = p->n_rays * (int64_t)p->n_samples;
+  // second stream: the grid gradient follows the render kernel there, beside the
+  // hash-table gradient's passes
+  hipStream_t ss = MODE != kFwd ? (hipStream_t)p->scatter_stream : nullptr;
+  hipEvent_t const* se = (hipEvent_t const*)p->scatter_events;
+  RenderWs w{};
+  BinDev b{};
+  bool binned_hash = false;
+  int dl = 0;
+  if (MODE != kFwd) {
+    w = render_ws(*p);
+    binned_hash = w.bins.use && ((uintptr_t)p->grad_table & 7) == 0;
+    dl = dedup_levels(a.h);
+    if (binned_hash) {
+      char* ws = (char*)p->workspace;
+      b.nbl = w.bins.nbl;
+      b.nbins = w.bins.nbins;
+      b.ntile = w.bins.ntile;
+      b.nrows = w.bins.nrows;
+      b.nchunk = w.bins.nchunk;
+      b.cnt = (int32_t*)(ws + w.off_cnt);
+      b.csum = (int32_t*)(ws + w.off_csum);
+      b.base = (int32_t*)(ws + w.off_base);
+      b.ridx = (uint32_t*)(ws + w.off_ridx);
+      b.rval = (float4*)(ws + w.off_rval);
+      b.fact_level = dl;
+    }
+  }
+  // pass 1 and 2 of the binned hash-table gradient: record counts per tile and bin, then
+  // every tile's first position in each bin
+  auto count_and_scan = [&](hipStream_t st) -> int {
+    const unsigned nwg = (unsigned)(a.h.L * b.ntile);
+    hipLaunchKernelGGL(hash_bin_count_kernel, dim3(nwg), dim3(256), (size_t)b.nbl * sizeof(int32_t), st, a, b, dl);
+    VXN_LAUNCH_CHECK();
+    const dim3 cgrid((unsigned)((b.nbl + 255) / 256), (unsigned)b.nchunk, (unsigned)a.h.L);
+    hipLaunchKernelGGL(bin_col_sums_kernel, cgrid, dim3(256), 0, st, b);
+    VXN_LAUNCH_CHECK();
+    hipLaunchKernelGGL(bin_col_scan_kernel, dim3((unsigned)((b.nbins + 255) / 256)), dim3(256), 0, st, b);
+    VXN_LAUNCH_CHECK();
+    hipLaunchKernelGGL(bin_base_scan_kernel, dim3(1), dim3(kColThreads), 0, st, b);
+    VXN_LAUNCH_CHECK();
+    hipLaunchKernelGGL(bin_col_positions_kernel, cgrid, dim3(256), 0, st, b);
+    VXN_LAUNCH_CHECK();
+    return 0;
+  };
   if (ev) VXN_HIP(hipEventRecord(ev[0], s));
   if (MODE != kFwd && p->mlp_mode == VXN_MLP_BF16_MFMA) {
-    const int64_t ns = p->n_rays * (int64_t)p->n_samples;
     const int64_t chunks = (ns + 255) / 256;
     const unsigned nb_hf = (unsigned)(8 * ((a.h.L + 7) / 8) * chunks);
     hipLaunchKernelGGL(hash_features_kernel<0>, dim3(nb_hf), dim3(256), 0, s, a, chunks);
@@ -2304,7 +2490,7 @@ static int launch_render_nv(const vxn_render_args* p, void* stream) {
   }
   // the binned grid gradient writes grad_grid with plain read-add-writes: work that the
   // caller forks from fork_event (atomic adds into grad_grid) starts after it
-  const int gbins = MODE != kFwd ? render_ws(*p).gbins : 0;
+  const int gbins = w.gbins;
   if (p->fork_event && gbins == 0) VXN_HIP(hipEventRecord((hipEvent_t)p->fork_event, s));
   if (p->mlp_mode == VXN_MLP_BF16_MFMA)
     hipLaunchKernelGGL((render_kernel<MODE, kMlpBf16, 0, NV>), dim3(nb), dim3(kRT), 0, s, a);
@@ -2313,12 +2499,15 @@ static int launch_render_nv(const vxn_render_args* p, void* stream) {
   VXN_LAUNCH_CHECK();
   if (ev) VXN_HIP(hipEventRecord(ev[2], s));
   if (MODE != kFwd) {
-    const int64_t ns = p->n_rays * (int64_t)p->n_samples;
     const int sb = (int)((ns + 255) / 256);
-    const RenderWs w = render_ws(*p);
-    hipStream_t ss = (hipStream_t)p->scatter_stream;
-    hipEvent_t const* se = (hipEvent_t const*)p->scatter_events;
-    if (w.gbins > 0) {
+    // the grid gradient needs only the render kernel's per-sample geometry gradients
+    hipStream_t gs = s;
+    if (ss) {
+      VXN_HIP(hipEventRecord(se[0], s));
+      VXN_HIP(hipStreamWaitEvent(ss, se[0], 0));
+      gs = ss;
+    }
+    if (gbins > 0) {
       char* ws = (char*)p->workspace;
       GridBinDev gb;
       gb.tx = (a.g.X + kGtX - 1) / kGtX;
@@ -2330,6 +2519,7 @@ static int launch_render_nv(const vxn_render_args* p, void* stream) {
       gb.base = (int32_t*)(ws + w.off_gbase);
       gb.ref = (uint2*)(ws + w.off_gref);
       gb.val = (float*)(ws + w.off_gval);
+      gb.edge = (uint64_t*)(ws + w.off_gedge);
       // the column scans of the hash gradient on the tile count rows (one "level")
       BinDev cb = {};
       cb.nbl = w.gbins;
@@ -2340,66 +2530,35 @@ static int launch_render_nv(const vxn_render_args* p, void* stream) {
       cb.csum = (int32_t*)(ws + w.off_gcsum);
       cb.base = gb.base;
       const unsigned gwg = (unsigned)((w.grows + 3) / 4);
-      hipLaunchKernelGGL(grid_bin_count_kernel, dim3(gwg), dim3(256), (size_t)4 * w.gbins * sizeof(int32_t), s, a,
+      hipLaunchKernelGGL(grid_bin_count_kernel, dim3(gwg), dim3(256), (size_t)4 * w.gbins * sizeof(int32_t), gs, a,
                          gb);
       VXN_LAUNCH_CHECK();
       const dim3 cgrid((unsigned)((cb.nbl + 255) / 256), (unsigned)cb.nchunk, 1u);
-      hipLaunchKernelGGL(bin_col_sums_kernel, cgrid, dim3(256), 0, s, cb);
+      hipLaunchKernelGGL(bin_col_sums_kernel, cgrid, dim3(256), 0, gs, cb);
       VXN_LAUNCH_CHECK();
-      hipLaunchKernelGGL(bin_col_scan_kernel, dim3((unsigned)((cb.nbins + 255) / 256)), dim3(256), 0, s, cb);
+      hipLaunchKernelGGL(bin_col_scan_kernel, dim3((unsigned)((cb.nbins + 255) / 256)), dim3(256), 0, gs, cb);
       VXN_LAUNCH_CHECK();
-      hipLaunchKernelGGL(bin_base_scan_kernel, dim3(1), dim3(kColThreads), 0, s, cb);
+      hipLaunchKernelGGL(bin_base_scan_kernel, dim3(1), dim3(kColThreads), 0, gs, cb);
       VXN_LAUNCH_CHECK();
-      hipLaunchKernelGGL(bin_col_positions_kernel, cgrid, dim3(256), 0, s, cb);
+      hipLaunchKernelGGL(bin_col_positions_kernel, cgrid, dim3(256), 0, gs, cb);
       VXN_LAUNCH_CHECK();
-      hipLaunchKernelGGL(grid_bin_write_kernel, dim3(gwg), dim3(256), grid_write_lds(w.gbins), s, a, gb);
+      hipLaunchKernelGGL(grid_bin_write_kernel, dim3(gwg), dim3(256), grid_write_lds(w.gbins), gs, a, gb);
       VXN_LAUNCH_CHECK();
-      hipLaunchKernelGGL(grid_bin_sum_kernel, dim3((unsigned)(4 * w.gbins)), dim3(64 * kGsumWaves), kGsumLds, s, gb,
+      hipLaunchKernelGGL(grid_bin_sum_kernel, dim3((unsigned)(4 * w.gbins)), dim3(64 * kGsumWaves), kGsumLds, gs, gb,
                          a.g, a.grad_grid);
       VXN_LAUNCH_CHECK();
-      if (p->fork_event) VXN_HIP(hipEventRecord((hipEvent_t)p->fork_event, s));
-      if (ev) VXN_HIP(hipEventRecord(ev[3], s));
-    } else if (ss) {
-      // the atomic grid scatter needs only the render kernel's per-sample geometry
-      // gradients: on a second stream it runs beside the hash-table gradient
-      VXN_HIP(hipEventRecord(se[0], s));
-      VXN_HIP(hipStreamWaitEvent(ss, se[0], 0));
-      hipLaunchKernelGGL(grid_scatter_kernel, dim3(sb), dim3(256), 0, ss, a);
+      hipLaunchKernelGGL(grid_edge_kernel, dim3(1), dim3(64), 0, gs, a, (const uint64_t*)gb.edge,
+                         (int64_t)w.grows * kGridRowChunks);
       VXN_LAUNCH_CHECK();
-      if (ev) VXN_HIP(hipEventRecord(ev[3], ss));
-      VXN_HIP(hipEventRecord(se[1], ss));
+      if (p->fork_event) VXN_HIP(hipEventRecord((hipEvent_t)p->fork_event, gs));
     } else {
-      hipLaunchKernelGGL(grid_scatter_kernel, dim3(sb), dim3(256), 0, s, a);
+      hipLaunchKernelGGL(grid_scatter_kernel, dim3(sb), dim3(256), 0, gs, a);
       VXN_LAUNCH_CHECK();
-      if (ev) VXN_HIP(hipEventRecord(ev[3], s));
     }
-    if (w.bins.use && ((uintptr_t)p->grad_table & 7) == 0) {
-      char* ws = (char*)p->workspace;
-      BinDev b;
-      b.nbl = w.bins.nbl;
-      b.nbins = w.bins.nbins;
-      b.ntile = w.bins.ntile;
-      b.nrows = w.bins.nrows;
-      b.nchunk = w.bins.nchunk;
-      b.cnt = (int32_t*)(ws + w.off_cnt);
-      b.csum = (int32_t*)(ws + w.off_csum);
-      b.base = (int32_t*)(ws + w.off_base);
-      b.ridx = (uint32_t*)(ws + w.off_ridx);
-      b.rval = (float4*)(ws + w.off_rval);
-      const int dl = dedup_levels(a.h);
-      b.fact_level = dl;
-      const unsigned nwg = (unsigned)(a.h.L * b.ntile);
-      hipLaunchKernelGGL(hash_bin_count_kernel, dim3(nwg), dim3(256), (size_t)b.nbl * sizeof(int32_t), s, a, b, dl);
-      VXN_LAUNCH_CHECK();
-      const dim3 cgrid((unsigned)((b.nbl + 255) / 256), (unsigned)b.nchunk, (unsigned)a.h.L);
-      hipLaunchKernelGGL(bin_col_sums_kernel, cgrid, dim3(256), 0, s, b);
-      VXN_LAUNCH_CHECK();
-      hipLaunchKernelGGL(bin_col_scan_kernel, dim3((unsigned)((b.nbins + 255) / 256)), dim3(256), 0, s, b);
-      VXN_LAUNCH_CHECK();
-      hipLaunchKernelGGL(bin_base_scan_kernel, dim3(1), dim3(kColThreads), 0, s, b);
-      VXN_LAUNCH_CHECK();
-      hipLaunchKernelGGL(bin_col_positions_kernel, cgrid, dim3(256), 0, s, b);
-      VXN_LAUNCH_CHECK();
+    if (ev) VXN_HIP(hipEventRecord(ev[3], gs));
+    if (ss) VXN_HIP(hipEventRecord(se[1], ss));
+    if (binned_hash) {
+      if (int e = count_and_scan(s)) return e;
       // levels below dl carry run sums (record values + entry array), the rest factored records
       if (dl > 0) {
         hipLaunchKernelGGL(hash_bin_write_kernel<false>, dim3((unsigned)(dl * b.ntile)), dim3(256),
@@ -2422,7 +2581,7 @@ static int launch_render_nv(const vxn_render_args* p, void* stream) {
         hipLaunchKernelGGL(zero_f32_kernel, dim3(nz), dim3(256), 0, s, p->grad_table, nf);
         VXN_LAUNCH_CHECK();
       }
-      hipLaunchKernelGGL(hash_scatter_kernel, dim3(sb), dim3(256), 0, s, a, dedup_levels(a.h));
+      hipLaunchKernelGGL(hash_scatter_kernel, dim3(sb), dim3(256), 0, s, a, dl);
       VXN_LAUNCH_CHECK();
     }
     if (ev) VXN_HIP(hipEventRecord(ev[4], s));

@@ -96,14 +96,15 @@ class StepConfig:
     # start them just before the render kernel (else right after the sampler, beside the
     # hash-feature gather; measured slower: the render kernel loses work-group slots to them)
     fork_after_features: bool = True
-    # run the grid scatter on a third stream, beside the hash-table gradient (off: its
-    # atomics slow the binned passes by more than the overlap saves, 2.50 -> 2.48 ms BF16
-    # and 3.29 -> 3.45 ms FP32 on the 8192-ray step)
-    concurrent_scatter: bool = False
+    # run the grid gradient on a third stream, beside the hash-table gradient (8192-ray
+    # step: BF16 form 1.77 -> 1.73 ms with the atomic scatter, 1.68 ms binned; FP32 form
+    # 2.57 -> 2.55 ms atomic)
+    concurrent_scatter: bool = True
     # the grid gradient voxel-binned (vxn_render_args.grid_binned: the same on every run, one
-    # plain read-add-write per vertex) instead of the atomic scatter; measured 0.25 ms
-    # against 0.21 ms on the 8192-ray step, so off by default
-    binned_grid_gradient: bool = False
+    # plain read-add-write per vertex) instead of the atomic scatter.  None: binned for the
+    # BF16 form, atomic for the FP32 form, the faster of the two for each beside the
+    # hash-table gradient (FP32: 2.73 ms binned against 2.55 ms atomic)
+    binned_grid_gradient: Optional[bool] = None
     # max-abs of the weighted eikonal gradient during a post-upsample cooldown
     # (trainer.py:118, scheduler.eik_grad_clamp)
     eik_grad_clamp: float = 0.1
@@ -192,14 +193,14 @@ class FusedTrainStep:
         cfg = self.cfg
         R, S = mid.shape
         key = (R, S, mid.device, self.grid.data_ptr(), self.table.data_ptr(), self.mlp.data_ptr(),
-               self.variance.data_ptr(), cfg.binned_grid_gradient)
+               self.variance.data_ptr(), self._grid_binned())
         a = self._args
         if a is None or self._args_key != key:
             rc = Fn.RenderConfig(self.box, self.level_res, self.level_w, cfg.sphere_radius, cfg.cos_anneal_ratio,
                                  cfg.use_iter_cos, mlp_mode=cfg.mlp_mode,
                                  gradient_mode=N.gradient_mode_code(cfg.gradient_mode),
                                  view_embed=N.view_embed_code(cfg.view_embed_type),
-                                 grid_binned=cfg.binned_grid_gradient)
+                                 grid_binned=self._grid_binned())
             a = N.RenderArgs()
             Fn.fill_render_args(a, rc, self.grid, self.table, self.mlp, 0.0, rays_o, rays_d, mid, dists)
             a.variance = N.dptr(self.variance, torch.float32, "variance")
@@ -239,6 +240,10 @@ class FusedTrainStep:
         a.kernel_events = (ctypes.cast(self.kernel_events.array, ctypes.c_void_p)
                            if self.kernel_events is not None else None)
         return a
+
+    def _grid_binned(self) -> bool:
+        b = self.cfg.binned_grid_gradient
+        return self.cfg.mlp_mode == N.VXN_MLP_BF16_MFMA if b is None else bool(b)
 
     def sample(self, rays_o, rays_d, t_rand=None, u_rand=None, vr_bitmask=None):
         cfg = self.cfg
