@@ -196,6 +196,7 @@ static GridDesc coords_desc(const float* grid, int X, int Y, int Z, const float*
   for (int a = 0; a < 3; ++a) {
     g.mn[a] = 0.0f;
     g.den[a] = 1.0f;
+    g.rden[a] = 1.0f;
     g.rm1[a] = (float)res[a] - 1.0f;
     g.bound[a] = (float)((double)res[a] - 1.0001);
     g.eps[a] = eps3[a];

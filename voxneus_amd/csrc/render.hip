@@ -77,6 +77,7 @@ struct RenderDev {
   const float *o, *d, *mid, *dist;
   int64_t R;
   int S;
+  vxn_fastdiv sdiv;  // sample / S (R * S < 2^31)
   GridDesc g;
   HashDesc h;
   const float* mlp;
@@ -287,9 +288,9 @@ __global__ void __launch_bounds__(kRT, 2) render_kernel(RenderDev a) {
     }
     const float gr = norm3(gv);
     const float nrm[3] = {gv[0] / (gr + 1e-8f), gv[1] / (gr + 1e-8f), gv[2] / (gr + 1e-8f)};
-    const float x0 = fminf(fmaxf((px - G.mn[0]) / G.den[0], 0.0f), 1.0f);
-    const float x1 = fminf(fmaxf((py - G.mn[1]) / G.den[1], 0.0f), 1.0f);
-    const float x2 = fminf(fmaxf((pz - G.mn[2]) / G.den[2], 0.0f), 1.0f);
+    const float x0 = fminf(fmaxf(over_den(G, 0, px - G.mn[0]), 0.0f), 1.0f);
+    const float x1 = fminf(fmaxf(over_den(G, 1, py - G.mn[1]), 0.0f), 1.0f);
+    const float x2 = fminf(fmaxf(over_den(G, 2, pz - G.mn[2]), 0.0f), 1.0f);
     float x[35];  // 32 hash features, then the unit normal (MLP input columns 0-31, 48-50)
     if constexpr (!kPre) {
 #pragma unroll
@@ -864,8 +865,10 @@ VXN_DEV bool seg_sum(uint64_t key, float (&v)[NV], int lane) {
 }
 
 // Sample position of record i (ray = i / S).
+VXN_DEV int64_t ray_of(const RenderDev& a, int64_t i) { return (int64_t)vxn_fastdiv_apply(a.sdiv, (uint32_t)i); }
+
 VXN_DEV void sample_pos(const RenderDev& a, int64_t i, float& px, float& py, float& pz) {
-  const int64_t ray = i / a.S;
+  const int64_t ray = ray_of(a, i);
   const float m = a.mid[i];
   px = a.o[3 * ray] + a.d[3 * ray] * m;
   py = a.o[3 * ray + 1] + a.d[3 * ray + 1] * m;
@@ -889,7 +892,7 @@ __global__ void __launch_bounds__(256) hash_features_kernel(RenderDev a, int64_t
   const GridDesc* g = &a.g;
   const HashDesc* h = &a.h;
   if constexpr (MULTI) {  // serving batch: the ray's scene, rays past the batch size skipped
-    const int64_t ray = i / a.S;
+    const int64_t ray = ray_of(a, i);
     if (ray >= min((int64_t)max(*a.n_rays_dev, 0), a.R)) return;
     const int si = a.ray_scene[ray];
     if (si < 0 || si >= a.n_scenes) return;
@@ -900,9 +903,9 @@ __global__ void __launch_bounds__(256) hash_features_kernel(RenderDev a, int64_t
   }
   float px, py, pz;
   sample_pos(a, i, px, py, pz);
-  const float x0 = fminf(fmaxf((px - g->mn[0]) / g->den[0], 0.0f), 1.0f);
-  const float x1 = fminf(fmaxf((py - g->mn[1]) / g->den[1], 0.0f), 1.0f);
-  const float x2 = fminf(fmaxf((pz - g->mn[2]) / g->den[2], 0.0f), 1.0f);
+  const float x0 = fminf(fmaxf(over_den(*g, 0, px - g->mn[0]), 0.0f), 1.0f);
+  const float x1 = fminf(fmaxf(over_den(*g, 1, py - g->mn[1]), 0.0f), 1.0f);
+  const float x2 = fminf(fmaxf(over_den(*g, 2, pz - g->mn[2]), 0.0f), 1.0f);
   float f0, f1;
   hash_level_fwd<true>(*h, l, x0, x1, x2, f0, f1);
   ((uint32_t*)a.hfeat)[l * NS + i] = pk_bf16(f0, f1);
@@ -919,9 +922,9 @@ __global__ void __launch_bounds__(256) hash_scatter_kernel(RenderDev a, int dedu
   if (in) {
     float px, py, pz;
     sample_pos(a, i, px, py, pz);
-    x0 = fminf(fmaxf((px - a.g.mn[0]) / a.g.den[0], 0.0f), 1.0f);
-    x1 = fminf(fmaxf((py - a.g.mn[1]) / a.g.den[1], 0.0f), 1.0f);
-    x2 = fminf(fmaxf((pz - a.g.mn[2]) / a.g.den[2], 0.0f), 1.0f);
+    x0 = fminf(fmaxf(over_den(a.g, 0, px - a.g.mn[0]), 0.0f), 1.0f);
+    x1 = fminf(fmaxf(over_den(a.g, 1, py - a.g.mn[1]), 0.0f), 1.0f);
+    x2 = fminf(fmaxf(over_den(a.g, 2, pz - a.g.mn[2]), 0.0f), 1.0f);
   }
   for (int l = 0; l < a.h.L; ++l) {
     float d0 = in ? a.gfeat[(2 * l) * NS + i] : 0.0f;
@@ -1042,7 +1045,7 @@ VXN_DEV LevelIn level_in(const RenderDev& a, int64_t i, int l) {
   const int64_t NS = a.R * (int64_t)a.S;
   LevelIn x{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
   if (i < NS) {
-    const int64_t ray = i / a.S;
+    const int64_t ray = ray_of(a, i);
     x.d0 = a.gfeat[(2 * l) * NS + i];
     x.d1 = a.gfeat[(2 * l + 1) * NS + i];
     x.m = a.mid[i];
@@ -1076,9 +1079,9 @@ VXN_DEV bool hash_level_eval(const RenderDev& a, int64_t i, const LevelIn& x, in
   if (in) {
     // sample_pos's arithmetic on the loaded ray
     const float px = x.ox + x.rx * x.m, py = x.oy + x.ry * x.m, pz = x.oz + x.rz * x.m;
-    x0 = fminf(fmaxf((px - a.g.mn[0]) / a.g.den[0], 0.0f), 1.0f);
-    x1 = fminf(fmaxf((py - a.g.mn[1]) / a.g.den[1], 0.0f), 1.0f);
-    x2 = fminf(fmaxf((pz - a.g.mn[2]) / a.g.den[2], 0.0f), 1.0f);
+    x0 = fminf(fmaxf(over_den(a.g, 0, px - a.g.mn[0]), 0.0f), 1.0f);
+    x1 = fminf(fmaxf(over_den(a.g, 1, py - a.g.mn[1]), 0.0f), 1.0f);
+    x2 = fminf(fmaxf(over_den(a.g, 2, pz - a.g.mn[2]), 0.0f), 1.0f);
   }
   c = hash_cell(x0, x1, x2, a.h.res[l], a.h.mask);
   if constexpr (VALUES) {
@@ -2321,7 +2324,8 @@ static int build_render_dev(const vxn_render_args& A, RenderDev& a, int mode) {
   VXN_REQUIRE(A.view_embed == VXN_VIEW_SH || A.view_embed == VXN_VIEW_PE, VXN_ERR_VALUE, "unknown view_embed %d",
               A.view_embed);
   VXN_REQUIRE(A.rays_o && A.rays_d && A.mid_z && A.dists && A.table && A.mlp, VXN_ERR_NULL, "null input");
-  VXN_REQUIRE(A.n_rays * (int64_t)A.n_samples < ((int64_t)1 << 40), VXN_ERR_SHAPE, "too many samples");
+  VXN_REQUIRE(A.n_rays * (int64_t)A.n_samples < ((int64_t)1 << 31), VXN_ERR_SHAPE,
+              "n_rays * n_samples must be below 2^31 (got %lld)", (long long)(A.n_rays * (int64_t)A.n_samples));
   HashDesc h;
   if (int e = make_hash_desc(h, A.table, A.table_dtype, A.n_levels, A.log2_T, A.n_features, A.level_res, A.level_w))
     return e;
@@ -2331,6 +2335,7 @@ static int build_render_dev(const vxn_render_args& A, RenderDev& a, int mode) {
   a.dist = A.dists;
   a.R = A.n_rays;
   a.S = A.n_samples;
+  a.sdiv = vxn_fastdiv_make((uint32_t)A.n_samples);
   a.g = make_grid_desc(A.grid, A.X, A.Y, A.Z, A.box);
   a.g.analytical = A.gradient_mode;
   a.h = h;
@@ -2665,6 +2670,7 @@ static int serve_shape(const vxn_serve_render_args& A, int& steps, int& per, int
   per = steps ? (A.n_importance / steps > 1 ? A.n_importance / steps : 1) : 0;
   S = A.n_samples + steps * per;
   VXN_REQUIRE(S <= kRT, VXN_ERR_UNSUPPORTED, "at most %d samples per ray are supported (got %d)", kRT, S);
+  VXN_REQUIRE(A.capacity * (int64_t)S < ((int64_t)1 << 31), VXN_ERR_SHAPE, "capacity * samples must be below 2^31");
   VXN_REQUIRE(per <= 64 && steps <= 20, VXN_ERR_UNSUPPORTED, "at most 64 samples per step and 20 steps");
   return 0;
 }
@@ -2711,6 +2717,7 @@ extern "C" int vxn_serve_render(const vxn_serve_render_args* p, void* stream) {
   a.dist = dists;
   a.R = A.capacity;
   a.S = S;
+  a.sdiv = vxn_fastdiv_make((uint32_t)S);
   a.cos_anneal = A.cos_anneal_ratio;
   a.iter_cos = A.use_iter_cos;
   a.rgb = A.rgb;

@@ -11,6 +11,9 @@
 #include <stdint.h>
 
 #define VXN_DEV __device__ __forceinline__
+#define VXN_HD __host__ __device__
+
+#include "vxn_fastdiv.h"
 
 namespace vxn {
 
@@ -22,6 +25,7 @@ struct GridDesc {
   int X, Y, Z;
   float mn[3];     // xyz_min
   float den[3];    // (xyz_max - xyz_min) + 1e-8f           (sdf_grid.py:106, voxneus.py:144)
+  float rden[3];   // RN(1 / den): the divisions by den are div_rn (correctly rounded)
   float rm1[3];    // float(res) - 1.0f                      (coordinates.py:28)
   float bound[3];  // float32(res - 1.0001)                  (coordinates.py:73)
   float eps[3];    // (xyz_max - xyz_min) / (res - 1)        (sdf_grid.py:83-88)
@@ -31,6 +35,28 @@ struct GridDesc {
 VXN_DEV int64_t lin3(const GridDesc& g, int x, int y, int z) {
   return ((int64_t)x * g.Y + y) * g.Z + z;
 }
+
+// Correctly rounded a / b from y = RN(1 / b) (b > 0), by Markstein's residual correction
+// (P. Markstein, "IA-64 and Elementary Functions", 2000, ch. 8; Muller et al., "Handbook
+// of Floating-Point Arithmetic", the fma-based division theorems): with y correctly
+// rounded and q within one ulp of a / b, the residual e = a - b q is exact in one fma and
+// RN(q + e y) = RN(a / b).  q = RN(a y) is within 1.5 ulp, so one correction first brings
+// it within one ulp and a second rounds correctly (an exact q gets zero residuals and is
+// kept).  Without branches: the quotient takes the sign of a y (the division's sign,
+// signed zeros included), and an infinite a y is the quotient of an infinite a (a NaN a
+// stays NaN).  Quotients in the subnormal range may differ from the division in the last
+// place.
+VXN_DEV float div_rn(float a, float b, float y) {
+  const float q0 = a * y;
+  const float e0 = fmaf(-q0, b, a);
+  const float q1 = fmaf(e0, y, q0);
+  const float e1 = fmaf(-q1, b, a);
+  const float q2 = copysignf(fmaf(e1, y, q1), q0);
+  return __builtin_isinf(q0) ? q0 : q2;
+}
+
+// (p - mn) / den of one axis, correctly rounded (div_rn with y = RN(1 / den)).
+VXN_DEV float over_den(const GridDesc& g, int a, float v) { return div_rn(v, g.den[a], g.rden[a]); }
 
 // ------------------------------------------------------------------------------------
 // exp and sigmoid exactly as PyTorch's CPU float kernels evaluate them: the vectorised
@@ -93,17 +119,17 @@ struct GsStencil {
   unsigned valid;   // bit k: corner k inside the grid
 };
 
-VXN_DEV float gs_unnorm(float p, float mn, float den, int size) {
-  float n = (p - mn) / den;
+VXN_DEV float gs_unnorm(float p, float mn, float den, float rden, int size) {
+  float n = div_rn(p - mn, den, rden);
   n = n * 2.0f - 1.0f;
   float c = ((n + 1.0f) / 2.0f) * (float)(size - 1);
   return fminf((float)(size - 1), fmaxf(c, 0.0f));
 }
 
 VXN_DEV GsStencil gs_stencil(const GridDesc& g, float px, float py, float pz) {
-  const float cx = gs_unnorm(px, g.mn[0], g.den[0], g.X);
-  const float cy = gs_unnorm(py, g.mn[1], g.den[1], g.Y);
-  const float cz = gs_unnorm(pz, g.mn[2], g.den[2], g.Z);
+  const float cx = gs_unnorm(px, g.mn[0], g.den[0], g.rden[0], g.X);
+  const float cy = gs_unnorm(py, g.mn[1], g.den[1], g.rden[1], g.Y);
+  const float cz = gs_unnorm(pz, g.mn[2], g.den[2], g.rden[2], g.Z);
   const float fx = floorf(cx), fy = floorf(cy), fz = floorf(cz);
   const float gx = fx + 1.0f, gy = fy + 1.0f, gz = fz + 1.0f;
   GsStencil s;
@@ -174,7 +200,7 @@ VXN_DEV Corners corner_decomp(const GridDesc& g, float px, float py, float pz) {
   Corners c;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    const float coord = ((p[a] - g.mn[a]) / g.den[a]) * g.rm1[a];
+    const float coord = over_den(g, a, p[a] - g.mn[a]) * g.rm1[a];
     const float cc = fminf(fmaxf(coord, 0.0f), g.bound[a]);
     int i0 = (int)floorf(cc);
     int i1 = i0 + 1;
@@ -218,24 +244,6 @@ VXN_DEV void fd_axis(int i, int res, float eps, float reps, int& ip, int& im, fl
   y = (ip - im == 2) ? 0.5f * reps : reps;
 }
 
-// Correctly rounded a / b from y = RN(1 / b) (b > 0), by Markstein's residual correction
-// (P. Markstein, "IA-64 and Elementary Functions", 2000, ch. 8; Muller et al., "Handbook
-// of Floating-Point Arithmetic", the fma-based division theorems): with y correctly
-// rounded and q within one ulp of a / b, the residual e = a - b q is exact in one fma and
-// RN(q + e y) = RN(a / b).  q = RN(a y) is within 1.5 ulp, so one correction first brings
-// it within one ulp and a second rounds correctly (an exact q gets zero residuals and is
-// kept).  Without branches: the quotient takes the sign of a y (the division's sign,
-// signed zeros included), and an infinite a y is the quotient of an infinite a (a NaN a
-// stays NaN).  Quotients in the subnormal range may differ from the division in the last
-// place.
-VXN_DEV float div_rn(float a, float b, float y) {
-  const float q0 = a * y;
-  const float e0 = fmaf(-q0, b, a);
-  const float q1 = fmaf(e0, y, q0);
-  const float e1 = fmaf(-q1, b, a);
-  const float q2 = copysignf(fmaf(e1, y, q1), q0);
-  return __builtin_isinf(q0) ? q0 : q2;
-}
 
 // Eq. 7: grad = sum_k w_k n_k, summed in PyTorch's CPU order for an 8-long reduction
 // over a strided dimension: ((t0 + t4) + (t1 + t5)) + (t2 + t6)) + (t3 + t7).

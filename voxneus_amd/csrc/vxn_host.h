@@ -53,6 +53,7 @@ inline GridDesc make_grid_desc(const float* grid, int X, int Y, int Z, const flo
     const float mn = box6[a], mx = box6[3 + a];
     g.mn[a] = mn;
     g.den[a] = (mx - mn) + 1e-8f;
+    g.rden[a] = 1.0f / g.den[a];
     g.rm1[a] = (float)res[a] - 1.0f;
     g.bound[a] = (float)((double)res[a] - 1.0001);
     g.eps[a] = (mx - mn) / ((float)res[a] - 1.0f);

@@ -14,6 +14,7 @@ struct OccDesc {
   int res;
   float mn[3];
   float den[3];  // (occ_max - occ_min) + 1e-8f
+  float rden[3]; // RN(1 / den), for div_rn
   float bound;   // float32(res - 1.0001)
 };
 
@@ -39,6 +40,7 @@ inline OccDesc make_occ_desc(const uint8_t* occ, int res, const float* box6) {
   for (int k = 0; k < 3; ++k) {
     o.mn[k] = box6[k];
     o.den[k] = (box6[3 + k] - box6[k]) + 1e-8f;
+    o.rden[k] = 1.0f / o.den[k];
   }
   o.bound = (float)((double)res - 1.0001);
   return o;
@@ -51,7 +53,7 @@ VXN_DEV float occ_factor(const OccDesc& o, float px, float py, float pz) {
   int c[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    const float nrm = (p[k] - o.mn[k]) / o.den[k];
+    const float nrm = div_rn(p[k] - o.mn[k], o.den[k], o.rden[k]);
     inb = inb && (nrm >= 0.0f) && (nrm <= 1.0f);
     const float v = fminf(fmaxf(nrm * (float)o.res, 0.0f), o.bound);
     c[k] = (int)v;
