@@ -110,6 +110,12 @@ struct RenderDev {
   // BF16 backward / training: hash features gathered beforehand by hash_features_kernel,
   // [16][R*S] bf16 pairs packed in uint32
   void* hfeat;
+  // backward / training with the binned hash-table gradient: live[i] = 1 where sample i has
+  // a nonzero weight or alpha gradient (every other sample has zero gradients), then the
+  // live samples' indices in order (live_idx, n_live of them), which the record passes walk
+  uint8_t* live;
+  const int32_t* live_idx;
+  const int32_t* n_live;
   // serving batches (forward only): per-ray scene of a packed batch; R is the capacity
   // and the batch size is read from n_rays_dev
   const SceneDev* scenes;
@@ -538,6 +544,9 @@ __global__ void __launch_bounds__(kRT, 2) render_kernel(RenderDev a) {
     float dA = gw * T;
     if (ov <= 1.0f && ov >= 0.0f) dA -= (float)(suffix / (double)oma);
     const float dAraw = valid ? dA * inside : 0.0f;
+    // every gradient of the sample is a multiple of w (colour, normal) or of dAraw (SDF,
+    // cosine): zero both and the sample adds nothing
+    const bool glive = w != 0.0f || dAraw != 0.0f;
     const float dq = (q >= 0.0f && q <= 1.0f) ? dAraw : 0.0f;
     float dpp = dq / den;
     const float dpn = -dq / den;
@@ -733,6 +742,7 @@ __global__ void __launch_bounds__(kRT, 2) render_kernel(RenderDev a) {
     if (valid) {
       const float rr = gr + 1e-8f;
       const float dotv = (dn[0] * gv[0] + dn[1] * gv[1]) + dn[2] * gv[2];
+      if (a.live) a.live[si] = glive ? 1 : 0;
       a.ggeo[si] = dsdf;
 #pragma unroll
       for (int k = 0; k < 3; ++k)
@@ -867,6 +877,15 @@ VXN_DEV bool seg_sum(uint64_t key, float (&v)[NV], int lane) {
 // Sample position of record i (ray = i / S).
 VXN_DEV int64_t ray_of(const RenderDev& a, int64_t i) { return (int64_t)vxn_fastdiv_apply(a.sdiv, (uint32_t)i); }
 
+// The record passes walk the live samples in order when the render listed them: position k
+// of the walk is sample live_idx[k], and past the walk's length the sample count R * S (out
+// of range: no records).  Without the list the walk is every sample.
+VXN_DEV int64_t walk_length(const RenderDev& a) { return a.live_idx ? (int64_t)*a.n_live : a.R * (int64_t)a.S; }
+VXN_DEV int64_t walk_sample(const RenderDev& a, int64_t k, int64_t nwalk) {
+  if (k >= nwalk) return a.R * (int64_t)a.S;
+  return a.live_idx ? (int64_t)a.live_idx[k] : k;
+}
+
 VXN_DEV void sample_pos(const RenderDev& a, int64_t i, float& px, float& py, float& pz) {
   const int64_t ray = ray_of(a, i);
   const float m = a.mid[i];
@@ -1000,6 +1019,88 @@ __global__ void __launch_bounds__(256) hash_scatter_kernel(RenderDev a, int dedu
 // only one wavefront touches, so the records' order in a bin, and the gradient, are the
 // same on every run.
 // ------------------------------------------------------------------------------------
+// The live samples' indices in order: per block of kLiveBlock samples its live count, then
+// each block's indices at the sum of the counts before it (the block adds them up itself;
+// the last block writes the total).  Flags are 0 or 1 bytes, so a 4-byte word's popcount
+// counts its live samples.
+constexpr int kLiveBlock = 4096;  // 256 threads x 16 samples
+
+VXN_DEV int live_count16(const uint8_t* __restrict__ live, int64_t s, int64_t NS, uint32_t (&w)[4]) {
+  if (s + 16 <= NS) {
+    const uint4 v = *reinterpret_cast<const uint4*>(live + s);
+    w[0] = v.x;
+    w[1] = v.y;
+    w[2] = v.z;
+    w[3] = v.w;
+  } else {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      uint32_t x = 0u;
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (s + 4 * q + j < NS) x |= (uint32_t)live[s + 4 * q + j] << (8 * j);
+      w[q] = x;
+    }
+  }
+  return __popc(w[0]) + __popc(w[1]) + __popc(w[2]) + __popc(w[3]);
+}
+
+// Exclusive scan of one value per thread over a 256-thread work-group; returns the total.
+VXN_DEV int block_scan256(int v, int& excl, int* ws4) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int inc = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int t = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += t;
+  }
+  if (lane == 63) ws4[wid] = inc;
+  __syncthreads();
+  int pre = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < 4; ++w) {
+    pre += w < wid ? ws4[w] : 0;
+    tot += ws4[w];
+  }
+  excl = pre + inc - v;
+  return tot;
+}
+
+__global__ void __launch_bounds__(256) live_count_kernel(const uint8_t* __restrict__ live, int64_t NS,
+                                                         int32_t* __restrict__ bcnt) {
+  __shared__ int ws4[4];
+  uint32_t w[4];
+  const int c = live_count16(live, (int64_t)blockIdx.x * kLiveBlock + threadIdx.x * 16, NS, w);
+  int excl;
+  const int tot = block_scan256(c, excl, ws4);
+  if (threadIdx.x == 0) bcnt[blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(256) live_list_kernel(const uint8_t* __restrict__ live, int64_t NS,
+                                                        const int32_t* __restrict__ bcnt, int32_t* __restrict__ idx,
+                                                        int32_t* __restrict__ n_live) {
+  __shared__ int ws4[4];
+  __shared__ int red[4];
+  const int b = blockIdx.x;
+  // this block's offset: the counts of the blocks before it
+  int part = 0;
+  for (int j = threadIdx.x; j < b; j += 256) part += bcnt[j];
+  int e0;
+  const int off = block_scan256(part, e0, red);
+  uint32_t w[4];
+  const int64_t s = (int64_t)b * kLiveBlock + threadIdx.x * 16;
+  const int c = live_count16(live, s, NS, w);
+  int excl;
+  const int tot = block_scan256(c, excl, ws4);
+  int pos = off + excl;
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      if ((w[q] >> (8 * j)) & 0xFFu) idx[pos++] = (int32_t)(s + 4 * q + j);
+  if (b == (int)gridDim.x - 1 && threadIdx.x == 0) *n_live = off + tot;
+}
+
 constexpr int kBinBits = 11;
 constexpr int kBinEntries = 1 << kBinBits;
 constexpr int kBinTileWaves = 4;            // wavefronts per work-group of the count / write passes
@@ -1137,8 +1238,10 @@ __global__ void __launch_bounds__(256) hash_bin_count_kernel(RenderDev a, BinDev
   for (int k = threadIdx.x; k < b.nbl; k += 256) cnt_lds[k] = 0;
   __syncthreads();
   const int64_t i0 = (int64_t)tile * kBinTile + (int64_t)wid * kBinWaveSamples;
+  const int64_t nw = walk_length(a);
   for (int c = 0; c < kBinWaveChunks; ++c) {
-    hash_level_records<false>(a, i0 + c * 64 + lane, lane, l, dedup_levels,
+    if (i0 + c * 64 >= nw) break;
+    hash_level_records<false>(a, walk_sample(a, i0 + c * 64 + lane, nw), lane, l, dedup_levels,
                               [&](bool tail, const HashCell& cell, const float(&)[16], float, float) {
       if (tail) {
 #pragma unroll
@@ -1185,10 +1288,18 @@ __global__ void __launch_bounds__(256) hash_bin_write_kernel(RenderDev a, BinDev
   const int per = (nbl + 255) / 256;  // bins per thread in the scan (consecutive)
   const int64_t i0 = (int64_t)tile * kBinTile + (int64_t)wid * kBinWaveSamples + lane;
   constexpr uint32_t m = kBinEntries - 1;
-  LevelIn nxt = level_in(a, i0, l);
+  // the walk's positions i0 + 64 c; a tile past the walk's end has no records
+  const int64_t nw = walk_length(a);
+  if ((int64_t)tile * kBinTile >= nw) return;
+  int64_t inxt = walk_sample(a, i0, nw);
+  LevelIn nxt = level_in(a, inxt, l);
   for (int c = 0; c < kBinWaveChunks; ++c) {
     const LevelIn cur = nxt;
-    if (c + 1 < kBinWaveChunks) nxt = level_in(a, i0 + (c + 1) * 64, l);
+    const int64_t icur = inxt;
+    if (c + 1 < kBinWaveChunks) {
+      inxt = walk_sample(a, i0 + (c + 1) * 64, nw);
+      nxt = level_in(a, inxt, l);
+    }
     for (int k = lane; k < nbl; k += 64) mine[k] = 0;
     wave_sync();
     // plain scalar arrays: the HIP vector types would keep these in scratch memory
@@ -1204,7 +1315,7 @@ __global__ void __launch_bounds__(256) hash_bin_write_kernel(RenderDev a, BinDev
     float v[16];
     float d0, d1;
     bool tail;
-    if (hash_level_eval<!FACT>(a, i0 + c * 64, cur, lane, l, dedup_levels, cell, v, d0, d1, tail) && tail) {
+    if (hash_level_eval<!FACT>(a, icur, cur, lane, l, dedup_levels, cell, v, d0, d1, tail) && tail) {
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const uint32_t ia = cell.idx[q], ib = cell.idx[q + 4];
@@ -2258,7 +2369,8 @@ static BinPlan bin_plan(int64_t ns, int L, int log2_T, const int32_t* level_res)
 // | grid tile count rows | their column chunk sums | tile bases | tile lists | bricks.
 struct RenderWs {
   int64_t off_sc, off_norm, off_rows, off_feat, off_geo, off_hf, off_cnt, off_csum, off_base, off_ridx, off_rval;
-  int64_t off_gcnt, off_gcsum, off_gbase, off_gref, off_gval, off_gedge, total;
+  int64_t off_gcnt, off_gcsum, off_gbase, off_gref, off_gval, off_gedge, off_live, off_lidx, off_lcnt, off_nlive,
+      total;
   BinPlan bins;
   int gbins;   // grid tiles of the binned grid gradient (0: the atomic grid scatter)
   int grows;   // its count rows
@@ -2287,6 +2399,16 @@ static RenderWs render_ws(int64_t R, int S, int L, int log2_T, const int32_t* le
     w.off_csum = w.off_base = w.off_ridx = w.off_rval = w.off_cnt;
     end = w.off_cnt;
   }
+  // the live-sample list of the record passes
+  if (w.bins.use) {
+    w.off_live = end;
+    w.off_lidx = align256(w.off_live + ns);
+    w.off_lcnt = align256(w.off_lidx + ns * 4);
+    w.off_nlive = align256(w.off_lcnt + ((ns + kLiveBlock - 1) / kLiveBlock) * 4);
+    end = align256(w.off_nlive + 4);
+  } else {
+    w.off_live = w.off_lidx = w.off_lcnt = w.off_nlive = end;
+  }
   w.gbins = gbins;
   w.grows = (int)((ns + kGridRowSamples - 1) / kGridRowSamples);
   w.off_gcnt = end;
@@ -2312,6 +2434,7 @@ static RenderWs render_ws(const vxn_render_args& A) {
 }
 
 static int build_render_dev(const vxn_render_args& A, RenderDev& a, int mode) {
+  a = RenderDev{};  // every optional pointer starts null
   if (int e = check_grid(A.grid, A.X, A.Y, A.Z, A.box)) return e;
   if (int e = check_gradient_mode(A.gradient_mode)) return e;
   VXN_REQUIRE(A.n_rays >= 0, VXN_ERR_SHAPE, "n_rays must be >= 0");
@@ -2396,6 +2519,7 @@ static int build_render_dev(const vxn_render_args& A, RenderDev& a, int mode) {
     a.gfeat = (float*)(ws + w.off_feat);
     a.ggeo = (float*)(ws + w.off_geo);
     a.hfeat = ws + w.off_hf;
+    if (w.bins.use) a.live = (uint8_t*)(ws + w.off_live);
   }
   if (mode == kTrain) {
     VXN_REQUIRE(A.target_rgb != nullptr, VXN_ERR_NULL, "target_rgb is null");
@@ -2422,7 +2546,7 @@ static int dedup_levels(const HashDesc& h) {
 template <int MODE, int NV>
 static int launch_render_nv(const vxn_render_args* p, void* stream) {
   VXN_REQUIRE(p != nullptr, VXN_ERR_NULL, "args is null");
-  RenderDev a;
+  RenderDev a{};
   if (int e = build_render_dev(*p, a, MODE)) return e;
   hipStream_t s = (hipStream_t)stream;
   if (p->n_rays == 0) {
@@ -2563,6 +2687,19 @@ static int launch_render_nv(const vxn_render_args* p, void* stream) {
     if (ev) VXN_HIP(hipEventRecord(ev[3], gs));
     if (ss) VXN_HIP(hipEventRecord(se[1], ss));
     if (binned_hash) {
+      // the live samples in order: the record passes walk only them
+      char* wsp = (char*)p->workspace;
+      const unsigned nlb = (unsigned)((ns + kLiveBlock - 1) / kLiveBlock);
+      int32_t* lcnt = (int32_t*)(wsp + w.off_lcnt);
+      int32_t* lidx = (int32_t*)(wsp + w.off_lidx);
+      int32_t* nlive = (int32_t*)(wsp + w.off_nlive);
+      hipLaunchKernelGGL(live_count_kernel, dim3(nlb), dim3(256), 0, s, (const uint8_t*)a.live, ns, lcnt);
+      VXN_LAUNCH_CHECK();
+      hipLaunchKernelGGL(live_list_kernel, dim3(nlb), dim3(256), 0, s, (const uint8_t*)a.live, ns,
+                         (const int32_t*)lcnt, lidx, nlive);
+      VXN_LAUNCH_CHECK();
+      a.live_idx = lidx;
+      a.n_live = nlive;
       if (int e = count_and_scan(s)) return e;
       // levels below dl carry run sums (record values + entry array), the rest factored records
       if (dl > 0) {
