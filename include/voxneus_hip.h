@@ -222,7 +222,7 @@ typedef struct vxn_render_args {
   float sphere_radius;
   float cos_anneal_ratio;
   int32_t use_iter_cos;
-  /* forward outputs (nullable) */
+  /* forward outputs (nullable); the training entry also writes rgb when it is given */
   float* rgb;            /* [R,3] */
   float* depth;          /* [R] */
   float* normal;         /* [R,3] */

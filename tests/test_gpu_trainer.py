@@ -179,6 +179,9 @@ def test_fused_trainer_first_step_matches_oracle():
     assert rel_err(st.grad_table.cpu().numpy(), ref.embeddings.grad.numpy()) < 1e-5
     assert abs(float(st.grad_v.reshape(())) - float(ref.variance.grad)) <= 1e-5 * max(1e-3, abs(float(ref.variance.grad)))
     assert h["s_amplified"] is False and h["active_lambda"] == 2.0 and h["w_eik"] == 1e-2
+    # PSNR from the colour MSE (trainer.py:234-236) under the Huber loss too
+    mse = float(((r["render"]["rgb"].detach().double() - b["rgb"].double()) ** 2).mean())
+    assert abs(h["psnr"] - (-10.0 * np.log10(max(mse, 1e-8)))) < 1e-4
     # the optimizer moved the model's own parameters
     assert not np.array_equal(model.sdf_grid.grid.detach().cpu().numpy()[0, 0], grid0)
     assert np.array_equal(model.radiance_mlp.flat().detach().cpu().numpy(), st.mlp.cpu().numpy())

@@ -428,6 +428,11 @@ __global__ void __launch_bounds__(kRT, 2) render_kernel(RenderDev a) {
     block_sums<8>(comp, red, lane, wid);
     const float Nr = sqrtf(comp[5] * comp[5] + comp[6] * comp[6] + comp[7] * comp[7]);
 
+    if (MODE == kTrain && t == 0 && a.rgb) {
+      a.rgb[3 * ray] = comp[0];
+      a.rgb[3 * ray + 1] = comp[1];
+      a.rgb[3 * ray + 2] = comp[2];
+    }
     if (MODE == kFwd) {
       if (t == 0) {
         if (a.rgb) {

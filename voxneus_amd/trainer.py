@@ -165,11 +165,24 @@ class FusedTrainer:
                 "lr_mlp": self.lr_mlp * f, "sparse_updated": self.optimizer.n_updated,
                 "active_lambda": active_lambda, "loss_name": self.loss_type, "vr_counts": out["vr_counts"]}
 
+    def _psnr(self, loss_rgb: float) -> float:
+        lo = self.last_out
+        err = (lo["rgb"].double() - lo["target"].reshape(-1, 3).double()) ** 2
+        m = lo.get("target_mask")
+        if m is not None:
+            fg = m.reshape(-1, 1).double().expand_as(err)
+            fg_sum = float(fg.sum())
+            mse = float((err * fg).sum()) / fg_sum if fg_sum > 0 else loss_rgb
+        else:
+            mse = float(err.mean())
+        return -10.0 * math.log10(max(mse, 1e-8))
+
     def read_metrics(self, metrics: Dict[str, Any]) -> Dict[str, Any]:
         """The reference's metrics dictionary (trainer.py:361-382) as host values, read
-        right after ``train_step`` (one synchronisation).  ``psnr`` is reported for the MSE
-        loss, whose colour loss is the (foreground) MSE itself; the Huber step does not
-        form the MSE and reports None."""
+        right after ``train_step`` (one synchronisation).  ``psnr`` is -10 log10 of the
+        step's colour MSE for every loss type, as trainer.py:217-240 computes it: the
+        foreground-weighted MSE with a mask (the colour loss itself when the mask is empty),
+        else the MSE over all rays and channels, from the colours the step rendered."""
         out = {}
         for k, v in metrics.items():
             out[k] = float(v.reshape(-1)[0]) if torch.is_tensor(v) and k != "vr_counts" else v
@@ -182,6 +195,6 @@ class FusedTrainer:
         s = min(max(math.exp(v * 10.0), 1e-4), st.s_max)
         out["s"], out["std"] = s, 1.0 / s
         out["lr_s"] = self.lr_variance * (self.s_amplification_k if out["s_amplified"] else 1.0)
-        out["psnr"] = (-10.0 * math.log10(max(out["loss_rgb"], 1e-8)) if self.loss_type.lower() == "mse" else None)
+        out["psnr"] = self._psnr(out["loss_rgb"])
         out["occ_pct"] = self.model.occupancy_grid.occupied_percentage
         return out

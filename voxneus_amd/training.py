@@ -155,6 +155,7 @@ class FusedTrainStep:
         self.grad_v = torch.zeros(1, device=dev, dtype=torch.float32)
         self.loss = torch.zeros(1, device=dev, dtype=torch.float64)
         self.loss_parts = torch.zeros(2, device=dev, dtype=torch.float64)  # colour loss, mask BCE
+        self.rgb = None  # the step's rendered colours [R, 3] (for the PSNR of the metrics)
         self.amplified = torch.zeros(1, device=dev, dtype=torch.int32)
         self.bitmask = torch.zeros(Fn.vr_bitmask_words((X, Y, Z)), device=dev, dtype=torch.int32)
         self.vr_counts = torch.zeros(2, device=dev, dtype=torch.int32)
@@ -227,6 +228,9 @@ class FusedTrainStep:
         a.rays_o, a.rays_d = N.dptr(rays_o, torch.float32, "rays_o"), N.dptr(rays_d, torch.float32, "rays_d")
         a.mid_z, a.dists = N.dptr(mid, torch.float32, "mid_z"), N.dptr(dists, torch.float32, "dists")
         a.target_rgb = N.dptr(target, torch.float32, "target")
+        if self.rgb is None or self.rgb.shape[0] != R:
+            self.rgb = torch.empty((R, 3), device=mid.device, dtype=torch.float32)
+        a.rgb = N.dptr(self.rgb)
         a.target_mask = N.dptr(target_mask, torch.float32, "target_mask")
         a.vr_bitmask = N.dptr(self.bitmask)
         # amplify_s_gradient (scheduler.py:14-33) on dL/dv in the reductions
@@ -325,7 +329,7 @@ class FusedTrainStep:
         if cooldown:
             self._eik_clamp()
         return {"loss": self.loss, "loss_parts": self.loss_parts, "reg_losses": self.reg_losses,
-                "vr_counts": self.vr_counts,
+                "vr_counts": self.vr_counts, "rgb": self.rgb, "target": target, "target_mask": target_mask,
                 "amplified": self.amplified, "z_vals": z, "dists": dists, "mid_z_vals": mid}
 
     def rescale_grid(self, new_resolution) -> None:
