@@ -549,10 +549,11 @@ __global__ void __launch_bounds__(kRT, 2) render_kernel(RenderDev a) {
     float dA = gw * T;
     if (ov <= 1.0f && ov >= 0.0f) dA -= (float)(suffix / (double)oma);
     const float dAraw = valid ? dA * inside : 0.0f;
-    // every gradient of the sample is a multiple of w (colour, normal) or of dAraw (SDF,
-    // cosine): zero both and the sample adds nothing
-    const bool glive = w != 0.0f || dAraw != 0.0f;
     const float dq = (q >= 0.0f && q <= 1.0f) ? dAraw : 0.0f;
+    // every gradient of the sample is a multiple of w (colour, normal) or of dq (SDF,
+    // cosine): zero both and the sample adds nothing (an empty-space sample, alpha clamped
+    // to 0, has a nonzero dA from the samples behind it but dq = 0)
+    const bool glive = w != 0.0f || dq != 0.0f;
     float dpp = dq / den;
     const float dpn = -dq / den;
     if (pp >= 1e-5f) dpp -= dq * q / den;
@@ -1132,14 +1133,14 @@ struct BinDev {
 constexpr uint32_t kFactFirst = 1u << 22;   // split pair: only the x corner, (1 - fx) g
 constexpr uint32_t kFactSecond = 2u << 22;  // split pair: only the x + 1 corner, fx g
 
-// Level l of the 64 consecutive samples a wavefront holds (lane = sample i): calls
-// emit(tail, cell, v, d0, d1) with every lane of the wavefront when some lane is in range:
-// tail marks the lanes that emit records (the last lane of each run of samples in one
-// cell below `dedup_levels`, every sample above), v their 8 corners' weighted gradients
-// (run sums) and d0, d1 the level's feature gradients of the lane's own sample.  A sample
-// whose two gradients are zero (most often one past the transmittance cutoff) is not live
-// and emits nothing; counting only live samples keeps the record passes to the samples
-// that contribute.
+// Level l of the 64 consecutive samples a wavefront holds (lane = sample i, the walk's
+// samples in order): calls emit(tail, cell, v, d0, d1) with every lane of the wavefront
+// when some lane is live: tail marks the lanes that emit records (the last lane of each
+// run of samples in one cell below `dedup_levels`, every live sample above), v their 8
+// corners' weighted gradients (run sums) and d0, d1 the level's feature gradients of the
+// lane's own sample.  Walking the render's list of samples with a gradient, every listed
+// sample is live; without the list a sample is live where its two gradients are not zero.
+// Either way the count and write passes apply the same rule, so they agree.
 // VALUES = false leaves v unset: only the tails are needed.
 // The inputs of sample i at level l: its two feature gradients, mid-point depth and ray
 // (zero past the last sample), loaded ahead of their use by the write pass.
@@ -1147,13 +1148,16 @@ struct LevelIn {
   float d0, d1, m, ox, oy, oz, rx, ry, rz;
 };
 
+template <bool GRAD = true>
 VXN_DEV LevelIn level_in(const RenderDev& a, int64_t i, int l) {
   const int64_t NS = a.R * (int64_t)a.S;
   LevelIn x{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
   if (i < NS) {
     const int64_t ray = ray_of(a, i);
-    x.d0 = a.gfeat[(2 * l) * NS + i];
-    x.d1 = a.gfeat[(2 * l + 1) * NS + i];
+    if constexpr (GRAD) {
+      x.d0 = a.gfeat[(2 * l) * NS + i];
+      x.d1 = a.gfeat[(2 * l + 1) * NS + i];
+    }
     x.m = a.mid[i];
     x.ox = a.o[3 * ray];
     x.oy = a.o[3 * ray + 1];
@@ -1179,7 +1183,9 @@ VXN_DEV bool hash_level_eval(const RenderDev& a, int64_t i, const LevelIn& x, in
     d0 = d0 * lw;
     d1 = d1 * lw;
   }
-  const bool live = d0 != 0.0f || d1 != 0.0f;
+  // walking the live list, every listed sample emits its records (its gradients can still
+  // be zero at a level: such records add zeros), so the count pass reads no gradients
+  const bool live = a.live_idx ? in : (d0 != 0.0f || d1 != 0.0f);
   if (__ballot(live) == 0) return false;
   float x0 = 0.0f, x1 = 0.0f, x2 = 0.0f;
   if (in) {
@@ -1219,7 +1225,8 @@ VXN_DEV void hash_level_records(const RenderDev& a, int64_t i, int lane, int l, 
   float v[16];
   float d0, d1;
   bool tail;
-  if (hash_level_eval<VALUES>(a, i, level_in(a, i, l), lane, l, dedup_levels, c, v, d0, d1, tail))
+  const LevelIn x = (VALUES || !a.live_idx) ? level_in<true>(a, i, l) : level_in<false>(a, i, l);
+  if (hash_level_eval<VALUES>(a, i, x, lane, l, dedup_levels, c, v, d0, d1, tail))
     emit(tail, c, v, d0, d1);
 }
 
@@ -1814,8 +1821,9 @@ VXN_DEV bool grid_brick(const RenderDev& a, const GridIn& x, int lane, float (&v
 
 __global__ void __launch_bounds__(256) grid_scatter_kernel(RenderDev a) {
   const int lane = threadIdx.x & 63;
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  const GridIn x = grid_in(a, i);
+  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;  // position of the walk
+  if (blockIdx.x * (int64_t)blockDim.x >= walk_length(a)) return;
+  const GridIn x = grid_in(a, walk_sample(a, k, walk_length(a)));
   float v[32];
   uint64_t key;
   int c0[3];
@@ -1943,11 +1951,12 @@ __global__ void __launch_bounds__(256) grid_bin_count_kernel(RenderDev a, GridBi
   for (int k = lane; k < b.nbins; k += 64) row[k] = 0;
   wave_sync();
   const bool ana = a.g.analytical != 0;
-  const int64_t i0 = r * kGridRowSamples + lane;
-  GridIn nxt = grid_in(a, i0);
+  const int64_t i0 = r * kGridRowSamples + lane;  // positions of the walk
+  const int64_t nw = walk_length(a);
+  GridIn nxt = grid_in(a, walk_sample(a, i0, nw));
   for (int c = 0; c < kGridRowChunks; ++c) {
     const GridIn cur = nxt;
-    if (c + 1 < kGridRowChunks) nxt = grid_in(a, i0 + (c + 1) * 64);
+    if (c + 1 < kGridRowChunks) nxt = grid_in(a, walk_sample(a, i0 + (c + 1) * 64, nw));
     float v[32];
     uint64_t key;
     int c0[3];
@@ -1976,11 +1985,12 @@ __global__ void __launch_bounds__(256) grid_bin_write_kernel(RenderDev a, GridBi
   for (int k = lane; k < b.nbins; k += 64) pos[k] = prow[k];
   wave_sync();
   const bool ana = a.g.analytical != 0;
-  const int64_t i0 = r * kGridRowSamples;
-  GridIn nxt = grid_in(a, i0 + lane);
+  const int64_t i0 = r * kGridRowSamples;  // positions of the walk
+  const int64_t nw = walk_length(a);
+  GridIn nxt = grid_in(a, walk_sample(a, i0 + lane, nw));
   for (int c = 0; c < kGridRowChunks; ++c) {
     const GridIn cur = nxt;
-    if (c + 1 < kGridRowChunks) nxt = grid_in(a, i0 + (c + 1) * 64 + lane);
+    if (c + 1 < kGridRowChunks) nxt = grid_in(a, walk_sample(a, i0 + (c + 1) * 64 + lane, nw));
     float v[32];
     uint64_t key;
     int c0[3];
@@ -2160,14 +2170,15 @@ constexpr int kEdgeList = 2048;  // listed samples per flush
 
 __global__ void __launch_bounds__(64) grid_edge_kernel(RenderDev a, const uint64_t* __restrict__ edge,
                                                        int64_t nwords) {
-  __shared__ int32_t list[kEdgeList];
+  __shared__ int32_t list[kEdgeList];  // positions of the walk
   const int lane = threadIdx.x;
+  const int64_t nw = walk_length(a);
   int n = 0;  // listed samples (wave-uniform)
   auto flush = [&]() {
     wave_sync();
     for (int b = 0; b < n; b += 64) {
       GridIn x{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-      if (b + lane < n) x = grid_in(a, (int64_t)list[b + lane]);
+      if (b + lane < n) x = grid_in(a, walk_sample(a, (int64_t)list[b + lane], nw));
       float v[32];
       uint64_t key;
       int c0[3];
@@ -2634,6 +2645,22 @@ static int launch_render_nv(const vxn_render_args* p, void* stream) {
   if (ev) VXN_HIP(hipEventRecord(ev[2], s));
   if (MODE != kFwd) {
     const int sb = (int)((ns + 255) / 256);
+    if (binned_hash) {
+      // the samples with a gradient, in order: the record passes and the grid gradient walk
+      // only them
+      char* wsp = (char*)p->workspace;
+      const unsigned nlb = (unsigned)((ns + kLiveBlock - 1) / kLiveBlock);
+      int32_t* lcnt = (int32_t*)(wsp + w.off_lcnt);
+      int32_t* lidx = (int32_t*)(wsp + w.off_lidx);
+      int32_t* nlive = (int32_t*)(wsp + w.off_nlive);
+      hipLaunchKernelGGL(live_count_kernel, dim3(nlb), dim3(256), 0, s, (const uint8_t*)a.live, ns, lcnt);
+      VXN_LAUNCH_CHECK();
+      hipLaunchKernelGGL(live_list_kernel, dim3(nlb), dim3(256), 0, s, (const uint8_t*)a.live, ns,
+                         (const int32_t*)lcnt, lidx, nlive);
+      VXN_LAUNCH_CHECK();
+      a.live_idx = lidx;
+      a.n_live = nlive;
+    }
     // the grid gradient needs only the render kernel's per-sample geometry gradients
     hipStream_t gs = s;
     if (ss) {
@@ -2692,19 +2719,6 @@ static int launch_render_nv(const vxn_render_args* p, void* stream) {
     if (ev) VXN_HIP(hipEventRecord(ev[3], gs));
     if (ss) VXN_HIP(hipEventRecord(se[1], ss));
     if (binned_hash) {
-      // the live samples in order: the record passes walk only them
-      char* wsp = (char*)p->workspace;
-      const unsigned nlb = (unsigned)((ns + kLiveBlock - 1) / kLiveBlock);
-      int32_t* lcnt = (int32_t*)(wsp + w.off_lcnt);
-      int32_t* lidx = (int32_t*)(wsp + w.off_lidx);
-      int32_t* nlive = (int32_t*)(wsp + w.off_nlive);
-      hipLaunchKernelGGL(live_count_kernel, dim3(nlb), dim3(256), 0, s, (const uint8_t*)a.live, ns, lcnt);
-      VXN_LAUNCH_CHECK();
-      hipLaunchKernelGGL(live_list_kernel, dim3(nlb), dim3(256), 0, s, (const uint8_t*)a.live, ns,
-                         (const int32_t*)lcnt, lidx, nlive);
-      VXN_LAUNCH_CHECK();
-      a.live_idx = lidx;
-      a.n_live = nlive;
       if (int e = count_and_scan(s)) return e;
       // levels below dl carry run sums (record values + entry array), the rest factored records
       if (dl > 0) {
