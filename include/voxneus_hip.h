@@ -316,6 +316,11 @@ typedef struct vxn_render_args {
      grad_grid must have no other writer during the call (fork_event marks the end of the
      binned passes). */
   int32_t grid_binned;
+  /* nullable hipEvent_t: the binned grid gradient's read-add-writes of grad_grid (and its
+     edge pass) wait for it, so other work adding into grad_grid on another stream (the
+     fused training step's regularisers, started right after the sampler) is ordered before
+     them instead of after fork_event */
+  void* grid_wait_event;
 } vxn_render_args;
 
 /* Workspace of the backward and training entries (the forward needs none): per-sample

@@ -95,7 +95,7 @@ class RenderArgs(ctypes.Structure):
         ("view_embed", c_i32), ("loss_type", c_i32), ("target_mask", c_vp), ("mask_weight", c_f),
         ("loss_parts", c_vp), ("scatter_stream", c_vp), ("scatter_events", c_vp),
         ("variance", c_vp), ("s_max", c_f), ("s_out", c_vp), ("d_variance", c_vp), ("s_amp_k", c_f),
-        ("amplified", c_vp), ("grad_table_overwrite", c_i32), ("grid_binned", c_i32),
+        ("amplified", c_vp), ("grad_table_overwrite", c_i32), ("grid_binned", c_i32), ("grid_wait_event", c_vp),
     ]
 
 

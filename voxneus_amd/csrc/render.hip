@@ -2705,6 +2705,7 @@ static int launch_render_nv(const vxn_render_args* p, void* stream) {
       VXN_LAUNCH_CHECK();
       hipLaunchKernelGGL(grid_bin_write_kernel, dim3(gwg), dim3(256), grid_write_lds(w.gbins), gs, a, gb);
       VXN_LAUNCH_CHECK();
+      if (p->grid_wait_event) VXN_HIP(hipStreamWaitEvent(gs, (hipEvent_t)p->grid_wait_event, 0));
       hipLaunchKernelGGL(grid_bin_sum_kernel, dim3((unsigned)(4 * w.gbins)), dim3(64 * kGsumWaves), kGsumLds, gs, gb,
                          a.g, a.grad_grid);
       VXN_LAUNCH_CHECK();

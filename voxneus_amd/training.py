@@ -93,8 +93,10 @@ class StepConfig:
     # mark V_R in the sampler and run the regularisers on a second stream, beside the
     # render and the hash-table gradient
     concurrent_regularisers: bool = True
-    # start them just before the render kernel (else right after the sampler, beside the
-    # hash-feature gather; measured slower: the render kernel loses work-group slots to them)
+    # start them just before the render kernel, or with the binned grid gradient after it
+    # (else right after the sampler, beside the hash-feature gather, the binned grid sum
+    # waiting for them through vxn_render_args.grid_wait_event; measured slower on the
+    # 8192-ray step: BF16 1.63 -> 1.72 ms, FP32 2.50 -> 2.58 ms)
     fork_after_features: bool = True
     # run the grid gradient on a third stream, beside the hash-table gradient (8192-ray
     # step: BF16 form 1.77 -> 1.73 ms with the atomic scatter, 1.68 ms binned; FP32 form
@@ -241,6 +243,7 @@ class FusedTrainStep:
         else:
             a.scatter_stream, a.scatter_events = None, None
         a.fork_event = None
+        a.grid_wait_event = None
         a.kernel_events = (ctypes.cast(self.kernel_events.array, ctypes.c_void_p)
                            if self.kernel_events is not None else None)
         return a
@@ -313,15 +316,21 @@ class FusedTrainStep:
                 # whose plain read-add-writes of grad_grid the regularisers' atomics follow)
                 a.fork_event = self.side.events[0]
             else:
-                self.side.fork(main)  # right after the sampler
+                # right after the sampler, beside the hash-feature gather; the binned grid
+                # gradient's read-add-writes of grad_grid wait for them
+                self.side.fork(main)
+                with torch.cuda.stream(self.side.stream):
+                    self._regularisers()
+                self.side.finish()
+                a.grid_wait_event = self.side.events[1]
         N.check(N.lib().vxn_render_train(ctypes.byref(a), main), "render_train")
         if early:
-            # queued after the render launch, so the render's kernels are dispatched first
             if cfg.fork_after_features:
+                # queued after the render launch, so the render's kernels are dispatched first
                 self.side.fork(None)
-            with torch.cuda.stream(self.side.stream):
-                self._regularisers()
-            self.side.finish()
+                with torch.cuda.stream(self.side.stream):
+                    self._regularisers()
+                self.side.finish()
         else:
             self._regularisers()
         if early:
