@@ -86,3 +86,31 @@ def test_binned_on_second_stream_equals_atomic():
     a = _grid_grads(sc, sc.grid, o, d, t)[0]
     b = _grid_grads(sc, sc.grid, o, d, t, binned_grid_gradient=True, concurrent_scatter=True)[0]
     assert _rel(b, a) <= 1e-6
+
+
+def test_regularisers_after_sampler_with_binned_grid_gradient():
+    """The regularisers started right after the sampler (StepConfig.fork_after_features =
+    False): the binned grid sum waits for them through vxn_render_args.grid_wait_event, so
+    the grid gradient equals the default order's (regularisers after the grid sum) up to
+    the order of the adds (the regularisers' per-tile device atomics meet at tile borders,
+    so neither order repeats bit for bit; an update lost to an unordered read-add-write
+    would stand out far above 1e-6)."""
+    from voxneus_amd.training import FusedTrainStep, StepConfig
+    sc = scenes.config1_scene(seed=0)
+    o, d, t = scenes.config_rays("sphere", 256, seed=4)
+    o, d, t = (torch.from_numpy(v).to(DEV) for v in (o, d, t))
+    out = []
+    for early in (False, True):
+        st = FusedTrainStep(torch.from_numpy(sc.grid).to(DEV), torch.from_numpy(sc.embeddings).to(DEV),
+                            _mlp(sc).to(DEV), torch.tensor(0.3, device=DEV),
+                            R.hash_level_resolutions(16, 16, sc.hash_max_resolution),
+                            cfg=StepConfig(w_eik=0.1, w_curv=5e-4, binned_grid_gradient=True,
+                                           fork_after_features=not early))
+        g = []
+        for _ in range(2):
+            st.step(o, d, t)
+            torch.cuda.synchronize()
+            g.append(st.grad_grid.clone())
+        assert _rel(g[1], g[0]) <= 1e-6
+        out.append(g[0])
+    assert _rel(out[1], out[0]) <= 1e-6
