@@ -253,7 +253,7 @@ typedef struct vxn_render_args {
   void* workspace;
   int64_t workspace_bytes;
   /* nullable profiling hook: hipEvent_t[6] recorded on the stream before the hash-feature
-     gather and after it, the render kernel, the grid gradient (the atomic scatter or the
+     gather (BF16 form: the geometry pass and the gather) and after it, the render kernel, the grid gradient (the atomic scatter or the
      binned passes), the hash-table gradient (its binned passes or the atomic scatter) and
      the reductions (forward: only the first three) */
   void* const* kernel_events;

@@ -116,6 +116,12 @@ struct RenderDev {
   uint8_t* live;
   const int32_t* live_idx;
   const int32_t* n_live;
+  // BF16 backward / training: the geometry pass's SDF and gradient of every sample
+  // ([4][R*S]: sdf, gx, gy, gz), read by the render kernel in place of its own stencil, and
+  // need[i] = 0 where sample i's colour reaches no output or gradient (q < 0 or outside
+  // the sphere), whose hash features the gather then skips (zeros)
+  float* geo;
+  uint8_t* need;
   // serving batches (forward only): per-ray scene of a packed batch; R is the capacity
   // and the batch size is read from n_rays_dev
   const SceneDev* scenes;
@@ -163,6 +169,50 @@ VXN_DEV void build_mlp_fragments(const float* W0, const float* W1, u32x4_t* frag
   }
 }
 
+// SDF (grid_sample) and Eq. 7 / Eq. 4 gradient at a sample position: the render kernel's
+// and the geometry pass's common code, so both get the same values.
+VXN_DEV void sample_sdf_grad(const GridDesc& G, float px, float py, float pz, float& sdf, float (&gv)[3]) {
+  const GsStencil gs = gs_stencil(G, px, py, pz);
+  const Corners cn = corner_decomp(G, px, py, pz);
+  if (!G.analytical && brick_interior(G, cn)) {
+    Brick bk;
+    load_brick(G, cn, bk);
+    if (!gs_eval_brick(gs, cn, bk, sdf)) sdf = gs_eval(G, gs);
+    eq7_gradient_brick(G, cn, bk, gv);
+  } else {
+    sdf = gs_eval(G, gs);
+    grid_gradient(G, cn, gv);
+  }
+}
+
+// Eq. 2 of one sample (neus_alpha.py:12-81) up to the clamp: q = (p_prev - p_next) /
+// max(p_prev, 1e-5) with the iter_cos rule, and the inside-sphere factor.  A sample with
+// q < 0 or outside the sphere has alpha = 0 and, since the alpha gradient passes only
+// where 0 <= q <= 1 and inside, no gradient depends on its colour.
+struct NeusQ {
+  float tcos, off, fp, fn, pp, pn, den, q, inside;
+};
+
+VXN_DEV NeusQ neus_q(const float nrm[3], float dx, float dy, float dz, float dist, float sdf, float sv, float sr,
+                     float px, float py, float pz, int iter_cos, float cos_anneal) {
+  NeusQ r;
+  r.tcos = (nrm[0] * dx + nrm[1] * dy) + nrm[2] * dz;
+  float cosv = r.tcos;
+  if (iter_cos) {
+    const float ra = fmaxf(-r.tcos * 0.5f + 0.5f, 0.0f), rb = fmaxf(-r.tcos, 0.0f);
+    cosv = -(ra * (1.0f - cos_anneal) + rb * cos_anneal);
+  }
+  r.off = 0.5f * dist * cosv;
+  r.fp = sdf - r.off;
+  r.fn = sdf + r.off;
+  r.pp = sigmoid_ref(sv * r.fp);
+  r.pn = sigmoid_ref(sv * r.fn);
+  r.den = fmaxf(r.pp, 1e-5f);
+  r.q = (r.pp - r.pn) / r.den;
+  r.inside = (sr > 0.0f) ? ((sqrtf(px * px + py * py + pz * pz) < sr) ? 1.0f : 0.0f) : 1.0f;
+  return r;
+}
+
 template <int MODE, int MLPM, int MULTI, int NV>
 __global__ void __launch_bounds__(kRT, 2) render_kernel(RenderDev a) {
   using ML = MlpLayout<NV>;
@@ -172,6 +222,7 @@ __global__ void __launch_bounds__(kRT, 2) render_kernel(RenderDev a) {
   // BF16 backward / training and BF16 serving: the hash features come from
   // hash_features_kernel (a.hfeat)
   constexpr bool kPre = kBF && (MODE != kFwd || MULTI);
+  // BF16 backward / training: the SDF and its gradient come from geo_kernel (a.geo)
   __shared__ Red red;
   __shared__ __align__(16) float preh[64];  // b0 + W0[:, 32:32+NV] . view(d): the per-ray part of the hidden layer
   __shared__ int nlive_s[2];  // samples per wavefront with a non-zero output gradient
@@ -280,17 +331,13 @@ __global__ void __launch_bounds__(kRT, 2) render_kernel(RenderDev a) {
     }
     // ---------------------------------------------------------------- sample forward
     const float px = ox + dx * mid, py = oy + dy * mid, pz = oz + dz * mid;
-    const GsStencil gs = gs_stencil(G, px, py, pz);
-    const Corners cn = corner_decomp(G, px, py, pz);
     float sdf, gv[3];
-    if (!G.analytical && brick_interior(G, cn)) {
-      Brick bk;
-      load_brick(G, cn, bk);
-      if (!gs_eval_brick(gs, cn, bk, sdf)) sdf = gs_eval(G, gs);
-      eq7_gradient_brick(G, cn, bk, gv);
+    if constexpr (kBFB && !MULTI) {  // the geometry pass's values (the same code, run ahead)
+      sdf = valid ? a.geo[si] : 0.0f;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) gv[k] = valid ? a.geo[(1 + k) * NS + si] : 0.0f;
     } else {
-      sdf = gs_eval(G, gs);
-      grid_gradient(G, cn, gv);
+      sample_sdf_grad(G, px, py, pz, sdf, gv);
     }
     const float gr = norm3(gv);
     const float nrm[3] = {gv[0] / (gr + 1e-8f), gv[1] / (gr + 1e-8f), gv[2] / (gr + 1e-8f)};
@@ -404,19 +451,10 @@ __global__ void __launch_bounds__(kRT, 2) render_kernel(RenderDev a) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) rgb[c] = sigmoid_ref(o3[c]);
     // ---------------------------------------------------------------- Eq. 2 alpha
-    const float tcos = (nrm[0] * dx + nrm[1] * dy) + nrm[2] * dz;
-    float cosv = tcos;
-    if (a.iter_cos) {
-      const float ra = fmaxf(-tcos * 0.5f + 0.5f, 0.0f), rb = fmaxf(-tcos, 0.0f);
-      cosv = -(ra * (1.0f - a.cos_anneal) + rb * a.cos_anneal);
-    }
-    const float off = 0.5f * dist * cosv;
-    const float fp = sdf - off, fn = sdf + off;
-    const float pp = sigmoid_ref(sv * fp), pn = sigmoid_ref(sv * fn);
-    const float den = fmaxf(pp, 1e-5f);
-    const float q = (pp - pn) / den;
+    const NeusQ nq = neus_q(nrm, dx, dy, dz, dist, sdf, sv, sr, px, py, pz, a.iter_cos, a.cos_anneal);
+    const float tcos = nq.tcos, fp = nq.fp, fn = nq.fn, pp = nq.pp, pn = nq.pn, den = nq.den, q = nq.q;
+    const float inside = nq.inside;
     const float araw = fminf(fmaxf(q, 0.0f), 1.0f);
-    const float inside = (sr > 0.0f) ? ((sqrtf(px * px + py * py + pz * pz) < sr) ? 1.0f : 0.0f) : 1.0f;
     const float alpha = valid ? araw * inside : 0.0f;
     const float ov = (1.0f - alpha) + 1e-7f;
     const float oma = fminf(fmaxf(ov, 0.0f), 1.0f);
@@ -464,6 +502,7 @@ __global__ void __launch_bounds__(kRT, 2) render_kernel(RenderDev a) {
           a.normal_s[3 * si + 2] = nrm[2];
         }
         if (a.corners) {
+          const Corners cn = corner_decomp(G, px, py, pz);
 #pragma unroll
           for (int k = 0; k < 8; ++k) {
             int cx, cy, cz;
@@ -477,6 +516,7 @@ __global__ void __launch_bounds__(kRT, 2) render_kernel(RenderDev a) {
     }
     if (a.vr) {
       // V_R marks: a sample in the same cell as the previous lane's marks nothing new.
+      const Corners cn = corner_decomp(G, px, py, pz);
       const int key = valid ? (int)lin3(G, cn.c0[0], cn.c0[1], cn.c0[2]) : -1;
       const int kp = __shfl_up(key, 1, 64);
       if (valid && (lane == 0 || kp != key)) vr_mark_cell(G, cn, a.vr);
@@ -900,6 +940,33 @@ VXN_DEV void sample_pos(const RenderDev& a, int64_t i, float& px, float& py, flo
   pz = a.o[3 * ray + 2] + a.d[3 * ray + 2] * m;
 }
 
+// Geometry pass of the BF16 backward / training (one sample per thread, ahead of the
+// hash-feature gather): the sample's SDF and gradient (the render kernel reads them in
+// place of its own stencil) and whether its colour matters (need): a sample with q < 0 or
+// outside the sphere has alpha = 0 and no gradient through its colour, so its hash
+// features are not gathered.  The same helpers and operation order as the render kernel.
+__global__ void __launch_bounds__(256) geo_kernel(RenderDev a) {
+  const int64_t NS = a.R * (int64_t)a.S;
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= NS) return;
+  const int64_t ray = ray_of(a, i);
+  const float ox = a.o[3 * ray], oy = a.o[3 * ray + 1], oz = a.o[3 * ray + 2];
+  const float dx = a.d[3 * ray], dy = a.d[3 * ray + 1], dz = a.d[3 * ray + 2];
+  const float mid = a.mid[i], dist = a.dist[i];
+  const float px = ox + dx * mid, py = oy + dy * mid, pz = oz + dz * mid;
+  float sdf, gv[3];
+  sample_sdf_grad(a.g, px, py, pz, sdf, gv);
+  a.geo[i] = sdf;
+  a.geo[NS + i] = gv[0];
+  a.geo[2 * NS + i] = gv[1];
+  a.geo[3 * NS + i] = gv[2];
+  const float gr = norm3(gv);
+  const float nrm[3] = {gv[0] / (gr + 1e-8f), gv[1] / (gr + 1e-8f), gv[2] / (gr + 1e-8f)};
+  const float sv = a.var_dev ? s_of_variance(*a.var_dev, a.s_max) : (a.s_dev ? *a.s_dev : a.s);
+  const NeusQ nq = neus_q(nrm, dx, dy, dz, dist, sdf, sv, a.sphere_r, px, py, pz, a.iter_cos, a.cos_anneal);
+  a.need[i] = (!(nq.q < 0.0f) && nq.inside != 0.0f) ? 1 : 0;
+}
+
 // Hash features of every sample, level-major and XCD-aware.  Work-groups are dispatched to
 // the 8 XCDs round-robin (work-group b runs on XCD b % 8), so work-group b gathers level
 // (b % 8) + 8 * pass for one 256-sample chunk, pass 0 for all chunks first: each XCD reads
@@ -924,6 +991,9 @@ __global__ void __launch_bounds__(256) hash_features_kernel(RenderDev a, int64_t
     g = &a.scenes[si].g;
     h = &a.scenes[si].h;
   } else if (l >= a.h.L) {
+    return;
+  } else if (a.need && !a.need[i]) {
+    ((uint32_t*)a.hfeat)[l * NS + i] = 0u;  // colour unused: no gather
     return;
   }
   float px, py, pz;
@@ -2386,7 +2456,7 @@ static BinPlan bin_plan(int64_t ns, int L, int log2_T, const int32_t* level_res)
 struct RenderWs {
   int64_t off_sc, off_norm, off_rows, off_feat, off_geo, off_hf, off_cnt, off_csum, off_base, off_ridx, off_rval;
   int64_t off_gcnt, off_gcsum, off_gbase, off_gref, off_gval, off_gedge, off_live, off_lidx, off_lcnt, off_nlive,
-      total;
+      off_gsd, off_need, total;
   BinPlan bins;
   int gbins;   // grid tiles of the binned grid gradient (0: the atomic grid scatter)
   int grows;   // its count rows
@@ -2415,6 +2485,10 @@ static RenderWs render_ws(int64_t R, int S, int L, int log2_T, const int32_t* le
     w.off_csum = w.off_base = w.off_ridx = w.off_rval = w.off_cnt;
     end = w.off_cnt;
   }
+  // the geometry pass's per-sample SDF and gradient, and the need flags
+  w.off_gsd = end;
+  w.off_need = align256(w.off_gsd + ns * 16);
+  end = align256(w.off_need + ns);
   // the live-sample list of the record passes
   if (w.bins.use) {
     w.off_live = end;
@@ -2536,6 +2610,10 @@ static int build_render_dev(const vxn_render_args& A, RenderDev& a, int mode) {
     a.ggeo = (float*)(ws + w.off_geo);
     a.hfeat = ws + w.off_hf;
     if (w.bins.use) a.live = (uint8_t*)(ws + w.off_live);
+    if (A.mlp_mode == VXN_MLP_BF16_MFMA) {  // the geometry pass runs ahead of the feature gather
+      a.geo = (float*)(ws + w.off_gsd);
+      a.need = (uint8_t*)(ws + w.off_need);
+    }
   }
   if (mode == kTrain) {
     VXN_REQUIRE(A.target_rgb != nullptr, VXN_ERR_NULL, "target_rgb is null");
@@ -2623,6 +2701,8 @@ static int launch_render_nv(const vxn_render_args* p, void* stream) {
   };
   if (ev) VXN_HIP(hipEventRecord(ev[0], s));
   if (MODE != kFwd && p->mlp_mode == VXN_MLP_BF16_MFMA) {
+    hipLaunchKernelGGL(geo_kernel, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, s, a);
+    VXN_LAUNCH_CHECK();
     const int64_t chunks = (ns + 255) / 256;
     const unsigned nb_hf = (unsigned)(8 * ((a.h.L + 7) / 8) * chunks);
     hipLaunchKernelGGL(hash_features_kernel<0>, dim3(nb_hf), dim3(256), 0, s, a, chunks);
