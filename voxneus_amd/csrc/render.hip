@@ -344,10 +344,19 @@ __global__ void __launch_bounds__(kRT, 2) render_kernel(RenderDev a) {
     const float x0 = fminf(fmaxf(over_den(G, 0, px - G.mn[0]), 0.0f), 1.0f);
     const float x1 = fminf(fmaxf(over_den(G, 1, py - G.mn[1]), 0.0f), 1.0f);
     const float x2 = fminf(fmaxf(over_den(G, 2, pz - G.mn[2]), 0.0f), 1.0f);
+    // Eq. 2 up to the clamp (needs only the geometry)
+    const NeusQ nq = neus_q(nrm, dx, dy, dz, dist, sdf, sv, sr, px, py, pz, a.iter_cos, a.cos_anneal);
     float x[35];  // 32 hash features, then the unit normal (MLP input columns 0-31, 48-50)
     if constexpr (!kPre) {
+      // backward / training: a sample with q < 0 or outside the sphere has alpha = 0 and no
+      // gradient through its colour, so its features are not gathered (zeros in their place)
+      if (MODE == kFwd || (!(nq.q < 0.0f) && nq.inside != 0.0f)) {
 #pragma unroll
-      for (int l = 0; l < 16; ++l) hash_level_fwd<kBF>(Hd, l, x0, x1, x2, x[2 * l], x[2 * l + 1]);
+        for (int l = 0; l < 16; ++l) hash_level_fwd<kBF>(Hd, l, x0, x1, x2, x[2 * l], x[2 * l + 1]);
+      } else {
+#pragma unroll
+        for (int k = 0; k < 32; ++k) x[k] = 0.0f;
+      }
     }
     x[32] = nrm[0];
     x[33] = nrm[1];
@@ -451,7 +460,6 @@ __global__ void __launch_bounds__(kRT, 2) render_kernel(RenderDev a) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) rgb[c] = sigmoid_ref(o3[c]);
     // ---------------------------------------------------------------- Eq. 2 alpha
-    const NeusQ nq = neus_q(nrm, dx, dy, dz, dist, sdf, sv, sr, px, py, pz, a.iter_cos, a.cos_anneal);
     const float tcos = nq.tcos, fp = nq.fp, fn = nq.fn, pp = nq.pp, pn = nq.pn, den = nq.den, q = nq.q;
     const float inside = nq.inside;
     const float araw = fminf(fmaxf(q, 0.0f), 1.0f);
