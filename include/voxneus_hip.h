@@ -259,10 +259,12 @@ typedef struct vxn_render_args {
   void* const* kernel_events;
   int32_t gradient_mode; /* VXN_GRADIENT_*: the normal of every sample (VoxNeuS gradient_mode) */
   /* nullable hipEvent_t, recorded on `stream` after the hash-feature gather, just before
-     the render kernel (with grid_binned: after the binned grid gradient, whose plain
-     read-add-writes of grad_grid must not overlap other writers, on the stream that runs
-     it): a caller can start independent work on another stream from it, such as atomic
-     adds into grad_grid (the fused training step starts its regularisers there) */
+     the render kernel (with grid_binned: after the binned grid gradient's tile sums, whose
+     plain read-add-writes of grad_grid must not overlap other writers, on the stream that
+     runs them; the ordered edge pass that follows adds with atomics): a caller can start
+     independent work on another stream from it, such as atomic adds into grad_grid (the
+     fused training step starts its regularisers there; grad_grid is then the same on every
+     run only up to the order of those atomics) */
   void* fork_event;
   /* nullable device float: the NeuS sharpness s read on the device in place of `s`, so a
      caller that keeps s on the device (s = clamp(exp(10 v)), models/variance.py:31) needs
@@ -313,8 +315,8 @@ typedef struct vxn_render_args {
      atomic but are added by one wavefront in sample order, so the grid gradient of the
      call is the same on every run.  Grids up
      to 1024 vertices per axis and 4096 tiles; otherwise (and with 0) the atomic scatter.
-     grad_grid must have no other writer during the call (fork_event marks the end of the
-     binned passes). */
+     grad_grid must have no other writer during the call except atomic adds forked from
+     fork_event (recorded after the tile sums). */
   int32_t grid_binned;
   /* nullable hipEvent_t: the binned grid gradient's read-add-writes of grad_grid (and its
      edge pass) wait for it, so other work adding into grad_grid on another stream (the

@@ -122,6 +122,9 @@ struct RenderDev {
   // the sphere), whose hash features the gather then skips (zeros)
   float* geo;
   uint8_t* need;
+  // the record passes walk lists (binned hash-table gradient): ggeo (and in the FP32 form
+  // gfeat) is then written only for samples that a list can hold (0 <= q, inside)
+  int sparse;
   // serving batches (forward only): per-ray scene of a packed batch; R is the capacity
   // and the batch size is read from n_rays_dev
   const SceneDev* scenes;
@@ -693,7 +696,7 @@ __global__ void __launch_bounds__(kRT, 2) render_kernel(RenderDev a) {
       for (int s2 = 0; s2 < nlive; ++s2) accb1 += Os[s2 * 3 + t];
 #pragma unroll
     for (int k = 0; k < 3; ++k) dn[k] += dxa[32 + k];
-    if (valid) {
+    if (valid && (!a.sparse || (!(q < 0.0f) && inside != 0.0f))) {
 #pragma unroll
       for (int i = 0; i < 32; ++i) a.gfeat[i * NS + si] = dxa[i];
     }
@@ -797,10 +800,13 @@ __global__ void __launch_bounds__(kRT, 2) render_kernel(RenderDev a) {
       const float rr = gr + 1e-8f;
       const float dotv = (dn[0] * gv[0] + dn[1] * gv[1]) + dn[2] * gv[2];
       if (a.live) a.live[si] = glive ? 1 : 0;
-      a.ggeo[si] = dsdf;
+      // (the BF16 form reads the geometry pass's flag: q and inside are not kept that long)
+      if (!a.sparse || (kBFB ? (a.need[si] & 1) != 0 : (!(q < 0.0f) && inside != 0.0f))) {
+        a.ggeo[si] = dsdf;
 #pragma unroll
-      for (int k = 0; k < 3; ++k)
-        a.ggeo[(1 + k) * NS + si] = dn[k] / rr - (gr > 0.0f ? gv[k] * dotv / (gr * rr * rr) : 0.0f);
+        for (int k = 0; k < 3; ++k)
+          a.ggeo[(1 + k) * NS + si] = dn[k] / rr - (gr > 0.0f ? gv[k] * dotv / (gr * rr * rr) : 0.0f);
+      }
     }
     __syncthreads();
   }
@@ -952,7 +958,9 @@ VXN_DEV void sample_pos(const RenderDev& a, int64_t i, float& px, float& py, flo
 // hash-feature gather): the sample's SDF and gradient (the render kernel reads them in
 // place of its own stencil) and whether its colour matters (need): a sample with q < 0 or
 // outside the sphere has alpha = 0 and no gradient through its colour, so its hash
-// features are not gathered.  The same helpers and operation order as the render kernel.
+// features are not gathered; with q = 0 its colour still enters its alpha gradient but its
+// features get no gradient (w = 0), so the hash-table gradient's records skip it too.
+// The same helpers and operation order as the render kernel.
 __global__ void __launch_bounds__(256) geo_kernel(RenderDev a) {
   const int64_t NS = a.R * (int64_t)a.S;
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -972,7 +980,10 @@ __global__ void __launch_bounds__(256) geo_kernel(RenderDev a) {
   const float nrm[3] = {gv[0] / (gr + 1e-8f), gv[1] / (gr + 1e-8f), gv[2] / (gr + 1e-8f)};
   const float sv = a.var_dev ? s_of_variance(*a.var_dev, a.s_max) : (a.s_dev ? *a.s_dev : a.s);
   const NeusQ nq = neus_q(nrm, dx, dy, dz, dist, sdf, sv, a.sphere_r, px, py, pz, a.iter_cos, a.cos_anneal);
-  a.need[i] = (!(nq.q < 0.0f) && nq.inside != 0.0f) ? 1 : 0;
+  // bit 0: 0 <= q, inside (the colour and the SDF gradients can matter); bit 1: 0 < q,
+  // inside (alpha > 0: the hash features can have a gradient)
+  const bool in = nq.inside != 0.0f;
+  a.need[i] = (uint8_t)(((!(nq.q < 0.0f) && in) ? 1 : 0) | ((nq.q > 0.0f && in) ? 2 : 0));
 }
 
 // Hash features of every sample, level-major and XCD-aware.  Work-groups are dispatched to
@@ -1105,11 +1116,11 @@ __global__ void __launch_bounds__(256) hash_scatter_kernel(RenderDev a, int dedu
 // ------------------------------------------------------------------------------------
 // The live samples' indices in order: per block of kLiveBlock samples its live count, then
 // each block's indices at the sum of the counts before it (the block adds them up itself;
-// the last block writes the total).  Flags are 0 or 1 bytes, so a 4-byte word's popcount
-// counts its live samples.
+// the last block writes the total).  A sample is listed where bit `bit` of its flag byte
+// is set; masked to that bit, a 4-byte word's popcount counts its listed samples.
 constexpr int kLiveBlock = 4096;  // 256 threads x 16 samples
 
-VXN_DEV int live_count16(const uint8_t* __restrict__ live, int64_t s, int64_t NS, uint32_t (&w)[4]) {
+VXN_DEV int live_count16(const uint8_t* __restrict__ live, int64_t s, int64_t NS, int bit, uint32_t (&w)[4]) {
   if (s + 16 <= NS) {
     const uint4 v = *reinterpret_cast<const uint4*>(live + s);
     w[0] = v.x;
@@ -1126,6 +1137,8 @@ VXN_DEV int live_count16(const uint8_t* __restrict__ live, int64_t s, int64_t NS
       w[q] = x;
     }
   }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) w[q] = (w[q] >> bit) & 0x01010101u;
   return __popc(w[0]) + __popc(w[1]) + __popc(w[2]) + __popc(w[3]);
 }
 
@@ -1150,17 +1163,17 @@ VXN_DEV int block_scan256(int v, int& excl, int* ws4) {
   return tot;
 }
 
-__global__ void __launch_bounds__(256) live_count_kernel(const uint8_t* __restrict__ live, int64_t NS,
+__global__ void __launch_bounds__(256) live_count_kernel(const uint8_t* __restrict__ live, int64_t NS, int bit,
                                                          int32_t* __restrict__ bcnt) {
   __shared__ int ws4[4];
   uint32_t w[4];
-  const int c = live_count16(live, (int64_t)blockIdx.x * kLiveBlock + threadIdx.x * 16, NS, w);
+  const int c = live_count16(live, (int64_t)blockIdx.x * kLiveBlock + threadIdx.x * 16, NS, bit, w);
   int excl;
   const int tot = block_scan256(c, excl, ws4);
   if (threadIdx.x == 0) bcnt[blockIdx.x] = tot;
 }
 
-__global__ void __launch_bounds__(256) live_list_kernel(const uint8_t* __restrict__ live, int64_t NS,
+__global__ void __launch_bounds__(256) live_list_kernel(const uint8_t* __restrict__ live, int64_t NS, int bit,
                                                         const int32_t* __restrict__ bcnt, int32_t* __restrict__ idx,
                                                         int32_t* __restrict__ n_live) {
   __shared__ int ws4[4];
@@ -1173,7 +1186,7 @@ __global__ void __launch_bounds__(256) live_list_kernel(const uint8_t* __restric
   const int off = block_scan256(part, e0, red);
   uint32_t w[4];
   const int64_t s = (int64_t)b * kLiveBlock + threadIdx.x * 16;
-  const int c = live_count16(live, s, NS, w);
+  const int c = live_count16(live, s, NS, bit, w);
   int excl;
   const int tot = block_scan256(c, excl, ws4);
   int pos = off + excl;
@@ -2464,7 +2477,7 @@ static BinPlan bin_plan(int64_t ns, int L, int log2_T, const int32_t* level_res)
 struct RenderWs {
   int64_t off_sc, off_norm, off_rows, off_feat, off_geo, off_hf, off_cnt, off_csum, off_base, off_ridx, off_rval;
   int64_t off_gcnt, off_gcsum, off_gbase, off_gref, off_gval, off_gedge, off_live, off_lidx, off_lcnt, off_nlive,
-      off_gsd, off_need, total;
+      off_gsd, off_need, off_lidx2, off_lcnt2, off_nlive2, total;
   BinPlan bins;
   int gbins;   // grid tiles of the binned grid gradient (0: the atomic grid scatter)
   int grows;   // its count rows
@@ -2503,9 +2516,13 @@ static RenderWs render_ws(int64_t R, int S, int L, int log2_T, const int32_t* le
     w.off_lidx = align256(w.off_live + ns);
     w.off_lcnt = align256(w.off_lidx + ns * 4);
     w.off_nlive = align256(w.off_lcnt + ((ns + kLiveBlock - 1) / kLiveBlock) * 4);
-    end = align256(w.off_nlive + 4);
+    // the grid gradient's own list (BF16 form: the hash-table gradient's list is shorter)
+    w.off_lidx2 = align256(w.off_nlive + 4);
+    w.off_lcnt2 = align256(w.off_lidx2 + ns * 4);
+    w.off_nlive2 = align256(w.off_lcnt2 + ((ns + kLiveBlock - 1) / kLiveBlock) * 4);
+    end = align256(w.off_nlive2 + 4);
   } else {
-    w.off_live = w.off_lidx = w.off_lcnt = w.off_nlive = end;
+    w.off_live = w.off_lidx = w.off_lcnt = w.off_nlive = w.off_lidx2 = w.off_lcnt2 = w.off_nlive2 = end;
   }
   w.gbins = gbins;
   w.grows = (int)((ns + kGridRowSamples - 1) / kGridRowSamples);
@@ -2707,10 +2724,57 @@ static int launch_render_nv(const vxn_render_args* p, void* stream) {
     VXN_LAUNCH_CHECK();
     return 0;
   };
+  // the samples listed in order by their byte flags: the record passes and the grid
+  // gradient walk only them
+  // (bit `bit` of each flag byte; list k = 0 for the record passes, 1 for the grid gradient)
+  const int32_t* grid_idx = nullptr;
+  const int32_t* grid_n = nullptr;
+  auto list_samples = [&](hipStream_t st, const uint8_t* flags, int bit, int k) -> int {
+    char* wsp = (char*)p->workspace;
+    const unsigned nlb = (unsigned)((ns + kLiveBlock - 1) / kLiveBlock);
+    int32_t* lcnt = (int32_t*)(wsp + (k ? w.off_lcnt2 : w.off_lcnt));
+    int32_t* lidx = (int32_t*)(wsp + (k ? w.off_lidx2 : w.off_lidx));
+    int32_t* nlive = (int32_t*)(wsp + (k ? w.off_nlive2 : w.off_nlive));
+    hipLaunchKernelGGL(live_count_kernel, dim3(nlb), dim3(256), 0, st, flags, ns, bit, lcnt);
+    VXN_LAUNCH_CHECK();
+    hipLaunchKernelGGL(live_list_kernel, dim3(nlb), dim3(256), 0, st, flags, ns, bit, (const int32_t*)lcnt, lidx,
+                       nlive);
+    VXN_LAUNCH_CHECK();
+    if (k) {
+      grid_idx = lidx;
+      grid_n = nlive;
+    } else {
+      a.live_idx = lidx;
+      a.n_live = nlive;
+    }
+    return 0;
+  };
+  // BF16 backward / training: the geometry pass's need flags cover every sample with a
+  // gradient (w != 0 needs alpha > 0, dq != 0 needs 0 <= q <= 1, both inside the sphere;
+  // only w != 0 gives the hash features a gradient), so the record passes walk the list
+  // of bit 1 and the grid gradient that of bit 0, and the hash-table gradient's count and
+  // scans, which read no gradients, run before the render kernel (beside the feature
+  // gather and the render kernel when scatter_stream is given: se[1] marks them)
+  bool early_list = false;
+  a.sparse = binned_hash ? 1 : 0;
   if (ev) VXN_HIP(hipEventRecord(ev[0], s));
   if (MODE != kFwd && p->mlp_mode == VXN_MLP_BF16_MFMA) {
     hipLaunchKernelGGL(geo_kernel, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, s, a);
     VXN_LAUNCH_CHECK();
+    if (binned_hash) {
+      a.live = nullptr;  // the render kernel's flags are not needed
+      early_list = true;
+      hipStream_t ls = s;
+      if (ss) {
+        VXN_HIP(hipEventRecord(se[0], s));
+        VXN_HIP(hipStreamWaitEvent(ss, se[0], 0));
+        ls = ss;
+      }
+      if (int e = list_samples(ls, a.need, 1, 0)) return e;
+      if (int e = count_and_scan(ls)) return e;
+      if (int e = list_samples(ls, a.need, 0, 1)) return e;
+      if (ss) VXN_HIP(hipEventRecord(se[1], ss));
+    }
     const int64_t chunks = (ns + 255) / 256;
     const unsigned nb_hf = (unsigned)(8 * ((a.h.L + 7) / 8) * chunks);
     hipLaunchKernelGGL(hash_features_kernel<0>, dim3(nb_hf), dim3(256), 0, s, a, chunks);
@@ -2733,21 +2797,18 @@ static int launch_render_nv(const vxn_render_args* p, void* stream) {
   if (ev) VXN_HIP(hipEventRecord(ev[2], s));
   if (MODE != kFwd) {
     const int sb = (int)((ns + 255) / 256);
-    if (binned_hash) {
-      // the samples with a gradient, in order: the record passes and the grid gradient walk
-      // only them
-      char* wsp = (char*)p->workspace;
-      const unsigned nlb = (unsigned)((ns + kLiveBlock - 1) / kLiveBlock);
-      int32_t* lcnt = (int32_t*)(wsp + w.off_lcnt);
-      int32_t* lidx = (int32_t*)(wsp + w.off_lidx);
-      int32_t* nlive = (int32_t*)(wsp + w.off_nlive);
-      hipLaunchKernelGGL(live_count_kernel, dim3(nlb), dim3(256), 0, s, (const uint8_t*)a.live, ns, lcnt);
-      VXN_LAUNCH_CHECK();
-      hipLaunchKernelGGL(live_list_kernel, dim3(nlb), dim3(256), 0, s, (const uint8_t*)a.live, ns,
-                         (const int32_t*)lcnt, lidx, nlive);
-      VXN_LAUNCH_CHECK();
-      a.live_idx = lidx;
-      a.n_live = nlive;
+    if (early_list) {
+      // the counts and positions from the second stream (se[1] is recorded again below)
+      if (ss) VXN_HIP(hipStreamWaitEvent(s, se[1], 0));
+    } else if (binned_hash) {
+      // the samples with a gradient (the render kernel's flags: w != 0 or dq != 0)
+      if (int e = list_samples(s, (const uint8_t*)a.live, 0, 0)) return e;
+    }
+    // the grid gradient's passes walk their own list where there is one
+    RenderDev ag = a;
+    if (grid_idx) {
+      ag.live_idx = grid_idx;
+      ag.n_live = grid_n;
     }
     // the grid gradient needs only the render kernel's per-sample geometry gradients
     hipStream_t gs = s;
@@ -2779,7 +2840,7 @@ static int launch_render_nv(const vxn_render_args* p, void* stream) {
       cb.csum = (int32_t*)(ws + w.off_gcsum);
       cb.base = gb.base;
       const unsigned gwg = (unsigned)((w.grows + 3) / 4);
-      hipLaunchKernelGGL(grid_bin_count_kernel, dim3(gwg), dim3(256), (size_t)4 * w.gbins * sizeof(int32_t), gs, a,
+      hipLaunchKernelGGL(grid_bin_count_kernel, dim3(gwg), dim3(256), (size_t)4 * w.gbins * sizeof(int32_t), gs, ag,
                          gb);
       VXN_LAUNCH_CHECK();
       const dim3 cgrid((unsigned)((cb.nbl + 255) / 256), (unsigned)cb.nchunk, 1u);
@@ -2791,24 +2852,27 @@ static int launch_render_nv(const vxn_render_args* p, void* stream) {
       VXN_LAUNCH_CHECK();
       hipLaunchKernelGGL(bin_col_positions_kernel, cgrid, dim3(256), 0, gs, cb);
       VXN_LAUNCH_CHECK();
-      hipLaunchKernelGGL(grid_bin_write_kernel, dim3(gwg), dim3(256), grid_write_lds(w.gbins), gs, a, gb);
+      hipLaunchKernelGGL(grid_bin_write_kernel, dim3(gwg), dim3(256), grid_write_lds(w.gbins), gs, ag, gb);
       VXN_LAUNCH_CHECK();
       if (p->grid_wait_event) VXN_HIP(hipStreamWaitEvent(gs, (hipEvent_t)p->grid_wait_event, 0));
       hipLaunchKernelGGL(grid_bin_sum_kernel, dim3((unsigned)(4 * w.gbins)), dim3(64 * kGsumWaves), kGsumLds, gs, gb,
                          a.g, a.grad_grid);
       VXN_LAUNCH_CHECK();
-      hipLaunchKernelGGL(grid_edge_kernel, dim3(1), dim3(64), 0, gs, a, (const uint64_t*)gb.edge,
+      // the edge pass adds with atomics: forked atomic adds into grad_grid may run beside it
+      if (p->fork_event) VXN_HIP(hipEventRecord((hipEvent_t)p->fork_event, gs));
+      hipLaunchKernelGGL(grid_edge_kernel, dim3(1), dim3(64), 0, gs, ag, (const uint64_t*)gb.edge,
                          (int64_t)w.grows * kGridRowChunks);
       VXN_LAUNCH_CHECK();
-      if (p->fork_event) VXN_HIP(hipEventRecord((hipEvent_t)p->fork_event, gs));
     } else {
-      hipLaunchKernelGGL(grid_scatter_kernel, dim3(sb), dim3(256), 0, gs, a);
+      hipLaunchKernelGGL(grid_scatter_kernel, dim3(sb), dim3(256), 0, gs, ag);
       VXN_LAUNCH_CHECK();
     }
     if (ev) VXN_HIP(hipEventRecord(ev[3], gs));
     if (ss) VXN_HIP(hipEventRecord(se[1], ss));
     if (binned_hash) {
-      if (int e = count_and_scan(s)) return e;
+      if (!early_list) {
+        if (int e = count_and_scan(s)) return e;
+      }
       // levels below dl carry run sums (record values + entry array), the rest factored records
       if (dl > 0) {
         hipLaunchKernelGGL(hash_bin_write_kernel<false>, dim3((unsigned)(dl * b.ntile)), dim3(256),
