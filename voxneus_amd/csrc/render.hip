@@ -1095,7 +1095,7 @@ __global__ void __launch_bounds__(256) hash_scatter_kernel(RenderDev a, int dedu
 // Binned hash-table gradient.  Device-scope float atomics sustain about 21 G requests/s
 // however they are spread, which bounds hash_scatter_kernel.  Here the same records are
 // binned by table region instead of added in place: (1) each wavefront counts, for one
-// level and 1024 consecutive samples, the records it produces per bin of 2048 consecutive
+// level and 512 consecutive samples, the records it produces per bin of 2048 consecutive
 // entries (LDS counters of its own), one row of counts per wavefront; (2) a column scan
 // over [level][row][bin] turns the counts into each row's first position in every bin;
 // (3) the wavefronts write their records there; (4) four wavefronts per bin, each owning a
@@ -1201,9 +1201,9 @@ __global__ void __launch_bounds__(256) live_list_kernel(const uint8_t* __restric
 constexpr int kBinBits = 11;
 constexpr int kBinEntries = 1 << kBinBits;
 constexpr int kBinTileWaves = 4;            // wavefronts per work-group of the count / write passes
-constexpr int kBinWaveChunks = 16;          // 64-sample chunks per wavefront
+constexpr int kBinWaveChunks = 8;           // 64-sample chunks per wavefront
 constexpr int kBinWaveSamples = 64 * kBinWaveChunks;
-constexpr int kBinTile = kBinWaveSamples * kBinTileWaves;  // 4096 samples per work-group
+constexpr int kBinTile = kBinWaveSamples * kBinTileWaves;  // 2048 samples per work-group
 constexpr int kMaxBinsPerLevel = 1024;      // tables up to 2^21 entries per level
 
 struct BinDev {
@@ -1321,8 +1321,8 @@ VXN_DEV void hash_level_records(const RenderDev& a, int64_t i, int lane, int l, 
     emit(tail, c, v, d0, d1);
 }
 
-// Passes 1 and 3 run one work-group per (level, 4096-sample tile), level-major: its four
-// wavefronts walk 1024 consecutive samples each, in 16 chunks of 64, and the tile owns one
+// Passes 1 and 3 run one work-group per (level, 2048-sample tile), level-major: its four
+// wavefronts walk 512 consecutive samples each, in 8 chunks of 64, and the tile owns one
 // row of counters (LDS, then cnt[level][tile]).  Pass 3 takes the chunks in lockstep: the
 // records of chunk c of all four wavefronts (at most kStageSlots) are ranked in each bin,
 // sorted by bin through LDS, and stored as runs that continue the tile's previous run in
@@ -1685,7 +1685,7 @@ __global__ void __launch_bounds__(64 * kSumWaves) hash_bin_sum_kernel(BinDev b, 
   // one's are added, so the loads of a wavefront stay in flight while it works in LDS.
   // Indices past the bin's end are clamped onto its last record (always a valid address
   // inside the loop) and that record's use is masked.
-  constexpr int U = 4;
+  constexpr int U = 2;
   constexpr int32_t kStep = U * 64 * kSumWaves;
   float4 nv[U];
   uint32_t nr[U];
