@@ -1662,7 +1662,7 @@ __global__ void __launch_bounds__(256) bin_col_positions_kernel(BinDev b) {
 // is straight-line LDS code.  (Four groups' halves tagged together in one round, lane + 64
 // x group, took more rounds than they saved.)  The bins run on XCD bin % 8 (work-group b on
 // XCD b % 8).
-constexpr int kSumWaves = 4;
+constexpr int kSumWaves = 2;  // two copies per work-group: more, smaller work-groups per CU
 constexpr int kSumSlots = kBinEntries + 64;
 
 __global__ void __launch_bounds__(64 * kSumWaves) hash_bin_sum_kernel(BinDev b, float* __restrict__ grad_table,
@@ -1977,7 +1977,7 @@ __global__ void __launch_bounds__(256) grid_scatter_kernel(RenderDev a) {
 // tile they touch and summed per tile in LDS, so every vertex of grad_grid is written by one
 // plain read-add-write.  The border and grid_sample edge cases above stay atomic, but are
 // added by one wavefront in sample order (grid_edge_kernel), so their order is fixed too.
-//   1. grid_bin_count_kernel: one wavefront per count row of kGridRowSamples samples (16
+//   1. grid_bin_count_kernel: one wavefront per count row of kGridRowSamples samples (8
 //      chunks of 64, the runs of grid_scatter_kernel): per tile, the row's (run, tile) pairs.
 //   2. the bin column scans of the hash gradient turn the rows into positions.
 //   3. grid_bin_write_kernel: the same runs again; a run's brick (32 floats) goes to run
@@ -1991,7 +1991,7 @@ __global__ void __launch_bounds__(256) grid_scatter_kernel(RenderDev a) {
 //   5. grid_edge_kernel: the samples the write pass flagged, in sample order.
 // ------------------------------------------------------------------------------------
 constexpr int kGtX = 32, kGtY = 32, kGtZ = 16;  // vertices per grid tile (64 KB of LDS)
-constexpr int kGridRowSamples = 1024;           // samples per count row: one wavefront
+constexpr int kGridRowSamples = 512;            // samples per count row: one wavefront
 constexpr int kGridRowChunks = kGridRowSamples / 64;
 constexpr int kGridMaxBins = 4096;              // tiles of a grid the binned passes take
 constexpr int kGridMaxDim = 1024;               // cell coordinates are packed in 10 bits
@@ -2759,7 +2759,7 @@ static int launch_render_nv(const vxn_render_args* p, void* stream) {
   a.sparse = binned_hash ? 1 : 0;
   if (ev) VXN_HIP(hipEventRecord(ev[0], s));
   if (MODE != kFwd && p->mlp_mode == VXN_MLP_BF16_MFMA) {
-    hipLaunchKernelGGL(geo_kernel, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(geo_kernel, dim3((unsigned)((ns + 127) / 128)), dim3(128), 0, s, a);
     VXN_LAUNCH_CHECK();
     if (binned_hash) {
       a.live = nullptr;  // the render kernel's flags are not needed
