@@ -335,20 +335,36 @@ __global__ void __launch_bounds__(kRT, 2) render_kernel(RenderDev a) {
     // ---------------------------------------------------------------- sample forward
     const float px = ox + dx * mid, py = oy + dy * mid, pz = oz + dz * mid;
     float sdf, gv[3];
-    if constexpr (kBFB && !MULTI) {  // the geometry pass's values (the same code, run ahead)
+    Corners cn;  // the sample's cell: the stencil's, then the corners output and the V_R marks
+    if constexpr (kBFB && !MULTI) {  // the geometry pass's values (sample_sdf_grad, run ahead)
       sdf = valid ? a.geo[si] : 0.0f;
 #pragma unroll
       for (int k = 0; k < 3; ++k) gv[k] = valid ? a.geo[(1 + k) * NS + si] : 0.0f;
     } else {
-      sample_sdf_grad(G, px, py, pz, sdf, gv);
+      // sample_sdf_grad's code, with the cell kept for the outputs below (written out here:
+      // the FP32 serving render measured 18 % slower through the helper)
+      const GsStencil gs = gs_stencil(G, px, py, pz);
+      cn = corner_decomp(G, px, py, pz);
+      if (!G.analytical && brick_interior(G, cn)) {
+        Brick bk;
+        load_brick(G, cn, bk);
+        if (!gs_eval_brick(gs, cn, bk, sdf)) sdf = gs_eval(G, gs);
+        eq7_gradient_brick(G, cn, bk, gv);
+      } else {
+        sdf = gs_eval(G, gs);
+        grid_gradient(G, cn, gv);
+      }
     }
     const float gr = norm3(gv);
     const float nrm[3] = {gv[0] / (gr + 1e-8f), gv[1] / (gr + 1e-8f), gv[2] / (gr + 1e-8f)};
     const float x0 = fminf(fmaxf(over_den(G, 0, px - G.mn[0]), 0.0f), 1.0f);
     const float x1 = fminf(fmaxf(over_den(G, 1, py - G.mn[1]), 0.0f), 1.0f);
     const float x2 = fminf(fmaxf(over_den(G, 2, pz - G.mn[2]), 0.0f), 1.0f);
-    // Eq. 2 up to the clamp (needs only the geometry)
-    const NeusQ nq = neus_q(nrm, dx, dy, dz, dist, sdf, sv, sr, px, py, pz, a.iter_cos, a.cos_anneal);
+    // Eq. 2 up to the clamp (needs only the geometry): ahead of the features where it decides
+    // whether they are gathered (FP32 backward / training), after the MLP otherwise
+    constexpr bool kEarlyQ = MODE != kFwd && !kPre;
+    NeusQ nq;
+    if constexpr (kEarlyQ) nq = neus_q(nrm, dx, dy, dz, dist, sdf, sv, sr, px, py, pz, a.iter_cos, a.cos_anneal);
     float x[35];  // 32 hash features, then the unit normal (MLP input columns 0-31, 48-50)
     if constexpr (!kPre) {
       // backward / training: a sample with q < 0 or outside the sphere has alpha = 0 and no
@@ -463,6 +479,7 @@ __global__ void __launch_bounds__(kRT, 2) render_kernel(RenderDev a) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) rgb[c] = sigmoid_ref(o3[c]);
     // ---------------------------------------------------------------- Eq. 2 alpha
+    if constexpr (!kEarlyQ) nq = neus_q(nrm, dx, dy, dz, dist, sdf, sv, sr, px, py, pz, a.iter_cos, a.cos_anneal);
     const float tcos = nq.tcos, fp = nq.fp, fn = nq.fn, pp = nq.pp, pn = nq.pn, den = nq.den, q = nq.q;
     const float inside = nq.inside;
     const float araw = fminf(fmaxf(q, 0.0f), 1.0f);
@@ -513,7 +530,7 @@ __global__ void __launch_bounds__(kRT, 2) render_kernel(RenderDev a) {
           a.normal_s[3 * si + 2] = nrm[2];
         }
         if (a.corners) {
-          const Corners cn = corner_decomp(G, px, py, pz);
+          if constexpr (kBFB && !MULTI) cn = corner_decomp(G, px, py, pz);
 #pragma unroll
           for (int k = 0; k < 8; ++k) {
             int cx, cy, cz;
@@ -527,7 +544,7 @@ __global__ void __launch_bounds__(kRT, 2) render_kernel(RenderDev a) {
     }
     if (a.vr) {
       // V_R marks: a sample in the same cell as the previous lane's marks nothing new.
-      const Corners cn = corner_decomp(G, px, py, pz);
+      if constexpr (kBFB && !MULTI) cn = corner_decomp(G, px, py, pz);
       const int key = valid ? (int)lin3(G, cn.c0[0], cn.c0[1], cn.c0[2]) : -1;
       const int kp = __shfl_up(key, 1, 64);
       if (valid && (lane == 0 || kp != key)) vr_mark_cell(G, cn, a.vr);
