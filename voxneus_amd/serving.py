@@ -439,10 +439,12 @@ class Server:
 
         while pending or live_r or live_q or in_flight:
             now = time.perf_counter() - t0
+            blocked = False  # the oldest arrived request did not fit in its arena
             while pending and (not realtime or requests[pending[0]].arrival <= now):
                 i = pending[0]
                 lv = self._admit(i, requests[i])
                 if lv is None:
+                    blocked = True
                     break  # first come, first served: wait for memory
                 lv.res.admitted = now
                 (live_r if lv.res.kind == "render" else live_q).append(lv)
@@ -453,11 +455,14 @@ class Server:
                 if in_flight:
                     retire(True)
                 elif pending:
+                    # with nothing live or in flight the arenas are empty: a request that did
+                    # not fit then never will (one that arrived after the admission pass is
+                    # admitted on the next pass)
+                    if blocked and not live_r and not live_q:
+                        raise RuntimeError("a pending request cannot be admitted: the arena is too small")
                     wait = requests[pending[0]].arrival - (time.perf_counter() - t0)
                     if wait > 0:
                         time.sleep(min(wait, 0.01))
-                    elif not live_r and not live_q:
-                        raise RuntimeError("a pending request cannot be admitted: the arena is too small")
                 continue
             if len(in_flight) >= self.max_in_flight:
                 retire(True)
