@@ -41,21 +41,28 @@ RAYS = 8192
 SAMPLES = 128
 LATTICE = 320
 # Algorithmic bytes per sample (no cache credit), per kernel of the fused render launch:
-#   hash_features  (BF16 form) 16 levels x 8 corners x 2 features = 256 bf16 table entries
-#                  = 512 B gathered, 16 feature pairs written (64 B);
-#   render_kernel  grid forward 32 words (8 corners + 24 face neighbours) = 128 B, the hash
-#                  features (BF16: 64 B read back; FP32: 1024 B gathered in the kernel),
-#                  per-ray streaming ~9 B;
+#   hash_features  (BF16 form) the geometry pass, grid stencil 32 words = 128 B in, SDF +
+#                  gradient 16 B and a flag byte out, then the sample lists and the gather,
+#                  16 levels x 8 corners x 2 features = 256 bf16 table entries = 512 B
+#                  gathered, 16 feature pairs written (64 B), counted for every sample as the
+#                  reference gathers them (the kernel skips the samples whose colour reaches
+#                  no output: q < 0 or outside the sphere);
+#   render_kernel  grid forward 32 words (8 corners + 24 face neighbours) = 128 B (BF16: the
+#                  geometry pass's 16 B read back), the hash features (BF16: 64 B read back;
+#                  FP32: 1024 B gathered in the kernel), per-ray streaming ~9 B;
 #   hash_gradient  the hash-table gradient stage (binned: count, scan, record write, per-bin
-#                  sums; 6 launches between the same two events), accounted as the 256 FP32
-#                  read-modify-writes it replaces = 2048 B;
+#                  sums between the same two events; in the BF16 form the count and scans run
+#                  before the render kernel, beside the gather, so the span holds the record
+#                  writes and sums), accounted as the 256 FP32 read-modify-writes it replaces
+#                  = 2048 B;
 #   grid_gradient  the grid gradient (binned: count, scan, brick write, per-tile sums; or the
 #                  atomic scatter), accounted as the 32 words read-modify-write it replaces
 #                  = 256 B.
 KERNEL_BYTES = {
     # FP32 form: the render kernel gathers the hash features itself (no hash_features launch)
     "fp32": {"render_kernel": 128 + 1024 + 9, "hash_gradient": 2048, "grid_gradient": 256},
-    "bf16": {"hash_features": 512 + 64, "render_kernel": 128 + 64 + 9, "hash_gradient": 2048, "grid_gradient": 256},
+    "bf16": {"hash_features": 128 + 17 + 512 + 64, "render_kernel": 16 + 64 + 9, "hash_gradient": 2048,
+             "grid_gradient": 256},
 }
 # the sampler: 112 SDF queries x 8 corners x 4 B per 128-sample ray (no cache credit)
 SIDE_BYTES_PER_SAMPLE = {"upsample": 28}
