@@ -147,9 +147,16 @@ def run_gpu(sc, batches, form, steps, warmup, serial_grid=False):
     kern = {k: float(np.mean([e.elapsed_ms(i, j) for e in ev])) for k, (i, j) in kev.items()}
     loss = out["loss"].item()
     # the step's launches outside the render call, each timed alone on the last step's inputs:
-    # the sampler (with its V_R marks) and the regularisers over the V_R bitmask
+    # the sampler (with its V_R marks as the step makes them: the cell clear, the per-cell
+    # marks and the bitmask written from them) and the regularisers over the V_R bitmask
     o, d, _ = gb[(steps - 1) % len(gb)]
-    kern["upsample"] = _event_ms(lambda: st.sample(o, d, None, None, st.bitmask))
+
+    def sample():
+        if st.vr_cells is not None:
+            st.vr_cells.zero_()
+        st.sample(o, d, None, None, st.bitmask, st.vr_cells)
+
+    kern["upsample"] = _event_ms(sample)
     kern["reg_vr"] = _event_ms(st._regularisers)
     n_vr = int(st.vr_counts[0].item())
     return {"ms_per_step": dt / steps * 1e3, "rays_per_s": RAYS * steps / dt, "kernel_ms": kern, "loss": loss,

@@ -190,6 +190,11 @@ typedef struct vxn_sampler_args {
                           (near_far_from_sphere, rendering/rays.py:102-128), computed in-kernel */
   uint32_t* vr_bitmask; /* nullable [ceil(X*Y*Z/32)]: bit set for all 8 corners of the cell of
                            every final sample (at o + d * mid_z), as vxn_render_args.vr_bitmask */
+  uint32_t* vr_cells;   /* nullable [ceil(X*Y*Z/32)], all zero on entry, used with vr_bitmask when
+                           Z is a multiple of 32: each sample sets one bit, its cell's (left
+                           set), and a pass then writes vr_bitmask as those cells' corners: the
+                           same bits as above, but vr_bitmask is overwritten, not added to (no
+                           reference counterpart: fewer atomics for the same V_R) */
 } vxn_sampler_args;
 
 int vxn_upsample(const vxn_sampler_args* a, void* stream);

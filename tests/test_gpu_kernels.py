@@ -124,6 +124,33 @@ def test_sampler_bit_exact_deterministic(Fn):
     assert np.array_equal(mid.cpu().numpy(), g["mid_det"])
 
 
+@pytest.mark.parametrize("res", [64, 48])
+def test_sampler_vr_cell_marks_equal_corner_marks(Fn, res):
+    """The sampler's V_R with vr_cells (one mark per sample cell, then the bitmask written
+    whole as those cells' corners; grids whose Z is a multiple of 32) is the corner marks'
+    bitmask bit for bit, overwriting what the bitmask held; with Z not a multiple of 32 the
+    sampler marks the corners as without vr_cells and the cells stay zero."""
+    sc = scenes.make_scene("multi", res, 15, seed=3)
+    o, d = (torch.from_numpy(a).to(DEV) for a in scenes.hemisphere_ray_batch(2048, seed=7))
+    grid = torch.from_numpy(sc.grid).to(DEV)
+    words = Fn.vr_bitmask_words((res, res, res))
+    ref = torch.zeros(words, device=DEV, dtype=torch.int32)
+    z0 = Fn.upsample(o, d, None, None, grid, BOX, perturb=False, bound_radius=1.0, vr_bitmask=ref)[0]
+    cells = torch.zeros(words, device=DEV, dtype=torch.int32)
+    fill = 0x5A5A5A5A if res % 32 == 0 else 0
+    got = torch.full((words,), fill, device=DEV, dtype=torch.int32)
+    z1 = Fn.upsample(o, d, None, None, grid, BOX, perturb=False, bound_radius=1.0, vr_bitmask=got, vr_cells=cells)[0]
+    torch.cuda.synchronize()
+    assert torch.equal(z0, z1)
+    n_ref = int(np.unpackbits(ref.cpu().numpy().view(np.uint8)).sum())
+    assert n_ref > 1000
+    assert torch.equal(got, ref), "V_R from the cell marks differs from the corner marks"
+    if res % 32 == 0:
+        assert int(cells.ne(0).sum()) > 0
+    else:
+        assert int(cells.ne(0).sum()) == 0
+
+
 def test_sampler_bit_exact_injected_uniforms(Fn):
     g = load_golden("unit_sampler.npz")
     sc, occ = _sampler_inputs(g)

@@ -72,7 +72,7 @@ class SamplerArgs(ctypes.Structure):
         ("n_samples", c_i32), ("n_importance", c_i32), ("up_sample_steps", c_i32), ("perturb", c_i32),
         ("t_lin", c_vp), ("u_det", c_vp), ("t_rand", c_vp), ("u_rand", c_vp),
         ("z_vals", c_vp), ("dists", c_vp), ("mid_z", c_vp), ("inds", c_vp), ("bound_radius", c_f),
-        ("vr_bitmask", c_vp),
+        ("vr_bitmask", c_vp), ("vr_cells", c_vp),
     ]
 
 

@@ -28,6 +28,7 @@ struct SamplerDev {
   float *z_out, *dists_out, *mid_out;
   int32_t* inds;
   uint32_t* vr;  // nullable V_R bitmask: the corners of every final sample's cell
+  uint32_t* vr_cells;  // nullable (then vr is null): one bit per final sample's cell (low corner)
   float bound_r;
   // serving batches: per-ray scene of a packed batch; R is then the capacity and the
   // batch size is read from n_rays_dev
@@ -321,10 +322,12 @@ __global__ void __launch_bounds__(kSampThreads, 8) upsample_kernel(SamplerDev a)
         a.mid_out[ray * n + i] = midv[h];
       }
     }
-    if (!MULTI && a.vr) {
+    if (!MULTI && (a.vr || a.vr_cells)) {
       // V_R of the render's samples (their positions are o + d * mid, as the render forms
       // them), marked here so that the regularisers need not wait for the render: a
-      // sample in the same cell as the sample before it marks nothing new
+      // sample in the same cell as the sample before it marks nothing new.  With vr_cells
+      // a sample sets one bit (its cell's) in place of its 8 corners' (4-8 atomics), and
+      // vr_dilate_kernel turns the cells into their corners afterwards
       Corners cn[2];
       int key[2] = {-1, -1};
 #pragma unroll
@@ -338,11 +341,44 @@ __global__ void __launch_bounds__(kSampThreads, 8) upsample_kernel(SamplerDev a)
       const int kp0 = __shfl_up(key[0], 1, 64);
       const int k63 = __shfl(key[0], 63, 64);
       const int kp1 = t == 0 ? k63 : __shfl_up(key[1], 1, 64);
-      if (t < n && (t == 0 || kp0 != key[0])) vr_mark_cell(G, cn[0], a.vr);
-      if (t + 64 < n && kp1 != key[1]) vr_mark_cell(G, cn[1], a.vr);
+      if (a.vr_cells) {
+        if (t < n && (t == 0 || kp0 != key[0])) atomicOr(a.vr_cells + (key[0] >> 5), 1u << (key[0] & 31));
+        if (t + 64 < n && kp1 != key[1]) atomicOr(a.vr_cells + (key[1] >> 5), 1u << (key[1] & 31));
+      } else {
+        if (t < n && (t == 0 || kp0 != key[0])) vr_mark_cell(G, cn[0], a.vr);
+        if (t + 64 < n && kp1 != key[1]) vr_mark_cell(G, cn[1], a.vr);
+      }
     }
     __syncthreads();
   }
+}
+
+// V_R from the cell marks: vertex (x, y, z) is a corner of cell c exactly when
+// c = (x - i, y - j, z - k) for some i, j, k in {0, 1} (corner_decomp's corners are c0 and
+// min(c0 + 1, res - 1) per axis), so each bitmask word is the OR of the cell words of its
+// own and the three lower (x, y) columns, each with itself shifted up by one along z (the
+// bit below a word's first comes from the column's previous word).  Every column holds
+// whole words (Z a multiple of 32).  Overwrites vr.
+__global__ void __launch_bounds__(256) vr_dilate_kernel(const uint32_t* __restrict__ cells, uint32_t* __restrict__ vr,
+                                                        int Y, int wpc, int64_t nwords) {
+  const int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (w >= nwords) return;
+  const int64_t col = w / wpc;
+  const int zw = (int)(w - col * wpc);
+  const int64_t x = col / Y;
+  const int y = (int)(col - x * Y);
+  uint32_t acc = 0u;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      if (x < i || y < j) continue;
+      const int64_t cw = w - ((int64_t)i * Y + j) * wpc;
+      const uint32_t c = cells[cw];
+      const uint32_t below = zw > 0 ? cells[cw - 1] : 0u;
+      acc |= c | (c << 1) | (below >> 31);
+    }
+  vr[w] = acc;
 }
 
 int launch_serve_upsample(const SceneDev* scenes, int n_scenes, const int32_t* ray_scene, const int32_t* n_dev,
@@ -371,6 +407,7 @@ int launch_serve_upsample(const SceneDev* scenes, int n_scenes, const int32_t* r
   a.n_rays_dev = n_dev;
   a.n_scenes = n_scenes;
   a.vr = nullptr;
+  a.vr_cells = nullptr;
   const int nb = cap < 8192 ? (int)cap : 8192;
   hipLaunchKernelGGL(upsample_kernel<1>, dim3(nb), dim3(kSampThreads), 0, stream, a);
   VXN_LAUNCH_CHECK();
@@ -402,7 +439,7 @@ extern "C" int vxn_upsample(const vxn_sampler_args* p, void* stream) {
   VXN_REQUIRE(!A.perturb || (A.t_rand && (!steps || A.u_rand)), VXN_ERR_NULL,
               "t_rand and u_rand are required when perturb is set");
   if (A.occ_mask) VXN_REQUIRE(A.occ_res >= 1, VXN_ERR_SHAPE, "occ_res must be >= 1");
-  SamplerDev a;
+  SamplerDev a{};
   a.o = A.rays_o;
   a.d = A.rays_d;
   a.near = A.near;
@@ -422,7 +459,10 @@ extern "C" int vxn_upsample(const vxn_sampler_args* p, void* stream) {
   a.dists_out = A.dists;
   a.mid_out = A.mid_z;
   a.inds = A.inds;
-  a.vr = A.vr_bitmask;
+  // cell marks and the dilation pass where every grid column holds whole bitmask words
+  const bool cells = A.vr_bitmask && A.vr_cells && A.Z % 32 == 0;
+  a.vr = cells ? nullptr : A.vr_bitmask;
+  a.vr_cells = cells ? A.vr_cells : nullptr;
   a.bound_r = A.bound_radius;
   a.scenes = nullptr;
   a.ray_scene = nullptr;
@@ -431,5 +471,11 @@ extern "C" int vxn_upsample(const vxn_sampler_args* p, void* stream) {
   const int nb = A.n_rays < 8192 ? (int)A.n_rays : 8192;
   hipLaunchKernelGGL(upsample_kernel<0>, dim3(nb), dim3(kSampThreads), 0, (hipStream_t)stream, a);
   VXN_LAUNCH_CHECK();
+  if (cells) {
+    const int64_t nwords = (int64_t)A.X * A.Y * (A.Z / 32);
+    hipLaunchKernelGGL(vr_dilate_kernel, dim3((unsigned)((nwords + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                       (const uint32_t*)A.vr_cells, A.vr_bitmask, A.Y, A.Z / 32, nwords);
+    VXN_LAUNCH_CHECK();
+  }
   return 0;
 }

@@ -415,10 +415,13 @@ def _linspace_dev(a: float, b: float, n: int, device) -> torch.Tensor:
 
 def upsample(rays_o, rays_d, near, far, grid3, box, n_samples=64, n_importance=64, up_sample_steps=4,
              perturb=False, occ_mask=None, occ_box=None, t_rand=None, u_rand=None, return_inds=False,
-             bound_radius=1.0, vr_bitmask=None):
+             bound_radius=1.0, vr_bitmask=None, vr_cells=None):
     """Multi-step SDF upsampling on the GPU; returns (z_vals, dists, mid_z[, inds]).
     With ``near = far = None`` the kernel derives them from the ``bound_radius`` sphere.
-    ``vr_bitmask`` (optional int32 words) receives the V_R marks of the final samples."""
+    ``vr_bitmask`` (optional int32 words) receives the V_R marks of the final samples.
+    ``vr_cells`` (optional, as many words, all zero; used when the grid's Z is a multiple
+    of 32) takes one mark per sample cell instead, and ``vr_bitmask`` is then overwritten
+    with their corners (the same bits with far fewer atomics); the cells stay marked."""
     dev = rays_o.device
     R = rays_o.shape[0]
     steps = up_sample_steps if (n_importance > 0 and up_sample_steps > 0) else 0
@@ -461,6 +464,10 @@ def upsample(rays_o, rays_d, near, far, grid3, box, n_samples=64, n_importance=6
     a.z_vals, a.dists, a.mid_z = N.dptr(z), N.dptr(dists), N.dptr(mid)
     a.inds = N.dptr(inds)
     a.vr_bitmask = N.dptr(vr_bitmask)
+    if vr_bitmask is not None and vr_cells is not None:
+        if vr_cells.numel() < vr_bitmask.numel() or vr_cells.dtype != vr_bitmask.dtype:
+            raise ValueError("vr_cells must match vr_bitmask's words")
+        a.vr_cells = N.dptr(vr_cells)
     a.bound_radius = float(bound_radius)
     N.check(N.lib().vxn_upsample(ctypes.byref(a), _stream(ro)), "upsample")
     if return_inds:

@@ -160,6 +160,8 @@ class FusedTrainStep:
         self.rgb = None  # the step's rendered colours [R, 3] (for the PSNR of the metrics)
         self.amplified = torch.zeros(1, device=dev, dtype=torch.int32)
         self.bitmask = torch.zeros(Fn.vr_bitmask_words((X, Y, Z)), device=dev, dtype=torch.int32)
+        # the sampler's per-cell V_R marks (whole bitmask words per grid column only)
+        self.vr_cells = torch.zeros_like(self.bitmask) if Z % 32 == 0 else None
         self.vr_counts = torch.zeros(2, device=dev, dtype=torch.int32)
         nb = ctypes.c_int64(0)
         N.check(N.lib().vxn_reg_vr_workspace_bytes(X, Y, Z, ctypes.byref(nb)))
@@ -252,12 +254,12 @@ class FusedTrainStep:
         b = self.cfg.binned_grid_gradient
         return self.cfg.mlp_mode == N.VXN_MLP_BF16_MFMA if b is None else bool(b)
 
-    def sample(self, rays_o, rays_d, t_rand=None, u_rand=None, vr_bitmask=None):
+    def sample(self, rays_o, rays_d, t_rand=None, u_rand=None, vr_bitmask=None, vr_cells=None):
         cfg = self.cfg
         # ray bounds from the unit sphere (trainer.py:188) are computed inside the sampler
         return Fn.upsample(rays_o, rays_d, None, None, self.grid, self.box, cfg.n_samples, cfg.n_importance,
                            cfg.up_sample_steps, cfg.perturb, self.occ_mask, self.occ_box, t_rand, u_rand,
-                           bound_radius=1.0, vr_bitmask=vr_bitmask)
+                           bound_radius=1.0, vr_bitmask=vr_bitmask, vr_cells=vr_cells)
 
     def _cooldown(self) -> bool:
         return self.post_upsample_cooldown > 0 and self.cfg.w_eik > 0
@@ -299,12 +301,15 @@ class FusedTrainStep:
             self._g_eik = torch.zeros_like(self.grid)
             self._clamp_ws = Fn.eik_clamp_workspace(self.grid.device)
         self.grad_grid.zero_()  # the render call overwrites grad_table
-        self.bitmask.zero_()
         # with the sampler in this step, it marks V_R and the regularisers run on the side
         # stream beside the render kernel and the scatters; both only add into grad_grid
         early = self.side is not None and zs is None
+        # the sampler marks one bit per cell and then writes the bitmask whole from the cells
+        # where the grid's columns hold whole words; otherwise the corners' bits
+        cells = self.vr_cells if early else None
+        (cells if cells is not None else self.bitmask).zero_()
         if zs is None:
-            z, dists, mid = self.sample(rays_o, rays_d, t_rand, u_rand, self.bitmask if early else None)
+            z, dists, mid = self.sample(rays_o, rays_d, t_rand, u_rand, self.bitmask if early else None, cells)
         else:
             z, dists, mid = zs
         main = Fn._stream(mid)
@@ -358,6 +363,7 @@ class FusedTrainStep:
         X, Y, Z = grid.shape
         self.grad_grid = torch.zeros_like(self.grid)
         self.bitmask = torch.zeros(Fn.vr_bitmask_words((X, Y, Z)), device=dev, dtype=torch.int32)
+        self.vr_cells = torch.zeros_like(self.bitmask) if Z % 32 == 0 else None
         nb = ctypes.c_int64(0)
         N.check(N.lib().vxn_reg_vr_workspace_bytes(X, Y, Z, ctypes.byref(nb)))
         self.reg_ws = torch.empty(int(nb.value), device=dev, dtype=torch.uint8)
