@@ -172,8 +172,9 @@ VXN_DEV void build_mlp_fragments(const float* W0, const float* W1, u32x4_t* frag
   }
 }
 
-// SDF (grid_sample) and Eq. 7 / Eq. 4 gradient at a sample position: the render kernel's
-// and the geometry pass's common code, so both get the same values.
+// SDF (grid_sample) and Eq. 7 / Eq. 4 gradient at a sample position, for the geometry
+// pass; the render kernel writes the same operations out itself (keeping the cell for its
+// other outputs), so both get the same values.
 VXN_DEV void sample_sdf_grad(const GridDesc& G, float px, float py, float pz, float& sdf, float (&gv)[3]) {
   const GsStencil gs = gs_stencil(G, px, py, pz);
   const Corners cn = corner_decomp(G, px, py, pz);
@@ -977,7 +978,7 @@ VXN_DEV void sample_pos(const RenderDev& a, int64_t i, float& px, float& py, flo
 // outside the sphere has alpha = 0 and no gradient through its colour, so its hash
 // features are not gathered; with q = 0 its colour still enters its alpha gradient but its
 // features get no gradient (w = 0), so the hash-table gradient's records skip it too.
-// The same helpers and operation order as the render kernel.
+// q comes from neus_q on these values, as in the render kernel, so both agree.
 __global__ void __launch_bounds__(256) geo_kernel(RenderDev a) {
   const int64_t NS = a.R * (int64_t)a.S;
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
