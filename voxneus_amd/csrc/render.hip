@@ -2673,10 +2673,12 @@ static int build_render_dev(const vxn_render_args& A, RenderDev& a, int mode) {
   return 0;
 }
 
-// Hash levels whose cells are wide enough for consecutive samples to share them.
+// Hash levels whose cells are wide enough for consecutive samples to share them (up to
+// res 2048: merging runs of 1.05-1.2 samples there still beat factored records, whose
+// repeated entries within a record group cost the sum pass extra tag rounds).
 static int dedup_levels(const HashDesc& h) {
   int n = 0;
-  while (n < h.L && h.res[n] <= 1024.0f) ++n;
+  while (n < h.L && h.res[n] <= 2048.0f) ++n;
   return n;
 }
 
